@@ -1,0 +1,65 @@
+"""Loader of the HIP projector library (libprojector.so).
+
+The library is loaded from ``ctprojector_amd/kernel/bin/libprojector.so`` as
+the reference loads its CUDA library with ctypes
+(src/ct_projector/projector/numpy/ct_projector.py:15-17).  There is no CPU
+fallback: if the library is missing or cannot be loaded, every projector call
+raises ``ProjectorLibraryError``.
+"""
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, 'kernel', 'bin', 'libprojector.so')
+
+# Exported C symbols and their declarations in include/ctprojector.h.
+SYMBOLS = [
+    'SetDevice',
+    'cSiddonConeProjectionArbitrary', 'cupySiddonConeProjectionArbitrary',
+    'cSiddonConeBackprojectionArbitrary', 'cupySiddonConeBackprojectionArbitrary',
+    'cDistanceDrivenFanProjection', 'cupyDistanceDrivenFanProjection',
+    'cDistanceDrivenParallelProjection', 'cupyDistanceDrivenParallelProjection',
+    'cDistanceDrivenFanBackprojection', 'cupyDistanceDrivenFanBackprojection',
+    'cDistanceDrivenParallelBackprojection', 'cupyDistanceDrivenParallelBackprojection',
+    'cDistanceDrivenTomoProjection', 'cupyDistanceDrivenTomoProjection',
+    'cDistanceDrivenTomoBackprojection', 'cupyDistanceDrivenTomoBackprojection',
+    'cfbpFanFilter', 'cupyfbpFanFilter', 'cfbpParallelFilter', 'cupyfbpParallelFilter',
+    'cfbpFanBackprojection', 'cupyfbpFanBackprojection',
+    'ctpGetLastErrorMessage', 'ctpReleaseWorkspace',
+]
+
+
+class ProjectorLibraryError(RuntimeError):
+    pass
+
+
+_module = None
+
+
+def module():
+    """The loaded ctypes library (loaded on first use)."""
+    global _module
+    if _module is None:
+        if not os.path.exists(LIB_PATH):
+            raise ProjectorLibraryError(
+                '%s is not built; run `python -m ctprojector_amd.build` (hipcc, gfx950)' % LIB_PATH)
+        try:
+            lib = ctypes.CDLL(LIB_PATH)
+        except OSError as e:
+            raise ProjectorLibraryError('cannot load %s: %s' % (LIB_PATH, e))
+        for name in SYMBOLS:
+            fn = getattr(lib, name)
+            fn.restype = ctypes.c_int
+        lib.ctpGetLastErrorMessage.restype = ctypes.c_char_p
+        _module = lib
+    return _module
+
+
+def last_error_message() -> str:
+    msg = module().ctpGetLastErrorMessage()
+    return msg.decode() if msg else ''
+
+
+def release_workspace() -> int:
+    return module().ctpReleaseWorkspace()

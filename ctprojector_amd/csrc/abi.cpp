@@ -1,0 +1,433 @@
+// C ABI of libprojector.so.  The exported symbols, argument lists and
+// semantics are those of the reference's libprojector.so (declared in
+// include/ctprojector.h with the reference file:line each replaces):
+//   c*     host pointers: staged through device buffers allocated per call;
+//   cupy*  device pointers owned by the caller (any HIP device memory, e.g.
+//          torch tensors); outputs must be zero-initialised where the reference
+//          accumulates.
+// Every call runs on the current device's null stream and returns when the
+// results are complete.  Failures never abort: the function prints
+// "<name>() failed: <reason>" to stderr like the reference and returns a
+// non-zero hipError_t code (0 = success); ctpGetLastErrorMessage() returns the
+// text of the calling thread's last failure.
+#include "common.h"
+#include "ops.h"
+#include "runtime.h"
+
+#include <cstdio>
+#include <cstring>
+#include <iostream>
+#include <limits>
+#include <string>
+#include <vector>
+
+using namespace ctp;
+
+namespace {
+
+template <typename F>
+int guarded(const char* name, F&& f) {
+    (void)hipGetLastError();   // do not report errors left over from earlier work
+    try {
+        f();
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) throw Error((int)e, std::string("kernel failure: ") + hipGetErrorString(e));
+        return 0;
+    } catch (const Error& e) {
+        std::string m = std::string(name) + "() failed: " + e.what();
+        set_last_error(m);
+        std::cerr << m << std::endl;
+        return e.code ? e.code : (int)hipErrorUnknown;
+    } catch (const std::exception& e) {
+        std::string m = std::string(name) + "() failed: " + e.what();
+        set_last_error(m);
+        std::cerr << m << std::endl;
+        return (int)hipErrorUnknown;
+    } catch (...) {
+        std::string m = std::string(name) + "() failed: unknown exception";
+        set_last_error(m);
+        std::cerr << m << std::endl;
+        return (int)hipErrorUnknown;
+    }
+}
+
+constexpr size_t kIntMax = (size_t)std::numeric_limits<int>::max();
+
+void need(bool ok, const std::string& msg) {
+    if (!ok) fail(msg);
+}
+
+Grid make_grid(size_t nx, size_t ny, size_t nz, float dx, float dy, float dz, float cx, float cy, float cz) {
+    need(nx <= kIntMax && ny <= kIntMax && nz <= kIntMax, "image dimension too large");
+    need(nx * ny <= kIntMax && nx * ny * nz <= kIntMax, "image of more than 2^31-1 voxels per batch item");
+    need(ny <= 65535 && nz <= 65535, "ny or nz above 65535");
+    need(dx != 0 && dy != 0 && dz != 0, "zero voxel size");
+    return Grid{(int)nx, (int)ny, (int)nz, dx, dy, dz, cx, cy, cz};
+}
+
+Det make_det(size_t nu, size_t nv, size_t nview, float du, float dv, float off_u, float off_v) {
+    need(nu <= kIntMax && nv <= kIntMax && nview <= kIntMax, "detector dimension too large");
+    need(nview <= 65535, "more than 65535 views per call");
+    need(nv <= 65535 * 16, "more than 1048560 detector rows");
+    need(nu * nv <= kIntMax, "detector of more than 2^31-1 cells");
+    need(du != 0 && dv != 0, "zero detector pitch");
+    return Det{(int)nu, (int)nv, du, dv, off_u, off_v};
+}
+
+bool empty(size_t B, size_t a, size_t b, size_t c, size_t d, size_t e, size_t f) {
+    return B == 0 || a == 0 || b == 0 || c == 0 || d == 0 || e == 0 || f == 0;
+}
+
+void need_ptr(const void* p, const char* what) {
+    if (!p) fail(std::string("null pointer: ") + what);
+}
+
+void up(void* dst, const void* src, size_t bytes) {
+    if (bytes) CTP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+}
+void down(void* dst, const void* src, size_t bytes) {
+    if (bytes) CTP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+}
+std::vector<float> host_copy(const float* dev, size_t n) {
+    std::vector<float> h(n);
+    down(h.data(), dev, n * sizeof(float));
+    return h;
+}
+void zero(void* dev, size_t bytes) {
+    if (bytes) CTP_CHECK(hipMemset(dev, 0, bytes));
+}
+void sync() { CTP_CHECK(hipStreamSynchronize(nullptr)); }
+
+}  // namespace
+
+#define CTP_EXPORT extern "C" __attribute__((visibility("default")))
+
+CTP_EXPORT int SetDevice(int device) {
+    (void)hipGetLastError();
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) return (int)e;
+    if (device < 0 || device >= n) return (int)hipErrorInvalidDevice;
+    return (int)hipSetDevice(device);
+}
+
+CTP_EXPORT const char* ctpGetLastErrorMessage(void) { return last_error(); }
+
+CTP_EXPORT int ctpReleaseWorkspace(void) {
+    return guarded("ctpReleaseWorkspace", [&] { release_all_workspaces(); });
+}
+
+// ===========================================================================
+// Siddon cone beam, arbitrary trajectory (cuda/siddonCone.cu:137-424)
+// ===========================================================================
+#define CONE_ARGS                                                                                                   \
+    const float *detCenter, const float *detU, const float *detV, const float *src, size_t nBatches, size_t nx,   \
+        size_t ny, size_t nz, float dx, float dy, float dz, float cx, float cy, float cz, size_t nu, size_t nv,     \
+        size_t nview, float du, float dv, float off_u, float off_v
+
+static void cone_common(size_t nBatches, size_t nx, size_t ny, size_t nz, float dx, float dy, float dz, float cx,
+                        float cy, float cz, size_t nu, size_t nv, size_t nview, float du, float dv, float off_u,
+                        float off_v, Grid& g, Det& d) {
+    g = make_grid(nx, ny, nz, dx, dy, dz, cx, cy, cz);
+    d = make_det(nu, nv, nview, du, dv, off_u, off_v);
+    need(nBatches <= kIntMax, "batch too large");
+}
+
+CTP_EXPORT int cupySiddonConeProjectionArbitrary(float* prj, const float* img, CONE_ARGS) {
+    return guarded("cupySiddonConeProjectionArbitrary", [&] {
+        if (empty(nBatches, nx, ny, nz, nu, nv, nview)) return;
+        Grid g; Det d;
+        cone_common(nBatches, nx, ny, nz, dx, dy, dz, cx, cy, cz, nu, nv, nview, du, dv, off_u, off_v, g, d);
+        need_ptr(prj, "prj"); need_ptr(img, "img"); need_ptr(detCenter, "detCenter"); need_ptr(detU, "detU");
+        need_ptr(detV, "detV"); need_ptr(src, "src");
+        siddon_cone_fp(prj, img, ConeGeomDev{detCenter, detU, detV, src}, (int)nBatches, g, d, (int)nview, nullptr);
+        sync();
+    });
+}
+
+CTP_EXPORT int cSiddonConeProjectionArbitrary(float* prj, const float* img, CONE_ARGS) {
+    return guarded("cSiddonConeProjectionArbitrary", [&] {
+        if (empty(nBatches, nx, ny, nz, nu, nv, nview)) return;
+        Grid g; Det d;
+        cone_common(nBatches, nx, ny, nz, dx, dy, dz, cx, cy, cz, nu, nv, nview, du, dv, off_u, off_v, g, d);
+        need_ptr(prj, "prj"); need_ptr(img, "img"); need_ptr(detCenter, "detCenter"); need_ptr(detU, "detU");
+        need_ptr(detV, "detV"); need_ptr(src, "src");
+        const size_t nimg = nBatches * nx * ny * nz, nprj = nBatches * nu * nv * nview, ng = 3 * nview;
+        DeviceBuffer di(nimg * 4), dp(nprj * 4), dg(4 * ng * 4);
+        float* geo = dg.as<float>();
+        up(di.p, img, nimg * 4);
+        up(geo, detCenter, ng * 4); up(geo + ng, detU, ng * 4); up(geo + 2 * ng, detV, ng * 4); up(geo + 3 * ng, src, ng * 4);
+        zero(dp.p, nprj * 4);   // the host API returns A img (cuda/siddonCone.cu:209)
+        siddon_cone_fp(dp.as<float>(), di.as<float>(), ConeGeomDev{geo, geo + ng, geo + 2 * ng, geo + 3 * ng},
+                       (int)nBatches, g, d, (int)nview, nullptr);
+        sync();
+        down(prj, dp.p, nprj * 4);
+    });
+}
+
+CTP_EXPORT int cupySiddonConeBackprojectionArbitrary(float* img, const float* prj, CONE_ARGS) {
+    return guarded("cupySiddonConeBackprojectionArbitrary", [&] {
+        if (empty(nBatches, nx, ny, nz, nu, nv, nview)) return;
+        Grid g; Det d;
+        cone_common(nBatches, nx, ny, nz, dx, dy, dz, cx, cy, cz, nu, nv, nview, du, dv, off_u, off_v, g, d);
+        need_ptr(prj, "prj"); need_ptr(img, "img"); need_ptr(detCenter, "detCenter"); need_ptr(detU, "detU");
+        need_ptr(detV, "detV"); need_ptr(src, "src");
+        const size_t ng = 3 * nview;
+        std::vector<float> hc = host_copy(detCenter, ng), hu = host_copy(detU, ng), hv = host_copy(detV, ng),
+                           hs = host_copy(src, ng);
+        siddon_cone_bp(img, prj, ConeGeomDev{detCenter, detU, detV, src},
+                       ConeGeomHost{hc.data(), hu.data(), hv.data(), hs.data()}, (int)nBatches, g, d, (int)nview,
+                       nullptr);
+        sync();
+    });
+}
+
+CTP_EXPORT int cSiddonConeBackprojectionArbitrary(float* img, const float* prj, CONE_ARGS) {
+    return guarded("cSiddonConeBackprojectionArbitrary", [&] {
+        if (empty(nBatches, nx, ny, nz, nu, nv, nview)) return;
+        Grid g; Det d;
+        cone_common(nBatches, nx, ny, nz, dx, dy, dz, cx, cy, cz, nu, nv, nview, du, dv, off_u, off_v, g, d);
+        need_ptr(prj, "prj"); need_ptr(img, "img"); need_ptr(detCenter, "detCenter"); need_ptr(detU, "detU");
+        need_ptr(detV, "detV"); need_ptr(src, "src");
+        const size_t nimg = nBatches * nx * ny * nz, nprj = nBatches * nu * nv * nview, ng = 3 * nview;
+        DeviceBuffer di(nimg * 4), dp(nprj * 4), dg(4 * ng * 4);
+        float* geo = dg.as<float>();
+        up(dp.p, prj, nprj * 4);
+        up(geo, detCenter, ng * 4); up(geo + ng, detU, ng * 4); up(geo + 2 * ng, detV, ng * 4); up(geo + 3 * ng, src, ng * 4);
+        zero(di.p, nimg * 4);
+        siddon_cone_bp(di.as<float>(), dp.as<float>(), ConeGeomDev{geo, geo + ng, geo + 2 * ng, geo + 3 * ng},
+                       ConeGeomHost{detCenter, detU, detV, src}, (int)nBatches, g, d, (int)nview, nullptr);
+        sync();
+        down(img, di.p, nimg * 4);
+    });
+}
+
+// ===========================================================================
+// Distance driven fan / parallel (cuda/distanceDriven2D.cu:109-634)
+// ===========================================================================
+#define DD_GRID_ARGS                                                                                          \
+    size_t nBatches, size_t nx, size_t ny, size_t nz, float dx, float dy, float dz, float cx, float cy, float cz, \
+        size_t nu, size_t nv, size_t nview, float da, float dv, float off_a, float off_v
+
+template <bool FP, bool DEV>
+static int dd_entry(const char* name, float* out, const float* in, const float* deg, size_t nBatches, size_t nx,
+                    size_t ny, size_t nz, float dx, float dy, float dz, float cx, float cy, float cz, size_t nu,
+                    size_t nv, size_t nview, float da, float dv, float off_a, float off_v, float dsd, float dso,
+                    int type, bool fan) {
+    return guarded(name, [&] {
+        if (empty(nBatches, nx, ny, nz, nu, nv, nview)) return;
+        Grid g = make_grid(nx, ny, nz, dx, dy, dz, cx, cy, cz);
+        Det d = make_det(nu, nv, nview, da, dv, off_a, off_v);
+        need(nBatches <= kIntMax, "batch too large");
+        need_ptr(out, "out"); need_ptr(in, "in"); need_ptr(deg, "angles");
+        const size_t nimg = nBatches * nx * ny * nz, nprj = nBatches * nu * nv * nview;
+        if (DEV) {
+            std::vector<float> hdeg = host_copy(deg, nview);
+            if (FP) dd2d_fp(out, in, hdeg.data(), (int)nBatches, g, d, (int)nview, dsd, dso, type, fan, nullptr);
+            else dd2d_bp(out, in, hdeg.data(), (int)nBatches, g, d, (int)nview, dsd, dso, type, fan, nullptr);
+            sync();
+        } else {
+            const size_t nin = FP ? nimg : nprj, nout = FP ? nprj : nimg;
+            DeviceBuffer di(nin * 4), dout(nout * 4);
+            up(di.p, in, nin * 4);
+            zero(dout.p, nout * 4);
+            if (FP) dd2d_fp(dout.as<float>(), di.as<float>(), deg, (int)nBatches, g, d, (int)nview, dsd, dso, type, fan, nullptr);
+            else dd2d_bp(dout.as<float>(), di.as<float>(), deg, (int)nBatches, g, d, (int)nview, dsd, dso, type, fan, nullptr);
+            sync();
+            down(out, dout.p, nout * 4);
+        }
+    });
+}
+
+CTP_EXPORT int cupyDistanceDrivenFanProjection(float* prj, const float* img, const float* deg, DD_GRID_ARGS, float dsd,
+                                               float dso, int typeProjector) {
+    return dd_entry<true, true>("cupyDistanceDrivenFanProjection", prj, img, deg, nBatches, nx, ny, nz, dx, dy, dz, cx,
+                                cy, cz, nu, nv, nview, da, dv, off_a, off_v, dsd, dso, typeProjector, true);
+}
+CTP_EXPORT int cDistanceDrivenFanProjection(float* prj, const float* img, const float* deg, DD_GRID_ARGS, float dsd,
+                                            float dso, int typeProjector) {
+    return dd_entry<true, false>("cDistanceDrivenFanProjection", prj, img, deg, nBatches, nx, ny, nz, dx, dy, dz, cx,
+                                 cy, cz, nu, nv, nview, da, dv, off_a, off_v, dsd, dso, typeProjector, true);
+}
+CTP_EXPORT int cupyDistanceDrivenParallelProjection(float* prj, const float* img, const float* deg, DD_GRID_ARGS,
+                                                    int typeProjector) {
+    return dd_entry<true, true>("cupyDistanceDrivenParallelProjection", prj, img, deg, nBatches, nx, ny, nz, dx, dy, dz,
+                                cx, cy, cz, nu, nv, nview, da, dv, off_a, off_v, 1000.f, 500.f, typeProjector, false);
+}
+CTP_EXPORT int cDistanceDrivenParallelProjection(float* prj, const float* img, const float* deg, DD_GRID_ARGS,
+                                                 int typeProjector) {
+    return dd_entry<true, false>("cDistanceDrivenParallelProjection", prj, img, deg, nBatches, nx, ny, nz, dx, dy, dz,
+                                 cx, cy, cz, nu, nv, nview, da, dv, off_a, off_v, 1000.f, 500.f, typeProjector, false);
+}
+CTP_EXPORT int cupyDistanceDrivenFanBackprojection(float* img, const float* prj, const float* deg, DD_GRID_ARGS,
+                                                   float dsd, float dso, int typeProjector) {
+    return dd_entry<false, true>("cupyDistanceDrivenFanBackprojection", img, prj, deg, nBatches, nx, ny, nz, dx, dy, dz,
+                                 cx, cy, cz, nu, nv, nview, da, dv, off_a, off_v, dsd, dso, typeProjector, true);
+}
+CTP_EXPORT int cDistanceDrivenFanBackprojection(float* img, const float* prj, const float* deg, DD_GRID_ARGS, float dsd,
+                                                float dso, int typeProjector) {
+    return dd_entry<false, false>("cDistanceDrivenFanBackprojection", img, prj, deg, nBatches, nx, ny, nz, dx, dy, dz,
+                                  cx, cy, cz, nu, nv, nview, da, dv, off_a, off_v, dsd, dso, typeProjector, true);
+}
+CTP_EXPORT int cupyDistanceDrivenParallelBackprojection(float* img, const float* prj, const float* deg, DD_GRID_ARGS,
+                                                        int typeProjector) {
+    return dd_entry<false, true>("cupyDistanceDrivenParallelBackprojection", img, prj, deg, nBatches, nx, ny, nz, dx,
+                                 dy, dz, cx, cy, cz, nu, nv, nview, da, dv, off_a, off_v, 1000.f, 500.f, typeProjector,
+                                 false);
+}
+CTP_EXPORT int cDistanceDrivenParallelBackprojection(float* img, const float* prj, const float* deg, DD_GRID_ARGS,
+                                                     int typeProjector) {
+    return dd_entry<false, false>("cDistanceDrivenParallelBackprojection", img, prj, deg, nBatches, nx, ny, nz, dx, dy,
+                                  dz, cx, cy, cz, nu, nv, nview, da, dv, off_a, off_v, 1000.f, 500.f, typeProjector,
+                                  false);
+}
+
+// ===========================================================================
+// Distance driven tomosynthesis (cuda/distanceDriven3D.cu:184-439)
+// ===========================================================================
+#define TOMO_ARGS                                                                                                 \
+    const float *detCenter, const float *src, size_t nBatches, size_t nx, size_t ny, size_t nz, float dx, float dy, \
+        float dz, float cx, float cy, float cz, size_t nu, size_t nv, size_t nview, float du, float dv, float off_u, \
+        float off_v, int typeProjector
+
+template <bool FP, bool DEV>
+static int tomo_entry(const char* name, float* out, const float* in, const float* detCenter, const float* src,
+                      size_t nBatches, size_t nx, size_t ny, size_t nz, float dx, float dy, float dz, float cx, float cy,
+                      float cz, size_t nu, size_t nv, size_t nview, float du, float dv, float off_u, float off_v,
+                      int type) {
+    return guarded(name, [&] {
+        if (empty(nBatches, nx, ny, nz, nu, nv, nview)) return;
+        Grid g = make_grid(nx, ny, nz, dx, dy, dz, cx, cy, cz);
+        Det d = make_det(nu, nv, nview, du, dv, off_u, off_v);
+        need(nBatches <= kIntMax, "batch too large");
+        need_ptr(out, "out"); need_ptr(in, "in"); need_ptr(detCenter, "detCenter"); need_ptr(src, "src");
+        const size_t nimg = nBatches * nx * ny * nz, nprj = nBatches * nu * nv * nview, ng = 3 * nview;
+        if (DEV) {
+            if (FP) tomo_fp(out, in, detCenter, src, (int)nBatches, g, d, (int)nview, type, nullptr);
+            else tomo_bp(out, in, detCenter, src, (int)nBatches, g, d, (int)nview, type, nullptr);
+            sync();
+        } else {
+            const size_t nin = FP ? nimg : nprj, nout = FP ? nprj : nimg;
+            DeviceBuffer di(nin * 4), dout(nout * 4), dg(2 * ng * 4);
+            float* geo = dg.as<float>();
+            up(di.p, in, nin * 4);
+            up(geo, detCenter, ng * 4);
+            up(geo + ng, src, ng * 4);
+            zero(dout.p, nout * 4);
+            if (FP) tomo_fp(dout.as<float>(), di.as<float>(), geo, geo + ng, (int)nBatches, g, d, (int)nview, type, nullptr);
+            else tomo_bp(dout.as<float>(), di.as<float>(), geo, geo + ng, (int)nBatches, g, d, (int)nview, type, nullptr);
+            sync();
+            down(out, dout.p, nout * 4);
+        }
+    });
+}
+
+CTP_EXPORT int cDistanceDrivenTomoProjection(float* prj, const float* img, TOMO_ARGS) {
+    return tomo_entry<true, false>("cDistanceDrivenTomoProjection", prj, img, detCenter, src, nBatches, nx, ny, nz, dx,
+                                   dy, dz, cx, cy, cz, nu, nv, nview, du, dv, off_u, off_v, typeProjector);
+}
+CTP_EXPORT int cupyDistanceDrivenTomoProjection(float* prj, const float* img, TOMO_ARGS) {
+    return tomo_entry<true, true>("cupyDistanceDrivenTomoProjection", prj, img, detCenter, src, nBatches, nx, ny, nz, dx,
+                                  dy, dz, cx, cy, cz, nu, nv, nview, du, dv, off_u, off_v, typeProjector);
+}
+CTP_EXPORT int cDistanceDrivenTomoBackprojection(float* img, const float* prj, TOMO_ARGS) {
+    return tomo_entry<false, false>("cDistanceDrivenTomoBackprojection", img, prj, detCenter, src, nBatches, nx, ny, nz,
+                                    dx, dy, dz, cx, cy, cz, nu, nv, nview, du, dv, off_u, off_v, typeProjector);
+}
+CTP_EXPORT int cupyDistanceDrivenTomoBackprojection(float* img, const float* prj, TOMO_ARGS) {
+    return tomo_entry<false, true>("cupyDistanceDrivenTomoBackprojection", img, prj, detCenter, src, nBatches, nx, ny,
+                                   nz, dx, dy, dz, cx, cy, cz, nu, nv, nview, du, dv, off_u, off_v, typeProjector);
+}
+
+// ===========================================================================
+// Ramp filters (cuda/fbp.cu:165-258, cuda/fbpParallel.cu:111-198)
+// ===========================================================================
+template <bool DEV>
+static int filter_entry(const char* name, float* out, const float* in, int nBatches, size_t nu, size_t nv, size_t nview,
+                        float da, float off_u, float dsd, float dso, int filter, bool equispace) {
+    return guarded(name, [&] {
+        need(nBatches >= 0, "negative batch");
+        if (empty((size_t)nBatches, nu, nv, nview, 1, 1, 1)) return;
+        need(nu <= 5000 && nu * nv <= kIntMax && nview <= kIntMax, "detector dimension too large");
+        need(da != 0, "zero detector pitch");
+        need(equispace || dso != 0, "zero dso");
+        need_ptr(out, "out"); need_ptr(in, "in");
+        const size_t n = (size_t)nBatches * nu * nv * nview;
+        if (DEV) {
+            ramp_filter(out, in, nBatches, (int)nu, (int)nv, (int)nview, da, off_u, dsd, dso, filter, equispace, nullptr);
+            sync();
+        } else {
+            DeviceBuffer di(n * 4), dout(n * 4);
+            up(di.p, in, n * 4);
+            ramp_filter(dout.as<float>(), di.as<float>(), nBatches, (int)nu, (int)nv, (int)nview, da, off_u, dsd, dso,
+                        filter, equispace, nullptr);
+            sync();
+            down(out, dout.p, n * 4);
+        }
+    });
+}
+
+CTP_EXPORT int cupyfbpFanFilter(float* pFPrj, const float* pPrj, int nBatches, size_t nu, size_t nv, size_t nview,
+                                float da, float dv, float off_a, float off_v, float dsd, float dso, int typeFilter) {
+    (void)dv; (void)off_v;
+    return filter_entry<true>("cupyfbpFanFilter", pFPrj, pPrj, nBatches, nu, nv, nview, da, off_a, dsd, dso, typeFilter,
+                              false);
+}
+CTP_EXPORT int cfbpFanFilter(float* pFPrj, const float* pPrj, int nBatches, size_t nu, size_t nv, size_t nview,
+                             float da, float dv, float off_a, float off_v, float dsd, float dso, int typeFilter) {
+    (void)dv; (void)off_v;
+    return filter_entry<false>("cfbpFanFilter", pFPrj, pPrj, nBatches, nu, nv, nview, da, off_a, dsd, dso, typeFilter,
+                               false);
+}
+CTP_EXPORT int cupyfbpParallelFilter(float* pFPrj, const float* pPrj, int nBatches, size_t nu, size_t nv, size_t nview,
+                                     float du, float dv, float off_u, float off_v, int typeFilter) {
+    (void)dv; (void)off_v;
+    return filter_entry<true>("cupyfbpParallelFilter", pFPrj, pPrj, nBatches, nu, nv, nview, du, off_u, 0.f, 1.f,
+                              typeFilter, true);
+}
+CTP_EXPORT int cfbpParallelFilter(float* pFPrj, const float* pPrj, int nBatches, size_t nu, size_t nv, size_t nview,
+                                  float du, float dv, float off_u, float off_v, int typeFilter) {
+    (void)dv; (void)off_v;
+    return filter_entry<false>("cfbpParallelFilter", pFPrj, pPrj, nBatches, nu, nv, nview, du, off_u, 0.f, 1.f,
+                               typeFilter, true);
+}
+
+// ===========================================================================
+// Fan FBP backprojection (cuda/fbp.cu:362-481)
+// ===========================================================================
+template <bool DEV>
+static int fbp_bp_entry(const char* name, float* img, const float* prj, const float* deg, size_t nBatches, size_t nx,
+                        size_t ny, size_t nz, float dx, float dy, float dz, float cx, float cy, float cz, size_t nu,
+                        size_t nv, size_t nview, float da, float dv, float off_a, float off_v, float dsd, float dso) {
+    return guarded(name, [&] {
+        if (empty(nBatches, nx, ny, nz, nu, nv, nview)) return;
+        Grid g = make_grid(nx, ny, nz, dx, dy, dz, cx, cy, cz);
+        Det d = make_det(nu, nv, nview, da, dv, off_a, off_v);
+        need(nBatches <= kIntMax, "batch too large");
+        need_ptr(img, "img"); need_ptr(prj, "prj"); need_ptr(deg, "angles");
+        const size_t nimg = nBatches * nx * ny * nz, nprj = nBatches * nu * nv * nview;
+        if (DEV) {
+            std::vector<float> hdeg = host_copy(deg, nview);
+            fan_fbp_bp(img, prj, hdeg.data(), (int)nBatches, g, d, (int)nview, dsd, dso, nullptr);
+            sync();
+        } else {
+            DeviceBuffer di(nimg * 4), dp(nprj * 4);
+            up(dp.p, prj, nprj * 4);
+            zero(di.p, nimg * 4);
+            fan_fbp_bp(di.as<float>(), dp.as<float>(), deg, (int)nBatches, g, d, (int)nview, dsd, dso, nullptr);
+            sync();
+            down(img, di.p, nimg * 4);
+        }
+    });
+}
+
+CTP_EXPORT int cupyfbpFanBackprojection(float* pImg, const float* pPrj, const float* pDeg, DD_GRID_ARGS, float dsd,
+                                        float dso) {
+    return fbp_bp_entry<true>("cupyfbpFanBackprojection", pImg, pPrj, pDeg, nBatches, nx, ny, nz, dx, dy, dz, cx, cy, cz,
+                              nu, nv, nview, da, dv, off_a, off_v, dsd, dso);
+}
+CTP_EXPORT int cfbpFanBackprojection(float* pImg, const float* pPrj, const float* pDeg, DD_GRID_ARGS, float dsd,
+                                     float dso) {
+    return fbp_bp_entry<false>("cfbpFanBackprojection", pImg, pPrj, pDeg, nBatches, nx, ny, nz, dx, dy, dz, cx, cy, cz,
+                               nu, nv, nview, da, dv, off_a, off_v, dsd, dso);
+}

@@ -1,0 +1,46 @@
+// Shared host/device definitions for the MI355X projector library.
+//
+// Conventions follow the reference (wudufan/CTProjector):
+//   images       img[B][nz][ny][nx]          (cuda/projector.h:100-123)
+//   projections  prj[B][nview][nv][nu]
+//   geometry     rows [nview][3] as (x, y, z) with x on the fastest image axis
+//   image index space has 0 at the corner of the first voxel
+//                (PhysicsToImg, cuda/projector.h:63-71).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace ctp {
+
+struct Grid {                     // cuda/projector.h:7-18
+    int nx, ny, nz;
+    float dx, dy, dz, cx, cy, cz;
+};
+
+struct Det {                      // cuda/projector.h:20-28
+    int nu, nv;
+    float du, dv, off_u, off_v;
+};
+
+// typeProjector bits of the 2D distance-driven projector
+// (cuda/distanceDriven.h:33-44, cuda/distanceDriven2D.cu:12-25).
+enum : int { DD_FBP = 1, DD_BRANCHLESS = 2, DD_FORCE_FBP = 4 };
+
+// filter codes (cuda/fbp.h:5-7)
+enum : int { FILTER_RL = 0, FILTER_HAMMING = 1, FILTER_HANN = 2, FILTER_COSINE = 3 };
+
+// The reference's constants (cuda/projector.h:5, cuda/siddon.cu:11,
+// cuda/distanceDriven2DBoxInt.cu:18).
+constexpr float kPI = 3.141592657f;
+constexpr float kEps = 1e-6f;
+constexpr float kDDMargin = 0.12f;
+
+__host__ __device__ inline int clampi(int x, int s, int e) { return x < s ? s : (x >= e ? e - 1 : x); }
+__host__ __device__ inline float clampf(float x, float s, float e) { return x < s ? s : (x > e ? e : x); }
+
+inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
+
+}  // namespace ctp

@@ -1,0 +1,513 @@
+// Distance-driven fan-beam (equiangular) and parallel-beam projectors on
+// MI355X (gfx950): cuda/distanceDriven2DBoxInt.cu, cuda/distanceDriven2DBranchless.cu,
+// cuda/distanceDriven2DUtils.cu of the reference.
+//
+// The reference projector is a stack of 2D problems: a ray's footprint on an
+// image row depends on (view, detector column) only, and a pixel's footprint on
+// the detector on (view, x, y) only; z enters through a per-detector-row linear
+// interpolation (FP) or a per-slice row lookup (BP).  Here every image slice of
+// every batch item is a "channel":
+//
+//   FP: one lane per (view, column) carries CH+1 channel accumulators; for each
+//       image row it computes the footprint weights once and applies them to a
+//       contiguous channel vector of the image stored channel-last
+//       ([y][x][C] for y-main views, [x][y][C] for x-main views).  The detector
+//       rows are produced at the end from the accumulators (z interpolation).
+//   BP: one lane per pixel (x, y) carries CH channel accumulators; for each view
+//       it computes the detector footprint once and applies it to a contiguous
+//       channel vector of the pre-weighted projections stored
+//       [view][u][C] (one channel per image slice, the slice's detector row
+//       already selected).  No atomics: every pixel column has one owner.
+//
+// Decisions that are discontinuous in the view angle (the 45-degree main-axis
+// switch of the backprojector and of the BP pre-weight) are taken in FP64 on
+// the host so that they agree with the FP64 oracle.
+#include "common.h"
+#include "runtime.h"
+#include "ops.h"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace ctp {
+
+struct DDView {
+    float c, s;        // cos / sin of the view angle (FP64 rounded to FP32)
+    float deg;         // the view angle
+    float wx;          // blend weight of the y-main trace, clamp((|c|-|s|+m)/2m, 0, 1)
+    float rev;         // parallel FP with force-FBP: 1/(dy/|c|) or 1/(dx/|s|)
+    int flags;         // bit0: FP y-main trace, bit1: FP x-main trace, bit2: BP y-main (|c| > |s|)
+    float pad0, pad1;
+};
+
+static std::vector<DDView> make_dd_views(const float* deg, int nview, const Grid& g) {
+    std::vector<DDView> v(nview);
+    const double m = (double)kDDMargin;
+    for (int i = 0; i < nview; i++) {
+        const double d = deg[i];
+        const double c = std::cos(d), s = std::sin(d);
+        double wx = (std::fabs(c) - std::fabs(s) + m) / (2 * m);
+        wx = std::min(1.0, std::max(0.0, wx));
+        DDView& o = v[i];
+        o.c = (float)c;
+        o.s = (float)s;
+        o.deg = deg[i];
+        o.wx = (float)wx;
+        int f = 0;
+        if (std::fabs(c) > std::fabs(s) - m) f |= 1;
+        if (std::fabs(c) < std::fabs(s) + m) f |= 2;
+        const bool ymain = std::fabs(c) > std::fabs(s);
+        if (ymain) f |= 4;
+        o.rev = ymain ? (float)(std::fabs(c) / g.dy) : (float)(std::fabs(s) / g.dx);
+        o.flags = f;
+        o.pad0 = o.pad1 = 0;
+    }
+    return v;
+}
+
+// The branchless projectors switch main axes at 45 degrees without a blend
+// (cuda/distanceDriven2DBranchless.cu:73,423,513); with prefix sums evaluated
+// exactly they equal the box-integral projector with that switch.
+static void hard_switch(std::vector<DDView>& v) {
+    for (auto& o : v) {
+        const bool ymain = (o.flags & 4) != 0;
+        o.wx = ymain ? 1.f : 0.f;
+        o.flags = (o.flags & 4) | (ymain ? 1 : 2);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// layout transforms
+// ---------------------------------------------------------------------------
+// planes P[c][y][x] (c < C)  ->  O[pix][cs] with pix = y*nx+x (xmajor=false) or
+// x*ny+y (xmajor=true); channels >= C are written as zero.
+__global__ __launch_bounds__(256) void to_channel_last(float* __restrict__ out, const float* __restrict__ in, int C,
+                                                       int cs, int nx, int ny, bool xmajor) {
+    __shared__ float tile[32][33];
+    const int x0 = blockIdx.x * 32, c0 = blockIdx.y * 32, y = blockIdx.z;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
+    for (int k = ty; k < 32; k += 8) {
+        const int c = c0 + k, x = x0 + tx;
+        tile[k][tx] = (c < C && x < nx) ? in[((size_t)c * ny + y) * nx + x] : 0.f;
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        const int x = x0 + k, c = c0 + tx;
+        if (x < nx && c < cs) {
+            const size_t pix = xmajor ? (size_t)x * ny + y : (size_t)y * nx + x;
+            out[pix * cs + c] = tile[tx][k];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// forward projection
+// ---------------------------------------------------------------------------
+struct FPRow {        // one output row (b, iv) of a channel chunk
+    int b, iv;
+    int j0, j1;       // accumulator indices of slices iz and iz1 relative to the chunk
+    float w1;         // weight of iz1 (iz gets 1 - w1)
+    float pad[3];
+};
+
+// Footprint of one line [x1, x2] (index units, unclamped) on a pixel row of
+// length n: the reference's IntegralBoxX (cuda/cudaMath.cu:533-565) divided by
+// (x2 - x1), applied to a channel vector at 'row' (stride cs between pixels).
+template <int CH>
+__device__ inline void dd_line(float* acc, const float* __restrict__ row, int cs, float x1, float x2, int n, float scale) {
+    if (x1 > x2) { const float t = x1; x1 = x2; x2 = t; }
+    const float a = clampf(x1, 0.f, (float)n), b = clampf(x2, 0.f, (float)n);
+    if (!(b > a)) return;
+    const float inv = scale / (x2 - x1);
+    const int i1 = min((int)a, n - 1), i2 = min((int)b, n - 1);
+    for (int i = i1; i <= i2; i++) {
+        const float w = (fminf(b, (float)(i + 1)) - fmaxf(a, (float)i)) * inv;
+        const float* p = row + (size_t)i * cs;
+        if constexpr (CH % 4 == 0) {
+#pragma unroll
+            for (int j = 0; j < CH; j += 4) {
+                const float4 q = *reinterpret_cast<const float4*>(p + j);
+                acc[j] = fmaf(w, q.x, acc[j]);
+                acc[j + 1] = fmaf(w, q.y, acc[j + 1]);
+                acc[j + 2] = fmaf(w, q.z, acc[j + 2]);
+                acc[j + 3] = fmaf(w, q.w, acc[j + 3]);
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < CH; j++) acc[j] = fmaf(w, p[j], acc[j]);
+        }
+        acc[CH] = fmaf(w, p[CH], acc[CH]);
+    }
+}
+
+template <int CH, bool FAN>
+__global__ __launch_bounds__(128) void dd_fp_kernel(float* __restrict__ prj, const float* __restrict__ imgY,
+                                                    const float* __restrict__ imgX, const DDView* __restrict__ views,
+                                                    const FPRow* __restrict__ rows, const int* __restrict__ row_start,
+                                                    int cs, Grid g, Det d, int nview, float dsd, float dso, bool force) {
+    __shared__ float lds[128][CH + 2];
+    const int iu = blockIdx.x * 128 + threadIdx.x;
+    const int view = blockIdx.y, chunk = blockIdx.z;
+    const int c0 = chunk * CH;
+    const DDView V = views[view];
+    float acc[CH + 1];
+#pragma unroll
+    for (int j = 0; j <= CH; j++) acc[j] = 0.f;
+    const bool active = iu < d.nu;
+    if (active) {
+        const float c = V.c, s = V.s;
+        float2 s1, s2, d1, d2;
+        float a = 0.f;
+        if (FAN) {
+            // cuda/distanceDriven2DBoxInt.cu:211-226
+            a = (-((float)iu - (float)(d.nu - 1) / 2.0f) - d.off_u) * d.du;
+            const float2 src = make_float2(dso * s, -dso * c);
+            const float e1x = dsd * sinf(a - d.du / 2), e1y = -dso + dsd * cosf(a - d.du / 2);
+            const float e2x = dsd * sinf(a + d.du / 2), e2y = -dso + dsd * cosf(a + d.du / 2);
+            s1 = s2 = src;
+            d1 = make_float2(e1x * c - e1y * s, e1x * s + e1y * c);
+            d2 = make_float2(e2x * c - e2y * s, e2x * s + e2y * c);
+        } else {
+            // cuda/distanceDriven2DBoxInt.cu:141-158 (virtual source distance)
+            const float u = (-((float)iu - (float)(d.nu - 1) / 2.0f) - d.off_u) * d.du;
+            const float vd = (float)g.nx * g.dx + (float)g.ny * g.dy;
+            auto rot = [&](float px, float py) { return make_float2(px * c - py * s, px * s + py * c); };
+            s1 = rot(u - d.du / 2, -vd);
+            s2 = rot(u + d.du / 2, -vd);
+            d1 = rot(u - d.du / 2, vd);
+            d2 = rot(u + d.du / 2, vd);
+        }
+        auto toimg = [&](float2 p) {
+            return make_float2((p.x - g.cx) / g.dx + g.nx / 2.0f, (p.y - g.cy) / g.dy + g.ny / 2.0f);
+        };
+        s1 = toimg(s1); s2 = toimg(s2); d1 = toimg(d1); d2 = toimg(d2);
+        const float* __restrict__ bY = imgY + c0;
+        const float* __restrict__ bX = imgX + c0;
+        if (V.flags & 1) {
+            // trace along y, cuda/distanceDriven2DBoxInt.cu:48-71
+            const float r1 = (d1.x - s1.x) / (d1.y - s1.y);
+            const float r2 = (d2.x - s2.x) / (d2.y - s2.y);
+            const float norm = V.wx * g.dy / fabsf(cosf(V.deg - a));
+            for (int iy = 0; iy < g.ny; iy++) {
+                const float yy = (float)iy + 0.5f;
+                const float x1 = fmaf(r1, yy - s1.y, s1.x);
+                const float x2 = fmaf(r2, yy - s2.y, s2.x);
+                dd_line<CH>(acc, bY + (size_t)iy * g.nx * cs, cs, x1, x2, g.nx, norm);
+            }
+        }
+        if (V.flags & 2) {
+            // trace along x, cuda/distanceDriven2DBoxInt.cu:74-98
+            const float r1 = (d1.y - s1.y) / (d1.x - s1.x);
+            const float r2 = (d2.y - s2.y) / (d2.x - s2.x);
+            const float norm = (1.f - V.wx) * g.dx / fabsf(sinf(V.deg - a));
+            for (int ix = 0; ix < g.nx; ix++) {
+                const float xx = (float)ix + 0.5f;
+                const float y1 = fmaf(r1, xx - s1.x, s1.y);
+                const float y2 = fmaf(r2, xx - s2.x, s2.y);
+                dd_line<CH>(acc, bX + (size_t)ix * g.ny * cs, cs, y1, y2, g.ny, norm);
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j <= CH; j++) lds[threadIdx.x][j] = acc[j];
+    // rows are read back by the owning lane only: no barrier needed
+    if (!active) return;
+    const float rev = (!FAN && force) ? V.rev : 1.f;
+    const int r0 = row_start[chunk], r1 = row_start[chunk + 1];
+    for (int r = r0; r < r1; r++) {
+        const FPRow R = rows[r];
+        const float val = (1.f - R.w1) * lds[threadIdx.x][R.j0] + R.w1 * lds[threadIdx.x][R.j1];
+        prj[(((size_t)R.b * nview + view) * d.nv + R.iv) * d.nu + iu] = val * rev;
+    }
+}
+
+static int pick_ch(int C) {
+    int ch = 1;
+    while (ch < C && ch < 32) ch *= 2;
+    return ch;
+}
+
+void dd2d_fp(float* prj, const float* img, const float* deg_host, int B, const Grid& g, const Det& d, int nview,
+             float dsd, float dso, int type, bool fan, hipStream_t stream) {
+    const bool branchless = (type & DD_BRANCHLESS) != 0;
+    const int C = B * g.nz;
+    const int CH = pick_ch(C);
+    const int nchunk = ceil_div(C, CH);
+    const int cs = ((nchunk * CH + 1 + 3) / 4) * 4;
+    std::vector<DDView> views = make_dd_views(deg_host, nview, g);
+    if (branchless) hard_switch(views);
+    // rows of each chunk: detector row iv of batch b belongs to the chunk holding
+    // channel b*nz + iz(iv); the z mapping follows cuda/distanceDriven2DBoxInt.cu:35-42,214,227
+    std::vector<std::vector<FPRow>> per(nchunk);
+    for (int b = 0; b < B; b++)
+        for (int iv = 0; iv < d.nv; iv++) {
+            const double zr = ((double)iv - (d.nv - 1) / 2.0 + d.off_v) * d.dv;
+            double z = (zr - g.cz) / g.dz + g.nz / 2.0 - 0.5;
+            z = std::min((double)g.nz, std::max(0.0, z));
+            int iz = (int)z, iz1 = iz + 1;
+            iz = clampi(iz, 0, g.nz);
+            iz1 = clampi(iz1, 0, g.nz);
+            const double wz = z - iz;
+            const int cl = b * g.nz + iz;
+            const int k = cl / CH;
+            FPRow r{};
+            r.b = b; r.iv = iv;
+            r.j0 = cl - k * CH;
+            r.j1 = b * g.nz + iz1 - k * CH;
+            r.w1 = (float)wz;
+            per[k].push_back(r);
+        }
+    std::vector<FPRow> rows;
+    std::vector<int> start(nchunk + 1, 0);
+    for (int k = 0; k < nchunk; k++) {
+        start[k] = (int)rows.size();
+        rows.insert(rows.end(), per[k].begin(), per[k].end());
+    }
+    start[nchunk] = (int)rows.size();
+
+    const size_t npix = (size_t)g.nx * g.ny;
+    Workspace& ws = workspace_for(stream);
+    Plan pl;
+    pl.add<float>(npix * cs);
+    pl.add<float>(npix * cs);
+    pl.add<DDView>(nview);
+    pl.add<FPRow>(rows.size());
+    pl.add<int>(nchunk + 1);
+    ws.reserve(pl.bytes);
+    float* imgY = ws.take_n<float>(npix * cs);
+    float* imgX = ws.take_n<float>(npix * cs);
+    DDView* dviews = ws.take_n<DDView>(nview);
+    FPRow* drows = ws.take_n<FPRow>(rows.size());
+    int* dstart = ws.take_n<int>(nchunk + 1);
+    CTP_CHECK(hipMemcpyAsync(dviews, views.data(), sizeof(DDView) * nview, hipMemcpyHostToDevice, stream));
+    CTP_CHECK(hipMemcpyAsync(drows, rows.data(), sizeof(FPRow) * rows.size(), hipMemcpyHostToDevice, stream));
+    CTP_CHECK(hipMemcpyAsync(dstart, start.data(), sizeof(int) * (nchunk + 1), hipMemcpyHostToDevice, stream));
+
+    bool needY = false, needX = false;
+    for (auto& v : views) { needY |= (v.flags & 1) != 0; needX |= (v.flags & 2) != 0; }
+    dim3 tb(256);
+    if (needY) {
+        to_channel_last<<<dim3(ceil_div(g.nx, 32), ceil_div(cs, 32), g.ny), tb, 0, stream>>>(imgY, img, C, cs, g.nx, g.ny, false);
+        CTP_LAUNCH_CHECK();
+    }
+    if (needX) {
+        to_channel_last<<<dim3(ceil_div(g.nx, 32), ceil_div(cs, 32), g.ny), tb, 0, stream>>>(imgX, img, C, cs, g.nx, g.ny, true);
+        CTP_LAUNCH_CHECK();
+    }
+    // the branchless parallel FP has no force-FBP reverse weighting
+    // (cuda/distanceDriven2DBranchless.cu:321-360)
+    const bool force = (type & DD_FORCE_FBP) != 0 && !branchless;
+    dim3 grid(ceil_div(d.nu, 128), nview, nchunk), block(128);
+#define CTP_FP_CASE(N)                                                                                          \
+    case N:                                                                                                     \
+        if (fan) dd_fp_kernel<N, true><<<grid, block, 0, stream>>>(prj, imgY, imgX, dviews, drows, dstart, cs, g, d, nview, dsd, dso, force); \
+        else dd_fp_kernel<N, false><<<grid, block, 0, stream>>>(prj, imgY, imgX, dviews, drows, dstart, cs, g, d, nview, dsd, dso, force); \
+        break;
+    switch (CH) {
+        CTP_FP_CASE(1) CTP_FP_CASE(2) CTP_FP_CASE(4) CTP_FP_CASE(8) CTP_FP_CASE(16) CTP_FP_CASE(32)
+        default: fail("dd fp: bad channel chunk");
+    }
+#undef CTP_FP_CASE
+    CTP_LAUNCH_CHECK();
+    CTP_CHECK(hipStreamSynchronize(stream));   // host tables above are released on return
+}
+
+// ---------------------------------------------------------------------------
+// backprojection
+// ---------------------------------------------------------------------------
+// Weighted projections, one channel per image slice:
+//   sino[view][u][c] = w(view, u) * rows(z) of prj[b][view][.][u],  c = b*nz + z
+// with w the BP pre-weight (cuda/distanceDriven2DUtils.cu:8-65) or 1 for FBP.
+// Box-integral mode: one row, trunc(z/dv + off_v + nv/2 - 0.5), rows outside
+// [0, nv) give 0 (cuda/distanceDriven2DBoxInt.cu:350,383).  Branchless mode: the
+// clamped linear interpolation between rows of InterpolateXY
+// (cuda/distanceDriven2DBranchless.cu:444-447, cuda/cudaMath.cu:227-260); the
+// u-interpolation of the prefix sums there is the exact box integral used below.
+__global__ __launch_bounds__(256) void dd_bp_prep_kernel(float* __restrict__ sino, const float* __restrict__ prj,
+                                                         const DDView* __restrict__ views, const SliceRow* __restrict__ slice_row,
+                                                         int C, int cs, int nz, Det d, int nview, float dx, float dy,
+                                                         bool fan, bool preweight) {
+    __shared__ float tile[32][33];
+    const int u0 = blockIdx.x * 32, c0 = blockIdx.y * 32, view = blockIdx.z;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const DDView V = preweight ? views[view] : DDView{};
+    for (int k = ty; k < 32; k += 8) {
+        const int c = c0 + k, u = u0 + tx;
+        float val = 0.f;
+        if (c < C && u < d.nu) {
+            const int b = c / nz, z = c - b * nz;
+            const SliceRow R = slice_row[z];
+            if (R.r0 >= 0) {
+                const float* pb = prj + ((size_t)b * nview + view) * d.nv * d.nu + u;
+                val = pb[(size_t)R.r0 * d.nu];
+                if (R.w1 != 0.f) val = (1.f - R.w1) * val + R.w1 * pb[(size_t)R.r1 * d.nu];
+                if (preweight) {
+                    const float a = fan ? (-((float)u - (float)(d.nu - 1) / 2.0f) - d.off_u) * d.du : 0.f;
+                    val *= (V.flags & 4) ? dy / fabsf(cosf(V.deg - a)) : dx / fabsf(sinf(V.deg - a));
+                }
+            }
+        }
+        tile[k][tx] = val;
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        const int u = u0 + k, c = c0 + tx;
+        if (u < d.nu && c < cs) sino[((size_t)view * d.nu + u) * cs + c] = tile[tx][k];
+    }
+}
+
+void slices_to_channel_last(float* sino, const float* prj, const SliceRow* rows_dev, int C, int cs, int nz,
+                            const Det& d, int nview, hipStream_t stream) {
+    dd_bp_prep_kernel<<<dim3(ceil_div(d.nu, 32), ceil_div(cs, 32), nview), 256, 0, stream>>>(
+        sino, prj, nullptr, rows_dev, C, cs, nz, d, nview, 1.f, 1.f, false, false);
+    CTP_LAUNCH_CHECK();
+}
+
+// Detector footprint [a1, a2] (cell units, unclamped) of a pixel applied to the
+// channel vectors of one view: IntegralBoxX over a detector row divided by
+// (a2 - a1), cuda/distanceDriven2DBoxInt.cu:375-383.
+template <int CH>
+__device__ inline void dd_cells(float* acc, const float* __restrict__ base, int cs, float a1, float a2, int n, float scale) {
+    if (a1 > a2) { const float t = a1; a1 = a2; a2 = t; }
+    const float a = clampf(a1, 0.f, (float)n), b = clampf(a2, 0.f, (float)n);
+    if (!(b > a)) return;
+    const float inv = scale / (a2 - a1);
+    const int i1 = min((int)a, n - 1), i2 = min((int)b, n - 1);
+    for (int i = i1; i <= i2; i++) {
+        const float w = (fminf(b, (float)(i + 1)) - fmaxf(a, (float)i)) * inv;
+        const float* p = base + (size_t)i * cs;
+        if constexpr (CH % 4 == 0) {
+#pragma unroll
+            for (int j = 0; j < CH; j += 4) {
+                const float4 q = *reinterpret_cast<const float4*>(p + j);
+                acc[j] = fmaf(w, q.x, acc[j]);
+                acc[j + 1] = fmaf(w, q.y, acc[j + 1]);
+                acc[j + 2] = fmaf(w, q.z, acc[j + 2]);
+                acc[j + 3] = fmaf(w, q.w, acc[j + 3]);
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < CH; j++) acc[j] = fmaf(w, p[j], acc[j]);
+        }
+    }
+}
+
+template <int CH, bool FAN>
+__global__ __launch_bounds__(256) void dd_bp_kernel(float* __restrict__ img, const float* __restrict__ sino,
+                                                    const DDView* __restrict__ views, int C, int cs, Grid g, Det d,
+                                                    int nview, float dso, bool fbp) {
+    const int ix = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int iy = blockIdx.y * 16 + (threadIdx.x >> 4);
+    const int c0 = blockIdx.z * CH;
+    if (ix >= g.nx || iy >= g.ny) return;
+    // pixel centre and edges, cuda/distanceDriven2DBoxInt.cu:340-349
+    const float xc = ((float)ix - g.nx / 2.f) * g.dx + g.cx + g.dx / 2;
+    const float yc = ((float)iy - g.ny / 2.f) * g.dy + g.cy + g.dy / 2;
+    const float x1 = xc - g.dx / 2.0f, x2 = xc + g.dx / 2.0f;
+    const float y1 = yc - g.dy / 2.0f, y2 = yc + g.dy / 2.0f;
+    float acc[CH];
+#pragma unroll
+    for (int j = 0; j < CH; j++) acc[j] = 0.f;
+    const float half = (float)d.nu / 2.0f;
+    const size_t vstride = (size_t)d.nu * cs;
+    for (int view = 0; view < nview; view++) {
+        const DDView V = views[view];
+        const float c = V.c, s = V.s;
+        const float* __restrict__ base = sino + view * vstride + c0;
+        if (FAN) {
+            // cuda/distanceDriven2DBoxInt.cu:362-391 and GetProjectionOnDetector (Utils.cu:96-102)
+            float p1x, p1y, p2x, p2y;
+            if (V.flags & 4) { p1x = x1; p1y = yc; p2x = x2; p2y = yc; }
+            else { p1x = xc; p1y = y1; p2x = xc; p2y = y2; }
+            const float a1 = atanf((p1x * c + p1y * s) / (-p1x * s + p1y * c + dso));
+            const float a2 = atanf((p2x * c + p2y * s) / (-p2x * s + p2y * c + dso));
+            const float u1 = -(a1 / d.du + d.off_u) + half;
+            const float u2 = -(a2 / d.du + d.off_u) + half;
+            float scale = 1.f;
+            if (fbp) {
+                const float rx = xc * c + yc * s, ry = -xc * s + yc * c;
+                scale = dso * dso / (rx * rx + (dso + ry) * (dso + ry));
+            }
+            dd_cells<CH>(acc, base, cs, u1, u2, d.nu, scale);
+        } else {
+            // cuda/distanceDriven2DBoxInt.cu:439-493 (blended main axes)
+            if (V.flags & 1) {
+                const float u1 = -((x1 * c + yc * s) / d.du + d.off_u) + half;
+                const float u2 = -((x2 * c + yc * s) / d.du + d.off_u) + half;
+                dd_cells<CH>(acc, base, cs, u1, u2, d.nu, V.wx);
+            }
+            if (V.flags & 2) {
+                const float u1 = -((xc * c + y1 * s) / d.du + d.off_u) + half;
+                const float u2 = -((xc * c + y2 * s) / d.du + d.off_u) + half;
+                dd_cells<CH>(acc, base, cs, u1, u2, d.nu, 1.f - V.wx);
+            }
+        }
+    }
+    const size_t npix = (size_t)g.nx * g.ny;
+#pragma unroll
+    for (int j = 0; j < CH; j++) {
+        const int cc = c0 + j;
+        if (cc < C) img[(size_t)cc * npix + (size_t)iy * g.nx + ix] += acc[j];
+    }
+}
+
+void dd2d_bp(float* img, const float* prj, const float* deg_host, int B, const Grid& g, const Det& d, int nview,
+             float dsd, float dso, int type, bool fan, hipStream_t stream) {
+    (void)dsd;
+    const bool branchless = (type & DD_BRANCHLESS) != 0;
+    // the branchless BP ignores the force-FBP bit (cuda/distanceDriven2DBranchless.cu:547)
+    const bool fbp = branchless ? (type & DD_FBP) != 0 : (type & (DD_FBP | DD_FORCE_FBP)) != 0;
+    const int C = B * g.nz;
+    const int CH = pick_ch(C);
+    const int nchunk = ceil_div(C, CH);
+    const int cs = nchunk * CH;
+    std::vector<DDView> views = make_dd_views(deg_host, nview, g);
+    if (branchless) hard_switch(views);
+    // detector rows of each slice, cuda/distanceDriven2DBoxInt.cu:340-350
+    std::vector<SliceRow> srow(g.nz);
+    for (int z = 0; z < g.nz; z++) {
+        const double zc = (z - g.nz / 2.0) * g.dz + g.cz + g.dz / 2.0;
+        const double ivf = zc / d.dv + d.off_v + d.nv / 2.0 - 0.5;
+        SliceRow r{-1, -1, 0.f, 0};
+        if (!branchless) {
+            const int t = (int)ivf;   // the float row is passed to an int parameter: truncation
+            if (t >= 0 && t < d.nv) r = SliceRow{t, t, 0.f, 0};
+        } else {
+            const double y = std::min((double)d.nv, std::max(0.0, ivf));
+            const int iy = (int)y;
+            const int r0 = clampi(iy, 0, d.nv), r1 = clampi(iy + 1, 0, d.nv);
+            r = SliceRow{r0, r1, (float)(y - r0), 0};
+        }
+        srow[z] = r;
+    }
+    Workspace& ws = workspace_for(stream);
+    Plan pl;
+    pl.add<float>((size_t)nview * d.nu * cs);
+    pl.add<DDView>(nview);
+    pl.add<SliceRow>(g.nz);
+    ws.reserve(pl.bytes);
+    float* sino = ws.take_n<float>((size_t)nview * d.nu * cs);
+    DDView* dviews = ws.take_n<DDView>(nview);
+    SliceRow* dsrow = ws.take_n<SliceRow>(g.nz);
+    CTP_CHECK(hipMemcpyAsync(dviews, views.data(), sizeof(DDView) * nview, hipMemcpyHostToDevice, stream));
+    CTP_CHECK(hipMemcpyAsync(dsrow, srow.data(), sizeof(SliceRow) * g.nz, hipMemcpyHostToDevice, stream));
+    dd_bp_prep_kernel<<<dim3(ceil_div(d.nu, 32), ceil_div(cs, 32), nview), 256, 0, stream>>>(
+        sino, prj, dviews, dsrow, C, cs, g.nz, d, nview, g.dx, g.dy, fan, !fbp);
+    CTP_LAUNCH_CHECK();
+    dim3 grid(ceil_div(g.nx, 16), ceil_div(g.ny, 16), nchunk), block(256);
+#define CTP_BP_CASE(N)                                                                                         \
+    case N:                                                                                                    \
+        if (fan) dd_bp_kernel<N, true><<<grid, block, 0, stream>>>(img, sino, dviews, C, cs, g, d, nview, dso, fbp); \
+        else dd_bp_kernel<N, false><<<grid, block, 0, stream>>>(img, sino, dviews, C, cs, g, d, nview, dso, fbp); \
+        break;
+    switch (CH) {
+        CTP_BP_CASE(1) CTP_BP_CASE(2) CTP_BP_CASE(4) CTP_BP_CASE(8) CTP_BP_CASE(16) CTP_BP_CASE(32)
+        default: fail("dd bp: bad channel chunk");
+    }
+#undef CTP_BP_CASE
+    CTP_LAUNCH_CHECK();
+    CTP_CHECK(hipStreamSynchronize(stream));
+}
+
+}  // namespace ctp

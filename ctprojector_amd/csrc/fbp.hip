@@ -1,0 +1,297 @@
+// Ramp filtering (fan: equiangular, parallel: equispace) and the pixel-driven
+// fan FBP backprojection on MI355X (gfx950): cuda/fbpMath.cu, cuda/fbp.cu,
+// cuda/fbpParallel.cu of the reference.
+//
+// Ramp filter.  The reference zero-pads each detector row to L = 2nu-1, applies
+// the windowed ramp in the frequency domain with cuFFT and reads the result at
+// offset nu-1 (cuda/fbp.cu:106-134).  Because the windowed kernel h_w is
+// defined on exactly L = 2nu-1 samples, that is the exact linear convolution
+//     y[j] = scale * sum_m x[m] w[m] h_w[nu-1 + j - m]
+// (no wrap-around), which is what is computed here, directly, in FP32 with the
+// kernel built once in FP64 on the host.  h_w for the Hamming and Hann windows
+// is a 3-tap circular smoothing of the spatial Ram-Lak kernel; the cosine
+// window is applied with an explicit DFT in FP64.
+//
+// Fan FBP backprojection (bpFanKernel3D, cuda/fbp.cu:264-359): pixel driven;
+// the bilinear detector interpolation is split into a per-slice row
+// interpolation done once per view in a prep pass (channel-last layout) and a
+// per-pixel column interpolation shared by all slices.
+#include "common.h"
+#include "runtime.h"
+#include "ops.h"
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+namespace ctp {
+
+// ---------------------------------------------------------------------------
+// windowed ramp kernel (GetRamp, cuda/fbpMath.cu:90-268), FP64
+// ---------------------------------------------------------------------------
+static std::vector<double> ramp_kernel_fp64(int nu, double da, int filter, bool equispace) {
+    const int L = 2 * nu - 1;
+    const double PI = (double)kPI;
+    std::vector<double> r(L), h(L);
+    for (int i = 0; i < L; i++) {
+        const int k = i - (nu - 1);
+        if (k == 0) r[i] = 1.0 / (4 * da * da);
+        else if (k % 2 != 0) {
+            if (equispace) r[i] = -1.0 / (PI * PI * (double)k * k * da * da);
+            else { const double sk = std::sin(k * da); r[i] = -1.0 / (PI * PI * sk * sk); }
+        } else r[i] = 0;
+    }
+    if (filter == FILTER_HAMMING || filter == FILTER_HANN) {
+        // W[f] = A + B cos(2 pi f / L) multiplies the DFT  <=>  circular taps {A, B/2, B/2}
+        const double A = filter == FILTER_HAMMING ? (double)0.54f : (double)0.5f;
+        const double Bw = filter == FILTER_HAMMING ? (double)0.46f : (double)0.5f;
+        for (int i = 0; i < L; i++) h[i] = A * r[i] + 0.5 * Bw * (r[(i - 1 + L) % L] + r[(i + 1) % L]);
+    } else if (filter == FILTER_COSINE) {
+        std::vector<double> cs(L), sn(L);
+        for (int k = 0; k < L; k++) { cs[k] = std::cos(2 * M_PI * k / L); sn[k] = std::sin(2 * M_PI * k / L); }
+        std::vector<std::complex<double>> F(L);
+        for (int f = 0; f < L; f++) {
+            double re = 0, im = 0;
+            for (int i = 0; i < L; i++) {
+                const int k = (int)(((long)f * i) % L);
+                re += r[i] * cs[k];
+                im -= r[i] * sn[k];
+            }
+            const double w = std::fabs(std::cos(PI * f / (double)L));
+            F[f] = std::complex<double>(re * w, im * w);
+        }
+        for (int i = 0; i < L; i++) {
+            double re = 0;
+            for (int f = 0; f < L; f++) {
+                const int k = (int)(((long)f * i) % L);
+                re += F[f].real() * cs[k] - F[f].imag() * sn[k];
+            }
+            h[i] = re / L;
+        }
+    } else {
+        h = r;
+    }
+    return h;
+}
+
+static std::vector<float> cached_ramp(int nu, float da, int filter, bool equispace) {
+    static std::mutex mu;
+    static std::map<std::tuple<int, float, int, bool>, std::vector<float>> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    auto key = std::make_tuple(nu, da, filter, equispace);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    std::vector<double> h = ramp_kernel_fp64(nu, (double)da, filter, equispace);
+    std::vector<float> hf(h.begin(), h.end());
+    if (cache.size() > 64) cache.clear();
+    cache[key] = hf;
+    return hf;
+}
+
+// ---------------------------------------------------------------------------
+// direct linear convolution: RB rows per block, 4 outputs per thread
+// ---------------------------------------------------------------------------
+template <int RB>
+__global__ __launch_bounds__(256) void ramp_conv_kernel(float* __restrict__ out, const float* __restrict__ in,
+                                                        const float* __restrict__ h, const float* __restrict__ pw,
+                                                        long nrows, int nu, float scale) {
+    extern __shared__ float sm[];
+    const int L = 2 * nu - 1;
+    float* hs = sm;                 // [L + 4]
+    float* xs = sm + L + 4;         // [RB][nu]
+    const long r0 = (long)blockIdx.x * RB;
+    for (int i = threadIdx.x; i < L + 4; i += blockDim.x) hs[i] = i < L ? h[i] : 0.f;
+    for (int rr = 0; rr < RB; rr++) {
+        const long r = r0 + rr;
+        for (int i = threadIdx.x; i < nu; i += blockDim.x)
+            xs[rr * nu + i] = (r < nrows) ? in[r * nu + i] * pw[i] : 0.f;
+    }
+    __syncthreads();
+    for (int j0 = threadIdx.x * 4; j0 < nu; j0 += blockDim.x * 4) {
+        float acc[RB][4];
+#pragma unroll
+        for (int rr = 0; rr < RB; rr++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) acc[rr][q] = 0.f;
+        // h index for output j0+q and input m: nu-1 + j0 + q - m
+        int base = nu - 1 + j0;   // m = 0
+        float h0 = hs[base], h1 = hs[base + 1], h2 = hs[base + 2], h3 = hs[base + 3];
+        for (int m = 0; m < nu; m++) {
+#pragma unroll
+            for (int rr = 0; rr < RB; rr++) {
+                const float x = xs[rr * nu + m];
+                acc[rr][0] = fmaf(x, h0, acc[rr][0]);
+                acc[rr][1] = fmaf(x, h1, acc[rr][1]);
+                acc[rr][2] = fmaf(x, h2, acc[rr][2]);
+                acc[rr][3] = fmaf(x, h3, acc[rr][3]);
+            }
+            // slide the window: next m needs hs[base-1 .. base+2]
+            h3 = h2; h2 = h1; h1 = h0;
+            base--;
+            h0 = base >= 0 ? hs[base] : 0.f;
+        }
+#pragma unroll
+        for (int rr = 0; rr < RB; rr++) {
+            const long r = r0 + rr;
+            if (r >= nrows) break;
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                if (j0 + q < nu) out[r * nu + j0 + q] = acc[rr][q] * scale;
+        }
+    }
+}
+
+void ramp_filter(float* out, const float* prj, int B, int nu, int nv, int nview, float da, float off_u, float dsd,
+                 float dso, int filter, bool equispace, hipStream_t stream) {
+    (void)dsd;
+    if (nu <= 0) return;
+    if (nu > 5000) fail("ramp filter: nu > 5000 is not supported");
+    const std::vector<float> h = cached_ramp(nu, da, filter, equispace);
+    std::vector<float> pw(nu);
+    for (int i = 0; i < nu; i++) {
+        // fan pre-weight cos(gamma), cuda/fbp.cu:60-66 (not mirrored)
+        const double ang = ((i - (nu - 1) / 2.0) - off_u) * (double)da;
+        pw[i] = equispace ? 1.f : (float)std::cos(ang);
+    }
+    // cuda/fbp.cu:97-104, cuda/fbpParallel.cu:59; the 1/L cancels cuFFT's
+    // unnormalised inverse
+    const float scale = equispace ? (float)((double)kPI / nview * da) : (float)((double)kPI / nview * da / dso);
+    Workspace& ws = workspace_for(stream);
+    Plan pl;
+    pl.add<float>(2 * nu - 1);
+    pl.add<float>(nu);
+    ws.reserve(pl.bytes);
+    float* dh = ws.take_n<float>(2 * nu - 1);
+    float* dpw = ws.take_n<float>(nu);
+    CTP_CHECK(hipMemcpyAsync(dh, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice, stream));
+    CTP_CHECK(hipMemcpyAsync(dpw, pw.data(), sizeof(float) * nu, hipMemcpyHostToDevice, stream));
+    const long nrows = (long)B * nview * nv;
+    // LDS: the kernel (2nu+3 floats) and RB input rows; stays within 64 KB
+    const size_t sh4 = sizeof(float) * ((2 * nu - 1 + 4) + 4 * nu);
+    if (sh4 <= 65536) {
+        ramp_conv_kernel<4><<<(unsigned)ceil_div(nrows, 4), 256, sh4, stream>>>(out, prj, dh, dpw, nrows, nu, scale);
+    } else {
+        const size_t sh1 = sizeof(float) * ((2 * nu - 1 + 4) + nu);
+        ramp_conv_kernel<1><<<(unsigned)nrows, 256, sh1, stream>>>(out, prj, dh, dpw, nrows, nu, scale);
+    }
+    CTP_LAUNCH_CHECK();
+    CTP_CHECK(hipStreamSynchronize(stream));
+}
+
+// ---------------------------------------------------------------------------
+// fan FBP backprojection
+// ---------------------------------------------------------------------------
+struct FBPView {
+    float c, s;
+    float pad0, pad1;
+};
+
+template <int CH>
+__global__ __launch_bounds__(256) void fan_fbp_bp_kernel(float* __restrict__ img, const float* __restrict__ q,
+                                                         const FBPView* __restrict__ views, int C, int cs, Grid g, Det d,
+                                                         int nview, float dso) {
+    const int ix = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int iy = blockIdx.y * 16 + (threadIdx.x >> 4);
+    const int c0 = blockIdx.z * CH;
+    if (ix >= g.nx || iy >= g.ny) return;
+    // pixel centre, cuda/fbp.cu:288
+    const float px = ((float)ix + 0.5f - g.nx / 2.f) * g.dx + g.cx;
+    const float py = ((float)iy + 0.5f - g.ny / 2.f) * g.dy + g.cy;
+    float acc[CH];
+#pragma unroll
+    for (int j = 0; j < CH; j++) acc[j] = 0.f;
+    const float half = (d.nu - 1.0f) / 2.0f;
+    const size_t vstride = (size_t)d.nu * cs;
+    for (int view = 0; view < nview; view++) {
+        const FBPView V = views[view];
+        const float rx = px * V.c + py * V.s;
+        const float ry = -px * V.s + py * V.c;
+        const float a = atanf(rx / (ry + dso));
+        const float dist = dso * dso / (rx * rx + (dso + ry) * (dso + ry));
+        const float pu = -(a / d.du + d.off_u) + half;
+        // InterpolateXY with truncation, cuda/cudaMath.cu:227-260
+        if (pu < 0.f || pu >= (float)d.nu) continue;
+        const int u0 = (int)pu;
+        const int u1 = min(u0 + 1, d.nu - 1);
+        const float wu = pu - (float)u0;
+        const float w0 = (1.f - wu) * dist, w1 = wu * dist;
+        const float* __restrict__ p0 = q + view * vstride + (size_t)u0 * cs + c0;
+        const float* __restrict__ p1 = q + view * vstride + (size_t)u1 * cs + c0;
+        if constexpr (CH % 4 == 0) {
+#pragma unroll
+            for (int j = 0; j < CH; j += 4) {
+                const float4 a0 = *reinterpret_cast<const float4*>(p0 + j);
+                const float4 a1 = *reinterpret_cast<const float4*>(p1 + j);
+                acc[j] = fmaf(w0, a0.x, fmaf(w1, a1.x, acc[j]));
+                acc[j + 1] = fmaf(w0, a0.y, fmaf(w1, a1.y, acc[j + 1]));
+                acc[j + 2] = fmaf(w0, a0.z, fmaf(w1, a1.z, acc[j + 2]));
+                acc[j + 3] = fmaf(w0, a0.w, fmaf(w1, a1.w, acc[j + 3]));
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < CH; j++) acc[j] = fmaf(w0, p0[j], fmaf(w1, p1[j], acc[j]));
+        }
+    }
+    const size_t npix = (size_t)g.nx * g.ny;
+#pragma unroll
+    for (int j = 0; j < CH; j++) {
+        const int cc = c0 + j;
+        if (cc < C) img[(size_t)cc * npix + (size_t)iy * g.nx + ix] += acc[j];
+    }
+}
+
+void fan_fbp_bp(float* img, const float* prj, const float* deg_host, int B, const Grid& g, const Det& d, int nview,
+                float dsd, float dso, hipStream_t stream) {
+    (void)dsd;
+    const int C = B * g.nz;
+    int CH = 1;
+    while (CH < C && CH < 32) CH *= 2;
+    const int nchunk = ceil_div(C, CH);
+    const int cs = nchunk * CH;
+    std::vector<FBPView> views(nview);
+    for (int i = 0; i < nview; i++) {
+        views[i].c = (float)std::cos((double)deg_host[i]);
+        views[i].s = (float)std::sin((double)deg_host[i]);
+        views[i].pad0 = views[i].pad1 = 0;
+    }
+    // detector row coordinate of each slice centre, cuda/fbp.cu:288,321
+    std::vector<SliceRow> srow(g.nz);
+    for (int z = 0; z < g.nz; z++) {
+        const double zc = (z + 0.5 - g.nz / 2.0) * g.dz + g.cz;
+        const double pv = zc / d.dv + d.off_v + (d.nv - 1.0) / 2.0;
+        SliceRow r{-1, -1, 0.f, 0};
+        if (pv >= 0 && pv < d.nv) {
+            const int r0 = (int)pv;
+            r = SliceRow{r0, std::min(r0 + 1, d.nv - 1), (float)(pv - r0), 0};
+        }
+        srow[z] = r;
+    }
+    Workspace& ws = workspace_for(stream);
+    Plan pl;
+    pl.add<float>((size_t)nview * d.nu * cs);
+    pl.add<FBPView>(nview);
+    pl.add<SliceRow>(g.nz);
+    ws.reserve(pl.bytes);
+    float* q = ws.take_n<float>((size_t)nview * d.nu * cs);
+    FBPView* dviews = ws.take_n<FBPView>(nview);
+    SliceRow* dsrow = ws.take_n<SliceRow>(g.nz);
+    CTP_CHECK(hipMemcpyAsync(dviews, views.data(), sizeof(FBPView) * nview, hipMemcpyHostToDevice, stream));
+    CTP_CHECK(hipMemcpyAsync(dsrow, srow.data(), sizeof(SliceRow) * g.nz, hipMemcpyHostToDevice, stream));
+    slices_to_channel_last(q, prj, dsrow, C, cs, g.nz, d, nview, stream);
+    dim3 grid(ceil_div(g.nx, 16), ceil_div(g.ny, 16), nchunk), block(256);
+#define CTP_FBP_CASE(N) \
+    case N: fan_fbp_bp_kernel<N><<<grid, block, 0, stream>>>(img, q, dviews, C, cs, g, d, nview, dso); break;
+    switch (CH) {
+        CTP_FBP_CASE(1) CTP_FBP_CASE(2) CTP_FBP_CASE(4) CTP_FBP_CASE(8) CTP_FBP_CASE(16) CTP_FBP_CASE(32)
+        default: fail("fbp bp: bad channel chunk");
+    }
+#undef CTP_FBP_CASE
+    CTP_LAUNCH_CHECK();
+    CTP_CHECK(hipStreamSynchronize(stream));
+}
+
+}  // namespace ctp

@@ -1,0 +1,67 @@
+// Internal stream-ordered operations of the projector library.  All array
+// arguments are device pointers unless named host; every function enqueues its
+// work on 'stream' and may synchronise that stream only where noted.
+#pragma once
+
+#include "common.h"
+
+namespace ctp {
+
+struct ConeGeomDev {               // [nview][3] float rows (x, y, z), device
+    const float* det_center;
+    const float* det_u;
+    const float* det_v;
+    const float* src;
+};
+struct ConeGeomHost {              // the same rows in host memory
+    const float* det_center;
+    const float* det_u;
+    const float* det_v;
+    const float* src;
+};
+
+// Detector rows feeding an image slice: (1 - w1) * row r0 + w1 * row r1, or
+// nothing when r0 < 0.
+struct SliceRow {
+    int r0, r1;
+    float w1;
+    int pad;
+};
+
+// sino[view][u][c] = rows(z) of prj[b][view][.][u] for c = b*nz + z < C, zero for
+// C <= c < cs (channel-last layout shared by the pixel-driven backprojectors)
+void slices_to_channel_last(float* sino, const float* prj, const SliceRow* rows_dev, int C, int cs, int nz,
+                            const Det& d, int nview, hipStream_t stream);
+
+// ---- Siddon cone beam (cuda/siddonCone.cu) ----
+// prj += A img   (the reference's FP accumulates, cuda/siddon.cu:230)
+void siddon_cone_fp(float* prj, const float* img, const ConeGeomDev& geo, int B, const Grid& g, const Det& d,
+                    int nview, hipStream_t stream);
+// img += A^T prj  (exact transpose; synchronises 'stream' at the end)
+void siddon_cone_bp(float* img, const float* prj, const ConeGeomDev& geo, const ConeGeomHost& hgeo, int B,
+                    const Grid& g, const Det& d, int nview, hipStream_t stream);
+
+// ---- distance driven fan / parallel (cuda/distanceDriven2D*.cu) ----
+// 'deg_host' are the view angles in host memory (per-view decisions are taken
+// in FP64 on the host); prj is overwritten, img accumulates.
+void dd2d_fp(float* prj, const float* img, const float* deg_host, int B, const Grid& g, const Det& d, int nview,
+             float dsd, float dso, int type, bool fan, hipStream_t stream);
+void dd2d_bp(float* img, const float* prj, const float* deg_host, int B, const Grid& g, const Det& d, int nview,
+             float dsd, float dso, int type, bool fan, hipStream_t stream);
+
+// ---- distance driven tomosynthesis (cuda/distanceDriven3D*.cu) ----
+// geometry rows on device; outputs are overwritten (the reference writes '=')
+void tomo_fp(float* prj, const float* img, const float* det_center, const float* src, int B, const Grid& g,
+             const Det& d, int nview, int type, hipStream_t stream);
+void tomo_bp(float* img, const float* prj, const float* det_center, const float* src, int B, const Grid& g,
+             const Det& d, int nview, int type, hipStream_t stream);
+
+// ---- ramp filter and fan FBP backprojection (cuda/fbp*.cu) ----
+// out is overwritten with the filtered projections
+void ramp_filter(float* out, const float* prj, int B, int nu, int nv, int nview, float da, float off_u, float dsd,
+                 float dso, int filter, bool equispace, hipStream_t stream);
+// img += FBP-weighted pixel-driven fan backprojection
+void fan_fbp_bp(float* img, const float* prj, const float* deg_host, int B, const Grid& g, const Det& d, int nview,
+                float dsd, float dso, hipStream_t stream);
+
+}  // namespace ctp

@@ -1,0 +1,105 @@
+#include "runtime.h"
+
+#include <cstdio>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <utility>
+
+namespace ctp {
+
+namespace {
+thread_local std::string g_last_error;
+std::mutex g_ws_mutex;
+std::map<std::pair<int, hipStream_t>, std::unique_ptr<Workspace>> g_ws;
+}  // namespace
+
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+const char* last_error() { return g_last_error.c_str(); }
+
+void check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw Error((int)e, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void fail(const std::string& msg) { throw Error((int)hipErrorInvalidValue, msg); }
+
+Workspace::Workspace(hipStream_t s) : stream_(s) {}
+
+void Workspace::reset() { used_ = 0; }
+
+void Workspace::reserve(size_t bytes) {
+    used_ = 0;
+    if (bytes <= cap_) return;
+    if (base_) {
+        // the previous block may still be read by queued kernels on this stream
+        CTP_CHECK(hipStreamSynchronize(stream_));
+        CTP_CHECK(hipFree(base_));
+        base_ = nullptr;
+        cap_ = 0;
+    }
+    size_t want = bytes + bytes / 8;   // some head-room for slightly larger calls
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        e = hipMalloc(&p, bytes);
+        want = bytes;
+    }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error((int)e, "workspace allocation of " + std::to_string(bytes) + " bytes failed: " +
+                                hipGetErrorString(e));
+    }
+    base_ = static_cast<char*>(p);
+    cap_ = want;
+}
+
+void* Workspace::take(size_t bytes) {
+    size_t n = ((bytes + 255) / 256) * 256;
+    if (used_ + n > cap_) throw Error((int)hipErrorOutOfMemory, "workspace overrun (internal planning error)");
+    void* p = base_ + used_;
+    used_ += n;
+    return p;
+}
+
+Workspace& workspace_for(hipStream_t stream) {
+    int dev = 0;
+    CTP_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_ws_mutex);
+    auto key = std::make_pair(dev, stream);
+    auto it = g_ws.find(key);
+    if (it == g_ws.end()) it = g_ws.emplace(key, std::make_unique<Workspace>(stream)).first;
+    return *it->second;
+}
+
+void release_all_workspaces() {
+    std::lock_guard<std::mutex> lk(g_ws_mutex);
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    for (auto& kv : g_ws) {
+        Workspace& w = *kv.second;
+        if (w.base_) {
+            (void)hipSetDevice(kv.first.first);
+            (void)hipStreamSynchronize(w.stream_);
+            (void)hipFree(w.base_);
+        }
+    }
+    g_ws.clear();
+    (void)hipSetDevice(cur);
+}
+
+DeviceBuffer::DeviceBuffer(size_t bytes) : n(bytes) {
+    if (bytes == 0) return;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        p = nullptr;
+        throw Error((int)e, "device allocation of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+    }
+}
+
+DeviceBuffer::~DeviceBuffer() {
+    if (p) (void)hipFree(p);
+}
+
+}  // namespace ctp

@@ -1,0 +1,462 @@
+// Siddon cone-beam projector for arbitrary trajectories on MI355X (gfx950).
+//
+// Forward projection (cuda/siddonCone.cu:12-42, cuda/siddon.cu:80-233):
+//   one ray per lane; a 256-thread block covers a 16x16 detector patch of one
+//   view as four 8x8 wavefront tiles, so the 64 rays of a wavefront stay
+//   spatially coherent and their voxel gathers share cache lines.  The three-way
+//   axis choice of the reference's traversal is evaluated with selects, and the
+//   next crossing of each axis is recomputed from its step count with an FMA
+//   (alpha0 + n * dAlpha) instead of being accumulated.
+//
+// Backprojection (exact transpose of the forward projector, the reference's
+// cuda/siddon.cu:235-314 uses atomicAdd): voxel driven and atomic free.  For
+// each (voxel, view) the detector cells whose rays can cross the voxel lie in a
+// window of at most WU x WV cells, found from the projective map of the view
+// and a conservative bound of the voxel's shadow.  The chord of each candidate
+// ray through the voxel is the length of the intersection of its three slab
+// intervals (ray-box clipping), which is exactly the segment length the
+// reference's traversal assigns to that voxel.  Ray data (1/direction per axis
+// and projection value times ray length) come from a table prepared per call in
+// FP64.  Every voxel is written by exactly one thread: results are
+// deterministic.
+#include "common.h"
+#include "runtime.h"
+#include "ops.h"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace ctp {
+
+// ---------------------------------------------------------------------------
+// Forward projection
+// ---------------------------------------------------------------------------
+__device__ inline float3 ld3(const float* p, int i) { return make_float3(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
+
+__global__ __launch_bounds__(256) void siddon_cone_fp_kernel(float* __restrict__ prj, const float* __restrict__ img,
+                                                             const float* __restrict__ pdc, const float* __restrict__ pdu,
+                                                             const float* __restrict__ pdv, const float* __restrict__ psrc,
+                                                             Grid g, Det d, int nview) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int iu = (blockIdx.x * 2 + (wave & 1)) * 8 + (lane & 7);
+    const int iv = (blockIdx.y * 2 + (wave >> 1)) * 8 + (lane >> 3);
+    const int view = blockIdx.z;
+    if (iu >= d.nu || iv >= d.nv || view >= nview) return;
+
+    // detector cell centre, cuda/siddonCone.cu:27-31
+    const float u = ((float)iu - d.off_u - (float)(d.nu - 1) / 2.0f) * d.du;
+    const float v = ((float)iv - d.off_v - (float)(d.nv - 1) / 2.0f) * d.dv;
+    float3 s = ld3(psrc, view);
+    const float3 c = ld3(pdc, view), U = ld3(pdu, view), V = ld3(pdv, view);
+    float3 t = make_float3(c.x + U.x * u + V.x * v, c.y + U.y * u + V.y * v, c.z + U.z * u + V.z * v);
+
+    // MoveSourceDstNearGrid, cuda/siddon.cu:318-338
+    {
+        const float ex = (float)g.nx * g.dx, ey = (float)g.ny * g.dy, ez = (float)g.nz * g.dz;
+        const float r = sqrtf(ex * ex + ey * ey + ez * ez);
+        float3 w = make_float3(t.x - s.x, t.y - s.y, t.z - s.z);
+        float len = sqrtf(w.x * w.x + w.y * w.y + w.z * w.z);
+        if (!(len < 1e-6f)) w = make_float3(w.x / len, w.y / len, w.z / len);
+        const float t0 = (g.cx - s.x) * w.x + (g.cy - s.y) * w.y + (g.cz - s.z) * w.z;
+        const float t1 = (t.x - g.cx) * w.x + (t.y - g.cy) * w.y + (t.z - g.cz) * w.z;
+        if (t0 > r) s = make_float3(s.x + (t0 - r) * w.x, s.y + (t0 - r) * w.y, s.z + (t0 - r) * w.z);
+        if (t1 > r) t = make_float3(t.x - (t1 - r) * w.x, t.y - (t1 - r) * w.y, t.z - (t1 - r) * w.z);
+    }
+
+    // InitializeSiddon, cuda/siddon.cu:80-149
+    const float dist = sqrtf((s.x - t.x) * (s.x - t.x) + (s.y - t.y) * (s.y - t.y) + (s.z - t.z) * (s.z - t.z));
+    const float3 si = make_float3((s.x - g.cx) / g.dx + g.nx / 2.0f, (s.y - g.cy) / g.dy + g.ny / 2.0f,
+                                  (s.z - g.cz) / g.dz + g.nz / 2.0f);
+    const float3 ti = make_float3((t.x - g.cx) / g.dx + g.nx / 2.0f, (t.y - g.cy) / g.dy + g.ny / 2.0f,
+                                  (t.z - g.cz) / g.dz + g.nz / 2.0f);
+    const float dx = ti.x - si.x, dy = ti.y - si.y, dz = ti.z - si.z;
+    const bool px = fabsf(dx) > kEps, py = fabsf(dy) > kEps, pz = fabsf(dz) > kEps;
+    if ((!px && (si.x < 0 || si.x > g.nx)) || (!py && (si.y < 0 || si.y > g.ny)) ||
+        (!pz && (si.z < 0 || si.z > g.nz)))
+        return;
+    const float rdx = 1.0f / dx, rdy = 1.0f / dy, rdz = 1.0f / dz;
+    float a0x = 1.f, aNx = 0.f, a0y = 1.f, aNy = 0.f, a0z = 1.f, aNz = 0.f;
+    if (px) { a0x = -si.x * rdx; aNx = ((float)g.nx - si.x) * rdx; }
+    if (py) { a0y = -si.y * rdy; aNy = ((float)g.ny - si.y) * rdy; }
+    if (pz) { a0z = -si.z * rdz; aNz = ((float)g.nz - si.z) * rdz; }
+    const float amin = fmaxf(0.f, fmaxf(fminf(a0x, aNx), fmaxf(fminf(a0y, aNy), fminf(a0z, aNz))));
+    const float amax = fminf(1.f, fminf(fmaxf(a0x, aNx), fminf(fmaxf(a0y, aNy), fmaxf(a0z, aNz))));
+    if (amax <= amin) return;
+
+    int ix = (int)(si.x + amin * dx), iy = (int)(si.y + amin * dy), iz = (int)(si.z + amin * dz);
+    if (px) ix = min(max(ix, 0), g.nx - 1);
+    if (py) iy = min(max(iy, 0), g.ny - 1);
+    if (pz) iz = min(max(iz, 0), g.nz - 1);
+    if (ix < 0 || ix >= g.nx || iy < 0 || iy >= g.ny || iz < 0 || iz >= g.nz) return;
+
+    // first crossings and steps, scaled to mm
+    const float ax0 = px ? ((float)(dx > kEps ? ix + 1 : ix) - si.x) * rdx * dist : 2.f * dist;
+    const float ay0 = py ? ((float)(dy > kEps ? iy + 1 : iy) - si.y) * rdy * dist : 2.f * dist;
+    const float az0 = pz ? ((float)(dz > kEps ? iz + 1 : iz) - si.z) * rdz * dist : 2.f * dist;
+    const float dax = px ? fabsf(rdx) * dist : 0.f;
+    const float day = py ? fabsf(rdy) * dist : 0.f;
+    const float daz = pz ? fabsf(rdz) * dist : 0.f;
+    const int sx = dx > 0 ? 1 : -1, sy = dy > 0 ? 1 : -1, sz = dz > 0 ? 1 : -1;
+    const int nxy = g.nx * g.ny;
+    const int mx = sx, my = sy * g.nx, mz = sz * nxy;
+
+    float ax = ax0, ay = ay0, az = az0;
+    float cxn = 0.f, cyn = 0.f, czn = 0.f;
+    float aprev = amin * dist;
+    int idx = (iz * g.ny + iy) * g.nx + ix;
+    float val = 0.f;
+    while (true) {
+        const float f = img[idx];
+        const bool bx = (ax < ay) & (ax < az);
+        const bool by = !bx & (ay < az);
+        const bool bz = !bx & !by;
+        const float anow = bx ? ax : (by ? ay : az);
+        val = fmaf(f, anow - aprev, val);
+        aprev = anow;
+        idx += bx ? mx : (by ? my : mz);
+        ix += bx ? sx : 0;
+        iy += by ? sy : 0;
+        iz += bz ? sz : 0;
+        cxn += bx ? 1.f : 0.f;
+        cyn += by ? 1.f : 0.f;
+        czn += bz ? 1.f : 0.f;
+        ax = bx ? fmaf(cxn, dax, ax0) : ax;
+        ay = by ? fmaf(cyn, day, ay0) : ay;
+        az = bz ? fmaf(czn, daz, az0) : az;
+        if ((unsigned)ix >= (unsigned)g.nx || (unsigned)iy >= (unsigned)g.ny || (unsigned)iz >= (unsigned)g.nz) break;
+    }
+    prj[((size_t)view * d.nv + iv) * d.nu + iu] += val;
+}
+
+void siddon_cone_fp(float* prj, const float* img, const ConeGeomDev& geo, int B, const Grid& g, const Det& d,
+                    int nview, hipStream_t stream) {
+    const size_t nimg = (size_t)g.nx * g.ny * g.nz, nprj = (size_t)d.nu * d.nv * nview;
+    dim3 block(256), grid(ceil_div(d.nu, 16), ceil_div(d.nv, 16), nview);
+    for (int b = 0; b < B; b++) {
+        siddon_cone_fp_kernel<<<grid, block, 0, stream>>>(prj + b * nprj, img + b * nimg, geo.det_center, geo.det_u,
+                                                          geo.det_v, geo.src, g, d, nview);
+        CTP_LAUNCH_CHECK();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Backprojection
+// ---------------------------------------------------------------------------
+// Per-view constants of the voxel-driven backprojector (64 bytes; read through
+// the scalar cache since every lane of a block works on the same view).
+struct alignas(16) ConeViewBP {
+    float sx, sy, sz;       // source in image index space
+    float hu;               // conservative half extent of a voxel shadow, detector cells (u)
+    float pu[4];            // iu_f(X) numerator over (X, Y, Z, 1), X = voxel-centre index coordinates
+    float pv[4];            // iv_f(X) numerator
+    float pw[4];            // common denominator (> 0 in front of the source)
+    float hv;               // half extent along v
+    int flags;              // bit0: some ray of the view is parallel to an image axis
+    float fixx, fixy;       // trunc(source index) per axis, for axis-parallel rays
+    float fixz, pad0, pad1, pad2;
+};
+
+struct ConeBPPlan {
+    std::vector<ConeViewBP> views;
+    int wu = 0, wv = 0;
+};
+
+// Builds the per-view projective maps in FP64 from host geometry.
+static ConeBPPlan make_cone_bp_plan(const ConeGeomHost& h, const Grid& g, const Det& d, int nview) {
+    ConeBPPlan plan;
+    plan.views.resize(nview);
+    double wmax_u = 0, wmax_v = 0;
+    for (int i = 0; i < nview; i++) {
+        const double C[3] = {h.det_center[3 * i], h.det_center[3 * i + 1], h.det_center[3 * i + 2]};
+        const double U[3] = {h.det_u[3 * i], h.det_u[3 * i + 1], h.det_u[3 * i + 2]};
+        const double V[3] = {h.det_v[3 * i], h.det_v[3 * i + 1], h.det_v[3 * i + 2]};
+        const double S[3] = {h.src[3 * i], h.src[3 * i + 1], h.src[3 * i + 2]};
+        double n[3] = {U[1] * V[2] - U[2] * V[1], U[2] * V[0] - U[0] * V[2], U[0] * V[1] - U[1] * V[0]};
+        const double nn = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+        if (!(nn > 0)) fail("cone geometry: det_u and det_v are parallel in view " + std::to_string(i));
+        double D = (C[0] - S[0]) * n[0] + (C[1] - S[1]) * n[1] + (C[2] - S[2]) * n[2];
+        if (D < 0) { for (double& x : n) x = -x; D = -D; }
+        if (!(D > 0)) fail("cone geometry: source lies in the detector plane in view " + std::to_string(i));
+        // dual basis of (U, V) within the detector plane
+        const double Us[3] = {(V[1] * n[2] - V[2] * n[1]) / nn, (V[2] * n[0] - V[0] * n[2]) / nn,
+                              (V[0] * n[1] - V[1] * n[0]) / nn};
+        const double Vs[3] = {(n[1] * U[2] - n[2] * U[1]) / nn, (n[2] * U[0] - n[0] * U[2]) / nn,
+                              (n[0] * U[1] - n[1] * U[0]) / nn};
+        // orientation: (U x V).n must be positive for the dual basis above
+        double sgn = ((U[1] * V[2] - U[2] * V[1]) * n[0] + (U[2] * V[0] - U[0] * V[2]) * n[1] +
+                      (U[0] * V[1] - U[1] * V[0]) * n[2]) > 0 ? 1.0 : -1.0;
+        double Usx[3], Vsx[3];
+        for (int k = 0; k < 3; k++) { Usx[k] = Us[k] * sgn; Vsx[k] = Vs[k] * sgn; }
+        const double uS = (S[0] - C[0]) * Usx[0] + (S[1] - C[1]) * Usx[1] + (S[2] - C[2]) * Usx[2];
+        const double vS = (S[0] - C[0]) * Vsx[0] + (S[1] - C[1]) * Vsx[1] + (S[2] - C[2]) * Vsx[2];
+        // u_mm(x) = uS + D * ((x-S).Us) / ((x-S).n); cell index iu_f = u/du + off_u + (nu-1)/2
+        const double ou = d.off_u + (d.nu - 1) / 2.0, ov = d.off_v + (d.nv - 1) / 2.0;
+        // x_phys = (X - N/2) * dxyz + c with X in voxel index coordinates
+        const double dd[3] = {g.dx, g.dy, g.dz};
+        const double x0[3] = {g.cx - g.nx / 2.0 * g.dx, g.cy - g.ny / 2.0 * g.dy, g.cz - g.nz / 2.0 * g.dz};
+        double Pw[4], Pu[4], Pv[4];
+        Pw[3] = 0; Pu[3] = 0; Pv[3] = 0;
+        for (int k = 0; k < 3; k++) {
+            Pw[k] = n[k] * dd[k];
+            Pw[3] += n[k] * (x0[k] - S[k]);
+        }
+        double qu[4], qv[4];   // (x-S).Us and (x-S).Vs as affine maps of X
+        qu[3] = 0; qv[3] = 0;
+        for (int k = 0; k < 3; k++) {
+            qu[k] = Usx[k] * dd[k]; qu[3] += Usx[k] * (x0[k] - S[k]);
+            qv[k] = Vsx[k] * dd[k]; qv[3] += Vsx[k] * (x0[k] - S[k]);
+        }
+        for (int k = 0; k < 4; k++) {
+            Pu[k] = (uS / d.du + ou) * Pw[k] + D / d.du * qu[k];
+            Pv[k] = (vS / d.dv + ov) * Pw[k] + D / d.dv * qv[k];
+        }
+        // bound over the volume: corners of [0,N]^3 in index coordinates
+        double umin = 1e300, umax = -1e300, vmin = 1e300, vmax = -1e300, wmin = 1e300;
+        for (int cc = 0; cc < 8; cc++) {
+            const double X[3] = {(cc & 1) ? (double)g.nx : 0.0, (cc & 2) ? (double)g.ny : 0.0, (cc & 4) ? (double)g.nz : 0.0};
+            double w = Pw[3], a = Pu[3], b = Pv[3];
+            for (int k = 0; k < 3; k++) { w += Pw[k] * X[k]; a += Pu[k] * X[k]; b += Pv[k] * X[k]; }
+            wmin = std::min(wmin, w);
+            if (w > 0) {
+                umin = std::min(umin, a / w); umax = std::max(umax, a / w);
+                vmin = std::min(vmin, b / w); vmax = std::max(vmax, b / w);
+            }
+        }
+        const double bw = 0.5 * (std::fabs(Pw[0]) + std::fabs(Pw[1]) + std::fabs(Pw[2]));
+        if (!(wmin - bw > 0.02 * D))
+            fail("cone backprojection: the source is inside or too close to the volume in view " + std::to_string(i));
+        auto hnum = [&](const double* P, double uc) {
+            return 0.5 * (std::fabs(P[0] - uc * Pw[0]) + std::fabs(P[1] - uc * Pw[1]) + std::fabs(P[2] - uc * Pw[2]));
+        };
+        const double margin = 1e-3;
+        const double hu = std::max(hnum(Pu, umin), hnum(Pu, umax)) / (wmin - bw) + margin;
+        const double hv = std::max(hnum(Pv, vmin), hnum(Pv, vmax)) / (wmin - bw) + margin;
+        wmax_u = std::max(wmax_u, 2 * hu);
+        wmax_v = std::max(wmax_v, 2 * hv);
+
+        ConeViewBP& cv = plan.views[i];
+        cv.sx = (float)((S[0] - g.cx) / g.dx + g.nx / 2.0);
+        cv.sy = (float)((S[1] - g.cy) / g.dy + g.ny / 2.0);
+        cv.sz = (float)((S[2] - g.cz) / g.dz + g.nz / 2.0);
+        // normalise the projective rows so that the denominator is O(1)
+        const double scale = 1.0 / D;
+        for (int k = 0; k < 4; k++) {
+            cv.pu[k] = (float)(Pu[k] * scale);
+            cv.pv[k] = (float)(Pv[k] * scale);
+            cv.pw[k] = (float)(Pw[k] * scale);
+        }
+        cv.hu = (float)hu;
+        cv.hv = (float)hv;
+        cv.fixx = std::trunc((float)cv.sx);
+        cv.fixy = std::trunc((float)cv.sy);
+        cv.fixz = std::trunc((float)cv.sz);
+        // axis-parallel rays: the direction component w_k(u, v) is affine over the
+        // detector; flag the view when it can vanish on the detector.
+        int flags = 0;
+        const double umm[2] = {(0 - ou) * d.du, (d.nu - 1 - ou) * d.du};
+        const double vmm[2] = {(0 - ov) * d.dv, (d.nv - 1 - ov) * d.dv};
+        for (int k = 0; k < 3; k++) {
+            double lo = 1e300, hi = -1e300, mag = 0;
+            for (int a = 0; a < 2; a++)
+                for (int b = 0; b < 2; b++) {
+                    double w = C[k] + U[k] * umm[a] + V[k] * vmm[b] - S[k];
+                    lo = std::min(lo, w); hi = std::max(hi, w); mag = std::max(mag, std::fabs(w));
+                }
+            if (lo <= 1e-5 * mag && hi >= -1e-5 * mag) flags |= 1;
+        }
+        cv.flags = flags;
+        cv.pad0 = cv.pad1 = cv.pad2 = 0;
+    }
+    plan.wu = (int)std::floor(wmax_u) + 1;
+    plan.wv = (int)std::floor(wmax_v) + 1;
+    return plan;
+}
+
+// Ray table for one batch item: (1/w_x, 1/w_y, 1/w_z, p * |q - S|) per ray in
+// image index space, w = q - S with q the detector cell centre.  Computed in
+// FP64.  Rays whose segment [S, q] (after MoveSourceDstNearGrid) misses the
+// grid get weight 0 (InitializeSiddon's no-intersection exits,
+// cuda/siddon.cu:98-115,155-161); axis-parallel components (|d_k| <= 1e-6 in the
+// reference, cuda/siddon.cu:15) are stored as 0.
+__global__ __launch_bounds__(256) void siddon_bp_table_kernel(float4* __restrict__ table, const float* __restrict__ prj,
+                                                              const float* __restrict__ pdc, const float* __restrict__ pdu,
+                                                              const float* __restrict__ pdv, const float* __restrict__ psrc,
+                                                              Grid g, Det d, int nview) {
+    const long n = (long)d.nu * d.nv * nview;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int iu = (int)(i % d.nu);
+    const int iv = (int)((i / d.nu) % d.nv);
+    const int view = (int)(i / ((long)d.nu * d.nv));
+    const double u = ((double)iu - d.off_u - (d.nu - 1) / 2.0) * d.du;
+    const double v = ((double)iv - d.off_v - (d.nv - 1) / 2.0) * d.dv;
+    double S[3], Q[3];
+    for (int k = 0; k < 3; k++) {
+        S[k] = psrc[3 * view + k];
+        Q[k] = pdc[3 * view + k] + pdu[3 * view + k] * u + pdv[3 * view + k] * v;
+    }
+    const double dd[3] = {g.dx, g.dy, g.dz};
+    const double cc[3] = {g.cx, g.cy, g.cz};
+    const int nn[3] = {g.nx, g.ny, g.nz};
+    // MoveSourceDstNearGrid in FP64
+    double Sm[3] = {S[0], S[1], S[2]}, Qm[3] = {Q[0], Q[1], Q[2]};
+    {
+        double ex = g.nx * (double)g.dx, ey = g.ny * (double)g.dy, ez = g.nz * (double)g.dz;
+        double r = sqrt(ex * ex + ey * ey + ez * ez);
+        double w[3] = {Q[0] - S[0], Q[1] - S[1], Q[2] - S[2]};
+        double len = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+        if (!(len < 1e-6)) for (int k = 0; k < 3; k++) w[k] /= len;
+        double t0 = 0, t1 = 0;
+        for (int k = 0; k < 3; k++) { t0 += (cc[k] - S[k]) * w[k]; t1 += (Q[k] - cc[k]) * w[k]; }
+        if (t0 > r) for (int k = 0; k < 3; k++) Sm[k] = S[k] + (t0 - r) * w[k];
+        if (t1 > r) for (int k = 0; k < 3; k++) Qm[k] = Q[k] - (t1 - r) * w[k];
+    }
+    double L = 0, si[3], wi[3], di[3];
+    for (int k = 0; k < 3; k++) {
+        L += (Q[k] - S[k]) * (Q[k] - S[k]);
+        si[k] = (S[k] - cc[k]) / dd[k] + nn[k] / 2.0;
+        wi[k] = (Q[k] - S[k]) / dd[k];
+        di[k] = (Qm[k] - Sm[k]) / dd[k];
+    }
+    L = sqrt(L);
+    // segment [Sm, Qm] against the grid box [0, n]^3
+    const double smi[3] = {(Sm[0] - cc[0]) / dd[0] + nn[0] / 2.0, (Sm[1] - cc[1]) / dd[1] + nn[1] / 2.0,
+                           (Sm[2] - cc[2]) / dd[2] + nn[2] / 2.0};
+    double amin = 0, amax = 1;
+    bool hit = true;
+    float inv[3];
+    for (int k = 0; k < 3; k++) {
+        if (fabs(di[k]) > 1e-6) {
+            double a0 = (0 - smi[k]) / di[k], a1 = (nn[k] - smi[k]) / di[k];
+            amin = fmax(amin, fmin(a0, a1));
+            amax = fmin(amax, fmax(a0, a1));
+        } else if (smi[k] < 0 || smi[k] > nn[k]) {
+            hit = false;
+        }
+        inv[k] = (fabs(di[k]) > 1e-6) ? (float)(1.0 / wi[k]) : 0.f;
+    }
+    if (!(amax > amin)) hit = false;
+    for (int k = 0; k < 3; k++) {
+        if (!(fabs(di[k]) > 1e-6)) {
+            // the reference's first voxel along an axis-parallel direction is
+            // trunc(coordinate); outside [0, n) the ray contributes nothing
+            int fixed = (int)(smi[k] + amin * di[k]);
+            if (fixed < 0 || fixed >= nn[k]) hit = false;
+        }
+    }
+    (void)si;
+    const float p = prj[i];
+    table[i] = make_float4(inv[0], inv[1], inv[2], hit ? (float)(p * L) : 0.f);
+}
+
+template <int WU, int WV, int NZR>
+__global__ __launch_bounds__(256) void siddon_cone_bp_kernel(float* __restrict__ img, const float4* __restrict__ table,
+                                                             const ConeViewBP* __restrict__ views, int nview, Grid g,
+                                                             int nu, int nv) {
+    const int ix = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int iy = blockIdx.y * 16 + (threadIdx.x >> 4);
+    const int iz0 = blockIdx.z * NZR;
+    if (ix >= g.nx || iy >= g.ny) return;
+    const float xc = ix + 0.5f, yc = iy + 0.5f;
+    float acc[NZR];
+#pragma unroll
+    for (int k = 0; k < NZR; k++) acc[k] = 0.f;
+    const size_t nuv = (size_t)nu * nv;
+    for (int view = 0; view < nview; view++) {
+        const ConeViewBP cv = views[view];
+        const float4* __restrict__ tab = table + view * nuv;
+        const float ex = xc - cv.sx, ey = yc - cv.sy;
+        const float bu = fmaf(cv.pu[0], xc, fmaf(cv.pu[1], yc, cv.pu[3]));
+        const float bv = fmaf(cv.pv[0], xc, fmaf(cv.pv[1], yc, cv.pv[3]));
+        const float bw = fmaf(cv.pw[0], xc, fmaf(cv.pw[1], yc, cv.pw[3]));
+        const bool slow = cv.flags & 1;
+#pragma unroll
+        for (int k = 0; k < NZR; k++) {
+            const float zc = (float)(iz0 + k) + 0.5f;
+            const float den = fmaf(cv.pw[2], zc, bw);
+            const float rd = __builtin_amdgcn_rcpf(den);
+            const float uc = fmaf(cv.pu[2], zc, bu) * rd;
+            const float vc = fmaf(cv.pv[2], zc, bv) * rd;
+            const int u0 = min(max((int)ceilf(uc - cv.hu), 0), nu - WU);
+            const int v0 = min(max((int)ceilf(vc - cv.hv), 0), nv - WV);
+            const float ez = zc - cv.sz;
+            float a = 0.f;
+#pragma unroll
+            for (int jv = 0; jv < WV; jv++) {
+#pragma unroll
+                for (int ju = 0; ju < WU; ju++) {
+                    const float4 r = tab[(size_t)(v0 + jv) * nu + (u0 + ju)];
+                    const float mx = ex * r.x, my = ey * r.y, mz = ez * r.z;
+                    float ix0 = fmaf(fabsf(r.x), -0.5f, mx), ix1 = fmaf(fabsf(r.x), 0.5f, mx);
+                    float iy0 = fmaf(fabsf(r.y), -0.5f, my), iy1 = fmaf(fabsf(r.y), 0.5f, my);
+                    float iz0f = fmaf(fabsf(r.z), -0.5f, mz), iz1f = fmaf(fabsf(r.z), 0.5f, mz);
+                    if (slow) {
+                        // axis-parallel component: inside the slab iff the voxel index
+                        // equals the ray's (constant) index along that axis
+                        const float big = 3.0e38f;
+                        if (r.x == 0.f) { const bool in = (float)ix == cv.fixx; ix0 = in ? -big : big; ix1 = in ? big : -big; }
+                        if (r.y == 0.f) { const bool in = (float)iy == cv.fixy; iy0 = in ? -big : big; iy1 = in ? big : -big; }
+                        if (r.z == 0.f) { const bool in = (float)(iz0 + k) == cv.fixz; iz0f = in ? -big : big; iz1f = in ? big : -big; }
+                    }
+                    const float tin = fmaxf(fmaxf(fmaxf(ix0, iy0), iz0f), 0.f);
+                    const float tout = fminf(fminf(ix1, iy1), iz1f);
+                    a = fmaf(fmaxf(tout - tin, 0.f), r.w, a);
+                }
+            }
+            acc[k] += a;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NZR; k++) {
+        const int iz = iz0 + k;
+        if (iz < g.nz) img[((size_t)iz * g.ny + iy) * g.nx + ix] += acc[k];
+    }
+}
+
+template <int WU, int WV>
+static void launch_bp(float* img, const float4* table, const ConeViewBP* views, int nview, const Grid& g, const Det& d,
+                      hipStream_t stream) {
+    constexpr int NZR = 4;
+    dim3 block(256), grid(ceil_div(g.nx, 16), ceil_div(g.ny, 16), ceil_div(g.nz, NZR));
+    siddon_cone_bp_kernel<WU, WV, NZR><<<grid, block, 0, stream>>>(img, table, views, nview, g, d.nu, d.nv);
+    CTP_LAUNCH_CHECK();
+}
+
+void siddon_cone_bp(float* img, const float* prj, const ConeGeomDev& geo, const ConeGeomHost& hgeo, int B,
+                    const Grid& g, const Det& d, int nview, hipStream_t stream) {
+    ConeBPPlan plan = make_cone_bp_plan(hgeo, g, d, nview);
+    int wu = plan.wu, wv = plan.wv;
+    if (wu > 4 || wv > 4)
+        fail("cone backprojection: voxel shadow spans more than 4 detector cells (" + std::to_string(wu) + "x" +
+             std::to_string(wv) + "); use finer voxels or coarser detector cells");
+    wu = std::min(std::max(wu, 1), d.nu);
+    wv = std::min(std::max(wv, 1), d.nv);
+    const size_t nimg = (size_t)g.nx * g.ny * g.nz, nprj = (size_t)d.nu * d.nv * nview;
+    Workspace& ws = workspace_for(stream);
+    Plan pl;
+    pl.add<ConeViewBP>(nview);
+    pl.add<float4>(nprj);
+    ws.reserve(pl.bytes);
+    ConeViewBP* dviews = ws.take_n<ConeViewBP>(nview);
+    float4* table = ws.take_n<float4>(nprj);
+    CTP_CHECK(hipMemcpyAsync(dviews, plan.views.data(), sizeof(ConeViewBP) * nview, hipMemcpyHostToDevice, stream));
+    for (int b = 0; b < B; b++) {
+        siddon_bp_table_kernel<<<ceil_div((long)nprj, 256), 256, 0, stream>>>(table, prj + b * nprj, geo.det_center,
+                                                                            geo.det_u, geo.det_v, geo.src, g, d, nview);
+        CTP_LAUNCH_CHECK();
+        float* im = img + b * nimg;
+#define CTP_BP_CASE(A, Bv) \
+    if (wu == A && wv == Bv) { launch_bp<A, Bv>(im, table, dviews, nview, g, d, stream); continue; }
+        CTP_BP_CASE(1, 1) CTP_BP_CASE(1, 2) CTP_BP_CASE(2, 1) CTP_BP_CASE(2, 2)
+        CTP_BP_CASE(2, 3) CTP_BP_CASE(3, 2) CTP_BP_CASE(3, 3) CTP_BP_CASE(1, 3) CTP_BP_CASE(3, 1)
+        CTP_BP_CASE(1, 4) CTP_BP_CASE(4, 1) CTP_BP_CASE(2, 4) CTP_BP_CASE(4, 2) CTP_BP_CASE(3, 4)
+        CTP_BP_CASE(4, 3) CTP_BP_CASE(4, 4)
+#undef CTP_BP_CASE
+        fail("cone backprojection: unsupported window");
+    }
+    // host plan vector must outlive the async copy
+    CTP_CHECK(hipStreamSynchronize(stream));
+}
+
+}  // namespace ctp

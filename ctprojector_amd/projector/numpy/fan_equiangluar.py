@@ -1,0 +1,88 @@
+"""Equiangular fan-beam geometry (the reference's
+src/ct_projector/projector/numpy/fan_equiangluar.py, module name spelt as
+there).  ``projector.du`` is the detector pitch in mm at distance dsd; the
+kernels receive the angular pitch du/dsd (:60,117,287,354)."""
+
+from ctypes import POINTER, c_float, c_int, c_ulong
+from typing import Union
+
+import numpy as np
+
+from ... import _lib
+from ..geometry import filter_code
+from .ct_projector import ct_projector
+
+
+def _p(a):
+    return a.ctypes.data_as(POINTER(c_float))
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _grid(img_shape, p):
+    return (c_ulong(img_shape[0]), c_ulong(img_shape[3]), c_ulong(img_shape[2]), c_ulong(img_shape[1]),
+            c_float(p.dx), c_float(p.dy), c_float(p.dz), c_float(p.cx), c_float(p.cy), c_float(p.cz))
+
+
+def _det(prj_shape, p):
+    return (c_ulong(prj_shape[3]), c_ulong(prj_shape[2]), c_ulong(prj_shape[1]),
+            c_float(p.du / p.dsd), c_float(p.dv), c_float(p.off_u), c_float(p.off_v),
+            c_float(p.dsd), c_float(p.dso))
+
+
+def ramp_filter(projector: ct_projector, prj: np.array, filter_type: str = 'hann') -> np.array:
+    """Ramp filtering for the equiangular detector; 'hamming', 'hann', 'cosine',
+    anything else is Ram-Lak (fan_equiangluar.py:19-72)."""
+    prj = _f32(prj)
+    fprj = np.zeros(prj.shape, np.float32)
+    err = _lib.module().cfbpFanFilter(
+        _p(fprj), _p(prj), c_ulong(prj.shape[0]), c_ulong(prj.shape[3]), c_ulong(prj.shape[2]),
+        c_ulong(prj.shape[1]), c_float(projector.du / projector.dsd), c_float(projector.dv),
+        c_float(projector.off_u), c_float(projector.off_v), c_float(projector.dsd), c_float(projector.dso),
+        c_int(filter_code(filter_type)))
+    if err != 0:
+        print(err)
+    return fprj
+
+
+def fbp_bp(projector: ct_projector, prj: np.array, angles: np.array) -> np.array:
+    """Pixel-driven FBP-weighted backprojection (fan_equiangluar.py:75-128)."""
+    prj = _f32(prj)
+    angles = _f32(angles)
+    img = np.zeros([prj.shape[0], projector.nz, projector.ny, projector.nx], np.float32)
+    err = _lib.module().cfbpFanBackprojection(_p(img), _p(prj), _p(angles), *_grid(img.shape, projector),
+                                              *_det(prj.shape, projector))
+    if err != 0:
+        print(err)
+    return img
+
+
+def distance_driven_fp(projector: ct_projector, img: np.array, angles: np.array,
+                       branchless: Union[bool, int] = False) -> np.array:
+    """Distance-driven forward projection (fan_equiangluar.py:241-299)."""
+    type_projector = 2 if branchless else 0
+    img = _f32(img)
+    angles = _f32(angles)
+    prj = np.zeros([img.shape[0], len(angles), projector.nv, projector.nu], np.float32)
+    err = _lib.module().cDistanceDrivenFanProjection(_p(prj), _p(img), _p(angles), *_grid(img.shape, projector),
+                                                     *_det(prj.shape, projector), c_int(type_projector))
+    if err != 0:
+        print(err)
+    return prj
+
+
+def distance_driven_bp(projector: ct_projector, prj: np.array, angles: np.array, is_fbp: Union[bool, int] = False,
+                       branchless: Union[bool, int] = False) -> np.array:
+    """Distance-driven backprojection; is_fbp selects the FBP weighting
+    (fan_equiangluar.py:302-366)."""
+    type_projector = (1 if is_fbp else 0) + (2 if branchless else 0)
+    prj = _f32(prj)
+    angles = _f32(angles)
+    img = np.zeros([prj.shape[0], projector.nz, projector.ny, projector.nx], np.float32)
+    err = _lib.module().cDistanceDrivenFanBackprojection(_p(img), _p(prj), _p(angles), *_grid(img.shape, projector),
+                                                         *_det(prj.shape, projector), c_int(type_projector))
+    if err != 0:
+        print(err)
+    return img

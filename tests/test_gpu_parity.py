@@ -1,0 +1,199 @@
+"""GPU parity: every hot-path entry point of libprojector.so against the FP64
+oracle on small seeded cases (rel-L2 tolerance 1e-4, FP32 arithmetic), plus the
+adjoint dot-product test of the Siddon pair.  Calls go through the C ABI (the
+host-array ``c*`` entry points via the numpy front end)."""
+
+import numpy as np
+import pytest
+
+from ctprojector_amd.projector.numpy import cone, fan_equiangluar as fan, parallel, tomo
+from ctprojector_amd.projector.numpy.ct_projector import ct_projector
+
+from helpers import rel_l2, small_cone, small_fan, small_parallel, Geo
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-4
+
+
+def _proj(g):
+    return g.to(ct_projector())
+
+
+# ----------------------------------------------------------------- Siddon cone
+@pytest.mark.parametrize('case', ['plain', 'offsets'])
+def test_siddon_cone_fp_bp(oracle, case):
+    if case == 'plain':
+        g, geo, img = small_cone()
+    else:
+        g, geo, img = small_cone(seed=11, off=(0.3, -0.7), c=(1.5, -2.0, 0.5))
+    p = _proj(g)
+    args = (geo.det_center, geo.det_u, geo.det_v, geo.src)
+    fp = cone.siddon_fp_arbitrary(p, img, *args)
+    ref = oracle.siddon_cone_fp(g, img, *args)
+    e = rel_l2(fp, ref)
+    print('siddon fp rel-L2', e)
+    assert e < TOL
+    y = np.random.default_rng(1).uniform(size=ref.shape).astype(np.float32)
+    bp = cone.siddon_bp_arbitrary(p, y, *args)
+    refb = oracle.siddon_cone_bp(g, y, *args)
+    e = rel_l2(bp, refb)
+    print('siddon bp rel-L2', e)
+    assert e < TOL
+
+
+def test_siddon_cone_adjoint():
+    g, geo, img = small_cone(seed=5)
+    p = _proj(g)
+    args = (geo.det_center, geo.det_u, geo.det_v, geo.src)
+    rng = np.random.default_rng(2)
+    x = rng.uniform(size=img.shape).astype(np.float32)
+    fpx = cone.siddon_fp_arbitrary(p, x, *args)
+    y = rng.uniform(size=fpx.shape).astype(np.float32)
+    bty = cone.siddon_bp_arbitrary(p, y, *args)
+    lhs = np.dot(fpx.ravel().astype(np.float64), y.ravel().astype(np.float64))
+    rhs = np.dot(x.ravel().astype(np.float64), bty.ravel().astype(np.float64))
+    rel = abs(lhs - rhs) / (np.linalg.norm(fpx) * np.linalg.norm(y))
+    print('adjoint', lhs, rhs, rel)
+    assert rel < 1e-6
+
+
+def test_siddon_cone_batch():
+    g, geo, img = small_cone(seed=9, n=16, nview=6, nu=24, nv=20)
+    img2 = np.concatenate([img, 2 * img[:, ::-1].copy()], 0)
+    p = _proj(g)
+    args = (geo.det_center, geo.det_u, geo.det_v, geo.src)
+    fp = cone.siddon_fp_arbitrary(p, img2, *args)
+    for b in range(2):
+        assert rel_l2(fp[b:b + 1], cone.siddon_fp_arbitrary(p, img2[b:b + 1], *args)) < 1e-6
+
+
+# ------------------------------------------------------------ DD fan / parallel
+@pytest.mark.parametrize('branchless', [False, True])
+@pytest.mark.parametrize('variant', ['plain', 'offsets', 'batch'])
+def test_dd_fan_fp(oracle, branchless, variant):
+    kw = {}
+    if variant == 'offsets':
+        kw = dict(off=(0.4, 0.3), c=(2.0, -1.0, 0.6))
+    if variant == 'batch':
+        kw = dict(B=2)
+    g, ang, img = small_fan(**kw)
+    p = _proj(g)
+    fp = fan.distance_driven_fp(p, img, ang, branchless=branchless)
+    ref = oracle.dd_fan_fp(g, img, ang, branchless=branchless)
+    e = rel_l2(fp, ref)
+    print('dd fan fp', branchless, variant, e)
+    assert e < TOL
+
+
+@pytest.mark.parametrize('branchless', [False, True])
+@pytest.mark.parametrize('is_fbp', [False, True])
+@pytest.mark.parametrize('variant', ['plain', 'offsets', 'batch'])
+def test_dd_fan_bp(oracle, branchless, is_fbp, variant):
+    kw = {}
+    if variant == 'offsets':
+        kw = dict(off=(0.4, 0.3), c=(2.0, -1.0, 0.6))
+    if variant == 'batch':
+        kw = dict(B=2)
+    g, ang, img = small_fan(**kw)
+    p = _proj(g)
+    y = np.random.default_rng(4).uniform(size=[img.shape[0], len(ang), g.nv, g.nu]).astype(np.float32)
+    bp = fan.distance_driven_bp(p, y, ang, is_fbp=is_fbp, branchless=branchless)
+    ref = oracle.dd_fan_bp(g, y, ang, is_fbp=is_fbp, branchless=branchless)
+    e = rel_l2(bp, ref)
+    print('dd fan bp', branchless, is_fbp, variant, e)
+    assert e < TOL
+
+
+@pytest.mark.parametrize('branchless', [False, True])
+@pytest.mark.parametrize('variant', ['plain', 'offsets', 'batch'])
+def test_dd_parallel_fp_bp(oracle, branchless, variant):
+    kw = {}
+    if variant == 'offsets':
+        kw = dict(off=(0.4, -0.2))
+    if variant == 'batch':
+        kw = dict(B=2)
+    g, ang, img = small_parallel(**kw)
+    p = _proj(g)
+    fp = parallel.distance_driven_fp(p, img, ang, branchless=branchless)
+    ref = oracle.dd_par_fp(g, img, ang, branchless=branchless)
+    e = rel_l2(fp, ref)
+    print('dd par fp', branchless, variant, e)
+    assert e < TOL
+    for is_fbp in (False, True):
+        y = np.random.default_rng(6).uniform(size=ref.shape).astype(np.float32)
+        bp = parallel.distance_driven_bp(p, y, ang, is_fbp=is_fbp, branchless=branchless)
+        refb = oracle.dd_par_bp(g, y, ang, is_fbp=is_fbp, branchless=branchless)
+        e = rel_l2(bp, refb)
+        print('dd par bp', branchless, is_fbp, variant, e)
+        assert e < TOL
+
+
+# --------------------------------------------------------------- FBP pieces
+@pytest.mark.parametrize('filt', ['hann', 'hamming', 'cosine', 'ramp'])
+def test_fan_ramp_filter(oracle, filt):
+    g, ang, img = small_fan(off=(0.25, 0.0))
+    p = _proj(g)
+    prj = oracle.dd_fan_fp(g, img, ang).astype(np.float32)
+    out = fan.ramp_filter(p, prj, filt)
+    ref = oracle.fan_ramp_filter(g, prj, filt)
+    e = rel_l2(out, ref)
+    print('fan filter', filt, e)
+    assert e < TOL
+
+
+@pytest.mark.parametrize('filt', ['hann', 'hamming', 'cosine', 'ramp'])
+def test_parallel_ramp_filter(oracle, filt):
+    g, ang, img = small_parallel()
+    p = _proj(g)
+    prj = oracle.dd_par_fp(g, img, ang).astype(np.float32)
+    out = parallel.ramp_filter(p, prj, filt)
+    ref = oracle.par_ramp_filter(g, prj, filt)
+    e = rel_l2(out, ref)
+    print('par filter', filt, e)
+    assert e < TOL
+
+
+@pytest.mark.parametrize('variant', ['plain', 'offsets', 'batch'])
+def test_fan_fbp_bp(oracle, variant):
+    kw = dict(nz=7, nv=6)
+    if variant == 'offsets':
+        kw.update(off=(0.4, 0.3), c=(2.0, -1.0, 0.6))
+    if variant == 'batch':
+        kw.update(B=2)
+    g, ang, img = small_fan(**kw)
+    p = _proj(g)
+    y = np.random.default_rng(8).uniform(size=[img.shape[0], len(ang), g.nv, g.nu]).astype(np.float32)
+    bp = fan.fbp_bp(p, y, ang)
+    ref = oracle.fan_fbp_bp(g, y, ang)
+    e = rel_l2(bp, ref)
+    print('fan fbp bp', variant, e)
+    assert e < TOL
+
+
+# ----------------------------------------------------------------- tomo
+def _tomo_case(B=1):
+    g = Geo(nx=40, ny=36, nz=12, dx=1.0, dy=1.0, dz=1.0, cz=20.0, nu=56, nv=52, du=1.0, dv=1.0)
+    nview = 7
+    t = np.linspace(-0.4, 0.4, nview)
+    src = np.stack([300 * np.sin(t), 5 * np.ones_like(t), 300 * np.cos(t) + 20], 1).astype(np.float32)
+    dc = np.stack([np.zeros_like(t), np.zeros_like(t), -10 * np.ones_like(t)], 1).astype(np.float32)
+    from ctprojector_amd.phantom import ellipsoid_phantom
+    img = np.stack([ellipsoid_phantom(12, 36, 40, 1.0, 1.0, 1.0, seed=21 + b) for b in range(B)], 0)
+    return g, dc, src, img
+
+
+@pytest.mark.parametrize('branchless', [False, True])
+def test_tomo_fp_bp(oracle, branchless):
+    g, dc, src, img = _tomo_case(B=2)
+    p = _proj(g)
+    fp = tomo.distance_driven_fp(p, img, dc, src, branchless=branchless)
+    ref = oracle.dd_tomo_fp(g, img, dc, src, branchless=branchless)
+    e = rel_l2(fp, ref)
+    print('tomo fp', branchless, e)
+    assert e < TOL
+    y = np.random.default_rng(3).uniform(size=ref.shape).astype(np.float32)
+    bp = tomo.distance_driven_bp(p, y, dc, src, branchless=branchless)
+    refb = oracle.dd_tomo_bp(g, y, dc, src, branchless=branchless)
+    e = rel_l2(bp, refb)
+    print('tomo bp', branchless, e)
+    assert e < TOL

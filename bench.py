@@ -1,0 +1,267 @@
+#!/usr/bin/env python3
+"""Projector hot-path benchmark on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-extra]
+
+One step = one forward projection plus one backprojection of the
+fan-beam distance-driven workload: volume 1x32x512x512 at 0.7 mm (1 mm slices),
+984 views over 360 degrees, 888 columns x 32 rows, dso 595 mm, dsd 1085.6 mm,
+seeded ellipsoid phantom (synthetic).  Inputs are resident on the GPU when the
+timed region starts; outputs are produced by the library's device entry points
+(``cupy*`` symbols of libprojector.so) through the tensor front end.
+
+The JSON line reports
+  value        FP+BP voxel-updates per second (1 voxel-update = one (voxel, view)
+               pair in one pass; a step counts 2 * B*nx*ny*nz*nview), in GVU/s;
+  roofline     for the dominant kernel: algorithmic bytes per launch (4 B per
+               voxel-update: the FP32 voxel operand streamed once per view)
+               divided by its mean launch time, measured with HIP events on the
+               launching stream, against the 8 TB/s HBM peak;
+  cpu_baseline the FP32 OpenMP restatement of the reference algorithm (the
+               oracle's FP32 mirror) on a bounded view subset of the same
+               workload, on this host's cores;
+  extra        the cone-beam Siddon FP+BP (256^3 volume at 0.5 mm, 720 views,
+               512x384 panel) and the fan ramp filter + FBP backprojection,
+               measured the same way.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def kernel_stats(times, name):
+    ms, n = times.get(name, (0.0, 0))
+    return (ms / n if n else 0.0), n
+
+
+def fan_workload_bench(steps, warmup):
+    import torch
+    from ctprojector_amd import _lib
+    from ctprojector_amd.phantom import fan_512x32_workload
+    from ctprojector_amd.projector.cupy import ct_projector, fan_equiangular as fan
+
+    w, img = fan_512x32_workload()
+    p = ct_projector()
+    for k in ('nx', 'ny', 'nz', 'dx', 'dy', 'dz', 'cx', 'cy', 'cz', 'nu', 'nv', 'du', 'dv', 'off_u', 'off_v',
+              'dsd', 'dso', 'nview'):
+        setattr(p, k, getattr(w, k))
+    dev = torch.device('cuda', 0)
+    img_d = torch.from_numpy(img).to(dev)
+    ang_d = torch.from_numpy(w.angles).to(dev)
+    nvu = img.shape[0] * w.nx * w.ny * w.nz * w.nview
+
+    log('[bench] fan DD warmup x%d' % warmup)
+    for _ in range(warmup):
+        prj = fan.distance_driven_fp(p, img_d, ang_d)
+        _ = fan.distance_driven_bp(p, prj, ang_d)
+    torch.cuda.synchronize()
+
+    _lib.set_kernel_timing(True)
+    _lib.reset_kernel_times()
+    log('[bench] fan DD timed x%d' % steps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        prj = fan.distance_driven_fp(p, img_d, ang_d)
+        bp = fan.distance_driven_bp(p, prj, ang_d)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    times = _lib.kernel_times()
+    _lib.set_kernel_timing(False)
+    ms_step = (t1 - t0) / steps * 1e3
+    finite = bool(torch.isfinite(bp).all().item()) and bool(torch.isfinite(prj).all().item())
+    return dict(w=w, img=img, nvu=nvu, ms_step=ms_step, times=times, finite=finite)
+
+
+def cone_extra(steps):
+    import torch
+    from ctprojector_amd import _lib
+    from ctprojector_amd.phantom import cone_workload
+    from ctprojector_amd.projector.cupy import ct_projector, cone
+
+    w, img, geo = cone_workload(2, 720)
+    p = ct_projector()
+    for k in ('nx', 'ny', 'nz', 'dx', 'dy', 'dz', 'cx', 'cy', 'cz', 'nu', 'nv', 'du', 'dv', 'off_u', 'off_v'):
+        setattr(p, k, getattr(w, k))
+    dev = torch.device('cuda', 0)
+    img_d = torch.from_numpy(img).to(dev)
+    g = [torch.from_numpy(a).to(dev) for a in (geo.det_center, geo.det_u, geo.det_v, geo.src)]
+    nvu = w.nx * w.ny * w.nz * w.nview
+    log('[bench] cone Siddon warmup')
+    prj = cone.siddon_fp_arbitrary(p, img_d, *g)
+    _ = cone.siddon_bp_arbitrary(p, prj, *g)
+    torch.cuda.synchronize()
+    _lib.set_kernel_timing(True)
+    _lib.reset_kernel_times()
+    log('[bench] cone Siddon timed x%d' % steps)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        prj = cone.siddon_fp_arbitrary(p, img_d, *g)
+        bp = cone.siddon_bp_arbitrary(p, prj, *g)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    times = _lib.kernel_times()
+    _lib.set_kernel_timing(False)
+    ms = (t1 - t0) / steps * 1e3
+    fp_ms, _ = kernel_stats(times, 'siddon_cone_fp')
+    bp_ms, _ = kernel_stats(times, 'siddon_cone_bp')
+    tab_ms, _ = kernel_stats(times, 'siddon_bp_table')
+    out = {
+        'workload': 'cone Siddon FP+BP, 256^3 @0.5 mm, 720 views, 512x384 panel, jittered helical',
+        'ms_per_step': round(ms, 3),
+        'value_gvu_s': round(2 * nvu / (ms / 1e3) / 1e9, 2),
+        'kernel_ms': {'siddon_cone_fp': round(fp_ms, 3), 'siddon_bp_table': round(tab_ms, 3),
+                      'siddon_cone_bp': round(bp_ms, 3)},
+        'roofline_frac_fp_bp_kernels': round(2 * 4 * nvu / ((fp_ms + bp_ms + tab_ms) / 1e3) / 1e9 / PEAK_HBM_GBS, 4)
+        if fp_ms > 0 else None,
+    }
+    del prj, bp
+    return out
+
+
+def fbp_extra(steps, w, img):
+    import torch
+    from ctprojector_amd import _lib
+    from ctprojector_amd.projector.cupy import ct_projector, fan_equiangular as fan
+
+    p = ct_projector()
+    for k in ('nx', 'ny', 'nz', 'dx', 'dy', 'dz', 'cx', 'cy', 'cz', 'nu', 'nv', 'du', 'dv', 'off_u', 'off_v',
+              'dsd', 'dso', 'nview'):
+        setattr(p, k, getattr(w, k))
+    dev = torch.device('cuda', 0)
+    ang_d = torch.from_numpy(w.angles).to(dev)
+    prj = fan.distance_driven_fp(p, torch.from_numpy(img).to(dev), ang_d)
+    fprj = fan.ramp_filter(p, prj, 'hann')
+    _ = fan.fbp_bp(p, fprj, ang_d)
+    torch.cuda.synchronize()
+    _lib.set_kernel_timing(True)
+    _lib.reset_kernel_times()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fprj = fan.ramp_filter(p, prj, 'hann')
+        rec = fan.fbp_bp(p, fprj, ang_d)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    times = _lib.kernel_times()
+    _lib.set_kernel_timing(False)
+    nvu = w.nx * w.ny * w.nz * w.nview
+    nsamp = w.nu * w.nv * w.nview
+    rf_ms, _ = kernel_stats(times, 'ramp_filter')
+    bp_ms, _ = kernel_stats(times, 'fan_fbp_bp')
+    return {
+        'workload': 'fan ramp_filter (Hann) + fbp_bp, same volume/scan',
+        'ms_per_step': round((t1 - t0) / steps * 1e3, 3),
+        'kernel_ms': {'ramp_filter': round(rf_ms, 3), 'fan_fbp_bp': round(bp_ms, 3)},
+        'ramp_filter_hbm_frac': round(8 * nsamp / (rf_ms / 1e3) / 1e9 / PEAK_HBM_GBS, 4) if rf_ms else None,
+        'fbp_bp_gvu_s': round(nvu / (bp_ms / 1e3) / 1e9, 2) if bp_ms else None,
+    }
+
+
+def cpu_baseline(w, img, target_s=12.0):
+    """FP32 OpenMP mirror of the reference algorithm on a view subset."""
+    from oracle import oracle as O
+
+    O.build()
+    threads = O.num_threads()
+    nvox = img.shape[0] * w.nx * w.ny * w.nz
+
+    def run(nv):
+        ang = w.angles[:nv]
+        t = time.perf_counter()
+        prj = O.dd_fan_fp(w, img, ang, prec='f')
+        t1 = time.perf_counter()
+        O.dd_fan_bp(w, prj, ang, prec='f')
+        t2 = time.perf_counter()
+        return t1 - t, t2 - t1
+
+    probe = 4
+    a, b = run(probe)
+    nv = int(min(w.nview, max(probe, probe * target_s / max(a + b, 1e-3))))
+    log('[bench] cpu baseline: %d of %d views on %d threads' % (nv, w.nview, threads))
+    a, b = run(nv)
+    gvu = 2 * nvox * nv / (a + b) / 1e9
+    return {'value': round(gvu, 4), 'unit': 'GVU/s', 'cores': threads, 'kind': 'port',
+            'sample': 'fan DD FP+BP of the same volume over the first %d of %d views (FP %.2fs, BP %.2fs); '
+                      'FP32 OpenMP restatement of the reference kernels (oracle/oracle.cpp)' % (nv, w.nview, a, b)}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=10)
+    ap.add_argument('--warmup', type=int, default=2)
+    ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--no-extra', action='store_true')
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print('bench.py measures one GPU only: the projector path runs on a single MI355X here and multi-GPU '
+              'sharding (views for FP, z-slabs for BP) is not part of this build; run with --gpus 1.',
+              file=sys.stderr)
+        return 2
+    import torch
+    if not torch.cuda.is_available():
+        print('bench.py needs a HIP device', file=sys.stderr)
+        return 1
+
+    r = fan_workload_bench(args.steps, args.warmup)
+    w, nvu, times = r['w'], r['nvu'], r['times']
+    fp_ms, fp_n = kernel_stats(times, 'dd_fan_fp')
+    bp_ms, bp_n = kernel_stats(times, 'dd_fan_bp')
+    dom, dom_ms = ('dd_fan_fp', fp_ms) if fp_ms >= bp_ms else ('dd_fan_bp', bp_ms)
+    achieved = 4.0 * nvu / (dom_ms / 1e3) / 1e9 if dom_ms > 0 else 0.0
+    value = 2 * nvu / (r['ms_step'] / 1e3) / 1e9
+    line = {
+        'metric': 'FP+BP Gray-voxel updates/s & % HBM roofline; served recons/s + p50/p99 latency',
+        'value': round(value, 2),
+        'unit': 'GVU/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': round(r['ms_step'], 3),
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f32',
+        'data': 'synthetic (seeded ellipsoid phantom, rng 1002)',
+        'config': {'workload': 'fan-beam equiangular distance-driven FP+BP, volume 1x32x512x512 @0.7 mm, '
+                               '984 views/360 deg, 888x32 detector, dso 595, dsd 1085.6',
+                   'batch': 1, 'nx': 512, 'ny': 512, 'nz': 32, 'nview': 984, 'nu': 888, 'nv': 32},
+        'roofline': {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': PEAK_HBM_GBS, 'unit': 'GB/s',
+                     'frac': round(achieved / PEAK_HBM_GBS, 4), 'traffic': None, 'kernel': dom,
+                     'kernel_ms': round(dom_ms, 3),
+                     'kernels_ms': {k: round(v[0] / max(v[1], 1), 3) for k, v in times.items()}},
+        'finite': r['finite'],
+    }
+    if not args.no_extra:
+        extra = {}
+        try:
+            extra['cone'] = cone_extra(2)
+        except Exception as e:  # report, do not hide
+            extra['cone'] = {'error': repr(e)}
+        try:
+            extra['fbp'] = fbp_extra(3, w, r['img'])
+        except Exception as e:
+            extra['fbp'] = {'error': repr(e)}
+        line['extra'] = extra
+    if not args.no_cpu_baseline:
+        line['cpu_baseline'] = cpu_baseline(w, r['img'])
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == '__main__':
+    sys.exit(main())

@@ -27,6 +27,7 @@ SYMBOLS = [
     'cfbpFanFilter', 'cupyfbpFanFilter', 'cfbpParallelFilter', 'cupyfbpParallelFilter',
     'cfbpFanBackprojection', 'cupyfbpFanBackprojection',
     'ctpGetLastErrorMessage', 'ctpReleaseWorkspace',
+    'ctpSetKernelTiming', 'ctpResetKernelTimes', 'ctpGetKernelTime', 'ctpKernelTimeReport',
 ]
 
 
@@ -63,3 +64,24 @@ def last_error_message() -> str:
 
 def release_workspace() -> int:
     return module().ctpReleaseWorkspace()
+
+
+def set_kernel_timing(enable: bool) -> None:
+    """Bracket every kernel launch with HIP events on its stream (measurement)."""
+    module().ctpSetKernelTiming(ctypes.c_int(1 if enable else 0))
+
+
+def reset_kernel_times() -> None:
+    module().ctpResetKernelTimes()
+
+
+def kernel_times() -> dict:
+    """{kernel name: (total ms, launches)} accumulated since the last reset."""
+    n = module().ctpKernelTimeReport(None, ctypes.c_size_t(0))
+    buf = ctypes.create_string_buffer(n + 1)
+    module().ctpKernelTimeReport(buf, ctypes.c_size_t(n + 1))
+    out = {}
+    for line in buf.value.decode().splitlines():
+        name, ms, cnt = line.split()
+        out[name] = (float(ms), int(cnt))
+    return out

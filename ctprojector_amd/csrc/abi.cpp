@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstring>
 #include <iostream>
+#include <algorithm>
 #include <limits>
 #include <string>
 #include <vector>
@@ -32,6 +33,7 @@ int guarded(const char* name, F&& f) {
         f();
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) throw Error((int)e, std::string("kernel failure: ") + hipGetErrorString(e));
+        timing_collect();
         return 0;
     } catch (const Error& e) {
         std::string m = std::string(name) + "() failed: " + e.what();
@@ -112,6 +114,31 @@ CTP_EXPORT int SetDevice(int device) {
 }
 
 CTP_EXPORT const char* ctpGetLastErrorMessage(void) { return last_error(); }
+
+// Kernel timing: when enabled, every kernel launch of the library is bracketed
+// by HIP events on its stream and the elapsed times are accumulated per kernel
+// name (for measurement; adds two event records per launch).
+CTP_EXPORT int ctpSetKernelTiming(int enable) {
+    timing_enable(enable != 0);
+    return 0;
+}
+CTP_EXPORT int ctpResetKernelTimes(void) {
+    timing_reset();
+    return 0;
+}
+CTP_EXPORT int ctpGetKernelTime(const char* name, double* total_ms, long* count) {
+    return timing_get(name, total_ms, count) ? 0 : (int)hipErrorNotFound;
+}
+// Writes "name total_ms count" lines into buf (NUL-terminated, truncated to n).
+CTP_EXPORT int ctpKernelTimeReport(char* buf, size_t n) {
+    std::string r = timing_report();
+    if (buf && n) {
+        size_t k = std::min(n - 1, r.size());
+        std::memcpy(buf, r.data(), k);
+        buf[k] = 0;
+    }
+    return (int)r.size();
+}
 
 CTP_EXPORT int ctpReleaseWorkspace(void) {
     return guarded("ctpReleaseWorkspace", [&] { release_all_workspaces(); });

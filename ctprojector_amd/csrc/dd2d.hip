@@ -288,10 +288,12 @@ void dd2d_fp(float* prj, const float* img, const float* deg_host, int B, const G
     for (auto& v : views) { needY |= (v.flags & 1) != 0; needX |= (v.flags & 2) != 0; }
     dim3 tb(256);
     if (needY) {
+        TimedLaunch t("dd_transpose", stream);
         to_channel_last<<<dim3(ceil_div(g.nx, 32), ceil_div(cs, 32), g.ny), tb, 0, stream>>>(imgY, img, C, cs, g.nx, g.ny, false);
         CTP_LAUNCH_CHECK();
     }
     if (needX) {
+        TimedLaunch t("dd_transpose", stream);
         to_channel_last<<<dim3(ceil_div(g.nx, 32), ceil_div(cs, 32), g.ny), tb, 0, stream>>>(imgX, img, C, cs, g.nx, g.ny, true);
         CTP_LAUNCH_CHECK();
     }
@@ -299,6 +301,8 @@ void dd2d_fp(float* prj, const float* img, const float* deg_host, int B, const G
     // (cuda/distanceDriven2DBranchless.cu:321-360)
     const bool force = (type & DD_FORCE_FBP) != 0 && !branchless;
     dim3 grid(ceil_div(d.nu, 128), nview, nchunk), block(128);
+    {
+    TimedLaunch t(fan ? "dd_fan_fp" : "dd_par_fp", stream);
 #define CTP_FP_CASE(N)                                                                                          \
     case N:                                                                                                     \
         if (fan) dd_fp_kernel<N, true><<<grid, block, 0, stream>>>(prj, imgY, imgX, dviews, drows, dstart, cs, g, d, nview, dsd, dso, force); \
@@ -310,6 +314,7 @@ void dd2d_fp(float* prj, const float* img, const float* deg_host, int B, const G
     }
 #undef CTP_FP_CASE
     CTP_LAUNCH_CHECK();
+    }
     CTP_CHECK(hipStreamSynchronize(stream));   // host tables above are released on return
 }
 
@@ -359,6 +364,7 @@ __global__ __launch_bounds__(256) void dd_bp_prep_kernel(float* __restrict__ sin
 
 void slices_to_channel_last(float* sino, const float* prj, const SliceRow* rows_dev, int C, int cs, int nz,
                             const Det& d, int nview, hipStream_t stream) {
+    TimedLaunch t("fbp_bp_prep", stream);
     dd_bp_prep_kernel<<<dim3(ceil_div(d.nu, 32), ceil_div(cs, 32), nview), 256, 0, stream>>>(
         sino, prj, nullptr, rows_dev, C, cs, nz, d, nview, 1.f, 1.f, false, false);
     CTP_LAUNCH_CHECK();
@@ -492,9 +498,14 @@ void dd2d_bp(float* img, const float* prj, const float* deg_host, int B, const G
     SliceRow* dsrow = ws.take_n<SliceRow>(g.nz);
     CTP_CHECK(hipMemcpyAsync(dviews, views.data(), sizeof(DDView) * nview, hipMemcpyHostToDevice, stream));
     CTP_CHECK(hipMemcpyAsync(dsrow, srow.data(), sizeof(SliceRow) * g.nz, hipMemcpyHostToDevice, stream));
-    dd_bp_prep_kernel<<<dim3(ceil_div(d.nu, 32), ceil_div(cs, 32), nview), 256, 0, stream>>>(
-        sino, prj, dviews, dsrow, C, cs, g.nz, d, nview, g.dx, g.dy, fan, !fbp);
-    CTP_LAUNCH_CHECK();
+    {
+        TimedLaunch t("dd_bp_prep", stream);
+        dd_bp_prep_kernel<<<dim3(ceil_div(d.nu, 32), ceil_div(cs, 32), nview), 256, 0, stream>>>(
+            sino, prj, dviews, dsrow, C, cs, g.nz, d, nview, g.dx, g.dy, fan, !fbp);
+        CTP_LAUNCH_CHECK();
+    }
+    {
+    TimedLaunch tb(fan ? "dd_fan_bp" : "dd_par_bp", stream);
     dim3 grid(ceil_div(g.nx, 16), ceil_div(g.ny, 16), nchunk), block(256);
 #define CTP_BP_CASE(N)                                                                                         \
     case N:                                                                                                    \
@@ -507,6 +518,7 @@ void dd2d_bp(float* img, const float* prj, const float* deg_host, int B, const G
     }
 #undef CTP_BP_CASE
     CTP_LAUNCH_CHECK();
+    }
     CTP_CHECK(hipStreamSynchronize(stream));
 }
 

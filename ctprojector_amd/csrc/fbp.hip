@@ -172,6 +172,8 @@ void ramp_filter(float* out, const float* prj, int B, int nu, int nv, int nview,
     const long nrows = (long)B * nview * nv;
     // LDS: the kernel (2nu+3 floats) and RB input rows; stays within 64 KB
     const size_t sh4 = sizeof(float) * ((2 * nu - 1 + 4) + 4 * nu);
+    {
+    TimedLaunch t("ramp_filter", stream);
     if (sh4 <= 65536) {
         ramp_conv_kernel<4><<<(unsigned)ceil_div(nrows, 4), 256, sh4, stream>>>(out, prj, dh, dpw, nrows, nu, scale);
     } else {
@@ -179,6 +181,7 @@ void ramp_filter(float* out, const float* prj, int B, int nu, int nv, int nview,
         ramp_conv_kernel<1><<<(unsigned)nrows, 256, sh1, stream>>>(out, prj, dh, dpw, nrows, nu, scale);
     }
     CTP_LAUNCH_CHECK();
+    }
     CTP_CHECK(hipStreamSynchronize(stream));
 }
 
@@ -283,6 +286,8 @@ void fan_fbp_bp(float* img, const float* prj, const float* deg_host, int B, cons
     CTP_CHECK(hipMemcpyAsync(dsrow, srow.data(), sizeof(SliceRow) * g.nz, hipMemcpyHostToDevice, stream));
     slices_to_channel_last(q, prj, dsrow, C, cs, g.nz, d, nview, stream);
     dim3 grid(ceil_div(g.nx, 16), ceil_div(g.ny, 16), nchunk), block(256);
+    {
+    TimedLaunch t("fan_fbp_bp", stream);
 #define CTP_FBP_CASE(N) \
     case N: fan_fbp_bp_kernel<N><<<grid, block, 0, stream>>>(img, q, dviews, C, cs, g, d, nview, dso); break;
     switch (CH) {
@@ -291,6 +296,7 @@ void fan_fbp_bp(float* img, const float* prj, const float* deg_host, int B, cons
     }
 #undef CTP_FBP_CASE
     CTP_LAUNCH_CHECK();
+    }
     CTP_CHECK(hipStreamSynchronize(stream));
 }
 

@@ -5,10 +5,24 @@
 #include <memory>
 #include <mutex>
 #include <utility>
+#include <vector>
+#include <sstream>
 
 namespace ctp {
 
 namespace {
+struct Pending {
+    std::string name;
+    hipEvent_t a, b;
+};
+struct Stat {
+    double ms = 0;
+    long n = 0;
+};
+std::mutex g_t_mutex;
+bool g_t_on = false;
+std::vector<Pending> g_pending;
+std::map<std::string, Stat> g_stats;
 thread_local std::string g_last_error;
 std::mutex g_ws_mutex;
 std::map<std::pair<int, hipStream_t>, std::unique_ptr<Workspace>> g_ws;
@@ -100,6 +114,64 @@ DeviceBuffer::DeviceBuffer(size_t bytes) : n(bytes) {
 
 DeviceBuffer::~DeviceBuffer() {
     if (p) (void)hipFree(p);
+}
+
+void timing_enable(bool on) {
+    std::lock_guard<std::mutex> lk(g_t_mutex);
+    g_t_on = on;
+}
+bool timing_enabled() {
+    std::lock_guard<std::mutex> lk(g_t_mutex);
+    return g_t_on;
+}
+void timing_reset() {
+    std::lock_guard<std::mutex> lk(g_t_mutex);
+    g_stats.clear();
+}
+void timing_collect() {
+    std::lock_guard<std::mutex> lk(g_t_mutex);
+    for (auto& p : g_pending) {
+        float ms = 0;
+        if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            Stat& st = g_stats[p.name];
+            st.ms += ms;
+            st.n += 1;
+        }
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    g_pending.clear();
+    (void)hipGetLastError();
+}
+bool timing_get(const char* name, double* total_ms, long* count) {
+    std::lock_guard<std::mutex> lk(g_t_mutex);
+    auto it = g_stats.find(name);
+    if (it == g_stats.end()) return false;
+    if (total_ms) *total_ms = it->second.ms;
+    if (count) *count = it->second.n;
+    return true;
+}
+std::string timing_report() {
+    std::lock_guard<std::mutex> lk(g_t_mutex);
+    std::ostringstream o;
+    for (auto& kv : g_stats) o << kv.first << " " << kv.second.ms << " " << kv.second.n << "\n";
+    return o.str();
+}
+
+TimedLaunch::TimedLaunch(const char* name, hipStream_t s) : name_(name), s_(s) {
+    if (!timing_enabled()) return;
+    if (hipEventCreate(&a_) != hipSuccess || hipEventCreate(&b_) != hipSuccess) {
+        (void)hipGetLastError();
+        a_ = b_ = nullptr;
+        return;
+    }
+    (void)hipEventRecord(a_, s_);
+}
+TimedLaunch::~TimedLaunch() {
+    if (!a_) return;
+    (void)hipEventRecord(b_, s_);
+    std::lock_guard<std::mutex> lk(g_t_mutex);
+    g_pending.push_back(Pending{name_, a_, b_});
 }
 
 }  // namespace ctp

@@ -72,4 +72,24 @@ struct DeviceBuffer {
     template <typename T> T* as() { return static_cast<T*>(p); }
 };
 
+// Per-kernel timing with HIP events on the launching stream (off by default).
+// Launch sites wrap their kernel with TimedLaunch; the C ABI collects the
+// elapsed times after each call has synchronised.
+void timing_enable(bool on);
+bool timing_enabled();
+void timing_reset();
+void timing_collect();                       // after the stream(s) drained
+bool timing_get(const char* name, double* total_ms, long* count);
+std::string timing_report();
+
+class TimedLaunch {
+public:
+    TimedLaunch(const char* name, hipStream_t s);
+    ~TimedLaunch();
+private:
+    const char* name_;
+    hipStream_t s_;
+    hipEvent_t a_ = nullptr, b_ = nullptr;
+};
+
 }  // namespace ctp

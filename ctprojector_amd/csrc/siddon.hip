@@ -134,6 +134,7 @@ void siddon_cone_fp(float* prj, const float* img, const ConeGeomDev& geo, int B,
     const size_t nimg = (size_t)g.nx * g.ny * g.nz, nprj = (size_t)d.nu * d.nv * nview;
     dim3 block(256), grid(ceil_div(d.nu, 16), ceil_div(d.nv, 16), nview);
     for (int b = 0; b < B; b++) {
+        TimedLaunch t("siddon_cone_fp", stream);
         siddon_cone_fp_kernel<<<grid, block, 0, stream>>>(prj + b * nprj, img + b * nimg, geo.det_center, geo.det_u,
                                                           geo.det_v, geo.src, g, d, nview);
         CTP_LAUNCH_CHECK();
@@ -419,6 +420,7 @@ static void launch_bp(float* img, const float4* table, const ConeViewBP* views, 
                       hipStream_t stream) {
     constexpr int NZR = 4;
     dim3 block(256), grid(ceil_div(g.nx, 16), ceil_div(g.ny, 16), ceil_div(g.nz, NZR));
+    TimedLaunch t("siddon_cone_bp", stream);
     siddon_cone_bp_kernel<WU, WV, NZR><<<grid, block, 0, stream>>>(img, table, views, nview, g, d.nu, d.nv);
     CTP_LAUNCH_CHECK();
 }
@@ -442,9 +444,12 @@ void siddon_cone_bp(float* img, const float* prj, const ConeGeomDev& geo, const 
     float4* table = ws.take_n<float4>(nprj);
     CTP_CHECK(hipMemcpyAsync(dviews, plan.views.data(), sizeof(ConeViewBP) * nview, hipMemcpyHostToDevice, stream));
     for (int b = 0; b < B; b++) {
-        siddon_bp_table_kernel<<<ceil_div((long)nprj, 256), 256, 0, stream>>>(table, prj + b * nprj, geo.det_center,
-                                                                            geo.det_u, geo.det_v, geo.src, g, d, nview);
-        CTP_LAUNCH_CHECK();
+        {
+            TimedLaunch tt("siddon_bp_table", stream);
+            siddon_bp_table_kernel<<<ceil_div((long)nprj, 256), 256, 0, stream>>>(table, prj + b * nprj, geo.det_center,
+                                                                                geo.det_u, geo.det_v, geo.src, g, d, nview);
+            CTP_LAUNCH_CHECK();
+        }
         float* im = img + b * nimg;
 #define CTP_BP_CASE(A, Bv) \
     if (wu == A && wv == Bv) { launch_bp<A, Bv>(im, table, dviews, nview, g, d, stream); continue; }

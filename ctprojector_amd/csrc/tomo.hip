@@ -91,6 +91,7 @@ void tomo_fp(float* prj, const float* img, const float* det_center, const float*
     const size_t nimg = (size_t)g.nx * g.ny * g.nz, nprj = (size_t)d.nu * d.nv * nview;
     dim3 grid(ceil_div(d.nu, 32), ceil_div(d.nv, 8), nview), block(256);
     for (int b = 0; b < B; b++) {
+        TimedLaunch t("tomo_fp", stream);
         tomo_fp_kernel<<<grid, block, 0, stream>>>(prj + b * nprj, img + b * nimg, det_center, src, g, d, nview);
         CTP_LAUNCH_CHECK();
     }
@@ -154,6 +155,7 @@ void tomo_bp(float* img, const float* prj, const float* det_center, const float*
     for (int b = 0; b < B; b++) {
         tomo_preweight_kernel<<<ceil_div((long)nprj, 256), 256, 0, stream>>>(w, prj + b * nprj, det_center, src, g.dz, d, nview);
         CTP_LAUNCH_CHECK();
+        TimedLaunch t("tomo_bp", stream);
         tomo_bp_kernel<<<grid, block, 0, stream>>>(img + b * nimg, w, det_center, src, g, d, nview);
         CTP_LAUNCH_CHECK();
     }

@@ -28,6 +28,7 @@ SYMBOLS = [
     'cfbpFanBackprojection', 'cupyfbpFanBackprojection',
     'ctpGetLastErrorMessage', 'ctpReleaseWorkspace',
     'ctpSetKernelTiming', 'ctpResetKernelTimes', 'ctpGetKernelTime', 'ctpKernelTimeReport',
+    'ctpSetTuning', 'ctpGetTuning',
 ]
 
 
@@ -64,6 +65,16 @@ def last_error_message() -> str:
 
 def release_workspace() -> int:
     return module().ctpReleaseWorkspace()
+
+
+def set_tuning(key: str, value: int) -> None:
+    """Set a launch-shape knob (0 restores the built-in default)."""
+    if module().ctpSetTuning(key.encode(), ctypes.c_int(int(value))) != 0:
+        raise KeyError('unknown tuning key %r' % key)
+
+
+def get_tuning(key: str) -> int:
+    return module().ctpGetTuning(key.encode())
 
 
 def set_kernel_timing(enable: bool) -> None:

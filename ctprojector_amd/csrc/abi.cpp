@@ -140,6 +140,14 @@ CTP_EXPORT int ctpKernelTimeReport(char* buf, size_t n) {
     return (int)r.size();
 }
 
+// Launch-shape knobs (see runtime.h); returns hipErrorInvalidValue for an
+// unknown key.
+CTP_EXPORT int ctpSetTuning(const char* key, int value) {
+    if (!key || !tuning_set(key, value)) return (int)hipErrorInvalidValue;
+    return 0;
+}
+CTP_EXPORT int ctpGetTuning(const char* key) { return key ? tuning_get(key, 0) : 0; }
+
 CTP_EXPORT int ctpReleaseWorkspace(void) {
     return guarded("ctpReleaseWorkspace", [&] { release_all_workspaces(); });
 }

@@ -24,6 +24,11 @@ bool g_t_on = false;
 std::vector<Pending> g_pending;
 std::map<std::string, Stat> g_stats;
 thread_local std::string g_last_error;
+std::mutex g_tune_mutex;
+std::map<std::string, int> g_tune = {
+    {"cone_fp_tile_u", 0},   // detector columns per wavefront tile of the Siddon FP (8, 16, 32, 64)
+    {"cone_bp_zrun", 0},     // voxels along z per thread of the Siddon BP (1, 2, 4, 8)
+};
 std::mutex g_ws_mutex;
 std::map<std::pair<int, hipStream_t>, std::unique_ptr<Workspace>> g_ws;
 }  // namespace
@@ -114,6 +119,19 @@ DeviceBuffer::DeviceBuffer(size_t bytes) : n(bytes) {
 
 DeviceBuffer::~DeviceBuffer() {
     if (p) (void)hipFree(p);
+}
+
+int tuning_get(const char* key, int dflt) {
+    std::lock_guard<std::mutex> lk(g_tune_mutex);
+    auto it = g_tune.find(key);
+    return (it == g_tune.end() || it->second == 0) ? dflt : it->second;
+}
+bool tuning_set(const char* key, int value) {
+    std::lock_guard<std::mutex> lk(g_tune_mutex);
+    auto it = g_tune.find(key);
+    if (it == g_tune.end()) return false;
+    it->second = value;
+    return true;
 }
 
 void timing_enable(bool on) {

@@ -72,6 +72,12 @@ struct DeviceBuffer {
     template <typename T> T* as() { return static_cast<T*>(p); }
 };
 
+// Launch-shape knobs (tile shapes, run lengths) read by the kernels' host
+// code; set by the autotuner or by the caller through ctpSetTuning().  Unknown
+// keys are rejected.  Value 0 means "use the built-in default".
+int tuning_get(const char* key, int dflt);
+bool tuning_set(const char* key, int value);
+
 // Per-kernel timing with HIP events on the launching stream (off by default).
 // Launch sites wrap their kernel with TimedLaunch; the C ABI collects the
 // elapsed times after each call has synchronised.

@@ -34,13 +34,52 @@ namespace ctp {
 // ---------------------------------------------------------------------------
 __device__ inline float3 ld3(const float* p, int i) { return make_float3(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
 
+// Views whose central ray runs mostly along x read the volume transposed to
+// [z][x][y], so that the memory-contiguous axis is the one across neighbouring
+// detector columns (lanes) in every view; the traversal then runs on the
+// swapped grid (Siddon's segment lengths do not depend on the axis order).
+__global__ void siddon_swap_flags_kernel(int* __restrict__ flags, const float* __restrict__ pdc,
+                                         const float* __restrict__ psrc, int nview) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nview) return;
+    const float wx = pdc[3 * v] - psrc[3 * v], wy = pdc[3 * v + 1] - psrc[3 * v + 1];
+    flags[v] = fabsf(wx) > fabsf(wy) ? 1 : 0;
+}
+
+// [z][y][x] -> [z][x][y]
+__global__ __launch_bounds__(256) void transpose_xy_kernel(float* __restrict__ out, const float* __restrict__ in, int nx,
+                                                           int ny) {
+    __shared__ float tile[32][33];
+    const int x0 = blockIdx.x * 32, y0 = blockIdx.y * 32, z = blockIdx.z;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const size_t plane = (size_t)nx * ny * z;
+    for (int k = ty; k < 32; k += 8) {
+        const int y = y0 + k, x = x0 + tx;
+        tile[k][tx] = (x < nx && y < ny) ? in[plane + (size_t)y * nx + x] : 0.f;
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        const int x = x0 + k, y = y0 + tx;
+        if (x < nx && y < ny) out[plane + (size_t)x * ny + y] = tile[tx][k];
+    }
+}
+
+template <int TU, int TV>
 __global__ __launch_bounds__(256) void siddon_cone_fp_kernel(float* __restrict__ prj, const float* __restrict__ img,
+                                                             const float* __restrict__ imgT, const int* __restrict__ swapf,
                                                              const float* __restrict__ pdc, const float* __restrict__ pdu,
                                                              const float* __restrict__ pdv, const float* __restrict__ psrc,
                                                              Grid g, Det d, int nview) {
+    static_assert(TU * TV == 64, "one wavefront per tile");
+    // per-view layout choice and the volume's buffer resource, built before any
+    // divergent branch so that they stay in scalar registers (one view per block)
+    const int sw = __builtin_amdgcn_readfirstlane(swapf[blockIdx.z]);
+    const __amdgpu_buffer_rsrc_t vol = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(sw ? imgT : img), (short)0, 4 * g.nx * g.ny * g.nz, 0x00020000);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int iu = (blockIdx.x * 2 + (wave & 1)) * 8 + (lane & 7);
-    const int iv = (blockIdx.y * 2 + (wave >> 1)) * 8 + (lane >> 3);
+    // a block is 4 tiles stacked along v
+    const int iu = blockIdx.x * TU + (lane % TU);
+    const int iv = (blockIdx.y * 4 + wave) * TV + (lane / TU);
     const int view = blockIdx.z;
     if (iu >= d.nu || iv >= d.nv || view >= nview) return;
 
@@ -62,6 +101,13 @@ __global__ __launch_bounds__(256) void siddon_cone_fp_kernel(float* __restrict__
         const float t1 = (t.x - g.cx) * w.x + (t.y - g.cy) * w.y + (t.z - g.cz) * w.z;
         if (t0 > r) s = make_float3(s.x + (t0 - r) * w.x, s.y + (t0 - r) * w.y, s.z + (t0 - r) * w.z);
         if (t1 > r) t = make_float3(t.x - (t1 - r) * w.x, t.y - (t1 - r) * w.y, t.z - (t1 - r) * w.z);
+    }
+    if (sw) {
+        const float a = s.x; s.x = s.y; s.y = a;
+        const float b = t.x; t.x = t.y; t.y = b;
+        const int n = g.nx; g.nx = g.ny; g.ny = n;
+        const float e = g.dx; g.dx = g.dy; g.dy = e;
+        const float f = g.cx; g.cx = g.cy; g.cy = f;
     }
 
     // InitializeSiddon, cuda/siddon.cu:80-149
@@ -97,34 +143,52 @@ __global__ __launch_bounds__(256) void siddon_cone_fp_kernel(float* __restrict__
     const float dax = px ? fabsf(rdx) * dist : 0.f;
     const float day = py ? fabsf(rdy) * dist : 0.f;
     const float daz = pz ? fabsf(rdz) * dist : 0.f;
-    const int sx = dx > 0 ? 1 : -1, sy = dy > 0 ? 1 : -1, sz = dz > 0 ? 1 : -1;
     const int nxy = g.nx * g.ny;
-    const int mx = sx, my = sy * g.nx, mz = sz * nxy;
+    // byte offsets into the volume through a buffer resource: no 64-bit address
+    // arithmetic per step (the volume is below 2 GiB, checked on the host)
+    const int mx = dx > 0 ? 4 : -4, my = dy > 0 ? 4 * g.nx : -4 * g.nx, mz = dz > 0 ? 4 * nxy : -4 * nxy;
+    // crossings left along each axis before the index leaves the grid; the
+    // traversal ends with the crossing that takes a counter below zero
+    // (cuda/siddon.cu:169-228 stops when the index leaves the grid)
+    int rx = dx > 0 ? g.nx - 1 - ix : ix;
+    int ry = dy > 0 ? g.ny - 1 - iy : iy;
+    int rz = dz > 0 ? g.nz - 1 - iz : iz;
 
     float ax = ax0, ay = ay0, az = az0;
-    float cxn = 0.f, cyn = 0.f, czn = 0.f;
     float aprev = amin * dist;
-    int idx = (iz * g.ny + iy) * g.nx + ix;
+    int off = 4 * ((iz * g.ny + iy) * g.nx + ix);
     float val = 0.f;
-    while (true) {
-        const float f = img[idx];
-        const bool bx = (ax < ay) & (ax < az);
-        const bool by = !bx & (ay < az);
-        const bool bz = !bx & !by;
-        const float anow = bx ? ax : (by ? ay : az);
-        val = fmaf(f, anow - aprev, val);
-        aprev = anow;
-        idx += bx ? mx : (by ? my : mz);
-        ix += bx ? sx : 0;
-        iy += by ? sy : 0;
-        iz += bz ? sz : 0;
-        cxn += bx ? 1.f : 0.f;
-        cyn += by ? 1.f : 0.f;
-        czn += bz ? 1.f : 0.f;
-        ax = bx ? fmaf(cxn, dax, ax0) : ax;
-        ay = by ? fmaf(cyn, day, ay0) : ay;
-        az = bz ? fmaf(czn, daz, az0) : az;
-        if ((unsigned)ix >= (unsigned)g.nx || (unsigned)iy >= (unsigned)g.ny || (unsigned)iz >= (unsigned)g.nz) break;
+    bool inside = true;
+    // Software pipelined: the voxel sequence does not depend on the voxel
+    // values, so NP steps are traced first and their NP gathers are issued
+    // together.  Steps past the exit read voxel 0 with weight 0.
+    constexpr int NP = 4;
+    while (inside) {
+        int id[NP];
+        float len[NP];
+#pragma unroll
+        for (int k = 0; k < NP; k++) {
+            const float anow = fminf(fminf(ax, ay), az);
+            const bool bx = ax == anow;
+            const bool by = !bx && ay == anow;
+            const bool bz = !bx && !by;
+            id[k] = inside ? off : 0;
+            len[k] = inside ? anow - aprev : 0.f;
+            aprev = anow;
+            off += bx ? mx : (by ? my : mz);
+            rx -= bx ? 1 : 0;
+            ry -= by ? 1 : 0;
+            rz -= bz ? 1 : 0;
+            ax = bx ? ax + dax : ax;
+            ay = by ? ay + day : ay;
+            az = bz ? az + daz : az;
+            inside = inside && ((rx | ry | rz) >= 0);
+        }
+        float f[NP];
+#pragma unroll
+        for (int k = 0; k < NP; k++) f[k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(vol, id[k], 0, 0));
+#pragma unroll
+        for (int k = 0; k < NP; k++) val = fmaf(f[k], len[k], val);
     }
     prj[((size_t)view * d.nv + iv) * d.nu + iu] += val;
 }
@@ -132,11 +196,34 @@ __global__ __launch_bounds__(256) void siddon_cone_fp_kernel(float* __restrict__
 void siddon_cone_fp(float* prj, const float* img, const ConeGeomDev& geo, int B, const Grid& g, const Det& d,
                     int nview, hipStream_t stream) {
     const size_t nimg = (size_t)g.nx * g.ny * g.nz, nprj = (size_t)d.nu * d.nv * nview;
-    dim3 block(256), grid(ceil_div(d.nu, 16), ceil_div(d.nv, 16), nview);
+    if (nimg >= (1ull << 29)) fail("cone forward projection: volumes of 2^29 voxels or more are not supported");
+    Workspace& ws = workspace_for(stream);
+    Plan pl;
+    pl.add<float>(nimg);
+    pl.add<int>(nview);
+    ws.reserve(pl.bytes);
+    float* imgT = ws.take_n<float>(nimg);
+    int* swapf = ws.take_n<int>(nview);
+    siddon_swap_flags_kernel<<<ceil_div(nview, 256), 256, 0, stream>>>(swapf, geo.det_center, geo.src, nview);
+    CTP_LAUNCH_CHECK();
+    const int TU = tuning_get("cone_fp_tile_u", 32);
+    if (TU != 8 && TU != 16 && TU != 32 && TU != 64) fail("cone_fp_tile_u must be 8, 16, 32 or 64");
+    const int TV = 64 / TU;
+    dim3 block(256), grid(ceil_div(d.nu, TU), ceil_div(d.nv, 4 * TV), nview);
     for (int b = 0; b < B; b++) {
+        {
+            TimedLaunch t("siddon_transpose", stream);
+            transpose_xy_kernel<<<dim3(ceil_div(g.nx, 32), ceil_div(g.ny, 32), g.nz), 256, 0, stream>>>(
+                imgT, img + b * nimg, g.nx, g.ny);
+            CTP_LAUNCH_CHECK();
+        }
         TimedLaunch t("siddon_cone_fp", stream);
-        siddon_cone_fp_kernel<<<grid, block, 0, stream>>>(prj + b * nprj, img + b * nimg, geo.det_center, geo.det_u,
-                                                          geo.det_v, geo.src, g, d, nview);
+#define CTP_FP_T(A, Bv)                                                                                          \
+    if (TU == A) siddon_cone_fp_kernel<A, Bv><<<grid, block, 0, stream>>>(prj + b * nprj, img + b * nimg, imgT, swapf, \
+                                                                          geo.det_center, geo.det_u, geo.det_v,     \
+                                                                          geo.src, g, d, nview);
+        CTP_FP_T(8, 8) CTP_FP_T(16, 4) CTP_FP_T(32, 2) CTP_FP_T(64, 1)
+#undef CTP_FP_T
         CTP_LAUNCH_CHECK();
     }
 }
@@ -351,6 +438,67 @@ __global__ __launch_bounds__(256) void siddon_bp_table_kernel(float4* __restrict
     table[i] = make_float4(inv[0], inv[1], inv[2], hit ? (float)(p * L) : 0.f);
 }
 
+// Chord of one candidate ray through the voxel at (ex, ey, ez) + [-1/2, 1/2]^3
+// relative to the source (index units): intersection of the three slab
+// intervals t_k = (X_k - S_k) * inv_k, clipped to t >= 0 (the ray starts at the
+// source), times the ray weight (projection value x ray length).
+template <bool SLOW>
+__device__ inline float chord(float4 r, float ex, float ey, float ez, bool inx, bool iny, bool inz) {
+    const float mx = ex * r.x, my = ey * r.y, mz = ez * r.z;
+    float x0 = fmaf(fabsf(r.x), -0.5f, mx), x1 = fmaf(fabsf(r.x), 0.5f, mx);
+    float y0 = fmaf(fabsf(r.y), -0.5f, my), y1 = fmaf(fabsf(r.y), 0.5f, my);
+    float z0 = fmaf(fabsf(r.z), -0.5f, mz), z1 = fmaf(fabsf(r.z), 0.5f, mz);
+    if (SLOW) {
+        // axis-parallel component (stored as 0): inside the slab iff the voxel
+        // index equals the ray's constant index along that axis
+        const float big = 3.0e38f;
+        if (r.x == 0.f) { x0 = inx ? -big : big; x1 = inx ? big : -big; }
+        if (r.y == 0.f) { y0 = iny ? -big : big; y1 = iny ? big : -big; }
+        if (r.z == 0.f) { z0 = inz ? -big : big; z1 = inz ? big : -big; }
+    }
+    const float tin = fmaxf(fmaxf(fmaxf(x0, y0), z0), 0.f);
+    const float tout = fminf(fminf(x1, y1), z1);
+    return fmaxf(tout - tin, 0.f) * r.w;
+}
+
+template <int WU, int WV, int NZR, bool SLOW>
+__device__ inline void bp_view(float* acc, const ConeViewBP& cv, const float4* __restrict__ tab, int nu, int nv,
+                               int ix, int iy, int iz0, float xc, float yc) {
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc((void*)tab, (short)0, 16 * nu * nv, 0x00020000);
+    const float ex = xc - cv.sx, ey = yc - cv.sy;
+    const float bu = fmaf(cv.pu[0], xc, fmaf(cv.pu[1], yc, cv.pu[3]));
+    const float bv = fmaf(cv.pv[0], xc, fmaf(cv.pv[1], yc, cv.pv[3]));
+    const float bw = fmaf(cv.pw[0], xc, fmaf(cv.pw[1], yc, cv.pw[3]));
+    const bool inx = (float)ix == cv.fixx, iny = (float)iy == cv.fixy;
+#pragma unroll
+    for (int k = 0; k < NZR; k++) {
+        const float zc = (float)(iz0 + k) + 0.5f;
+        const float rd = __builtin_amdgcn_rcpf(fmaf(cv.pw[2], zc, bw));
+        const float uc = fmaf(cv.pu[2], zc, bu) * rd;
+        const float vc = fmaf(cv.pv[2], zc, bv) * rd;
+        const int u0 = min(max((int)ceilf(uc - cv.hu), 0), nu - WU);
+        const int v0 = min(max((int)ceilf(vc - cv.hv), 0), nv - WV);
+        const float ez = zc - cv.sz;
+        const bool inz = (float)(iz0 + k) == cv.fixz;
+        const int base = 16 * (v0 * nu + u0);
+        float4 r[WV][WU];
+#pragma unroll
+        for (int jv = 0; jv < WV; jv++)
+#pragma unroll
+            for (int ju = 0; ju < WU; ju++) {
+                const int o = base + 16 * (jv * nu + ju);
+                r[jv][ju] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rs, o, 0, 0));
+            }
+        float a = 0.f;
+#pragma unroll
+        for (int jv = 0; jv < WV; jv++)
+#pragma unroll
+            for (int ju = 0; ju < WU; ju++) a += chord<SLOW>(r[jv][ju], ex, ey, ez, inx, iny, inz);
+        acc[k] += a;
+    }
+}
+
 template <int WU, int WV, int NZR>
 __global__ __launch_bounds__(256) void siddon_cone_bp_kernel(float* __restrict__ img, const float4* __restrict__ table,
                                                              const ConeViewBP* __restrict__ views, int nview, Grid g,
@@ -366,47 +514,9 @@ __global__ __launch_bounds__(256) void siddon_cone_bp_kernel(float* __restrict__
     const size_t nuv = (size_t)nu * nv;
     for (int view = 0; view < nview; view++) {
         const ConeViewBP cv = views[view];
-        const float4* __restrict__ tab = table + view * nuv;
-        const float ex = xc - cv.sx, ey = yc - cv.sy;
-        const float bu = fmaf(cv.pu[0], xc, fmaf(cv.pu[1], yc, cv.pu[3]));
-        const float bv = fmaf(cv.pv[0], xc, fmaf(cv.pv[1], yc, cv.pv[3]));
-        const float bw = fmaf(cv.pw[0], xc, fmaf(cv.pw[1], yc, cv.pw[3]));
-        const bool slow = cv.flags & 1;
-#pragma unroll
-        for (int k = 0; k < NZR; k++) {
-            const float zc = (float)(iz0 + k) + 0.5f;
-            const float den = fmaf(cv.pw[2], zc, bw);
-            const float rd = __builtin_amdgcn_rcpf(den);
-            const float uc = fmaf(cv.pu[2], zc, bu) * rd;
-            const float vc = fmaf(cv.pv[2], zc, bv) * rd;
-            const int u0 = min(max((int)ceilf(uc - cv.hu), 0), nu - WU);
-            const int v0 = min(max((int)ceilf(vc - cv.hv), 0), nv - WV);
-            const float ez = zc - cv.sz;
-            float a = 0.f;
-#pragma unroll
-            for (int jv = 0; jv < WV; jv++) {
-#pragma unroll
-                for (int ju = 0; ju < WU; ju++) {
-                    const float4 r = tab[(size_t)(v0 + jv) * nu + (u0 + ju)];
-                    const float mx = ex * r.x, my = ey * r.y, mz = ez * r.z;
-                    float ix0 = fmaf(fabsf(r.x), -0.5f, mx), ix1 = fmaf(fabsf(r.x), 0.5f, mx);
-                    float iy0 = fmaf(fabsf(r.y), -0.5f, my), iy1 = fmaf(fabsf(r.y), 0.5f, my);
-                    float iz0f = fmaf(fabsf(r.z), -0.5f, mz), iz1f = fmaf(fabsf(r.z), 0.5f, mz);
-                    if (slow) {
-                        // axis-parallel component: inside the slab iff the voxel index
-                        // equals the ray's (constant) index along that axis
-                        const float big = 3.0e38f;
-                        if (r.x == 0.f) { const bool in = (float)ix == cv.fixx; ix0 = in ? -big : big; ix1 = in ? big : -big; }
-                        if (r.y == 0.f) { const bool in = (float)iy == cv.fixy; iy0 = in ? -big : big; iy1 = in ? big : -big; }
-                        if (r.z == 0.f) { const bool in = (float)(iz0 + k) == cv.fixz; iz0f = in ? -big : big; iz1f = in ? big : -big; }
-                    }
-                    const float tin = fmaxf(fmaxf(fmaxf(ix0, iy0), iz0f), 0.f);
-                    const float tout = fminf(fminf(ix1, iy1), iz1f);
-                    a = fmaf(fmaxf(tout - tin, 0.f), r.w, a);
-                }
-            }
-            acc[k] += a;
-        }
+        const float4* tab = table + view * nuv;
+        if (cv.flags & 1) bp_view<WU, WV, NZR, true>(acc, cv, tab, nu, nv, ix, iy, iz0, xc, yc);
+        else bp_view<WU, WV, NZR, false>(acc, cv, tab, nu, nv, ix, iy, iz0, xc, yc);
     }
 #pragma unroll
     for (int k = 0; k < NZR; k++) {
@@ -415,14 +525,26 @@ __global__ __launch_bounds__(256) void siddon_cone_bp_kernel(float* __restrict__
     }
 }
 
-template <int WU, int WV>
-static void launch_bp(float* img, const float4* table, const ConeViewBP* views, int nview, const Grid& g, const Det& d,
-                      hipStream_t stream) {
-    constexpr int NZR = 4;
+template <int WU, int WV, int NZR>
+static void launch_bp_n(float* img, const float4* table, const ConeViewBP* views, int nview, const Grid& g,
+                        const Det& d, hipStream_t stream) {
     dim3 block(256), grid(ceil_div(g.nx, 16), ceil_div(g.ny, 16), ceil_div(g.nz, NZR));
     TimedLaunch t("siddon_cone_bp", stream);
     siddon_cone_bp_kernel<WU, WV, NZR><<<grid, block, 0, stream>>>(img, table, views, nview, g, d.nu, d.nv);
     CTP_LAUNCH_CHECK();
+}
+
+template <int WU, int WV>
+static void launch_bp(float* img, const float4* table, const ConeViewBP* views, int nview, const Grid& g, const Det& d,
+                      hipStream_t stream) {
+    const int nzr = tuning_get("cone_bp_zrun", 4);
+    switch (nzr) {
+        case 1: launch_bp_n<WU, WV, 1>(img, table, views, nview, g, d, stream); break;
+        case 2: launch_bp_n<WU, WV, 2>(img, table, views, nview, g, d, stream); break;
+        case 4: launch_bp_n<WU, WV, 4>(img, table, views, nview, g, d, stream); break;
+        case 8: launch_bp_n<WU, WV, 8>(img, table, views, nview, g, d, stream); break;
+        default: fail("cone_bp_zrun must be 1, 2, 4 or 8");
+    }
 }
 
 void siddon_cone_bp(float* img, const float* prj, const ConeGeomDev& geo, const ConeGeomHost& hgeo, int B,

@@ -26,14 +26,20 @@ using namespace ctp;
 
 namespace {
 
+// set by every helper that touches the device during a call; calls that only
+// validate their arguments (or have nothing to do) never initialise HIP
+thread_local bool t_device_used = false;
+
 template <typename F>
 int guarded(const char* name, F&& f) {
-    (void)hipGetLastError();   // do not report errors left over from earlier work
+    t_device_used = false;
     try {
         f();
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) throw Error((int)e, std::string("kernel failure: ") + hipGetErrorString(e));
-        timing_collect();
+        if (t_device_used) {
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) throw Error((int)e, std::string("kernel failure: ") + hipGetErrorString(e));
+            timing_collect();
+        }
         return 0;
     } catch (const Error& e) {
         std::string m = std::string(name) + "() failed: " + e.what();
@@ -84,10 +90,20 @@ void need_ptr(const void* p, const char* what) {
     if (!p) fail(std::string("null pointer: ") + what);
 }
 
+// first device touch of a call: clear errors left over from earlier work
+void touch() {
+    if (!t_device_used) {
+        t_device_used = true;
+        (void)hipGetLastError();
+    }
+}
+
 void up(void* dst, const void* src, size_t bytes) {
+    touch();
     if (bytes) CTP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
 }
 void down(void* dst, const void* src, size_t bytes) {
+    touch();
     if (bytes) CTP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
 }
 std::vector<float> host_copy(const float* dev, size_t n) {
@@ -96,9 +112,13 @@ std::vector<float> host_copy(const float* dev, size_t n) {
     return h;
 }
 void zero(void* dev, size_t bytes) {
+    touch();
     if (bytes) CTP_CHECK(hipMemset(dev, 0, bytes));
 }
-void sync() { CTP_CHECK(hipStreamSynchronize(nullptr)); }
+void sync() {
+    touch();
+    CTP_CHECK(hipStreamSynchronize(nullptr));
+}
 
 }  // namespace
 
@@ -175,6 +195,7 @@ CTP_EXPORT int cupySiddonConeProjectionArbitrary(float* prj, const float* img, C
         cone_common(nBatches, nx, ny, nz, dx, dy, dz, cx, cy, cz, nu, nv, nview, du, dv, off_u, off_v, g, d);
         need_ptr(prj, "prj"); need_ptr(img, "img"); need_ptr(detCenter, "detCenter"); need_ptr(detU, "detU");
         need_ptr(detV, "detV"); need_ptr(src, "src");
+        touch();
         siddon_cone_fp(prj, img, ConeGeomDev{detCenter, detU, detV, src}, (int)nBatches, g, d, (int)nview, nullptr);
         sync();
     });
@@ -188,6 +209,7 @@ CTP_EXPORT int cSiddonConeProjectionArbitrary(float* prj, const float* img, CONE
         need_ptr(prj, "prj"); need_ptr(img, "img"); need_ptr(detCenter, "detCenter"); need_ptr(detU, "detU");
         need_ptr(detV, "detV"); need_ptr(src, "src");
         const size_t nimg = nBatches * nx * ny * nz, nprj = nBatches * nu * nv * nview, ng = 3 * nview;
+        touch();
         DeviceBuffer di(nimg * 4), dp(nprj * 4), dg(4 * ng * 4);
         float* geo = dg.as<float>();
         up(di.p, img, nimg * 4);
@@ -225,6 +247,7 @@ CTP_EXPORT int cSiddonConeBackprojectionArbitrary(float* img, const float* prj, 
         need_ptr(prj, "prj"); need_ptr(img, "img"); need_ptr(detCenter, "detCenter"); need_ptr(detU, "detU");
         need_ptr(detV, "detV"); need_ptr(src, "src");
         const size_t nimg = nBatches * nx * ny * nz, nprj = nBatches * nu * nv * nview, ng = 3 * nview;
+        touch();
         DeviceBuffer di(nimg * 4), dp(nprj * 4), dg(4 * ng * 4);
         float* geo = dg.as<float>();
         up(dp.p, prj, nprj * 4);
@@ -263,6 +286,7 @@ static int dd_entry(const char* name, float* out, const float* in, const float* 
             sync();
         } else {
             const size_t nin = FP ? nimg : nprj, nout = FP ? nprj : nimg;
+            touch();
             DeviceBuffer di(nin * 4), dout(nout * 4);
             up(di.p, in, nin * 4);
             zero(dout.p, nout * 4);
@@ -338,11 +362,13 @@ static int tomo_entry(const char* name, float* out, const float* in, const float
         need_ptr(out, "out"); need_ptr(in, "in"); need_ptr(detCenter, "detCenter"); need_ptr(src, "src");
         const size_t nimg = nBatches * nx * ny * nz, nprj = nBatches * nu * nv * nview, ng = 3 * nview;
         if (DEV) {
+            touch();
             if (FP) tomo_fp(out, in, detCenter, src, (int)nBatches, g, d, (int)nview, type, nullptr);
             else tomo_bp(out, in, detCenter, src, (int)nBatches, g, d, (int)nview, type, nullptr);
             sync();
         } else {
             const size_t nin = FP ? nimg : nprj, nout = FP ? nprj : nimg;
+            touch();
             DeviceBuffer di(nin * 4), dout(nout * 4), dg(2 * ng * 4);
             float* geo = dg.as<float>();
             up(di.p, in, nin * 4);
@@ -389,9 +415,11 @@ static int filter_entry(const char* name, float* out, const float* in, int nBatc
         need_ptr(out, "out"); need_ptr(in, "in");
         const size_t n = (size_t)nBatches * nu * nv * nview;
         if (DEV) {
+            touch();
             ramp_filter(out, in, nBatches, (int)nu, (int)nv, (int)nview, da, off_u, dsd, dso, filter, equispace, nullptr);
             sync();
         } else {
+            touch();
             DeviceBuffer di(n * 4), dout(n * 4);
             up(di.p, in, n * 4);
             ramp_filter(dout.as<float>(), di.as<float>(), nBatches, (int)nu, (int)nv, (int)nview, da, off_u, dsd, dso,
@@ -446,6 +474,7 @@ static int fbp_bp_entry(const char* name, float* img, const float* prj, const fl
             fan_fbp_bp(img, prj, hdeg.data(), (int)nBatches, g, d, (int)nview, dsd, dso, nullptr);
             sync();
         } else {
+            touch();
             DeviceBuffer di(nimg * 4), dp(nprj * 4);
             up(dp.p, prj, nprj * 4);
             zero(di.p, nimg * 4);

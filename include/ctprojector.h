@@ -1,0 +1,173 @@
+/*
+ * ctprojector.h - C ABI of libprojector.so (MI355X / gfx950 build).
+ *
+ * Drop-in replacement for the reference's libprojector.so of wudufan/CTProjector
+ * (src/ct_projector/kernel/projector/cuda/, abbreviated cuda/ below).  Every
+ * function keeps the reference's name, argument order and meaning; the
+ * reference file:line of the function it replaces is given above each
+ * declaration.
+ *
+ * Conventions (as in the reference):
+ *   images       float img[B][nz][ny][nx]
+ *   projections  float prj[B][nview][nv][nu]
+ *   geometry     float [nview][3] rows (x, y, z); x is the fastest image axis
+ *   angles       float [nview], radians
+ *   fan-beam calls take the angular pitch da = du / dsd in place of du
+ *   c*     functions take host pointers and stage through device memory;
+ *   cupy*  functions take device pointers (any HIP device memory, e.g. torch
+ *          tensors); where the reference accumulates (Siddon FP, every BP), the
+ *          caller passes a zero-initialised output.
+ * All calls run on the current HIP device, return when the results are
+ * complete, and return 0 on success or a non-zero hipError_t code.  A failure
+ * prints "<function>() failed: <reason>" to stderr (as the reference does) but
+ * never aborts the process; ctpGetLastErrorMessage() returns that text.
+ *
+ * typeProjector of the fan/parallel distance-driven projector is the
+ * reference's bit mask (cuda/distanceDriven2D.cu:12-25): bit0 FBP weighting,
+ * bit1 branchless, bit2 force FBP weighting.  For the tomosynthesis projector
+ * typeProjector 1 selects the branchless mode (cuda/distanceDriven3D.cu:27,77).
+ * Filter codes: 0 Ram-Lak, 1 Hamming, 2 Hann, 3 cosine (cuda/fbp.h:5-7).
+ */
+#ifndef CTPROJECTOR_H
+#define CTPROJECTOR_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* cuda/projector.cu:12 */
+int SetDevice(int device);
+
+/* ---- Siddon cone beam, arbitrary trajectory ------------------------------ */
+/* cuda/siddonCone.cu:137  prj = A img (host arrays) */
+int cSiddonConeProjectionArbitrary(float* prj, const float* img, const float* detCenter, const float* detU,
+                                   const float* detV, const float* src, size_t nBatches, size_t nx, size_t ny,
+                                   size_t nz, float dx, float dy, float dz, float cx, float cy, float cz, size_t nu,
+                                   size_t nv, size_t nview, float du, float dv, float off_u, float off_v);
+/* cuda/siddonCone.cu:233  prj += A img (device arrays) */
+int cupySiddonConeProjectionArbitrary(float* prj, const float* img, const float* detCenter, const float* detU,
+                                      const float* detV, const float* src, size_t nBatches, size_t nx, size_t ny,
+                                      size_t nz, float dx, float dy, float dz, float cx, float cy, float cz, size_t nu,
+                                      size_t nv, size_t nview, float du, float dv, float off_u, float off_v);
+/* cuda/siddonCone.cu:282  img = A^T prj (host arrays; exact transpose of the FP) */
+int cSiddonConeBackprojectionArbitrary(float* img, const float* prj, const float* detCenter, const float* detU,
+                                       const float* detV, const float* src, size_t nBatches, size_t nx, size_t ny,
+                                       size_t nz, float dx, float dy, float dz, float cx, float cy, float cz,
+                                       size_t nu, size_t nv, size_t nview, float du, float dv, float off_u,
+                                       float off_v);
+/* cuda/siddonCone.cu:378  img += A^T prj (device arrays) */
+int cupySiddonConeBackprojectionArbitrary(float* img, const float* prj, const float* detCenter, const float* detU,
+                                          const float* detV, const float* src, size_t nBatches, size_t nx, size_t ny,
+                                          size_t nz, float dx, float dy, float dz, float cx, float cy, float cz,
+                                          size_t nu, size_t nv, size_t nview, float du, float dv, float off_u,
+                                          float off_v);
+
+/* ---- distance driven, fan beam (equiangular) and parallel beam ----------- */
+/* cuda/distanceDriven2D.cu:109 */
+int cupyDistanceDrivenFanProjection(float* prj, const float* img, const float* deg, size_t nBatches, size_t nx,
+                                    size_t ny, size_t nz, float dx, float dy, float dz, float cx, float cy, float cz,
+                                    size_t nu, size_t nv, size_t nview, float da, float dv, float off_a, float off_v,
+                                    float dsd, float dso, int typeProjector);
+/* cuda/distanceDriven2D.cu:158 */
+int cupyDistanceDrivenParallelProjection(float* prj, const float* img, const float* deg, size_t nBatches, size_t nx,
+                                         size_t ny, size_t nz, float dx, float dy, float dz, float cx, float cy,
+                                         float cz, size_t nu, size_t nv, size_t nview, float da, float dv,
+                                         float off_a, float off_v, int typeProjector);
+/* cuda/distanceDriven2D.cu:205 */
+int cDistanceDrivenFanProjection(float* prj, const float* img, const float* deg, size_t nBatches, size_t nx,
+                                 size_t ny, size_t nz, float dx, float dy, float dz, float cx, float cy, float cz,
+                                 size_t nu, size_t nv, size_t nview, float da, float dv, float off_a, float off_v,
+                                 float dsd, float dso, int typeProjector);
+/* cuda/distanceDriven2D.cu:285 */
+int cDistanceDrivenParallelProjection(float* prj, const float* img, const float* deg, size_t nBatches, size_t nx,
+                                      size_t ny, size_t nz, float dx, float dy, float dz, float cx, float cy,
+                                      float cz, size_t nu, size_t nv, size_t nview, float da, float dv, float off_a,
+                                      float off_v, int typeProjector);
+/* cuda/distanceDriven2D.cu:384 */
+int cupyDistanceDrivenFanBackprojection(float* img, const float* prj, const float* deg, size_t nBatches, size_t nx,
+                                        size_t ny, size_t nz, float dx, float dy, float dz, float cx, float cy,
+                                        float cz, size_t nu, size_t nv, size_t nview, float da, float dv,
+                                        float off_a, float off_v, float dsd, float dso, int typeProjector);
+/* cuda/distanceDriven2D.cu:434 */
+int cupyDistanceDrivenParallelBackprojection(float* img, const float* prj, const float* deg, size_t nBatches,
+                                             size_t nx, size_t ny, size_t nz, float dx, float dy, float dz, float cx,
+                                             float cy, float cz, size_t nu, size_t nv, size_t nview, float da,
+                                             float dv, float off_a, float off_v, int typeProjector);
+/* cuda/distanceDriven2D.cu:482 */
+int cDistanceDrivenFanBackprojection(float* img, const float* prj, const float* deg, size_t nBatches, size_t nx,
+                                     size_t ny, size_t nz, float dx, float dy, float dz, float cx, float cy, float cz,
+                                     size_t nu, size_t nv, size_t nview, float da, float dv, float off_a,
+                                     float off_v, float dsd, float dso, int typeProjector);
+/* cuda/distanceDriven2D.cu:560 */
+int cDistanceDrivenParallelBackprojection(float* img, const float* prj, const float* deg, size_t nBatches,
+                                          size_t nx, size_t ny, size_t nz, float dx, float dy, float dz, float cx,
+                                          float cy, float cz, size_t nu, size_t nv, size_t nview, float da, float dv,
+                                          float off_a, float off_v, int typeProjector);
+
+/* ---- distance driven tomosynthesis --------------------------------------- */
+/* cuda/distanceDriven3D.cu:184 */
+int cDistanceDrivenTomoProjection(float* prj, const float* img, const float* detCenter, const float* src,
+                                  size_t nBatches, size_t nx, size_t ny, size_t nz, float dx, float dy, float dz,
+                                  float cx, float cy, float cz, size_t nu, size_t nv, size_t nview, float du,
+                                  float dv, float off_u, float off_v, int typeProjector);
+/* cuda/distanceDriven3D.cu:264 */
+int cupyDistanceDrivenTomoProjection(float* prj, const float* img, const float* detCenter, const float* src,
+                                     size_t nBatches, size_t nx, size_t ny, size_t nz, float dx, float dy, float dz,
+                                     float cx, float cy, float cz, size_t nu, size_t nv, size_t nview, float du,
+                                     float dv, float off_u, float off_v, int typeProjector);
+/* cuda/distanceDriven3D.cu:312 */
+int cDistanceDrivenTomoBackprojection(float* img, const float* prj, const float* detCenter, const float* src,
+                                      size_t nBatches, size_t nx, size_t ny, size_t nz, float dx, float dy, float dz,
+                                      float cx, float cy, float cz, size_t nu, size_t nv, size_t nview, float du,
+                                      float dv, float off_u, float off_v, int typeProjector);
+/* cuda/distanceDriven3D.cu:393 */
+int cupyDistanceDrivenTomoBackprojection(float* img, const float* prj, const float* detCenter, const float* src,
+                                         size_t nBatches, size_t nx, size_t ny, size_t nz, float dx, float dy,
+                                         float dz, float cx, float cy, float cz, size_t nu, size_t nv, size_t nview,
+                                         float du, float dv, float off_u, float off_v, int typeProjector);
+
+/* ---- ramp filters and fan FBP backprojection ----------------------------- */
+/* cuda/fbp.cu:165 */
+int cupyfbpFanFilter(float* pFPrj, const float* pPrj, int nBatches, size_t nu, size_t nv, size_t nview, float da,
+                     float dv, float off_a, float off_v, float dsd, float dso, int typeFilter);
+/* cuda/fbp.cu:204 */
+int cfbpFanFilter(float* pFPrj, const float* pPrj, int nBatches, size_t nu, size_t nv, size_t nview, float da,
+                  float dv, float off_a, float off_v, float dsd, float dso, int typeFilter);
+/* cuda/fbpParallel.cu:111 */
+int cupyfbpParallelFilter(float* pFPrj, const float* pPrj, int nBatches, size_t nu, size_t nv, size_t nview, float du,
+                          float dv, float off_u, float off_v, int typeFilter);
+/* cuda/fbpParallel.cu:146 */
+int cfbpParallelFilter(float* pFPrj, const float* pPrj, int nBatches, size_t nu, size_t nv, size_t nview, float du,
+                       float dv, float off_u, float off_v, int typeFilter);
+/* cuda/fbp.cu:362 */
+int cupyfbpFanBackprojection(float* pImg, const float* pPrj, const float* pDeg, size_t nBatches, size_t nx, size_t ny,
+                             size_t nz, float dx, float dy, float dz, float cx, float cy, float cz, size_t nu,
+                             size_t nv, size_t nview, float da, float dv, float off_a, float off_v, float dsd,
+                             float dso);
+/* cuda/fbp.cu:409 */
+int cfbpFanBackprojection(float* pImg, const float* pPrj, const float* pDeg, size_t nBatches, size_t nx, size_t ny,
+                          size_t nz, float dx, float dy, float dz, float cx, float cy, float cz, size_t nu, size_t nv,
+                          size_t nview, float da, float dv, float off_a, float off_v, float dsd, float dso);
+
+/* ---- library services (no counterpart in the reference) ------------------ */
+/* Text of the calling thread's last failure ("" if none). */
+const char* ctpGetLastErrorMessage(void);
+/* Frees the device scratch memory the library keeps per (device, stream). */
+int ctpReleaseWorkspace(void);
+/* Launch-shape knobs used by the autotuner ("cone_fp_tile_u", "cone_bp_zrun");
+ * 0 restores the built-in default; unknown keys return hipErrorInvalidValue. */
+int ctpSetTuning(const char* key, int value);
+int ctpGetTuning(const char* key);
+/* Per-kernel timing with HIP events on the launching stream (measurement). */
+int ctpSetKernelTiming(int enable);
+int ctpResetKernelTimes(void);
+int ctpGetKernelTime(const char* name, double* total_ms, long* count);
+int ctpKernelTimeReport(char* buf, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CTPROJECTOR_H */

@@ -2,6 +2,8 @@
 """Projector hot-path benchmark on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-extra]
+                    [--calibrate-copy]
+    python bench.py --summarize TRACE_DIR [--fetch DIR --write DIR [--hits DIR]]
 
 One step = one forward projection plus one backprojection of the
 fan-beam distance-driven workload: volume 1x32x512x512 at 0.7 mm (1 mm slices),
@@ -23,6 +25,16 @@ The JSON line reports
   extra        the cone-beam Siddon FP+BP (256^3 volume at 0.5 mm, 720 views,
                512x384 panel) and the fan ramp filter + FBP backprojection,
                measured the same way.
+  roofline.traffic  HBM bytes per launch of the dominant kernel, read from the
+               committed rocprofv3 counter summary (profiles/) when it lists
+               that kernel, else null.
+
+--calibrate-copy runs three 256 MiB device-to-device copies first, so that a
+rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE pass over the same command can be
+scaled by what the counters read for a copy of known size.  --summarize turns
+rocprofv3 output directories (kernel trace with --stats, and optionally the
+FETCH_SIZE, WRITE_SIZE and TCC hit/miss passes) into the markdown tables kept
+under profiles/.
 """
 
 import argparse
@@ -37,6 +49,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
+COPY_BYTES = 256 << 20
+TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'hbm_traffic_fan512x32_cone256.md')
 
 
 def log(*a):
@@ -199,6 +213,94 @@ def cpu_baseline(w, img, target_s=12.0):
                       'FP32 OpenMP restatement of the reference kernels (oracle/oracle.cpp)' % (nv, w.nview, a, b)}
 
 
+def calibrate_copy():
+    """Three plain device copies of COPY_BYTES (the counter calibration)."""
+    import torch
+    a = torch.ones(COPY_BYTES // 4, device='cuda')
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log('[bench] calibration copies: 3 x %d bytes' % COPY_BYTES)
+    del a, b
+
+
+def short_name(kname):
+    n = kname.split('(')[0].replace('void ', '').replace('ctp::', '')
+    return n.split('<')[0]
+
+
+# timing names (HIP-event scopes in the library) -> kernel symbols in rocprofv3 output
+TIMED_KERNELS = {'dd_fan_fp': 'dd_fp_kernel', 'dd_par_fp': 'dd_fp_kernel', 'dd_fan_bp': 'dd_bp_kernel',
+                 'dd_par_bp': 'dd_bp_kernel', 'siddon_cone_fp': 'siddon_cone_fp_kernel',
+                 'fan_fbp_bp': 'fan_fbp_bp_kernel', 'ramp_filter': 'ramp_conv_kernel'}
+
+
+def read_traffic(name):
+    """HBM bytes per launch (FETCH + WRITE, calibrated) of the kernel timed as
+    ``name``, from the committed counter summary, or None."""
+    kernel = TIMED_KERNELS.get(name, name)
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                cells = [c.strip() for c in line.strip().strip('|').split('|')]
+                if len(cells) >= 4 and cells[0] == kernel:
+                    return int(float(cells[3]) * (1 << 20))
+    except OSError:
+        pass
+    return None
+
+
+def summarize(trace_dir, fetch_dir=None, write_dir=None, hits_dir=None):
+    """Markdown tables from rocprofv3 result databases (run_results.db)."""
+    import sqlite3
+
+    def db(d):
+        return sqlite3.connect(os.path.join(d, 'run_results.db')).cursor()
+
+    out = []
+    cur = db(trace_dir)
+    rows = list(cur.execute('select name, total_calls, total_duration, average, percentage from top_kernels'))
+    out.append('| kernel | calls | total ms | average us | % of GPU time |')
+    out.append('|---|---:|---:|---:|---:|')
+    for name, calls, tot, avg, pct in rows:
+        # top_kernels durations are in microseconds
+        out.append('| %s | %d | %.3f | %.1f | %.2f |' % (short_name(name), calls, tot / 1e3, avg, pct))
+    if fetch_dir and write_dir:
+        def counters(d, cname):
+            res = {}
+            for k, v in db(d).execute('select kernel_name, sum(value) from counters_collection '
+                                      'where counter_name = ? group by dispatch_id', (cname,)):
+                res.setdefault(short_name(k), []).append(v)
+            return res
+        fetch = counters(fetch_dir, 'FETCH_SIZE')
+        write = counters(write_dir, 'WRITE_SIZE')
+        cal_f = max(fetch.get('__amd_rocclr_copyBuffer', [0]))
+        cal_w = max(write.get('__amd_rocclr_copyBuffer', [0]))
+        kib = COPY_BYTES / 1024.0
+        sf = kib / cal_f if cal_f else 1.0
+        sw = kib / cal_w if cal_w else 1.0
+        hits = counters(hits_dir, 'TCC_HIT_sum') if hits_dir else {}
+        miss = counters(hits_dir, 'TCC_MISS_sum') if hits_dir else {}
+        out.append('')
+        out.append('Copy calibration: a %d MiB device copy reads FETCH_SIZE %.1f KiB and WRITE_SIZE %.1f KiB;'
+                   ' counters below are scaled by %.4f (fetch) and %.4f (write).'
+                   % (COPY_BYTES >> 20, cal_f, cal_w, sf, sw))
+        out.append('')
+        out.append('| kernel | fetch MiB/launch | write MiB/launch | HBM MiB/launch | L2 hit % | launches |')
+        out.append('|---|---:|---:|---:|---:|---:|')
+        for k in sorted(fetch, key=lambda k: -max(fetch[k])):
+            if k == '__amd_rocclr_copyBuffer':
+                continue
+            f = sum(fetch[k]) / len(fetch[k]) * sf / 1024
+            w = (sum(write.get(k, [0])) / max(len(write.get(k, [])), 1)) * sw / 1024
+            h, m = sum(hits.get(k, [0])), sum(miss.get(k, [0]))
+            hr = '%.1f' % (100.0 * h / (h + m)) if h + m else '-'
+            out.append('| %s | %.1f | %.1f | %.1f | %s | %d |' % (k, f, w, f + w, hr, len(fetch[k])))
+    print('\n'.join(out))
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument('--gpus', type=int, default=1)
@@ -206,7 +308,14 @@ def main(argv=None):
     ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-extra', action='store_true')
+    ap.add_argument('--calibrate-copy', action='store_true')
+    ap.add_argument('--summarize', metavar='TRACE_DIR')
+    ap.add_argument('--fetch', metavar='DIR')
+    ap.add_argument('--write', metavar='DIR')
+    ap.add_argument('--hits', metavar='DIR')
     args = ap.parse_args(argv)
+    if args.summarize:
+        return summarize(args.summarize, args.fetch, args.write, args.hits)
     if args.gpus != 1:
         print('bench.py measures one GPU only: the projector path runs on a single MI355X here and multi-GPU '
               'sharding (views for FP, z-slabs for BP) is not part of this build; run with --gpus 1.',
@@ -217,6 +326,8 @@ def main(argv=None):
         print('bench.py needs a HIP device', file=sys.stderr)
         return 1
 
+    if args.calibrate_copy:
+        calibrate_copy()
     r = fan_workload_bench(args.steps, args.warmup)
     w, nvu, times = r['w'], r['nvu'], r['times']
     fp_ms, fp_n = kernel_stats(times, 'dd_fan_fp')
@@ -241,7 +352,7 @@ def main(argv=None):
                                '984 views/360 deg, 888x32 detector, dso 595, dsd 1085.6',
                    'batch': 1, 'nx': 512, 'ny': 512, 'nz': 32, 'nview': 984, 'nu': 888, 'nv': 32},
         'roofline': {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': PEAK_HBM_GBS, 'unit': 'GB/s',
-                     'frac': round(achieved / PEAK_HBM_GBS, 4), 'traffic': None, 'kernel': dom,
+                     'frac': round(achieved / PEAK_HBM_GBS, 4), 'traffic': read_traffic(dom), 'kernel': dom,
                      'kernel_ms': round(dom_ms, 3),
                      'kernels_ms': {k: round(v[0] / max(v[1], 1), 3) for k, v in times.items()}},
         'finite': r['finite'],

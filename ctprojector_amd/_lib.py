@@ -39,10 +39,23 @@ class ProjectorLibraryError(RuntimeError):
 _module = None
 
 
+def _init_torch_runtime():
+    """PyTorch-ROCm bundles its own HIP runtime.  When both runtimes live in one
+    process, the one that initialises the device second may find no device if
+    it is torch's; so torch's runtime, when torch is installed, initialises
+    first.  Device pointers are valid across the two (one HSA address space)."""
+    try:
+        import torch
+    except ImportError:
+        return
+    torch.cuda.is_available()
+
+
 def module():
     """The loaded ctypes library (loaded on first use)."""
     global _module
     if _module is None:
+        _init_torch_runtime()
         if not os.path.exists(LIB_PATH):
             raise ProjectorLibraryError(
                 '%s is not built; run `python -m ctprojector_amd.build` (hipcc, gfx950)' % LIB_PATH)

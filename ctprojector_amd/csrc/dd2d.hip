@@ -400,12 +400,15 @@ __device__ inline void dd_cells(float* acc, const float* __restrict__ base, int 
 }
 
 template <int CH, bool FAN>
-__global__ __launch_bounds__(256) void dd_bp_kernel(float* __restrict__ img, const float* __restrict__ sino,
-                                                    const DDView* __restrict__ views, int C, int cs, Grid g, Det d,
-                                                    int nview, float dso, bool fbp) {
+__global__ __launch_bounds__(256) void dd_bp_kernel(float* __restrict__ img, float* __restrict__ part,
+                                                    const float* __restrict__ sino, const DDView* __restrict__ views,
+                                                    int C, int cs, int nchunk, int vpg, Grid g, Det d, int nview,
+                                                    float dso, bool fbp) {
     const int ix = blockIdx.x * 16 + (threadIdx.x & 15);
     const int iy = blockIdx.y * 16 + (threadIdx.x >> 4);
-    const int c0 = blockIdx.z * CH;
+    const int grp = blockIdx.z / nchunk;
+    const int c0 = (blockIdx.z - grp * nchunk) * CH;
+    const int vbeg = grp * vpg, vend = min(nview, vbeg + vpg);
     if (ix >= g.nx || iy >= g.ny) return;
     // pixel centre and edges, cuda/distanceDriven2DBoxInt.cu:340-349
     const float xc = ((float)ix - g.nx / 2.f) * g.dx + g.cx + g.dx / 2;
@@ -417,7 +420,7 @@ __global__ __launch_bounds__(256) void dd_bp_kernel(float* __restrict__ img, con
     for (int j = 0; j < CH; j++) acc[j] = 0.f;
     const float half = (float)d.nu / 2.0f;
     const size_t vstride = (size_t)d.nu * cs;
-    for (int view = 0; view < nview; view++) {
+    for (int view = vbeg; view < vend; view++) {
         const DDView V = views[view];
         const float c = V.c, s = V.s;
         const float* __restrict__ base = sino + view * vstride + c0;
@@ -450,12 +453,35 @@ __global__ __launch_bounds__(256) void dd_bp_kernel(float* __restrict__ img, con
             }
         }
     }
-    const size_t npix = (size_t)g.nx * g.ny;
-#pragma unroll
-    for (int j = 0; j < CH; j++) {
-        const int cc = c0 + j;
-        if (cc < C) img[(size_t)cc * npix + (size_t)iy * g.nx + ix] += acc[j];
+    store_bp_result<CH>(img, part, acc, grp, c0, C, (size_t)g.nx * g.ny, (size_t)iy * g.nx + ix);
+}
+
+// Number of view groups a pixel-driven backprojection is split into so that
+// the launch has enough waves in flight to hide the per-view load latency
+// (one lane per pixel alone gives 4 waves per SIMD at 512x512).  Partial
+// images of the groups are summed in a fixed order afterwards: deterministic.
+int bp_view_groups(size_t npix, int nchunk, int nview) {
+    int k = tuning_get("bp_view_groups", 0);
+    if (k <= 0) {
+        const size_t waves = (npix + 63) / 64 * (size_t)nchunk;
+        k = (int)std::min<size_t>(8, std::max<size_t>(1, 16384 / std::max<size_t>(waves, 1)));
     }
+    k = std::max(1, std::min(k, std::max(1, nview / 8)));
+    return k;
+}
+
+__global__ __launch_bounds__(256) void reduce_parts_kernel(float* __restrict__ img, const float* __restrict__ part,
+                                                           int groups, size_t n) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.f;
+    for (int k = 0; k < groups; k++) s += part[(size_t)k * n + i];
+    img[i] += s;
+}
+
+void reduce_parts(float* img, const float* part, int groups, size_t n, hipStream_t stream) {
+    reduce_parts_kernel<<<(unsigned)ceil_div((long)n, 256), 256, 0, stream>>>(img, part, groups, n);
+    CTP_LAUNCH_CHECK();
 }
 
 void dd2d_bp(float* img, const float* prj, const float* deg_host, int B, const Grid& g, const Det& d, int nview,
@@ -487,15 +513,20 @@ void dd2d_bp(float* img, const float* prj, const float* deg_host, int B, const G
         }
         srow[z] = r;
     }
+    const size_t npix = (size_t)g.nx * g.ny;
+    const int groups = bp_view_groups(npix, nchunk, nview);
+    const int vpg = ceil_div(nview, groups);
     Workspace& ws = workspace_for(stream);
     Plan pl;
     pl.add<float>((size_t)nview * d.nu * cs);
     pl.add<DDView>(nview);
     pl.add<SliceRow>(g.nz);
+    if (groups > 1) pl.add<float>((size_t)groups * C * npix);
     ws.reserve(pl.bytes);
     float* sino = ws.take_n<float>((size_t)nview * d.nu * cs);
     DDView* dviews = ws.take_n<DDView>(nview);
     SliceRow* dsrow = ws.take_n<SliceRow>(g.nz);
+    float* part = groups > 1 ? ws.take_n<float>((size_t)groups * C * npix) : nullptr;
     CTP_CHECK(hipMemcpyAsync(dviews, views.data(), sizeof(DDView) * nview, hipMemcpyHostToDevice, stream));
     CTP_CHECK(hipMemcpyAsync(dsrow, srow.data(), sizeof(SliceRow) * g.nz, hipMemcpyHostToDevice, stream));
     {
@@ -506,11 +537,13 @@ void dd2d_bp(float* img, const float* prj, const float* deg_host, int B, const G
     }
     {
     TimedLaunch tb(fan ? "dd_fan_bp" : "dd_par_bp", stream);
-    dim3 grid(ceil_div(g.nx, 16), ceil_div(g.ny, 16), nchunk), block(256);
+    dim3 grid(ceil_div(g.nx, 16), ceil_div(g.ny, 16), nchunk * groups), block(256);
 #define CTP_BP_CASE(N)                                                                                         \
     case N:                                                                                                    \
-        if (fan) dd_bp_kernel<N, true><<<grid, block, 0, stream>>>(img, sino, dviews, C, cs, g, d, nview, dso, fbp); \
-        else dd_bp_kernel<N, false><<<grid, block, 0, stream>>>(img, sino, dviews, C, cs, g, d, nview, dso, fbp); \
+        if (fan) dd_bp_kernel<N, true><<<grid, block, 0, stream>>>(img, part, sino, dviews, C, cs, nchunk, vpg, g, d, \
+                                                                   nview, dso, fbp);                                \
+        else dd_bp_kernel<N, false><<<grid, block, 0, stream>>>(img, part, sino, dviews, C, cs, nchunk, vpg, g, d,  \
+                                                                nview, dso, fbp);                                   \
         break;
     switch (CH) {
         CTP_BP_CASE(1) CTP_BP_CASE(2) CTP_BP_CASE(4) CTP_BP_CASE(8) CTP_BP_CASE(16) CTP_BP_CASE(32)
@@ -518,6 +551,7 @@ void dd2d_bp(float* img, const float* prj, const float* deg_host, int B, const G
     }
 #undef CTP_BP_CASE
     CTP_LAUNCH_CHECK();
+    if (part) reduce_parts(img, part, groups, (size_t)C * npix, stream);
     }
     CTP_CHECK(hipStreamSynchronize(stream));
 }

@@ -194,12 +194,15 @@ struct FBPView {
 };
 
 template <int CH>
-__global__ __launch_bounds__(256) void fan_fbp_bp_kernel(float* __restrict__ img, const float* __restrict__ q,
-                                                         const FBPView* __restrict__ views, int C, int cs, Grid g, Det d,
-                                                         int nview, float dso) {
+__global__ __launch_bounds__(256) void fan_fbp_bp_kernel(float* __restrict__ img, float* __restrict__ part,
+                                                         const float* __restrict__ q, const FBPView* __restrict__ views,
+                                                         int C, int cs, int nchunk, int vpg, Grid g, Det d, int nview,
+                                                         float dso) {
     const int ix = blockIdx.x * 16 + (threadIdx.x & 15);
     const int iy = blockIdx.y * 16 + (threadIdx.x >> 4);
-    const int c0 = blockIdx.z * CH;
+    const int grp = blockIdx.z / nchunk;
+    const int c0 = (blockIdx.z - grp * nchunk) * CH;
+    const int vbeg = grp * vpg, vend = min(nview, vbeg + vpg);
     if (ix >= g.nx || iy >= g.ny) return;
     // pixel centre, cuda/fbp.cu:288
     const float px = ((float)ix + 0.5f - g.nx / 2.f) * g.dx + g.cx;
@@ -209,7 +212,7 @@ __global__ __launch_bounds__(256) void fan_fbp_bp_kernel(float* __restrict__ img
     for (int j = 0; j < CH; j++) acc[j] = 0.f;
     const float half = (d.nu - 1.0f) / 2.0f;
     const size_t vstride = (size_t)d.nu * cs;
-    for (int view = 0; view < nview; view++) {
+    for (int view = vbeg; view < vend; view++) {
         const FBPView V = views[view];
         const float rx = px * V.c + py * V.s;
         const float ry = -px * V.s + py * V.c;
@@ -239,12 +242,7 @@ __global__ __launch_bounds__(256) void fan_fbp_bp_kernel(float* __restrict__ img
             for (int j = 0; j < CH; j++) acc[j] = fmaf(w0, p0[j], fmaf(w1, p1[j], acc[j]));
         }
     }
-    const size_t npix = (size_t)g.nx * g.ny;
-#pragma unroll
-    for (int j = 0; j < CH; j++) {
-        const int cc = c0 + j;
-        if (cc < C) img[(size_t)cc * npix + (size_t)iy * g.nx + ix] += acc[j];
-    }
+    store_bp_result<CH>(img, part, acc, grp, c0, C, (size_t)g.nx * g.ny, (size_t)iy * g.nx + ix);
 }
 
 void fan_fbp_bp(float* img, const float* prj, const float* deg_host, int B, const Grid& g, const Det& d, int nview,
@@ -273,29 +271,37 @@ void fan_fbp_bp(float* img, const float* prj, const float* deg_host, int B, cons
         }
         srow[z] = r;
     }
+    const size_t npix = (size_t)g.nx * g.ny;
+    const int groups = bp_view_groups(npix, nchunk, nview);
+    const int vpg = ceil_div(nview, groups);
     Workspace& ws = workspace_for(stream);
     Plan pl;
     pl.add<float>((size_t)nview * d.nu * cs);
     pl.add<FBPView>(nview);
     pl.add<SliceRow>(g.nz);
+    if (groups > 1) pl.add<float>((size_t)groups * C * npix);
     ws.reserve(pl.bytes);
     float* q = ws.take_n<float>((size_t)nview * d.nu * cs);
     FBPView* dviews = ws.take_n<FBPView>(nview);
     SliceRow* dsrow = ws.take_n<SliceRow>(g.nz);
+    float* part = groups > 1 ? ws.take_n<float>((size_t)groups * C * npix) : nullptr;
     CTP_CHECK(hipMemcpyAsync(dviews, views.data(), sizeof(FBPView) * nview, hipMemcpyHostToDevice, stream));
     CTP_CHECK(hipMemcpyAsync(dsrow, srow.data(), sizeof(SliceRow) * g.nz, hipMemcpyHostToDevice, stream));
     slices_to_channel_last(q, prj, dsrow, C, cs, g.nz, d, nview, stream);
-    dim3 grid(ceil_div(g.nx, 16), ceil_div(g.ny, 16), nchunk), block(256);
+    dim3 grid(ceil_div(g.nx, 16), ceil_div(g.ny, 16), nchunk * groups), block(256);
     {
     TimedLaunch t("fan_fbp_bp", stream);
-#define CTP_FBP_CASE(N) \
-    case N: fan_fbp_bp_kernel<N><<<grid, block, 0, stream>>>(img, q, dviews, C, cs, g, d, nview, dso); break;
+#define CTP_FBP_CASE(N)                                                                                        \
+    case N:                                                                                                    \
+        fan_fbp_bp_kernel<N><<<grid, block, 0, stream>>>(img, part, q, dviews, C, cs, nchunk, vpg, g, d, nview, dso); \
+        break;
     switch (CH) {
         CTP_FBP_CASE(1) CTP_FBP_CASE(2) CTP_FBP_CASE(4) CTP_FBP_CASE(8) CTP_FBP_CASE(16) CTP_FBP_CASE(32)
         default: fail("fbp bp: bad channel chunk");
     }
 #undef CTP_FBP_CASE
     CTP_LAUNCH_CHECK();
+    if (part) reduce_parts(img, part, groups, (size_t)C * npix, stream);
     }
     CTP_CHECK(hipStreamSynchronize(stream));
 }

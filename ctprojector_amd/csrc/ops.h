@@ -49,6 +49,26 @@ void dd2d_fp(float* prj, const float* img, const float* deg_host, int B, const G
 void dd2d_bp(float* img, const float* prj, const float* deg_host, int B, const Grid& g, const Det& d, int nview,
              float dsd, float dso, int type, bool fan, hipStream_t stream);
 
+// ---- pixel-driven backprojection helpers shared by dd2d.hip and fbp.hip ----
+// view groups for a launch over npix pixels x nchunk channel chunks
+int bp_view_groups(size_t npix, int nchunk, int nview);
+// img[i] += sum_k part[k * n + i], k = 0 .. groups-1 in order
+void reduce_parts(float* img, const float* part, int groups, size_t n, hipStream_t stream);
+
+// Stores a lane's CH channel accumulators: img += acc (one view group) or the
+// group's partial image (several groups, summed by reduce_parts).
+template <int CH>
+__device__ inline void store_bp_result(float* __restrict__ img, float* __restrict__ part, const float* acc, int grp,
+                                       int c0, int C, size_t npix, size_t pix) {
+#pragma unroll
+    for (int j = 0; j < CH; j++) {
+        const int cc = c0 + j;
+        if (cc >= C) continue;
+        if (part) part[((size_t)grp * C + cc) * npix + pix] = acc[j];
+        else img[(size_t)cc * npix + pix] += acc[j];
+    }
+}
+
 // ---- distance driven tomosynthesis (cuda/distanceDriven3D*.cu) ----
 // geometry rows on device; outputs are overwritten (the reference writes '=')
 void tomo_fp(float* prj, const float* img, const float* det_center, const float* src, int B, const Grid& g,

@@ -28,6 +28,8 @@ std::mutex g_tune_mutex;
 std::map<std::string, int> g_tune = {
     {"cone_fp_tile_u", 0},   // detector columns per wavefront tile of the Siddon FP (8, 16, 32, 64)
     {"cone_bp_zrun", 0},     // voxels along z per thread of the Siddon BP (1, 2, 4, 8)
+    {"cone_bp_mode", 0},     // Siddon BP: 0 column mode where the geometry allows it, 1 general
+    {"bp_view_groups", 0},   // view groups of the pixel-driven DD / FBP backprojections (1..8)
 };
 std::mutex g_ws_mutex;
 std::map<std::pair<int, hipStream_t>, std::unique_ptr<Workspace>> g_ws;

@@ -64,43 +64,33 @@ __global__ __launch_bounds__(256) void transpose_xy_kernel(float* __restrict__ o
     }
 }
 
-template <int TU, int TV>
-__global__ __launch_bounds__(256) void siddon_cone_fp_kernel(float* __restrict__ prj, const float* __restrict__ img,
-                                                             const float* __restrict__ imgT, const int* __restrict__ swapf,
-                                                             const float* __restrict__ pdc, const float* __restrict__ pdu,
-                                                             const float* __restrict__ pdv, const float* __restrict__ psrc,
-                                                             Grid g, Det d, int nview) {
-    static_assert(TU * TV == 64, "one wavefront per tile");
-    // per-view layout choice and the volume's buffer resource, built before any
-    // divergent branch so that they stay in scalar registers (one view per block)
-    const int sw = __builtin_amdgcn_readfirstlane(swapf[blockIdx.z]);
-    const __amdgpu_buffer_rsrc_t vol = __builtin_amdgcn_make_buffer_rsrc(
-        (void*)(sw ? imgT : img), (short)0, 4 * g.nx * g.ny * g.nz, 0x00020000);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    // a block is 4 tiles stacked along v
-    const int iu = blockIdx.x * TU + (lane % TU);
-    const int iv = (blockIdx.y * 4 + wave) * TV + (lane / TU);
-    const int view = blockIdx.z;
-    if (iu >= d.nu || iv >= d.nv || view >= nview) return;
+// Traversal state of one ray: the reference's InitializeSiddon
+// (cuda/siddon.cu:80-149) after MoveSourceDstNearGrid (cuda/siddon.cu:318-338),
+// restated with integer crossing counters for the exit test.
+struct SiddonRay {
+    float ax, ay, az;      // next crossing along each axis (mm from the moved source)
+    float dax, day, daz;   // spacing of the crossings along each axis
+    float a;               // position of the last crossing (mm)
+    int off;               // byte offset of the current voxel
+    int mx, my, mz;        // byte offset steps along x, y, z
+    int rx, ry, rz;        // crossings left along each axis before the index leaves the grid
+};
 
-    // detector cell centre, cuda/siddonCone.cu:27-31
-    const float u = ((float)iu - d.off_u - (float)(d.nu - 1) / 2.0f) * d.du;
-    const float v = ((float)iv - d.off_v - (float)(d.nv - 1) / 2.0f) * d.dv;
-    float3 s = ld3(psrc, view);
-    const float3 c = ld3(pdc, view), U = ld3(pdu, view), V = ld3(pdv, view);
-    float3 t = make_float3(c.x + U.x * u + V.x * v, c.y + U.y * u + V.y * v, c.z + U.z * u + V.z * v);
-
+// s: source, t: detector point (physical).  With sw the x and y axes of the
+// grid and of the points are exchanged after the move (volume read as
+// [z][x][y]).  Returns false when the ray misses the grid.
+__device__ inline bool siddon_setup(SiddonRay& r, float3 s, float3 t, Grid& g, bool sw) {
     // MoveSourceDstNearGrid, cuda/siddon.cu:318-338
     {
         const float ex = (float)g.nx * g.dx, ey = (float)g.ny * g.dy, ez = (float)g.nz * g.dz;
-        const float r = sqrtf(ex * ex + ey * ey + ez * ez);
+        const float rr = sqrtf(ex * ex + ey * ey + ez * ez);
         float3 w = make_float3(t.x - s.x, t.y - s.y, t.z - s.z);
         float len = sqrtf(w.x * w.x + w.y * w.y + w.z * w.z);
         if (!(len < 1e-6f)) w = make_float3(w.x / len, w.y / len, w.z / len);
         const float t0 = (g.cx - s.x) * w.x + (g.cy - s.y) * w.y + (g.cz - s.z) * w.z;
         const float t1 = (t.x - g.cx) * w.x + (t.y - g.cy) * w.y + (t.z - g.cz) * w.z;
-        if (t0 > r) s = make_float3(s.x + (t0 - r) * w.x, s.y + (t0 - r) * w.y, s.z + (t0 - r) * w.z);
-        if (t1 > r) t = make_float3(t.x - (t1 - r) * w.x, t.y - (t1 - r) * w.y, t.z - (t1 - r) * w.z);
+        if (t0 > rr) s = make_float3(s.x + (t0 - rr) * w.x, s.y + (t0 - rr) * w.y, s.z + (t0 - rr) * w.z);
+        if (t1 > rr) t = make_float3(t.x - (t1 - rr) * w.x, t.y - (t1 - rr) * w.y, t.z - (t1 - rr) * w.z);
     }
     if (sw) {
         const float a = s.x; s.x = s.y; s.y = a;
@@ -120,7 +110,7 @@ __global__ __launch_bounds__(256) void siddon_cone_fp_kernel(float* __restrict__
     const bool px = fabsf(dx) > kEps, py = fabsf(dy) > kEps, pz = fabsf(dz) > kEps;
     if ((!px && (si.x < 0 || si.x > g.nx)) || (!py && (si.y < 0 || si.y > g.ny)) ||
         (!pz && (si.z < 0 || si.z > g.nz)))
-        return;
+        return false;
     const float rdx = 1.0f / dx, rdy = 1.0f / dy, rdz = 1.0f / dz;
     float a0x = 1.f, aNx = 0.f, a0y = 1.f, aNy = 0.f, a0z = 1.f, aNz = 0.f;
     if (px) { a0x = -si.x * rdx; aNx = ((float)g.nx - si.x) * rdx; }
@@ -128,35 +118,86 @@ __global__ __launch_bounds__(256) void siddon_cone_fp_kernel(float* __restrict__
     if (pz) { a0z = -si.z * rdz; aNz = ((float)g.nz - si.z) * rdz; }
     const float amin = fmaxf(0.f, fmaxf(fminf(a0x, aNx), fmaxf(fminf(a0y, aNy), fminf(a0z, aNz))));
     const float amax = fminf(1.f, fminf(fmaxf(a0x, aNx), fminf(fmaxf(a0y, aNy), fmaxf(a0z, aNz))));
-    if (amax <= amin) return;
+    if (amax <= amin) return false;
 
     int ix = (int)(si.x + amin * dx), iy = (int)(si.y + amin * dy), iz = (int)(si.z + amin * dz);
     if (px) ix = min(max(ix, 0), g.nx - 1);
     if (py) iy = min(max(iy, 0), g.ny - 1);
     if (pz) iz = min(max(iz, 0), g.nz - 1);
-    if (ix < 0 || ix >= g.nx || iy < 0 || iy >= g.ny || iz < 0 || iz >= g.nz) return;
+    if (ix < 0 || ix >= g.nx || iy < 0 || iy >= g.ny || iz < 0 || iz >= g.nz) return false;
 
     // first crossings and steps, scaled to mm
-    const float ax0 = px ? ((float)(dx > kEps ? ix + 1 : ix) - si.x) * rdx * dist : 2.f * dist;
-    const float ay0 = py ? ((float)(dy > kEps ? iy + 1 : iy) - si.y) * rdy * dist : 2.f * dist;
-    const float az0 = pz ? ((float)(dz > kEps ? iz + 1 : iz) - si.z) * rdz * dist : 2.f * dist;
-    const float dax = px ? fabsf(rdx) * dist : 0.f;
-    const float day = py ? fabsf(rdy) * dist : 0.f;
-    const float daz = pz ? fabsf(rdz) * dist : 0.f;
+    r.ax = px ? ((float)(dx > kEps ? ix + 1 : ix) - si.x) * rdx * dist : 2.f * dist;
+    r.ay = py ? ((float)(dy > kEps ? iy + 1 : iy) - si.y) * rdy * dist : 2.f * dist;
+    r.az = pz ? ((float)(dz > kEps ? iz + 1 : iz) - si.z) * rdz * dist : 2.f * dist;
+    r.dax = px ? fabsf(rdx) * dist : 0.f;
+    r.day = py ? fabsf(rdy) * dist : 0.f;
+    r.daz = pz ? fabsf(rdz) * dist : 0.f;
     const int nxy = g.nx * g.ny;
-    // byte offsets into the volume through a buffer resource: no 64-bit address
-    // arithmetic per step (the volume is below 2 GiB, checked on the host)
-    const int mx = dx > 0 ? 4 : -4, my = dy > 0 ? 4 * g.nx : -4 * g.nx, mz = dz > 0 ? 4 * nxy : -4 * nxy;
-    // crossings left along each axis before the index leaves the grid; the
-    // traversal ends with the crossing that takes a counter below zero
+    // byte offsets into the volume (below 2 GiB, checked on the host)
+    r.mx = dx > 0 ? 4 : -4;
+    r.my = dy > 0 ? 4 * g.nx : -4 * g.nx;
+    r.mz = dz > 0 ? 4 * nxy : -4 * nxy;
+    // the traversal ends with the crossing that takes a counter below zero
     // (cuda/siddon.cu:169-228 stops when the index leaves the grid)
-    int rx = dx > 0 ? g.nx - 1 - ix : ix;
-    int ry = dy > 0 ? g.ny - 1 - iy : iy;
-    int rz = dz > 0 ? g.nz - 1 - iz : iz;
+    r.rx = dx > 0 ? g.nx - 1 - ix : ix;
+    r.ry = dy > 0 ? g.ny - 1 - iy : iy;
+    r.rz = dz > 0 ? g.nz - 1 - iz : iz;
+    r.a = amin * dist;
+    r.off = 4 * ((iz * g.ny + iy) * g.nx + ix);
+    return true;
+}
 
-    float ax = ax0, ay = ay0, az = az0;
-    float aprev = amin * dist;
-    int off = 4 * ((iz * g.ny + iy) * g.nx + ix);
+// One step of the traversal: returns the segment length assigned to the voxel
+// at r.off before the step and advances to the next voxel.
+__device__ inline float siddon_step(SiddonRay& r, bool& inside) {
+    const float anow = fminf(fminf(r.ax, r.ay), r.az);
+    const bool bx = r.ax == anow;
+    const bool by = !bx && r.ay == anow;
+    const bool bz = !bx && !by;
+    const float len = anow - r.a;
+    r.a = anow;
+    r.off += bx ? r.mx : (by ? r.my : r.mz);
+    r.rx -= bx ? 1 : 0;
+    r.ry -= by ? 1 : 0;
+    r.rz -= bz ? 1 : 0;
+    r.ax = bx ? r.ax + r.dax : r.ax;
+    r.ay = by ? r.ay + r.day : r.ay;
+    r.az = bz ? r.az + r.daz : r.az;
+    inside = (r.rx | r.ry | r.rz) >= 0;
+    return len;
+}
+
+// Detector cell centre of ray (iu, iv) of a view, cuda/siddonCone.cu:27-31.
+__device__ inline float3 cell_centre(const float* pdc, const float* pdu, const float* pdv, int view, int iu, int iv,
+                                     const Det& d) {
+    const float u = ((float)iu - d.off_u - (float)(d.nu - 1) / 2.0f) * d.du;
+    const float v = ((float)iv - d.off_v - (float)(d.nv - 1) / 2.0f) * d.dv;
+    const float3 c = ld3(pdc, view), U = ld3(pdu, view), V = ld3(pdv, view);
+    return make_float3(c.x + U.x * u + V.x * v, c.y + U.y * u + V.y * v, c.z + U.z * u + V.z * v);
+}
+
+template <int TU, int TV>
+__global__ __launch_bounds__(256) void siddon_cone_fp_kernel(float* __restrict__ prj, const float* __restrict__ img,
+                                                             const float* __restrict__ imgT, const int* __restrict__ swapf,
+                                                             const float* __restrict__ pdc, const float* __restrict__ pdu,
+                                                             const float* __restrict__ pdv, const float* __restrict__ psrc,
+                                                             Grid g, Det d, int nview) {
+    static_assert(TU * TV == 64, "one wavefront per tile");
+    // per-view layout choice and the volume's buffer resource, built before any
+    // divergent branch so that they stay in scalar registers (one view per block)
+    const int sw = __builtin_amdgcn_readfirstlane(swapf[blockIdx.z]);
+    const __amdgpu_buffer_rsrc_t vol = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(sw ? imgT : img), (short)0, 4 * g.nx * g.ny * g.nz, 0x00020000);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // a block is 4 tiles stacked along v
+    const int iu = blockIdx.x * TU + (lane % TU);
+    const int iv = (blockIdx.y * 4 + wave) * TV + (lane / TU);
+    const int view = blockIdx.z;
+    if (iu >= d.nu || iv >= d.nv || view >= nview) return;
+
+    SiddonRay r;
+    if (!siddon_setup(r, ld3(psrc, view), cell_centre(pdc, pdu, pdv, view, iu, iv, d), g, sw != 0)) return;
     float val = 0.f;
     bool inside = true;
     // Software pipelined: the voxel sequence does not depend on the voxel
@@ -168,21 +209,11 @@ __global__ __launch_bounds__(256) void siddon_cone_fp_kernel(float* __restrict__
         float len[NP];
 #pragma unroll
         for (int k = 0; k < NP; k++) {
-            const float anow = fminf(fminf(ax, ay), az);
-            const bool bx = ax == anow;
-            const bool by = !bx && ay == anow;
-            const bool bz = !bx && !by;
-            id[k] = inside ? off : 0;
-            len[k] = inside ? anow - aprev : 0.f;
-            aprev = anow;
-            off += bx ? mx : (by ? my : mz);
-            rx -= bx ? 1 : 0;
-            ry -= by ? 1 : 0;
-            rz -= bz ? 1 : 0;
-            ax = bx ? ax + dax : ax;
-            ay = by ? ay + day : ay;
-            az = bz ? az + daz : az;
-            inside = inside && ((rx | ry | rz) >= 0);
+            id[k] = inside ? r.off : 0;
+            bool in_next;
+            const float l = siddon_step(r, in_next);
+            len[k] = inside ? l : 0.f;
+            inside = inside && in_next;
         }
         float f[NP];
 #pragma unroll
@@ -525,6 +556,229 @@ __global__ __launch_bounds__(256) void siddon_cone_bp_kernel(float* __restrict__
     }
 }
 
+// ---------------------------------------------------------------------------
+// Column mode: det_v parallel to z and det_u orthogonal to z in every view (the
+// circular / helical scanners of the reference's examples).  A ray's direction
+// then has x, y components that depend on its detector column only and a z
+// component that depends on its row only, so the x/y slab interval of a
+// candidate column is shared by every voxel of a z-run and every candidate
+// row.  Tables: per ray p*|q-S| (4 B), per view and column (1/w_x, 1/w_y), per
+// view and row 1/w_z.
+//
+// The reference treats a direction component as axis-parallel when its
+// index-space extent after MoveSourceDstNearGrid is at most 1e-6
+// (cuda/siddon.cu:15,98-115).  Here a column (row) component is parallel when
+// it is exactly zero; a ray for which the two rules disagree (0 < |d_k| <= 1e-6)
+// gets table weight 0 and is traced and scattered by the table kernel itself,
+// with the forward projector's arithmetic, so the result is the reference's
+// in every case.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void siddon_bp_coltable_kernel(float* __restrict__ wtab, float2* __restrict__ utab,
+                                                                 float* __restrict__ vtab, float* __restrict__ img,
+                                                                 const float* __restrict__ prj,
+                                                                 const float* __restrict__ pdc, const float* __restrict__ pdu,
+                                                                 const float* __restrict__ pdv, const float* __restrict__ psrc,
+                                                                 Grid g, Det d, int nview) {
+    const long n = (long)d.nu * d.nv * nview;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int iu = (int)(i % d.nu);
+    const int iv = (int)((i / d.nu) % d.nv);
+    const int view = (int)(i / ((long)d.nu * d.nv));
+    const double u = ((double)iu - d.off_u - (d.nu - 1) / 2.0) * d.du;
+    const double v = ((double)iv - d.off_v - (d.nv - 1) / 2.0) * d.dv;
+    double S[3], Q[3];
+    for (int k = 0; k < 3; k++) {
+        S[k] = psrc[3 * view + k];
+        Q[k] = pdc[3 * view + k] + pdu[3 * view + k] * u + pdv[3 * view + k] * v;
+    }
+    const double dd[3] = {g.dx, g.dy, g.dz};
+    const double cc[3] = {g.cx, g.cy, g.cz};
+    const int nn[3] = {g.nx, g.ny, g.nz};
+    double Sm[3] = {S[0], S[1], S[2]}, Qm[3] = {Q[0], Q[1], Q[2]};
+    {
+        double ex = g.nx * (double)g.dx, ey = g.ny * (double)g.dy, ez = g.nz * (double)g.dz;
+        double r = sqrt(ex * ex + ey * ey + ez * ez);
+        double w[3] = {Q[0] - S[0], Q[1] - S[1], Q[2] - S[2]};
+        double len = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+        if (!(len < 1e-6)) for (int k = 0; k < 3; k++) w[k] /= len;
+        double t0 = 0, t1 = 0;
+        for (int k = 0; k < 3; k++) { t0 += (cc[k] - S[k]) * w[k]; t1 += (Q[k] - cc[k]) * w[k]; }
+        if (t0 > r) for (int k = 0; k < 3; k++) Sm[k] = S[k] + (t0 - r) * w[k];
+        if (t1 > r) for (int k = 0; k < 3; k++) Qm[k] = Q[k] - (t1 - r) * w[k];
+    }
+    double L = 0, wi[3], di[3], smi[3];
+    bool rpar[3], cpar[3];
+    for (int k = 0; k < 3; k++) {
+        L += (Q[k] - S[k]) * (Q[k] - S[k]);
+        wi[k] = (Q[k] - S[k]) / dd[k];
+        di[k] = (Qm[k] - Sm[k]) / dd[k];
+        smi[k] = (Sm[k] - cc[k]) / dd[k] + nn[k] / 2.0;
+        rpar[k] = !(fabs(di[k]) > 1e-6);
+        cpar[k] = (Q[k] - S[k]) == 0.0;
+    }
+    L = sqrt(L);
+    double amin = 0, amax = 1;
+    bool hit = true;
+    for (int k = 0; k < 3; k++) {
+        if (!rpar[k]) {
+            double a0 = (0 - smi[k]) / di[k], a1 = (nn[k] - smi[k]) / di[k];
+            amin = fmax(amin, fmin(a0, a1));
+            amax = fmin(amax, fmax(a0, a1));
+        } else if (smi[k] < 0 || smi[k] > nn[k]) {
+            hit = false;
+        }
+    }
+    if (!(amax > amin)) hit = false;
+    for (int k = 0; k < 3; k++) {
+        if (rpar[k]) {
+            int fixed = (int)(smi[k] + amin * di[k]);
+            if (fixed < 0 || fixed >= nn[k]) hit = false;
+        }
+    }
+    if (iv == 0) utab[(long)view * d.nu + iu] = make_float2(cpar[0] ? 0.f : (float)(1.0 / wi[0]),
+                                                             cpar[1] ? 0.f : (float)(1.0 / wi[1]));
+    if (iu == 0) vtab[(long)view * d.nv + iv] = cpar[2] ? 0.f : (float)(1.0 / wi[2]);
+    const bool exception = rpar[0] != cpar[0] || rpar[1] != cpar[1] || rpar[2] != cpar[2];
+    const float p = prj[i];
+    wtab[i] = (hit && !exception) ? (float)(p * L) : 0.f;
+    if (exception && p != 0.f) {
+        SiddonRay r;
+        Grid gg = g;
+        if (siddon_setup(r, ld3(psrc, view), cell_centre(pdc, pdu, pdv, view, iu, iv, d), gg, false)) {
+            bool inside = true;
+            while (inside) {
+                const int off = r.off;
+                const float len = siddon_step(r, inside);
+                atomicAdd(img + (off >> 2), p * len);
+            }
+        }
+    }
+}
+
+template <int WU, int WV, int NZR, bool SLOW>
+__device__ inline void bp_view_col(float* acc, const ConeViewBP& cv, const float* __restrict__ wt,
+                                   const float2* __restrict__ ut, const float* __restrict__ vt, int nu, int nv, int ix,
+                                   int iy, int iz0, float xc, float yc) {
+    const __amdgpu_buffer_rsrc_t rw = __builtin_amdgcn_make_buffer_rsrc((void*)wt, (short)0, 4 * nu * nv, 0x00020000);
+    const __amdgpu_buffer_rsrc_t ru = __builtin_amdgcn_make_buffer_rsrc((void*)ut, (short)0, 8 * nu, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rv = __builtin_amdgcn_make_buffer_rsrc((void*)vt, (short)0, 4 * nv, 0x00020000);
+    const float ex = xc - cv.sx, ey = yc - cv.sy;
+    // pu[2] = pw[2] = 0 in column mode: the u coordinate and the denominator
+    // do not depend on z
+    const float rd = __builtin_amdgcn_rcpf(fmaf(cv.pw[0], xc, fmaf(cv.pw[1], yc, cv.pw[3])));
+    const float uc = fmaf(cv.pu[0], xc, fmaf(cv.pu[1], yc, cv.pu[3])) * rd;
+    const float bv = fmaf(cv.pv[0], xc, fmaf(cv.pv[1], yc, cv.pv[3]));
+    const int u0 = min(max((int)ceilf(uc - cv.hu), 0), nu - WU);
+    const bool inx = (float)ix == cv.fixx, iny = (float)iy == cv.fixy;
+    const float big = 3.0e38f;
+    float lo[WU], hi[WU];
+#pragma unroll
+    for (int j = 0; j < WU; j++) {
+        const float2 q = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(ru, 8 * (u0 + j), 0, 0));
+        const float mx = ex * q.x, my = ey * q.y;
+        float x0 = fmaf(fabsf(q.x), -0.5f, mx), x1 = fmaf(fabsf(q.x), 0.5f, mx);
+        float y0 = fmaf(fabsf(q.y), -0.5f, my), y1 = fmaf(fabsf(q.y), 0.5f, my);
+        if (SLOW) {
+            if (q.x == 0.f) { x0 = inx ? -big : big; x1 = inx ? big : -big; }
+            if (q.y == 0.f) { y0 = iny ? -big : big; y1 = iny ? big : -big; }
+        }
+        lo[j] = fmaxf(fmaxf(x0, y0), 0.f);
+        hi[j] = fminf(x1, y1);
+    }
+#pragma unroll
+    for (int k = 0; k < NZR; k++) {
+        const float zc = (float)(iz0 + k) + 0.5f;
+        const float vc = fmaf(cv.pv[2], zc, bv) * rd;
+        const int v0 = min(max((int)ceilf(vc - cv.hv), 0), nv - WV);
+        const float ez = zc - cv.sz;
+        const bool inz = (float)(iz0 + k) == cv.fixz;
+        float qz[WV], w[WV][WU];
+#pragma unroll
+        for (int jv = 0; jv < WV; jv++) {
+            qz[jv] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rv, 4 * (v0 + jv), 0, 0));
+#pragma unroll
+            for (int j = 0; j < WU; j++)
+                w[jv][j] = __uint_as_float(
+                    __builtin_amdgcn_raw_buffer_load_b32(rw, 4 * ((v0 + jv) * nu + u0 + j), 0, 0));
+        }
+        float a = 0.f;
+#pragma unroll
+        for (int jv = 0; jv < WV; jv++) {
+            const float mz = ez * qz[jv];
+            float z0 = fmaf(fabsf(qz[jv]), -0.5f, mz), z1 = fmaf(fabsf(qz[jv]), 0.5f, mz);
+            if (SLOW && qz[jv] == 0.f) { z0 = inz ? -big : big; z1 = inz ? big : -big; }
+#pragma unroll
+            for (int j = 0; j < WU; j++)
+                a = fmaf(fmaxf(fminf(hi[j], z1) - fmaxf(lo[j], z0), 0.f), w[jv][j], a);
+        }
+        acc[k] += a;
+    }
+}
+
+template <int WU, int WV, int NZR>
+__global__ __launch_bounds__(256) void siddon_cone_bp_col_kernel(float* __restrict__ img, const float* __restrict__ wtab,
+                                                                 const float2* __restrict__ utab,
+                                                                 const float* __restrict__ vtab,
+                                                                 const ConeViewBP* __restrict__ views, int nview,
+                                                                 Grid g, int nu, int nv) {
+    const int ix = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int iy = blockIdx.y * 16 + (threadIdx.x >> 4);
+    const int iz0 = blockIdx.z * NZR;
+    if (ix >= g.nx || iy >= g.ny) return;
+    const float xc = ix + 0.5f, yc = iy + 0.5f;
+    float acc[NZR];
+#pragma unroll
+    for (int k = 0; k < NZR; k++) acc[k] = 0.f;
+    const size_t nuv = (size_t)nu * nv;
+    for (int view = 0; view < nview; view++) {
+        const ConeViewBP cv = views[view];
+        const float* wt = wtab + view * nuv;
+        const float2* ut = utab + (size_t)view * nu;
+        const float* vt = vtab + (size_t)view * nv;
+        if (cv.flags & 1) bp_view_col<WU, WV, NZR, true>(acc, cv, wt, ut, vt, nu, nv, ix, iy, iz0, xc, yc);
+        else bp_view_col<WU, WV, NZR, false>(acc, cv, wt, ut, vt, nu, nv, ix, iy, iz0, xc, yc);
+    }
+#pragma unroll
+    for (int k = 0; k < NZR; k++) {
+        const int iz = iz0 + k;
+        if (iz < g.nz) img[((size_t)iz * g.ny + iy) * g.nx + ix] += acc[k];
+    }
+}
+
+template <int WU, int WV, int NZR>
+static void launch_bp_col_n(float* img, const float* wtab, const float2* utab, const float* vtab,
+                            const ConeViewBP* views, int nview, const Grid& g, const Det& d, hipStream_t stream) {
+    dim3 block(256), grid(ceil_div(g.nx, 16), ceil_div(g.ny, 16), ceil_div(g.nz, NZR));
+    TimedLaunch t("siddon_cone_bp", stream);
+    siddon_cone_bp_col_kernel<WU, WV, NZR><<<grid, block, 0, stream>>>(img, wtab, utab, vtab, views, nview, g, d.nu, d.nv);
+    CTP_LAUNCH_CHECK();
+}
+
+template <int WU, int WV>
+static void launch_bp_col(float* img, const float* wtab, const float2* utab, const float* vtab,
+                          const ConeViewBP* views, int nview, const Grid& g, const Det& d, hipStream_t stream) {
+    const int nzr = tuning_get("cone_bp_zrun", 4);
+    switch (nzr) {
+        case 1: launch_bp_col_n<WU, WV, 1>(img, wtab, utab, vtab, views, nview, g, d, stream); break;
+        case 2: launch_bp_col_n<WU, WV, 2>(img, wtab, utab, vtab, views, nview, g, d, stream); break;
+        case 4: launch_bp_col_n<WU, WV, 4>(img, wtab, utab, vtab, views, nview, g, d, stream); break;
+        case 8: launch_bp_col_n<WU, WV, 8>(img, wtab, utab, vtab, views, nview, g, d, stream); break;
+        default: fail("cone_bp_zrun must be 1, 2, 4 or 8");
+    }
+}
+
+// every view has det_v = (0, 0, v) and det_u = (ux, uy, 0) exactly, and the
+// FP32 projective rows have no z term in u and in the denominator
+static bool column_mode(const ConeGeomHost& h, const ConeBPPlan& plan, int nview) {
+    for (int i = 0; i < nview; i++) {
+        if (h.det_v[3 * i] != 0.f || h.det_v[3 * i + 1] != 0.f || h.det_u[3 * i + 2] != 0.f) return false;
+        const ConeViewBP& cv = plan.views[i];
+        if (cv.pu[2] != 0.f || cv.pw[2] != 0.f) return false;
+    }
+    return true;
+}
+
 template <int WU, int WV, int NZR>
 static void launch_bp_n(float* img, const float4* table, const ConeViewBP* views, int nview, const Grid& g,
                         const Det& d, hipStream_t stream) {
@@ -556,23 +810,57 @@ void siddon_cone_bp(float* img, const float* prj, const ConeGeomDev& geo, const 
              std::to_string(wv) + "); use finer voxels or coarser detector cells");
     wu = std::min(std::max(wu, 1), d.nu);
     wv = std::min(std::max(wv, 1), d.nv);
+    // cone_bp_mode: 0 = column mode where the geometry allows it, 1 = general
+    const bool col = tuning_get("cone_bp_mode", 0) == 0 && column_mode(hgeo, plan, nview);
     const size_t nimg = (size_t)g.nx * g.ny * g.nz, nprj = (size_t)d.nu * d.nv * nview;
     Workspace& ws = workspace_for(stream);
     Plan pl;
     pl.add<ConeViewBP>(nview);
-    pl.add<float4>(nprj);
+    if (col) {
+        pl.add<float>(nprj);
+        pl.add<float2>((size_t)nview * d.nu);
+        pl.add<float>((size_t)nview * d.nv);
+    } else {
+        pl.add<float4>(nprj);
+    }
     ws.reserve(pl.bytes);
     ConeViewBP* dviews = ws.take_n<ConeViewBP>(nview);
-    float4* table = ws.take_n<float4>(nprj);
+    float* wtab = nullptr;
+    float2* utab = nullptr;
+    float* vtab = nullptr;
+    float4* table = nullptr;
+    if (col) {
+        wtab = ws.take_n<float>(nprj);
+        utab = ws.take_n<float2>((size_t)nview * d.nu);
+        vtab = ws.take_n<float>((size_t)nview * d.nv);
+    } else {
+        table = ws.take_n<float4>(nprj);
+    }
     CTP_CHECK(hipMemcpyAsync(dviews, plan.views.data(), sizeof(ConeViewBP) * nview, hipMemcpyHostToDevice, stream));
     for (int b = 0; b < B; b++) {
+        float* im = img + b * nimg;
+        if (col) {
+            {
+                TimedLaunch tt("siddon_bp_table", stream);
+                siddon_bp_coltable_kernel<<<ceil_div((long)nprj, 256), 256, 0, stream>>>(
+                    wtab, utab, vtab, im, prj + b * nprj, geo.det_center, geo.det_u, geo.det_v, geo.src, g, d, nview);
+                CTP_LAUNCH_CHECK();
+            }
+#define CTP_BPC_CASE(A, Bv) \
+    if (wu == A && wv == Bv) { launch_bp_col<A, Bv>(im, wtab, utab, vtab, dviews, nview, g, d, stream); continue; }
+            CTP_BPC_CASE(1, 1) CTP_BPC_CASE(1, 2) CTP_BPC_CASE(2, 1) CTP_BPC_CASE(2, 2)
+            CTP_BPC_CASE(2, 3) CTP_BPC_CASE(3, 2) CTP_BPC_CASE(3, 3) CTP_BPC_CASE(1, 3) CTP_BPC_CASE(3, 1)
+            CTP_BPC_CASE(1, 4) CTP_BPC_CASE(4, 1) CTP_BPC_CASE(2, 4) CTP_BPC_CASE(4, 2) CTP_BPC_CASE(3, 4)
+            CTP_BPC_CASE(4, 3) CTP_BPC_CASE(4, 4)
+#undef CTP_BPC_CASE
+            fail("cone backprojection: unsupported window");
+        }
         {
             TimedLaunch tt("siddon_bp_table", stream);
             siddon_bp_table_kernel<<<ceil_div((long)nprj, 256), 256, 0, stream>>>(table, prj + b * nprj, geo.det_center,
                                                                                 geo.det_u, geo.det_v, geo.src, g, d, nview);
             CTP_LAUNCH_CHECK();
         }
-        float* im = img + b * nimg;
 #define CTP_BP_CASE(A, Bv) \
     if (wu == A && wv == Bv) { launch_bp<A, Bv>(im, table, dviews, nview, g, d, stream); continue; }
         CTP_BP_CASE(1, 1) CTP_BP_CASE(1, 2) CTP_BP_CASE(2, 1) CTP_BP_CASE(2, 2)

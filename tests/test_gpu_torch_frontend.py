@@ -13,8 +13,8 @@ pytestmark = pytest.mark.gpu
 @pytest.fixture(scope='module')
 def torch():
     import torch as T
-    if not T.cuda.is_available():
-        pytest.skip('no HIP device')
+    # a GPU test: the device must be visible to torch in this process
+    assert T.cuda.is_available(), 'torch sees no HIP device'
     return T
 
 

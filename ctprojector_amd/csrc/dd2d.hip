@@ -467,6 +467,7 @@ int bp_view_groups(size_t npix, int nchunk, int nview) {
         k = (int)std::min<size_t>(8, std::max<size_t>(1, 16384 / std::max<size_t>(waves, 1)));
     }
     k = std::max(1, std::min(k, std::max(1, nview / 8)));
+    k = std::max(1, std::min(k, 65535 / std::max(nchunk, 1)));   // grid.z = nchunk * groups
     return k;
 }
 

@@ -309,6 +309,8 @@ def main(argv=None):
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-extra', action='store_true')
     ap.add_argument('--calibrate-copy', action='store_true')
+    ap.add_argument('--tune', action='append', default=[], metavar='KEY=VALUE',
+                    help='library launch-shape knob (ctpSetTuning), repeatable')
     ap.add_argument('--summarize', metavar='TRACE_DIR')
     ap.add_argument('--fetch', metavar='DIR')
     ap.add_argument('--write', metavar='DIR')
@@ -326,6 +328,11 @@ def main(argv=None):
         print('bench.py needs a HIP device', file=sys.stderr)
         return 1
 
+    if args.tune:
+        from ctprojector_amd import _lib
+        for kv in args.tune:
+            k, v = kv.split('=')
+            _lib.set_tuning(k, int(v))
     if args.calibrate_copy:
         calibrate_copy()
     r = fan_workload_bench(args.steps, args.warmup)

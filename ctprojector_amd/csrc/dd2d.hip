@@ -141,6 +141,39 @@ __device__ inline void dd_line(float* acc, const float* __restrict__ row, int cs
     }
 }
 
+// The two edge rays of detector column iu in image index coordinates: from
+// s1 to d1 and from s2 to d2 (cuda/distanceDriven2DBoxInt.cu:141-158 parallel
+// with its virtual source distance, :211-226 fan).  Returns the column's fan
+// angle (0 for parallel beams).
+template <bool FAN>
+__device__ inline float fp_cell_edges(int iu, const DDView& V, const Grid& g, const Det& d, float dsd, float dso,
+                                      float2& s1, float2& s2, float2& d1, float2& d2) {
+    const float c = V.c, s = V.s;
+    float a = 0.f;
+    if (FAN) {
+        a = (-((float)iu - (float)(d.nu - 1) / 2.0f) - d.off_u) * d.du;
+        const float2 src = make_float2(dso * s, -dso * c);
+        const float e1x = dsd * sinf(a - d.du / 2), e1y = -dso + dsd * cosf(a - d.du / 2);
+        const float e2x = dsd * sinf(a + d.du / 2), e2y = -dso + dsd * cosf(a + d.du / 2);
+        s1 = s2 = src;
+        d1 = make_float2(e1x * c - e1y * s, e1x * s + e1y * c);
+        d2 = make_float2(e2x * c - e2y * s, e2x * s + e2y * c);
+    } else {
+        const float u = (-((float)iu - (float)(d.nu - 1) / 2.0f) - d.off_u) * d.du;
+        const float vd = (float)g.nx * g.dx + (float)g.ny * g.dy;
+        auto rot = [&](float px, float py) { return make_float2(px * c - py * s, px * s + py * c); };
+        s1 = rot(u - d.du / 2, -vd);
+        s2 = rot(u + d.du / 2, -vd);
+        d1 = rot(u - d.du / 2, vd);
+        d2 = rot(u + d.du / 2, vd);
+    }
+    auto toimg = [&](float2 p) {
+        return make_float2((p.x - g.cx) / g.dx + g.nx / 2.0f, (p.y - g.cy) / g.dy + g.ny / 2.0f);
+    };
+    s1 = toimg(s1); s2 = toimg(s2); d1 = toimg(d1); d2 = toimg(d2);
+    return a;
+}
+
 template <int CH, bool FAN>
 __global__ __launch_bounds__(128) void dd_fp_kernel(float* __restrict__ prj, const float* __restrict__ imgY,
                                                     const float* __restrict__ imgX, const DDView* __restrict__ views,
@@ -156,32 +189,8 @@ __global__ __launch_bounds__(128) void dd_fp_kernel(float* __restrict__ prj, con
     for (int j = 0; j <= CH; j++) acc[j] = 0.f;
     const bool active = iu < d.nu;
     if (active) {
-        const float c = V.c, s = V.s;
         float2 s1, s2, d1, d2;
-        float a = 0.f;
-        if (FAN) {
-            // cuda/distanceDriven2DBoxInt.cu:211-226
-            a = (-((float)iu - (float)(d.nu - 1) / 2.0f) - d.off_u) * d.du;
-            const float2 src = make_float2(dso * s, -dso * c);
-            const float e1x = dsd * sinf(a - d.du / 2), e1y = -dso + dsd * cosf(a - d.du / 2);
-            const float e2x = dsd * sinf(a + d.du / 2), e2y = -dso + dsd * cosf(a + d.du / 2);
-            s1 = s2 = src;
-            d1 = make_float2(e1x * c - e1y * s, e1x * s + e1y * c);
-            d2 = make_float2(e2x * c - e2y * s, e2x * s + e2y * c);
-        } else {
-            // cuda/distanceDriven2DBoxInt.cu:141-158 (virtual source distance)
-            const float u = (-((float)iu - (float)(d.nu - 1) / 2.0f) - d.off_u) * d.du;
-            const float vd = (float)g.nx * g.dx + (float)g.ny * g.dy;
-            auto rot = [&](float px, float py) { return make_float2(px * c - py * s, px * s + py * c); };
-            s1 = rot(u - d.du / 2, -vd);
-            s2 = rot(u + d.du / 2, -vd);
-            d1 = rot(u - d.du / 2, vd);
-            d2 = rot(u + d.du / 2, vd);
-        }
-        auto toimg = [&](float2 p) {
-            return make_float2((p.x - g.cx) / g.dx + g.nx / 2.0f, (p.y - g.cy) / g.dy + g.ny / 2.0f);
-        };
-        s1 = toimg(s1); s2 = toimg(s2); d1 = toimg(d1); d2 = toimg(d2);
+        const float a = fp_cell_edges<FAN>(iu, V, g, d, dsd, dso, s1, s2, d1, d2);
         const float* __restrict__ bY = imgY + c0;
         const float* __restrict__ bX = imgX + c0;
         if (V.flags & 1) {
@@ -220,6 +229,251 @@ __global__ __launch_bounds__(128) void dd_fp_kernel(float* __restrict__ prj, con
         const float val = (1.f - R.w1) * lds[threadIdx.x][R.j0] + R.w1 * lds[threadIdx.x][R.j1];
         prj[(((size_t)R.b * nview + view) * d.nv + R.iv) * d.nu + iu] = val * rev;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Forward projection with LDS-staged image rows
+// ---------------------------------------------------------------------------
+// A block is VB views x 64 detector columns (one wavefront per view; the views
+// of a block share their trace flags).  For each image row of a trace, the
+// union of the block's footprints on that row lies between the outermost cell
+// edges of the block (rays from one source do not cross between source and
+// detector), so the block copies that pixel range of the row (channel-last,
+// CHP floats per pixel) into LDS once and every lane integrates its box from
+// LDS: the row is fetched once per VB views instead of once per view, and the
+// per-lane reads move from the vector-memory path to LDS.  Rows are double
+// buffered with one barrier per row.
+constexpr int kFPViews = 4;   // views per block (VB)
+
+struct FPGroup {               // consecutive views with equal trace flags
+    int first, count;
+};
+
+// Integral of one lane's line pair over staged pixels [xlo, xlo + width) held
+// in LDS at 'buf' (CHP floats per pixel); the same arithmetic as dd_line.
+template <int CH, int CHP>
+__device__ inline void dd_line_lds(float* acc, const float* buf, int xlo, float x1, float x2, int n, float scale) {
+    if (x1 > x2) { const float t = x1; x1 = x2; x2 = t; }
+    const float a = clampf(x1, 0.f, (float)n), b = clampf(x2, 0.f, (float)n);
+    if (!(b > a)) return;
+    const float inv = scale / (x2 - x1);
+    const int i1 = min((int)a, n - 1), i2 = min((int)b, n - 1);
+    for (int i = i1; i <= i2; i++) {
+        const float w = (fminf(b, (float)(i + 1)) - fmaxf(a, (float)i)) * inv;
+        const float* p = buf + (i - xlo) * CHP;
+#pragma unroll
+        for (int j = 0; j < CHP; j += 4) {
+            const float4 q = *reinterpret_cast<const float4*>(p + j);
+            if (j < CH + 1) acc[j] = fmaf(w, q.x, acc[j]);
+            if (j + 1 < CH + 1) acc[j + 1] = fmaf(w, q.y, acc[j + 1]);
+            if (j + 2 < CH + 1) acc[j + 2] = fmaf(w, q.z, acc[j + 2]);
+            if (j + 3 < CH + 1) acc[j + 3] = fmaf(w, q.w, acc[j + 3]);
+        }
+    }
+}
+
+// One trace (rows along y for XT = false, along x for XT = true) of a block.
+template <int CH, int CHP, int VB, int KREG>
+__device__ inline void fp_trace_lds(float* acc, float* stage, const float (*edges)[4][3], int nviews_blk,
+                                    const float* __restrict__ src, int cs, int c0, int nrow, int nlen, int wmax,
+                                    bool active, float r1, float o1, float b1, float r2, float o2, float b2,
+                                    float norm) {
+    constexpr int Q = CHP / 4;             // float4 per pixel
+    const int nthr = 64 * VB;
+    // pixel range of row 'row' staged by the block
+    auto range = [&](int row, int& lo, int& width) {
+        const float t = (float)row + 0.5f;
+        float mn = 3.0e38f, mx = -3.0e38f;
+        for (int w = 0; w < nviews_blk; w++)
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const float x = fmaf(edges[w][k][0], t - edges[w][k][1], edges[w][k][2]);
+                mn = fminf(mn, x);
+                mx = fmaxf(mx, x);
+            }
+        // clamp both ends into [-1, nlen + 1] (fminf/fmaxf also map NaN there)
+        mn = fminf(fmaxf(mn, -1.f), (float)nlen + 1.f);
+        mx = fmaxf(fminf(mx, (float)nlen + 1.f), -1.f);
+        lo = max(0, (int)floorf(mn) - 1);
+        const int hi = min(nlen - 1, (int)floorf(mx) + 1);
+        width = min(max(hi - lo + 1, 0), wmax);
+    };
+    float4 reg[KREG];                       // in-flight row segment of this thread
+    int lo_next = 0, w_next = 0;
+    auto fetch = [&](int row) {
+        range(row, lo_next, w_next);
+        const float* base = src + ((size_t)row * nlen + lo_next) * cs + c0;
+        const int total = w_next * Q;
+#pragma unroll
+        for (int k = 0; k < KREG; k++) {
+            const int t = threadIdx.x + k * nthr;
+            if (t < total) {
+                const int px = t / Q, q = t - px * Q;
+                reg[k] = *reinterpret_cast<const float4*>(base + (size_t)px * cs + 4 * q);
+            }
+        }
+    };
+    auto put = [&](float* buf) {
+        const int total = w_next * Q;
+#pragma unroll
+        for (int k = 0; k < KREG; k++) {
+            const int t = threadIdx.x + k * nthr;
+            if (t < total) reinterpret_cast<float4*>(buf)[t] = reg[k];
+        }
+    };
+    fetch(0);
+    put(stage);
+    int lo_cur = lo_next;
+    __syncthreads();
+    for (int row = 0; row < nrow; row++) {
+        float* cur = stage + (row & 1) * wmax * CHP;
+        float* nxt = stage + ((row + 1) & 1) * wmax * CHP;
+        const int lo_row = lo_cur;
+        if (row + 1 < nrow) fetch(row + 1);
+        if (active) {
+            const float t = (float)row + 0.5f;
+            const float x1 = fmaf(r1, t - o1, b1);
+            const float x2 = fmaf(r2, t - o2, b2);
+            dd_line_lds<CH, CHP>(acc, cur, lo_row, x1, x2, nlen, norm);
+        }
+        if (row + 1 < nrow) {
+            put(nxt);
+            lo_cur = lo_next;
+        }
+        __syncthreads();
+    }
+}
+
+template <int CH, bool FAN, int KREG>
+__global__ __launch_bounds__(64 * kFPViews) void dd_fp_lds_kernel(
+        float* __restrict__ prj, const float* __restrict__ imgY, const float* __restrict__ imgX,
+        const DDView* __restrict__ views, const FPGroup* __restrict__ groups, const FPRow* __restrict__ rows,
+        const int* __restrict__ row_start, int cs, int wmax, Grid g, Det d, int nview, float dsd, float dso,
+        bool force) {
+    constexpr int VB = kFPViews;
+    constexpr int CHP = ((CH + 1) + 3) / 4 * 4;
+    extern __shared__ float4 smem4[];
+    float* stage = reinterpret_cast<float*>(smem4);
+    __shared__ float edges[VB][4][3];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int col0 = blockIdx.x * 64;
+    const FPGroup G = groups[blockIdx.y];
+    const int chunk = blockIdx.z, c0 = chunk * CH;
+    const bool vok = w < G.count;
+    const int view = G.first + (vok ? w : 0);
+    const DDView V = views[view];
+    const int flags = views[G.first].flags;
+    const int iu = col0 + lane;
+    const bool active = vok && iu < d.nu;
+    const int last = min(63, d.nu - 1 - col0);
+    float acc[CH + 1];
+#pragma unroll
+    for (int j = 0; j <= CH; j++) acc[j] = 0.f;
+    float2 s1, s2, d1, d2;
+    const float a = fp_cell_edges<FAN>(min(iu, d.nu - 1), V, g, d, dsd, dso, s1, s2, d1, d2);
+    if (flags & 1) {
+        // trace along y, cuda/distanceDriven2DBoxInt.cu:48-71
+        const float r1 = (d1.x - s1.x) / (d1.y - s1.y);
+        const float r2 = (d2.x - s2.x) / (d2.y - s2.y);
+        if (vok && (lane == 0 || lane == last)) {
+            const int k = lane == 0 ? 0 : 2;
+            edges[w][k][0] = r1; edges[w][k][1] = s1.y; edges[w][k][2] = s1.x;
+            edges[w][k + 1][0] = r2; edges[w][k + 1][1] = s2.y; edges[w][k + 1][2] = s2.x;
+        }
+        __syncthreads();
+        const float norm = V.wx * g.dy / fabsf(cosf(V.deg - a));
+        fp_trace_lds<CH, CHP, VB, KREG>(acc, stage, edges, G.count, imgY, cs, c0, g.ny, g.nx, wmax, active,
+                                         r1, s1.y, s1.x, r2, s2.y, s2.x, norm);
+    }
+    if (flags & 2) {
+        // trace along x, cuda/distanceDriven2DBoxInt.cu:74-98
+        const float q1 = (d1.y - s1.y) / (d1.x - s1.x);
+        const float q2 = (d2.y - s2.y) / (d2.x - s2.x);
+        __syncthreads();                   // edges of a previous trace are no longer read
+        if (vok && (lane == 0 || lane == last)) {
+            const int k = lane == 0 ? 0 : 2;
+            edges[w][k][0] = q1; edges[w][k][1] = s1.x; edges[w][k][2] = s1.y;
+            edges[w][k + 1][0] = q2; edges[w][k + 1][1] = s2.x; edges[w][k + 1][2] = s2.y;
+        }
+        __syncthreads();
+        const float norm = (1.f - V.wx) * g.dx / fabsf(sinf(V.deg - a));
+        fp_trace_lds<CH, CHP, VB, KREG>(acc, stage, edges, G.count, imgX, cs, c0, g.nx, g.ny, wmax, active,
+                                        q1, s1.x, s1.y, q2, s2.x, s2.y, norm);
+    }
+    // z interpolation of the detector rows from the accumulators (each lane
+    // reads back only its own accumulators)
+    float* mine = stage + threadIdx.x * (CH + 2);
+#pragma unroll
+    for (int j = 0; j <= CH; j++) mine[j] = acc[j];
+    if (!active) return;
+    const float rev = (!FAN && force) ? V.rev : 1.f;
+    const int r0 = row_start[chunk], rend = row_start[chunk + 1];
+    for (int r = r0; r < rend; r++) {
+        const FPRow R = rows[r];
+        const float val = (1.f - R.w1) * mine[R.j0] + R.w1 * mine[R.j1];
+        prj[(((size_t)R.b * nview + view) * d.nv + R.iv) * d.nu + iu] = val * rev;
+    }
+}
+
+// Host copy of fp_cell_edges in FP64: the widest staged range (pixels) of any
+// block, or a huge value when a ray runs (nearly) along a traced row.
+static double fp_stage_width(const std::vector<DDView>& views, const std::vector<FPGroup>& groups, const Grid& g,
+                             const Det& d, float dsd, float dso, bool fan) {
+    double wmax = 0;
+    auto edge_lines = [&](int iu, const DDView& V, double ln[2][2][3]) {
+        // ln[trace][edge] = (slope, origin, offset): x = slope * (t - origin) + offset
+        const double c = V.c, s = V.s;
+        double s1[2], s2[2], d1[2], d2[2];
+        if (fan) {
+            const double a = (-((double)iu - (d.nu - 1) / 2.0) - d.off_u) * d.du;
+            const double src[2] = {dso * s, -dso * c};
+            const double e1x = dsd * std::sin(a - d.du / 2), e1y = -dso + dsd * std::cos(a - d.du / 2);
+            const double e2x = dsd * std::sin(a + d.du / 2), e2y = -dso + dsd * std::cos(a + d.du / 2);
+            s1[0] = s2[0] = src[0]; s1[1] = s2[1] = src[1];
+            d1[0] = e1x * c - e1y * s; d1[1] = e1x * s + e1y * c;
+            d2[0] = e2x * c - e2y * s; d2[1] = e2x * s + e2y * c;
+        } else {
+            const double u = (-((double)iu - (d.nu - 1) / 2.0) - d.off_u) * d.du;
+            const double vd = (double)g.nx * g.dx + (double)g.ny * g.dy;
+            auto rot = [&](double px, double py, double* o) { o[0] = px * c - py * s; o[1] = px * s + py * c; };
+            rot(u - d.du / 2, -vd, s1); rot(u + d.du / 2, -vd, s2);
+            rot(u - d.du / 2, vd, d1); rot(u + d.du / 2, vd, d2);
+        }
+        auto toimg = [&](double* p) { p[0] = (p[0] - g.cx) / g.dx + g.nx / 2.0; p[1] = (p[1] - g.cy) / g.dy + g.ny / 2.0; };
+        toimg(s1); toimg(s2); toimg(d1); toimg(d2);
+        ln[0][0][0] = (d1[0] - s1[0]) / (d1[1] - s1[1]); ln[0][0][1] = s1[1]; ln[0][0][2] = s1[0];
+        ln[0][1][0] = (d2[0] - s2[0]) / (d2[1] - s2[1]); ln[0][1][1] = s2[1]; ln[0][1][2] = s2[0];
+        ln[1][0][0] = (d1[1] - s1[1]) / (d1[0] - s1[0]); ln[1][0][1] = s1[0]; ln[1][0][2] = s1[1];
+        ln[1][1][0] = (d2[1] - s2[1]) / (d2[0] - s2[0]); ln[1][1][1] = s2[0]; ln[1][1][2] = s2[1];
+    };
+    const int ncb = ceil_div(d.nu, 64);
+    for (const FPGroup& G : groups)
+        for (int cb = 0; cb < ncb; cb++) {
+            const int cols[2] = {cb * 64, std::min(d.nu - 1, cb * 64 + 63)};
+            for (int tr = 0; tr < 2; tr++) {
+                if (!(views[G.first].flags & (1 << tr))) continue;
+                const int nrow = tr == 0 ? g.ny : g.nx;
+                for (int end = 0; end < 2; end++) {
+                    const double t = end == 0 ? 0.5 : nrow - 0.5;
+                    double mn = 1e300, mx = -1e300;
+                    for (int k = 0; k < G.count; k++)
+                        for (int ci = 0; ci < 2; ci++) {
+                            double ln[2][2][3];
+                            edge_lines(cols[ci], views[G.first + k], ln);
+                            for (int e = 0; e < 2; e++) {
+                                const double x = ln[tr][e][0] * (t - ln[tr][e][1]) + ln[tr][e][2];
+                                if (!std::isfinite(x)) return 1e300;
+                                mn = std::min(mn, x);
+                                mx = std::max(mx, x);
+                            }
+                        }
+                    // unclamped: the width between the outermost edges is convex in t,
+                    // so its maximum over the rows is at one of the two ends
+                    wmax = std::max(wmax, mx - mn);
+                }
+            }
+        }
+    return wmax;
 }
 
 static int pick_ch(int C) {
@@ -266,6 +520,25 @@ void dd2d_fp(float* prj, const float* img, const float* deg_host, int B, const G
     }
     start[nchunk] = (int)rows.size();
 
+    // LDS-staged kernel: groups of up to kFPViews consecutive views with equal
+    // trace flags; staged width bounded in FP64 (+ margins for the +-1 pixel
+    // guard and floor rounding)
+    std::vector<FPGroup> groups;
+    for (int i = 0; i < nview;) {
+        int n = 1;
+        while (n < kFPViews && i + n < nview && (views[i + n].flags & 3) == (views[i].flags & 3)) n++;
+        groups.push_back(FPGroup{i, n});
+        i += n;
+    }
+    const int CHP = ((CH + 1) + 3) / 4 * 4;
+    const int kreg_max = 8;
+    const int wcap = std::min(kreg_max * 64 * kFPViews / (CHP / 4), 65536 / (2 * CHP * 4));
+    const double wneed = fp_stage_width(views, groups, g, d, dsd, dso, fan) + 6.0;
+    const bool use_lds = tuning_get("dd_fp_mode", 0) == 0 && wneed <= (double)wcap && groups.size() <= 65535;
+    const int wmax = use_lds ? (int)std::ceil(wneed) : 0;
+    int kreg = 1;
+    while (use_lds && kreg * 64 * kFPViews < wmax * (CHP / 4)) kreg *= 2;
+
     const size_t npix = (size_t)g.nx * g.ny;
     Workspace& ws = workspace_for(stream);
     Plan pl;
@@ -274,15 +547,18 @@ void dd2d_fp(float* prj, const float* img, const float* deg_host, int B, const G
     pl.add<DDView>(nview);
     pl.add<FPRow>(rows.size());
     pl.add<int>(nchunk + 1);
+    pl.add<FPGroup>(groups.size());
     ws.reserve(pl.bytes);
     float* imgY = ws.take_n<float>(npix * cs);
     float* imgX = ws.take_n<float>(npix * cs);
     DDView* dviews = ws.take_n<DDView>(nview);
     FPRow* drows = ws.take_n<FPRow>(rows.size());
     int* dstart = ws.take_n<int>(nchunk + 1);
+    FPGroup* dgroups = ws.take_n<FPGroup>(groups.size());
     CTP_CHECK(hipMemcpyAsync(dviews, views.data(), sizeof(DDView) * nview, hipMemcpyHostToDevice, stream));
     CTP_CHECK(hipMemcpyAsync(drows, rows.data(), sizeof(FPRow) * rows.size(), hipMemcpyHostToDevice, stream));
     CTP_CHECK(hipMemcpyAsync(dstart, start.data(), sizeof(int) * (nchunk + 1), hipMemcpyHostToDevice, stream));
+    CTP_CHECK(hipMemcpyAsync(dgroups, groups.data(), sizeof(FPGroup) * groups.size(), hipMemcpyHostToDevice, stream));
 
     bool needY = false, needX = false;
     for (auto& v : views) { needY |= (v.flags & 1) != 0; needX |= (v.flags & 2) != 0; }
@@ -300,8 +576,26 @@ void dd2d_fp(float* prj, const float* img, const float* deg_host, int B, const G
     // the branchless parallel FP has no force-FBP reverse weighting
     // (cuda/distanceDriven2DBranchless.cu:321-360)
     const bool force = (type & DD_FORCE_FBP) != 0 && !branchless;
+    if (use_lds) {
+        dim3 grid(ceil_div(d.nu, 64), (unsigned)groups.size(), nchunk), block(64 * kFPViews);
+        const size_t smem = 4 * std::max<size_t>((size_t)2 * wmax * CHP, (size_t)64 * kFPViews * (CH + 2));
+        TimedLaunch t(fan ? "dd_fan_fp" : "dd_par_fp", stream);
+#define CTP_FPL(N, K)                                                                                           \
+    if (CH == N && kreg == K) {                                                                                \
+        if (fan) dd_fp_lds_kernel<N, true, K><<<grid, block, smem, stream>>>(prj, imgY, imgX, dviews, dgroups,   \
+                                                                       drows, dstart, cs, wmax, g, d, nview, dsd, \
+                                                                       dso, force);                              \
+        else dd_fp_lds_kernel<N, false, K><<<grid, block, smem, stream>>>(prj, imgY, imgX, dviews, dgroups,      \
+                                                                        drows, dstart, cs, wmax, g, d, nview, dsd, \
+                                                                        dso, force);                             \
+    }
+#define CTP_FPL_K(N) CTP_FPL(N, 1) CTP_FPL(N, 2) CTP_FPL(N, 4) CTP_FPL(N, 8)
+        CTP_FPL_K(1) CTP_FPL_K(2) CTP_FPL_K(4) CTP_FPL_K(8) CTP_FPL_K(16) CTP_FPL_K(32)
+#undef CTP_FPL_K
+#undef CTP_FPL
+        CTP_LAUNCH_CHECK();
+    } else {
     dim3 grid(ceil_div(d.nu, 128), nview, nchunk), block(128);
-    {
     TimedLaunch t(fan ? "dd_fan_fp" : "dd_par_fp", stream);
 #define CTP_FP_CASE(N)                                                                                          \
     case N:                                                                                                     \

@@ -182,19 +182,36 @@ __global__ __launch_bounds__(256) void siddon_cone_fp_kernel(float* __restrict__
                                                              const float* __restrict__ imgT, const int* __restrict__ swapf,
                                                              const float* __restrict__ pdc, const float* __restrict__ pdu,
                                                              const float* __restrict__ pdv, const float* __restrict__ psrc,
-                                                             Grid g, Det d, int nview) {
+                                                             Grid g, Det d, int nview, int nub, int ntile,
+                                                             int xcd_order) {
     static_assert(TU * TV == 64, "one wavefront per tile");
+    // Block -> (view, detector tile).  Workgroups are dispatched round-robin
+    // over the 8 XCDs (each with its own L2).  In XCD order, XCD x receives the
+    // x-th eighth of every view's tiles in row-major order, i.e. a band of
+    // detector rows whose rays cross one slab of the volume, and walks through
+    // the views with it; otherwise tiles are in plain view-major order.
+    const int per = (ntile + 7) >> 3;                 // tiles per XCD and view
+    int view, tile;
+    if (xcd_order) {
+        const int xcd = blockIdx.x & 7, j = blockIdx.x >> 3;
+        view = j / per;
+        tile = xcd * per + (j - view * per);
+    } else {
+        view = blockIdx.x / (8 * per);
+        tile = blockIdx.x - view * 8 * per;
+    }
+    if (tile >= ntile || view >= nview) return;
+    const int ub = tile % nub, vb = tile / nub;
     // per-view layout choice and the volume's buffer resource, built before any
     // divergent branch so that they stay in scalar registers (one view per block)
-    const int sw = __builtin_amdgcn_readfirstlane(swapf[blockIdx.z]);
+    const int sw = __builtin_amdgcn_readfirstlane(swapf[view]);
     const __amdgpu_buffer_rsrc_t vol = __builtin_amdgcn_make_buffer_rsrc(
         (void*)(sw ? imgT : img), (short)0, 4 * g.nx * g.ny * g.nz, 0x00020000);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     // a block is 4 tiles stacked along v
-    const int iu = blockIdx.x * TU + (lane % TU);
-    const int iv = (blockIdx.y * 4 + wave) * TV + (lane / TU);
-    const int view = blockIdx.z;
-    if (iu >= d.nu || iv >= d.nv || view >= nview) return;
+    const int iu = ub * TU + (lane % TU);
+    const int iv = (vb * 4 + wave) * TV + (lane / TU);
+    if (iu >= d.nu || iv >= d.nv) return;
 
     SiddonRay r;
     if (!siddon_setup(r, ld3(psrc, view), cell_centre(pdc, pdu, pdv, view, iu, iv, d), g, sw != 0)) return;
@@ -240,7 +257,12 @@ void siddon_cone_fp(float* prj, const float* img, const ConeGeomDev& geo, int B,
     const int TU = tuning_get("cone_fp_tile_u", 32);
     if (TU != 8 && TU != 16 && TU != 32 && TU != 64) fail("cone_fp_tile_u must be 8, 16, 32 or 64");
     const int TV = 64 / TU;
-    dim3 block(256), grid(ceil_div(d.nu, TU), ceil_div(d.nv, 4 * TV), nview);
+    const int nub = ceil_div(d.nu, TU), nvb = ceil_div(d.nv, 4 * TV);
+    const int ntile = nub * nvb;
+    const long nblocks = (long)nview * 8 * ((ntile + 7) / 8);
+    if (nblocks > 0x7fffffffL) fail("cone forward projection: too many detector tiles");
+    const int xcd_order = tuning_get("cone_fp_order", 0) == 0 ? 1 : 0;   // 0: XCD bands, 1: plain
+    dim3 block(256), grid((unsigned)nblocks);
     for (int b = 0; b < B; b++) {
         {
             TimedLaunch t("siddon_transpose", stream);
@@ -252,7 +274,8 @@ void siddon_cone_fp(float* prj, const float* img, const ConeGeomDev& geo, int B,
 #define CTP_FP_T(A, Bv)                                                                                          \
     if (TU == A) siddon_cone_fp_kernel<A, Bv><<<grid, block, 0, stream>>>(prj + b * nprj, img + b * nimg, imgT, swapf, \
                                                                           geo.det_center, geo.det_u, geo.det_v,     \
-                                                                          geo.src, g, d, nview);
+                                                                          geo.src, g, d, nview, nub, ntile, \
+                                                                          xcd_order);
         CTP_FP_T(8, 8) CTP_FP_T(16, 4) CTP_FP_T(32, 2) CTP_FP_T(64, 1)
 #undef CTP_FP_T
         CTP_LAUNCH_CHECK();

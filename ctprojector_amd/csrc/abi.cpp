@@ -495,3 +495,62 @@ CTP_EXPORT int cfbpFanBackprojection(float* pImg, const float* pPrj, const float
     return fbp_bp_entry<false>("cfbpFanBackprojection", pImg, pPrj, pDeg, nBatches, nx, ny, nz, dx, dy, dz, cx, cy, cz,
                                nu, nv, nview, da, dv, off_a, off_v, dsd, dso);
 }
+
+// ===========================================================================
+// TV prior for SQS (the reference's libprior.so, prior-cuda/TotalVariation.cu)
+// ===========================================================================
+static void tv_dims(size_t B, size_t nx, size_t ny, size_t nz) {
+    need(nx <= kIntMax && ny <= 65535 && nz <= 65535 && B <= 65535, "tv prior: dimension too large");
+    need(nx * ny * nz <= kIntMax, "tv prior: more than 2^31-1 voxels per batch item");
+}
+
+template <bool DEV>
+static int tv_entry(const char* name, float* s1, float* s2, float* var, const float* img, float wx, float wy, float wz,
+                    size_t B, size_t nx, size_t ny, size_t nz, float eps) {
+    return guarded(name, [&] {
+        if (B == 0 || nx == 0 || ny == 0 || nz == 0) return;
+        tv_dims(B, nx, ny, nz);
+        need_ptr(s1, "s1"); need_ptr(s2, "s2"); need_ptr(var, "var"); need_ptr(img, "img");
+        const size_t n = B * nx * ny * nz;
+        if (DEV) {
+            touch();
+            tv_sqs(s1, s2, var, img, wx, wy, wz, (int)B, (int)nx, (int)ny, (int)nz, eps, nullptr);
+            sync();
+        } else {
+            touch();
+            DeviceBuffer a(n * 4), b(n * 4), c(n * 4), x(n * 4);
+            up(x.p, img, n * 4);
+            tv_sqs(a.as<float>(), b.as<float>(), c.as<float>(), x.as<float>(), wx, wy, wz, (int)B, (int)nx, (int)ny,
+                   (int)nz, eps, nullptr);
+            sync();
+            down(s1, a.p, n * 4);
+            down(s2, b.p, n * 4);
+            down(var, c.p, n * 4);
+        }
+    });
+}
+
+CTP_EXPORT int cupyTVSQS3D(float* s1, float* s2, float* var, const float* img, float wx, float wy, float wz,
+                           size_t nBatches, size_t nx, size_t ny, size_t nz, float eps) {
+    return tv_entry<true>("cupyTVSQS3D", s1, s2, var, img, wx, wy, wz, nBatches, nx, ny, nz, eps);
+}
+CTP_EXPORT int cTVSQS3D(float* s1, float* s2, float* var, const float* img, float wx, float wy, float wz,
+                        size_t nBatches, size_t nx, size_t ny, size_t nz, float eps) {
+    return tv_entry<false>("cTVSQS3D", s1, s2, var, img, wx, wy, wz, nBatches, nx, ny, nz, eps);
+}
+
+// One SQS step with the TV prior, fused (device pointers):
+//   out = img - (acc * bp + beta * s1(img)) / (norm + beta * s2(img)); var(img) if var != NULL
+CTP_EXPORT int ctpTVSQSUpdate3D(float* out, float* var, const float* img, const float* bp, const float* norm, float beta,
+                                float acc, float wx, float wy, float wz, size_t nBatches, size_t nx, size_t ny,
+                                size_t nz, float eps) {
+    return guarded("ctpTVSQSUpdate3D", [&] {
+        if (nBatches == 0 || nx == 0 || ny == 0 || nz == 0) return;
+        tv_dims(nBatches, nx, ny, nz);
+        need_ptr(out, "out"); need_ptr(img, "img"); need_ptr(bp, "bp"); need_ptr(norm, "norm");
+        touch();
+        tv_sqs_update(out, var, img, bp, norm, beta, acc, wx, wy, wz, (int)nBatches, (int)nx, (int)ny, (int)nz, eps,
+                      nullptr);
+        sync();
+    });
+}

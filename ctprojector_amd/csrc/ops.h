@@ -84,4 +84,10 @@ void ramp_filter(float* out, const float* prj, int B, int nu, int nv, int nview,
 void fan_fbp_bp(float* img, const float* prj, const float* deg_host, int B, const Grid& g, const Det& d, int nview,
                 float dsd, float dso, hipStream_t stream);
 
+// ---- anisotropic TV prior for SQS (prior-cuda/TotalVariation.cu) ----
+void tv_sqs(float* s1, float* s2, float* var, const float* img, float wx, float wy, float wz, int B, int nx, int ny,
+            int nz, float eps, hipStream_t stream);
+void tv_sqs_update(float* out, float* var, const float* img, const float* bp, const float* norm, float beta, float acc,
+                   float wx, float wy, float wz, int B, int nx, int ny, int nz, float eps, hipStream_t stream);
+
 }  // namespace ctp

@@ -1,0 +1,112 @@
+// Anisotropic TV prior for SQS on MI355X (gfx950): the reference's
+// prior-cuda/TotalVariation.cu (Variation3DKernel :21-55, TVSQS3DKernel
+// :57-139, TVSQS3D_gpu :141-180).
+//
+// The reference runs two kernels per batch item: one writes the variation map
+// var, the next reads img and var again to form s1 and s2 (24 B of HBM traffic
+// per voxel).  Here one kernel computes var at the voxel and at its three
+// forward neighbours directly from the image (the neighbour values come from
+// cache), and writes s1, s2, var once (16 B per voxel).  tv_sqs_update fuses
+// the SQS step x <- x - (a * bp + beta * s1) / (norm + beta * s2) into the same
+// pass, so s1 and s2 never reach memory (16 B per voxel: img, bp, norm in, x out).
+#include "common.h"
+#include "runtime.h"
+#include "ops.h"
+
+namespace ctp {
+
+struct TVW {
+    float wx, wy, wz, eps;
+};
+
+// variation at voxel (ix, iy, iz) of one volume: the reference's
+// Variation3DKernel with zero differences at the lower edges
+__device__ inline float tv_var(const float* __restrict__ img, size_t ind, int ix, int iy, int iz, int nx, size_t nxy,
+                               const TVW& w) {
+    const float x = img[ind];
+    const float dx = ix > 0 ? x - img[ind - 1] : 0.f;
+    const float dy = iy > 0 ? x - img[ind - nx] : 0.f;
+    const float dz = iz > 0 ? x - img[ind - nxy] : 0.f;
+    return sqrtf(w.wx * dx * dx + w.wy * dy * dy + w.wz * dz * dz + w.eps);
+}
+
+// s1, s2 of the reference's TVSQS3DKernel (the value at the upper edges uses
+// the voxel's own variation for the missing neighbour, as there)
+__device__ inline void tv_s12(const float* __restrict__ img, size_t ind, int ix, int iy, int iz, int nx, int ny,
+                              int nz, size_t nxy, const TVW& w, float& s1, float& s2, float& v) {
+    const float x = img[ind];
+    v = tv_var(img, ind, ix, iy, iz, nx, nxy, w);
+    float dx0 = ix > 0 ? x - img[ind - 1] : 0.f;
+    float dy0 = iy > 0 ? x - img[ind - nx] : 0.f;
+    float dz0 = iz > 0 ? x - img[ind - nxy] : 0.f;
+    float dx1 = 0.f, dy1 = 0.f, dz1 = 0.f, varx = v, vary = v, varz = v;
+    if (ix < nx - 1) { dx1 = x - img[ind + 1]; varx = tv_var(img, ind + 1, ix + 1, iy, iz, nx, nxy, w); }
+    if (iy < ny - 1) { dy1 = x - img[ind + nx]; vary = tv_var(img, ind + nx, ix, iy + 1, iz, nx, nxy, w); }
+    if (iz < nz - 1) { dz1 = x - img[ind + nxy]; varz = tv_var(img, ind + nxy, ix, iy, iz + 1, nx, nxy, w); }
+    dx0 *= w.wx; dx1 *= w.wx;
+    dy0 *= w.wy; dy1 *= w.wy;
+    dz0 *= w.wz; dz1 *= w.wz;
+    s1 = (dx0 + dy0 + dz0) / v + dx1 / varx + dy1 / vary + dz1 / varz;
+    s2 = (w.wx + w.wy + w.wz) / v + w.wx / varx + w.wy / vary + w.wz / varz;
+}
+
+// grid: (ceil(nx/64), ny, nz * B); one lane per voxel along x
+__global__ __launch_bounds__(64) void tv_sqs_kernel(float* __restrict__ s1, float* __restrict__ s2,
+                                                    float* __restrict__ var, const float* __restrict__ img, int nx,
+                                                    int ny, int nz, TVW w) {
+    const int ix = blockIdx.x * 64 + threadIdx.x;
+    const int iy = blockIdx.y;
+    const int b = blockIdx.z / nz, iz = blockIdx.z - b * nz;
+    if (ix >= nx) return;
+    const size_t nxy = (size_t)nx * ny;
+    const size_t base = (size_t)b * nxy * nz;
+    const size_t ind = (size_t)iz * nxy + (size_t)iy * nx + ix;
+    float a, c, v;
+    tv_s12(img + base, ind, ix, iy, iz, nx, ny, nz, nxy, w, a, c, v);
+    s1[base + ind] = a;
+    s2[base + ind] = c;
+    var[base + ind] = v;
+}
+
+// out = img - (acc * bp + beta * s1) / (norm + beta * s2); var optional
+__global__ __launch_bounds__(64) void tv_sqs_update_kernel(float* __restrict__ out, float* __restrict__ var,
+                                                           const float* __restrict__ img, const float* __restrict__ bp,
+                                                           const float* __restrict__ norm, float beta, float acc,
+                                                           int nx, int ny, int nz, TVW w) {
+    const int ix = blockIdx.x * 64 + threadIdx.x;
+    const int iy = blockIdx.y;
+    const int b = blockIdx.z / nz, iz = blockIdx.z - b * nz;
+    if (ix >= nx) return;
+    const size_t nxy = (size_t)nx * ny;
+    const size_t base = (size_t)b * nxy * nz;
+    const size_t ind = (size_t)iz * nxy + (size_t)iy * nx + ix;
+    float a, c, v;
+    tv_s12(img + base, ind, ix, iy, iz, nx, ny, nz, nxy, w, a, c, v);
+    const size_t i = base + ind;
+    out[i] = img[i] - (bp[i] * acc + beta * a) / (norm[i] + beta * c);
+    if (var) var[i] = v;
+}
+
+static dim3 tv_grid(int B, int nx, int ny, int nz) {
+    if ((long)nz * B > 65535 || ny > 65535) fail("tv prior: ny and nz * batches must not exceed 65535");
+    return dim3(ceil_div(nx, 64), ny, nz * B);
+}
+
+void tv_sqs(float* s1, float* s2, float* var, const float* img, float wx, float wy, float wz, int B, int nx, int ny,
+            int nz, float eps, hipStream_t stream) {
+    const TVW w{wx, wy, wz, eps};
+    TimedLaunch t("tv_sqs", stream);
+    tv_sqs_kernel<<<tv_grid(B, nx, ny, nz), 64, 0, stream>>>(s1, s2, var, img, nx, ny, nz, w);
+    CTP_LAUNCH_CHECK();
+}
+
+void tv_sqs_update(float* out, float* var, const float* img, const float* bp, const float* norm, float beta, float acc,
+                   float wx, float wy, float wz, int B, int nx, int ny, int nz, float eps, hipStream_t stream) {
+    if (out == img) fail("tv_sqs_update: the output must not alias the input image (neighbours are read)");
+    const TVW w{wx, wy, wz, eps};
+    TimedLaunch t("tv_sqs_update", stream);
+    tv_sqs_update_kernel<<<tv_grid(B, nx, ny, nz), 64, 0, stream>>>(out, var, img, bp, norm, beta, acc, nx, ny, nz, w);
+    CTP_LAUNCH_CHECK();
+}
+
+}  // namespace ctp

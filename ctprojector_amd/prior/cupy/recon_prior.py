@@ -1,0 +1,52 @@
+"""TV prior on device tensors: the reference's
+src/ct_projector/prior/cupy/recon_prior.py:85-135 (``tv_sqs``) with torch
+tensors, plus ``tv_sqs_update``, the fused SQS-TV step.  The NLM priors of that
+module are not on the projector path and are not provided."""
+
+from ctypes import c_float, c_ulong
+from typing import Tuple
+
+import torch
+
+from ... import _lib
+from ...projector.cupy._dev import dev_f32, device_of, on_device, ptr
+
+
+def tv_sqs(img: torch.Tensor, weights: Tuple[float, float, float], eps: float = 1e-8):
+    """Anisotropic TV surrogate of img [batch, nz, ny, nx]; weights along (z, y, x).
+    Returns (s1, s2, var) as device tensors."""
+    dev = device_of(img)
+    img = dev_f32(img, dev)
+    s1 = torch.zeros_like(img)
+    s2 = torch.zeros_like(img)
+    var = torch.zeros_like(img)
+    with on_device(dev):
+        err = _lib.module().cupyTVSQS3D(
+            ptr(s1), ptr(s2), ptr(var), ptr(img), c_float(weights[2]), c_float(weights[1]), c_float(weights[0]),
+            c_ulong(img.shape[0]), c_ulong(img.shape[3]), c_ulong(img.shape[2]), c_ulong(img.shape[1]), c_float(eps))
+    if err != 0:
+        print(err)
+    return s1, s2, var
+
+
+def tv_sqs_update(img: torch.Tensor, bp: torch.Tensor, norm_img: torch.Tensor, beta: float,
+                  weights: Tuple[float, float, float], eps: float = 1e-8, acc_fac: float = 1.0,
+                  return_var: bool = False):
+    """One fused SQS step with the TV prior (one pass over the volume):
+    img - (acc_fac * bp + beta * s1) / (norm_img + beta * s2), with s1, s2 the
+    surrogate of ``tv_sqs(img, weights, eps)``.  Returns the new image (and the
+    variation map of ``img`` when ``return_var``)."""
+    dev = device_of(img)
+    img = dev_f32(img, dev)
+    bp = dev_f32(bp, dev)
+    norm_img = dev_f32(norm_img, dev).expand_as(img).contiguous()
+    out = torch.empty_like(img)
+    var = torch.empty_like(img) if return_var else None
+    with on_device(dev):
+        err = _lib.module().ctpTVSQSUpdate3D(
+            ptr(out), ptr(var) if var is not None else None, ptr(img), ptr(bp), ptr(norm_img), c_float(beta),
+            c_float(acc_fac), c_float(weights[2]), c_float(weights[1]), c_float(weights[0]),
+            c_ulong(img.shape[0]), c_ulong(img.shape[3]), c_ulong(img.shape[2]), c_ulong(img.shape[1]), c_float(eps))
+    if err != 0:
+        raise RuntimeError('ctpTVSQSUpdate3D failed (%d): %s' % (err, _lib.last_error_message()))
+    return (out, var) if return_var else out

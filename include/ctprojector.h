@@ -151,6 +151,23 @@ int cfbpFanBackprojection(float* pImg, const float* pPrj, const float* pDeg, siz
                           size_t nz, float dx, float dy, float dz, float cx, float cy, float cz, size_t nu, size_t nv,
                           size_t nview, float da, float dv, float off_a, float off_v, float dsd, float dso);
 
+/* ---- TV prior for SQS (the reference's libprior.so) ---------------------- */
+/* prior-cuda/TotalVariation.cu:182  s1, s2, var of the anisotropic TV surrogate
+ * (device arrays [B][nz][ny][nx]); wx, wy, wz weight the x, y, z differences */
+int cupyTVSQS3D(float* s1, float* s2, float* var, const float* img, float wx, float wy, float wz, size_t nBatches,
+                size_t nx, size_t ny, size_t nz, float eps);
+/* prior-cuda/TotalVariation.cu:217  the same on host arrays (every batch item
+ * processed; the reference allocates one batch item's worth, :231-246) */
+int cTVSQS3D(float* s1, float* s2, float* var, const float* img, float wx, float wy, float wz, size_t nBatches,
+             size_t nx, size_t ny, size_t nz, float eps);
+/* No counterpart in the reference (it composes tv_sqs and array arithmetic in
+ * Python, recon/cupy/sqs_algorithms.py:40-48): one fused SQS step,
+ * out = img - (acc*bp + beta*s1(img)) / (norm + beta*s2(img)); var(img) written
+ * when var is not NULL.  out must not alias img.  Device arrays. */
+int ctpTVSQSUpdate3D(float* out, float* var, const float* img, const float* bp, const float* norm, float beta,
+                     float acc, float wx, float wy, float wz, size_t nBatches, size_t nx, size_t ny, size_t nz,
+                     float eps);
+
 /* ---- library services (no counterpart in the reference) ------------------ */
 /* Text of the calling thread's last failure ("" if none). */
 const char* ctpGetLastErrorMessage(void);

@@ -216,3 +216,48 @@ def fan_fbp_bp(p, prj, angles, prec='d'):
         _ptr(img), _ptr(prj), _ptr(angles), _u(prj.shape[0]), *_grid(p, img.shape),
         *_det(p, prj.shape[3], prj.shape[2], prj.shape[1], du=np.float32(p.du / p.dsd)), _f(p.dsd), _f(p.dso))
     return img
+
+
+# ------------------------------------------------------------------- TV prior
+def tv_sqs(img, weights, eps=1e-8):
+    """Anisotropic TV surrogate of the reference's prior-cuda/TotalVariation.cu
+    (Variation3DKernel :21-55, TVSQS3DKernel :57-139), in FP64 numpy.
+
+    img [B, nz, ny, nx]; weights (wz, wy, wx) as in
+    src/ct_projector/prior/numpy/recon_prior.py:166-206 (the wrapper passes
+    weights[2], [1], [0] as wx, wy, wz).  Returns (s1, s2, var)."""
+    x = np.asarray(img, np.float64)
+    wz, wy, wx = (float(np.float32(w)) for w in weights)
+    eps = float(np.float32(eps))
+
+    def back(a, axis):
+        # a - a[i-1] along axis, 0 at the first index (the lower edge)
+        d = np.zeros_like(a)
+        sl = [slice(None)] * a.ndim
+        sl[axis] = slice(1, None)
+        sp = [slice(None)] * a.ndim
+        sp[axis] = slice(None, -1)
+        d[tuple(sl)] = a[tuple(sl)] - a[tuple(sp)]
+        return d
+
+    def fwd(a, axis):
+        # (a - a[i+1], value of b at i+1) along axis, (0, b[i]) at the last index
+        d = np.zeros_like(a)
+        sl = [slice(None)] * a.ndim
+        sl[axis] = slice(None, -1)
+        sn = [slice(None)] * a.ndim
+        sn[axis] = slice(1, None)
+        d[tuple(sl)] = a[tuple(sl)] - a[tuple(sn)]
+        return d, tuple(sl), tuple(sn)
+
+    dx0, dy0, dz0 = back(x, 3), back(x, 2), back(x, 1)
+    var = np.sqrt(wx * dx0 ** 2 + wy * dy0 ** 2 + wz * dz0 ** 2 + eps)
+    s1 = (wx * dx0 + wy * dy0 + wz * dz0) / var
+    s2 = (wx + wy + wz) / var
+    for axis, w in ((3, wx), (2, wy), (1, wz)):
+        d1, sl, sn = fwd(x, axis)
+        vn = var.copy()                  # the voxel's own variation at the upper edge
+        vn[sl] = var[sn]
+        s1 = s1 + w * d1 / vn
+        s2 = s2 + w / vn
+    return s1, s2, var

@@ -26,6 +26,7 @@ SYMBOLS = [
     'cDistanceDrivenTomoBackprojection', 'cupyDistanceDrivenTomoBackprojection',
     'cfbpFanFilter', 'cupyfbpFanFilter', 'cfbpParallelFilter', 'cupyfbpParallelFilter',
     'cfbpFanBackprojection', 'cupyfbpFanBackprojection',
+    'ctpDistanceDrivenFanProjectionAdjoint', 'ctpDistanceDrivenParallelProjectionAdjoint',
     'cupyTVSQS3D', 'cTVSQS3D', 'ctpTVSQSUpdate3D',
     'ctpGetLastErrorMessage', 'ctpReleaseWorkspace',
     'ctpSetKernelTiming', 'ctpResetKernelTimes', 'ctpGetKernelTime', 'ctpKernelTimeReport',

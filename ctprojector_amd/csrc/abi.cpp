@@ -341,6 +341,35 @@ CTP_EXPORT int cDistanceDrivenParallelBackprojection(float* img, const float* pr
                                   false);
 }
 
+// img += A^T prj, the exact transpose of the distance-driven forward projector
+// (device pointers; no counterpart in the reference): for gradients.
+static int dd_adjoint_entry(const char* name, float* img, const float* prj, const float* deg, size_t nBatches,
+                            size_t nx, size_t ny, size_t nz, float dx, float dy, float dz, float cx, float cy, float cz,
+                            size_t nu, size_t nv, size_t nview, float da, float dv, float off_a, float off_v, float dsd,
+                            float dso, int type, bool fan) {
+    return guarded(name, [&] {
+        if (empty(nBatches, nx, ny, nz, nu, nv, nview)) return;
+        Grid g = make_grid(nx, ny, nz, dx, dy, dz, cx, cy, cz);
+        Det d = make_det(nu, nv, nview, da, dv, off_a, off_v);
+        need(nBatches <= kIntMax, "batch too large");
+        need_ptr(img, "img"); need_ptr(prj, "prj"); need_ptr(deg, "angles");
+        std::vector<float> hdeg = host_copy(deg, nview);
+        dd2d_fp_adjoint(img, prj, hdeg.data(), (int)nBatches, g, d, (int)nview, dsd, dso, type, fan, nullptr);
+        sync();
+    });
+}
+
+CTP_EXPORT int ctpDistanceDrivenFanProjectionAdjoint(float* img, const float* prj, const float* deg, DD_GRID_ARGS,
+                                                     float dsd, float dso, int typeProjector) {
+    return dd_adjoint_entry("ctpDistanceDrivenFanProjectionAdjoint", img, prj, deg, nBatches, nx, ny, nz, dx, dy, dz,
+                            cx, cy, cz, nu, nv, nview, da, dv, off_a, off_v, dsd, dso, typeProjector, true);
+}
+CTP_EXPORT int ctpDistanceDrivenParallelProjectionAdjoint(float* img, const float* prj, const float* deg, DD_GRID_ARGS,
+                                                          int typeProjector) {
+    return dd_adjoint_entry("ctpDistanceDrivenParallelProjectionAdjoint", img, prj, deg, nBatches, nx, ny, nz, dx, dy,
+                            dz, cx, cy, cz, nu, nv, nview, da, dv, off_a, off_v, 1000.f, 500.f, typeProjector, false);
+}
+
 // ===========================================================================
 // Distance driven tomosynthesis (cuda/distanceDriven3D.cu:184-439)
 // ===========================================================================

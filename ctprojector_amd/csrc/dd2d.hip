@@ -851,4 +851,224 @@ void dd2d_bp(float* img, const float* prj, const float* deg_host, int B, const G
     CTP_CHECK(hipStreamSynchronize(stream));
 }
 
+// ---------------------------------------------------------------------------
+// exact adjoint of the forward projector (no counterpart in the reference)
+// ---------------------------------------------------------------------------
+// img += A^T y for the forward projector above (BoxInt or branchless, fan or
+// parallel, blend band, z interpolation, force-FBP reverse weight), for
+// gradients.  The reference's backprojector is not the transpose of its
+// forward projector (different footprints and weights), so it cannot serve as
+// the exact gradient.  Deterministic gather: one lane per pixel, CH slice
+// channels; per view and trace the pixel's edge interval is projected onto the
+// detector to find the candidate columns, and each candidate's weight is
+// recomputed from the same FP32 line parameters the forward projector uses.
+struct FPLine {            // per (view, column): edge lines of both traces and their norms
+    float r1, o1, b1, r2, o2, b2, ny;   // trace along y: x = r * (t - o) + b
+    float q1, p1, c1, q2, p2, c2, nx;   // trace along x: y = q * (t - p) + c
+    float pad0, pad1;
+};
+
+template <bool FAN>
+__global__ __launch_bounds__(256) void dd_fp_lines_kernel(FPLine* __restrict__ lines, const DDView* __restrict__ views,
+                                                          Grid g, Det d, int nview, float dsd, float dso) {
+    const int iu = blockIdx.x * 256 + threadIdx.x, view = blockIdx.y;
+    if (iu >= d.nu) return;
+    const DDView V = views[view];
+    float2 s1, s2, d1, d2;
+    const float a = fp_cell_edges<FAN>(iu, V, g, d, dsd, dso, s1, s2, d1, d2);
+    FPLine L{};
+    if (V.flags & 1) {
+        L.r1 = (d1.x - s1.x) / (d1.y - s1.y); L.o1 = s1.y; L.b1 = s1.x;
+        L.r2 = (d2.x - s2.x) / (d2.y - s2.y); L.o2 = s2.y; L.b2 = s2.x;
+        L.ny = V.wx * g.dy / fabsf(cosf(V.deg - a));
+    }
+    if (V.flags & 2) {
+        L.q1 = (d1.y - s1.y) / (d1.x - s1.x); L.p1 = s1.x; L.c1 = s1.y;
+        L.q2 = (d2.y - s2.y) / (d2.x - s2.x); L.p2 = s2.x; L.c2 = s2.y;
+        L.nx = (1.f - V.wx) * g.dx / fabsf(sinf(V.deg - a));
+    }
+    lines[(size_t)view * d.nu + iu] = L;
+}
+
+// G[view][u][c] = rev * sum over the detector rows of slice channel c of their
+// z-interpolation weight times y (the transpose of the forward projector's
+// per-row interpolation of the slice accumulators)
+struct ChanRow {
+    int iv;
+    float w;
+};
+
+__global__ __launch_bounds__(256) void dd_adj_rows_kernel(float* __restrict__ G, const float* __restrict__ prj,
+                                                          const DDView* __restrict__ views,
+                                                          const int* __restrict__ cstart, const ChanRow* __restrict__ crow,
+                                                          int C, int cs, int nz, Det d, int nview, bool rev) {
+    const int u = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int c = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int view = blockIdx.z;
+    if (u >= d.nu || c >= cs) return;
+    float sum = 0.f;
+    if (c < C) {
+        const int b = c / nz;
+        const float* pb = prj + ((size_t)b * nview + view) * d.nv * d.nu + u;
+        for (int k = cstart[c]; k < cstart[c + 1]; k++) sum = fmaf(crow[k].w, pb[(size_t)crow[k].iv * d.nu], sum);
+        if (rev) sum *= views[view].rev;
+    }
+    G[((size_t)view * d.nu + u) * cs + c] = sum;
+}
+
+template <int CH>
+__device__ inline void adj_apply(float* acc, const float* __restrict__ p, float w) {
+    if constexpr (CH % 4 == 0) {
+#pragma unroll
+        for (int j = 0; j < CH; j += 4) {
+            const float4 q = *reinterpret_cast<const float4*>(p + j);
+            acc[j] = fmaf(w, q.x, acc[j]);
+            acc[j + 1] = fmaf(w, q.y, acc[j + 1]);
+            acc[j + 2] = fmaf(w, q.z, acc[j + 2]);
+            acc[j + 3] = fmaf(w, q.w, acc[j + 3]);
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < CH; j++) acc[j] = fmaf(w, p[j], acc[j]);
+    }
+}
+
+// detector column coordinate (column iu spans [iu, iu + 1]) of a physical point
+template <bool FAN>
+__device__ inline float adj_ucoord(float px, float py, float c, float s, const Det& d, float dso) {
+    const float half = (float)d.nu / 2.0f;
+    if (FAN) return -(atanf((px * c + py * s) / (-px * s + py * c + dso)) / d.du + d.off_u) + half;
+    return -((px * c + py * s) / d.du + d.off_u) + half;
+}
+
+template <int CH, bool FAN>
+__global__ __launch_bounds__(256) void dd_fp_adjoint_kernel(float* __restrict__ img, float* __restrict__ part,
+                                                            const float* __restrict__ G, const FPLine* __restrict__ lines,
+                                                            const DDView* __restrict__ views, int C, int cs, int nchunk,
+                                                            int vpg, Grid g, Det d, int nview, float dso) {
+    const int ix = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int iy = blockIdx.y * 16 + (threadIdx.x >> 4);
+    const int grp = blockIdx.z / nchunk;
+    const int c0 = (blockIdx.z - grp * nchunk) * CH;
+    const int vbeg = grp * vpg, vend = min(nview, vbeg + vpg);
+    if (ix >= g.nx || iy >= g.ny) return;
+    // pixel edges in physical coordinates
+    const float xa = ((float)ix - g.nx / 2.f) * g.dx + g.cx, xb = xa + g.dx;
+    const float ya = ((float)iy - g.ny / 2.f) * g.dy + g.cy, yb = ya + g.dy;
+    const float xc = xa + g.dx / 2, yc = ya + g.dy / 2;
+    float acc[CH];
+#pragma unroll
+    for (int j = 0; j < CH; j++) acc[j] = 0.f;
+    for (int view = vbeg; view < vend; view++) {
+        const DDView V = views[view];
+        const FPLine* __restrict__ Lv = lines + (size_t)view * d.nu;
+        const float* __restrict__ Gv = G + (size_t)view * d.nu * cs + c0;
+#pragma unroll
+        for (int tr = 0; tr < 2; tr++) {
+            if (!(V.flags & (1 << tr))) continue;
+            // the pixel's interval on its traced row: [ix, ix+1] at y = iy + 0.5 (tr 0)
+            // or [iy, iy+1] at x = ix + 0.5 (tr 1), in index units
+            const float ua = tr == 0 ? adj_ucoord<FAN>(xa, yc, V.c, V.s, d, dso) : adj_ucoord<FAN>(xc, ya, V.c, V.s, d, dso);
+            const float ub = tr == 0 ? adj_ucoord<FAN>(xb, yc, V.c, V.s, d, dso) : adj_ucoord<FAN>(xc, yb, V.c, V.s, d, dso);
+            const int k0 = max((int)floorf(fminf(ua, ub)) - 1, 0);
+            const int k1 = min((int)floorf(fmaxf(ua, ub)) + 1, d.nu - 1);
+            const float t = tr == 0 ? (float)iy + 0.5f : (float)ix + 0.5f;
+            const int n = tr == 0 ? g.nx : g.ny;
+            const float lo = tr == 0 ? (float)ix : (float)iy;
+            for (int k = k0; k <= k1; k++) {
+                const FPLine L = Lv[k];
+                float x1, x2, sc;
+                if (tr == 0) { x1 = fmaf(L.r1, t - L.o1, L.b1); x2 = fmaf(L.r2, t - L.o2, L.b2); sc = L.ny; }
+                else { x1 = fmaf(L.q1, t - L.p1, L.c1); x2 = fmaf(L.q2, t - L.p2, L.c2); sc = L.nx; }
+                // the forward projector's IntegralBoxX weight of this pixel (dd_line)
+                if (x1 > x2) { const float tt = x1; x1 = x2; x2 = tt; }
+                const float a = clampf(x1, 0.f, (float)n), b = clampf(x2, 0.f, (float)n);
+                if (!(b > a)) continue;
+                const float w = (fminf(b, lo + 1.f) - fmaxf(a, lo)) * (sc / (x2 - x1));
+                if (w > 0.f) adj_apply<CH>(acc, Gv + (size_t)k * cs, w);
+            }
+        }
+    }
+    store_bp_result<CH>(img, part, acc, grp, c0, C, (size_t)g.nx * g.ny, (size_t)iy * g.nx + ix);
+}
+
+void dd2d_fp_adjoint(float* img, const float* prj, const float* deg_host, int B, const Grid& g, const Det& d,
+                     int nview, float dsd, float dso, int type, bool fan, hipStream_t stream) {
+    const bool branchless = (type & DD_BRANCHLESS) != 0;
+    const int C = B * g.nz;
+    const int CH = pick_ch(C);
+    const int nchunk = ceil_div(C, CH);
+    const int cs = nchunk * CH;
+    std::vector<DDView> views = make_dd_views(deg_host, nview, g);
+    if (branchless) hard_switch(views);
+    const bool rev = !fan && (type & DD_FORCE_FBP) != 0 && !branchless;
+    // channel -> (detector row, weight) lists: the transpose of the forward
+    // projector's row table (same z mapping as dd2d_fp)
+    std::vector<std::vector<ChanRow>> per(C);
+    for (int b = 0; b < B; b++)
+        for (int iv = 0; iv < d.nv; iv++) {
+            const double zr = ((double)iv - (d.nv - 1) / 2.0 + d.off_v) * d.dv;
+            double z = (zr - g.cz) / g.dz + g.nz / 2.0 - 0.5;
+            z = std::min((double)g.nz, std::max(0.0, z));
+            int iz = (int)z, iz1 = iz + 1;
+            iz = clampi(iz, 0, g.nz);
+            iz1 = clampi(iz1, 0, g.nz);
+            const float w1 = (float)(z - iz);
+            per[b * g.nz + iz].push_back(ChanRow{iv, 1.f - w1});
+            per[b * g.nz + iz1].push_back(ChanRow{iv, w1});
+        }
+    std::vector<int> cstart(C + 1, 0);
+    std::vector<ChanRow> crow;
+    for (int c = 0; c < C; c++) {
+        cstart[c] = (int)crow.size();
+        crow.insert(crow.end(), per[c].begin(), per[c].end());
+    }
+    cstart[C] = (int)crow.size();
+    const size_t npix = (size_t)g.nx * g.ny;
+    const int groups = bp_view_groups(npix, nchunk, nview);
+    const int vpg = ceil_div(nview, groups);
+    Workspace& ws = workspace_for(stream);
+    Plan pl;
+    pl.add<FPLine>((size_t)nview * d.nu);
+    pl.add<float>((size_t)nview * d.nu * cs);
+    pl.add<DDView>(nview);
+    pl.add<int>(C + 1);
+    pl.add<ChanRow>(std::max<size_t>(crow.size(), 1));
+    if (groups > 1) pl.add<float>((size_t)groups * C * npix);
+    ws.reserve(pl.bytes);
+    FPLine* dlines = ws.take_n<FPLine>((size_t)nview * d.nu);
+    float* G = ws.take_n<float>((size_t)nview * d.nu * cs);
+    DDView* dviews = ws.take_n<DDView>(nview);
+    int* dcstart = ws.take_n<int>(C + 1);
+    ChanRow* dcrow = ws.take_n<ChanRow>(std::max<size_t>(crow.size(), 1));
+    float* part = groups > 1 ? ws.take_n<float>((size_t)groups * C * npix) : nullptr;
+    CTP_CHECK(hipMemcpyAsync(dviews, views.data(), sizeof(DDView) * nview, hipMemcpyHostToDevice, stream));
+    CTP_CHECK(hipMemcpyAsync(dcstart, cstart.data(), sizeof(int) * (C + 1), hipMemcpyHostToDevice, stream));
+    if (!crow.empty())
+        CTP_CHECK(hipMemcpyAsync(dcrow, crow.data(), sizeof(ChanRow) * crow.size(), hipMemcpyHostToDevice, stream));
+    TimedLaunch t(fan ? "dd_fan_fp_adjoint" : "dd_par_fp_adjoint", stream);
+    if (fan) dd_fp_lines_kernel<true><<<dim3(ceil_div(d.nu, 256), nview), 256, 0, stream>>>(dlines, dviews, g, d, nview, dsd, dso);
+    else dd_fp_lines_kernel<false><<<dim3(ceil_div(d.nu, 256), nview), 256, 0, stream>>>(dlines, dviews, g, d, nview, dsd, dso);
+    CTP_LAUNCH_CHECK();
+    dd_adj_rows_kernel<<<dim3(ceil_div(d.nu, 64), ceil_div(cs, 4), nview), 256, 0, stream>>>(
+        G, prj, dviews, dcstart, dcrow, C, cs, g.nz, d, nview, rev);
+    CTP_LAUNCH_CHECK();
+    dim3 grid(ceil_div(g.nx, 16), ceil_div(g.ny, 16), nchunk * groups), block(256);
+#define CTP_ADJ_CASE(N)                                                                                         \
+    case N:                                                                                                     \
+        if (fan) dd_fp_adjoint_kernel<N, true><<<grid, block, 0, stream>>>(img, part, G, dlines, dviews, C, cs, nchunk, \
+                                                                           vpg, g, d, nview, dso);              \
+        else dd_fp_adjoint_kernel<N, false><<<grid, block, 0, stream>>>(img, part, G, dlines, dviews, C, cs, nchunk, \
+                                                                        vpg, g, d, nview, dso);                 \
+        break;
+    switch (CH) {
+        CTP_ADJ_CASE(1) CTP_ADJ_CASE(2) CTP_ADJ_CASE(4) CTP_ADJ_CASE(8) CTP_ADJ_CASE(16) CTP_ADJ_CASE(32)
+        default: fail("dd adjoint: bad channel chunk");
+    }
+#undef CTP_ADJ_CASE
+    CTP_LAUNCH_CHECK();
+    if (part) reduce_parts(img, part, groups, (size_t)C * npix, stream);
+    CTP_CHECK(hipStreamSynchronize(stream));
+}
+
 }  // namespace ctp

@@ -48,6 +48,9 @@ void dd2d_fp(float* prj, const float* img, const float* deg_host, int B, const G
              float dsd, float dso, int type, bool fan, hipStream_t stream);
 void dd2d_bp(float* img, const float* prj, const float* deg_host, int B, const Grid& g, const Det& d, int nview,
              float dsd, float dso, int type, bool fan, hipStream_t stream);
+// img += A^T prj, the exact transpose of dd2d_fp (same type bits)
+void dd2d_fp_adjoint(float* img, const float* prj, const float* deg_host, int B, const Grid& g, const Det& d,
+                     int nview, float dsd, float dso, int type, bool fan, hipStream_t stream);
 
 // ---- pixel-driven backprojection helpers shared by dd2d.hip and fbp.hip ----
 // view groups for a launch over npix pixels x nchunk channel chunks

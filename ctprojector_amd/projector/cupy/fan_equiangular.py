@@ -83,3 +83,20 @@ def distance_driven_bp(projector, prj, angles, is_fbp: Union[bool, int] = False,
     if err != 0:
         print(err)
     return img
+
+
+def distance_driven_fp_adjoint(projector, prj, angles, branchless: Union[bool, int] = False) -> torch.Tensor:
+    """Exact transpose of ``distance_driven_fp`` (the gradient of the forward
+    projection; the reference's backprojector is not this transpose).  No
+    counterpart in the reference."""
+    dev = device_of(prj, angles)
+    prj = dev_f32(prj, dev)
+    angles = dev_f32(angles, dev)
+    img = torch.zeros([prj.shape[0], projector.nz, projector.ny, projector.nx], dtype=torch.float32, device=dev)
+    with on_device(dev):
+        err = _lib.module().ctpDistanceDrivenFanProjectionAdjoint(
+            ptr(img), ptr(prj), ptr(angles), *_grid(img.shape, projector), *_det(prj.shape, projector),
+            c_int(2 if branchless else 0))
+    if err != 0:
+        raise RuntimeError('ctpDistanceDrivenFanProjectionAdjoint failed (%d): %s' % (err, _lib.last_error_message()))
+    return img

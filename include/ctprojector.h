@@ -106,6 +106,19 @@ int cDistanceDrivenParallelBackprojection(float* img, const float* prj, const fl
                                           float cy, float cz, size_t nu, size_t nv, size_t nview, float da, float dv,
                                           float off_a, float off_v, int typeProjector);
 
+/* No counterpart in the reference (whose backprojector is not the transpose of
+ * its forward projector): img += A^T prj for the fan / parallel forward
+ * projector above with the same typeProjector bits, device arrays; the exact
+ * gradient of the forward projection. */
+int ctpDistanceDrivenFanProjectionAdjoint(float* img, const float* prj, const float* deg, size_t nBatches, size_t nx,
+                                          size_t ny, size_t nz, float dx, float dy, float dz, float cx, float cy,
+                                          float cz, size_t nu, size_t nv, size_t nview, float da, float dv, float off_a,
+                                          float off_v, float dsd, float dso, int typeProjector);
+int ctpDistanceDrivenParallelProjectionAdjoint(float* img, const float* prj, const float* deg, size_t nBatches,
+                                               size_t nx, size_t ny, size_t nz, float dx, float dy, float dz, float cx,
+                                               float cy, float cz, size_t nu, size_t nv, size_t nview, float da,
+                                               float dv, float off_a, float off_v, int typeProjector);
+
 /* ---- distance driven tomosynthesis --------------------------------------- */
 /* cuda/distanceDriven3D.cu:184 */
 int cDistanceDrivenTomoProjection(float* prj, const float* img, const float* detCenter, const float* src,

@@ -1,0 +1,187 @@
+"""Differentiable projectors and unrolled SQS-TV for PyTorch.
+
+The reference exposes its projectors to TensorFlow as ops whose gradients are
+registered in Python: grad(FP) = BP and grad(BP) = FP
+(src/ct_projector/projector/tensorflow/cone.py:193-276,
+circular_2d.py:140-191,392-443).  Here each projector is a
+``torch.autograd.Function`` whose backward is a hand-written HIP kernel:
+
+* Siddon cone beam: the backprojector is the exact transpose of the forward
+  projector, so fp / bp are an exact adjoint pair;
+* distance-driven fan / parallel beam: the reference's backprojector is not the
+  transpose of its forward projector.  ``adjoint='exact'`` (default) uses the
+  library's exact transpose (``ctpDistance*ProjectionAdjoint``) as the gradient
+  of ``fp``; ``adjoint='reference'`` uses the reference's backprojector, as its
+  TensorFlow op does.  ``op.adjoint(y)`` is the exact transpose as a
+  differentiable operator (its gradient is ``fp``);
+* tomosynthesis: the reference's convention (gradient of fp = bp).
+
+Backward passes are themselves built from these Functions, so higher-order
+gradients work.  ``sqs_tv_unrolled`` runs K SQS-TV iterations as a
+differentiable network (TV surrogate in torch arithmetic, projectors through
+the kernels above) with optional per-iteration checkpointing.
+"""
+
+from typing import Callable, Optional, Sequence, Tuple
+
+import torch
+import torch.utils.checkpoint
+
+from .projector.cupy import cone, fan_equiangular, parallel, tomo
+
+
+class _Linear(torch.autograd.Function):
+    """y = A x for a linear map given as a pair (A, A^T) of tensor functions."""
+
+    @staticmethod
+    def forward(ctx, x, fwd: Callable, adj: Callable):
+        ctx.fwd, ctx.adj = fwd, adj
+        return fwd(x)
+
+    @staticmethod
+    def backward(ctx, g):
+        return _Linear.apply(g.contiguous(), ctx.adj, ctx.fwd), None, None
+
+
+def linear(x: torch.Tensor, fwd: Callable, adj: Callable) -> torch.Tensor:
+    """Apply the linear map ``fwd`` with gradient ``adj`` (its transpose)."""
+    return _Linear.apply(x, fwd, adj)
+
+
+class _Op:
+    """fp / bp of one geometry as differentiable tensor functions."""
+
+    def _fp(self, x):
+        raise NotImplementedError
+
+    def _bp(self, y):
+        raise NotImplementedError
+
+    def _adj(self, y):
+        return self._bp(y)
+
+    def fp(self, x: torch.Tensor) -> torch.Tensor:
+        """Forward projection; gradient = the transpose (exact where available)."""
+        return linear(x, self._fp, self._adj)
+
+    def bp(self, y: torch.Tensor) -> torch.Tensor:
+        """The reference's backprojection; gradient = forward projection (the
+        reference's TensorFlow convention)."""
+        return linear(y, self._bp, self._fp)
+
+    def adjoint(self, y: torch.Tensor) -> torch.Tensor:
+        """Exact transpose of fp (= bp where bp is exact); gradient = fp."""
+        return linear(y, self._adj, self._fp)
+
+
+class ConeSiddon(_Op):
+    """Siddon cone beam, arbitrary trajectory (geometry rows (x, y, z))."""
+
+    def __init__(self, projector, det_center, det_u, det_v, src):
+        self.p = projector
+        self.geo = (det_center, det_u, det_v, src)
+
+    def _fp(self, x):
+        return cone.siddon_fp_arbitrary(self.p, x, *self.geo)
+
+    def _bp(self, y):
+        return cone.siddon_bp_arbitrary(self.p, y, *self.geo)
+
+
+class _DD(_Op):
+    mod = None
+
+    def __init__(self, projector, angles, branchless: bool = False, adjoint: str = 'exact'):
+        if adjoint not in ('exact', 'reference'):
+            raise ValueError("adjoint must be 'exact' or 'reference'")
+        self.p, self.angles, self.branchless, self.mode = projector, angles, branchless, adjoint
+
+    def _fp(self, x):
+        return self.mod.distance_driven_fp(self.p, x, self.angles, branchless=self.branchless)
+
+    def _bp(self, y):
+        return self.mod.distance_driven_bp(self.p, y, self.angles, branchless=self.branchless)
+
+    def _exact(self, y):
+        return self.mod.distance_driven_fp_adjoint(self.p, y, self.angles, branchless=self.branchless)
+
+    def fp(self, x):
+        return linear(x, self._fp, self._exact if self.mode == 'exact' else self._bp)
+
+    def adjoint(self, y):
+        return linear(y, self._exact, self._fp)
+
+
+class FanDD(_DD):
+    """Distance-driven equiangular fan beam (projector.du is the arc pitch at dsd)."""
+    mod = fan_equiangular
+
+
+class ParallelDD(_DD):
+    """Distance-driven parallel beam."""
+    mod = parallel
+
+
+class TomoDD(_Op):
+    """Distance-driven tomosynthesis (the reference's convention: grad fp = bp)."""
+
+    def __init__(self, projector, det_center, src, branchless: bool = False):
+        self.p, self.dc, self.src, self.branchless = projector, det_center, src, branchless
+
+    def _fp(self, x):
+        return tomo.distance_driven_fp(self.p, x, self.dc, self.src, branchless=self.branchless)
+
+    def _bp(self, y):
+        return tomo.distance_driven_bp(self.p, y, self.dc, self.src, branchless=self.branchless)
+
+
+# --------------------------------------------------------------------- TV
+def tv_sqs_torch(img: torch.Tensor, weights: Sequence[float] = (1.0, 1.0, 1.0), eps: float = 1e-8
+                 ) -> Tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
+    """The anisotropic TV surrogate (s1, s2, var) of prior-cuda/TotalVariation.cu
+    (the HIP kernel behind ``prior.cupy.tv_sqs``) in torch arithmetic, so that
+    autograd differentiates through it.  weights along (z, y, x)."""
+    wz, wy, wx = (float(w) for w in weights)
+
+    def back(a, dim):
+        return torch.cat([torch.zeros_like(a.narrow(dim, 0, 1)), a.narrow(dim, 1, a.shape[dim] - 1)
+                          - a.narrow(dim, 0, a.shape[dim] - 1)], dim)
+
+    dx0, dy0, dz0 = back(img, 3), back(img, 2), back(img, 1)
+    var = torch.sqrt(wx * dx0 * dx0 + wy * dy0 * dy0 + wz * dz0 * dz0 + eps)
+    s1 = (wx * dx0 + wy * dy0 + wz * dz0) / var
+    s2 = (wx + wy + wz) / var
+    for dim, w in ((3, wx), (2, wy), (1, wz)):
+        n = img.shape[dim]
+        d1 = torch.cat([img.narrow(dim, 0, n - 1) - img.narrow(dim, 1, n - 1),
+                        torch.zeros_like(img.narrow(dim, 0, 1))], dim)
+        vn = torch.cat([var.narrow(dim, 1, n - 1), var.narrow(dim, n - 1, 1)], dim)
+        s1 = s1 + w * d1 / vn
+        s2 = s2 + w / vn
+    return s1, s2, var
+
+
+def sqs_tv_unrolled(op: _Op, prj: torch.Tensor, x0: torch.Tensor, norm_img: torch.Tensor, projector_norm: float,
+                    beta, niter: int, weights: Sequence[float] = (1.0, 1.0, 1.0), eps: float = 1e-8,
+                    weight: Optional[torch.Tensor] = None, checkpoint: bool = True) -> torch.Tensor:
+    """K differentiable SQS-TV iterations (recon/cupy/sqs_algorithms.py:22-57 with
+    prior_func = tv_sqs): x <- x - (A^T w (A x - p) / n^2 + beta s1) / (norm_img + beta s2).
+    ``op.adjoint`` supplies A^T so that the network's gradient is exact.
+    ``checkpoint`` recomputes each iteration in the backward pass instead of
+    keeping its intermediates (memory per iteration: one volume)."""
+    w = 1.0 if weight is None else weight
+    pn = float(projector_norm)
+
+    def step(x, p):
+        r = op.fp(x) / pn - p / pn
+        g = op.adjoint(r * w) / pn
+        s1, s2, _ = tv_sqs_torch(x, weights, eps)
+        return x - (g + beta * s1) / (norm_img + beta * s2)
+
+    x = x0
+    for _ in range(niter):
+        if checkpoint and torch.is_grad_enabled():
+            x = torch.utils.checkpoint.checkpoint(step, x, prj, use_reentrant=False)
+        else:
+            x = step(x, prj)
+    return x

@@ -534,7 +534,9 @@ void dd2d_fp(float* prj, const float* img, const float* deg_host, int B, const G
     const int kreg_max = 8;
     const int wcap = std::min(kreg_max * 64 * kFPViews / (CHP / 4), 65536 / (2 * CHP * 4));
     const double wneed = fp_stage_width(views, groups, g, d, dsd, dso, fan) + 6.0;
-    const bool use_lds = tuning_get("dd_fp_mode", 0) == 0 && wneed <= (double)wcap && groups.size() <= 65535;
+    // the direct kernel is the default (measured faster at 512x512x32: the
+    // staged kernel's prefetch registers spill)
+    const bool use_lds = tuning_get("dd_fp_mode", 1) == 2 && wneed <= (double)wcap && groups.size() <= 65535;
     const int wmax = use_lds ? (int)std::ceil(wneed) : 0;
     int kreg = 1;
     while (use_lds && kreg * 64 * kFPViews < wmax * (CHP / 4)) kreg *= 2;
@@ -755,11 +757,10 @@ __global__ __launch_bounds__(256) void dd_bp_kernel(float* __restrict__ img, flo
 // (one lane per pixel alone gives 4 waves per SIMD at 512x512).  Partial
 // images of the groups are summed in a fixed order afterwards: deterministic.
 int bp_view_groups(size_t npix, int nchunk, int nview) {
-    int k = tuning_get("bp_view_groups", 0);
-    if (k <= 0) {
-        const size_t waves = (npix + 63) / 64 * (size_t)nchunk;
-        k = (int)std::min<size_t>(8, std::max<size_t>(1, 16384 / std::max<size_t>(waves, 1)));
-    }
+    // measured on MI355X: at 512x512x32 the partial images cost more than the
+    // extra waves gain, so one group is the default
+    (void)npix;
+    int k = tuning_get("bp_view_groups", 1);
     k = std::max(1, std::min(k, std::max(1, nview / 8)));
     k = std::max(1, std::min(k, 65535 / std::max(nchunk, 1)));   // grid.z = nchunk * groups
     return k;

@@ -30,8 +30,8 @@ std::map<std::string, int> g_tune = {
     {"cone_bp_zrun", 0},     // voxels along z per thread of the Siddon BP (1, 2, 4, 8)
     {"cone_fp_order", 0},    // Siddon FP block order: 0 XCD row bands, 1 plain view-major
     {"cone_bp_mode", 0},     // Siddon BP: 0 column mode where the geometry allows it, 1 general
-    {"dd_fp_mode", 0},       // DD FP: 0 LDS-staged rows where the footprint fits, 1 direct
-    {"bp_view_groups", 0},   // view groups of the pixel-driven DD / FBP backprojections (1..8)
+    {"dd_fp_mode", 0},       // DD FP: 1 direct (default), 2 LDS-staged rows where the footprint fits
+    {"bp_view_groups", 0},   // view groups of the pixel-driven DD / FBP backprojections (default 1)
 };
 std::mutex g_ws_mutex;
 std::map<std::pair<int, hipStream_t>, std::unique_ptr<Workspace>> g_ws;

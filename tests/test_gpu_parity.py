@@ -283,3 +283,18 @@ def test_bp_view_groups_match_single_group(tuning, fbp):
     tuning('bp_view_groups', 7)
     c = run()
     assert rel_l2(a, b) < 1e-6 and rel_l2(c, b) < 1e-6
+
+
+@pytest.mark.parametrize('fan_beam', [True, False])
+def test_dd_fp_staged_rows_match_direct(oracle, tuning, fan_beam):
+    """The LDS-staged forward projector (dd_fp_mode 2) against the oracle."""
+    tuning('dd_fp_mode', 2)
+    if fan_beam:
+        g, ang, img = small_fan(nz=3, nv=4, off=(0.4, 0.3))
+        fp = fan.distance_driven_fp(_proj(g), img, ang)
+        ref = oracle.dd_fan_fp(g, img, ang)
+    else:
+        g, ang, img = small_parallel(nz=2, nv=2)
+        fp = parallel.distance_driven_fp(_proj(g), img, ang)
+        ref = oracle.dd_par_fp(g, img, ang)
+    assert rel_l2(fp, ref) < TOL

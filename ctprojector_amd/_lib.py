@@ -28,6 +28,7 @@ SYMBOLS = [
     'cfbpFanBackprojection', 'cupyfbpFanBackprojection',
     'ctpDistanceDrivenFanProjectionAdjoint', 'ctpDistanceDrivenParallelProjectionAdjoint',
     'cupyTVSQS3D', 'cTVSQS3D', 'ctpTVSQSUpdate3D',
+    'ctpConePlanCreate', 'ctpConePlanForward', 'ctpConePlanBackward', 'ctpConePlanConfig', 'ctpConePlanDestroy',
     'ctpGetLastErrorMessage', 'ctpReleaseWorkspace',
     'ctpSetKernelTiming', 'ctpResetKernelTimes', 'ctpGetKernelTime', 'ctpKernelTimeReport',
     'ctpSetTuning', 'ctpGetTuning',

@@ -1,0 +1,130 @@
+"""Batched cone-beam projection: many volumes of different sizes, each with its
+own trajectory and detector, forward- and back-projected by one persistent
+launch per direction (``ctpConePlan*`` of libprojector.so), with launch shapes
+chosen by the library's autotuner and the launches capturable in a HIP graph.
+
+The reference projects a batch by looping over its items on the host with a
+launch and a device synchronisation per item (cuda/siddonCone.cu:76-135); its
+batches must share one geometry and one size.
+
+    jobs = [ConeJob(projector_a, geo_a), ConeJob(projector_b, geo_b), ...]
+    batch = ConeBatch(jobs, autotune=True)
+    batch.img[0].copy_(volume_a); ...
+    batch.forward()                       # batch.prj[j] = A_j batch.img[j]
+    g = batch.capture('forward')          # HIP graph of the same launches
+    g.replay()
+
+Buffers ``img[j]`` ([nz, ny, nx]) and ``prj[j]`` ([nview, nv, nu]) are device
+tensors owned by the batch and bound to the plan; forward overwrites prj,
+backward overwrites img.  Calls are asynchronous on torch's current stream.
+"""
+
+import ctypes
+from ctypes import c_float, c_int, c_size_t, c_void_p
+from dataclasses import dataclass
+from typing import List, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib
+
+
+class _CJob(ctypes.Structure):
+    _fields_ = [('nx', c_size_t), ('ny', c_size_t), ('nz', c_size_t),
+                ('dx', c_float), ('dy', c_float), ('dz', c_float), ('cx', c_float), ('cy', c_float), ('cz', c_float),
+                ('nu', c_size_t), ('nv', c_size_t), ('nview', c_size_t),
+                ('du', c_float), ('dv', c_float), ('off_u', c_float), ('off_v', c_float),
+                ('det_center', c_void_p), ('det_u', c_void_p), ('det_v', c_void_p), ('src', c_void_p),
+                ('img', c_void_p), ('prj', c_void_p)]
+
+
+@dataclass
+class ConeJob:
+    """One volume: ``projector`` carries nx..nz, dx..cz, nu, nv, du, dv, off_u,
+    off_v (the reference's ct_projector attributes); ``geometry`` has
+    det_center, det_u, det_v, src as [nview, 3] arrays (x, y, z)."""
+    projector: object
+    geometry: object
+
+
+def _f32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+
+
+class ConeBatch:
+    def __init__(self, jobs: Sequence[ConeJob], autotune: bool = False, device=None):
+        self.device = torch.device('cuda', torch.cuda.current_device()) if device is None else torch.device(device)
+        self.img: List[torch.Tensor] = []
+        self.prj: List[torch.Tensor] = []
+        self._host = []
+        cj = (_CJob * len(jobs))()
+        for k, job in enumerate(jobs):
+            p, geo = job.projector, job.geometry
+            arrs = [_f32(geo.det_center), _f32(geo.det_u), _f32(geo.det_v), _f32(geo.src)]
+            nview = arrs[0].shape[0]
+            self._host.append(arrs)
+            img = torch.zeros([p.nz, p.ny, p.nx], dtype=torch.float32, device=self.device)
+            prj = torch.zeros([nview, p.nv, p.nu], dtype=torch.float32, device=self.device)
+            self.img.append(img)
+            self.prj.append(prj)
+            cj[k] = _CJob(p.nx, p.ny, p.nz, p.dx, p.dy, p.dz, p.cx, p.cy, p.cz, p.nu, p.nv, nview,
+                          p.du, p.dv, p.off_u, p.off_v, *[a.ctypes.data for a in arrs],
+                          img.data_ptr(), prj.data_ptr())
+        self._plan = c_void_p()
+        torch.cuda.synchronize(self.device)
+        with torch.cuda.device(self.device):
+            err = _lib.module().ctpConePlanCreate(ctypes.byref(self._plan), cj, c_int(len(jobs)),
+                                                  c_int(1 if autotune else 0))
+        if err != 0:
+            raise RuntimeError('ctpConePlanCreate failed (%d): %s' % (err, _lib.last_error_message()))
+
+    def _stream(self):
+        return c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def forward(self) -> None:
+        """prj[j] = A_j img[j] for every job (asynchronous on the current stream)."""
+        with torch.cuda.device(self.device):
+            err = _lib.module().ctpConePlanForward(self._plan, self._stream())
+        if err != 0:
+            raise RuntimeError('ctpConePlanForward failed (%d): %s' % (err, _lib.last_error_message()))
+
+    def backward(self) -> None:
+        """img[j] = A_j^T prj[j] for every job (asynchronous on the current stream)."""
+        with torch.cuda.device(self.device):
+            err = _lib.module().ctpConePlanBackward(self._plan, self._stream())
+        if err != 0:
+            raise RuntimeError('ctpConePlanBackward failed (%d): %s' % (err, _lib.last_error_message()))
+
+    def config(self) -> dict:
+        """Launch shapes in use (after autotuning): forward tile width, XCD block
+        order, backprojection z-run."""
+        a, b, c = c_int(), c_int(), c_int()
+        _lib.module().ctpConePlanConfig(self._plan, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        return {'cone_fp_tile_u': a.value, 'cone_fp_xcd_order': b.value, 'cone_bp_zrun': c.value}
+
+    def capture(self, which: str = 'forward') -> torch.cuda.CUDAGraph:
+        """A HIP graph of ``forward``, ``backward`` or ``both`` (forward then
+        backward); replay() reruns it on the bound buffers."""
+        steps = {'forward': [self.forward], 'backward': [self.backward],
+                 'both': [self.forward, self.backward]}[which]
+        for f in steps:                      # warm-up outside the capture
+            f()
+        torch.cuda.synchronize(self.device)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            for f in steps:
+                f()
+        return g
+
+    def close(self) -> None:
+        if getattr(self, '_plan', None) is not None and self._plan.value:
+            torch.cuda.synchronize(self.device)
+            _lib.module().ctpConePlanDestroy(self._plan)
+            self._plan = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -583,3 +583,79 @@ CTP_EXPORT int ctpTVSQSUpdate3D(float* out, float* var, const float* img, const 
         sync();
     });
 }
+
+// ===========================================================================
+// Batched cone-beam plans (no counterpart in the reference)
+// ===========================================================================
+struct ctpConeJob {
+    size_t nx, ny, nz;
+    float dx, dy, dz, cx, cy, cz;
+    size_t nu, nv, nview;
+    float du, dv, off_u, off_v;
+    const float* det_center;
+    const float* det_u;
+    const float* det_v;
+    const float* src;
+    float* img;
+    float* prj;
+};
+
+CTP_EXPORT int ctpConePlanCreate(void** plan, const ctpConeJob* jobs, int njobs, int autotune) {
+    return guarded("ctpConePlanCreate", [&] {
+        need_ptr(plan, "plan");
+        *plan = nullptr;
+        need(njobs > 0 && jobs != nullptr, "no jobs");
+        std::vector<ConeJobSpec> specs;
+        for (int j = 0; j < njobs; j++) {
+            const ctpConeJob& J = jobs[j];
+            need(!empty(1, J.nx, J.ny, J.nz, J.nu, J.nv, J.nview), "empty job");
+            ConeJobSpec S{};
+            S.g = make_grid(J.nx, J.ny, J.nz, J.dx, J.dy, J.dz, J.cx, J.cy, J.cz);
+            S.d = make_det(J.nu, J.nv, J.nview, J.du, J.dv, J.off_u, J.off_v);
+            S.nview = (int)J.nview;
+            need_ptr(J.det_center, "det_center"); need_ptr(J.det_u, "det_u");
+            need_ptr(J.det_v, "det_v"); need_ptr(J.src, "src");
+            need_ptr(J.img, "img"); need_ptr(J.prj, "prj");
+            S.det_center = J.det_center; S.det_u = J.det_u; S.det_v = J.det_v; S.src = J.src;
+            S.img = J.img;
+            S.prj = J.prj;
+            specs.push_back(S);
+        }
+        touch();
+        *plan = new ConePlan(specs, autotune != 0);
+    });
+}
+
+CTP_EXPORT int ctpConePlanForward(void* plan, void* stream) {
+    return guarded("ctpConePlanForward", [&] {
+        need_ptr(plan, "plan");
+        static_cast<ConePlan*>(plan)->forward(static_cast<hipStream_t>(stream));
+    });
+}
+
+CTP_EXPORT int ctpConePlanBackward(void* plan, void* stream) {
+    return guarded("ctpConePlanBackward", [&] {
+        need_ptr(plan, "plan");
+        static_cast<ConePlan*>(plan)->backward(static_cast<hipStream_t>(stream));
+    });
+}
+
+CTP_EXPORT int ctpConePlanConfig(void* plan, int* tile_u, int* xcd_order, int* zrun) {
+    return guarded("ctpConePlanConfig", [&] {
+        need_ptr(plan, "plan");
+        int a, b, c;
+        static_cast<ConePlan*>(plan)->config(a, b, c);
+        if (tile_u) *tile_u = a;
+        if (xcd_order) *xcd_order = b;
+        if (zrun) *zrun = c;
+    });
+}
+
+CTP_EXPORT int ctpConePlanDestroy(void* plan) {
+    return guarded("ctpConePlanDestroy", [&] {
+        if (!plan) return;
+        touch();
+        CTP_CHECK(hipDeviceSynchronize());
+        delete static_cast<ConePlan*>(plan);
+    });
+}

@@ -5,6 +5,9 @@
 
 #include "common.h"
 
+#include <memory>
+#include <vector>
+
 namespace ctp {
 
 struct ConeGeomDev {               // [nview][3] float rows (x, y, z), device
@@ -51,6 +54,29 @@ void dd2d_bp(float* img, const float* prj, const float* deg_host, int B, const G
 // img += A^T prj, the exact transpose of dd2d_fp (same type bits)
 void dd2d_fp_adjoint(float* img, const float* prj, const float* deg_host, int B, const Grid& g, const Det& d,
                      int nview, float dsd, float dso, int type, bool fan, hipStream_t stream);
+
+// ---- batched cone-beam plans (no counterpart in the reference) ----
+struct ConeJobSpec {
+    Grid g;
+    Det d;
+    int nview;
+    const float *det_center, *det_u, *det_v, *src;   // host [nview][3], read at creation
+    float* img;                                      // device [nz][ny][nx], bound
+    float* prj;                                      // device [nview][nv][nu], bound
+};
+
+class ConePlan {
+public:
+    ConePlan(const std::vector<ConeJobSpec>& jobs, bool autotune);
+    ~ConePlan();
+    void forward(hipStream_t s);    // prj = A img for every job (overwrites)
+    void backward(hipStream_t s);   // img = A^T prj for every job (overwrites)
+    void tune();
+    void config(int& tile_u, int& xcd_order, int& zrun) const;
+    struct Impl;
+private:
+    std::unique_ptr<Impl> impl_;
+};
 
 // ---- pixel-driven backprojection helpers shared by dd2d.hip and fbp.hip ----
 // view groups for a launch over npix pixels x nchunk channel chunks

@@ -182,6 +182,12 @@ std::string timing_report() {
 
 TimedLaunch::TimedLaunch(const char* name, hipStream_t s) : name_(name), s_(s) {
     if (!timing_enabled()) return;
+    // no event records into a stream that is being captured into a graph
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s_, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) {
+        (void)hipGetLastError();
+        return;
+    }
     if (hipEventCreate(&a_) != hipSuccess || hipEventCreate(&b_) != hipSuccess) {
         (void)hipGetLastError();
         a_ = b_ = nullptr;

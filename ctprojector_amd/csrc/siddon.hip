@@ -177,34 +177,18 @@ __device__ inline float3 cell_centre(const float* pdc, const float* pdu, const f
     return make_float3(c.x + U.x * u + V.x * v, c.y + U.y * u + V.y * v, c.z + U.z * u + V.z * v);
 }
 
-template <int TU, int TV>
-__global__ __launch_bounds__(256) void siddon_cone_fp_kernel(float* __restrict__ prj, const float* __restrict__ img,
-                                                             const float* __restrict__ imgT, const int* __restrict__ swapf,
-                                                             const float* __restrict__ pdc, const float* __restrict__ pdu,
-                                                             const float* __restrict__ pdv, const float* __restrict__ psrc,
-                                                             Grid g, Det d, int nview, int nub, int ntile,
-                                                             int xcd_order) {
+// One 4-wavefront block of rays (a TU x 4TV detector tile) of one view.
+// ACC: prj += line integral for rays that hit the grid (the reference's
+// semantics, cuda/siddon.cu:230); otherwise prj = line integral (0 on a miss).
+template <int TU, int TV, bool ACC>
+__device__ inline void siddon_fp_block(float* __restrict__ prj, const float* __restrict__ img,
+                                       const float* __restrict__ imgT, int swap, const float* __restrict__ pdc,
+                                       const float* __restrict__ pdu, const float* __restrict__ pdv,
+                                       const float* __restrict__ psrc, Grid g, const Det& d, int view, int ub, int vb) {
     static_assert(TU * TV == 64, "one wavefront per tile");
-    // Block -> (view, detector tile).  Workgroups are dispatched round-robin
-    // over the 8 XCDs (each with its own L2).  In XCD order, XCD x receives the
-    // x-th eighth of every view's tiles in row-major order, i.e. a band of
-    // detector rows whose rays cross one slab of the volume, and walks through
-    // the views with it; otherwise tiles are in plain view-major order.
-    const int per = (ntile + 7) >> 3;                 // tiles per XCD and view
-    int view, tile;
-    if (xcd_order) {
-        const int xcd = blockIdx.x & 7, j = blockIdx.x >> 3;
-        view = j / per;
-        tile = xcd * per + (j - view * per);
-    } else {
-        view = blockIdx.x / (8 * per);
-        tile = blockIdx.x - view * 8 * per;
-    }
-    if (tile >= ntile || view >= nview) return;
-    const int ub = tile % nub, vb = tile / nub;
-    // per-view layout choice and the volume's buffer resource, built before any
-    // divergent branch so that they stay in scalar registers (one view per block)
-    const int sw = __builtin_amdgcn_readfirstlane(swapf[view]);
+    // layout choice and the volume's buffer resource, built before any divergent
+    // branch so that they stay in scalar registers (one view per block)
+    const int sw = __builtin_amdgcn_readfirstlane(swap);
     const __amdgpu_buffer_rsrc_t vol = __builtin_amdgcn_make_buffer_rsrc(
         (void*)(sw ? imgT : img), (short)0, 4 * g.nx * g.ny * g.nz, 0x00020000);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -212,9 +196,12 @@ __global__ __launch_bounds__(256) void siddon_cone_fp_kernel(float* __restrict__
     const int iu = ub * TU + (lane % TU);
     const int iv = (vb * 4 + wave) * TV + (lane / TU);
     if (iu >= d.nu || iv >= d.nv) return;
-
+    float* out = prj + ((size_t)view * d.nv + iv) * d.nu + iu;
     SiddonRay r;
-    if (!siddon_setup(r, ld3(psrc, view), cell_centre(pdc, pdu, pdv, view, iu, iv, d), g, sw != 0)) return;
+    if (!siddon_setup(r, ld3(psrc, view), cell_centre(pdc, pdu, pdv, view, iu, iv, d), g, sw != 0)) {
+        if (!ACC) *out = 0.f;
+        return;
+    }
     float val = 0.f;
     bool inside = true;
     // Software pipelined: the voxel sequence does not depend on the voxel
@@ -238,7 +225,40 @@ __global__ __launch_bounds__(256) void siddon_cone_fp_kernel(float* __restrict__
 #pragma unroll
         for (int k = 0; k < NP; k++) val = fmaf(f[k], len[k], val);
     }
-    prj[((size_t)view * d.nv + iv) * d.nu + iu] += val;
+    if (ACC) *out += val;
+    else *out = val;
+}
+
+// Block -> (view, detector tile).  Workgroups are dispatched round-robin over
+// the 8 XCDs (each with its own L2).  In XCD order, XCD x receives the x-th
+// eighth of every view's tiles in row-major order, i.e. a band of detector rows
+// whose rays cross one slab of the volume, and walks through the views with
+// it; otherwise tiles are in plain view-major order.  b counts blocks from a
+// multiple of 8.
+__device__ inline void fp_tile_of(long b, int ntile, int xcd_order, int& view, int& tile) {
+    const int per = (ntile + 7) >> 3;                 // tiles per XCD and view
+    if (xcd_order) {
+        const int xcd = (int)(b & 7);
+        const long j = b >> 3;
+        view = (int)(j / per);
+        tile = xcd * per + (int)(j - (long)view * per);
+    } else {
+        view = (int)(b / (8L * per));
+        tile = (int)(b - (long)view * 8 * per);
+    }
+}
+
+template <int TU, int TV>
+__global__ __launch_bounds__(256) void siddon_cone_fp_kernel(float* __restrict__ prj, const float* __restrict__ img,
+                                                             const float* __restrict__ imgT, const int* __restrict__ swapf,
+                                                             const float* __restrict__ pdc, const float* __restrict__ pdu,
+                                                             const float* __restrict__ pdv, const float* __restrict__ psrc,
+                                                             Grid g, Det d, int nview, int nub, int ntile,
+                                                             int xcd_order) {
+    int view, tile;
+    fp_tile_of(blockIdx.x, ntile, xcd_order, view, tile);
+    if (tile >= ntile || view >= nview) return;
+    siddon_fp_block<TU, TV, true>(prj, img, imgT, swapf[view], pdc, pdu, pdv, psrc, g, d, view, tile % nub, tile / nub);
 }
 
 void siddon_cone_fp(float* prj, const float* img, const ConeGeomDev& geo, int B, const Grid& g, const Det& d,
@@ -739,15 +759,16 @@ __device__ inline void bp_view_col(float* acc, const ConeViewBP& cv, const float
     }
 }
 
+// One block of the column-mode backprojector: 16x16 voxel columns x NZR slices,
+// img += A^T y over all views.
 template <int WU, int WV, int NZR>
-__global__ __launch_bounds__(256) void siddon_cone_bp_col_kernel(float* __restrict__ img, const float* __restrict__ wtab,
-                                                                 const float2* __restrict__ utab,
-                                                                 const float* __restrict__ vtab,
-                                                                 const ConeViewBP* __restrict__ views, int nview,
-                                                                 Grid g, int nu, int nv) {
-    const int ix = blockIdx.x * 16 + (threadIdx.x & 15);
-    const int iy = blockIdx.y * 16 + (threadIdx.x >> 4);
-    const int iz0 = blockIdx.z * NZR;
+__device__ inline void bp_col_block(float* __restrict__ img, const float* __restrict__ wtab,
+                                    const float2* __restrict__ utab, const float* __restrict__ vtab,
+                                    const ConeViewBP* __restrict__ views, int nview, const Grid& g, int nu, int nv,
+                                    int bx, int by, int bz) {
+    const int ix = bx * 16 + (threadIdx.x & 15);
+    const int iy = by * 16 + (threadIdx.x >> 4);
+    const int iz0 = bz * NZR;
     if (ix >= g.nx || iy >= g.ny) return;
     const float xc = ix + 0.5f, yc = iy + 0.5f;
     float acc[NZR];
@@ -767,6 +788,15 @@ __global__ __launch_bounds__(256) void siddon_cone_bp_col_kernel(float* __restri
         const int iz = iz0 + k;
         if (iz < g.nz) img[((size_t)iz * g.ny + iy) * g.nx + ix] += acc[k];
     }
+}
+
+template <int WU, int WV, int NZR>
+__global__ __launch_bounds__(256) void siddon_cone_bp_col_kernel(float* __restrict__ img, const float* __restrict__ wtab,
+                                                                 const float2* __restrict__ utab,
+                                                                 const float* __restrict__ vtab,
+                                                                 const ConeViewBP* __restrict__ views, int nview,
+                                                                 Grid g, int nu, int nv) {
+    bp_col_block<WU, WV, NZR>(img, wtab, utab, vtab, views, nview, g, nu, nv, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 template <int WU, int WV, int NZR>
@@ -895,6 +925,436 @@ void siddon_cone_bp(float* img, const float* prj, const ConeGeomDev& geo, const 
     }
     // host plan vector must outlive the async copy
     CTP_CHECK(hipStreamSynchronize(stream));
+}
+
+// ===========================================================================
+// Batched plans (no counterpart in the reference): many cone-beam jobs --
+// volumes of different sizes, each with its own trajectory and detector -- in
+// one persistent launch per direction.  Everything that depends only on the
+// geometry (device copies of the geometry, swap flags, FP64 projective maps,
+// per-ray length x hit factors, column tables, the rare near-parallel rays) is
+// prepared once when the plan is created; forward() and backward() then only
+// launch kernels on the caller's stream: no allocation, host synchronisation,
+// copy or memset, so they can be captured into a HIP graph and replayed.
+// ===========================================================================
+struct ConeJobDev {
+    Grid g;
+    Det d;
+    int nview;
+    float* img;               // bound volume [nz][ny][nx]
+    float* prj;               // bound projections [nview][nv][nu]
+    float* imgT;              // transposed copy for x-dominant views (null if none)
+    const float *dc, *du, *dv, *src;
+    const int* swapf;
+    const ConeViewBP* views;
+    float* wtab;              // per call: prj * lhit
+    const float* lhit;        // per ray: |q - S| if the ray hits the grid (and is regular), else 0
+    const float2* utab;
+    const float* vtab;
+    int nub, ntile;           // forward tiles per view
+    int nbx, nby, nbz;        // backprojection blocks
+    long fp_begin, bp_begin;  // first block of the job in the batched launches
+};
+
+// Per-ray geometry of the column mode (FP64): ray length, hit, near-parallel
+// exception, 1/w_x, 1/w_y (column) and 1/w_z (row).  The same rules as
+// siddon_bp_coltable_kernel.
+__device__ inline void col_ray(long i, const float* __restrict__ pdc, const float* __restrict__ pdu,
+                               const float* __restrict__ pdv, const float* __restrict__ psrc, const Grid& g,
+                               const Det& d, double& L, bool& hit, bool& exception, float2& uinv, float& vinv) {
+    const int iu = (int)(i % d.nu);
+    const int iv = (int)((i / d.nu) % d.nv);
+    const int view = (int)(i / ((long)d.nu * d.nv));
+    const double u = ((double)iu - d.off_u - (d.nu - 1) / 2.0) * d.du;
+    const double v = ((double)iv - d.off_v - (d.nv - 1) / 2.0) * d.dv;
+    double S[3], Q[3];
+    for (int k = 0; k < 3; k++) {
+        S[k] = psrc[3 * view + k];
+        Q[k] = pdc[3 * view + k] + pdu[3 * view + k] * u + pdv[3 * view + k] * v;
+    }
+    const double dd[3] = {g.dx, g.dy, g.dz};
+    const double cc[3] = {g.cx, g.cy, g.cz};
+    const int nn[3] = {g.nx, g.ny, g.nz};
+    double Sm[3] = {S[0], S[1], S[2]}, Qm[3] = {Q[0], Q[1], Q[2]};
+    {
+        double ex = g.nx * (double)g.dx, ey = g.ny * (double)g.dy, ez = g.nz * (double)g.dz;
+        double r = sqrt(ex * ex + ey * ey + ez * ez);
+        double w[3] = {Q[0] - S[0], Q[1] - S[1], Q[2] - S[2]};
+        double len = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+        if (!(len < 1e-6)) for (int k = 0; k < 3; k++) w[k] /= len;
+        double t0 = 0, t1 = 0;
+        for (int k = 0; k < 3; k++) { t0 += (cc[k] - S[k]) * w[k]; t1 += (Q[k] - cc[k]) * w[k]; }
+        if (t0 > r) for (int k = 0; k < 3; k++) Sm[k] = S[k] + (t0 - r) * w[k];
+        if (t1 > r) for (int k = 0; k < 3; k++) Qm[k] = Q[k] - (t1 - r) * w[k];
+    }
+    double wi[3], di[3], smi[3];
+    bool rpar[3], cpar[3];
+    L = 0;
+    for (int k = 0; k < 3; k++) {
+        L += (Q[k] - S[k]) * (Q[k] - S[k]);
+        wi[k] = (Q[k] - S[k]) / dd[k];
+        di[k] = (Qm[k] - Sm[k]) / dd[k];
+        smi[k] = (Sm[k] - cc[k]) / dd[k] + nn[k] / 2.0;
+        rpar[k] = !(fabs(di[k]) > 1e-6);
+        cpar[k] = (Q[k] - S[k]) == 0.0;
+    }
+    L = sqrt(L);
+    double amin = 0, amax = 1;
+    hit = true;
+    for (int k = 0; k < 3; k++) {
+        if (!rpar[k]) {
+            double a0 = (0 - smi[k]) / di[k], a1 = (nn[k] - smi[k]) / di[k];
+            amin = fmax(amin, fmin(a0, a1));
+            amax = fmin(amax, fmax(a0, a1));
+        } else if (smi[k] < 0 || smi[k] > nn[k]) {
+            hit = false;
+        }
+    }
+    if (!(amax > amin)) hit = false;
+    for (int k = 0; k < 3; k++)
+        if (rpar[k]) {
+            int fixed = (int)(smi[k] + amin * di[k]);
+            if (fixed < 0 || fixed >= nn[k]) hit = false;
+        }
+    uinv = make_float2(cpar[0] ? 0.f : (float)(1.0 / wi[0]), cpar[1] ? 0.f : (float)(1.0 / wi[1]));
+    vinv = cpar[2] ? 0.f : (float)(1.0 / wi[2]);
+    exception = rpar[0] != cpar[0] || rpar[1] != cpar[1] || rpar[2] != cpar[2];
+}
+
+// plan creation: lhit, utab, vtab and the list of exception rays (count only
+// when list is null)
+__global__ __launch_bounds__(256) void siddon_plan_rays_kernel(float* __restrict__ lhit, float2* __restrict__ utab,
+                                                               float* __restrict__ vtab, int* __restrict__ nexc,
+                                                               int* __restrict__ exc, const float* __restrict__ pdc,
+                                                               const float* __restrict__ pdu,
+                                                               const float* __restrict__ pdv,
+                                                               const float* __restrict__ psrc, Grid g, Det d, int nview) {
+    const long n = (long)d.nu * d.nv * nview;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double L;
+    bool hit, ex;
+    float2 ui;
+    float vi;
+    col_ray(i, pdc, pdu, pdv, psrc, g, d, L, hit, ex, ui, vi);
+    const int iu = (int)(i % d.nu), iv = (int)((i / d.nu) % d.nv), view = (int)(i / ((long)d.nu * d.nv));
+    if (iv == 0) utab[(long)view * d.nu + iu] = ui;
+    if (iu == 0) vtab[(long)view * d.nv + iv] = vi;
+    lhit[i] = (hit && !ex) ? (float)L : 0.f;
+    if (ex) {
+        const int k = atomicAdd(nexc, 1);
+        if (exc) exc[k] = (int)i;
+    }
+}
+
+__global__ __launch_bounds__(256) void fill_zero_kernel(float* __restrict__ p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0.f;
+}
+
+__global__ __launch_bounds__(256) void mul_kernel(float* __restrict__ w, const float* __restrict__ a,
+                                                  const float* __restrict__ b, size_t n) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) w[i] = a[i] * b[i];
+}
+
+// ray-driven scatter of the listed rays (the forward projector's traversal),
+// img += p * len
+__global__ __launch_bounds__(256) void siddon_scatter_rays_kernel(float* __restrict__ img, const float* __restrict__ prj,
+                                                                  const int* __restrict__ list, int n,
+                                                                  const float* __restrict__ pdc,
+                                                                  const float* __restrict__ pdu,
+                                                                  const float* __restrict__ pdv,
+                                                                  const float* __restrict__ psrc, Grid g, Det d) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    const int i = list[k];
+    const int iu = i % d.nu, iv = (i / d.nu) % d.nv, view = i / (d.nu * d.nv);
+    const float p = prj[i];
+    if (p == 0.f) return;
+    SiddonRay r;
+    if (!siddon_setup(r, ld3(psrc, view), cell_centre(pdc, pdu, pdv, view, iu, iv, d), g, false)) return;
+    bool inside = true;
+    while (inside) {
+        const int off = r.off;
+        const float len = siddon_step(r, inside);
+        atomicAdd(img + (off >> 2), p * len);
+    }
+}
+
+template <int TU, int TV>
+__global__ __launch_bounds__(256) void siddon_cone_fp_batch_kernel(const ConeJobDev* __restrict__ jobs, int njobs,
+                                                                   long total, int xcd_order) {
+    for (long b = blockIdx.x; b < total; b += gridDim.x) {
+        int j = 0;
+        while (j + 1 < njobs && b >= jobs[j + 1].fp_begin) j++;
+        const ConeJobDev* J = jobs + j;
+        int view, tile;
+        fp_tile_of(b - J->fp_begin, J->ntile, xcd_order, view, tile);
+        if (tile >= J->ntile || view >= J->nview) continue;
+        siddon_fp_block<TU, TV, false>(J->prj, J->img, J->imgT, J->swapf[view], J->dc, J->du, J->dv, J->src, J->g,
+                                       J->d, view, tile % J->nub, tile / J->nub);
+    }
+}
+
+template <int WU, int WV, int NZR>
+__global__ __launch_bounds__(256) void siddon_cone_bp_batch_kernel(const ConeJobDev* __restrict__ jobs, int njobs,
+                                                                   long total) {
+    for (long b = blockIdx.x; b < total; b += gridDim.x) {
+        int j = 0;
+        while (j + 1 < njobs && b >= jobs[j + 1].bp_begin) j++;
+        const ConeJobDev* J = jobs + j;
+        const long local = b - J->bp_begin;
+        const int bx = (int)(local % J->nbx);
+        const long rest = local / J->nbx;
+        const int by = (int)(rest % J->nby), bz = (int)(rest / J->nby);
+        bp_col_block<WU, WV, NZR>(J->img, J->wtab, J->utab, J->vtab, J->views, J->nview, J->g, J->d.nu, J->d.nv, bx,
+                                  by, bz);
+    }
+}
+
+template <typename K>
+static int persistent_grid(K kernel, long total) {
+    int dev = 0, nb = 0;
+    CTP_CHECK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    CTP_CHECK(hipGetDeviceProperties(&prop, dev));
+    CTP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(kernel), 256, 0));
+    long grid = (long)std::max(nb, 1) * prop.multiProcessorCount;
+    grid = std::max(8L, grid / 8 * 8);                   // a multiple of 8: XCD of a block is fixed
+    return (int)std::min(grid, (total + 7) / 8 * 8);
+}
+
+struct ConePlan::Impl {
+    std::vector<ConeJobSpec> spec;
+    std::vector<ConeJobDev> jobs;
+    std::vector<void*> mem;
+    std::vector<int> nexc;
+    std::vector<int*> exc;
+    std::vector<bool> need_t;
+    ConeJobDev* djobs = nullptr;
+    int tu = 32, order = 1, nzr = 4, wu = 1, wv = 1;
+    long fp_total = 0, bp_total = 0;
+
+    template <typename T>
+    T* alloc(size_t n) {
+        void* p = nullptr;
+        CTP_CHECK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+        mem.push_back(p);
+        return static_cast<T*>(p);
+    }
+    ~Impl() {
+        for (void* p : mem) (void)hipFree(p);
+    }
+    void layout();                 // block offsets for the current tuning; uploads the job table
+    void forward(hipStream_t s);
+    void backward(hipStream_t s);
+};
+
+void ConePlan::Impl::layout() {
+    fp_total = bp_total = 0;
+    for (auto& J : jobs) {
+        const int TV = 64 / tu;
+        J.nub = ceil_div(J.d.nu, tu);
+        J.ntile = J.nub * ceil_div(J.d.nv, 4 * TV);
+        J.nbx = ceil_div(J.g.nx, 16);
+        J.nby = ceil_div(J.g.ny, 16);
+        J.nbz = ceil_div(J.g.nz, nzr);
+        J.fp_begin = fp_total;
+        J.bp_begin = bp_total;
+        fp_total += (long)J.nview * 8 * ((J.ntile + 7) / 8);
+        bp_total += (long)J.nbx * J.nby * J.nbz;
+    }
+    CTP_CHECK(hipMemcpy(djobs, jobs.data(), sizeof(ConeJobDev) * jobs.size(), hipMemcpyHostToDevice));
+}
+
+void ConePlan::Impl::forward(hipStream_t s) {
+    for (size_t j = 0; j < jobs.size(); j++) {
+        if (!need_t[j]) continue;
+        const ConeJobDev& J = jobs[j];
+        transpose_xy_kernel<<<dim3(ceil_div(J.g.nx, 32), ceil_div(J.g.ny, 32), J.g.nz), 256, 0, s>>>(J.imgT, J.img,
+                                                                                                     J.g.nx, J.g.ny);
+        CTP_LAUNCH_CHECK();
+    }
+    const int nj = (int)jobs.size();
+    TimedLaunch t("siddon_cone_fp_batch", s);
+#define CTP_FPB(A, Bv)                                                                                     \
+    if (tu == A) {                                                                                        \
+        auto k = siddon_cone_fp_batch_kernel<A, Bv>;                                                      \
+        k<<<persistent_grid(k, fp_total), 256, 0, s>>>(djobs, nj, fp_total, order);                       \
+    }
+    CTP_FPB(8, 8) CTP_FPB(16, 4) CTP_FPB(32, 2) CTP_FPB(64, 1)
+#undef CTP_FPB
+    CTP_LAUNCH_CHECK();
+}
+
+void ConePlan::Impl::backward(hipStream_t s) {
+    for (size_t j = 0; j < jobs.size(); j++) {
+        const ConeJobDev& J = jobs[j];
+        const size_t nimg = (size_t)J.g.nx * J.g.ny * J.g.nz, nprj = (size_t)J.d.nu * J.d.nv * J.nview;
+        fill_zero_kernel<<<(unsigned)ceil_div((long)nimg, 256), 256, 0, s>>>(J.img, nimg);
+        CTP_LAUNCH_CHECK();
+        mul_kernel<<<(unsigned)ceil_div((long)nprj, 256), 256, 0, s>>>(J.wtab, J.prj, J.lhit, nprj);
+        CTP_LAUNCH_CHECK();
+        if (nexc[j] > 0) {
+            siddon_scatter_rays_kernel<<<ceil_div(nexc[j], 256), 256, 0, s>>>(J.img, J.prj, exc[j], nexc[j], J.dc, J.du,
+                                                                              J.dv, J.src, J.g, J.d);
+            CTP_LAUNCH_CHECK();
+        }
+    }
+    const int nj = (int)jobs.size();
+    TimedLaunch t("siddon_cone_bp_batch", s);
+#define CTP_BPB(A, Bv, Z)                                                                                  \
+    if (wu == A && wv == Bv && nzr == Z) {                                                                \
+        auto k = siddon_cone_bp_batch_kernel<A, Bv, Z>;                                                   \
+        k<<<persistent_grid(k, bp_total), 256, 0, s>>>(djobs, nj, bp_total);                              \
+    }
+#define CTP_BPB_Z(A, Bv) CTP_BPB(A, Bv, 2) CTP_BPB(A, Bv, 4) CTP_BPB(A, Bv, 8)
+    CTP_BPB_Z(1, 1) CTP_BPB_Z(1, 2) CTP_BPB_Z(2, 1) CTP_BPB_Z(2, 2) CTP_BPB_Z(1, 3) CTP_BPB_Z(3, 1)
+    CTP_BPB_Z(2, 3) CTP_BPB_Z(3, 2) CTP_BPB_Z(3, 3)
+#undef CTP_BPB_Z
+#undef CTP_BPB
+    CTP_LAUNCH_CHECK();
+}
+
+ConePlan::ConePlan(const std::vector<ConeJobSpec>& specs, bool autotune) : impl_(new Impl) {
+    Impl& P = *impl_;
+    P.spec = specs;
+    for (const ConeJobSpec& S : specs) {
+        const Grid& g = S.g;
+        const Det& d = S.d;
+        const int nview = S.nview;
+        const size_t nimg = (size_t)g.nx * g.ny * g.nz, nprj = (size_t)d.nu * d.nv * nview;
+        if (nimg >= (1ull << 29)) fail("cone plan: volumes of 2^29 voxels or more are not supported");
+        ConeGeomHost hg{S.det_center, S.det_u, S.det_v, S.src};
+        ConeBPPlan bp = make_cone_bp_plan(hg, g, d, nview);
+        if (!column_mode(hg, bp, nview))
+            fail("cone plan: every view must have det_v along z and det_u horizontal (column geometry)");
+        if (bp.wu > 3 || bp.wv > 3)
+            fail("cone plan: voxel shadow spans more than 3 detector cells; use the single-volume entry points");
+        P.wu = std::max(P.wu, std::min(bp.wu, d.nu));
+        P.wv = std::max(P.wv, std::min(bp.wv, d.nv));
+        ConeJobDev J{};
+        J.g = g;
+        J.d = d;
+        J.nview = nview;
+        J.img = S.img;
+        J.prj = S.prj;
+        float* geo = P.alloc<float>(12 * (size_t)nview);
+        const float* hs[4] = {S.det_center, S.det_u, S.det_v, S.src};
+        for (int k = 0; k < 4; k++)
+            CTP_CHECK(hipMemcpy(geo + 3 * k * (size_t)nview, hs[k], 12 * (size_t)nview, hipMemcpyHostToDevice));
+        J.dc = geo; J.du = geo + 3 * nview; J.dv = geo + 6 * nview; J.src = geo + 9 * nview;
+        // swap flags from the host geometry (as siddon_swap_flags_kernel)
+        std::vector<int> sw(nview);
+        bool any = false;
+        for (int i = 0; i < nview; i++) {
+            const float wx = S.det_center[3 * i] - S.src[3 * i], wy = S.det_center[3 * i + 1] - S.src[3 * i + 1];
+            sw[i] = std::fabs(wx) > std::fabs(wy) ? 1 : 0;
+            any |= sw[i] != 0;
+        }
+        int* dsw = P.alloc<int>(nview);
+        CTP_CHECK(hipMemcpy(dsw, sw.data(), sizeof(int) * nview, hipMemcpyHostToDevice));
+        J.swapf = dsw;
+        J.imgT = any ? P.alloc<float>(nimg) : nullptr;
+        P.need_t.push_back(any);
+        ConeViewBP* dv = P.alloc<ConeViewBP>(nview);
+        CTP_CHECK(hipMemcpy(dv, bp.views.data(), sizeof(ConeViewBP) * nview, hipMemcpyHostToDevice));
+        J.views = dv;
+        float* lhit = P.alloc<float>(nprj);
+        float2* ut = P.alloc<float2>((size_t)nview * d.nu);
+        float* vt = P.alloc<float>((size_t)nview * d.nv);
+        int* cnt = P.alloc<int>(1);
+        CTP_CHECK(hipMemset(cnt, 0, sizeof(int)));
+        siddon_plan_rays_kernel<<<ceil_div((long)nprj, 256), 256>>>(lhit, ut, vt, cnt, nullptr, J.dc, J.du, J.dv, J.src,
+                                                                   g, d, nview);
+        CTP_LAUNCH_CHECK();
+        int n = 0;
+        CTP_CHECK(hipMemcpy(&n, cnt, sizeof(int), hipMemcpyDeviceToHost));
+        int* list = nullptr;
+        if (n > 0) {
+            list = P.alloc<int>(n);
+            CTP_CHECK(hipMemset(cnt, 0, sizeof(int)));
+            siddon_plan_rays_kernel<<<ceil_div((long)nprj, 256), 256>>>(lhit, ut, vt, cnt, list, J.dc, J.du, J.dv,
+                                                                       J.src, g, d, nview);
+            CTP_LAUNCH_CHECK();
+        }
+        P.nexc.push_back(n);
+        P.exc.push_back(list);
+        J.lhit = lhit;
+        J.utab = ut;
+        J.vtab = vt;
+        J.wtab = P.alloc<float>(nprj);
+        P.jobs.push_back(J);
+    }
+    for (const ConeJobDev& J : P.jobs)
+        if (J.d.nu < P.wu || J.d.nv < P.wv) fail("cone plan: detector smaller than the batch's candidate window");
+    P.djobs = P.alloc<ConeJobDev>(P.jobs.size());
+    P.tu = tuning_get("cone_fp_tile_u", 32);
+    P.order = tuning_get("cone_fp_order", 0) == 0 ? 1 : 0;
+    P.nzr = tuning_get("cone_bp_zrun", 4);
+    if (P.tu != 8 && P.tu != 16 && P.tu != 32 && P.tu != 64) fail("cone_fp_tile_u must be 8, 16, 32 or 64");
+    if (P.nzr != 2 && P.nzr != 4 && P.nzr != 8) fail("cone plan: cone_bp_zrun must be 2, 4 or 8");
+    P.layout();
+    CTP_CHECK(hipDeviceSynchronize());
+    if (autotune) tune();
+}
+
+ConePlan::~ConePlan() = default;
+
+void ConePlan::forward(hipStream_t s) { impl_->forward(s); }
+void ConePlan::backward(hipStream_t s) { impl_->backward(s); }
+void ConePlan::config(int& tile_u, int& xcd_order, int& zrun) const {
+    tile_u = impl_->tu;
+    xcd_order = impl_->order;
+    zrun = impl_->nzr;
+}
+
+// Autotuner: times every forward tile shape / block order and every
+// backprojection z-run on the plan's own buffers (one warm-up and two timed
+// runs each, HIP events on the null stream) and keeps the fastest.  The bound
+// buffers' contents are overwritten.
+void ConePlan::tune() {
+    Impl& P = *impl_;
+    hipEvent_t a, b;
+    CTP_CHECK(hipEventCreate(&a));
+    CTP_CHECK(hipEventCreate(&b));
+    auto time = [&](auto&& fn) {
+        fn();
+        CTP_CHECK(hipEventRecord(a, nullptr));
+        fn();
+        fn();
+        CTP_CHECK(hipEventRecord(b, nullptr));
+        CTP_CHECK(hipEventSynchronize(b));
+        float ms = 0;
+        CTP_CHECK(hipEventElapsedTime(&ms, a, b));
+        return ms;
+    };
+    float best = 1e30f;
+    int btu = P.tu, bor = P.order;
+    for (int tu : {16, 32, 64})
+        for (int order : {1, 0}) {
+            P.tu = tu;
+            P.order = order;
+            P.layout();
+            const float ms = time([&] { P.forward(nullptr); });
+            if (ms < best) { best = ms; btu = tu; bor = order; }
+        }
+    P.tu = btu;
+    P.order = bor;
+    best = 1e30f;
+    int bz = P.nzr;
+    for (int z : {2, 4, 8}) {
+        P.nzr = z;
+        P.layout();
+        const float ms = time([&] { P.backward(nullptr); });
+        if (ms < best) { best = ms; bz = z; }
+    }
+    P.nzr = bz;
+    P.layout();
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    CTP_CHECK(hipDeviceSynchronize());
 }
 
 }  // namespace ctp

@@ -181,6 +181,36 @@ int ctpTVSQSUpdate3D(float* out, float* var, const float* img, const float* bp, 
                      float acc, float wx, float wy, float wz, size_t nBatches, size_t nx, size_t ny, size_t nz,
                      float eps);
 
+/* ---- batched cone-beam plans (no counterpart in the reference) ------------ */
+/* Many Siddon cone-beam jobs (volumes of different sizes, each with its own
+ * trajectory and detector) computed by one persistent launch per direction.
+ * ctpConePlanCreate reads the host geometry, prepares every geometry-only
+ * table on the device and binds the jobs' device buffers img and prj (which
+ * must stay allocated while the plan lives).  Forward (prj = A img) and
+ * Backward (img = A^T prj) overwrite their outputs and only launch kernels on
+ * 'stream' (a hipStream_t, NULL for the null stream): no allocation, copy,
+ * memset or host synchronisation, so they can be captured into a HIP graph.
+ * They return before the work completes.  Jobs need the column geometry
+ * (det_v along z, det_u horizontal in every view).  autotune != 0 times the
+ * launch shapes on the bound buffers (overwriting them) and keeps the fastest. */
+typedef struct ctpConeJob {
+    size_t nx, ny, nz;
+    float dx, dy, dz, cx, cy, cz;
+    size_t nu, nv, nview;
+    float du, dv, off_u, off_v;
+    const float* det_center; /* host [nview][3] */
+    const float* det_u;
+    const float* det_v;
+    const float* src;
+    float* img;              /* device [nz][ny][nx] */
+    float* prj;              /* device [nview][nv][nu] */
+} ctpConeJob;
+int ctpConePlanCreate(void** plan, const ctpConeJob* jobs, int njobs, int autotune);
+int ctpConePlanForward(void* plan, void* stream);
+int ctpConePlanBackward(void* plan, void* stream);
+int ctpConePlanConfig(void* plan, int* tile_u, int* xcd_order, int* zrun);
+int ctpConePlanDestroy(void* plan);
+
 /* ---- library services (no counterpart in the reference) ------------------ */
 /* Text of the calling thread's last failure ("" if none). */
 const char* ctpGetLastErrorMessage(void);

@@ -533,10 +533,11 @@ void dd2d_fp(float* prj, const float* img, const float* deg_host, int B, const G
     const int CHP = ((CH + 1) + 3) / 4 * 4;
     const int kreg_max = 8;
     const int wcap = std::min(kreg_max * 64 * kFPViews / (CHP / 4), 65536 / (2 * CHP * 4));
-    const double wneed = fp_stage_width(views, groups, g, d, dsd, dso, fan) + 6.0;
     // the direct kernel is the default (measured faster at 512x512x32: the
     // staged kernel's prefetch registers spill)
-    const bool use_lds = tuning_get("dd_fp_mode", 1) == 2 && wneed <= (double)wcap && groups.size() <= 65535;
+    const bool want_lds = tuning_get("dd_fp_mode", 1) == 2 && groups.size() <= 65535;
+    const double wneed = want_lds ? fp_stage_width(views, groups, g, d, dsd, dso, fan) + 6.0 : 1e300;
+    const bool use_lds = want_lds && wneed <= (double)wcap;
     const int wmax = use_lds ? (int)std::ceil(wneed) : 0;
     int kreg = 1;
     while (use_lds && kreg * 64 * kFPViews < wmax * (CHP / 4)) kreg *= 2;

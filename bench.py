@@ -147,6 +147,114 @@ def cone_extra(steps):
     return out
 
 
+def cone_batch_extra(steps):
+    """The four uneven cone-beam volumes (128^3 @1, 192^3 @0.75, 256^3 @0.5,
+    128x256x256 @1/0.75 mm; 720 views each, own jittered trajectories) in one
+    batched plan: autotuned, forward and backward each one persistent launch,
+    replayed from HIP graphs.  Compared with the same work through the
+    single-volume device entry points, one volume after another."""
+    import torch
+    from ctprojector_amd.batched import ConeBatch, ConeJob
+    from ctprojector_amd.phantom import CONE_SAMPLES, cone_workload
+    from ctprojector_amd.projector.cupy import ct_projector, cone
+
+    jobs, imgs, geos, projs = [], [], [], []
+    for j in range(len(CONE_SAMPLES)):
+        w, img, geo = cone_workload(j, 720)
+        p = ct_projector()
+        for k in ('nx', 'ny', 'nz', 'dx', 'dy', 'dz', 'cx', 'cy', 'cz', 'nu', 'nv', 'du', 'dv', 'off_u', 'off_v'):
+            setattr(p, k, getattr(w, k))
+        jobs.append(ConeJob(p, geo))
+        imgs.append(img)
+        geos.append(geo)
+        projs.append(p)
+    nvu = sum(int(np.prod(im.shape)) * 720 for im in imgs)
+    log('[bench] cone batch: plan + autotune')
+    t0 = time.perf_counter()
+    bt = ConeBatch(jobs, autotune=True)
+    torch.cuda.synchronize()
+    plan_s = time.perf_counter() - t0
+    saved = [torch.from_numpy(im[0]).cuda() for im in imgs]
+    for t, x in zip(bt.img, saved):
+        t.copy_(x)
+    gf = bt.capture('forward')
+    gb = bt.capture('backward')
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    total = 0.0
+    log('[bench] cone batch timed x%d' % steps)
+    for _ in range(steps + 1):
+        for t, x in zip(bt.img, saved):
+            t.copy_(x)
+        ev[0].record()
+        gf.replay()
+        gb.replay()
+        ev[1].record()
+        torch.cuda.synchronize()
+        if _ > 0:
+            total += ev[0].elapsed_time(ev[1])
+    ms = total / steps
+    finite = all(bool(torch.isfinite(t).all()) for t in bt.img)
+    cfg = bt.config()
+    bt.close()
+    del bt
+    # sequential single-volume calls (device entry points, one sync per call)
+    log('[bench] cone batch: sequential single-volume calls')
+    dg = [[torch.from_numpy(a).cuda() for a in (g.det_center, g.det_u, g.det_v, g.src)] for g in geos]
+    seq = 0.0
+    for _ in range(2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for p, x, g in zip(projs, saved, dg):
+            y = cone.siddon_fp_arbitrary(p, x[None], *g)
+            _ = cone.siddon_bp_arbitrary(p, y, *g)
+        torch.cuda.synchronize()
+        seq = time.perf_counter() - t0
+    return {
+        'workload': 'batched cone Siddon FP+BP: 128^3 @1, 192^3 @0.75, 256^3 @0.5, 128x256x256 @1/0.75/0.75 mm, '
+                    '720 views each (own jittered helical trajectory), 512x384 panel; one persistent launch per '
+                    'direction, autotuned, HIP-graph replay',
+        'ms_per_step': round(ms, 3),
+        'value_gvu_s': round(2 * nvu / (ms / 1e3) / 1e9, 2),
+        'sequential_ms_per_step': round(seq * 1e3, 3),
+        'speedup_vs_sequential': round(seq * 1e3 / ms, 3),
+        'plan_and_autotune_s': round(plan_s, 3),
+        'autotuned': cfg,
+        'finite': finite,
+    }
+
+
+def serving_extra(count=16, seed=1005):
+    """Served reconstruction stream: Poisson arrivals (200/s) of half fan-beam
+    FBP (256^2-512^2 x 1-64 slices, 360-984 views) and half cone-beam SQS-TV
+    (128^3-256^3, 10-50 iterations, 360-984 views), continuous batching with
+    HBM-budget admission, against sequential plain launches of the same requests."""
+    import torch
+    from ctprojector_amd.serving import Server, make_stream, run_sequential
+    log('[bench] serving: %d requests (simulating sinograms)' % count)
+    reqs = make_stream(count, seed=seed)
+    free, _ = torch.cuda.mem_get_info()
+    srv = Server(hbm_budget=int(0.5 * free), workers=4)
+    srv.warm(reqs)
+    log('[bench] serving: served run')
+    st = srv.serve(reqs)
+    log('[bench] serving: sequential run')
+    seq_s, _ = run_sequential(make_stream(count, seed=seed))
+    seq_rate = count / seq_s
+    return {
+        'workload': 'served stream of %d requests (Poisson 200/s, seed %d): fan FBP 256^2-512^2 x 1-64 slices and '
+                    'cone SQS-TV 128^3-256^3 x 10-50 iterations, 360-984 views' % (count, seed),
+        'recons_per_s': round(st.recons_per_s, 3),
+        'latency_p50_s': round(st.latency_p50_s, 3),
+        'latency_p99_s': round(st.latency_p99_s, 3),
+        'makespan_s': round(st.makespan_s, 3),
+        'sequential_recons_per_s': round(seq_rate, 3),
+        'speedup_vs_sequential': round(st.recons_per_s / seq_rate, 3),
+        'peak_concurrent_requests': st.peak_concurrent,
+        'batching': st.launches_batched,
+        'requests': {'fbp': sum(r.kind == 'fbp' for r in reqs), 'sqs': sum(r.kind == 'sqs' for r in reqs)},
+    }
+
+
 def fbp_extra(steps, w, img):
     import torch
     from ctprojector_amd import _lib
@@ -308,6 +416,7 @@ def main(argv=None):
     ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-extra', action='store_true')
+    ap.add_argument('--no-serving', action='store_true', help='skip the served-stream extra')
     ap.add_argument('--calibrate-copy', action='store_true')
     ap.add_argument('--tune', action='append', default=[], metavar='KEY=VALUE',
                     help='library launch-shape knob (ctpSetTuning), repeatable')
@@ -371,9 +480,18 @@ def main(argv=None):
         except Exception as e:  # report, do not hide
             extra['cone'] = {'error': repr(e)}
         try:
+            extra['cone_batch'] = cone_batch_extra(3)
+        except Exception as e:
+            extra['cone_batch'] = {'error': repr(e)}
+        try:
             extra['fbp'] = fbp_extra(3, w, r['img'])
         except Exception as e:
             extra['fbp'] = {'error': repr(e)}
+        if not args.no_serving:
+            try:
+                extra['serving'] = serving_extra()
+            except Exception as e:
+                extra['serving'] = {'error': repr(e)}
         line['extra'] = extra
     if not args.no_cpu_baseline:
         line['cpu_baseline'] = cpu_baseline(w, r['img'])

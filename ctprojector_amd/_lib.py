@@ -29,7 +29,7 @@ SYMBOLS = [
     'ctpDistanceDrivenFanProjectionAdjoint', 'ctpDistanceDrivenParallelProjectionAdjoint',
     'cupyTVSQS3D', 'cTVSQS3D', 'ctpTVSQSUpdate3D',
     'ctpConePlanCreate', 'ctpConePlanForward', 'ctpConePlanBackward', 'ctpConePlanConfig', 'ctpConePlanDestroy',
-    'ctpGetLastErrorMessage', 'ctpReleaseWorkspace',
+    'ctpGetLastErrorMessage', 'ctpReleaseWorkspace', 'ctpSetStream',
     'ctpSetKernelTiming', 'ctpResetKernelTimes', 'ctpGetKernelTime', 'ctpKernelTimeReport',
     'ctpSetTuning', 'ctpGetTuning',
 ]
@@ -112,3 +112,9 @@ def kernel_times() -> dict:
         name, ms, cnt = line.split()
         out[name] = (float(ms), int(cnt))
     return out
+
+
+def set_stream(handle) -> None:
+    """Run the calling thread's subsequent library calls on this HIP stream
+    (a hipStream_t as an int, e.g. torch.cuda.Stream.cuda_stream; 0 = null stream)."""
+    module().ctpSetStream(ctypes.c_void_p(int(handle) if handle else None))

@@ -90,6 +90,12 @@ void need_ptr(const void* p, const char* what) {
     if (!p) fail(std::string("null pointer: ") + what);
 }
 
+// Stream of the calling thread's library calls (ctpSetStream); the null stream
+// by default.  Every entry point enqueues its work there and synchronises only
+// that stream, so threads with their own streams overlap on the device.
+thread_local hipStream_t t_stream = nullptr;
+hipStream_t cur_stream() { return t_stream; }
+
 // first device touch of a call: clear errors left over from earlier work
 void touch() {
     if (!t_device_used) {
@@ -100,11 +106,15 @@ void touch() {
 
 void up(void* dst, const void* src, size_t bytes) {
     touch();
-    if (bytes) CTP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    if (!bytes) return;
+    CTP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, cur_stream()));
+    CTP_CHECK(hipStreamSynchronize(cur_stream()));
 }
 void down(void* dst, const void* src, size_t bytes) {
     touch();
-    if (bytes) CTP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    if (!bytes) return;
+    CTP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, cur_stream()));
+    CTP_CHECK(hipStreamSynchronize(cur_stream()));
 }
 std::vector<float> host_copy(const float* dev, size_t n) {
     std::vector<float> h(n);
@@ -113,11 +123,11 @@ std::vector<float> host_copy(const float* dev, size_t n) {
 }
 void zero(void* dev, size_t bytes) {
     touch();
-    if (bytes) CTP_CHECK(hipMemset(dev, 0, bytes));
+    if (bytes) CTP_CHECK(hipMemsetAsync(dev, 0, bytes, cur_stream()));
 }
 void sync() {
     touch();
-    CTP_CHECK(hipStreamSynchronize(nullptr));
+    CTP_CHECK(hipStreamSynchronize(cur_stream()));
 }
 
 }  // namespace
@@ -134,6 +144,13 @@ CTP_EXPORT int SetDevice(int device) {
 }
 
 CTP_EXPORT const char* ctpGetLastErrorMessage(void) { return last_error(); }
+
+// Sets the HIP stream (a hipStream_t; NULL = the null stream) on which the
+// calling thread's subsequent library calls run.
+CTP_EXPORT int ctpSetStream(void* stream) {
+    t_stream = static_cast<hipStream_t>(stream);
+    return 0;
+}
 
 // Kernel timing: when enabled, every kernel launch of the library is bracketed
 // by HIP events on its stream and the elapsed times are accumulated per kernel
@@ -196,7 +213,7 @@ CTP_EXPORT int cupySiddonConeProjectionArbitrary(float* prj, const float* img, C
         need_ptr(prj, "prj"); need_ptr(img, "img"); need_ptr(detCenter, "detCenter"); need_ptr(detU, "detU");
         need_ptr(detV, "detV"); need_ptr(src, "src");
         touch();
-        siddon_cone_fp(prj, img, ConeGeomDev{detCenter, detU, detV, src}, (int)nBatches, g, d, (int)nview, nullptr);
+        siddon_cone_fp(prj, img, ConeGeomDev{detCenter, detU, detV, src}, (int)nBatches, g, d, (int)nview, cur_stream());
         sync();
     });
 }
@@ -216,7 +233,7 @@ CTP_EXPORT int cSiddonConeProjectionArbitrary(float* prj, const float* img, CONE
         up(geo, detCenter, ng * 4); up(geo + ng, detU, ng * 4); up(geo + 2 * ng, detV, ng * 4); up(geo + 3 * ng, src, ng * 4);
         zero(dp.p, nprj * 4);   // the host API returns A img (cuda/siddonCone.cu:209)
         siddon_cone_fp(dp.as<float>(), di.as<float>(), ConeGeomDev{geo, geo + ng, geo + 2 * ng, geo + 3 * ng},
-                       (int)nBatches, g, d, (int)nview, nullptr);
+                       (int)nBatches, g, d, (int)nview, cur_stream());
         sync();
         down(prj, dp.p, nprj * 4);
     });
@@ -234,7 +251,7 @@ CTP_EXPORT int cupySiddonConeBackprojectionArbitrary(float* img, const float* pr
                            hs = host_copy(src, ng);
         siddon_cone_bp(img, prj, ConeGeomDev{detCenter, detU, detV, src},
                        ConeGeomHost{hc.data(), hu.data(), hv.data(), hs.data()}, (int)nBatches, g, d, (int)nview,
-                       nullptr);
+                       cur_stream());
         sync();
     });
 }
@@ -254,7 +271,7 @@ CTP_EXPORT int cSiddonConeBackprojectionArbitrary(float* img, const float* prj, 
         up(geo, detCenter, ng * 4); up(geo + ng, detU, ng * 4); up(geo + 2 * ng, detV, ng * 4); up(geo + 3 * ng, src, ng * 4);
         zero(di.p, nimg * 4);
         siddon_cone_bp(di.as<float>(), dp.as<float>(), ConeGeomDev{geo, geo + ng, geo + 2 * ng, geo + 3 * ng},
-                       ConeGeomHost{detCenter, detU, detV, src}, (int)nBatches, g, d, (int)nview, nullptr);
+                       ConeGeomHost{detCenter, detU, detV, src}, (int)nBatches, g, d, (int)nview, cur_stream());
         sync();
         down(img, di.p, nimg * 4);
     });
@@ -281,8 +298,8 @@ static int dd_entry(const char* name, float* out, const float* in, const float* 
         const size_t nimg = nBatches * nx * ny * nz, nprj = nBatches * nu * nv * nview;
         if (DEV) {
             std::vector<float> hdeg = host_copy(deg, nview);
-            if (FP) dd2d_fp(out, in, hdeg.data(), (int)nBatches, g, d, (int)nview, dsd, dso, type, fan, nullptr);
-            else dd2d_bp(out, in, hdeg.data(), (int)nBatches, g, d, (int)nview, dsd, dso, type, fan, nullptr);
+            if (FP) dd2d_fp(out, in, hdeg.data(), (int)nBatches, g, d, (int)nview, dsd, dso, type, fan, cur_stream());
+            else dd2d_bp(out, in, hdeg.data(), (int)nBatches, g, d, (int)nview, dsd, dso, type, fan, cur_stream());
             sync();
         } else {
             const size_t nin = FP ? nimg : nprj, nout = FP ? nprj : nimg;
@@ -290,8 +307,8 @@ static int dd_entry(const char* name, float* out, const float* in, const float* 
             DeviceBuffer di(nin * 4), dout(nout * 4);
             up(di.p, in, nin * 4);
             zero(dout.p, nout * 4);
-            if (FP) dd2d_fp(dout.as<float>(), di.as<float>(), deg, (int)nBatches, g, d, (int)nview, dsd, dso, type, fan, nullptr);
-            else dd2d_bp(dout.as<float>(), di.as<float>(), deg, (int)nBatches, g, d, (int)nview, dsd, dso, type, fan, nullptr);
+            if (FP) dd2d_fp(dout.as<float>(), di.as<float>(), deg, (int)nBatches, g, d, (int)nview, dsd, dso, type, fan, cur_stream());
+            else dd2d_bp(dout.as<float>(), di.as<float>(), deg, (int)nBatches, g, d, (int)nview, dsd, dso, type, fan, cur_stream());
             sync();
             down(out, dout.p, nout * 4);
         }
@@ -354,7 +371,7 @@ static int dd_adjoint_entry(const char* name, float* img, const float* prj, cons
         need(nBatches <= kIntMax, "batch too large");
         need_ptr(img, "img"); need_ptr(prj, "prj"); need_ptr(deg, "angles");
         std::vector<float> hdeg = host_copy(deg, nview);
-        dd2d_fp_adjoint(img, prj, hdeg.data(), (int)nBatches, g, d, (int)nview, dsd, dso, type, fan, nullptr);
+        dd2d_fp_adjoint(img, prj, hdeg.data(), (int)nBatches, g, d, (int)nview, dsd, dso, type, fan, cur_stream());
         sync();
     });
 }
@@ -392,8 +409,8 @@ static int tomo_entry(const char* name, float* out, const float* in, const float
         const size_t nimg = nBatches * nx * ny * nz, nprj = nBatches * nu * nv * nview, ng = 3 * nview;
         if (DEV) {
             touch();
-            if (FP) tomo_fp(out, in, detCenter, src, (int)nBatches, g, d, (int)nview, type, nullptr);
-            else tomo_bp(out, in, detCenter, src, (int)nBatches, g, d, (int)nview, type, nullptr);
+            if (FP) tomo_fp(out, in, detCenter, src, (int)nBatches, g, d, (int)nview, type, cur_stream());
+            else tomo_bp(out, in, detCenter, src, (int)nBatches, g, d, (int)nview, type, cur_stream());
             sync();
         } else {
             const size_t nin = FP ? nimg : nprj, nout = FP ? nprj : nimg;
@@ -404,8 +421,8 @@ static int tomo_entry(const char* name, float* out, const float* in, const float
             up(geo, detCenter, ng * 4);
             up(geo + ng, src, ng * 4);
             zero(dout.p, nout * 4);
-            if (FP) tomo_fp(dout.as<float>(), di.as<float>(), geo, geo + ng, (int)nBatches, g, d, (int)nview, type, nullptr);
-            else tomo_bp(dout.as<float>(), di.as<float>(), geo, geo + ng, (int)nBatches, g, d, (int)nview, type, nullptr);
+            if (FP) tomo_fp(dout.as<float>(), di.as<float>(), geo, geo + ng, (int)nBatches, g, d, (int)nview, type, cur_stream());
+            else tomo_bp(dout.as<float>(), di.as<float>(), geo, geo + ng, (int)nBatches, g, d, (int)nview, type, cur_stream());
             sync();
             down(out, dout.p, nout * 4);
         }
@@ -445,14 +462,14 @@ static int filter_entry(const char* name, float* out, const float* in, int nBatc
         const size_t n = (size_t)nBatches * nu * nv * nview;
         if (DEV) {
             touch();
-            ramp_filter(out, in, nBatches, (int)nu, (int)nv, (int)nview, da, off_u, dsd, dso, filter, equispace, nullptr);
+            ramp_filter(out, in, nBatches, (int)nu, (int)nv, (int)nview, da, off_u, dsd, dso, filter, equispace, cur_stream());
             sync();
         } else {
             touch();
             DeviceBuffer di(n * 4), dout(n * 4);
             up(di.p, in, n * 4);
             ramp_filter(dout.as<float>(), di.as<float>(), nBatches, (int)nu, (int)nv, (int)nview, da, off_u, dsd, dso,
-                        filter, equispace, nullptr);
+                        filter, equispace, cur_stream());
             sync();
             down(out, dout.p, n * 4);
         }
@@ -500,14 +517,14 @@ static int fbp_bp_entry(const char* name, float* img, const float* prj, const fl
         const size_t nimg = nBatches * nx * ny * nz, nprj = nBatches * nu * nv * nview;
         if (DEV) {
             std::vector<float> hdeg = host_copy(deg, nview);
-            fan_fbp_bp(img, prj, hdeg.data(), (int)nBatches, g, d, (int)nview, dsd, dso, nullptr);
+            fan_fbp_bp(img, prj, hdeg.data(), (int)nBatches, g, d, (int)nview, dsd, dso, cur_stream());
             sync();
         } else {
             touch();
             DeviceBuffer di(nimg * 4), dp(nprj * 4);
             up(dp.p, prj, nprj * 4);
             zero(di.p, nimg * 4);
-            fan_fbp_bp(di.as<float>(), dp.as<float>(), deg, (int)nBatches, g, d, (int)nview, dsd, dso, nullptr);
+            fan_fbp_bp(di.as<float>(), dp.as<float>(), deg, (int)nBatches, g, d, (int)nview, dsd, dso, cur_stream());
             sync();
             down(img, di.p, nimg * 4);
         }
@@ -543,14 +560,14 @@ static int tv_entry(const char* name, float* s1, float* s2, float* var, const fl
         const size_t n = B * nx * ny * nz;
         if (DEV) {
             touch();
-            tv_sqs(s1, s2, var, img, wx, wy, wz, (int)B, (int)nx, (int)ny, (int)nz, eps, nullptr);
+            tv_sqs(s1, s2, var, img, wx, wy, wz, (int)B, (int)nx, (int)ny, (int)nz, eps, cur_stream());
             sync();
         } else {
             touch();
             DeviceBuffer a(n * 4), b(n * 4), c(n * 4), x(n * 4);
             up(x.p, img, n * 4);
             tv_sqs(a.as<float>(), b.as<float>(), c.as<float>(), x.as<float>(), wx, wy, wz, (int)B, (int)nx, (int)ny,
-                   (int)nz, eps, nullptr);
+                   (int)nz, eps, cur_stream());
             sync();
             down(s1, a.p, n * 4);
             down(s2, b.p, n * 4);
@@ -579,7 +596,7 @@ CTP_EXPORT int ctpTVSQSUpdate3D(float* out, float* var, const float* img, const 
         need_ptr(out, "out"); need_ptr(img, "img"); need_ptr(bp, "bp"); need_ptr(norm, "norm");
         touch();
         tv_sqs_update(out, var, img, bp, norm, beta, acc, wx, wy, wz, (int)nBatches, (int)nx, (int)ny, (int)nz, eps,
-                      nullptr);
+                      cur_stream());
         sync();
     });
 }

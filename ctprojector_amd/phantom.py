@@ -218,10 +218,10 @@ CONE_SAMPLES = [  # (nz, ny, nx, dz, dy, dx)
 
 def cone_workload(sample: int = 2, nview: int = 720) -> Tuple[Workload, np.ndarray, ConeGeometry]:
     """Cone-beam Siddon sample j of the batched workload: helical-like trajectory
-    (z from -20 to +20 mm), +-2 mm seeded jitter of source and detector centre,
-    flat panel 512x384 @1 mm."""
+    (z from -20 to +20 mm), +-2 mm jitter of source and detector centre seeded per
+    sample (1030 + j), flat panel 512x384 @1 mm, ellipsoid phantom (seed 1130 + j)."""
     nz, ny, nx, dz, dy, dx = CONE_SAMPLES[sample]
     w = Workload(nx, ny, nz, dx, dy, dz, 512, 384, 1.0, 1.0, nview, dsd=1085.6, dso=595.0)
-    geo = circular_cone_geometry(nview, 595.0, 1085.6, -20.0, 20.0, jitter=2.0, seed=1003)
-    img = ellipsoid_phantom(nz, ny, nx, dz, dy, dx, seed=1030 + sample)[None]
+    geo = circular_cone_geometry(nview, 595.0, 1085.6, -20.0, 20.0, jitter=2.0, seed=1030 + sample)
+    img = ellipsoid_phantom(nz, ny, nx, dz, dy, dx, seed=1130 + sample)[None]
     return w, img, geo

@@ -1,14 +1,17 @@
 """Device-pointer plumbing shared by the tensor front end.
 
 Tensors are passed to the ``cupy*`` entry points of libprojector.so as raw
-device pointers.  The library runs on the current HIP device's null stream and
-returns when its results are complete; work queued by torch on another stream
-is drained first so that inputs are ready."""
+device pointers.  Each call runs on torch's current stream of the tensors'
+device (``ctpSetStream``; torch and the library share one HIP runtime, see
+``_lib``), so it is ordered after the work torch has queued there, and returns
+when its results are complete."""
 
 from ctypes import c_void_p
 
 import numpy as np
 import torch
+
+from ... import _lib
 
 
 def device_of(*tensors):
@@ -33,8 +36,8 @@ def ptr(t: torch.Tensor) -> c_void_p:
 
 
 class on_device:
-    """Make ``device`` current for the library call and order it after torch's
-    pending work on the current stream."""
+    """Make ``device`` current and run the library call on torch's current
+    stream of that device."""
 
     def __init__(self, device):
         self.device = device
@@ -42,9 +45,7 @@ class on_device:
 
     def __enter__(self):
         self._ctx.__enter__()
-        s = torch.cuda.current_stream(self.device)
-        if s.cuda_stream != 0:
-            s.synchronize()
+        _lib.set_stream(torch.cuda.current_stream(self.device).cuda_stream)
         return self
 
     def __exit__(self, *exc):

@@ -214,6 +214,10 @@ int ctpConePlanDestroy(void* plan);
 /* ---- library services (no counterpart in the reference) ------------------ */
 /* Text of the calling thread's last failure ("" if none). */
 const char* ctpGetLastErrorMessage(void);
+/* Sets the HIP stream (hipStream_t, NULL = null stream) on which the calling
+ * thread's subsequent calls run; each call then synchronises only that stream,
+ * so threads with their own streams overlap on the device. */
+int ctpSetStream(void* stream);
 /* Frees the device scratch memory the library keeps per (device, stream). */
 int ctpReleaseWorkspace(void);
 /* Launch-shape knobs used by the autotuner ("cone_fp_tile_u", "cone_bp_zrun");

@@ -60,7 +60,9 @@ def fan_geometry(n: int, nz: int) -> _Geo:
 
 
 def cone_geometry(n: int) -> _Geo:
-    d = CONE_FOV / n
+    # at most 1 mm voxels: the voxel-driven backprojector supports voxel shadows
+    # of up to 4 detector cells (small volumes cover a smaller field of view)
+    d = min(CONE_FOV / n, 1.0)
     return _Geo(nx=n, ny=n, nz=n, dx=d, dy=d, dz=d, nu=CONE_NU, nv=CONE_NV, du=1.0, dv=1.0, dsd=CONE_DSD,
                 dso=CONE_DSO)
 

@@ -1211,7 +1211,8 @@ void ConePlan::Impl::backward(hipStream_t s) {
     }
 #define CTP_BPB_Z(A, Bv) CTP_BPB(A, Bv, 2) CTP_BPB(A, Bv, 4) CTP_BPB(A, Bv, 8)
     CTP_BPB_Z(1, 1) CTP_BPB_Z(1, 2) CTP_BPB_Z(2, 1) CTP_BPB_Z(2, 2) CTP_BPB_Z(1, 3) CTP_BPB_Z(3, 1)
-    CTP_BPB_Z(2, 3) CTP_BPB_Z(3, 2) CTP_BPB_Z(3, 3)
+    CTP_BPB_Z(2, 3) CTP_BPB_Z(3, 2) CTP_BPB_Z(3, 3) CTP_BPB_Z(1, 4) CTP_BPB_Z(4, 1) CTP_BPB_Z(2, 4)
+    CTP_BPB_Z(4, 2) CTP_BPB_Z(3, 4) CTP_BPB_Z(4, 3) CTP_BPB_Z(4, 4)
 #undef CTP_BPB_Z
 #undef CTP_BPB
     CTP_LAUNCH_CHECK();
@@ -1230,8 +1231,8 @@ ConePlan::ConePlan(const std::vector<ConeJobSpec>& specs, bool autotune) : impl_
         ConeBPPlan bp = make_cone_bp_plan(hg, g, d, nview);
         if (!column_mode(hg, bp, nview))
             fail("cone plan: every view must have det_v along z and det_u horizontal (column geometry)");
-        if (bp.wu > 3 || bp.wv > 3)
-            fail("cone plan: voxel shadow spans more than 3 detector cells; use the single-volume entry points");
+        if (bp.wu > 4 || bp.wv > 4)
+            fail("cone plan: voxel shadow spans more than 4 detector cells; use finer voxels or coarser cells");
         P.wu = std::max(P.wu, std::min(bp.wu, d.nu));
         P.wv = std::max(P.wv, std::min(bp.wv, d.nv));
         ConeJobDev J{};

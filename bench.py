@@ -179,19 +179,22 @@ def cone_batch_extra(steps):
         t.copy_(x)
     gf = bt.capture('forward')
     gb = bt.capture('backward')
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-    total = 0.0
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    total = fp_t = bp_t = 0.0
     log('[bench] cone batch timed x%d' % steps)
     for _ in range(steps + 1):
         for t, x in zip(bt.img, saved):
             t.copy_(x)
         ev[0].record()
         gf.replay()
-        gb.replay()
         ev[1].record()
+        gb.replay()
+        ev[2].record()
         torch.cuda.synchronize()
         if _ > 0:
-            total += ev[0].elapsed_time(ev[1])
+            total += ev[0].elapsed_time(ev[2])
+            fp_t += ev[0].elapsed_time(ev[1])
+            bp_t += ev[1].elapsed_time(ev[2])
     ms = total / steps
     finite = all(bool(torch.isfinite(t).all()) for t in bt.img)
     cfg = bt.config()
@@ -214,6 +217,8 @@ def cone_batch_extra(steps):
                     '720 views each (own jittered helical trajectory), 512x384 panel; one persistent launch per '
                     'direction, autotuned, HIP-graph replay',
         'ms_per_step': round(ms, 3),
+        'fp_ms': round(fp_t / steps, 3),
+        'bp_ms': round(bp_t / steps, 3),
         'value_gvu_s': round(2 * nvu / (ms / 1e3) / 1e9, 2),
         'sequential_ms_per_step': round(seq * 1e3, 3),
         'speedup_vs_sequential': round(seq * 1e3 / ms, 3),

@@ -699,10 +699,12 @@ __global__ __launch_bounds__(256) void siddon_bp_coltable_kernel(float* __restri
     }
 }
 
+// WU x WV is the largest candidate window compiled in; wu x wv (<= WU x WV,
+// uniform across the block) is the window used.
 template <int WU, int WV, int NZR, bool SLOW>
 __device__ inline void bp_view_col(float* acc, const ConeViewBP& cv, const float* __restrict__ wt,
                                    const float2* __restrict__ ut, const float* __restrict__ vt, int nu, int nv, int ix,
-                                   int iy, int iz0, float xc, float yc) {
+                                   int iy, int iz0, float xc, float yc, int wu, int wv) {
     const __amdgpu_buffer_rsrc_t rw = __builtin_amdgcn_make_buffer_rsrc((void*)wt, (short)0, 4 * nu * nv, 0x00020000);
     const __amdgpu_buffer_rsrc_t ru = __builtin_amdgcn_make_buffer_rsrc((void*)ut, (short)0, 8 * nu, 0x00020000);
     const __amdgpu_buffer_rsrc_t rv = __builtin_amdgcn_make_buffer_rsrc((void*)vt, (short)0, 4 * nv, 0x00020000);
@@ -712,12 +714,15 @@ __device__ inline void bp_view_col(float* acc, const ConeViewBP& cv, const float
     const float rd = __builtin_amdgcn_rcpf(fmaf(cv.pw[0], xc, fmaf(cv.pw[1], yc, cv.pw[3])));
     const float uc = fmaf(cv.pu[0], xc, fmaf(cv.pu[1], yc, cv.pu[3])) * rd;
     const float bv = fmaf(cv.pv[0], xc, fmaf(cv.pv[1], yc, cv.pv[3]));
-    const int u0 = min(max((int)ceilf(uc - cv.hu), 0), nu - WU);
+    const int u0 = min(max((int)ceilf(uc - cv.hu), 0), nu - wu);
     const bool inx = (float)ix == cv.fixx, iny = (float)iy == cv.fixy;
     const float big = 3.0e38f;
     float lo[WU], hi[WU];
 #pragma unroll
     for (int j = 0; j < WU; j++) {
+        lo[j] = 0.f;
+        hi[j] = 0.f;
+        if (j >= wu) continue;
         const float2 q = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(ru, 8 * (u0 + j), 0, 0));
         const float mx = ex * q.x, my = ey * q.y;
         float x0 = fmaf(fabsf(q.x), -0.5f, mx), x1 = fmaf(fabsf(q.x), 0.5f, mx);
@@ -733,27 +738,33 @@ __device__ inline void bp_view_col(float* acc, const ConeViewBP& cv, const float
     for (int k = 0; k < NZR; k++) {
         const float zc = (float)(iz0 + k) + 0.5f;
         const float vc = fmaf(cv.pv[2], zc, bv) * rd;
-        const int v0 = min(max((int)ceilf(vc - cv.hv), 0), nv - WV);
+        const int v0 = min(max((int)ceilf(vc - cv.hv), 0), nv - wv);
         const float ez = zc - cv.sz;
         const bool inz = (float)(iz0 + k) == cv.fixz;
         float qz[WV], w[WV][WU];
 #pragma unroll
         for (int jv = 0; jv < WV; jv++) {
+            qz[jv] = 0.f;
+            if (jv >= wv) continue;
             qz[jv] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rv, 4 * (v0 + jv), 0, 0));
 #pragma unroll
-            for (int j = 0; j < WU; j++)
-                w[jv][j] = __uint_as_float(
-                    __builtin_amdgcn_raw_buffer_load_b32(rw, 4 * ((v0 + jv) * nu + u0 + j), 0, 0));
+            for (int j = 0; j < WU; j++) {
+                w[jv][j] = 0.f;
+                if (j < wu)
+                    w[jv][j] = __uint_as_float(
+                        __builtin_amdgcn_raw_buffer_load_b32(rw, 4 * ((v0 + jv) * nu + u0 + j), 0, 0));
+            }
         }
         float a = 0.f;
 #pragma unroll
         for (int jv = 0; jv < WV; jv++) {
+            if (jv >= wv) continue;
             const float mz = ez * qz[jv];
             float z0 = fmaf(fabsf(qz[jv]), -0.5f, mz), z1 = fmaf(fabsf(qz[jv]), 0.5f, mz);
             if (SLOW && qz[jv] == 0.f) { z0 = inz ? -big : big; z1 = inz ? big : -big; }
 #pragma unroll
             for (int j = 0; j < WU; j++)
-                a = fmaf(fmaxf(fminf(hi[j], z1) - fmaxf(lo[j], z0), 0.f), w[jv][j], a);
+                if (j < wu) a = fmaf(fmaxf(fminf(hi[j], z1) - fmaxf(lo[j], z0), 0.f), w[jv][j], a);
         }
         acc[k] += a;
     }
@@ -765,7 +776,7 @@ template <int WU, int WV, int NZR>
 __device__ inline void bp_col_block(float* __restrict__ img, const float* __restrict__ wtab,
                                     const float2* __restrict__ utab, const float* __restrict__ vtab,
                                     const ConeViewBP* __restrict__ views, int nview, const Grid& g, int nu, int nv,
-                                    int bx, int by, int bz) {
+                                    int bx, int by, int bz, int wu = WU, int wv = WV) {
     const int ix = bx * 16 + (threadIdx.x & 15);
     const int iy = by * 16 + (threadIdx.x >> 4);
     const int iz0 = bz * NZR;
@@ -780,8 +791,8 @@ __device__ inline void bp_col_block(float* __restrict__ img, const float* __rest
         const float* wt = wtab + view * nuv;
         const float2* ut = utab + (size_t)view * nu;
         const float* vt = vtab + (size_t)view * nv;
-        if (cv.flags & 1) bp_view_col<WU, WV, NZR, true>(acc, cv, wt, ut, vt, nu, nv, ix, iy, iz0, xc, yc);
-        else bp_view_col<WU, WV, NZR, false>(acc, cv, wt, ut, vt, nu, nv, ix, iy, iz0, xc, yc);
+        if (cv.flags & 1) bp_view_col<WU, WV, NZR, true>(acc, cv, wt, ut, vt, nu, nv, ix, iy, iz0, xc, yc, wu, wv);
+        else bp_view_col<WU, WV, NZR, false>(acc, cv, wt, ut, vt, nu, nv, ix, iy, iz0, xc, yc, wu, wv);
     }
 #pragma unroll
     for (int k = 0; k < NZR; k++) {
@@ -953,6 +964,7 @@ struct ConeJobDev {
     const float* vtab;
     int nub, ntile;           // forward tiles per view
     int nbx, nby, nbz;        // backprojection blocks
+    int wu, wv;               // backprojection candidate window of this job
     long fp_begin, bp_begin;  // first block of the job in the batched launches
 };
 
@@ -1082,10 +1094,29 @@ __global__ __launch_bounds__(256) void siddon_scatter_rays_kernel(float* __restr
     }
 }
 
+// Work distribution of the persistent kernels: a block takes its next block
+// index from a counter with an atomic add, so the blocks in flight stay a
+// compact window of the index space (as with a hardware-dispatched grid) even
+// when blocks take unequal times.  With 'lanes' = 8 the counters are per XCD
+// (a persistent block stays on the XCD of its first dispatch, blockIdx.x mod
+// 8) and hand out indices b = 8 * k + xcd, which keeps the XCD band mapping.
+__device__ inline long next_block(int* __restrict__ ctr, int lanes, long* slot) {
+    const int l = lanes == 8 ? (int)(blockIdx.x & 7) : 0;
+    __syncthreads();                                  // the previous index has been read by every thread
+    if (threadIdx.x == 0) *slot = (long)lanes * atomicAdd(ctr + l, 1) + l;
+    __syncthreads();
+    return *slot;
+}
+
+__global__ void reset_counters_kernel(int* __restrict__ ctr, int n) {
+    if ((int)threadIdx.x < n) ctr[threadIdx.x] = 0;
+}
+
 template <int TU, int TV>
 __global__ __launch_bounds__(256) void siddon_cone_fp_batch_kernel(const ConeJobDev* __restrict__ jobs, int njobs,
-                                                                   long total, int xcd_order) {
-    for (long b = blockIdx.x; b < total; b += gridDim.x) {
+                                                                   long total, int xcd_order, int* __restrict__ ctr) {
+    __shared__ long slot;
+    for (long b = next_block(ctr, 8, &slot); b < total; b = next_block(ctr, 8, &slot)) {
         int j = 0;
         while (j + 1 < njobs && b >= jobs[j + 1].fp_begin) j++;
         const ConeJobDev* J = jobs + j;
@@ -1099,8 +1130,9 @@ __global__ __launch_bounds__(256) void siddon_cone_fp_batch_kernel(const ConeJob
 
 template <int WU, int WV, int NZR>
 __global__ __launch_bounds__(256) void siddon_cone_bp_batch_kernel(const ConeJobDev* __restrict__ jobs, int njobs,
-                                                                   long total) {
-    for (long b = blockIdx.x; b < total; b += gridDim.x) {
+                                                                   long total, int* __restrict__ ctr) {
+    __shared__ long slot;
+    for (long b = next_block(ctr, 1, &slot); b < total; b = next_block(ctr, 1, &slot)) {
         int j = 0;
         while (j + 1 < njobs && b >= jobs[j + 1].bp_begin) j++;
         const ConeJobDev* J = jobs + j;
@@ -1109,7 +1141,7 @@ __global__ __launch_bounds__(256) void siddon_cone_bp_batch_kernel(const ConeJob
         const long rest = local / J->nbx;
         const int by = (int)(rest % J->nby), bz = (int)(rest / J->nby);
         bp_col_block<WU, WV, NZR>(J->img, J->wtab, J->utab, J->vtab, J->views, J->nview, J->g, J->d.nu, J->d.nv, bx,
-                                  by, bz);
+                                  by, bz, J->wu, J->wv);
     }
 }
 
@@ -1133,6 +1165,7 @@ struct ConePlan::Impl {
     std::vector<int*> exc;
     std::vector<bool> need_t;
     ConeJobDev* djobs = nullptr;
+    int* ctr = nullptr;           // work counters of the persistent launches (8 forward, 1 backward)
     int tu = 32, order = 1, nzr = 4, wu = 1, wv = 1;
     long fp_total = 0, bp_total = 0;
 
@@ -1177,11 +1210,13 @@ void ConePlan::Impl::forward(hipStream_t s) {
         CTP_LAUNCH_CHECK();
     }
     const int nj = (int)jobs.size();
+    reset_counters_kernel<<<1, 64, 0, s>>>(ctr, 16);
+    CTP_LAUNCH_CHECK();
     TimedLaunch t("siddon_cone_fp_batch", s);
 #define CTP_FPB(A, Bv)                                                                                     \
     if (tu == A) {                                                                                        \
         auto k = siddon_cone_fp_batch_kernel<A, Bv>;                                                      \
-        k<<<persistent_grid(k, fp_total), 256, 0, s>>>(djobs, nj, fp_total, order);                       \
+        k<<<persistent_grid(k, fp_total), 256, 0, s>>>(djobs, nj, fp_total, order, ctr);                  \
     }
     CTP_FPB(8, 8) CTP_FPB(16, 4) CTP_FPB(32, 2) CTP_FPB(64, 1)
 #undef CTP_FPB
@@ -1203,11 +1238,13 @@ void ConePlan::Impl::backward(hipStream_t s) {
         }
     }
     const int nj = (int)jobs.size();
+    reset_counters_kernel<<<1, 64, 0, s>>>(ctr + 8, 8);
+    CTP_LAUNCH_CHECK();
     TimedLaunch t("siddon_cone_bp_batch", s);
 #define CTP_BPB(A, Bv, Z)                                                                                  \
     if (wu == A && wv == Bv && nzr == Z) {                                                                \
         auto k = siddon_cone_bp_batch_kernel<A, Bv, Z>;                                                   \
-        k<<<persistent_grid(k, bp_total), 256, 0, s>>>(djobs, nj, bp_total);                              \
+        k<<<persistent_grid(k, bp_total), 256, 0, s>>>(djobs, nj, bp_total, ctr + 8);                     \
     }
 #define CTP_BPB_Z(A, Bv) CTP_BPB(A, Bv, 2) CTP_BPB(A, Bv, 4) CTP_BPB(A, Bv, 8)
     CTP_BPB_Z(1, 1) CTP_BPB_Z(1, 2) CTP_BPB_Z(2, 1) CTP_BPB_Z(2, 2) CTP_BPB_Z(1, 3) CTP_BPB_Z(3, 1)
@@ -1233,9 +1270,11 @@ ConePlan::ConePlan(const std::vector<ConeJobSpec>& specs, bool autotune) : impl_
             fail("cone plan: every view must have det_v along z and det_u horizontal (column geometry)");
         if (bp.wu > 4 || bp.wv > 4)
             fail("cone plan: voxel shadow spans more than 4 detector cells; use finer voxels or coarser cells");
-        P.wu = std::max(P.wu, std::min(bp.wu, d.nu));
-        P.wv = std::max(P.wv, std::min(bp.wv, d.nv));
         ConeJobDev J{};
+        J.wu = std::max(1, std::min(bp.wu, d.nu));
+        J.wv = std::max(1, std::min(bp.wv, d.nv));
+        P.wu = std::max(P.wu, J.wu);
+        P.wv = std::max(P.wv, J.wv);
         J.g = g;
         J.d = d;
         J.nview = nview;
@@ -1288,9 +1327,9 @@ ConePlan::ConePlan(const std::vector<ConeJobSpec>& specs, bool autotune) : impl_
         J.wtab = P.alloc<float>(nprj);
         P.jobs.push_back(J);
     }
-    for (const ConeJobDev& J : P.jobs)
-        if (J.d.nu < P.wu || J.d.nv < P.wv) fail("cone plan: detector smaller than the batch's candidate window");
+
     P.djobs = P.alloc<ConeJobDev>(P.jobs.size());
+    P.ctr = P.alloc<int>(16);
     P.tu = tuning_get("cone_fp_tile_u", 32);
     P.order = tuning_get("cone_fp_order", 0) == 0 ? 1 : 0;
     P.nzr = tuning_get("cone_bp_zrun", 4);

@@ -27,10 +27,10 @@ def torch():
 def _jobs():
     a = small_cone(seed=21, n=20, nview=8, nu=36, nv=30, d=1.0)
     b = small_cone(seed=22, n=28, nview=10, nu=40, nv=34, d=0.75, c=(1.0, -0.5, 0.3))
-    g, geo, _ = small_cone(seed=23, n=24, nview=6, nu=32, nv=28, d=1.0)
-    g.nz = 12                                    # a flat volume
+    g, geo, _ = small_cone(seed=23, n=24, nview=6, nu=32, nv=28, d=2.0)
+    g.nz = 12                                    # a flat volume of 2 mm voxels: a larger candidate window
     from ctprojector_amd.phantom import ellipsoid_phantom
-    c = (g, geo, ellipsoid_phantom(12, 24, 24, 1.0, 1.0, 1.0, seed=24)[None])
+    c = (g, geo, ellipsoid_phantom(12, 24, 24, 2.0, 2.0, 2.0, seed=24)[None])
     return [a, b, c]
 
 

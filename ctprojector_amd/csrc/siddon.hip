@@ -1100,12 +1100,15 @@ __global__ __launch_bounds__(256) void siddon_scatter_rays_kernel(float* __restr
 // when blocks take unequal times.  With 'lanes' = 8 the counters are per XCD
 // (a persistent block stays on the XCD of its first dispatch, blockIdx.x mod
 // 8) and hand out indices b = 8 * k + xcd, which keeps the XCD band mapping.
-__device__ inline long next_block(int* __restrict__ ctr, int lanes, long* slot) {
+// The index is read back through LDS and made wave-uniform (readfirstlane), so
+// that the job lookup and the job's parameters stay in scalar registers.
+// Block indices fit in 31 bits (checked on the host).
+__device__ inline int next_block(int* __restrict__ ctr, int lanes, int* slot) {
     const int l = lanes == 8 ? (int)(blockIdx.x & 7) : 0;
     __syncthreads();                                  // the previous index has been read by every thread
-    if (threadIdx.x == 0) *slot = (long)lanes * atomicAdd(ctr + l, 1) + l;
+    if (threadIdx.x == 0) *slot = lanes * atomicAdd(ctr + l, 1) + l;
     __syncthreads();
-    return *slot;
+    return __builtin_amdgcn_readfirstlane(*slot);
 }
 
 __global__ void reset_counters_kernel(int* __restrict__ ctr, int n) {
@@ -1115,8 +1118,8 @@ __global__ void reset_counters_kernel(int* __restrict__ ctr, int n) {
 template <int TU, int TV>
 __global__ __launch_bounds__(256) void siddon_cone_fp_batch_kernel(const ConeJobDev* __restrict__ jobs, int njobs,
                                                                    long total, int xcd_order, int* __restrict__ ctr) {
-    __shared__ long slot;
-    for (long b = next_block(ctr, 8, &slot); b < total; b = next_block(ctr, 8, &slot)) {
+    __shared__ int slot;
+    for (int b = next_block(ctr, 8, &slot); b < total; b = next_block(ctr, 8, &slot)) {
         int j = 0;
         while (j + 1 < njobs && b >= jobs[j + 1].fp_begin) j++;
         const ConeJobDev* J = jobs + j;
@@ -1131,8 +1134,8 @@ __global__ __launch_bounds__(256) void siddon_cone_fp_batch_kernel(const ConeJob
 template <int WU, int WV, int NZR>
 __global__ __launch_bounds__(256) void siddon_cone_bp_batch_kernel(const ConeJobDev* __restrict__ jobs, int njobs,
                                                                    long total, int* __restrict__ ctr) {
-    __shared__ long slot;
-    for (long b = next_block(ctr, 1, &slot); b < total; b = next_block(ctr, 1, &slot)) {
+    __shared__ int slot;
+    for (int b = next_block(ctr, 1, &slot); b < total; b = next_block(ctr, 1, &slot)) {
         int j = 0;
         while (j + 1 < njobs && b >= jobs[j + 1].bp_begin) j++;
         const ConeJobDev* J = jobs + j;
@@ -1185,6 +1188,9 @@ struct ConePlan::Impl {
 };
 
 void ConePlan::Impl::layout() {
+    // block indices (and the counters handing them out, which run past the end
+    // by one grid) must fit in 31 bits
+    const long limit = 0x7fffffffL - (1L << 24);
     fp_total = bp_total = 0;
     for (auto& J : jobs) {
         const int TV = 64 / tu;
@@ -1198,6 +1204,7 @@ void ConePlan::Impl::layout() {
         fp_total += (long)J.nview * 8 * ((J.ntile + 7) / 8);
         bp_total += (long)J.nbx * J.nby * J.nbz;
     }
+    if (fp_total > limit || bp_total > limit) fail("cone plan: too many blocks for one batch; split the jobs");
     CTP_CHECK(hipMemcpy(djobs, jobs.data(), sizeof(ConeJobDev) * jobs.size(), hipMemcpyHostToDevice));
 }
 

@@ -26,6 +26,7 @@ SYMBOLS = [
     'cDistanceDrivenTomoBackprojection', 'cupyDistanceDrivenTomoBackprojection',
     'cfbpFanFilter', 'cupyfbpFanFilter', 'cfbpParallelFilter', 'cupyfbpParallelFilter',
     'cfbpFanBackprojection', 'cupyfbpFanBackprojection',
+    'cSiddonFanProjection', 'cSiddonFanBackprojection',
     'ctpDistanceDrivenFanProjectionAdjoint', 'ctpDistanceDrivenParallelProjectionAdjoint',
     'cupyTVSQS3D', 'cTVSQS3D', 'ctpTVSQSUpdate3D',
     'ctpConePlanCreate', 'ctpConePlanForward', 'ctpConePlanBackward', 'ctpConePlanConfig', 'ctpConePlanDestroy',

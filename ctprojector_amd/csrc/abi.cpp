@@ -388,6 +388,46 @@ CTP_EXPORT int ctpDistanceDrivenParallelProjectionAdjoint(float* img, const floa
 }
 
 // ===========================================================================
+// Siddon fan beam (cuda/siddonFan.cu:128-274), host arrays
+// ===========================================================================
+template <bool FP>
+static int siddon_fan_entry(const char* name, float* out, const float* in, const float* deg, size_t nBatches,
+                            size_t nx, size_t ny, size_t nz, float dx, float dy, float dz, float cx, float cy, float cz,
+                            size_t nu, size_t nv, size_t nview, float da, float dv, float off_a, float off_v,
+                            float dsd, float dso) {
+    return guarded(name, [&] {
+        if (empty(nBatches, nx, ny, nz, nu, nv, nview)) return;
+        Grid g = make_grid(nx, ny, nz, dx, dy, dz, cx, cy, cz);
+        Det d = make_det(nu, nv, nview, da, dv, off_a, off_v);
+        need(nBatches <= kIntMax, "batch too large");
+        need_ptr(out, "out"); need_ptr(in, "in"); need_ptr(deg, "angles");
+        const size_t nimg = nBatches * nx * ny * nz, nprj = nBatches * nu * nv * nview;
+        const size_t nin = FP ? nimg : nprj, nout = FP ? nprj : nimg;
+        touch();
+        DeviceBuffer di(nin * 4), dout(nout * 4), dd(nview * 4);
+        up(di.p, in, nin * 4);
+        up(dd.p, deg, nview * 4);
+        zero(dout.p, nout * 4);
+        if (FP) siddon_fan_fp(dout.as<float>(), di.as<float>(), dd.as<float>(), (int)nBatches, g, d, (int)nview, dsd, dso,
+                              cur_stream());
+        else siddon_fan_bp(dout.as<float>(), di.as<float>(), dd.as<float>(), (int)nBatches, g, d, (int)nview, dsd, dso,
+                           cur_stream());
+        sync();
+        down(out, dout.p, nout * 4);
+    });
+}
+
+CTP_EXPORT int cSiddonFanProjection(float* prj, const float* img, const float* deg, DD_GRID_ARGS, float dsd, float dso) {
+    return siddon_fan_entry<true>("cSiddonFanProjection", prj, img, deg, nBatches, nx, ny, nz, dx, dy, dz, cx, cy, cz,
+                                  nu, nv, nview, da, dv, off_a, off_v, dsd, dso);
+}
+CTP_EXPORT int cSiddonFanBackprojection(float* img, const float* prj, const float* deg, DD_GRID_ARGS, float dsd,
+                                        float dso) {
+    return siddon_fan_entry<false>("cSiddonFanBackprojection", img, prj, deg, nBatches, nx, ny, nz, dx, dy, dz, cx, cy,
+                                   cz, nu, nv, nview, da, dv, off_a, off_v, dsd, dso);
+}
+
+// ===========================================================================
 // Distance driven tomosynthesis (cuda/distanceDriven3D.cu:184-439)
 // ===========================================================================
 #define TOMO_ARGS                                                                                                 \

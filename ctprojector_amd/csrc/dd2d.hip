@@ -895,10 +895,7 @@ __global__ __launch_bounds__(256) void dd_fp_lines_kernel(FPLine* __restrict__ l
 // G[view][u][c] = rev * sum over the detector rows of slice channel c of their
 // z-interpolation weight times y (the transpose of the forward projector's
 // per-row interpolation of the slice accumulators)
-struct ChanRow {
-    int iv;
-    float w;
-};
+using ChanRow = ChanRowHost;
 
 __global__ __launch_bounds__(256) void dd_adj_rows_kernel(float* __restrict__ G, const float* __restrict__ prj,
                                                           const DDView* __restrict__ views,
@@ -916,6 +913,13 @@ __global__ __launch_bounds__(256) void dd_adj_rows_kernel(float* __restrict__ G,
         if (rev) sum *= views[view].rev;
     }
     G[((size_t)view * d.nu + u) * cs + c] = sum;
+}
+
+void rows_to_channel_last(float* q, const float* prj, const int* cstart, const ChanRowHost* crow, int C, int cs,
+                          int nz, const Det& d, int nview, hipStream_t stream) {
+    dd_adj_rows_kernel<<<dim3(ceil_div(d.nu, 64), ceil_div(cs, 4), nview), 256, 0, stream>>>(
+        q, prj, nullptr, cstart, crow, C, cs, nz, d, nview, false);
+    CTP_LAUNCH_CHECK();
 }
 
 template <int CH>

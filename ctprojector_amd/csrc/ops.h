@@ -78,6 +78,22 @@ private:
     std::unique_ptr<Impl> impl_;
 };
 
+// ---- detector rows -> slice channels ----
+struct ChanRowHost {      // one detector row of a slice channel and its weight
+    int iv;
+    float w;
+};
+// q[view][u][c] = sum over the rows (iv, w) of channel c of w * prj[b][view][iv][u],
+// c = b * nz + z (channels C..cs-1 zero); cstart/crow: CSR lists on the device
+void rows_to_channel_last(float* q, const float* prj, const int* cstart, const ChanRowHost* crow, int C, int cs,
+                          int nz, const Det& d, int nview, hipStream_t stream);
+
+// ---- Siddon fan beam (cuda/siddonFan.cu); angles on the device ----
+void siddon_fan_fp(float* prj, const float* img, const float* deg_dev, int B, const Grid& g, const Det& d, int nview,
+                   float dsd, float dso, hipStream_t stream);
+void siddon_fan_bp(float* img, const float* prj, const float* deg_dev, int B, const Grid& g, const Det& d, int nview,
+                   float dsd, float dso, hipStream_t stream);
+
 // ---- pixel-driven backprojection helpers shared by dd2d.hip and fbp.hip ----
 // view groups for a launch over npix pixels x nchunk channel chunks
 int bp_view_groups(size_t npix, int nchunk, int nview);

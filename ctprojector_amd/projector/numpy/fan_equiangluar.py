@@ -59,6 +59,33 @@ def fbp_bp(projector: ct_projector, prj: np.array, angles: np.array) -> np.array
     return img
 
 
+def siddon_fp(projector: ct_projector, img: np.array, angles: np.array) -> np.array:
+    """Siddon fan-beam forward projection (fan_equiangluar.py:131-185):
+    img [batch, nz, ny, nx] -> prj [batch, nview, projector.nv, projector.nu]."""
+    img = _f32(img)
+    angles = _f32(angles)
+    prj = np.zeros([img.shape[0], len(angles), projector.nv, projector.nu], np.float32)
+    err = _lib.module().cSiddonFanProjection(_p(prj), _p(img), _p(angles), *_grid(img.shape, projector),
+                                             *_det(prj.shape, projector))
+    if err != 0:
+        print(err)
+    return prj
+
+
+def siddon_bp(projector: ct_projector, prj: np.array, angles: np.array) -> np.array:
+    """Siddon fan-beam backprojection, the exact transpose of siddon_fp
+    (fan_equiangluar.py:188-237): prj [batch, nview, nv, nu] -> img
+    [batch, projector.nz, projector.ny, projector.nx]."""
+    prj = _f32(prj)
+    angles = _f32(angles)
+    img = np.zeros([prj.shape[0], projector.nz, projector.ny, projector.nx], np.float32)
+    err = _lib.module().cSiddonFanBackprojection(_p(img), _p(prj), _p(angles), *_grid(img.shape, projector),
+                                                 *_det(prj.shape, projector))
+    if err != 0:
+        print(err)
+    return img
+
+
 def distance_driven_fp(projector: ct_projector, img: np.array, angles: np.array,
                        branchless: Union[bool, int] = False) -> np.array:
     """Distance-driven forward projection (fan_equiangluar.py:241-299)."""

@@ -119,6 +119,17 @@ int ctpDistanceDrivenParallelProjectionAdjoint(float* img, const float* prj, con
                                                float cy, float cz, size_t nu, size_t nv, size_t nview, float da,
                                                float dv, float off_a, float off_v, int typeProjector);
 
+/* ---- Siddon fan beam (equiangular arc, stacked slices; host arrays) ------- */
+/* cuda/siddonFan.cu:128  prj += A img; rows are horizontal planes z = (iv - (nv-1)/2 + off_v) dv */
+int cSiddonFanProjection(float* prj, const float* img, const float* deg, size_t nBatches, size_t nx, size_t ny,
+                         size_t nz, float dx, float dy, float dz, float cx, float cy, float cz, size_t nu, size_t nv,
+                         size_t nview, float da, float dv, float off_a, float off_v, float dsd, float dso);
+/* cuda/siddonFan.cu:202  img += A^T prj (exact transpose, deterministic; the reference uses atomics) */
+int cSiddonFanBackprojection(float* img, const float* prj, const float* deg, size_t nBatches, size_t nx, size_t ny,
+                             size_t nz, float dx, float dy, float dz, float cx, float cy, float cz, size_t nu,
+                             size_t nv, size_t nview, float da, float dv, float off_a, float off_v, float dsd,
+                             float dso);
+
 /* ---- distance driven tomosynthesis --------------------------------------- */
 /* cuda/distanceDriven3D.cu:184 */
 int cDistanceDrivenTomoProjection(float* prj, const float* img, const float* detCenter, const float* src,

@@ -292,6 +292,62 @@ void siddon_cone_bp(T* img, const float* prj, const float* pc, const float* pu, 
     }
 }
 
+// Siddon fan beam, cuda/siddonFan.cu:12-80: source (dso sin t, -dso cos t, z),
+// detector point on the arc, both at the row's height z (horizontal rays); no
+// MoveSourceDstNearGrid (that is applied to cone beams only).
+template <typename T>
+void fan_siddon_ray(long iu, long iv, T deg, const DetT<T>& det, T dsd, T dso, V3<T>& s, V3<T>& d) {
+    const T c = tcos(deg), sn = tsin(deg);
+    const T a = (-(T(iu) - T(det.nu - 1) / T(2)) - det.off_u) * det.du;
+    const T z = (T(iv) - T(det.nv - 1) / T(2) + det.off_v) * det.dv;
+    s = V3<T>{dso * sn, -dso * c, z};
+    const T rx = dsd * tsin(a), ry = -dso + dsd * tcos(a);
+    d = V3<T>{rx * c - ry * sn, rx * sn + ry * c, z};
+}
+
+template <typename T>
+void siddon_fan_fp(T* prj, const float* img, const float* deg, long B, const GridT<T>& g, const DetT<T>& det,
+                   long nview, T dsd, T dso) {
+    const long nimg = g.nx * g.ny * g.nz, nprj = det.nu * det.nv * nview;
+    for (long ib = 0; ib < B; ib++) {
+        const float* im = img + ib * nimg;
+        T* pr = prj + ib * nprj;
+#pragma omp parallel for collapse(2) schedule(dynamic, 4)
+        for (long iview = 0; iview < nview; iview++)
+            for (long iv = 0; iv < det.nv; iv++)
+                for (long iu = 0; iu < det.nu; iu++) {
+                    V3<T> s, d;
+                    fan_siddon_ray<T>(iu, iv, T(deg[iview]), det, dsd, dso, s, d);
+                    T val = 0;
+                    siddon_trace<T>(s, d, g, [&](long idx, T len) { val += T(im[idx]) * len; });
+                    pr[(iview * det.nv + iv) * det.nu + iu] += val;
+                }
+    }
+}
+
+template <typename T>
+void siddon_fan_bp(T* img, const float* prj, const float* deg, long B, const GridT<T>& g, const DetT<T>& det,
+                   long nview, T dsd, T dso) {
+    const long nimg = g.nx * g.ny * g.nz, nprj = det.nu * det.nv * nview;
+    for (long ib = 0; ib < B; ib++) {
+        T* im = img + ib * nimg;
+        const float* pr = prj + ib * nprj;
+#pragma omp parallel for collapse(2) schedule(dynamic, 4)
+        for (long iview = 0; iview < nview; iview++)
+            for (long iv = 0; iv < det.nv; iv++)
+                for (long iu = 0; iu < det.nu; iu++) {
+                    V3<T> s, d;
+                    fan_siddon_ray<T>(iu, iv, T(deg[iview]), det, dsd, dso, s, d);
+                    const T p = T(pr[(iview * det.nv + iv) * det.nu + iu]);
+                    siddon_trace<T>(s, d, g, [&](long idx, T len) {
+                        T add = p * len;
+#pragma omp atomic
+                        im[idx] += add;
+                    });
+                }
+    }
+}
+
 // ===========================================================================
 // Box integrals, cuda/cudaMath.cu:336-602
 // ===========================================================================
@@ -1053,6 +1109,18 @@ void orc_set_threads(int n) {
     }
 DEF_SIDDON(d, double)
 DEF_SIDDON(f, float)
+
+#define DEF_SIDDON_FAN(SUF, T)                                                                               \
+    void orc_siddon_fan_fp_##SUF(T* prj, const float* img, const float* deg, size_t B, GRIDARGS, DETARGS,    \
+                                 float dsd, float dso) {                                                     \
+        siddon_fan_fp<T>(prj, img, deg, (long)B, MKG(T), MKD(T), (long)nview, T(dsd), T(dso));                \
+    }                                                                                                        \
+    void orc_siddon_fan_bp_##SUF(T* img, const float* prj, const float* deg, size_t B, GRIDARGS, DETARGS,    \
+                                 float dsd, float dso) {                                                     \
+        siddon_fan_bp<T>(img, prj, deg, (long)B, MKG(T), MKD(T), (long)nview, T(dsd), T(dso));                \
+    }
+DEF_SIDDON_FAN(d, double)
+DEF_SIDDON_FAN(f, float)
 
 #define DEF_DD2D(SUF, T)                                                                                     \
     void orc_dd_fan_fp_##SUF(T* prj, const float* img, const float* deg, size_t B, GRIDARGS, DETARGS, float dsd, \

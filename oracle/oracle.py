@@ -107,6 +107,28 @@ def _dd_type(is_fbp=False, branchless=False, force_fbp=False):
     return int(bool(is_fbp)) + 2 * int(bool(branchless)) + 4 * int(bool(force_fbp))
 
 
+def siddon_fan_fp(p, img, angles, prec='d'):
+    """Siddon fan beam (cuda/siddonFan.cu:12-46), prj += A img."""
+    img = _f32(img)
+    angles = _f32(angles)
+    prj = np.zeros([img.shape[0], len(angles), p.nv, p.nu], _out_t(prec))
+    getattr(lib(), 'orc_siddon_fan_fp_' + prec)(
+        _ptr(prj), _ptr(img), _ptr(angles), _u(img.shape[0]), *_grid(p, img.shape),
+        *_det(p, p.nu, p.nv, len(angles), du=np.float32(p.du / p.dsd)), _f(p.dsd), _f(p.dso))
+    return prj
+
+
+def siddon_fan_bp(p, prj, angles, prec='d'):
+    """Siddon fan beam backprojection (cuda/siddonFan.cu:48-80, atomic scatter)."""
+    prj = _f32(prj)
+    angles = _f32(angles)
+    img = np.zeros([prj.shape[0], p.nz, p.ny, p.nx], _out_t(prec))
+    getattr(lib(), 'orc_siddon_fan_bp_' + prec)(
+        _ptr(img), _ptr(prj), _ptr(angles), _u(prj.shape[0]), *_grid(p, img.shape),
+        *_det(p, prj.shape[3], prj.shape[2], prj.shape[1], du=np.float32(p.du / p.dsd)), _f(p.dsd), _f(p.dso))
+    return img
+
+
 def dd_fan_fp(p, img, angles, branchless=False, type_projector=None, prec='d'):
     img = _f32(img)
     angles = _f32(angles)

@@ -298,3 +298,19 @@ def test_dd_fp_staged_rows_match_direct(oracle, tuning, fan_beam):
         fp = parallel.distance_driven_fp(_proj(g), img, ang)
         ref = oracle.dd_par_fp(g, img, ang)
     assert rel_l2(fp, ref) < TOL
+
+
+# ------------------------------------------------------------ Siddon fan beam
+@pytest.mark.parametrize('variant', ['plain', 'offsets_batch'])
+def test_siddon_fan(oracle, variant):
+    kw = {} if variant == 'plain' else dict(off=(0.4, 0.3), c=(1.5, -1.0, 0.4), B=2)
+    g, ang, img = small_fan(nz=3, nv=3, **kw)
+    p = _proj(g)
+    fp = fan.siddon_fp(p, img, ang)
+    assert rel_l2(fp, oracle.siddon_fan_fp(g, img, ang)) < TOL
+    y = np.random.default_rng(4).uniform(size=fp.shape).astype(np.float32)
+    bp = fan.siddon_bp(p, y, ang)
+    assert rel_l2(bp, oracle.siddon_fan_bp(g, y, ang)) < TOL
+    lhs = np.dot(fp.ravel().astype(np.float64), y.ravel().astype(np.float64))
+    rhs = np.dot(img.ravel().astype(np.float64), bp.ravel().astype(np.float64))
+    assert abs(lhs - rhs) / (np.linalg.norm(fp) * np.linalg.norm(y)) < 1e-6

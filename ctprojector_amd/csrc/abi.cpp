@@ -547,7 +547,8 @@ CTP_EXPORT int cfbpParallelFilter(float* pFPrj, const float* pPrj, int nBatches,
 template <bool DEV>
 static int fbp_bp_entry(const char* name, float* img, const float* prj, const float* deg, size_t nBatches, size_t nx,
                         size_t ny, size_t nz, float dx, float dy, float dz, float cx, float cy, float cz, size_t nu,
-                        size_t nv, size_t nview, float da, float dv, float off_a, float off_v, float dsd, float dso) {
+                        size_t nv, size_t nview, float da, float dv, float off_a, float off_v, float dsd, float dso,
+                        bool fan = true) {
     return guarded(name, [&] {
         if (empty(nBatches, nx, ny, nz, nu, nv, nview)) return;
         Grid g = make_grid(nx, ny, nz, dx, dy, dz, cx, cy, cz);
@@ -557,14 +558,14 @@ static int fbp_bp_entry(const char* name, float* img, const float* prj, const fl
         const size_t nimg = nBatches * nx * ny * nz, nprj = nBatches * nu * nv * nview;
         if (DEV) {
             std::vector<float> hdeg = host_copy(deg, nview);
-            fan_fbp_bp(img, prj, hdeg.data(), (int)nBatches, g, d, (int)nview, dsd, dso, cur_stream());
+            fan_fbp_bp(img, prj, hdeg.data(), (int)nBatches, g, d, (int)nview, dsd, dso, cur_stream(), fan);
             sync();
         } else {
             touch();
             DeviceBuffer di(nimg * 4), dp(nprj * 4);
             up(dp.p, prj, nprj * 4);
             zero(di.p, nimg * 4);
-            fan_fbp_bp(di.as<float>(), dp.as<float>(), deg, (int)nBatches, g, d, (int)nview, dsd, dso, cur_stream());
+            fan_fbp_bp(di.as<float>(), dp.as<float>(), deg, (int)nBatches, g, d, (int)nview, dsd, dso, cur_stream(), fan);
             sync();
             down(img, di.p, nimg * 4);
         }
@@ -575,6 +576,13 @@ CTP_EXPORT int cupyfbpFanBackprojection(float* pImg, const float* pPrj, const fl
                                         float dso) {
     return fbp_bp_entry<true>("cupyfbpFanBackprojection", pImg, pPrj, pDeg, nBatches, nx, ny, nz, dx, dy, dz, cx, cy, cz,
                               nu, nv, nview, da, dv, off_a, off_v, dsd, dso);
+}
+// cuda/fbpParallel.cu:273 (typeProjector is accepted and unused, as there)
+CTP_EXPORT int cfbpParallelBackprojection(float* pImg, const float* pPrj, const float* pDeg, DD_GRID_ARGS,
+                                          int typeProjector) {
+    (void)typeProjector;
+    return fbp_bp_entry<false>("cfbpParallelBackprojection", pImg, pPrj, pDeg, nBatches, nx, ny, nz, dx, dy, dz, cx, cy,
+                               cz, nu, nv, nview, da, dv, off_a, off_v, 1000.f, 500.f, false);
 }
 CTP_EXPORT int cfbpFanBackprojection(float* pImg, const float* pPrj, const float* pDeg, DD_GRID_ARGS, float dsd,
                                      float dso) {

@@ -193,7 +193,9 @@ struct FBPView {
     float pad0, pad1;
 };
 
-template <int CH>
+// FAN: bpFanKernel3D (cuda/fbp.cu:264-359); otherwise bpParallelKernel3D
+// (cuda/fbpParallel.cu:200-255): u = x cos + y sin, no distance weight.
+template <int CH, bool FAN>
 __global__ __launch_bounds__(256) void fan_fbp_bp_kernel(float* __restrict__ img, float* __restrict__ part,
                                                          const float* __restrict__ q, const FBPView* __restrict__ views,
                                                          int C, int cs, int nchunk, int vpg, Grid g, Det d, int nview,
@@ -216,8 +218,8 @@ __global__ __launch_bounds__(256) void fan_fbp_bp_kernel(float* __restrict__ img
         const FBPView V = views[view];
         const float rx = px * V.c + py * V.s;
         const float ry = -px * V.s + py * V.c;
-        const float a = atanf(rx / (ry + dso));
-        const float dist = dso * dso / (rx * rx + (dso + ry) * (dso + ry));
+        const float a = FAN ? atanf(rx / (ry + dso)) : rx;
+        const float dist = FAN ? dso * dso / (rx * rx + (dso + ry) * (dso + ry)) : 1.f;
         const float pu = -(a / d.du + d.off_u) + half;
         // InterpolateXY with truncation, cuda/cudaMath.cu:227-260
         if (pu < 0.f || pu >= (float)d.nu) continue;
@@ -246,7 +248,7 @@ __global__ __launch_bounds__(256) void fan_fbp_bp_kernel(float* __restrict__ img
 }
 
 void fan_fbp_bp(float* img, const float* prj, const float* deg_host, int B, const Grid& g, const Det& d, int nview,
-                float dsd, float dso, hipStream_t stream) {
+                float dsd, float dso, hipStream_t stream, bool fan) {
     (void)dsd;
     const int C = B * g.nz;
     int CH = 1;
@@ -290,10 +292,13 @@ void fan_fbp_bp(float* img, const float* prj, const float* deg_host, int B, cons
     slices_to_channel_last(q, prj, dsrow, C, cs, g.nz, d, nview, stream);
     dim3 grid(ceil_div(g.nx, 16), ceil_div(g.ny, 16), nchunk * groups), block(256);
     {
-    TimedLaunch t("fan_fbp_bp", stream);
+    TimedLaunch t(fan ? "fan_fbp_bp" : "par_pixel_bp", stream);
 #define CTP_FBP_CASE(N)                                                                                        \
     case N:                                                                                                    \
-        fan_fbp_bp_kernel<N><<<grid, block, 0, stream>>>(img, part, q, dviews, C, cs, nchunk, vpg, g, d, nview, dso); \
+        if (fan) fan_fbp_bp_kernel<N, true><<<grid, block, 0, stream>>>(img, part, q, dviews, C, cs, nchunk, vpg, g, \
+                                                                        d, nview, dso);                           \
+        else fan_fbp_bp_kernel<N, false><<<grid, block, 0, stream>>>(img, part, q, dviews, C, cs, nchunk, vpg, g, d, \
+                                                                     nview, dso);                                  \
         break;
     switch (CH) {
         CTP_FBP_CASE(1) CTP_FBP_CASE(2) CTP_FBP_CASE(4) CTP_FBP_CASE(8) CTP_FBP_CASE(16) CTP_FBP_CASE(32)

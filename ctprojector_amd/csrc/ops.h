@@ -125,9 +125,10 @@ void tomo_bp(float* img, const float* prj, const float* det_center, const float*
 // out is overwritten with the filtered projections
 void ramp_filter(float* out, const float* prj, int B, int nu, int nv, int nview, float da, float off_u, float dsd,
                  float dso, int filter, bool equispace, hipStream_t stream);
-// img += FBP-weighted pixel-driven fan backprojection
+// img += FBP-weighted pixel-driven fan backprojection (fan), or the parallel
+// pixel-driven backprojection (fan = false; dsd, dso unused)
 void fan_fbp_bp(float* img, const float* prj, const float* deg_host, int B, const Grid& g, const Det& d, int nview,
-                float dsd, float dso, hipStream_t stream);
+                float dsd, float dso, hipStream_t stream, bool fan = true);
 
 // ---- anisotropic TV prior for SQS (prior-cuda/TotalVariation.cu) ----
 void tv_sqs(float* s1, float* s2, float* var, const float* img, float wx, float wy, float wz, int B, int nx, int ny,

@@ -41,6 +41,19 @@ def ramp_filter(projector: ct_projector, prj: np.array, filter_type: str = 'hann
     return fprj
 
 
+def pixel_driven_bp(projector: ct_projector, prj: np.array, angles: np.array) -> np.array:
+    """Pixel-driven parallel-beam backprojection (parallel.py:75-129):
+    prj [batch, nview, nv, nu] -> img [batch, projector.nz, projector.ny, projector.nx]."""
+    prj = _f32(prj)
+    angles = _f32(angles)
+    img = np.zeros([prj.shape[0], projector.nz, projector.ny, projector.nx], np.float32)
+    err = _lib.module().cfbpParallelBackprojection(_p(img), _p(prj), _p(angles), *_grid(img.shape, projector),
+                                                   *_det(prj.shape, projector), c_int(0))
+    if err != 0:
+        print(err)
+    return img
+
+
 def distance_driven_fp(projector: ct_projector, img: np.array, angles: np.array,
                        branchless: Union[bool, int] = False) -> np.array:
     """Distance-driven parallel forward projection (parallel.py:131-190)."""

@@ -222,6 +222,14 @@ int ctpConePlanBackward(void* plan, void* stream);
 int ctpConePlanConfig(void* plan, int* tile_u, int* xcd_order, int* zrun);
 int ctpConePlanDestroy(void* plan);
 
+/* cuda/fbpParallel.cu:273  parallel-beam pixel-driven backprojection (host arrays):
+ * img += sum over views of the bilinear (truncating) sample of prj at
+ * u = -((x cos + y sin)/du + off_u) + (nu-1)/2; typeProjector is unused */
+int cfbpParallelBackprojection(float* pImg, const float* pPrj, const float* pDeg, size_t nBatches, size_t nx,
+                               size_t ny, size_t nz, float dx, float dy, float dz, float cx, float cy, float cz,
+                               size_t nu, size_t nv, size_t nview, float du, float dv, float off_u, float off_v,
+                               int typeProjector);
+
 /* ---- library services (no counterpart in the reference) ------------------ */
 /* Text of the calling thread's last failure ("" if none). */
 const char* ctpGetLastErrorMessage(void);

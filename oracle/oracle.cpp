@@ -1026,7 +1026,7 @@ void ramp_filter(T* out, const float* prj, long B, long nu, long nv, long nview,
 // thread, bilinear interpolation with truncation).  Output accumulates (+=).
 template <typename T>
 void fan_fbp_bp(T* img, const float* prj, const float* deg, long B, const GridT<T>& g, const DetT<T>& det, long nview,
-                T dsd, T dso) {
+                T dsd, T dso, bool fan = true) {
     (void)dsd;
     const long nzb = 5;
     const long nimg = g.nx * g.ny * g.nz, nprj = det.nu * det.nv * nview;
@@ -1045,8 +1045,10 @@ void fan_fbp_bp(T* img, const float* prj, const float* deg, long B, const GridT<
                         view_trig<T>(deg, iview, c, s);
                         T rx = pt.x * c + pt.y * s;
                         T ry = -pt.x * s + pt.y * c;
-                        T a = tatan(rx / (ry + dso));
-                        T dist = dso * dso / (rx * rx + (dso + ry) * (dso + ry));
+                        // fan: bpFanKernel3D (cuda/fbp.cu:300-330); parallel:
+                        // bpParallelKernel3D (cuda/fbpParallel.cu:224-240), no weight
+                        T a = fan ? tatan(rx / (ry + dso)) : rx;
+                        T dist = fan ? dso * dso / (rx * rx + (dso + ry) * (dso + ry)) : T(1);
                         T pu = -(a / det.du + det.off_u) + (T(det.nu) - T(1)) / T(2);
                         for (long k = 0; k < nzb; k++) {
                             T pv = (pt.z + T(k) * g.dz) / det.dv + det.off_v + (T(det.nv) - T(1)) / T(2);
@@ -1174,5 +1176,12 @@ void orc_ramp_kernel(double* h, size_t nu, double da, int filter, int equispace)
     }
 DEF_FBP(d, double)
 DEF_FBP(f, float)
+
+#define DEF_PARBP(SUF, T)                                                                                    \
+    void orc_par_pixel_bp_##SUF(T* img, const float* prj, const float* deg, size_t B, GRIDARGS, DETARGS) {    \
+        fan_fbp_bp<T>(img, prj, deg, (long)B, MKG(T), MKD(T), (long)nview, T(1000), T(500), false);           \
+    }
+DEF_PARBP(d, double)
+DEF_PARBP(f, float)
 
 }  // extern "C"

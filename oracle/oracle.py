@@ -283,3 +283,14 @@ def tv_sqs(img, weights, eps=1e-8):
         s1 = s1 + w * d1 / vn
         s2 = s2 + w / vn
     return s1, s2, var
+
+
+def par_pixel_bp(p, prj, angles, prec='d'):
+    """Parallel-beam pixel-driven backprojection (cuda/fbpParallel.cu:200-255)."""
+    prj = _f32(prj)
+    angles = _f32(angles)
+    img = np.zeros([prj.shape[0], p.nz, p.ny, p.nx], _out_t(prec))
+    getattr(lib(), 'orc_par_pixel_bp_' + prec)(
+        _ptr(img), _ptr(prj), _ptr(angles), _u(prj.shape[0]), *_grid(p, img.shape),
+        *_det(p, prj.shape[3], prj.shape[2], prj.shape[1]))
+    return img

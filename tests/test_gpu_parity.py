@@ -314,3 +314,12 @@ def test_siddon_fan(oracle, variant):
     lhs = np.dot(fp.ravel().astype(np.float64), y.ravel().astype(np.float64))
     rhs = np.dot(img.ravel().astype(np.float64), bp.ravel().astype(np.float64))
     assert abs(lhs - rhs) / (np.linalg.norm(fp) * np.linalg.norm(y)) < 1e-6
+
+
+@pytest.mark.parametrize('variant', ['plain', 'offsets_batch'])
+def test_parallel_pixel_bp(oracle, variant):
+    kw = {} if variant == 'plain' else dict(off=(0.3, -0.2), B=2)
+    g, ang, img = small_parallel(nz=2, nv=2, **kw)
+    y = np.random.default_rng(5).uniform(size=[img.shape[0], len(ang), g.nv, g.nu]).astype(np.float32)
+    bp = parallel.pixel_driven_bp(_proj(g), y, ang)
+    assert rel_l2(bp, oracle.par_pixel_bp(g, y, ang)) < TOL

@@ -138,3 +138,23 @@ def test_fbp_bp_close_to_dd_bp_with_fbp_weight(oracle):
     a = oracle.fan_fbp_bp(g, y, ang)
     b = oracle.dd_fan_bp(g, y, ang, is_fbp=True)
     assert rel_l2(a, b) < 0.02
+
+
+def test_parallel_pixel_bp_closed_forms(oracle):
+    """Bilinear interpolation reproduces a linear function exactly: a
+    projection equal to its own column index backprojects (one view) to
+    u(x, y) = -((x cos + y sin)/du + off_u) + (nu-1)/2 (pixel centres), which pins
+    the mirrored detector axis, the offset and the angle convention
+    (cuda/fbpParallel.cu:224-240); constant data over nview views gives nview."""
+    g = Geo(nx=12, ny=10, nz=2, dx=0.8, dy=1.1, dz=1.0, cx=0.4, cy=-0.3, nu=64, nv=2, du=0.7, dv=1.0, off_u=0.6)
+    t = 0.7
+    y = np.broadcast_to(np.arange(64, dtype=np.float32), [1, 1, 2, 64]).copy()
+    bp = oracle.par_pixel_bp(g, y, np.array([t], np.float32))[0]
+    f = lambda v: float(np.float32(v))
+    xs = (np.arange(12) + 0.5 - 6) * f(0.8) + f(0.4)
+    ys = (np.arange(10) + 0.5 - 5) * f(1.1) + f(-0.3)
+    c, s = np.cos(f(t)), np.sin(f(t))
+    u = -((xs[None, :] * c + ys[:, None] * s) / f(0.7) + f(0.6)) + 63 / 2.0
+    assert np.abs(bp[0] - u).max() < 1e-9
+    ones = np.ones([1, 9, 2, 64], np.float32)
+    assert np.abs(oracle.par_pixel_bp(g, ones, np.linspace(0, 3, 9).astype(np.float32)) - 9).max() < 1e-9

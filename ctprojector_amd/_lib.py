@@ -26,7 +26,7 @@ SYMBOLS = [
     'cDistanceDrivenTomoBackprojection', 'cupyDistanceDrivenTomoBackprojection',
     'cfbpFanFilter', 'cupyfbpFanFilter', 'cfbpParallelFilter', 'cupyfbpParallelFilter',
     'cfbpFanBackprojection', 'cupyfbpFanBackprojection',
-    'cSiddonFanProjection', 'cSiddonFanBackprojection', 'cfbpParallelBackprojection',
+    'cSiddonFanProjection', 'cSiddonFanBackprojection', 'cfbpParallelBackprojection', 'cupyFbpTomoFilter',
     'ctpDistanceDrivenFanProjectionAdjoint', 'ctpDistanceDrivenParallelProjectionAdjoint',
     'cupyTVSQS3D', 'cTVSQS3D', 'ctpTVSQSUpdate3D',
     'ctpConePlanCreate', 'ctpConePlanForward', 'ctpConePlanBackward', 'ctpConePlanConfig', 'ctpConePlanDestroy',

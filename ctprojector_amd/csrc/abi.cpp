@@ -577,6 +577,23 @@ CTP_EXPORT int cupyfbpFanBackprojection(float* pImg, const float* pPrj, const fl
     return fbp_bp_entry<true>("cupyfbpFanBackprojection", pImg, pPrj, pDeg, nBatches, nx, ny, nz, dx, dy, dz, cx, cy, cz,
                               nu, nv, nview, da, dv, off_a, off_v, dsd, dso);
 }
+// cuda/fbpTomo.cu:386  (device arrays; geometry rows (x, y, z))
+CTP_EXPORT int cupyFbpTomoFilter(float* fprj, const float* prj, const float* detCenter, const float* src, int nBatches,
+                                 size_t nu, size_t nv, size_t nview, float du, float dx, float dz, int typeFilter,
+                                 float cutoffX, float cutoffZ) {
+    return guarded("cupyFbpTomoFilter", [&] {
+        if (nBatches <= 0 || nu == 0 || nv == 0 || nview == 0) return;
+        need(nu <= kIntMax && nv <= kIntMax && nview <= 65535, "detector dimension too large");
+        need((size_t)nBatches * nview * nv <= (size_t)0x7fffffff, "too many detector rows");
+        need(du != 0, "zero detector pitch");
+        need_ptr(fprj, "fprj"); need_ptr(prj, "prj"); need_ptr(detCenter, "detCenter"); need_ptr(src, "src");
+        std::vector<float> hc = host_copy(detCenter, 3 * nview), hs = host_copy(src, 3 * nview);
+        tomo_filter(fprj, prj, hc.data(), hs.data(), nBatches, (int)nu, (int)nv, (int)nview, du, dx, dz, typeFilter,
+                    cutoffX, cutoffZ, cur_stream());
+        sync();
+    });
+}
+
 // cuda/fbpParallel.cu:273 (typeProjector is accepted and unused, as there)
 CTP_EXPORT int cfbpParallelBackprojection(float* pImg, const float* pPrj, const float* pDeg, DD_GRID_ARGS,
                                           int typeProjector) {

@@ -95,15 +95,22 @@ static std::vector<float> cached_ramp(int nu, float da, int filter, bool equispa
 // ---------------------------------------------------------------------------
 // direct linear convolution: RB rows per block, 4 outputs per thread
 // ---------------------------------------------------------------------------
+// nkern > 1: one kernel (and scale) per view, kernel of row r = (r / rows_per_kernel)
+// mod nkern (tomosynthesis filter; RB = 1 so that a block never spans two views).
 template <int RB>
 __global__ __launch_bounds__(256) void ramp_conv_kernel(float* __restrict__ out, const float* __restrict__ in,
                                                         const float* __restrict__ h, const float* __restrict__ pw,
-                                                        long nrows, int nu, float scale) {
+                                                        long nrows, int nu, float scale,
+                                                        const float* __restrict__ kscale, int rows_per_kernel,
+                                                        int nkern) {
     extern __shared__ float sm[];
     const int L = 2 * nu - 1;
     float* hs = sm;                 // [L + 4]
     float* xs = sm + L + 4;         // [RB][nu]
     const long r0 = (long)blockIdx.x * RB;
+    const int kk = nkern > 1 ? (int)((r0 / rows_per_kernel) % nkern) : 0;
+    if (kscale) scale = kscale[kk];
+    h += (size_t)kk * L;
     for (int i = threadIdx.x; i < L + 4; i += blockDim.x) hs[i] = i < L ? h[i] : 0.f;
     for (int rr = 0; rr < RB; rr++) {
         const long r = r0 + rr;
@@ -175,12 +182,108 @@ void ramp_filter(float* out, const float* prj, int B, int nu, int nv, int nview,
     {
     TimedLaunch t("ramp_filter", stream);
     if (sh4 <= 65536) {
-        ramp_conv_kernel<4><<<(unsigned)ceil_div(nrows, 4), 256, sh4, stream>>>(out, prj, dh, dpw, nrows, nu, scale);
+        ramp_conv_kernel<4><<<(unsigned)ceil_div(nrows, 4), 256, sh4, stream>>>(out, prj, dh, dpw, nrows, nu, scale,
+                                                                             nullptr, 1, 1);
     } else {
         const size_t sh1 = sizeof(float) * ((2 * nu - 1 + 4) + nu);
-        ramp_conv_kernel<1><<<(unsigned)nrows, 256, sh1, stream>>>(out, prj, dh, dpw, nrows, nu, scale);
+        ramp_conv_kernel<1><<<(unsigned)nrows, 256, sh1, stream>>>(out, prj, dh, dpw, nrows, nu, scale, nullptr, 1,
+                                                                   1);
     }
     CTP_LAUNCH_CHECK();
+    }
+    CTP_CHECK(hipStreamSynchronize(stream));
+}
+
+// ---------------------------------------------------------------------------
+// tomosynthesis FDK filter, cuda/fbpTomo.cu:47-368
+// ---------------------------------------------------------------------------
+// Per view, the reference builds the equispace Ram-Lak kernel for the virtual
+// pitch vdu = du cos(b) (b: the view's tilt in the x-z plane), multiplies its
+// DFT by a window W(f) of the x and z frequencies (GetFrequency, :47-65: the
+// frequency is clipped to the cutoff from above only, so W is not symmetric),
+// and keeps the real part of the inverse DFT.  For a real kernel that real
+// part is the inverse DFT with the symmetrised window (W(f) + W(-f)) / 2, i.e.
+// the circular convolution of the Ram-Lak kernel with the real, even kernel
+// w_s; it is computed here in FP64 and applied as an exact linear convolution
+// read at nu - 1 (as the padded FFT filtering, :330-356), scaled by
+// pi / nview * vdu.
+static double tomo_freq(int i, int L, double vdu, double d, double weight, double cutoff) {
+    const double fu = (i > (L + 1) / 2) ? (i - L) / (double)L / vdu : i / (double)L / vdu;
+    double w = fu * weight * d * 2;
+    if (w > cutoff) w = cutoff;
+    return w;
+}
+
+static std::vector<double> tomo_kernel(int nu, double vdu, double cosb, double sinb, double dx, double dz, int filter,
+                                       double cutx, double cutz) {
+    const int L = 2 * nu - 1;
+    const double PI = (double)kPI;
+    std::vector<double> r(L, 0.0), W(L, 1.0), ws(L), h(L, 0.0), ct(L);
+    for (int i = 0; i < L; i++) {
+        const int k = i - (nu - 1);
+        if (k == 0) r[i] = 1.0 / (4 * vdu * vdu);
+        else if (k % 2 != 0) r[i] = -1.0 / (PI * PI * (double)k * k * vdu * vdu);
+    }
+    if (filter == FILTER_HAMMING || filter == FILTER_HANN) {
+        const double A = filter == FILTER_HAMMING ? 0.54 : 0.5, Bw = filter == FILTER_HAMMING ? 0.46 : 0.5;
+        for (int i = 0; i < L; i++) {
+            const double wx = tomo_freq(i, L, vdu, dx, cosb, cutx), wz = tomo_freq(i, L, vdu, dz, sinb, cutz);
+            W[i] = (A + Bw * std::cos(PI * wx / cutx)) * (A + Bw * std::cos(PI * wz / cutz));
+        }
+    }
+    for (int m = 0; m < L; m++) ct[m] = std::cos(2 * M_PI * m / L);
+    for (int k = 0; k < L; k++) {           // w_s = IDFT of the symmetrised window (real, even)
+        double acc = 0;
+        for (int f = 0; f < L; f++) acc += 0.5 * (W[f] + W[(L - f) % L]) * ct[(long)f * k % L];
+        ws[k] = acc / L;
+    }
+    for (int i = 0; i < L; i++) {           // circular convolution r * w_s
+        double acc = 0;
+        for (int j = 0; j < L; j++) acc += r[j] * ws[((i - j) % L + L) % L];
+        h[i] = acc;
+    }
+    return h;
+}
+
+void tomo_filter(float* out, const float* prj, const float* det_center_host, const float* src_host, int B, int nu,
+                 int nv, int nview, float du, float dx, float dz, int filter, float cutx, float cutz,
+                 hipStream_t stream) {
+    if (nu <= 0 || nv <= 0 || nview <= 0 || B <= 0) return;
+    if (nu > 5000) fail("tomo filter: nu > 5000 is not supported");
+    if ((filter == FILTER_HAMMING || filter == FILTER_HANN) && !(cutx > 0 && cutz > 0))
+        fail("tomo filter: cutoffs must be positive");
+    const int L = 2 * nu - 1;
+    std::vector<float> h((size_t)nview * L), sc(nview), pw(nu, 1.f);
+    for (int v = 0; v < nview; v++) {
+        const double ex = (double)det_center_host[3 * v] - src_host[3 * v];
+        const double ez = (double)det_center_host[3 * v + 2] - src_host[3 * v + 2];
+        const double r = std::sqrt(ex * ex + ez * ez);
+        if (!(r > 0)) fail("tomo filter: source and detector centre coincide in x-z in view " + std::to_string(v));
+        const double sinb = ex / r, cosb = -ez / r;           // cuda/fbpTomo.cu:130-133
+        const double vdu = (double)du * cosb;
+        if (vdu == 0) fail("tomo filter: view " + std::to_string(v) + " is parallel to the detector");
+        const std::vector<double> hk = tomo_kernel(nu, vdu, cosb, sinb, dx, dz, filter, cutx, cutz);
+        for (int i = 0; i < L; i++) h[(size_t)v * L + i] = (float)hk[i];
+        sc[v] = (float)((double)kPI / nview * vdu);
+    }
+    Workspace& ws = workspace_for(stream);
+    Plan pl;
+    pl.add<float>((size_t)nview * L);
+    pl.add<float>(nview);
+    pl.add<float>(nu);
+    ws.reserve(pl.bytes);
+    float* dh = ws.take_n<float>((size_t)nview * L);
+    float* dsc = ws.take_n<float>(nview);
+    float* dpw = ws.take_n<float>(nu);
+    CTP_CHECK(hipMemcpyAsync(dh, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice, stream));
+    CTP_CHECK(hipMemcpyAsync(dsc, sc.data(), sizeof(float) * nview, hipMemcpyHostToDevice, stream));
+    CTP_CHECK(hipMemcpyAsync(dpw, pw.data(), sizeof(float) * nu, hipMemcpyHostToDevice, stream));
+    const long nrows = (long)B * nview * nv;
+    const size_t sh1 = sizeof(float) * ((2 * nu - 1 + 4) + nu);
+    {
+        TimedLaunch t("tomo_filter", stream);
+        ramp_conv_kernel<1><<<(unsigned)nrows, 256, sh1, stream>>>(out, prj, dh, dpw, nrows, nu, 1.f, dsc, nv, nview);
+        CTP_LAUNCH_CHECK();
     }
     CTP_CHECK(hipStreamSynchronize(stream));
 }

@@ -125,6 +125,10 @@ void tomo_bp(float* img, const float* prj, const float* det_center, const float*
 // out is overwritten with the filtered projections
 void ramp_filter(float* out, const float* prj, int B, int nu, int nv, int nview, float da, float off_u, float dsd,
                  float dso, int filter, bool equispace, hipStream_t stream);
+// tomosynthesis FDK filter (cuda/fbpTomo.cu); geometry rows on the host
+void tomo_filter(float* out, const float* prj, const float* det_center_host, const float* src_host, int B, int nu,
+                 int nv, int nview, float du, float dx, float dz, int filter, float cutx, float cutz,
+                 hipStream_t stream);
 // img += FBP-weighted pixel-driven fan backprojection (fan), or the parallel
 // pixel-driven backprojection (fan = false; dsd, dso unused)
 void fan_fbp_bp(float* img, const float* prj, const float* deg_host, int B, const Grid& g, const Det& d, int nview,

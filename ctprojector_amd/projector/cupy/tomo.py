@@ -43,3 +43,24 @@ def distance_driven_bp(projector, prj, det_center, src, branchless: bool = False
     if err != 0:
         print(err)
     return img
+
+
+def filter_fbp(projector, prj, det_center, src, filter_type: str = 'hann', cutoff_x: float = 1,
+               cutoff_y: float = 1) -> torch.Tensor:
+    """Filter of the projections for tomosynthesis FBP (cupy/tomo.py:157-220):
+    prj [batch, nview, nv, nu]; geometry rows (x, y, z); 'hamming', 'hann' or
+    Ram-Lak; cutoffs in (0, 1] along u (x) and v (z)."""
+    dev = device_of(prj)
+    prj = dev_f32(prj, dev)
+    det_center = dev_f32(det_center, dev)
+    src = dev_f32(src, dev)
+    code = {'hamming': 1, 'hann': 2}.get(filter_type.lower(), 0)
+    fprj = torch.zeros(prj.shape, dtype=torch.float32, device=dev)
+    with on_device(dev):
+        err = _lib.module().cupyFbpTomoFilter(
+            ptr(fprj), ptr(prj), ptr(det_center), ptr(src), c_int(prj.shape[0]), c_ulong(prj.shape[3]),
+            c_ulong(prj.shape[2]), c_ulong(prj.shape[1]), c_float(projector.du), c_float(projector.dx),
+            c_float(projector.dz), c_int(code), c_float(cutoff_x), c_float(cutoff_y))
+    if err != 0:
+        print(err)
+    return fprj

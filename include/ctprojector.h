@@ -230,6 +230,13 @@ int cfbpParallelBackprojection(float* pImg, const float* pPrj, const float* pDeg
                                size_t nu, size_t nv, size_t nview, float du, float dv, float off_u, float off_v,
                                int typeProjector);
 
+/* cuda/fbpTomo.cu:386  tomosynthesis FDK filter (device arrays): per view a ramp
+ * for the tilted virtual pitch du cos(b) with a Hamming (1) / Hann (2) window
+ * of the x and z frequencies (cutoffs in (0, 1]), other codes Ram-Lak */
+int cupyFbpTomoFilter(float* fprj, const float* prj, const float* detCenter, const float* src, int nBatches,
+                      size_t nu, size_t nv, size_t nview, float du, float dx, float dz, int typeFilter, float cutoffX,
+                      float cutoffZ);
+
 /* ---- library services (no counterpart in the reference) ------------------ */
 /* Text of the calling thread's last failure ("" if none). */
 const char* ctpGetLastErrorMessage(void);

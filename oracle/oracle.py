@@ -294,3 +294,46 @@ def par_pixel_bp(p, prj, angles, prec='d'):
         _ptr(img), _ptr(prj), _ptr(angles), _u(prj.shape[0]), *_grid(p, img.shape),
         *_det(p, prj.shape[3], prj.shape[2], prj.shape[1]))
     return img
+
+
+# ------------------------------------------------------------ tomo FDK filter
+def tomo_filter(prj, det_center, src, du, dx, dz, filter_type='hann', cutoff_x=1.0, cutoff_z=1.0):
+    """Tomosynthesis FDK filter, the reference's procedure of cuda/fbpTomo.cu:67-368
+    restated literally in FP64 with numpy.fft: per view the equispace Ram-Lak
+    kernel for vdu = du cos(b), its DFT times the (asymmetric) window W(f) of
+    GetFrequency (:47-65), the real part of the inverse DFT; rows zero-padded to
+    L = 2 nu - 1, multiplied by the kernel's real DFT, inverted, read at nu - 1,
+    scaled by pi / nview * vdu.  Filter codes as src/ct_projector/projector/cupy/tomo.py:190-195."""
+    prj = np.asarray(prj, np.float64)
+    B, nview, nv, nu = prj.shape
+    L = 2 * nu - 1
+    PI = float(np.float32(3.141592657))
+    f32 = lambda v: float(np.float32(v))
+    code = {'hamming': 1, 'hann': 2}.get(filter_type.lower(), 0)
+    out = np.zeros_like(prj)
+    for v in range(nview):
+        ex = float(det_center[v, 0]) - float(src[v, 0])
+        ez = float(det_center[v, 2]) - float(src[v, 2])
+        r = np.hypot(ex, ez)
+        sinb, cosb = ex / r, -ez / r
+        vdu = f32(du) * cosb
+        k = np.arange(L) - (nu - 1)
+        ramp = np.zeros(L)
+        ramp[k == 0] = 1 / (4 * vdu * vdu)
+        odd = k % 2 != 0
+        ramp[odd] = -1 / (PI * PI * k[odd].astype(float) ** 2 * vdu * vdu)
+        i = np.arange(L)
+        fu = np.where(i > (L + 1) // 2, (i - L) / L / vdu, i / L / vdu)
+        if code in (1, 2):
+            A, Bw = (0.54, 0.46) if code == 1 else (0.5, 0.5)
+            wx = np.minimum(fu * cosb * f32(dx) * 2, f32(cutoff_x))
+            wz = np.minimum(fu * sinb * f32(dz) * 2, f32(cutoff_z))
+            W = (A + Bw * np.cos(PI * wx / f32(cutoff_x))) * (A + Bw * np.cos(PI * wz / f32(cutoff_z)))
+        else:
+            W = np.ones(L)
+        h = np.fft.ifft(np.fft.fft(ramp) * W).real
+        x = np.zeros([B, nv, L])
+        x[..., :nu] = prj[:, v]
+        y = np.fft.irfft(np.fft.rfft(x, axis=-1) * np.fft.rfft(h), n=L, axis=-1)
+        out[:, v] = y[..., nu - 1:] * (PI / nview * vdu)
+    return out

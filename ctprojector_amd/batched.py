@@ -97,8 +97,8 @@ class ConeBatch:
             raise RuntimeError('ctpConePlanBackward failed (%d): %s' % (err, _lib.last_error_message()))
 
     def config(self) -> dict:
-        """Launch shapes in use (after autotuning): forward tile width, XCD block
-        order, backprojection z-run."""
+        """Launch shapes in use (after autotuning): forward tile width (0: the
+        column-driven form), XCD block order, backprojection z-run."""
         a, b, c = c_int(), c_int(), c_int()
         _lib.module().ctpConePlanConfig(self._plan, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
         return {'cone_fp_tile_u': a.value, 'cone_fp_xcd_order': b.value, 'cone_bp_zrun': c.value}

@@ -89,29 +89,7 @@ __device__ inline void siddon_fp_block(float* __restrict__ prj, const float* __r
         if (!ACC) *out = 0.f;
         return;
     }
-    float val = 0.f;
-    bool inside = true;
-    // Software pipelined: the voxel sequence does not depend on the voxel
-    // values, so NP steps are traced first and their NP gathers are issued
-    // together.  Steps past the exit read voxel 0 with weight 0.
-    constexpr int NP = 4;
-    while (inside) {
-        int id[NP];
-        float len[NP];
-#pragma unroll
-        for (int k = 0; k < NP; k++) {
-            id[k] = inside ? r.off : 0;
-            bool in_next;
-            const float l = siddon_step(r, in_next);
-            len[k] = inside ? l : 0.f;
-            inside = inside && in_next;
-        }
-        float f[NP];
-#pragma unroll
-        for (int k = 0; k < NP; k++) f[k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(vol, id[k], 0, 0));
-#pragma unroll
-        for (int k = 0; k < NP; k++) val = fmaf(f[k], len[k], val);
-    }
+    const float val = siddon_integral(vol, r);
     if (ACC) *out += val;
     else *out = val;
 }
@@ -148,10 +126,305 @@ __global__ __launch_bounds__(256) void siddon_cone_fp_kernel(float* __restrict__
     siddon_fp_block<TU, TV, true>(prj, img, imgT, swapf[view], pdc, pdu, pdv, psrc, g, d, view, tile % nub, tile / nub);
 }
 
+// ---------------------------------------------------------------------------
+// Column-driven forward projection
+// ---------------------------------------------------------------------------
+// When det_v has no x or y component, the rays of one detector column lie in a
+// vertical plane through the source: they share one line in the xy plane and
+// differ only in their slope along z.  One wavefront takes one column.  The 2D
+// traversal of the shared xy line (its cells and the interval of the in-plane
+// distance s spent in each) is the same in every lane and is stepped in
+// lockstep; each lane follows kColRays rays (detector rows) through it.  Inside
+// an xy cell a ray's z index changes by less than 1 (steeper rays are traced
+// ray by ray), so it meets z cells lo and lo + 1 at most: the s interval is
+// split at the plane z = lo + 1, and both voxels come from one 8-byte load of
+// the volume re-laid as z-contiguous columns, [y][x][z + 1] with P = nz + 3
+// floats per column and zeros at z + 1 = 0, nz + 1, nz + 2.  The 3D chord is
+// the s interval times sqrt(1 + slope^2): the sum is the line integral
+// Siddon's traversal computes (cuda/siddon.cu:169-233), up to rounding.
+//
+// Rays that this form would not reproduce exactly are traced with the
+// reference's traversal (siddon_setup / siddon_integral): a component of the
+// moved segment within the axis-parallel tolerance (|d| <= 1e-6 in index
+// units, except an exactly horizontal ray), and rays steeper than the bound.
+// Columns whose xy line has an exactly zero x or y component, and views whose
+// det_v is not along z, are traced ray by ray as a whole.
+constexpr int kColRays = 4;
+
+// [z][y][x] -> [y][x][z + 1] with zero padding (P floats per column)
+__global__ __launch_bounds__(256) void transpose_zcol_kernel(float* __restrict__ out, const float* __restrict__ in,
+                                                             int nx, int ny, int nz, int P) {
+    __shared__ float tile[32][33];
+    const int x0 = blockIdx.x * 32, z0 = blockIdx.y * 32, y = blockIdx.z;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int k = ty; k < 32; k += 8) {
+        const int z = z0 + k, x = x0 + tx;
+        tile[k][tx] = (x < nx && z < nz) ? in[((size_t)z * ny + y) * nx + x] : 0.f;
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        const int x = x0 + k, z = z0 + tx;
+        if (x >= nx) continue;
+        float* col = out + ((size_t)y * nx + x) * P;
+        if (z < nz) col[z + 1] = tile[tx][k];
+        if (blockIdx.y == 0 && tx < 3) col[tx == 0 ? 0 : nz + tx] = 0.f;
+    }
+}
+
+// Ray constants of the column form for detector row iv: slope along z per mm
+// of s (zs, index units), z index at s = 0 (z0), the 3D length per mm of s
+// (sec), and whether the ray must be traced instead.
+struct ColFPRay {
+    float zs, z0, sec;
+    bool exc;
+};
+
+__device__ inline ColFPRay colfp_ray(float3 S, float3 P, const Grid& g, float Lxy, float sO, float diag) {
+    ColFPRay c;
+    const float slope = (P.z - S.z) / Lxy;
+    c.zs = slope / g.dz;
+    c.z0 = (fmaf(slope, sO, S.z) - g.cz) / g.dz + g.nz * 0.5f;
+    c.sec = sqrtf(fmaf(slope, slope, 1.f));
+    float3 s = S, t = P;
+    siddon_move(s, t, g);
+    const float3 dd = siddon_index_dir(s, t, g);
+    const bool okx = fabsf(dd.x) > kEps, oky = fabsf(dd.y) > kEps;
+    const bool okz = fabsf(dd.z) > kEps || (dd.z == 0.f && P.z == S.z);
+    c.exc = !(okx && oky && okz) || !(fabsf(c.zs) * diag <= 0.99f);
+    return c;
+}
+
+// ACC: out += line integral if the ray hits; otherwise out = line integral (0 on a miss)
+template <bool ACC>
+__device__ inline void trace_into(float* out, __amdgpu_buffer_rsrc_t vol, float3 S, float3 P, Grid g) {
+    SiddonRay r;
+    if (siddon_setup(r, S, P, g, false)) {
+        const float v = siddon_integral(vol, r);
+        if (ACC) *out += v;
+        else *out = v;
+    } else if (!ACC) {
+        *out = 0.f;
+    }
+}
+
+template <bool ACC>
+__device__ inline void fp_column(float* __restrict__ prj, __amdgpu_buffer_rsrc_t volz, __amdgpu_buffer_rsrc_t vol,
+                                 const float* __restrict__ pdc, const float* __restrict__ pdu,
+                                 const float* __restrict__ pdv, const float* __restrict__ psrc, const Grid& g,
+                                 const Det& d, int view, int iu, int P) {
+    constexpr int R = kColRays;
+    const int lane = threadIdx.x & 63;
+    float* col = prj + (size_t)view * d.nv * d.nu + iu;        // row iv at col[iv * nu]
+    const float3 S = ld3(psrc, view);
+    const float3 V = ld3(pdv, view);
+    const float3 C0 = cell_centre(pdc, pdu, pdv, view, iu, 0, d);
+    const float ex0 = C0.x - S.x, ey0 = C0.y - S.y;
+    const float Lxy = sqrtf(ex0 * ex0 + ey0 * ey0);
+    const float ex = ex0 / Lxy, ey = ey0 / Lxy;
+    const float exi = ex / g.dx, eyi = ey / g.dy;
+    const bool shared = V.x == 0.f && V.y == 0.f && Lxy > 0.f && exi != 0.f && eyi != 0.f;
+    if (!shared) {
+        for (int iv = lane; iv < d.nv; iv += 64)
+            trace_into<ACC>(col + (size_t)iv * d.nu, vol, S, cell_centre(pdc, pdu, pdv, view, iu, iv, d), g);
+        return;
+    }
+    // s = 0 at the foot of the perpendicular from the grid centre (keeps |s|
+    // within the grid's half diagonal); (ox, oy): that point in index units
+    const float sO = (g.cx - S.x) * ex + (g.cy - S.y) * ey;
+    const float ox = (fmaf(ex, sO, S.x) - g.cx) / g.dx + g.nx * 0.5f;
+    const float oy = (fmaf(ey, sO, S.y) - g.cy) / g.dy + g.ny * 0.5f;
+    const float rxi = 1.f / exi, ryi = 1.f / eyi;
+    const float sx0 = -ox * rxi, sx1 = ((float)g.nx - ox) * rxi;
+    const float sy0 = -oy * ryi, sy1 = ((float)g.ny - oy) * ryi;
+    const float s_in = fmaxf(fminf(sx0, sx1), fminf(sy0, sy1));
+    const float s_out = fminf(fmaxf(sx0, sx1), fmaxf(sy0, sy1));
+    const bool line_hits = s_in < s_out;
+    const float diag = sqrtf(g.dx * g.dx + g.dy * g.dy);
+    const float nzf = (float)g.nz;
+
+    // rows whose rays reach the grid form one run [lo_iv, hi_iv]
+    int lo_iv = d.nv, hi_iv = -1;
+    bool any_exc = false;
+    for (int k = 0; k * 64 < d.nv; k++) {
+        const int iv = k * 64 + lane;
+        bool hit = false, exc = false;
+        if (iv < d.nv) {
+            const ColFPRay c = colfp_ray(S, cell_centre(pdc, pdu, pdv, view, iu, iv, d), g, Lxy, sO, diag);
+            const float za = fmaf(c.zs, s_in, c.z0), zb = fmaf(c.zs, s_out, c.z0);
+            exc = c.exc;
+            hit = !exc && line_hits && fmaxf(za, zb) > 0.f && fminf(za, zb) < nzf;
+        }
+        const unsigned long long mh = __ballot(hit), me = __ballot(exc);
+        if (mh) {
+            lo_iv = min(lo_iv, k * 64 + __ffsll((long long)mh) - 1);
+            hi_iv = max(hi_iv, k * 64 + 63 - __clzll((long long)mh));
+        }
+        any_exc = any_exc || me != 0;
+    }
+
+    if (hi_iv >= lo_iv) {
+        // 2D traversal start, the same in every lane (cuda/siddon.cu:113-149 in 2D)
+        // (integer state made scalar: every lane computes the same values)
+        int ix = __builtin_amdgcn_readfirstlane((int)floorf(fmaf(s_in, exi, ox)));
+        int iy = __builtin_amdgcn_readfirstlane((int)floorf(fmaf(s_in, eyi, oy)));
+        ix = min(max(ix, 0), g.nx - 1);
+        iy = min(max(iy, 0), g.ny - 1);
+        const bool px = __builtin_amdgcn_readfirstlane(exi > 0.f ? 1 : 0) != 0;
+        const bool py = __builtin_amdgcn_readfirstlane(eyi > 0.f ? 1 : 0) != 0;
+        const float sxn0 = ((float)(px ? ix + 1 : ix) - ox) * rxi;
+        const float syn0 = ((float)(py ? iy + 1 : iy) - oy) * ryi;
+        const float dsx = fabsf(rxi), dsy = fabsf(ryi);
+        const int rx0 = px ? g.nx - 1 - ix : ix, ry0 = py ? g.ny - 1 - iy : iy;
+        const int mxo = (px ? 4 : -4) * P, myo = (py ? 4 : -4) * P * g.nx;
+        const int off0 = 4 * P * (iy * g.nx + ix);        // byte offset of column (ix, iy)
+        for (int base = lo_iv; base <= hi_iv; base += 64 * R) {
+            float zs[R], z0[R], rs[R], q1[R], fa[R], acc[R], sec[R];
+            bool up[R], act[R];
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const int iv = base + 64 * r + lane;
+                ColFPRay c;
+                c.exc = true;
+                if (iv <= hi_iv) c = colfp_ray(S, cell_centre(pdc, pdu, pdv, view, iu, iv, d), g, Lxy, sO, diag);
+                act[r] = iv <= hi_iv && !c.exc;
+                // inactive lanes follow a flat ray below the grid (zero padding)
+                zs[r] = act[r] ? c.zs : 0.f;
+                z0[r] = act[r] ? c.z0 : -4.f;
+                sec[r] = c.sec;
+                rs[r] = zs[r] != 0.f ? 1.f / zs[r] : 0.f;
+                q1[r] = zs[r] != 0.f ? (1.f - z0[r]) * rs[r] : 3.0e38f;   // s at z = lo + 1: lo * rs + q1
+                up[r] = zs[r] >= 0.f;
+                fa[r] = __builtin_amdgcn_fmed3f(floorf(fmaf(zs[r], s_in, z0[r])), -1.f, nzf);
+                acc[r] = 0.f;
+            }
+            // min / max as medians with a sentinel (no canonicalisation of
+            // loop-carried operands)
+            constexpr float kBig = 3.0e38f;
+            float sa = s_in, sxn = sxn0, syn = syn0;
+            int off = off0, rx = rx0, ry = ry0;
+            // one xy cell; the loop below runs it twice per trip so that the
+            // carried values alternate registers instead of being copied
+            auto cell = [&]() {
+                // every lane holds the same values: the choice is made scalar
+                const int bxi = __builtin_amdgcn_readfirstlane(sxn <= syn ? 1 : 0);
+                const float sb = __builtin_amdgcn_fmed3f(bxi ? sxn : syn, sa, kBig);
+                const float Lc = sb - sa;
+                float lo[R], fb[R];
+                float2 f[R];
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    fb[r] = __builtin_amdgcn_fmed3f(floorf(fmaf(zs[r], sb, z0[r])), -1.f, nzf);
+                    lo[r] = __builtin_amdgcn_fmed3f(fa[r], fb[r], -kBig);
+                }
+                // the whole offset is the vector operand (checked against the
+                // buffer's size); (lo + 1) >= 0 is the padded z index
+#pragma unroll
+                for (int r = 0; r < R; r++)
+                    f[r] = __builtin_bit_cast(
+                        float2, __builtin_amdgcn_raw_buffer_load_b64(volz, 4 * ((int)lo[r] + 1) + off, 0, 0));
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const float tm = __builtin_amdgcn_fmed3f(fmaf(lo[r], rs[r], q1[r]), sa, sb);
+                    const float llo = fabsf(tm - (up[r] ? sa : sb));      // part below z = lo + 1
+                    acc[r] = fmaf(llo, f[r].x, fmaf(Lc - llo, f[r].y, acc[r]));
+                    fa[r] = fb[r];
+                }
+                sa = sb;
+                off += bxi ? mxo : myo;
+                rx -= bxi;
+                ry -= bxi ^ 1;
+                sxn = bxi ? sxn + dsx : sxn;
+                syn = bxi ? syn : syn + dsy;
+            };
+            while ((rx | ry) >= 0) {
+                cell();
+                if ((rx | ry) < 0) break;
+                cell();
+            }
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const int iv = base + 64 * r + lane;
+                if (act[r]) {
+                    if (ACC) col[(size_t)iv * d.nu] += sec[r] * acc[r];
+                    else col[(size_t)iv * d.nu] = sec[r] * acc[r];
+                }
+            }
+        }
+    }
+    // traced rows, and (without accumulation) zeros for the rows that miss;
+    // the rows written above are not touched again
+    if (any_exc || !ACC) {
+        for (int iv = lane; iv < d.nv; iv += 64) {
+            const float3 Pc = cell_centre(pdc, pdu, pdv, view, iu, iv, d);
+            if (colfp_ray(S, Pc, g, Lxy, sO, diag).exc) trace_into<ACC>(col + (size_t)iv * d.nu, vol, S, Pc, g);
+            else if (!ACC && (iv < lo_iv || iv > hi_iv)) col[(size_t)iv * d.nu] = 0.f;
+        }
+    }
+}
+
+// block: 4 adjacent detector columns of one view, one per wavefront
+__global__ __launch_bounds__(256) void siddon_cone_fp_col_kernel(float* __restrict__ prj, const float* __restrict__ img,
+                                                                 const float* __restrict__ imgz,
+                                                                 const float* __restrict__ pdc,
+                                                                 const float* __restrict__ pdu,
+                                                                 const float* __restrict__ pdv,
+                                                                 const float* __restrict__ psrc, Grid g, Det d,
+                                                                 int nview, int ngroup, int P, int xcd_order) {
+    int view, grp;
+    fp_tile_of(blockIdx.x, ngroup, xcd_order, view, grp);
+    const int iu = grp * 4 + (threadIdx.x >> 6);
+    if (grp >= ngroup || view >= nview || iu >= d.nu) return;
+    const __amdgpu_buffer_rsrc_t volz =
+        __builtin_amdgcn_make_buffer_rsrc((void*)imgz, (short)0, 4 * P * g.nx * g.ny, 0x00020000);
+    const __amdgpu_buffer_rsrc_t vol =
+        __builtin_amdgcn_make_buffer_rsrc((void*)img, (short)0, 4 * g.nx * g.ny * g.nz, 0x00020000);
+    fp_column<true>(prj, volz, vol, pdc, pdu, pdv, psrc, g, d, view, iu, P);
+}
+
+// Block order of the forward projection: tuning cone_fp_order 1 plain, 2 XCD
+// bands; by default plain for the column-driven form (mode 0; its work per
+// detector column varies too much across a view for bands) and XCD bands for
+// the ray-per-lane form.
+static int fp_xcd_order(int mode) {
+    const int o = tuning_get("cone_fp_order", 0);
+    if (o < 0 || o > 2) fail("cone_fp_order must be 1 (plain) or 2 (XCD bands)");
+    return o == 0 ? (mode == 0 ? 0 : 1) : (o == 2 ? 1 : 0);
+}
+
 void siddon_cone_fp(float* prj, const float* img, const ConeGeomDev& geo, int B, const Grid& g, const Det& d,
                     int nview, hipStream_t stream) {
     const size_t nimg = (size_t)g.nx * g.ny * g.nz, nprj = (size_t)d.nu * d.nv * nview;
     if (nimg >= (1ull << 29)) fail("cone forward projection: volumes of 2^29 voxels or more are not supported");
+    const int mode = tuning_get("cone_fp_mode", 0);           // 0: column form, 1: ray per lane
+    if (mode != 0 && mode != 1) fail("cone_fp_mode must be 0 or 1");
+    const int xcd_order = fp_xcd_order(mode);
+    if (mode == 0) {
+        const int P = g.nz + 3;
+        const size_t ncol = (size_t)g.nx * g.ny * P;
+        if (ncol >= (1ull << 29)) fail("cone forward projection: volumes of 2^29 voxels or more are not supported");
+        Workspace& ws = workspace_for(stream);
+        Plan pl;
+        pl.add<float>(ncol);
+        ws.reserve(pl.bytes);
+        float* imgz = ws.take_n<float>(ncol);
+        const int ngroup = ceil_div(d.nu, 4);
+        const long nblocks = (long)nview * 8 * ((ngroup + 7) / 8);
+        if (nblocks > 0x7fffffffL) fail("cone forward projection: too many detector columns");
+        for (int b = 0; b < B; b++) {
+            {
+                TimedLaunch t("siddon_transpose", stream);
+                transpose_zcol_kernel<<<dim3(ceil_div(g.nx, 32), ceil_div(g.nz, 32), g.ny), 256, 0, stream>>>(
+                    imgz, img + b * nimg, g.nx, g.ny, g.nz, P);
+                CTP_LAUNCH_CHECK();
+            }
+            TimedLaunch t("siddon_cone_fp", stream);
+            siddon_cone_fp_col_kernel<<<(unsigned)nblocks, 256, 0, stream>>>(
+                prj + b * nprj, img + b * nimg, imgz, geo.det_center, geo.det_u, geo.det_v, geo.src, g, d, nview,
+                ngroup, P, xcd_order);
+            CTP_LAUNCH_CHECK();
+        }
+        return;
+    }
     Workspace& ws = workspace_for(stream);
     Plan pl;
     pl.add<float>(nimg);
@@ -168,7 +441,6 @@ void siddon_cone_fp(float* prj, const float* img, const ConeGeomDev& geo, int B,
     const int ntile = nub * nvb;
     const long nblocks = (long)nview * 8 * ((ntile + 7) / 8);
     if (nblocks > 0x7fffffffL) fail("cone forward projection: too many detector tiles");
-    const int xcd_order = tuning_get("cone_fp_order", 0) == 0 ? 1 : 0;   // 0: XCD bands, 1: plain
     dim3 block(256), grid((unsigned)nblocks);
     for (int b = 0; b < B; b++) {
         {
@@ -842,6 +1114,8 @@ struct ConeJobDev {
     float* img;               // bound volume [nz][ny][nx]
     float* prj;               // bound projections [nview][nv][nu]
     float* imgT;              // transposed copy for x-dominant views (null if none)
+    float* imgz;              // z-contiguous padded copy for the column-driven form
+    int zcol;                 // floats per z column of imgz (nz + 3)
     const float *dc, *du, *dv, *src;
     const int* swapf;
     const ConeViewBP* views;
@@ -1018,6 +1292,28 @@ __global__ __launch_bounds__(256) void siddon_cone_fp_batch_kernel(const ConeJob
     }
 }
 
+// column-driven form: a block is 4 adjacent detector columns of one view
+__global__ __launch_bounds__(256) void siddon_cone_fp_batch_col_kernel(const ConeJobDev* __restrict__ jobs, int njobs,
+                                                                       long total, int xcd_order,
+                                                                       int* __restrict__ ctr) {
+    __shared__ int slot;
+    for (int b = next_block(ctr, 8, &slot); b < total; b = next_block(ctr, 8, &slot)) {
+        int j = 0;
+        while (j + 1 < njobs && b >= jobs[j + 1].fp_begin) j++;
+        const ConeJobDev* J = jobs + j;
+        int view, grp;
+        fp_tile_of(b - J->fp_begin, J->ntile, xcd_order, view, grp);
+        const int iu = grp * 4 + (threadIdx.x >> 6);
+        if (grp >= J->ntile || view >= J->nview || iu >= J->d.nu) continue;
+        const Grid g = J->g;
+        const __amdgpu_buffer_rsrc_t volz =
+            __builtin_amdgcn_make_buffer_rsrc((void*)J->imgz, (short)0, 4 * J->zcol * g.nx * g.ny, 0x00020000);
+        const __amdgpu_buffer_rsrc_t vol =
+            __builtin_amdgcn_make_buffer_rsrc((void*)J->img, (short)0, 4 * g.nx * g.ny * g.nz, 0x00020000);
+        fp_column<false>(J->prj, volz, vol, J->dc, J->du, J->dv, J->src, g, J->d, view, iu, J->zcol);
+    }
+}
+
 template <int WU, int WV, int NZR>
 __global__ __launch_bounds__(256) void siddon_cone_bp_batch_kernel(const ConeJobDev* __restrict__ jobs, int njobs,
                                                                    long total, int* __restrict__ ctr) {
@@ -1056,7 +1352,7 @@ struct ConePlan::Impl {
     std::vector<bool> need_t;
     ConeJobDev* djobs = nullptr;
     int* ctr = nullptr;           // work counters of the persistent launches (8 forward, 1 backward)
-    int tu = 32, order = 1, nzr = 4, wu = 1, wv = 1;
+    int tu = 0, order = 1, nzr = 4, wu = 1, wv = 1;   // tu 0: column-driven forward form
     long fp_total = 0, bp_total = 0;
 
     template <typename T>
@@ -1080,9 +1376,14 @@ void ConePlan::Impl::layout() {
     const long limit = 0x7fffffffL - (1L << 24);
     fp_total = bp_total = 0;
     for (auto& J : jobs) {
-        const int TV = 64 / tu;
-        J.nub = ceil_div(J.d.nu, tu);
-        J.ntile = J.nub * ceil_div(J.d.nv, 4 * TV);
+        if (tu == 0) {                      // groups of 4 detector columns
+            J.nub = 0;
+            J.ntile = ceil_div(J.d.nu, 4);
+        } else {
+            const int TV = 64 / tu;
+            J.nub = ceil_div(J.d.nu, tu);
+            J.ntile = J.nub * ceil_div(J.d.nv, 4 * TV);
+        }
         J.nbx = ceil_div(J.g.nx, 16);
         J.nby = ceil_div(J.g.ny, 16);
         J.nbz = ceil_div(J.g.nz, nzr);
@@ -1097,16 +1398,27 @@ void ConePlan::Impl::layout() {
 
 void ConePlan::Impl::forward(hipStream_t s) {
     for (size_t j = 0; j < jobs.size(); j++) {
-        if (!need_t[j]) continue;
         const ConeJobDev& J = jobs[j];
-        transpose_xy_kernel<<<dim3(ceil_div(J.g.nx, 32), ceil_div(J.g.ny, 32), J.g.nz), 256, 0, s>>>(J.imgT, J.img,
-                                                                                                     J.g.nx, J.g.ny);
-        CTP_LAUNCH_CHECK();
+        if (tu == 0) {
+            transpose_zcol_kernel<<<dim3(ceil_div(J.g.nx, 32), ceil_div(J.g.nz, 32), J.g.ny), 256, 0, s>>>(
+                J.imgz, J.img, J.g.nx, J.g.ny, J.g.nz, J.zcol);
+            CTP_LAUNCH_CHECK();
+        } else if (need_t[j]) {
+            transpose_xy_kernel<<<dim3(ceil_div(J.g.nx, 32), ceil_div(J.g.ny, 32), J.g.nz), 256, 0, s>>>(
+                J.imgT, J.img, J.g.nx, J.g.ny);
+            CTP_LAUNCH_CHECK();
+        }
     }
     const int nj = (int)jobs.size();
     reset_counters_kernel<<<1, 64, 0, s>>>(ctr, 16);
     CTP_LAUNCH_CHECK();
     TimedLaunch t("siddon_cone_fp_batch", s);
+    if (tu == 0) {
+        auto k = siddon_cone_fp_batch_col_kernel;
+        k<<<persistent_grid(k, fp_total), 256, 0, s>>>(djobs, nj, fp_total, order, ctr);
+        CTP_LAUNCH_CHECK();
+        return;
+    }
 #define CTP_FPB(A, Bv)                                                                                     \
     if (tu == A) {                                                                                        \
         auto k = siddon_cone_fp_batch_kernel<A, Bv>;                                                      \
@@ -1157,7 +1469,8 @@ ConePlan::ConePlan(const std::vector<ConeJobSpec>& specs, bool autotune) : impl_
         const Det& d = S.d;
         const int nview = S.nview;
         const size_t nimg = (size_t)g.nx * g.ny * g.nz, nprj = (size_t)d.nu * d.nv * nview;
-        if (nimg >= (1ull << 29)) fail("cone plan: volumes of 2^29 voxels or more are not supported");
+        if (nimg >= (1ull << 29) || (size_t)g.nx * g.ny * (g.nz + 3) >= (1ull << 29))
+            fail("cone plan: volumes of 2^29 voxels or more are not supported");
         ConeGeomHost hg{S.det_center, S.det_u, S.det_v, S.src};
         ConeBPPlan bp = make_cone_bp_plan(hg, g, d, nview);
         if (!column_mode(hg, bp, nview))
@@ -1192,6 +1505,8 @@ ConePlan::ConePlan(const std::vector<ConeJobSpec>& specs, bool autotune) : impl_
         J.swapf = dsw;
         J.imgT = any ? P.alloc<float>(nimg) : nullptr;
         P.need_t.push_back(any);
+        J.zcol = g.nz + 3;
+        J.imgz = P.alloc<float>((size_t)g.nx * g.ny * J.zcol);
         ConeViewBP* dv = P.alloc<ConeViewBP>(nview);
         CTP_CHECK(hipMemcpy(dv, bp.views.data(), sizeof(ConeViewBP) * nview, hipMemcpyHostToDevice));
         J.views = dv;
@@ -1224,10 +1539,13 @@ ConePlan::ConePlan(const std::vector<ConeJobSpec>& specs, bool autotune) : impl_
 
     P.djobs = P.alloc<ConeJobDev>(P.jobs.size());
     P.ctr = P.alloc<int>(16);
-    P.tu = tuning_get("cone_fp_tile_u", 32);
-    P.order = tuning_get("cone_fp_order", 0) == 0 ? 1 : 0;
+    const int mode = tuning_get("cone_fp_mode", 0);
+    if (mode != 0 && mode != 1) fail("cone_fp_mode must be 0 or 1");
+    P.tu = mode == 0 ? 0 : tuning_get("cone_fp_tile_u", 32);
+    P.order = fp_xcd_order(mode);
     P.nzr = tuning_get("cone_bp_zrun", 4);
-    if (P.tu != 8 && P.tu != 16 && P.tu != 32 && P.tu != 64) fail("cone_fp_tile_u must be 8, 16, 32 or 64");
+    if (P.tu != 0 && P.tu != 8 && P.tu != 16 && P.tu != 32 && P.tu != 64)
+        fail("cone_fp_tile_u must be 8, 16, 32 or 64");
     if (P.nzr != 2 && P.nzr != 4 && P.nzr != 8) fail("cone plan: cone_bp_zrun must be 2, 4 or 8");
     P.layout();
     CTP_CHECK(hipDeviceSynchronize());
@@ -1244,10 +1562,11 @@ void ConePlan::config(int& tile_u, int& xcd_order, int& zrun) const {
     zrun = impl_->nzr;
 }
 
-// Autotuner: times every forward tile shape / block order and every
-// backprojection z-run on the plan's own buffers (one warm-up and two timed
-// runs each, HIP events on the null stream) and keeps the fastest.  The bound
-// buffers' contents are overwritten.
+// Autotuner: times every forward form (column driven, or ray per lane with
+// each tile shape) with each block order, and every backprojection z-run, on
+// the plan's own buffers (one warm-up and two timed runs each, HIP events on
+// the null stream) and keeps the fastest.  The bound buffers' contents are
+// overwritten.
 void ConePlan::tune() {
     Impl& P = *impl_;
     hipEvent_t a, b;
@@ -1266,7 +1585,7 @@ void ConePlan::tune() {
     };
     float best = 1e30f;
     int btu = P.tu, bor = P.order;
-    for (int tu : {16, 32, 64})
+    for (int tu : {0, 16, 32, 64})
         for (int order : {1, 0}) {
             P.tu = tu;
             P.order = order;

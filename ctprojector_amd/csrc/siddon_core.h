@@ -25,19 +25,32 @@ struct SiddonRay {
 // [z][x][y]).  move: the cone projector slides both points towards the grid
 // first; the fan projector does not (cuda/siddonFan.cu:41-46).  Returns false
 // when the ray misses the grid.
+// MoveSourceDstNearGrid (cuda/siddon.cu:318-338): slides the source and the
+// detector point along their line to within the grid's diagonal of its centre.
+__device__ inline void siddon_move(float3& s, float3& t, const Grid& g) {
+    const float ex = (float)g.nx * g.dx, ey = (float)g.ny * g.dy, ez = (float)g.nz * g.dz;
+    const float rr = sqrtf(ex * ex + ey * ey + ez * ez);
+    float3 w = make_float3(t.x - s.x, t.y - s.y, t.z - s.z);
+    const float len = sqrtf(w.x * w.x + w.y * w.y + w.z * w.z);
+    if (!(len < 1e-6f)) w = make_float3(w.x / len, w.y / len, w.z / len);
+    const float t0 = (g.cx - s.x) * w.x + (g.cy - s.y) * w.y + (g.cz - s.z) * w.z;
+    const float t1 = (t.x - g.cx) * w.x + (t.y - g.cy) * w.y + (t.z - g.cz) * w.z;
+    if (t0 > rr) s = make_float3(s.x + (t0 - rr) * w.x, s.y + (t0 - rr) * w.y, s.z + (t0 - rr) * w.z);
+    if (t1 > rr) t = make_float3(t.x - (t1 - rr) * w.x, t.y - (t1 - rr) * w.y, t.z - (t1 - rr) * w.z);
+}
+
+// Index-space direction of the (moved) segment from s to t, as the traversal
+// set-up computes it (cuda/siddon.cu:86-96).
+__device__ inline float3 siddon_index_dir(float3 s, float3 t, const Grid& g) {
+    const float3 si = make_float3((s.x - g.cx) / g.dx + g.nx / 2.0f, (s.y - g.cy) / g.dy + g.ny / 2.0f,
+                                  (s.z - g.cz) / g.dz + g.nz / 2.0f);
+    const float3 ti = make_float3((t.x - g.cx) / g.dx + g.nx / 2.0f, (t.y - g.cy) / g.dy + g.ny / 2.0f,
+                                  (t.z - g.cz) / g.dz + g.nz / 2.0f);
+    return make_float3(ti.x - si.x, ti.y - si.y, ti.z - si.z);
+}
+
 __device__ inline bool siddon_setup(SiddonRay& r, float3 s, float3 t, Grid& g, bool sw, bool move = true) {
-    // MoveSourceDstNearGrid, cuda/siddon.cu:318-338
-    if (move) {
-        const float ex = (float)g.nx * g.dx, ey = (float)g.ny * g.dy, ez = (float)g.nz * g.dz;
-        const float rr = sqrtf(ex * ex + ey * ey + ez * ez);
-        float3 w = make_float3(t.x - s.x, t.y - s.y, t.z - s.z);
-        float len = sqrtf(w.x * w.x + w.y * w.y + w.z * w.z);
-        if (!(len < 1e-6f)) w = make_float3(w.x / len, w.y / len, w.z / len);
-        const float t0 = (g.cx - s.x) * w.x + (g.cy - s.y) * w.y + (g.cz - s.z) * w.z;
-        const float t1 = (t.x - g.cx) * w.x + (t.y - g.cy) * w.y + (t.z - g.cz) * w.z;
-        if (t0 > rr) s = make_float3(s.x + (t0 - rr) * w.x, s.y + (t0 - rr) * w.y, s.z + (t0 - rr) * w.z);
-        if (t1 > rr) t = make_float3(t.x - (t1 - rr) * w.x, t.y - (t1 - rr) * w.y, t.z - (t1 - rr) * w.z);
-    }
+    if (move) siddon_move(s, t, g);
     if (sw) {
         const float a = s.x; s.x = s.y; s.y = a;
         const float b = t.x; t.x = t.y; t.y = b;
@@ -96,7 +109,12 @@ __device__ inline bool siddon_setup(SiddonRay& r, float3 s, float3 t, Grid& g, b
 
 // One step of the traversal: returns the segment length assigned to the voxel
 // at r.off before the step and advances to the next voxel.
-__device__ inline float siddon_step(SiddonRay& r, bool& inside) {
+__device__ inline float siddon_step(SiddonRay& ray, bool& inside) {
+    // The state is worked on as a local copy: selects between fields of a
+    // referenced struct are otherwise folded into a load through a selected
+    // address, which keeps the ray in memory (the compiler then places it in
+    // LDS and every step waits on an LDS read).
+    SiddonRay r = ray;
     const float anow = fminf(fminf(r.ax, r.ay), r.az);
     const bool bx = r.ax == anow;
     const bool by = !bx && r.ay == anow;
@@ -111,7 +129,36 @@ __device__ inline float siddon_step(SiddonRay& r, bool& inside) {
     r.ay = by ? r.ay + r.day : r.ay;
     r.az = bz ? r.az + r.daz : r.az;
     inside = (r.rx | r.ry | r.rz) >= 0;
+    ray = r;
     return len;
+}
+
+// Line integral of the volume behind 'vol' along a set-up ray.  Software
+// pipelined: the voxel sequence does not depend on the voxel values, so NP
+// steps are traced first and their NP gathers are issued together; steps past
+// the exit read byte offset 0 with weight 0.
+__device__ inline float siddon_integral(__amdgpu_buffer_rsrc_t vol, SiddonRay& r) {
+    float val = 0.f;
+    bool inside = true;
+    constexpr int NP = 4;
+    while (inside) {
+        int id[NP];
+        float len[NP];
+#pragma unroll
+        for (int k = 0; k < NP; k++) {
+            id[k] = inside ? r.off : 0;
+            bool in_next;
+            const float l = siddon_step(r, in_next);
+            len[k] = inside ? l : 0.f;
+            inside = inside && in_next;
+        }
+        float f[NP];
+#pragma unroll
+        for (int k = 0; k < NP; k++) f[k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(vol, id[k], 0, 0));
+#pragma unroll
+        for (int k = 0; k < NP; k++) val = fmaf(f[k], len[k], val);
+    }
+    return val;
 }
 
 // Detector cell centre of ray (iu, iv) of a view, cuda/siddonCone.cu:27-31.

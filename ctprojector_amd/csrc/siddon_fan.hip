@@ -54,26 +54,7 @@ __global__ __launch_bounds__(256) void siddon_fan_fp_kernel(float* __restrict__ 
     fan_ray(deg[view], iu, iv, d, dsd, dso, s, t);
     SiddonRay r;
     if (!siddon_setup(r, s, t, g, false, false)) return;
-    float val = 0.f;
-    bool inside = true;
-    constexpr int NP = 4;
-    while (inside) {
-        int id[NP];
-        float len[NP];
-#pragma unroll
-        for (int k = 0; k < NP; k++) {
-            id[k] = inside ? r.off : 0;
-            bool in_next;
-            const float l = siddon_step(r, in_next);
-            len[k] = inside ? l : 0.f;
-            inside = inside && in_next;
-        }
-        float f[NP];
-#pragma unroll
-        for (int k = 0; k < NP; k++) f[k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(vol, id[k], 0, 0));
-#pragma unroll
-        for (int k = 0; k < NP; k++) val = fmaf(f[k], len[k], val);
-    }
+    const float val = siddon_integral(vol, r);
     prj[((size_t)view * d.nv + iv) * d.nu + iu] += val;
 }
 

@@ -203,7 +203,9 @@ int ctpTVSQSUpdate3D(float* out, float* var, const float* img, const float* bp, 
  * memset or host synchronisation, so they can be captured into a HIP graph.
  * They return before the work completes.  Jobs need the column geometry
  * (det_v along z, det_u horizontal in every view).  autotune != 0 times the
- * launch shapes on the bound buffers (overwriting them) and keeps the fastest. */
+ * launch shapes on the bound buffers (overwriting them) and keeps the fastest.
+ * ctpConePlanConfig reports them: forward tile width (0 = column-driven form),
+ * block order (1 = XCD bands), backprojection z-run. */
 typedef struct ctpConeJob {
     size_t nx, ny, nz;
     float dx, dy, dz, cx, cy, cz;

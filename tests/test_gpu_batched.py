@@ -93,20 +93,24 @@ def test_autotune_keeps_results(torch):
     plain = _batch(torch, cases)
     tuned = _batch(torch, cases, autotune=True)
     cfg = tuned.config()
-    assert cfg['cone_fp_tile_u'] in (16, 32, 64) and cfg['cone_bp_zrun'] in (2, 4, 8)
+    assert cfg['cone_fp_tile_u'] in (0, 16, 32, 64) and cfg['cone_bp_zrun'] in (2, 4, 8)
     for j, (g, geo, img) in enumerate(cases):
         plain.img[j].copy_(torch.from_numpy(img[0]))
         tuned.img[j].copy_(torch.from_numpy(img[0]))
     plain.forward()
     tuned.forward()
     torch.cuda.synchronize()
+    # the same forward form gives the same sums; the column-driven and the
+    # ray-per-lane forms are two FP32 restatements of them
+    print('autotuned', cfg)
+    tol = 1e-7 if cfg['cone_fp_tile_u'] == 0 else 1e-5
     for j in range(3):
-        assert rel_l2(tuned.prj[j].cpu().numpy(), plain.prj[j].cpu().numpy()) < 1e-7
+        assert rel_l2(tuned.prj[j].cpu().numpy(), plain.prj[j].cpu().numpy()) < tol
     plain.backward()
     tuned.backward()
     torch.cuda.synchronize()
     for j in range(3):
-        assert rel_l2(tuned.img[j].cpu().numpy(), plain.img[j].cpu().numpy()) < 1e-6
+        assert rel_l2(tuned.img[j].cpu().numpy(), plain.img[j].cpu().numpy()) < 10 * tol
     plain.close()
     tuned.close()
 

@@ -137,6 +137,38 @@ def test_siddon_near_axis_parallel_rows(oracle):
     _siddon_check(oracle, g, geo, img)
 
 
+@pytest.mark.parametrize('case', ['default', 'steep', 'axis'])
+def test_siddon_fp_column_and_ray_modes(oracle, tuning, case):
+    """The column-driven forward projector (rays of a detector column share
+    their xy line) and the ray-per-lane one agree with the oracle and with each
+    other.  'steep': slices thin enough that the outer rows exceed the column
+    form's slope bound and are traced ray by ray; 'axis': views at multiples
+    of 90 degrees with an odd detector width, so one column per view runs
+    exactly along x or y (traced ray by ray)."""
+    from ctprojector_amd.phantom import circular_cone_geometry
+    if case == 'default':
+        g, geo, img = small_cone(seed=21)
+    else:
+        nz, dz = (60, 0.2) if case == 'steep' else (12, 1.0)
+        g = Geo(nx=14, ny=12, nz=nz, dx=1.5, dy=1.6, dz=dz, cx=0.3, cy=-0.2, nu=25, nv=40, du=1.3, dv=0.9,
+                dsd=110.0, dso=60.0)
+        if case == 'axis':
+            geo = circular_cone_geometry(4, 60.0, 110.0, angle_range=2 * np.pi)
+        else:
+            geo = circular_cone_geometry(5, 60.0, 110.0, -1.0, 1.0, jitter=0.5, seed=4)
+        img = np.random.default_rng(22).uniform(size=[1, g.nz, g.ny, g.nx]).astype(np.float32)
+    p = _proj(g)
+    args = (geo.det_center, geo.det_u, geo.det_v, geo.src)
+    ref = oracle.siddon_cone_fp(g, img, *args)
+    a = cone.siddon_fp_arbitrary(p, img, *args)
+    tuning('cone_fp_mode', 1)
+    b = cone.siddon_fp_arbitrary(p, img, *args)
+    print(case, 'column', rel_l2(a, ref), 'ray', rel_l2(b, ref))
+    assert rel_l2(a, ref) < TOL
+    assert rel_l2(b, ref) < TOL
+    assert rel_l2(a, b) < TOL
+
+
 # ------------------------------------------------------------ DD fan / parallel
 @pytest.mark.parametrize('branchless', [False, True])
 @pytest.mark.parametrize('variant', ['plain', 'offsets', 'batch'])

@@ -775,12 +775,21 @@ __global__ __launch_bounds__(256) void siddon_cone_bp_kernel(float* __restrict__
 // with the forward projector's arithmetic, so the result is the reference's
 // in every case.
 // ---------------------------------------------------------------------------
+// Weight table of the column-mode backprojector, laid out so that a voxel's
+// candidate window is contiguous: entry (view, iv, iu, jv) = weight of ray
+// (iv + jv, iu), jv < dup (the window height).  Ray i = (view, iv, iu) writes
+// its weight into the entries (iv - jv, iu, jv) of its view.
+__device__ inline void store_window_weight(float* __restrict__ wtab, long i, int iv, int nu, int dup, float w) {
+    for (int jv = 0; jv < dup && jv <= iv; jv++) wtab[(i - (long)jv * nu) * dup + jv] = w;
+}
+
 __global__ __launch_bounds__(256) void siddon_bp_coltable_kernel(float* __restrict__ wtab, float2* __restrict__ utab,
-                                                                 float* __restrict__ vtab, float* __restrict__ img,
+                                                                 float2* __restrict__ vtab, ConeViewBP* __restrict__ views,
+                                                                 float* __restrict__ img,
                                                                  const float* __restrict__ prj,
                                                                  const float* __restrict__ pdc, const float* __restrict__ pdu,
                                                                  const float* __restrict__ pdv, const float* __restrict__ psrc,
-                                                                 Grid g, Det d, int nview) {
+                                                                 Grid g, Det d, int nview, int dup) {
     const long n = (long)d.nu * d.nv * nview;
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -840,10 +849,17 @@ __global__ __launch_bounds__(256) void siddon_bp_coltable_kernel(float* __restri
     }
     if (iv == 0) utab[(long)view * d.nu + iu] = make_float2(cpar[0] ? 0.f : (float)(1.0 / wi[0]),
                                                              cpar[1] ? 0.f : (float)(1.0 / wi[1]));
-    if (iu == 0) vtab[(long)view * d.nv + iv] = cpar[2] ? 0.f : (float)(1.0 / wi[2]);
+    if (iu == 0) {
+        const float q = cpar[2] ? 0.f : (float)(1.0 / wi[2]);
+        vtab[(long)view * d.nv + iv] = make_float2(q, 0.5f * fabsf(q));
+    }
+    // a zero table entry (a column or row exactly parallel to an axis) needs the
+    // backprojector's handling of parallel rays in this view (flags cleared by
+    // the host before the launch)
+    if ((iv == 0 && (cpar[0] || cpar[1])) || (iu == 0 && cpar[2])) views[view].flags = 1;
     const bool exception = rpar[0] != cpar[0] || rpar[1] != cpar[1] || rpar[2] != cpar[2];
     const float p = prj[i];
-    wtab[i] = (hit && !exception) ? (float)(p * L) : 0.f;
+    store_window_weight(wtab, i, iv, d.nu, dup, (hit && !exception) ? (float)(p * L) : 0.f);
     if (exception && p != 0.f) {
         SiddonRay r;
         Grid gg = g;
@@ -858,15 +874,34 @@ __global__ __launch_bounds__(256) void siddon_bp_coltable_kernel(float* __restri
     }
 }
 
+// N consecutive floats (N <= 4) from a buffer with one load instruction
+template <int N>
+__device__ inline void load_floats(float* out, __amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    static_assert(N >= 1 && N <= 4, "1 to 4 floats");
+    if constexpr (N == 1) {
+        out[0] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+    } else if constexpr (N == 2) {
+        const float2 v = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+        out[0] = v.x; out[1] = v.y;
+    } else if constexpr (N == 3) {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b96(r, voff, soff, 0);
+        out[0] = __uint_as_float(v[0]); out[1] = __uint_as_float(v[1]); out[2] = __uint_as_float(v[2]);
+    } else {
+        const float4 v = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+        out[0] = v.x; out[1] = v.y; out[2] = v.z; out[3] = v.w;
+    }
+}
+
 // WU x WV is the largest candidate window compiled in; wu x wv (<= WU x WV,
 // uniform across the block) is the window used.
 template <int WU, int WV, int NZR, bool SLOW>
 __device__ inline void bp_view_col(float* acc, const ConeViewBP& cv, const float* __restrict__ wt,
-                                   const float2* __restrict__ ut, const float* __restrict__ vt, int nu, int nv, int ix,
+                                   const float2* __restrict__ ut, const float2* __restrict__ vt, int nu, int nv, int ix,
                                    int iy, int iz0, float xc, float yc, int wu, int wv) {
-    const __amdgpu_buffer_rsrc_t rw = __builtin_amdgcn_make_buffer_rsrc((void*)wt, (short)0, 4 * nu * nv, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rw =
+        __builtin_amdgcn_make_buffer_rsrc((void*)wt, (short)0, 4 * (nu * nv * WV + 16), 0x00020000);
     const __amdgpu_buffer_rsrc_t ru = __builtin_amdgcn_make_buffer_rsrc((void*)ut, (short)0, 8 * nu, 0x00020000);
-    const __amdgpu_buffer_rsrc_t rv = __builtin_amdgcn_make_buffer_rsrc((void*)vt, (short)0, 4 * nv, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rv = __builtin_amdgcn_make_buffer_rsrc((void*)vt, (short)0, 8 * (nv + 4), 0x00020000);
     const float ex = xc - cv.sx, ey = yc - cv.sy;
     // pu[2] = pw[2] = 0 in column mode: the u coordinate and the denominator
     // do not depend on z
@@ -891,7 +926,7 @@ __device__ inline void bp_view_col(float* acc, const ConeViewBP& cv, const float
             if (q.y == 0.f) { y0 = iny ? -big : big; y1 = iny ? big : -big; }
         }
         lo[j] = fmaxf(fmaxf(x0, y0), 0.f);
-        hi[j] = fminf(x1, y1);
+        hi[j] = fmaxf(fminf(x1, y1), lo[j]);       // empty interval: hi = lo
     }
 #pragma unroll
     for (int k = 0; k < NZR; k++) {
@@ -900,30 +935,50 @@ __device__ inline void bp_view_col(float* acc, const ConeViewBP& cv, const float
         const int v0 = min(max((int)ceilf(vc - cv.hv), 0), nv - wv);
         const float ez = zc - cv.sz;
         const bool inz = (float)(iz0 + k) == cv.fixz;
-        float qz[WV], w[WV][WU];
+        // The weights of the whole WU x WV window are contiguous in the window
+        // layout (entry (iv, iu, jv) = weight of ray (iv + jv, iu)): one to
+        // four loads; the row table takes one load per two rows.  Entries past
+        // (wu, wv) are read but not used: a window load reaches at most
+        // (WU - wu) * WV floats past the view's last entry and a row-table load
+        // at most one row past nv - 1 (both tables are padded).
+        const int wbase = 4 * WV * (__umul24(v0, nu) + u0), vbase = 8 * v0;   // (nu < 2^24: checked on the host)
+        float qv[2 * WV];
+        float span[WU * WV];
 #pragma unroll
-        for (int jv = 0; jv < WV; jv++) {
-            qz[jv] = 0.f;
-            if (jv >= wv) continue;
-            qz[jv] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rv, 4 * (v0 + jv), 0, 0));
-#pragma unroll
-            for (int j = 0; j < WU; j++) {
-                w[jv][j] = 0.f;
-                if (j < wu)
-                    w[jv][j] = __uint_as_float(
-                        __builtin_amdgcn_raw_buffer_load_b32(rw, 4 * ((v0 + jv) * nu + u0 + j), 0, 0));
-            }
+        for (int jv = 0; jv < WV; jv += 2) {
+            const int r = min(jv, wv - 1);
+            if (jv + 1 < WV) load_floats<4>(qv + 2 * jv, rv, vbase, 8 * r);
+            else load_floats<2>(qv + 2 * jv, rv, vbase, 8 * r);
         }
+#pragma unroll
+        for (int c = 0; c < WU * WV; c += 4) {
+            constexpr int N = WU * WV;
+            if (N - c >= 4) load_floats<4>(span + c, rw, wbase, 4 * c);
+            else if (N - c == 3) load_floats<3>(span + c, rw, wbase, 4 * c);
+            else if (N - c == 2) load_floats<2>(span + c, rw, wbase, 4 * c);
+            else load_floats<1>(span + c, rw, wbase, 4 * c);
+        }
+        float w[WV][WU];
+#pragma unroll
+        for (int jv = 0; jv < WV; jv++)
+#pragma unroll
+            for (int j = 0; j < WU; j++) w[jv][j] = span[j * WV + jv];
+        float2 qz[WV];
+#pragma unroll
+        for (int jv = 0; jv < WV; jv++) qz[jv] = make_float2(qv[2 * jv], qv[2 * jv + 1]);
         float a = 0.f;
 #pragma unroll
         for (int jv = 0; jv < WV; jv++) {
             if (jv >= wv) continue;
-            const float mz = ez * qz[jv];
-            float z0 = fmaf(fabsf(qz[jv]), -0.5f, mz), z1 = fmaf(fabsf(qz[jv]), 0.5f, mz);
-            if (SLOW && qz[jv] == 0.f) { z0 = inz ? -big : big; z1 = inz ? big : -big; }
+            // z-slab interval of the ray: ez * qz -+ |qz| / 2
+            float z0 = fmaf(ez, qz[jv].x, -qz[jv].y), z1 = fmaf(ez, qz[jv].x, qz[jv].y);
+            if (SLOW && qz[jv].x == 0.f) { z0 = inz ? -big : big; z1 = big; }
+            // overlap of [z0, z1] with [lo, hi] (lo <= hi): both ends clamped into it
 #pragma unroll
             for (int j = 0; j < WU; j++)
-                if (j < wu) a = fmaf(fmaxf(fminf(hi[j], z1) - fmaxf(lo[j], z0), 0.f), w[jv][j], a);
+                if (j < wu)
+                    a = fmaf(__builtin_amdgcn_fmed3f(z1, lo[j], hi[j]) - __builtin_amdgcn_fmed3f(z0, lo[j], hi[j]),
+                             w[jv][j], a);
         }
         acc[k] += a;
     }
@@ -933,7 +988,7 @@ __device__ inline void bp_view_col(float* acc, const ConeViewBP& cv, const float
 // img += A^T y over all views.
 template <int WU, int WV, int NZR>
 __device__ inline void bp_col_block(float* __restrict__ img, const float* __restrict__ wtab,
-                                    const float2* __restrict__ utab, const float* __restrict__ vtab,
+                                    const float2* __restrict__ utab, const float2* __restrict__ vtab,
                                     const ConeViewBP* __restrict__ views, int nview, const Grid& g, int nu, int nv,
                                     int bx, int by, int bz, int wu = WU, int wv = WV) {
     const int ix = bx * 16 + (threadIdx.x & 15);
@@ -947,9 +1002,9 @@ __device__ inline void bp_col_block(float* __restrict__ img, const float* __rest
     const size_t nuv = (size_t)nu * nv;
     for (int view = 0; view < nview; view++) {
         const ConeViewBP cv = views[view];
-        const float* wt = wtab + view * nuv;
+        const float* wt = wtab + view * nuv * WV;
         const float2* ut = utab + (size_t)view * nu;
-        const float* vt = vtab + (size_t)view * nv;
+        const float2* vt = vtab + (size_t)view * nv;
         if (cv.flags & 1) bp_view_col<WU, WV, NZR, true>(acc, cv, wt, ut, vt, nu, nv, ix, iy, iz0, xc, yc, wu, wv);
         else bp_view_col<WU, WV, NZR, false>(acc, cv, wt, ut, vt, nu, nv, ix, iy, iz0, xc, yc, wu, wv);
     }
@@ -963,14 +1018,14 @@ __device__ inline void bp_col_block(float* __restrict__ img, const float* __rest
 template <int WU, int WV, int NZR>
 __global__ __launch_bounds__(256) void siddon_cone_bp_col_kernel(float* __restrict__ img, const float* __restrict__ wtab,
                                                                  const float2* __restrict__ utab,
-                                                                 const float* __restrict__ vtab,
+                                                                 const float2* __restrict__ vtab,
                                                                  const ConeViewBP* __restrict__ views, int nview,
                                                                  Grid g, int nu, int nv) {
     bp_col_block<WU, WV, NZR>(img, wtab, utab, vtab, views, nview, g, nu, nv, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 template <int WU, int WV, int NZR>
-static void launch_bp_col_n(float* img, const float* wtab, const float2* utab, const float* vtab,
+static void launch_bp_col_n(float* img, const float* wtab, const float2* utab, const float2* vtab,
                             const ConeViewBP* views, int nview, const Grid& g, const Det& d, hipStream_t stream) {
     dim3 block(256), grid(ceil_div(g.nx, 16), ceil_div(g.ny, 16), ceil_div(g.nz, NZR));
     TimedLaunch t("siddon_cone_bp", stream);
@@ -979,9 +1034,9 @@ static void launch_bp_col_n(float* img, const float* wtab, const float2* utab, c
 }
 
 template <int WU, int WV>
-static void launch_bp_col(float* img, const float* wtab, const float2* utab, const float* vtab,
+static void launch_bp_col(float* img, const float* wtab, const float2* utab, const float2* vtab,
                           const ConeViewBP* views, int nview, const Grid& g, const Det& d, hipStream_t stream) {
-    const int nzr = tuning_get("cone_bp_zrun", 4);
+    const int nzr = tuning_get("cone_bp_zrun", 8);
     switch (nzr) {
         case 1: launch_bp_col_n<WU, WV, 1>(img, wtab, utab, vtab, views, nview, g, d, stream); break;
         case 2: launch_bp_col_n<WU, WV, 2>(img, wtab, utab, vtab, views, nview, g, d, stream); break;
@@ -1034,15 +1089,15 @@ void siddon_cone_bp(float* img, const float* prj, const ConeGeomDev& geo, const 
     wu = std::min(std::max(wu, 1), d.nu);
     wv = std::min(std::max(wv, 1), d.nv);
     // cone_bp_mode: 0 = column mode where the geometry allows it, 1 = general
-    const bool col = tuning_get("cone_bp_mode", 0) == 0 && column_mode(hgeo, plan, nview);
+    const bool col = tuning_get("cone_bp_mode", 0) == 0 && column_mode(hgeo, plan, nview) && d.nu < (1 << 24);
     const size_t nimg = (size_t)g.nx * g.ny * g.nz, nprj = (size_t)d.nu * d.nv * nview;
     Workspace& ws = workspace_for(stream);
     Plan pl;
     pl.add<ConeViewBP>(nview);
     if (col) {
-        pl.add<float>(nprj);
+        pl.add<float>(nprj * wv + 16);                // window layout, padded: window loads
         pl.add<float2>((size_t)nview * d.nu);
-        pl.add<float>((size_t)nview * d.nv);
+        pl.add<float2>((size_t)nview * d.nv + 4);
     } else {
         pl.add<float4>(nprj);
     }
@@ -1050,15 +1105,19 @@ void siddon_cone_bp(float* img, const float* prj, const ConeGeomDev& geo, const 
     ConeViewBP* dviews = ws.take_n<ConeViewBP>(nview);
     float* wtab = nullptr;
     float2* utab = nullptr;
-    float* vtab = nullptr;
+    float2* vtab = nullptr;
     float4* table = nullptr;
     if (col) {
-        wtab = ws.take_n<float>(nprj);
+        wtab = ws.take_n<float>(nprj * wv + 16);
         utab = ws.take_n<float2>((size_t)nview * d.nu);
-        vtab = ws.take_n<float>((size_t)nview * d.nv);
+        vtab = ws.take_n<float2>((size_t)nview * d.nv + 4);
     } else {
         table = ws.take_n<float4>(nprj);
     }
+    // column mode: the table kernel flags the views that need the parallel-ray
+    // handling (a zero table entry); the geometric flag is conservative
+    if (col)
+        for (ConeViewBP& cv : plan.views) cv.flags = 0;
     CTP_CHECK(hipMemcpyAsync(dviews, plan.views.data(), sizeof(ConeViewBP) * nview, hipMemcpyHostToDevice, stream));
     for (int b = 0; b < B; b++) {
         float* im = img + b * nimg;
@@ -1066,7 +1125,8 @@ void siddon_cone_bp(float* img, const float* prj, const ConeGeomDev& geo, const 
             {
                 TimedLaunch tt("siddon_bp_table", stream);
                 siddon_bp_coltable_kernel<<<ceil_div((long)nprj, 256), 256, 0, stream>>>(
-                    wtab, utab, vtab, im, prj + b * nprj, geo.det_center, geo.det_u, geo.det_v, geo.src, g, d, nview);
+                    wtab, utab, vtab, dviews, im, prj + b * nprj, geo.det_center, geo.det_u, geo.det_v, geo.src, g, d,
+                    nview, wv);
                 CTP_LAUNCH_CHECK();
             }
 #define CTP_BPC_CASE(A, Bv) \
@@ -1119,10 +1179,10 @@ struct ConeJobDev {
     const float *dc, *du, *dv, *src;
     const int* swapf;
     const ConeViewBP* views;
-    float* wtab;              // per call: prj * lhit
+    float* wtab;              // per call: prj * lhit, window layout of height wv of the plan
     const float* lhit;        // per ray: |q - S| if the ray hits the grid (and is regular), else 0
     const float2* utab;
-    const float* vtab;
+    const float2* vtab;
     int nub, ntile;           // forward tiles per view
     int nbx, nby, nbz;        // backprojection blocks
     int wu, wv;               // backprojection candidate window of this job
@@ -1134,7 +1194,7 @@ struct ConeJobDev {
 // siddon_bp_coltable_kernel.
 __device__ inline void col_ray(long i, const float* __restrict__ pdc, const float* __restrict__ pdu,
                                const float* __restrict__ pdv, const float* __restrict__ psrc, const Grid& g,
-                               const Det& d, double& L, bool& hit, bool& exception, float2& uinv, float& vinv) {
+                               const Det& d, double& L, bool& hit, bool& exception, float2& uinv, float2& vinv) {
     const int iu = (int)(i % d.nu);
     const int iv = (int)((i / d.nu) % d.nv);
     const int view = (int)(i / ((long)d.nu * d.nv));
@@ -1190,14 +1250,16 @@ __device__ inline void col_ray(long i, const float* __restrict__ pdc, const floa
             if (fixed < 0 || fixed >= nn[k]) hit = false;
         }
     uinv = make_float2(cpar[0] ? 0.f : (float)(1.0 / wi[0]), cpar[1] ? 0.f : (float)(1.0 / wi[1]));
-    vinv = cpar[2] ? 0.f : (float)(1.0 / wi[2]);
+    const float qz = cpar[2] ? 0.f : (float)(1.0 / wi[2]);
+    vinv = make_float2(qz, 0.5f * fabsf(qz));
     exception = rpar[0] != cpar[0] || rpar[1] != cpar[1] || rpar[2] != cpar[2];
 }
 
 // plan creation: lhit, utab, vtab and the list of exception rays (count only
 // when list is null)
 __global__ __launch_bounds__(256) void siddon_plan_rays_kernel(float* __restrict__ lhit, float2* __restrict__ utab,
-                                                               float* __restrict__ vtab, int* __restrict__ nexc,
+                                                               float2* __restrict__ vtab,
+                                                               ConeViewBP* __restrict__ views, int* __restrict__ nexc,
                                                                int* __restrict__ exc, const float* __restrict__ pdc,
                                                                const float* __restrict__ pdu,
                                                                const float* __restrict__ pdv,
@@ -1207,12 +1269,13 @@ __global__ __launch_bounds__(256) void siddon_plan_rays_kernel(float* __restrict
     if (i >= n) return;
     double L;
     bool hit, ex;
-    float2 ui;
-    float vi;
+    float2 ui, vi;
     col_ray(i, pdc, pdu, pdv, psrc, g, d, L, hit, ex, ui, vi);
     const int iu = (int)(i % d.nu), iv = (int)((i / d.nu) % d.nv), view = (int)(i / ((long)d.nu * d.nv));
     if (iv == 0) utab[(long)view * d.nu + iu] = ui;
     if (iu == 0) vtab[(long)view * d.nv + iv] = vi;
+    // zero entries (axis-parallel column or row): parallel-ray handling in this view
+    if ((iv == 0 && (ui.x == 0.f || ui.y == 0.f)) || (iu == 0 && vi.x == 0.f)) views[view].flags = 1;
     lhit[i] = (hit && !ex) ? (float)L : 0.f;
     if (ex) {
         const int k = atomicAdd(nexc, 1);
@@ -1225,10 +1288,14 @@ __global__ __launch_bounds__(256) void fill_zero_kernel(float* __restrict__ p, s
     if (i < n) p[i] = 0.f;
 }
 
-__global__ __launch_bounds__(256) void mul_kernel(float* __restrict__ w, const float* __restrict__ a,
-                                                  const float* __restrict__ b, size_t n) {
+// weights prj * lhit in the window layout of the column backprojector
+__global__ __launch_bounds__(256) void window_weights_kernel(float* __restrict__ w, const float* __restrict__ prj,
+                                                             const float* __restrict__ lhit, size_t n, int nu, int nv,
+                                                             int dup) {
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (i < n) w[i] = a[i] * b[i];
+    if (i >= n) return;
+    const int iv = (int)((i / nu) % nv);
+    store_window_weight(w, (long)i, iv, nu, dup, prj[i] * lhit[i]);
 }
 
 // ray-driven scatter of the listed rays (the forward projector's traversal),
@@ -1435,7 +1502,8 @@ void ConePlan::Impl::backward(hipStream_t s) {
         const size_t nimg = (size_t)J.g.nx * J.g.ny * J.g.nz, nprj = (size_t)J.d.nu * J.d.nv * J.nview;
         fill_zero_kernel<<<(unsigned)ceil_div((long)nimg, 256), 256, 0, s>>>(J.img, nimg);
         CTP_LAUNCH_CHECK();
-        mul_kernel<<<(unsigned)ceil_div((long)nprj, 256), 256, 0, s>>>(J.wtab, J.prj, J.lhit, nprj);
+        window_weights_kernel<<<(unsigned)ceil_div((long)nprj, 256), 256, 0, s>>>(J.wtab, J.prj, J.lhit, nprj, J.d.nu,
+                                                                                   J.d.nv, wv);
         CTP_LAUNCH_CHECK();
         if (nexc[j] > 0) {
             siddon_scatter_rays_kernel<<<ceil_div(nexc[j], 256), 256, 0, s>>>(J.img, J.prj, exc[j], nexc[j], J.dc, J.du,
@@ -1475,6 +1543,7 @@ ConePlan::ConePlan(const std::vector<ConeJobSpec>& specs, bool autotune) : impl_
         ConeBPPlan bp = make_cone_bp_plan(hg, g, d, nview);
         if (!column_mode(hg, bp, nview))
             fail("cone plan: every view must have det_v along z and det_u horizontal (column geometry)");
+        if (d.nu >= (1 << 24)) fail("cone plan: detectors of 2^24 columns or more are not supported");
         if (bp.wu > 4 || bp.wv > 4)
             fail("cone plan: voxel shadow spans more than 4 detector cells; use finer voxels or coarser cells");
         ConeJobDev J{};
@@ -1508,14 +1577,15 @@ ConePlan::ConePlan(const std::vector<ConeJobSpec>& specs, bool autotune) : impl_
         J.zcol = g.nz + 3;
         J.imgz = P.alloc<float>((size_t)g.nx * g.ny * J.zcol);
         ConeViewBP* dv = P.alloc<ConeViewBP>(nview);
+        for (ConeViewBP& cv : bp.views) cv.flags = 0;        // set by siddon_plan_rays_kernel
         CTP_CHECK(hipMemcpy(dv, bp.views.data(), sizeof(ConeViewBP) * nview, hipMemcpyHostToDevice));
         J.views = dv;
         float* lhit = P.alloc<float>(nprj);
         float2* ut = P.alloc<float2>((size_t)nview * d.nu);
-        float* vt = P.alloc<float>((size_t)nview * d.nv);
+        float2* vt = P.alloc<float2>((size_t)nview * d.nv + 4);   // padded: row-table loads
         int* cnt = P.alloc<int>(1);
         CTP_CHECK(hipMemset(cnt, 0, sizeof(int)));
-        siddon_plan_rays_kernel<<<ceil_div((long)nprj, 256), 256>>>(lhit, ut, vt, cnt, nullptr, J.dc, J.du, J.dv, J.src,
+        siddon_plan_rays_kernel<<<ceil_div((long)nprj, 256), 256>>>(lhit, ut, vt, dv, cnt, nullptr, J.dc, J.du, J.dv, J.src,
                                                                    g, d, nview);
         CTP_LAUNCH_CHECK();
         int n = 0;
@@ -1524,7 +1594,7 @@ ConePlan::ConePlan(const std::vector<ConeJobSpec>& specs, bool autotune) : impl_
         if (n > 0) {
             list = P.alloc<int>(n);
             CTP_CHECK(hipMemset(cnt, 0, sizeof(int)));
-            siddon_plan_rays_kernel<<<ceil_div((long)nprj, 256), 256>>>(lhit, ut, vt, cnt, list, J.dc, J.du, J.dv,
+            siddon_plan_rays_kernel<<<ceil_div((long)nprj, 256), 256>>>(lhit, ut, vt, dv, cnt, list, J.dc, J.du, J.dv,
                                                                        J.src, g, d, nview);
             CTP_LAUNCH_CHECK();
         }
@@ -1533,17 +1603,19 @@ ConePlan::ConePlan(const std::vector<ConeJobSpec>& specs, bool autotune) : impl_
         J.lhit = lhit;
         J.utab = ut;
         J.vtab = vt;
-        J.wtab = P.alloc<float>(nprj);
         P.jobs.push_back(J);
     }
 
+    // weight tables in the window layout of the plan's largest window height
+    for (ConeJobDev& J : P.jobs)
+        J.wtab = P.alloc<float>((size_t)J.d.nu * J.d.nv * J.nview * P.wv + 16);
     P.djobs = P.alloc<ConeJobDev>(P.jobs.size());
     P.ctr = P.alloc<int>(16);
     const int mode = tuning_get("cone_fp_mode", 0);
     if (mode != 0 && mode != 1) fail("cone_fp_mode must be 0 or 1");
     P.tu = mode == 0 ? 0 : tuning_get("cone_fp_tile_u", 32);
     P.order = fp_xcd_order(mode);
-    P.nzr = tuning_get("cone_bp_zrun", 4);
+    P.nzr = tuning_get("cone_bp_zrun", 8);
     if (P.tu != 0 && P.tu != 8 && P.tu != 16 && P.tu != 32 && P.tu != 64)
         fail("cone_fp_tile_u must be 8, 16, 32 or 64");
     if (P.nzr != 2 && P.nzr != 4 && P.nzr != 8) fail("cone plan: cone_bp_zrun must be 2, 4 or 8");

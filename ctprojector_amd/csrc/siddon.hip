@@ -783,19 +783,14 @@ __device__ inline void store_window_weight(float* __restrict__ wtab, long i, int
     for (int jv = 0; jv < dup && jv <= iv; jv++) wtab[(i - (long)jv * nu) * dup + jv] = w;
 }
 
-__global__ __launch_bounds__(256) void siddon_bp_coltable_kernel(float* __restrict__ wtab, float2* __restrict__ utab,
-                                                                 float2* __restrict__ vtab, ConeViewBP* __restrict__ views,
-                                                                 float* __restrict__ img,
-                                                                 const float* __restrict__ prj,
-                                                                 const float* __restrict__ pdc, const float* __restrict__ pdu,
-                                                                 const float* __restrict__ pdv, const float* __restrict__ psrc,
-                                                                 Grid g, Det d, int nview, int dup) {
-    const long n = (long)d.nu * d.nv * nview;
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int iu = (int)(i % d.nu);
-    const int iv = (int)((i / d.nu) % d.nv);
-    const int view = (int)(i / ((long)d.nu * d.nv));
+// Ray i = (view, iv, iu) in FP64 with the reference's rules: the weight (0 for
+// rays that miss, or that follow the axis-parallel rule on an axis they are
+// not parallel to) and the exact scatter of those exception rays.
+__device__ inline void coltable_exact(long i, int iu, int iv, int view, float* __restrict__ wtab,
+                                      float* __restrict__ img,
+                                      const float* __restrict__ prj, const float* __restrict__ pdc,
+                                      const float* __restrict__ pdu, const float* __restrict__ pdv,
+                                      const float* __restrict__ psrc, const Grid& g, const Det& d, int dup) {
     const double u = ((double)iu - d.off_u - (d.nu - 1) / 2.0) * d.du;
     const double v = ((double)iv - d.off_v - (d.nv - 1) / 2.0) * d.dv;
     double S[3], Q[3];
@@ -847,16 +842,6 @@ __global__ __launch_bounds__(256) void siddon_bp_coltable_kernel(float* __restri
             if (fixed < 0 || fixed >= nn[k]) hit = false;
         }
     }
-    if (iv == 0) utab[(long)view * d.nu + iu] = make_float2(cpar[0] ? 0.f : (float)(1.0 / wi[0]),
-                                                             cpar[1] ? 0.f : (float)(1.0 / wi[1]));
-    if (iu == 0) {
-        const float q = cpar[2] ? 0.f : (float)(1.0 / wi[2]);
-        vtab[(long)view * d.nv + iv] = make_float2(q, 0.5f * fabsf(q));
-    }
-    // a zero table entry (a column or row exactly parallel to an axis) needs the
-    // backprojector's handling of parallel rays in this view (flags cleared by
-    // the host before the launch)
-    if ((iv == 0 && (cpar[0] || cpar[1])) || (iu == 0 && cpar[2])) views[view].flags = 1;
     const bool exception = rpar[0] != cpar[0] || rpar[1] != cpar[1] || rpar[2] != cpar[2];
     const float p = prj[i];
     store_window_weight(wtab, i, iv, d.nu, dup, (hit && !exception) ? (float)(p * L) : 0.f);
@@ -872,6 +857,86 @@ __global__ __launch_bounds__(256) void siddon_bp_coltable_kernel(float* __restri
             }
         }
     }
+}
+
+// Column and row tables of the column-mode BP (1/w_x, 1/w_y per column from
+// the ray of row 0; 1/w_z, |1/w_z|/2 per row from the ray of column 0; FP64,
+// 0 for an axis the ray is exactly parallel to) and the view's parallel-ray
+// flag.  One thread per column and per row of every view.
+__global__ __launch_bounds__(256) void siddon_bp_colaxes_kernel(float2* __restrict__ utab, float2* __restrict__ vtab,
+                                                                ConeViewBP* __restrict__ views,
+                                                                const float* __restrict__ pdc,
+                                                                const float* __restrict__ pdu,
+                                                                const float* __restrict__ pdv,
+                                                                const float* __restrict__ psrc, Grid g, Det d,
+                                                                int nview) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int per = d.nu + d.nv;
+    if (t >= (long)per * nview) return;
+    const int view = (int)(t / per), k = (int)(t % per);
+    const bool column = k < d.nu;
+    const int iu = column ? k : 0, iv = column ? 0 : k - d.nu;
+    const double u = ((double)iu - d.off_u - (d.nu - 1) / 2.0) * d.du;
+    const double v = ((double)iv - d.off_v - (d.nv - 1) / 2.0) * d.dv;
+    const double dd[3] = {g.dx, g.dy, g.dz};
+    double wi[3];
+    bool cpar[3];
+    for (int a = 0; a < 3; a++) {
+        const double S = psrc[3 * view + a];
+        const double Q = pdc[3 * view + a] + pdu[3 * view + a] * u + pdv[3 * view + a] * v;
+        wi[a] = (Q - S) / dd[a];
+        cpar[a] = (Q - S) == 0.0;
+    }
+    if (column) {
+        utab[(long)view * d.nu + iu] =
+            make_float2(cpar[0] ? 0.f : (float)(1.0 / wi[0]), cpar[1] ? 0.f : (float)(1.0 / wi[1]));
+        if (cpar[0] || cpar[1]) views[view].flags = 1;
+    } else {
+        const float q = cpar[2] ? 0.f : (float)(1.0 / wi[2]);
+        vtab[(long)view * d.nv + iv] = make_float2(q, 0.5f * fabsf(q));
+        if (cpar[2]) views[view].flags = 1;
+    }
+}
+
+// Column-mode BP weights per call.  A ray with every component of its (moved)
+// direction at least 1e-5 voxels from zero is regular under the reference's
+// rules whatever the rounding: its weight is prj * |Q - S|, with hits decided
+// by the geometry of the chords themselves (a ray that misses the grid has no
+// chord in any voxel).  Rays near an axis direction take the FP64 path.
+__global__ __launch_bounds__(256) void siddon_bp_coltable_kernel(float* __restrict__ wtab, float* __restrict__ img,
+                                                                 const float* __restrict__ prj,
+                                                                 const float* __restrict__ pdc, const float* __restrict__ pdu,
+                                                                 const float* __restrict__ pdv, const float* __restrict__ psrc,
+                                                                 Grid g, Det d, int nview, int dup) {
+    const long n = (long)d.nu * d.nv * nview;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int iu = (int)(i % d.nu);
+    const int iv = (int)((i / d.nu) % d.nv);
+    const int view = (int)(i / ((long)d.nu * d.nv));
+    // Q - S in FP64 as the exact path forms it (zero exactly when it is zero there)
+    const double u = ((double)iu - d.off_u - (d.nu - 1) / 2.0) * d.du;
+    const double v = ((double)iv - d.off_v - (d.nv - 1) / 2.0) * d.dv;
+    double w[3];
+    for (int k = 0; k < 3; k++)
+        w[k] = ((double)pdc[3 * view + k] + (double)pdu[3 * view + k] * u + (double)pdv[3 * view + k] * v) -
+               (double)psrc[3 * view + k];
+    // fraction of the ray kept by the move towards the grid (FP32 is enough: it
+    // scales the components, which are compared with a 10x margin)
+    const float3 Sf = ld3(psrc, view);
+    const float3 wf = make_float3((float)w[0], (float)w[1], (float)w[2]);
+    const float L = sqrtf(wf.x * wf.x + wf.y * wf.y + wf.z * wf.z);
+    const float ex = (float)g.nx * g.dx, ey = (float)g.ny * g.dy, ez = (float)g.nz * g.dz;
+    const float rr = sqrtf(ex * ex + ey * ey + ez * ez);
+    const float t0 = ((g.cx - Sf.x) * wf.x + (g.cy - Sf.y) * wf.y + (g.cz - Sf.z) * wf.z) / L;
+    const float frac = (L - fmaxf(t0 - rr, 0.f) - fmaxf(L - t0 - rr, 0.f)) / L;
+    const bool regular = fabsf(wf.x) * frac > 1e-5f * g.dx && fabsf(wf.y) * frac > 1e-5f * g.dy &&
+                         fabsf(wf.z) * frac > 1e-5f * g.dz && frac > 0.f;
+    if (!regular) {
+        coltable_exact(i, iu, iv, view, wtab, img, prj, pdc, pdu, pdv, psrc, g, d, dup);
+        return;
+    }
+    store_window_weight(wtab, i, iv, d.nu, dup, prj[i] * L);
 }
 
 // N consecutive floats (N <= 4) from a buffer with one load instruction
@@ -908,7 +973,9 @@ __device__ inline void bp_view_col(float* acc, const ConeViewBP& cv, const float
     const float rd = __builtin_amdgcn_rcpf(fmaf(cv.pw[0], xc, fmaf(cv.pw[1], yc, cv.pw[3])));
     const float uc = fmaf(cv.pu[0], xc, fmaf(cv.pu[1], yc, cv.pu[3])) * rd;
     const float bv = fmaf(cv.pv[0], xc, fmaf(cv.pv[1], yc, cv.pv[3]));
-    const int u0 = min(max((int)ceilf(uc - cv.hu), 0), nu - wu);
+    // window origins clamped as floats (one median) before the conversion
+    const int u0 = (int)__builtin_amdgcn_fmed3f(ceilf(uc - cv.hu), 0.f, (float)(nu - wu));
+    const float v0max = (float)(nv - wv);
     const bool inx = (float)ix == cv.fixx, iny = (float)iy == cv.fixy;
     const float big = 3.0e38f;
     float lo[WU], hi[WU];
@@ -932,7 +999,7 @@ __device__ inline void bp_view_col(float* acc, const ConeViewBP& cv, const float
     for (int k = 0; k < NZR; k++) {
         const float zc = (float)(iz0 + k) + 0.5f;
         const float vc = fmaf(cv.pv[2], zc, bv) * rd;
-        const int v0 = min(max((int)ceilf(vc - cv.hv), 0), nv - wv);
+        const int v0 = (int)__builtin_amdgcn_fmed3f(ceilf(vc - cv.hv), 0.f, v0max);
         const float ez = zc - cv.sz;
         const bool inz = (float)(iz0 + k) == cv.fixz;
         // The weights of the whole WU x WV window are contiguous in the window
@@ -1036,7 +1103,7 @@ static void launch_bp_col_n(float* img, const float* wtab, const float2* utab, c
 template <int WU, int WV>
 static void launch_bp_col(float* img, const float* wtab, const float2* utab, const float2* vtab,
                           const ConeViewBP* views, int nview, const Grid& g, const Det& d, hipStream_t stream) {
-    const int nzr = tuning_get("cone_bp_zrun", 8);
+    const int nzr = tuning_get("cone_bp_zrun", 4);
     switch (nzr) {
         case 1: launch_bp_col_n<WU, WV, 1>(img, wtab, utab, vtab, views, nview, g, d, stream); break;
         case 2: launch_bp_col_n<WU, WV, 2>(img, wtab, utab, vtab, views, nview, g, d, stream); break;
@@ -1124,8 +1191,11 @@ void siddon_cone_bp(float* img, const float* prj, const ConeGeomDev& geo, const 
         if (col) {
             {
                 TimedLaunch tt("siddon_bp_table", stream);
+                siddon_bp_colaxes_kernel<<<ceil_div((long)nview * (d.nu + d.nv), 256), 256, 0, stream>>>(
+                    utab, vtab, dviews, geo.det_center, geo.det_u, geo.det_v, geo.src, g, d, nview);
+                CTP_LAUNCH_CHECK();
                 siddon_bp_coltable_kernel<<<ceil_div((long)nprj, 256), 256, 0, stream>>>(
-                    wtab, utab, vtab, dviews, im, prj + b * nprj, geo.det_center, geo.det_u, geo.det_v, geo.src, g, d,
+                    wtab, im, prj + b * nprj, geo.det_center, geo.det_u, geo.det_v, geo.src, g, d,
                     nview, wv);
                 CTP_LAUNCH_CHECK();
             }
@@ -1615,7 +1685,7 @@ ConePlan::ConePlan(const std::vector<ConeJobSpec>& specs, bool autotune) : impl_
     if (mode != 0 && mode != 1) fail("cone_fp_mode must be 0 or 1");
     P.tu = mode == 0 ? 0 : tuning_get("cone_fp_tile_u", 32);
     P.order = fp_xcd_order(mode);
-    P.nzr = tuning_get("cone_bp_zrun", 8);
+    P.nzr = tuning_get("cone_bp_zrun", 4);
     if (P.tu != 0 && P.tu != 8 && P.tu != 16 && P.tu != 32 && P.tu != 64)
         fail("cone_fp_tile_u must be 8, 16, 32 or 64");
     if (P.nzr != 2 && P.nzr != 4 && P.nzr != 8) fail("cone plan: cone_bp_zrun must be 2, 4 or 8");

@@ -1249,7 +1249,7 @@ struct ConeJobDev {
     const float *dc, *du, *dv, *src;
     const int* swapf;
     const ConeViewBP* views;
-    float* wtab;              // per call: prj * lhit, window layout of height wv of the plan
+    float* wtab;              // per call: prj * lhit, window layout of height wv
     const float* lhit;        // per ray: |q - S| if the ray hits the grid (and is regular), else 0
     const float2* utab;
     const float2* vtab;
@@ -1488,8 +1488,17 @@ struct ConePlan::Impl {
     std::vector<int*> exc;
     std::vector<bool> need_t;
     ConeJobDev* djobs = nullptr;
-    int* ctr = nullptr;           // work counters of the persistent launches (8 forward, 1 backward)
-    int tu = 0, order = 1, nzr = 4, wu = 1, wv = 1;   // tu 0: column-driven forward form
+    // backprojection: one persistent launch per window shape, each with the
+    // exact-size kernel of its jobs (blocks numbered within the group)
+    struct BPGroup {
+        int wu, wv;
+        ConeJobDev* djobs;
+        int njobs;
+        long total;
+    };
+    std::vector<BPGroup> groups;
+    int* ctr = nullptr;           // work counters of the persistent launches (8 forward, one per backward group)
+    int tu = 0, order = 1, nzr = 4;   // tu 0: column-driven forward form
     long fp_total = 0, bp_total = 0;
 
     template <typename T>
@@ -1525,12 +1534,26 @@ void ConePlan::Impl::layout() {
         J.nby = ceil_div(J.g.ny, 16);
         J.nbz = ceil_div(J.g.nz, nzr);
         J.fp_begin = fp_total;
-        J.bp_begin = bp_total;
+        J.bp_begin = 0;
         fp_total += (long)J.nview * 8 * ((J.ntile + 7) / 8);
-        bp_total += (long)J.nbx * J.nby * J.nbz;
     }
-    if (fp_total > limit || bp_total > limit) fail("cone plan: too many blocks for one batch; split the jobs");
+    if (fp_total > limit) fail("cone plan: too many blocks for one batch; split the jobs");
     CTP_CHECK(hipMemcpy(djobs, jobs.data(), sizeof(ConeJobDev) * jobs.size(), hipMemcpyHostToDevice));
+    bp_total = 0;
+    for (BPGroup& G : groups) {
+        std::vector<ConeJobDev> gj;
+        G.total = 0;
+        for (const ConeJobDev& J : jobs) {
+            if (J.wu != G.wu || J.wv != G.wv) continue;
+            ConeJobDev c = J;
+            c.bp_begin = G.total;
+            G.total += (long)J.nbx * J.nby * J.nbz;
+            gj.push_back(c);
+        }
+        if (G.total > limit) fail("cone plan: too many blocks for one batch; split the jobs");
+        bp_total += G.total;
+        CTP_CHECK(hipMemcpy(G.djobs, gj.data(), sizeof(ConeJobDev) * gj.size(), hipMemcpyHostToDevice));
+    }
 }
 
 void ConePlan::Impl::forward(hipStream_t s) {
@@ -1547,7 +1570,7 @@ void ConePlan::Impl::forward(hipStream_t s) {
         }
     }
     const int nj = (int)jobs.size();
-    reset_counters_kernel<<<1, 64, 0, s>>>(ctr, 16);
+    reset_counters_kernel<<<1, 64, 0, s>>>(ctr, 8);
     CTP_LAUNCH_CHECK();
     TimedLaunch t("siddon_cone_fp_batch", s);
     if (tu == 0) {
@@ -1573,7 +1596,7 @@ void ConePlan::Impl::backward(hipStream_t s) {
         fill_zero_kernel<<<(unsigned)ceil_div((long)nimg, 256), 256, 0, s>>>(J.img, nimg);
         CTP_LAUNCH_CHECK();
         window_weights_kernel<<<(unsigned)ceil_div((long)nprj, 256), 256, 0, s>>>(J.wtab, J.prj, J.lhit, nprj, J.d.nu,
-                                                                                   J.d.nv, wv);
+                                                                                   J.d.nv, J.wv);
         CTP_LAUNCH_CHECK();
         if (nexc[j] > 0) {
             siddon_scatter_rays_kernel<<<ceil_div(nexc[j], 256), 256, 0, s>>>(J.img, J.prj, exc[j], nexc[j], J.dc, J.du,
@@ -1581,22 +1604,25 @@ void ConePlan::Impl::backward(hipStream_t s) {
             CTP_LAUNCH_CHECK();
         }
     }
-    const int nj = (int)jobs.size();
-    reset_counters_kernel<<<1, 64, 0, s>>>(ctr + 8, 8);
+    reset_counters_kernel<<<1, 64, 0, s>>>(ctr + 8, (int)groups.size());
     CTP_LAUNCH_CHECK();
     TimedLaunch t("siddon_cone_bp_batch", s);
+    for (size_t gi = 0; gi < groups.size(); gi++) {
+        const BPGroup& G = groups[gi];
+        int* gc = ctr + 8 + gi;
 #define CTP_BPB(A, Bv, Z)                                                                                  \
-    if (wu == A && wv == Bv && nzr == Z) {                                                                \
+    if (G.wu == A && G.wv == Bv && nzr == Z) {                                                            \
         auto k = siddon_cone_bp_batch_kernel<A, Bv, Z>;                                                   \
-        k<<<persistent_grid(k, bp_total), 256, 0, s>>>(djobs, nj, bp_total, ctr + 8);                     \
+        k<<<persistent_grid(k, G.total), 256, 0, s>>>(G.djobs, G.njobs, G.total, gc);                     \
     }
 #define CTP_BPB_Z(A, Bv) CTP_BPB(A, Bv, 2) CTP_BPB(A, Bv, 4) CTP_BPB(A, Bv, 8)
-    CTP_BPB_Z(1, 1) CTP_BPB_Z(1, 2) CTP_BPB_Z(2, 1) CTP_BPB_Z(2, 2) CTP_BPB_Z(1, 3) CTP_BPB_Z(3, 1)
-    CTP_BPB_Z(2, 3) CTP_BPB_Z(3, 2) CTP_BPB_Z(3, 3) CTP_BPB_Z(1, 4) CTP_BPB_Z(4, 1) CTP_BPB_Z(2, 4)
-    CTP_BPB_Z(4, 2) CTP_BPB_Z(3, 4) CTP_BPB_Z(4, 3) CTP_BPB_Z(4, 4)
+        CTP_BPB_Z(1, 1) CTP_BPB_Z(1, 2) CTP_BPB_Z(2, 1) CTP_BPB_Z(2, 2) CTP_BPB_Z(1, 3) CTP_BPB_Z(3, 1)
+        CTP_BPB_Z(2, 3) CTP_BPB_Z(3, 2) CTP_BPB_Z(3, 3) CTP_BPB_Z(1, 4) CTP_BPB_Z(4, 1) CTP_BPB_Z(2, 4)
+        CTP_BPB_Z(4, 2) CTP_BPB_Z(3, 4) CTP_BPB_Z(4, 3) CTP_BPB_Z(4, 4)
 #undef CTP_BPB_Z
 #undef CTP_BPB
-    CTP_LAUNCH_CHECK();
+        CTP_LAUNCH_CHECK();
+    }
 }
 
 ConePlan::ConePlan(const std::vector<ConeJobSpec>& specs, bool autotune) : impl_(new Impl) {
@@ -1619,8 +1645,6 @@ ConePlan::ConePlan(const std::vector<ConeJobSpec>& specs, bool autotune) : impl_
         ConeJobDev J{};
         J.wu = std::max(1, std::min(bp.wu, d.nu));
         J.wv = std::max(1, std::min(bp.wv, d.nv));
-        P.wu = std::max(P.wu, J.wu);
-        P.wv = std::max(P.wv, J.wv);
         J.g = g;
         J.d = d;
         J.nview = nview;
@@ -1676,11 +1700,18 @@ ConePlan::ConePlan(const std::vector<ConeJobSpec>& specs, bool autotune) : impl_
         P.jobs.push_back(J);
     }
 
-    // weight tables in the window layout of the plan's largest window height
-    for (ConeJobDev& J : P.jobs)
-        J.wtab = P.alloc<float>((size_t)J.d.nu * J.d.nv * J.nview * P.wv + 16);
+    // weight tables in the window layout of each job's window height
+    for (ConeJobDev& J : P.jobs) J.wtab = P.alloc<float>((size_t)J.d.nu * J.d.nv * J.nview * J.wv + 16);
     P.djobs = P.alloc<ConeJobDev>(P.jobs.size());
-    P.ctr = P.alloc<int>(16);
+    for (const ConeJobDev& J : P.jobs) {
+        bool found = false;
+        for (Impl::BPGroup& G : P.groups) found |= G.wu == J.wu && G.wv == J.wv;
+        if (found) continue;
+        int n = 0;
+        for (const ConeJobDev& K : P.jobs) n += K.wu == J.wu && K.wv == J.wv;
+        P.groups.push_back(Impl::BPGroup{J.wu, J.wv, P.alloc<ConeJobDev>(n), n, 0});
+    }
+    P.ctr = P.alloc<int>(8 + P.groups.size());
     const int mode = tuning_get("cone_fp_mode", 0);
     if (mode != 0 && mode != 1) fail("cone_fp_mode must be 0 or 1");
     P.tu = mode == 0 ? 0 : tuning_get("cone_fp_tile_u", 32);

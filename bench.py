@@ -134,14 +134,18 @@ def cone_extra(steps):
     fp_ms, _ = kernel_stats(times, 'siddon_cone_fp')
     bp_ms, _ = kernel_stats(times, 'siddon_cone_bp')
     tab_ms, _ = kernel_stats(times, 'siddon_bp_table')
+    tr_ms, _ = kernel_stats(times, 'siddon_transpose')
     out = {
         'workload': 'cone Siddon FP+BP, 256^3 @0.5 mm, 720 views, 512x384 panel, jittered helical',
         'ms_per_step': round(ms, 3),
         'value_gvu_s': round(2 * nvu / (ms / 1e3) / 1e9, 2),
-        'kernel_ms': {'siddon_cone_fp': round(fp_ms, 3), 'siddon_bp_table': round(tab_ms, 3),
-                      'siddon_cone_bp': round(bp_ms, 3)},
-        'roofline_frac_fp_bp_kernels': round(2 * 4 * nvu / ((fp_ms + bp_ms + tab_ms) / 1e3) / 1e9 / PEAK_HBM_GBS, 4)
-        if fp_ms > 0 else None,
+        'kernel_ms': {'siddon_transpose': round(tr_ms, 3), 'siddon_cone_fp': round(fp_ms, 3),
+                      'siddon_bp_table': round(tab_ms, 3), 'siddon_cone_bp': round(bp_ms, 3)},
+        # 4 B per voxel update and pass, over the summed kernel time of the step
+        'roofline_frac_fp_bp_kernels': round(2 * 4 * nvu / ((fp_ms + bp_ms + tab_ms + tr_ms) / 1e3) / 1e9 /
+                                             PEAK_HBM_GBS, 4) if fp_ms > 0 else None,
+        'traffic_bytes': {'siddon_cone_fp': read_traffic('siddon_cone_fp'),
+                          'siddon_cone_bp': read_traffic('siddon_cone_bp')},
     }
     del prj, bp
     return out
@@ -345,7 +349,8 @@ def short_name(kname):
 
 # timing names (HIP-event scopes in the library) -> kernel symbols in rocprofv3 output
 TIMED_KERNELS = {'dd_fan_fp': 'dd_fp_kernel', 'dd_par_fp': 'dd_fp_kernel', 'dd_fan_bp': 'dd_bp_kernel',
-                 'dd_par_bp': 'dd_bp_kernel', 'siddon_cone_fp': 'siddon_cone_fp_kernel',
+                 'dd_par_bp': 'dd_bp_kernel', 'siddon_cone_fp': 'siddon_cone_fp_col_kernel',
+                 'siddon_cone_bp': 'siddon_cone_bp_col_kernel',
                  'fan_fbp_bp': 'fan_fbp_bp_kernel', 'ramp_filter': 'ramp_conv_kernel'}
 
 

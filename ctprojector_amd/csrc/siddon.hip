@@ -134,7 +134,8 @@ __global__ __launch_bounds__(256) void siddon_cone_fp_kernel(float* __restrict__
 // differ only in their slope along z.  One wavefront takes one column.  The 2D
 // traversal of the shared xy line (its cells and the interval of the in-plane
 // distance s spent in each) is the same in every lane and is stepped in
-// lockstep; each lane follows kColRays rays (detector rows) through it.  Inside
+// lockstep; each lane follows up to kColRays rays (detector rows, as many as
+// the column's rows that reach the grid need) through it.  Inside
 // an xy cell a ray's z index changes by less than 1 (steeper rays are traced
 // ray by ray), so it meets z cells lo and lo + 1 at most: the s interval is
 // split at the plane z = lo + 1, and both voxels come from one 8-byte load of
@@ -149,7 +150,7 @@ __global__ __launch_bounds__(256) void siddon_cone_fp_kernel(float* __restrict__
 // units, except an exactly horizontal ray), and rays steeper than the bound.
 // Columns whose xy line has an exactly zero x or y component, and views whose
 // det_v is not along z, are traced ray by ray as a whole.
-constexpr int kColRays = 4;
+constexpr int kColRays = 5;
 
 // [z][y][x] -> [y][x][z + 1] with zero padding (P floats per column)
 __global__ __launch_bounds__(256) void transpose_zcol_kernel(float* __restrict__ out, const float* __restrict__ in,
@@ -207,12 +208,99 @@ __device__ inline void trace_into(float* out, __amdgpu_buffer_rsrc_t vol, float3
     }
 }
 
+// Rays base + 64 r + lane (r < R) of a detector column through its xy line:
+// see fp_column.
+struct ColLine {
+    float s_in, sxn0, syn0, dsx, dsy, Lxy, sO, diag, nzf;
+    int rx0, ry0, mxo, myo, off0;
+};
+
+template <int R, bool ACC>
+__device__ inline void fp_column_chunk(float* __restrict__ col, __amdgpu_buffer_rsrc_t volz, const ColLine& L,
+                                       float3 S, const float* __restrict__ pdc, const float* __restrict__ pdu,
+                                       const float* __restrict__ pdv, const Grid& g, const Det& d, int view, int iu,
+                                       int base, int hi_iv) {
+    const int lane = threadIdx.x & 63;
+    const float s_in = L.s_in, sxn0 = L.sxn0, syn0 = L.syn0, dsx = L.dsx, dsy = L.dsy, nzf = L.nzf;
+    const int rx0 = L.rx0, ry0 = L.ry0, mxo = L.mxo, myo = L.myo, off0 = L.off0;
+    float zs[R], z0[R], rs[R], q1[R], fa[R], acc[R], sec[R];
+    bool up[R], act[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int iv = base + 64 * r + lane;
+        ColFPRay c;
+        c.exc = true;
+        if (iv <= hi_iv) c = colfp_ray(S, cell_centre(pdc, pdu, pdv, view, iu, iv, d), g, L.Lxy, L.sO, L.diag);
+        act[r] = iv <= hi_iv && !c.exc;
+        // inactive lanes follow a flat ray below the grid (zero padding)
+        zs[r] = act[r] ? c.zs : 0.f;
+        z0[r] = act[r] ? c.z0 : -4.f;
+        sec[r] = c.sec;
+        rs[r] = zs[r] != 0.f ? 1.f / zs[r] : 0.f;
+        q1[r] = zs[r] != 0.f ? (1.f - z0[r]) * rs[r] : 3.0e38f;   // s at z = lo + 1: lo * rs + q1
+        up[r] = zs[r] >= 0.f;
+        fa[r] = __builtin_amdgcn_fmed3f(floorf(fmaf(zs[r], s_in, z0[r])), -1.f, nzf);
+        acc[r] = 0.f;
+    }
+    // min / max as medians with a sentinel (no canonicalisation of
+    // loop-carried operands)
+    constexpr float kBig = 3.0e38f;
+    float sa = s_in, sxn = sxn0, syn = syn0;
+    int off = off0, rx = rx0, ry = ry0;
+    // one xy cell; the loop below runs it twice per trip so that the
+    // carried values alternate registers instead of being copied
+    auto cell = [&]() {
+        // every lane holds the same values: the choice is made scalar
+        const int bxi = __builtin_amdgcn_readfirstlane(sxn <= syn ? 1 : 0);
+        const float sb = __builtin_amdgcn_fmed3f(bxi ? sxn : syn, sa, kBig);
+        const float Lc = sb - sa;
+        float lo[R], fb[R];
+        float2 f[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            fb[r] = __builtin_amdgcn_fmed3f(floorf(fmaf(zs[r], sb, z0[r])), -1.f, nzf);
+            lo[r] = __builtin_amdgcn_fmed3f(fa[r], fb[r], -kBig);
+        }
+        // the whole offset is the vector operand (checked against the
+        // buffer's size); (lo + 1) >= 0 is the padded z index
+#pragma unroll
+        for (int r = 0; r < R; r++)
+            f[r] = __builtin_bit_cast(
+                float2, __builtin_amdgcn_raw_buffer_load_b64(volz, 4 * ((int)lo[r] + 1) + off, 0, 0));
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const float tm = __builtin_amdgcn_fmed3f(fmaf(lo[r], rs[r], q1[r]), sa, sb);
+            const float llo = fabsf(tm - (up[r] ? sa : sb));      // part below z = lo + 1
+            acc[r] = fmaf(llo, f[r].x, fmaf(Lc - llo, f[r].y, acc[r]));
+            fa[r] = fb[r];
+        }
+        sa = sb;
+        off += bxi ? mxo : myo;
+        rx -= bxi;
+        ry -= bxi ^ 1;
+        sxn = bxi ? sxn + dsx : sxn;
+        syn = bxi ? syn : syn + dsy;
+    };
+    while ((rx | ry) >= 0) {
+        cell();
+        if ((rx | ry) < 0) break;
+        cell();
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int iv = base + 64 * r + lane;
+        if (act[r]) {
+            if (ACC) col[(size_t)iv * d.nu] += sec[r] * acc[r];
+            else col[(size_t)iv * d.nu] = sec[r] * acc[r];
+        }
+    }
+}
+
 template <bool ACC>
 __device__ inline void fp_column(float* __restrict__ prj, __amdgpu_buffer_rsrc_t volz, __amdgpu_buffer_rsrc_t vol,
                                  const float* __restrict__ pdc, const float* __restrict__ pdu,
                                  const float* __restrict__ pdv, const float* __restrict__ psrc, const Grid& g,
                                  const Det& d, int view, int iu, int P) {
-    constexpr int R = kColRays;
     const int lane = threadIdx.x & 63;
     float* col = prj + (size_t)view * d.nv * d.nu + iu;        // row iv at col[iv * nu]
     const float3 S = ld3(psrc, view);
@@ -277,78 +365,18 @@ __device__ inline void fp_column(float* __restrict__ prj, __amdgpu_buffer_rsrc_t
         const int rx0 = px ? g.nx - 1 - ix : ix, ry0 = py ? g.ny - 1 - iy : iy;
         const int mxo = (px ? 4 : -4) * P, myo = (py ? 4 : -4) * P * g.nx;
         const int off0 = 4 * P * (iy * g.nx + ix);        // byte offset of column (ix, iy)
-        for (int base = lo_iv; base <= hi_iv; base += 64 * R) {
-            float zs[R], z0[R], rs[R], q1[R], fa[R], acc[R], sec[R];
-            bool up[R], act[R];
-#pragma unroll
-            for (int r = 0; r < R; r++) {
-                const int iv = base + 64 * r + lane;
-                ColFPRay c;
-                c.exc = true;
-                if (iv <= hi_iv) c = colfp_ray(S, cell_centre(pdc, pdu, pdv, view, iu, iv, d), g, Lxy, sO, diag);
-                act[r] = iv <= hi_iv && !c.exc;
-                // inactive lanes follow a flat ray below the grid (zero padding)
-                zs[r] = act[r] ? c.zs : 0.f;
-                z0[r] = act[r] ? c.z0 : -4.f;
-                sec[r] = c.sec;
-                rs[r] = zs[r] != 0.f ? 1.f / zs[r] : 0.f;
-                q1[r] = zs[r] != 0.f ? (1.f - z0[r]) * rs[r] : 3.0e38f;   // s at z = lo + 1: lo * rs + q1
-                up[r] = zs[r] >= 0.f;
-                fa[r] = __builtin_amdgcn_fmed3f(floorf(fmaf(zs[r], s_in, z0[r])), -1.f, nzf);
-                acc[r] = 0.f;
+        const ColLine line{s_in, sxn0, syn0, dsx, dsy, Lxy, sO, diag, nzf, rx0, ry0, mxo, myo, off0};
+        // up to kColRays rays per lane and traversal, as many as the rows need
+        for (int base = lo_iv; base <= hi_iv;) {
+            const int nr = min(kColRays, (hi_iv - base + 64) / 64);
+            switch (nr) {
+                case 1: fp_column_chunk<1, ACC>(col, volz, line, S, pdc, pdu, pdv, g, d, view, iu, base, hi_iv); break;
+                case 2: fp_column_chunk<2, ACC>(col, volz, line, S, pdc, pdu, pdv, g, d, view, iu, base, hi_iv); break;
+                case 3: fp_column_chunk<3, ACC>(col, volz, line, S, pdc, pdu, pdv, g, d, view, iu, base, hi_iv); break;
+                case 4: fp_column_chunk<4, ACC>(col, volz, line, S, pdc, pdu, pdv, g, d, view, iu, base, hi_iv); break;
+                default: fp_column_chunk<5, ACC>(col, volz, line, S, pdc, pdu, pdv, g, d, view, iu, base, hi_iv); break;
             }
-            // min / max as medians with a sentinel (no canonicalisation of
-            // loop-carried operands)
-            constexpr float kBig = 3.0e38f;
-            float sa = s_in, sxn = sxn0, syn = syn0;
-            int off = off0, rx = rx0, ry = ry0;
-            // one xy cell; the loop below runs it twice per trip so that the
-            // carried values alternate registers instead of being copied
-            auto cell = [&]() {
-                // every lane holds the same values: the choice is made scalar
-                const int bxi = __builtin_amdgcn_readfirstlane(sxn <= syn ? 1 : 0);
-                const float sb = __builtin_amdgcn_fmed3f(bxi ? sxn : syn, sa, kBig);
-                const float Lc = sb - sa;
-                float lo[R], fb[R];
-                float2 f[R];
-#pragma unroll
-                for (int r = 0; r < R; r++) {
-                    fb[r] = __builtin_amdgcn_fmed3f(floorf(fmaf(zs[r], sb, z0[r])), -1.f, nzf);
-                    lo[r] = __builtin_amdgcn_fmed3f(fa[r], fb[r], -kBig);
-                }
-                // the whole offset is the vector operand (checked against the
-                // buffer's size); (lo + 1) >= 0 is the padded z index
-#pragma unroll
-                for (int r = 0; r < R; r++)
-                    f[r] = __builtin_bit_cast(
-                        float2, __builtin_amdgcn_raw_buffer_load_b64(volz, 4 * ((int)lo[r] + 1) + off, 0, 0));
-#pragma unroll
-                for (int r = 0; r < R; r++) {
-                    const float tm = __builtin_amdgcn_fmed3f(fmaf(lo[r], rs[r], q1[r]), sa, sb);
-                    const float llo = fabsf(tm - (up[r] ? sa : sb));      // part below z = lo + 1
-                    acc[r] = fmaf(llo, f[r].x, fmaf(Lc - llo, f[r].y, acc[r]));
-                    fa[r] = fb[r];
-                }
-                sa = sb;
-                off += bxi ? mxo : myo;
-                rx -= bxi;
-                ry -= bxi ^ 1;
-                sxn = bxi ? sxn + dsx : sxn;
-                syn = bxi ? syn : syn + dsy;
-            };
-            while ((rx | ry) >= 0) {
-                cell();
-                if ((rx | ry) < 0) break;
-                cell();
-            }
-#pragma unroll
-            for (int r = 0; r < R; r++) {
-                const int iv = base + 64 * r + lane;
-                if (act[r]) {
-                    if (ACC) col[(size_t)iv * d.nu] += sec[r] * acc[r];
-                    else col[(size_t)iv * d.nu] = sec[r] * acc[r];
-                }
-            }
+            base += 64 * nr;
         }
     }
     // traced rows, and (without accumulation) zeros for the rows that miss;
@@ -775,22 +803,24 @@ __global__ __launch_bounds__(256) void siddon_cone_bp_kernel(float* __restrict__
 // with the forward projector's arithmetic, so the result is the reference's
 // in every case.
 // ---------------------------------------------------------------------------
-// Weight table of the column-mode backprojector, laid out so that a voxel's
-// candidate window is contiguous: entry (view, iv, iu, jv) = weight of ray
-// (iv + jv, iu), jv < dup (the window height).  Ray i = (view, iv, iu) writes
-// its weight into the entries (iv - jv, iu, jv) of its view.
-__device__ inline void store_window_weight(float* __restrict__ wtab, long i, int iv, int nu, int dup, float w) {
-    for (int jv = 0; jv < dup && jv <= iv; jv++) wtab[(i - (long)jv * nu) * dup + jv] = w;
+__global__ __launch_bounds__(256) void fill_zero_i32_kernel(int* __restrict__ p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0;
 }
 
-// Ray i = (view, iv, iu) in FP64 with the reference's rules: the weight (0 for
-// rays that miss, or that follow the axis-parallel rule on an axis they are
-// not parallel to) and the exact scatter of those exception rays.
-__device__ inline void coltable_exact(long i, int iu, int iv, int view, float* __restrict__ wtab,
-                                      float* __restrict__ img,
+// Weight table of the column-mode backprojector, laid out so that a voxel's
+// candidate window is contiguous: entry (view, iv, iu, jv) = weight of ray
+// (iv + jv, iu) for jv < dup (the window height; 0 past the last row).  One
+// thread computes the dup weights of an entry and stores them together.
+
+// Ray i = (view, iv, iu) in FP64 with the reference's rules: returns the weight
+// (0 for rays that miss, or that follow the axis-parallel rule on an axis they
+// are not parallel to); with 'scatter', adds those exception rays to img by
+// the exact traversal.
+__device__ inline float coltable_exact(long i, int iu, int iv, int view, bool scatter, float* __restrict__ img,
                                       const float* __restrict__ prj, const float* __restrict__ pdc,
                                       const float* __restrict__ pdu, const float* __restrict__ pdv,
-                                      const float* __restrict__ psrc, const Grid& g, const Det& d, int dup) {
+                                      const float* __restrict__ psrc, const Grid& g, const Det& d) {
     const double u = ((double)iu - d.off_u - (d.nu - 1) / 2.0) * d.du;
     const double v = ((double)iv - d.off_v - (d.nv - 1) / 2.0) * d.dv;
     double S[3], Q[3];
@@ -844,8 +874,7 @@ __device__ inline void coltable_exact(long i, int iu, int iv, int view, float* _
     }
     const bool exception = rpar[0] != cpar[0] || rpar[1] != cpar[1] || rpar[2] != cpar[2];
     const float p = prj[i];
-    store_window_weight(wtab, i, iv, d.nu, dup, (hit && !exception) ? (float)(p * L) : 0.f);
-    if (exception && p != 0.f) {
+    if (scatter && exception && p != 0.f) {
         SiddonRay r;
         Grid gg = g;
         if (siddon_setup(r, ld3(psrc, view), cell_centre(pdc, pdu, pdv, view, iu, iv, d), gg, false)) {
@@ -857,6 +886,7 @@ __device__ inline void coltable_exact(long i, int iu, int iv, int view, float* _
             }
         }
     }
+    return (hit && !exception) ? (float)(p * L) : 0.f;
 }
 
 // Column and row tables of the column-mode BP (1/w_x, 1/w_y per column from
@@ -864,6 +894,7 @@ __device__ inline void coltable_exact(long i, int iu, int iv, int view, float* _
 // 0 for an axis the ray is exactly parallel to) and the view's parallel-ray
 // flag.  One thread per column and per row of every view.
 __global__ __launch_bounds__(256) void siddon_bp_colaxes_kernel(float2* __restrict__ utab, float2* __restrict__ vtab,
+                                                                float2* __restrict__ colw, float* __restrict__ roww,
                                                                 ConeViewBP* __restrict__ views,
                                                                 const float* __restrict__ pdc,
                                                                 const float* __restrict__ pdu,
@@ -879,21 +910,25 @@ __global__ __launch_bounds__(256) void siddon_bp_colaxes_kernel(float2* __restri
     const double u = ((double)iu - d.off_u - (d.nu - 1) / 2.0) * d.du;
     const double v = ((double)iv - d.off_v - (d.nv - 1) / 2.0) * d.dv;
     const double dd[3] = {g.dx, g.dy, g.dz};
-    double wi[3];
+    double w[3], wi[3];
     bool cpar[3];
     for (int a = 0; a < 3; a++) {
         const double S = psrc[3 * view + a];
         const double Q = pdc[3 * view + a] + pdu[3 * view + a] * u + pdv[3 * view + a] * v;
+        w[a] = Q - S;
         wi[a] = (Q - S) / dd[a];
         cpar[a] = (Q - S) == 0.0;
     }
+    // (in column geometry Q - S has x, y from the column and z from the row)
     if (column) {
         utab[(long)view * d.nu + iu] =
             make_float2(cpar[0] ? 0.f : (float)(1.0 / wi[0]), cpar[1] ? 0.f : (float)(1.0 / wi[1]));
+        colw[(long)view * d.nu + iu] = make_float2((float)w[0], (float)w[1]);
         if (cpar[0] || cpar[1]) views[view].flags = 1;
     } else {
         const float q = cpar[2] ? 0.f : (float)(1.0 / wi[2]);
         vtab[(long)view * d.nv + iv] = make_float2(q, 0.5f * fabsf(q));
+        roww[(long)view * d.nv + iv] = (float)w[2];
         if (cpar[2]) views[view].flags = 1;
     }
 }
@@ -903,40 +938,94 @@ __global__ __launch_bounds__(256) void siddon_bp_colaxes_kernel(float2* __restri
 // rules whatever the rounding: its weight is prj * |Q - S|, with hits decided
 // by the geometry of the chords themselves (a ray that misses the grid has no
 // chord in any voxel).  Rays near an axis direction take the FP64 path.
-__global__ __launch_bounds__(256) void siddon_bp_coltable_kernel(float* __restrict__ wtab, float* __restrict__ img,
-                                                                 const float* __restrict__ prj,
-                                                                 const float* __restrict__ pdc, const float* __restrict__ pdu,
-                                                                 const float* __restrict__ pdv, const float* __restrict__ psrc,
-                                                                 Grid g, Det d, int nview, int dup) {
-    const long n = (long)d.nu * d.nv * nview;
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int iu = (int)(i % d.nu);
-    const int iv = (int)((i / d.nu) % d.nv);
-    const int view = (int)(i / ((long)d.nu * d.nv));
-    // Q - S in FP64 as the exact path forms it (zero exactly when it is zero there)
-    const double u = ((double)iu - d.off_u - (d.nu - 1) / 2.0) * d.du;
-    const double v = ((double)iv - d.off_v - (d.nv - 1) / 2.0) * d.dv;
-    double w[3];
-    for (int k = 0; k < 3; k++)
-        w[k] = ((double)pdc[3 * view + k] + (double)pdu[3 * view + k] * u + (double)pdv[3 * view + k] * v) -
-               (double)psrc[3 * view + k];
-    // fraction of the ray kept by the move towards the grid (FP32 is enough: it
-    // scales the components, which are compared with a 10x margin)
+// Fast-path weight of ray (view, iv, iu) (w = prj * |Q - S|), or -1 when the
+// ray is not regular and needs the FP64 path.
+__device__ inline float coltable_fast(long i, int iu, int iv, int view, const float* __restrict__ prj,
+                                      const float2* __restrict__ colw, const float* __restrict__ roww,
+                                      const float* __restrict__ psrc, const Grid& g, const Det& d) {
+    // Q - S (the axes kernel's FP64 values rounded to FP32: zero exactly when zero)
+    const float2 cw = colw[(long)view * d.nu + iu];
+    const float3 wf = make_float3(cw.x, cw.y, roww[(long)view * d.nv + iv]);
+    // fraction of the ray kept by the move towards the grid (it scales the
+    // components, which are compared with a 10x margin: approximate
+    // reciprocals are enough)
     const float3 Sf = ld3(psrc, view);
-    const float3 wf = make_float3((float)w[0], (float)w[1], (float)w[2]);
-    const float L = sqrtf(wf.x * wf.x + wf.y * wf.y + wf.z * wf.z);
+    const float L = __builtin_amdgcn_sqrtf(wf.x * wf.x + wf.y * wf.y + wf.z * wf.z);
+    const float rL = __builtin_amdgcn_rcpf(L);
     const float ex = (float)g.nx * g.dx, ey = (float)g.ny * g.dy, ez = (float)g.nz * g.dz;
     const float rr = sqrtf(ex * ex + ey * ey + ez * ez);
-    const float t0 = ((g.cx - Sf.x) * wf.x + (g.cy - Sf.y) * wf.y + (g.cz - Sf.z) * wf.z) / L;
-    const float frac = (L - fmaxf(t0 - rr, 0.f) - fmaxf(L - t0 - rr, 0.f)) / L;
+    const float t0 = ((g.cx - Sf.x) * wf.x + (g.cy - Sf.y) * wf.y + (g.cz - Sf.z) * wf.z) * rL;
+    const float frac = (L - fmaxf(t0 - rr, 0.f) - fmaxf(L - t0 - rr, 0.f)) * rL;
     const bool regular = fabsf(wf.x) * frac > 1e-5f * g.dx && fabsf(wf.y) * frac > 1e-5f * g.dy &&
                          fabsf(wf.z) * frac > 1e-5f * g.dz && frac > 0.f;
-    if (!regular) {
-        coltable_exact(i, iu, iv, view, wtab, img, prj, pdc, pdu, pdv, psrc, g, d, dup);
-        return;
+    return regular ? prj[i] * L : -1.f;
+}
+
+// One thread per column and 8 consecutive rows of a view (grid: column
+// blocks, row groups, views): the weights of rays iv0 .. iv0 + 8 + dup - 2
+// (dup <= 4) fill the entries of rows iv0 .. iv0 + 7.  Rays that are not
+// regular get weight 0 here and flag their row for siddon_bp_colpatch_kernel.
+constexpr int kTabRows = 8;
+
+template <int DUP>
+__global__ __launch_bounds__(256) void siddon_bp_coltable_kernel(float* __restrict__ wtab, int* __restrict__ rowirr,
+                                                                 const float* __restrict__ prj,
+                                                                 const float2* __restrict__ colw,
+                                                                 const float* __restrict__ roww,
+                                                                 const float* __restrict__ psrc, Grid g, Det d) {
+    const int iu = blockIdx.x * 256 + threadIdx.x, iv0 = blockIdx.y * kTabRows, view = blockIdx.z;
+    if (iu >= d.nu) return;
+    const long i0 = ((long)view * d.nv + iv0) * d.nu + iu;
+    const int nrow = min(kTabRows + DUP - 1, d.nv - iv0);     // rays of this thread
+    float wr[kTabRows + DUP - 1];
+#pragma unroll
+    for (int r = 0; r < kTabRows + DUP - 1; r++) {
+        float w = 0.f;
+        if (r < nrow) {
+            w = coltable_fast(i0 + (long)r * d.nu, iu, iv0 + r, view, prj, colw, roww, psrc, g, d);
+            if (w < 0.f && r < kTabRows) rowirr[(long)view * d.nv + iv0 + r] = 1;
+        }
+        wr[r] = fmaxf(w, 0.f);
     }
-    store_window_weight(wtab, i, iv, d.nu, dup, prj[i] * L);
+#pragma unroll
+    for (int r = 0; r < kTabRows; r++) {
+        if (r < nrow) {
+            float* out = wtab + (i0 + (long)r * d.nu) * DUP;
+            if constexpr (DUP == 2) {
+                *reinterpret_cast<float2*>(out) = make_float2(wr[r], wr[r + 1]);
+            } else {
+#pragma unroll
+                for (int jv = 0; jv < DUP; jv++) out[jv] = wr[r + jv];
+            }
+        }
+    }
+}
+
+// The rays left at weight 0 by siddon_bp_coltable_kernel: FP64 weight into
+// their dup entries, and the exact scatter of exception rays.  Grid: (row
+// groups of 8, views); blocks of rows without a flag return at once.
+__global__ __launch_bounds__(256) void siddon_bp_colpatch_kernel(float* __restrict__ wtab, float* __restrict__ img,
+                                                                 const int* __restrict__ rowirr,
+                                                                 const float* __restrict__ prj,
+                                                                 const float2* __restrict__ colw,
+                                                                 const float* __restrict__ roww,
+                                                                 const float* __restrict__ pdc,
+                                                                 const float* __restrict__ pdu,
+                                                                 const float* __restrict__ pdv,
+                                                                 const float* __restrict__ psrc, Grid g, Det d,
+                                                                 int dup) {
+    const int view = blockIdx.y;
+    for (int r = 0; r < kTabRows; r++) {
+        const int iv = blockIdx.x * kTabRows + r;
+        if (iv >= d.nv) return;
+        if (rowirr[(long)view * d.nv + iv] == 0) continue;
+        for (int iu = threadIdx.x; iu < d.nu; iu += 256) {
+            const long i = ((long)view * d.nv + iv) * d.nu + iu;
+            if (coltable_fast(i, iu, iv, view, prj, colw, roww, psrc, g, d) >= 0.f) continue;
+            const float w = coltable_exact(i, iu, iv, view, true, img, prj, pdc, pdu, pdv, psrc, g, d);
+            for (int jv = 0; jv < dup && jv <= iv; jv++) wtab[(i - (long)jv * d.nu) * dup + jv] = w;
+        }
+    }
 }
 
 // N consecutive floats (N <= 4) from a buffer with one load instruction
@@ -1165,6 +1254,9 @@ void siddon_cone_bp(float* img, const float* prj, const ConeGeomDev& geo, const 
         pl.add<float>(nprj * wv + 16);                // window layout, padded: window loads
         pl.add<float2>((size_t)nview * d.nu);
         pl.add<float2>((size_t)nview * d.nv + 4);
+        pl.add<float2>((size_t)nview * d.nu);
+        pl.add<float>((size_t)nview * d.nv);
+        pl.add<int>((size_t)nview * d.nv);
     } else {
         pl.add<float4>(nprj);
     }
@@ -1173,11 +1265,17 @@ void siddon_cone_bp(float* img, const float* prj, const ConeGeomDev& geo, const 
     float* wtab = nullptr;
     float2* utab = nullptr;
     float2* vtab = nullptr;
+    float2* colw = nullptr;
+    float* roww = nullptr;
+    int* rowirr = nullptr;
     float4* table = nullptr;
     if (col) {
         wtab = ws.take_n<float>(nprj * wv + 16);
         utab = ws.take_n<float2>((size_t)nview * d.nu);
         vtab = ws.take_n<float2>((size_t)nview * d.nv + 4);
+        colw = ws.take_n<float2>((size_t)nview * d.nu);
+        roww = ws.take_n<float>((size_t)nview * d.nv);
+        rowirr = ws.take_n<int>((size_t)nview * d.nv);
     } else {
         table = ws.take_n<float4>(nprj);
     }
@@ -1192,11 +1290,26 @@ void siddon_cone_bp(float* img, const float* prj, const ConeGeomDev& geo, const 
             {
                 TimedLaunch tt("siddon_bp_table", stream);
                 siddon_bp_colaxes_kernel<<<ceil_div((long)nview * (d.nu + d.nv), 256), 256, 0, stream>>>(
-                    utab, vtab, dviews, geo.det_center, geo.det_u, geo.det_v, geo.src, g, d, nview);
+                    utab, vtab, colw, roww, dviews, geo.det_center, geo.det_u, geo.det_v, geo.src, g, d, nview);
                 CTP_LAUNCH_CHECK();
-                siddon_bp_coltable_kernel<<<ceil_div((long)nprj, 256), 256, 0, stream>>>(
-                    wtab, im, prj + b * nprj, geo.det_center, geo.det_u, geo.det_v, geo.src, g, d,
-                    nview, wv);
+                if (d.nv > 65535 || nview > 65535) fail("cone backprojection: more than 65535 rows or views");
+                fill_zero_i32_kernel<<<ceil_div((long)nview * d.nv, 256), 256, 0, stream>>>(rowirr,
+                                                                                             (size_t)nview * d.nv);
+                CTP_LAUNCH_CHECK();
+                {
+                    const dim3 tg(ceil_div(d.nu, 256), ceil_div(d.nv, kTabRows), nview);
+                    const float* pb = prj + b * nprj;
+                    switch (wv) {
+                        case 1: siddon_bp_coltable_kernel<1><<<tg, 256, 0, stream>>>(wtab, rowirr, pb, colw, roww, geo.src, g, d); break;
+                        case 2: siddon_bp_coltable_kernel<2><<<tg, 256, 0, stream>>>(wtab, rowirr, pb, colw, roww, geo.src, g, d); break;
+                        case 3: siddon_bp_coltable_kernel<3><<<tg, 256, 0, stream>>>(wtab, rowirr, pb, colw, roww, geo.src, g, d); break;
+                        default: siddon_bp_coltable_kernel<4><<<tg, 256, 0, stream>>>(wtab, rowirr, pb, colw, roww, geo.src, g, d); break;
+                    }
+                }
+                CTP_LAUNCH_CHECK();
+                siddon_bp_colpatch_kernel<<<dim3(ceil_div(d.nv, kTabRows), nview), 256, 0, stream>>>(
+                    wtab, im, rowirr, prj + b * nprj, colw, roww, geo.det_center, geo.det_u, geo.det_v, geo.src, g, d,
+                    wv);
                 CTP_LAUNCH_CHECK();
             }
 #define CTP_BPC_CASE(A, Bv) \
@@ -1359,13 +1472,18 @@ __global__ __launch_bounds__(256) void fill_zero_kernel(float* __restrict__ p, s
 }
 
 // weights prj * lhit in the window layout of the column backprojector
+// grid: (column blocks, rows, views)
 __global__ __launch_bounds__(256) void window_weights_kernel(float* __restrict__ w, const float* __restrict__ prj,
-                                                             const float* __restrict__ lhit, size_t n, int nu, int nv,
+                                                             const float* __restrict__ lhit, int nu, int nv,
                                                              int dup) {
-    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const int iv = (int)((i / nu) % nv);
-    store_window_weight(w, (long)i, iv, nu, dup, prj[i] * lhit[i]);
+    const int iu = blockIdx.x * 256 + threadIdx.x, iv = blockIdx.y;
+    if (iu >= nu) return;
+    const size_t i = ((size_t)blockIdx.z * nv + iv) * nu + iu;
+    float* out = w + i * dup;
+    for (int jv = 0; jv < dup; jv++) {
+        const size_t r = i + (size_t)jv * nu;
+        out[jv] = iv + jv < nv ? prj[r] * lhit[r] : 0.f;
+    }
 }
 
 // ray-driven scatter of the listed rays (the forward projector's traversal),
@@ -1592,11 +1710,11 @@ void ConePlan::Impl::forward(hipStream_t s) {
 void ConePlan::Impl::backward(hipStream_t s) {
     for (size_t j = 0; j < jobs.size(); j++) {
         const ConeJobDev& J = jobs[j];
-        const size_t nimg = (size_t)J.g.nx * J.g.ny * J.g.nz, nprj = (size_t)J.d.nu * J.d.nv * J.nview;
+        const size_t nimg = (size_t)J.g.nx * J.g.ny * J.g.nz;
         fill_zero_kernel<<<(unsigned)ceil_div((long)nimg, 256), 256, 0, s>>>(J.img, nimg);
         CTP_LAUNCH_CHECK();
-        window_weights_kernel<<<(unsigned)ceil_div((long)nprj, 256), 256, 0, s>>>(J.wtab, J.prj, J.lhit, nprj, J.d.nu,
-                                                                                   J.d.nv, J.wv);
+        window_weights_kernel<<<dim3(ceil_div(J.d.nu, 256), J.d.nv, J.nview), 256, 0, s>>>(J.wtab, J.prj, J.lhit,
+                                                                                            J.d.nu, J.d.nv, J.wv);
         CTP_LAUNCH_CHECK();
         if (nexc[j] > 0) {
             siddon_scatter_rays_kernel<<<ceil_div(nexc[j], 256), 256, 0, s>>>(J.img, J.prj, exc[j], nexc[j], J.dc, J.du,
@@ -1640,6 +1758,7 @@ ConePlan::ConePlan(const std::vector<ConeJobSpec>& specs, bool autotune) : impl_
         if (!column_mode(hg, bp, nview))
             fail("cone plan: every view must have det_v along z and det_u horizontal (column geometry)");
         if (d.nu >= (1 << 24)) fail("cone plan: detectors of 2^24 columns or more are not supported");
+        if (d.nv > 65535 || nview > 65535) fail("cone plan: more than 65535 detector rows or views");
         if (bp.wu > 4 || bp.wv > 4)
             fail("cone plan: voxel shadow spans more than 4 detector cells; use finer voxels or coarser cells");
         ConeJobDev J{};

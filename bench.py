@@ -147,7 +147,21 @@ def cone_extra(steps):
         'traffic_bytes': {'siddon_cone_fp': read_traffic('siddon_cone_fp'),
                           'siddon_cone_bp': read_traffic('siddon_cone_bp')},
     }
-    del prj, bp
+    # the same geometry for a batch of two volumes per call (column FP and BP
+    # share the traversal and the chords across batch items)
+    img2 = torch.cat([img_d, 0.5 * img_d], 0).contiguous()
+    prj = cone.siddon_fp_arbitrary(p, img2, *g)
+    _ = cone.siddon_bp_arbitrary(p, prj, *g)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        prj = cone.siddon_fp_arbitrary(p, img2, *g)
+        bp = cone.siddon_bp_arbitrary(p, prj, *g)
+    torch.cuda.synchronize()
+    ms2 = (time.perf_counter() - t0) / steps * 1e3
+    out['batch2_ms_per_step'] = round(ms2, 3)
+    out['batch2_value_gvu_s'] = round(2 * 2 * nvu / (ms2 / 1e3) / 1e9, 2)
+    del prj, bp, img2
     return out
 
 

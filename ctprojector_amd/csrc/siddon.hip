@@ -134,8 +134,10 @@ __global__ __launch_bounds__(256) void siddon_cone_fp_kernel(float* __restrict__
 // differ only in their slope along z.  One wavefront takes one column.  The 2D
 // traversal of the shared xy line (its cells and the interval of the in-plane
 // distance s spent in each) is the same in every lane and is stepped in
-// lockstep; each lane follows up to kColRays rays (detector rows, as many as
-// the column's rows that reach the grid need) through it.  Inside
+// lockstep; each lane follows up to 5 rays (detector rows, as many as the
+// column's rows that reach the grid need) through it, and batch items sharing
+// the geometry are projected together (the traversal and the split of each
+// cell done once for up to 4 volumes).  Inside
 // an xy cell a ray's z index changes by less than 1 (steeper rays are traced
 // ray by ray), so it meets z cells lo and lo + 1 at most: the s interval is
 // split at the plane z = lo + 1, and both voxels come from one 8-byte load of
@@ -150,11 +152,12 @@ __global__ __launch_bounds__(256) void siddon_cone_fp_kernel(float* __restrict__
 // units, except an exactly horizontal ray), and rays steeper than the bound.
 // Columns whose xy line has an exactly zero x or y component, and views whose
 // det_v is not along z, are traced ray by ray as a whole.
-constexpr int kColRays = 5;
 
-// [z][y][x] -> [y][x][z + 1] with zero padding (P floats per column)
+// [z][y][x] -> [y][x][z + 1] with zero padding (P floats per column); with
+// nb > 1 volumes interleaved, element (y, x, zp) of volume b at
+// ((y * nx + x) * P + zp) * nb + b
 __global__ __launch_bounds__(256) void transpose_zcol_kernel(float* __restrict__ out, const float* __restrict__ in,
-                                                             int nx, int ny, int nz, int P) {
+                                                             int nx, int ny, int nz, int P, int nb = 1, int b = 0) {
     __shared__ float tile[32][33];
     const int x0 = blockIdx.x * 32, z0 = blockIdx.y * 32, y = blockIdx.z;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
@@ -166,9 +169,9 @@ __global__ __launch_bounds__(256) void transpose_zcol_kernel(float* __restrict__
     for (int k = ty; k < 32; k += 8) {
         const int x = x0 + k, z = z0 + tx;
         if (x >= nx) continue;
-        float* col = out + ((size_t)y * nx + x) * P;
-        if (z < nz) col[z + 1] = tile[tx][k];
-        if (blockIdx.y == 0 && tx < 3) col[tx == 0 ? 0 : nz + tx] = 0.f;
+        float* col = out + ((size_t)y * nx + x) * P * nb + b;
+        if (z < nz) col[(z + 1) * nb] = tile[tx][k];
+        if (blockIdx.y == 0 && tx < 3) col[(tx == 0 ? 0 : nz + tx) * nb] = 0.f;
     }
 }
 
@@ -215,15 +218,20 @@ struct ColLine {
     int rx0, ry0, mxo, myo, off0;
 };
 
-template <int R, bool ACC>
+// NB volumes of one geometry at once: the traversal and the per-ray split of
+// each cell are shared, only the accumulations are per volume (volume b at
+// col + b * pstride); the volumes are interleaved in volz (z index zp of
+// volume b at float zp * NB + b of a column), so the pairs of all NB volumes
+// come in NB / 2 16-byte loads (one 8-byte load for NB = 1).
+template <int R, bool ACC, int NB>
 __device__ inline void fp_column_chunk(float* __restrict__ col, __amdgpu_buffer_rsrc_t volz, const ColLine& L,
                                        float3 S, const float* __restrict__ pdc, const float* __restrict__ pdu,
                                        const float* __restrict__ pdv, const Grid& g, const Det& d, int view, int iu,
-                                       int base, int hi_iv) {
+                                       int base, int hi_iv, long pstride) {
     const int lane = threadIdx.x & 63;
     const float s_in = L.s_in, sxn0 = L.sxn0, syn0 = L.syn0, dsx = L.dsx, dsy = L.dsy, nzf = L.nzf;
     const int rx0 = L.rx0, ry0 = L.ry0, mxo = L.mxo, myo = L.myo, off0 = L.off0;
-    float zs[R], z0[R], rs[R], q1[R], fa[R], acc[R], sec[R];
+    float zs[R], z0[R], rs[R], q1[R], fa[R], acc[R][NB], sec[R];
     bool up[R], act[R];
 #pragma unroll
     for (int r = 0; r < R; r++) {
@@ -240,7 +248,8 @@ __device__ inline void fp_column_chunk(float* __restrict__ col, __amdgpu_buffer_
         q1[r] = zs[r] != 0.f ? (1.f - z0[r]) * rs[r] : 3.0e38f;   // s at z = lo + 1: lo * rs + q1
         up[r] = zs[r] >= 0.f;
         fa[r] = __builtin_amdgcn_fmed3f(floorf(fmaf(zs[r], s_in, z0[r])), -1.f, nzf);
-        acc[r] = 0.f;
+#pragma unroll
+        for (int b = 0; b < NB; b++) acc[r][b] = 0.f;
     }
     // min / max as medians with a sentinel (no canonicalisation of
     // loop-carried operands)
@@ -255,7 +264,7 @@ __device__ inline void fp_column_chunk(float* __restrict__ col, __amdgpu_buffer_
         const float sb = __builtin_amdgcn_fmed3f(bxi ? sxn : syn, sa, kBig);
         const float Lc = sb - sa;
         float lo[R], fb[R];
-        float2 f[R];
+        float2 f[R][NB];
 #pragma unroll
         for (int r = 0; r < R; r++) {
             fb[r] = __builtin_amdgcn_fmed3f(floorf(fmaf(zs[r], sb, z0[r])), -1.f, nzf);
@@ -264,14 +273,31 @@ __device__ inline void fp_column_chunk(float* __restrict__ col, __amdgpu_buffer_
         // the whole offset is the vector operand (checked against the
         // buffer's size); (lo + 1) >= 0 is the padded z index
 #pragma unroll
-        for (int r = 0; r < R; r++)
-            f[r] = __builtin_bit_cast(
-                float2, __builtin_amdgcn_raw_buffer_load_b64(volz, 4 * ((int)lo[r] + 1) + off, 0, 0));
+        for (int r = 0; r < R; r++) {
+            const int o = 4 * NB * ((int)lo[r] + 1) + off;
+            if constexpr (NB == 1) {
+                f[r][0] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(volz, o, 0, 0));
+            } else if constexpr (NB == 2) {
+                // (v0[lo], v1[lo], v0[lo + 1], v1[lo + 1])
+                const float4 q = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(volz, o, 0, 0));
+                f[r][0] = make_float2(q.x, q.z);
+                f[r][1] = make_float2(q.y, q.w);
+            } else {
+                static_assert(NB == 4, "1, 2 or 4 volumes");
+                const float4 a = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(volz, o, 0, 0));
+                const float4 c = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(volz, o + 16, 0, 0));
+                f[r][0] = make_float2(a.x, c.x);
+                f[r][1] = make_float2(a.y, c.y);
+                f[r][2] = make_float2(a.z, c.z);
+                f[r][3] = make_float2(a.w, c.w);
+            }
+        }
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const float tm = __builtin_amdgcn_fmed3f(fmaf(lo[r], rs[r], q1[r]), sa, sb);
             const float llo = fabsf(tm - (up[r] ? sa : sb));      // part below z = lo + 1
-            acc[r] = fmaf(llo, f[r].x, fmaf(Lc - llo, f[r].y, acc[r]));
+#pragma unroll
+            for (int b = 0; b < NB; b++) acc[r][b] = fmaf(llo, f[r][b].x, fmaf(Lc - llo, f[r][b].y, acc[r][b]));
             fa[r] = fb[r];
         }
         sa = sb;
@@ -290,17 +316,26 @@ __device__ inline void fp_column_chunk(float* __restrict__ col, __amdgpu_buffer_
     for (int r = 0; r < R; r++) {
         const int iv = base + 64 * r + lane;
         if (act[r]) {
-            if (ACC) col[(size_t)iv * d.nu] += sec[r] * acc[r];
-            else col[(size_t)iv * d.nu] = sec[r] * acc[r];
+#pragma unroll
+            for (int b = 0; b < NB; b++) {
+                float* o = col + b * pstride + (size_t)iv * d.nu;
+                if (ACC) *o += sec[r] * acc[r][b];
+                else *o = sec[r] * acc[r][b];
+            }
         }
     }
 }
 
-template <bool ACC>
-__device__ inline void fp_column(float* __restrict__ prj, __amdgpu_buffer_rsrc_t volz, __amdgpu_buffer_rsrc_t vol,
+// rays per lane and traversal for NB volumes (accumulators and loads in
+// flight grow with both)
+template <int NB>
+constexpr int col_rays_max() { return NB == 1 ? 5 : (NB == 2 ? 5 : 3); }
+
+template <bool ACC, int NB>
+__device__ inline void fp_column(float* __restrict__ prj, __amdgpu_buffer_rsrc_t volz, const float* __restrict__ img,
                                  const float* __restrict__ pdc, const float* __restrict__ pdu,
                                  const float* __restrict__ pdv, const float* __restrict__ psrc, const Grid& g,
-                                 const Det& d, int view, int iu, int P) {
+                                 const Det& d, int view, int iu, int P, long pstride, long istride) {
     const int lane = threadIdx.x & 63;
     float* col = prj + (size_t)view * d.nv * d.nu + iu;        // row iv at col[iv * nu]
     const float3 S = ld3(psrc, view);
@@ -311,9 +346,17 @@ __device__ inline void fp_column(float* __restrict__ prj, __amdgpu_buffer_rsrc_t
     const float ex = ex0 / Lxy, ey = ey0 / Lxy;
     const float exi = ex / g.dx, eyi = ey / g.dy;
     const bool shared = V.x == 0.f && V.y == 0.f && Lxy > 0.f && exi != 0.f && eyi != 0.f;
+    // the reference traversal for rays outside the column form, each volume
+    auto trace_row = [&](int iv, float3 Pc) {
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            const __amdgpu_buffer_rsrc_t vol = __builtin_amdgcn_make_buffer_rsrc(
+                (void*)(img + b * istride), (short)0, 4 * g.nx * g.ny * g.nz, 0x00020000);
+            trace_into<ACC>(col + b * pstride + (size_t)iv * d.nu, vol, S, Pc, g);
+        }
+    };
     if (!shared) {
-        for (int iv = lane; iv < d.nv; iv += 64)
-            trace_into<ACC>(col + (size_t)iv * d.nu, vol, S, cell_centre(pdc, pdu, pdv, view, iu, iv, d), g);
+        for (int iv = lane; iv < d.nv; iv += 64) trace_row(iv, cell_centre(pdc, pdu, pdv, view, iu, iv, d));
         return;
     }
     // s = 0 at the foot of the perpendicular from the grid centre (keeps |s|
@@ -363,19 +406,26 @@ __device__ inline void fp_column(float* __restrict__ prj, __amdgpu_buffer_rsrc_t
         const float syn0 = ((float)(py ? iy + 1 : iy) - oy) * ryi;
         const float dsx = fabsf(rxi), dsy = fabsf(ryi);
         const int rx0 = px ? g.nx - 1 - ix : ix, ry0 = py ? g.ny - 1 - iy : iy;
-        const int mxo = (px ? 4 : -4) * P, myo = (py ? 4 : -4) * P * g.nx;
-        const int off0 = 4 * P * (iy * g.nx + ix);        // byte offset of column (ix, iy)
+        // byte offsets of the interleaved columns (NB volumes per z index)
+        const int mxo = (px ? 4 : -4) * P * NB, myo = (py ? 4 : -4) * P * NB * g.nx;
+        const int off0 = 4 * P * NB * (iy * g.nx + ix);   // column (ix, iy)
         const ColLine line{s_in, sxn0, syn0, dsx, dsy, Lxy, sO, diag, nzf, rx0, ry0, mxo, myo, off0};
-        // up to kColRays rays per lane and traversal, as many as the rows need
+        // up to col_rays_max rays per lane and traversal, as many as the rows need
+        constexpr int RM = col_rays_max<NB>();
         for (int base = lo_iv; base <= hi_iv;) {
-            const int nr = min(kColRays, (hi_iv - base + 64) / 64);
-            switch (nr) {
-                case 1: fp_column_chunk<1, ACC>(col, volz, line, S, pdc, pdu, pdv, g, d, view, iu, base, hi_iv); break;
-                case 2: fp_column_chunk<2, ACC>(col, volz, line, S, pdc, pdu, pdv, g, d, view, iu, base, hi_iv); break;
-                case 3: fp_column_chunk<3, ACC>(col, volz, line, S, pdc, pdu, pdv, g, d, view, iu, base, hi_iv); break;
-                case 4: fp_column_chunk<4, ACC>(col, volz, line, S, pdc, pdu, pdv, g, d, view, iu, base, hi_iv); break;
-                default: fp_column_chunk<5, ACC>(col, volz, line, S, pdc, pdu, pdv, g, d, view, iu, base, hi_iv); break;
+            const int nr = min(RM, (hi_iv - base + 64) / 64);
+#define CTP_CHUNK(Rv) fp_column_chunk<Rv, ACC, NB>(col, volz, line, S, pdc, pdu, pdv, g, d, view, iu, base, hi_iv, \
+                                                   pstride)
+            if (nr == 1) CTP_CHUNK(1);
+            else if (nr == 2) CTP_CHUNK(2);
+            else if constexpr (RM >= 3) {
+                if (nr == 3) CTP_CHUNK(3);
+                else if constexpr (RM >= 4) {
+                    if (nr == 4) CTP_CHUNK(4);
+                    else if constexpr (RM >= 5) CTP_CHUNK(5);
+                }
             }
+#undef CTP_CHUNK
             base += 64 * nr;
         }
     }
@@ -384,29 +434,34 @@ __device__ inline void fp_column(float* __restrict__ prj, __amdgpu_buffer_rsrc_t
     if (any_exc || !ACC) {
         for (int iv = lane; iv < d.nv; iv += 64) {
             const float3 Pc = cell_centre(pdc, pdu, pdv, view, iu, iv, d);
-            if (colfp_ray(S, Pc, g, Lxy, sO, diag).exc) trace_into<ACC>(col + (size_t)iv * d.nu, vol, S, Pc, g);
-            else if (!ACC && (iv < lo_iv || iv > hi_iv)) col[(size_t)iv * d.nu] = 0.f;
+            if (colfp_ray(S, Pc, g, Lxy, sO, diag).exc) {
+                trace_row(iv, Pc);
+            } else if (!ACC && (iv < lo_iv || iv > hi_iv)) {
+#pragma unroll
+                for (int b = 0; b < NB; b++) col[b * pstride + (size_t)iv * d.nu] = 0.f;
+            }
         }
     }
 }
 
-// block: 4 adjacent detector columns of one view, one per wavefront
+// block: 4 adjacent detector columns of one view, one per wavefront; NB
+// volumes (consecutive batch items) per launch
+template <int NB>
 __global__ __launch_bounds__(256) void siddon_cone_fp_col_kernel(float* __restrict__ prj, const float* __restrict__ img,
                                                                  const float* __restrict__ imgz,
                                                                  const float* __restrict__ pdc,
                                                                  const float* __restrict__ pdu,
                                                                  const float* __restrict__ pdv,
                                                                  const float* __restrict__ psrc, Grid g, Det d,
-                                                                 int nview, int ngroup, int P, int xcd_order) {
+                                                                 int nview, int ngroup, int P, int xcd_order,
+                                                                 long pstride, long istride) {
     int view, grp;
     fp_tile_of(blockIdx.x, ngroup, xcd_order, view, grp);
     const int iu = grp * 4 + (threadIdx.x >> 6);
     if (grp >= ngroup || view >= nview || iu >= d.nu) return;
     const __amdgpu_buffer_rsrc_t volz =
-        __builtin_amdgcn_make_buffer_rsrc((void*)imgz, (short)0, 4 * P * g.nx * g.ny, 0x00020000);
-    const __amdgpu_buffer_rsrc_t vol =
-        __builtin_amdgcn_make_buffer_rsrc((void*)img, (short)0, 4 * g.nx * g.ny * g.nz, 0x00020000);
-    fp_column<true>(prj, volz, vol, pdc, pdu, pdv, psrc, g, d, view, iu, P);
+        __builtin_amdgcn_make_buffer_rsrc((void*)imgz, (short)0, 4 * NB * P * g.nx * g.ny, 0x00020000);
+    fp_column<true, NB>(prj, volz, img, pdc, pdu, pdv, psrc, g, d, view, iu, P, pstride, istride);
 }
 
 // Block order of the forward projection: tuning cone_fp_order 1 plain, 2 XCD
@@ -430,26 +485,39 @@ void siddon_cone_fp(float* prj, const float* img, const ConeGeomDev& geo, int B,
         const int P = g.nz + 3;
         const size_t ncol = (size_t)g.nx * g.ny * P;
         if (ncol >= (1ull << 29)) fail("cone forward projection: volumes of 2^29 voxels or more are not supported");
+        // batch items sharing the geometry go through in groups of up to 4
+        // (byte offsets within a group's copies stay below 2^31)
+        int nbmax = 1;
+        while (nbmax < 4 && nbmax < B && (size_t)(2 * nbmax) * ncol < (1ull << 29)) nbmax *= 2;
         Workspace& ws = workspace_for(stream);
         Plan pl;
-        pl.add<float>(ncol);
+        pl.add<float>(ncol * nbmax);
         ws.reserve(pl.bytes);
-        float* imgz = ws.take_n<float>(ncol);
+        float* imgz = ws.take_n<float>(ncol * nbmax);
         const int ngroup = ceil_div(d.nu, 4);
         const long nblocks = (long)nview * 8 * ((ngroup + 7) / 8);
         if (nblocks > 0x7fffffffL) fail("cone forward projection: too many detector columns");
-        for (int b = 0; b < B; b++) {
+        for (int b = 0; b < B;) {
+            const int nb = B - b >= 4 && nbmax >= 4 ? 4 : (B - b >= 2 && nbmax >= 2 ? 2 : 1);
             {
                 TimedLaunch t("siddon_transpose", stream);
-                transpose_zcol_kernel<<<dim3(ceil_div(g.nx, 32), ceil_div(g.nz, 32), g.ny), 256, 0, stream>>>(
-                    imgz, img + b * nimg, g.nx, g.ny, g.nz, P);
-                CTP_LAUNCH_CHECK();
+                for (int k = 0; k < nb; k++) {
+                    transpose_zcol_kernel<<<dim3(ceil_div(g.nx, 32), ceil_div(g.nz, 32), g.ny), 256, 0, stream>>>(
+                        imgz, img + (b + k) * nimg, g.nx, g.ny, g.nz, P, nb, k);
+                    CTP_LAUNCH_CHECK();
+                }
             }
             TimedLaunch t("siddon_cone_fp", stream);
-            siddon_cone_fp_col_kernel<<<(unsigned)nblocks, 256, 0, stream>>>(
-                prj + b * nprj, img + b * nimg, imgz, geo.det_center, geo.det_u, geo.det_v, geo.src, g, d, nview,
-                ngroup, P, xcd_order);
+#define CTP_FPC(N)                                                                                                \
+    siddon_cone_fp_col_kernel<N><<<(unsigned)nblocks, 256, 0, stream>>>(                                          \
+        prj + b * nprj, img + b * nimg, imgz, geo.det_center, geo.det_u, geo.det_v, geo.src, g, d, nview, ngroup, P, \
+        xcd_order, (long)nprj, (long)nimg)
+            if (nb == 4) CTP_FPC(4);
+            else if (nb == 2) CTP_FPC(2);
+            else CTP_FPC(1);
+#undef CTP_FPC
             CTP_LAUNCH_CHECK();
+            b += nb;
         }
         return;
     }
@@ -1048,12 +1116,16 @@ __device__ inline void load_floats(float* out, __amdgpu_buffer_rsrc_t r, int vof
 
 // WU x WV is the largest candidate window compiled in; wu x wv (<= WU x WV,
 // uniform across the block) is the window used.
-template <int WU, int WV, int NZR, bool SLOW>
-__device__ inline void bp_view_col(float* acc, const ConeViewBP& cv, const float* __restrict__ wt,
+// NB volumes of one geometry: wt[b] is volume b's weight table of the view;
+// the chords are computed once and accumulated into acc[k * NB + b].
+template <int WU, int WV, int NZR, bool SLOW, int NB>
+__device__ inline void bp_view_col(float* acc, const ConeViewBP& cv, const float* const* wt,
                                    const float2* __restrict__ ut, const float2* __restrict__ vt, int nu, int nv, int ix,
                                    int iy, int iz0, float xc, float yc, int wu, int wv) {
-    const __amdgpu_buffer_rsrc_t rw =
-        __builtin_amdgcn_make_buffer_rsrc((void*)wt, (short)0, 4 * (nu * nv * WV + 16), 0x00020000);
+    __amdgpu_buffer_rsrc_t rw[NB];
+#pragma unroll
+    for (int b = 0; b < NB; b++)
+        rw[b] = __builtin_amdgcn_make_buffer_rsrc((void*)wt[b], (short)0, 4 * (nu * nv * WV + 16), 0x00020000);
     const __amdgpu_buffer_rsrc_t ru = __builtin_amdgcn_make_buffer_rsrc((void*)ut, (short)0, 8 * nu, 0x00020000);
     const __amdgpu_buffer_rsrc_t rv = __builtin_amdgcn_make_buffer_rsrc((void*)vt, (short)0, 8 * (nv + 4), 0x00020000);
     const float ex = xc - cv.sx, ey = yc - cv.sy;
@@ -1099,7 +1171,7 @@ __device__ inline void bp_view_col(float* acc, const ConeViewBP& cv, const float
         // at most one row past nv - 1 (both tables are padded).
         const int wbase = 4 * WV * (__umul24(v0, nu) + u0), vbase = 8 * v0;   // (nu < 2^24: checked on the host)
         float qv[2 * WV];
-        float span[WU * WV];
+        float span[NB][WU * WV];
 #pragma unroll
         for (int jv = 0; jv < WV; jv += 2) {
             const int r = min(jv, wv - 1);
@@ -1107,22 +1179,21 @@ __device__ inline void bp_view_col(float* acc, const ConeViewBP& cv, const float
             else load_floats<2>(qv + 2 * jv, rv, vbase, 8 * r);
         }
 #pragma unroll
-        for (int c = 0; c < WU * WV; c += 4) {
-            constexpr int N = WU * WV;
-            if (N - c >= 4) load_floats<4>(span + c, rw, wbase, 4 * c);
-            else if (N - c == 3) load_floats<3>(span + c, rw, wbase, 4 * c);
-            else if (N - c == 2) load_floats<2>(span + c, rw, wbase, 4 * c);
-            else load_floats<1>(span + c, rw, wbase, 4 * c);
-        }
-        float w[WV][WU];
+        for (int b = 0; b < NB; b++)
 #pragma unroll
-        for (int jv = 0; jv < WV; jv++)
-#pragma unroll
-            for (int j = 0; j < WU; j++) w[jv][j] = span[j * WV + jv];
+            for (int c = 0; c < WU * WV; c += 4) {
+                constexpr int N = WU * WV;
+                if (N - c >= 4) load_floats<4>(span[b] + c, rw[b], wbase, 4 * c);
+                else if (N - c == 3) load_floats<3>(span[b] + c, rw[b], wbase, 4 * c);
+                else if (N - c == 2) load_floats<2>(span[b] + c, rw[b], wbase, 4 * c);
+                else load_floats<1>(span[b] + c, rw[b], wbase, 4 * c);
+            }
         float2 qz[WV];
 #pragma unroll
         for (int jv = 0; jv < WV; jv++) qz[jv] = make_float2(qv[2 * jv], qv[2 * jv + 1]);
-        float a = 0.f;
+        float a[NB];
+#pragma unroll
+        for (int b = 0; b < NB; b++) a[b] = 0.f;
 #pragma unroll
         for (int jv = 0; jv < WV; jv++) {
             if (jv >= wv) continue;
@@ -1131,19 +1202,27 @@ __device__ inline void bp_view_col(float* acc, const ConeViewBP& cv, const float
             if (SLOW && qz[jv].x == 0.f) { z0 = inz ? -big : big; z1 = big; }
             // overlap of [z0, z1] with [lo, hi] (lo <= hi): both ends clamped into it
 #pragma unroll
-            for (int j = 0; j < WU; j++)
-                if (j < wu)
-                    a = fmaf(__builtin_amdgcn_fmed3f(z1, lo[j], hi[j]) - __builtin_amdgcn_fmed3f(z0, lo[j], hi[j]),
-                             w[jv][j], a);
+            for (int j = 0; j < WU; j++) {
+                if (j >= wu) continue;
+                const float ch = __builtin_amdgcn_fmed3f(z1, lo[j], hi[j]) - __builtin_amdgcn_fmed3f(z0, lo[j], hi[j]);
+#pragma unroll
+                for (int b = 0; b < NB; b++) a[b] = fmaf(ch, span[b][j * WV + jv], a[b]);
+            }
         }
-        acc[k] += a;
+#pragma unroll
+        for (int b = 0; b < NB; b++) acc[k * NB + b] += a[b];
     }
 }
 
+// weight tables (per-call window layout) of up to 4 volumes of one geometry
+struct WTabs {
+    const float* p[4];
+};
+
 // One block of the column-mode backprojector: 16x16 voxel columns x NZR slices,
-// img += A^T y over all views.
-template <int WU, int WV, int NZR>
-__device__ inline void bp_col_block(float* __restrict__ img, const float* __restrict__ wtab,
+// img += A^T y over all views, for NB volumes (volume b at img + b * istride).
+template <int WU, int WV, int NZR, int NB>
+__device__ inline void bp_col_block(float* __restrict__ img, long istride, const WTabs& wtab,
                                     const float2* __restrict__ utab, const float2* __restrict__ vtab,
                                     const ConeViewBP* __restrict__ views, int nview, const Grid& g, int nu, int nv,
                                     int bx, int by, int bz, int wu = WU, int wv = WV) {
@@ -1152,52 +1231,63 @@ __device__ inline void bp_col_block(float* __restrict__ img, const float* __rest
     const int iz0 = bz * NZR;
     if (ix >= g.nx || iy >= g.ny) return;
     const float xc = ix + 0.5f, yc = iy + 0.5f;
-    float acc[NZR];
+    float acc[NZR * NB];
 #pragma unroll
-    for (int k = 0; k < NZR; k++) acc[k] = 0.f;
+    for (int k = 0; k < NZR * NB; k++) acc[k] = 0.f;
     const size_t nuv = (size_t)nu * nv;
     for (int view = 0; view < nview; view++) {
         const ConeViewBP cv = views[view];
-        const float* wt = wtab + view * nuv * WV;
+        const float* wt[NB];
+#pragma unroll
+        for (int b = 0; b < NB; b++) wt[b] = wtab.p[b] + view * nuv * WV;
         const float2* ut = utab + (size_t)view * nu;
         const float2* vt = vtab + (size_t)view * nv;
-        if (cv.flags & 1) bp_view_col<WU, WV, NZR, true>(acc, cv, wt, ut, vt, nu, nv, ix, iy, iz0, xc, yc, wu, wv);
-        else bp_view_col<WU, WV, NZR, false>(acc, cv, wt, ut, vt, nu, nv, ix, iy, iz0, xc, yc, wu, wv);
+        if (cv.flags & 1) bp_view_col<WU, WV, NZR, true, NB>(acc, cv, wt, ut, vt, nu, nv, ix, iy, iz0, xc, yc, wu, wv);
+        else bp_view_col<WU, WV, NZR, false, NB>(acc, cv, wt, ut, vt, nu, nv, ix, iy, iz0, xc, yc, wu, wv);
     }
 #pragma unroll
     for (int k = 0; k < NZR; k++) {
         const int iz = iz0 + k;
-        if (iz < g.nz) img[((size_t)iz * g.ny + iy) * g.nx + ix] += acc[k];
+        if (iz < g.nz)
+#pragma unroll
+            for (int b = 0; b < NB; b++) img[b * istride + ((size_t)iz * g.ny + iy) * g.nx + ix] += acc[k * NB + b];
     }
 }
 
-template <int WU, int WV, int NZR>
-__global__ __launch_bounds__(256) void siddon_cone_bp_col_kernel(float* __restrict__ img, const float* __restrict__ wtab,
+template <int WU, int WV, int NZR, int NB>
+__global__ __launch_bounds__(256) void siddon_cone_bp_col_kernel(float* __restrict__ img, long istride, WTabs wtab,
                                                                  const float2* __restrict__ utab,
                                                                  const float2* __restrict__ vtab,
                                                                  const ConeViewBP* __restrict__ views, int nview,
                                                                  Grid g, int nu, int nv) {
-    bp_col_block<WU, WV, NZR>(img, wtab, utab, vtab, views, nview, g, nu, nv, blockIdx.x, blockIdx.y, blockIdx.z);
+    bp_col_block<WU, WV, NZR, NB>(img, istride, wtab, utab, vtab, views, nview, g, nu, nv, blockIdx.x, blockIdx.y,
+                                  blockIdx.z);
 }
 
-template <int WU, int WV, int NZR>
-static void launch_bp_col_n(float* img, const float* wtab, const float2* utab, const float2* vtab,
+template <int WU, int WV, int NZR, int NB>
+static void launch_bp_col_n(float* img, long istride, const WTabs& wtab, const float2* utab, const float2* vtab,
                             const ConeViewBP* views, int nview, const Grid& g, const Det& d, hipStream_t stream) {
     dim3 block(256), grid(ceil_div(g.nx, 16), ceil_div(g.ny, 16), ceil_div(g.nz, NZR));
     TimedLaunch t("siddon_cone_bp", stream);
-    siddon_cone_bp_col_kernel<WU, WV, NZR><<<grid, block, 0, stream>>>(img, wtab, utab, vtab, views, nview, g, d.nu, d.nv);
+    siddon_cone_bp_col_kernel<WU, WV, NZR, NB><<<grid, block, 0, stream>>>(img, istride, wtab, utab, vtab, views, nview,
+                                                                           g, d.nu, d.nv);
     CTP_LAUNCH_CHECK();
 }
 
+// nb volumes (1, 2 or 4; 2 and 4 with the default z-run only)
 template <int WU, int WV>
-static void launch_bp_col(float* img, const float* wtab, const float2* utab, const float2* vtab,
-                          const ConeViewBP* views, int nview, const Grid& g, const Det& d, hipStream_t stream) {
+static void launch_bp_col(float* img, long istride, int nb, const WTabs& wtab, const float2* utab,
+                          const float2* vtab, const ConeViewBP* views, int nview, const Grid& g, const Det& d,
+                          hipStream_t stream) {
     const int nzr = tuning_get("cone_bp_zrun", 4);
+    if (nb == 4 && nzr == 4) return launch_bp_col_n<WU, WV, 4, 4>(img, istride, wtab, utab, vtab, views, nview, g, d, stream);
+    if (nb == 2 && nzr == 4) return launch_bp_col_n<WU, WV, 4, 2>(img, istride, wtab, utab, vtab, views, nview, g, d, stream);
+    if (nb != 1) fail("cone backprojection: several volumes per launch need cone_bp_zrun 4");
     switch (nzr) {
-        case 1: launch_bp_col_n<WU, WV, 1>(img, wtab, utab, vtab, views, nview, g, d, stream); break;
-        case 2: launch_bp_col_n<WU, WV, 2>(img, wtab, utab, vtab, views, nview, g, d, stream); break;
-        case 4: launch_bp_col_n<WU, WV, 4>(img, wtab, utab, vtab, views, nview, g, d, stream); break;
-        case 8: launch_bp_col_n<WU, WV, 8>(img, wtab, utab, vtab, views, nview, g, d, stream); break;
+        case 1: launch_bp_col_n<WU, WV, 1, 1>(img, istride, wtab, utab, vtab, views, nview, g, d, stream); break;
+        case 2: launch_bp_col_n<WU, WV, 2, 1>(img, istride, wtab, utab, vtab, views, nview, g, d, stream); break;
+        case 4: launch_bp_col_n<WU, WV, 4, 1>(img, istride, wtab, utab, vtab, views, nview, g, d, stream); break;
+        case 8: launch_bp_col_n<WU, WV, 8, 1>(img, istride, wtab, utab, vtab, views, nview, g, d, stream); break;
         default: fail("cone_bp_zrun must be 1, 2, 4 or 8");
     }
 }
@@ -1247,11 +1337,17 @@ void siddon_cone_bp(float* img, const float* prj, const ConeGeomDev& geo, const 
     // cone_bp_mode: 0 = column mode where the geometry allows it, 1 = general
     const bool col = tuning_get("cone_bp_mode", 0) == 0 && column_mode(hgeo, plan, nview) && d.nu < (1 << 24);
     const size_t nimg = (size_t)g.nx * g.ny * g.nz, nprj = (size_t)d.nu * d.nv * nview;
+    // column mode: batch items sharing the geometry are backprojected up to 4
+    // per launch (default z-run; their weight tables within 4 GiB together)
+    const size_t tab = nprj * wv + 16;              // window layout, padded: window loads
+    int nbmax = 1;
+    if (col && tuning_get("cone_bp_zrun", 4) == 4)
+        while (nbmax < 4 && 2 * nbmax <= B && (size_t)(2 * nbmax) * tab * 4 <= (4ull << 30)) nbmax *= 2;
     Workspace& ws = workspace_for(stream);
     Plan pl;
     pl.add<ConeViewBP>(nview);
     if (col) {
-        pl.add<float>(nprj * wv + 16);                // window layout, padded: window loads
+        pl.add<float>(tab * nbmax);
         pl.add<float2>((size_t)nview * d.nu);
         pl.add<float2>((size_t)nview * d.nv + 4);
         pl.add<float2>((size_t)nview * d.nu);
@@ -1270,7 +1366,7 @@ void siddon_cone_bp(float* img, const float* prj, const ConeGeomDev& geo, const 
     int* rowirr = nullptr;
     float4* table = nullptr;
     if (col) {
-        wtab = ws.take_n<float>(nprj * wv + 16);
+        wtab = ws.take_n<float>(tab * nbmax);
         utab = ws.take_n<float2>((size_t)nview * d.nu);
         vtab = ws.take_n<float2>((size_t)nview * d.nv + 4);
         colw = ws.take_n<float2>((size_t)nview * d.nu);
@@ -1284,36 +1380,45 @@ void siddon_cone_bp(float* img, const float* prj, const ConeGeomDev& geo, const 
     if (col)
         for (ConeViewBP& cv : plan.views) cv.flags = 0;
     CTP_CHECK(hipMemcpyAsync(dviews, plan.views.data(), sizeof(ConeViewBP) * nview, hipMemcpyHostToDevice, stream));
-    for (int b = 0; b < B; b++) {
-        float* im = img + b * nimg;
-        if (col) {
-            {
+    if (col) {
+        if (d.nv > 65535 || nview > 65535) fail("cone backprojection: more than 65535 rows or views");
+        {
+            TimedLaunch tt("siddon_bp_table", stream);
+            siddon_bp_colaxes_kernel<<<ceil_div((long)nview * (d.nu + d.nv), 256), 256, 0, stream>>>(
+                utab, vtab, colw, roww, dviews, geo.det_center, geo.det_u, geo.det_v, geo.src, g, d, nview);
+            CTP_LAUNCH_CHECK();
+        }
+        for (int b = 0; b < B;) {
+            const int nb = B - b >= 4 && nbmax >= 4 ? 4 : (B - b >= 2 && nbmax >= 2 ? 2 : 1);
+            WTabs wt{{nullptr, nullptr, nullptr, nullptr}};
+            for (int k = 0; k < nb; k++) {
                 TimedLaunch tt("siddon_bp_table", stream);
-                siddon_bp_colaxes_kernel<<<ceil_div((long)nview * (d.nu + d.nv), 256), 256, 0, stream>>>(
-                    utab, vtab, colw, roww, dviews, geo.det_center, geo.det_u, geo.det_v, geo.src, g, d, nview);
-                CTP_LAUNCH_CHECK();
-                if (d.nv > 65535 || nview > 65535) fail("cone backprojection: more than 65535 rows or views");
+                float* wk = wtab + k * tab;
+                const float* pk = prj + (b + k) * nprj;
+                wt.p[k] = wk;
                 fill_zero_i32_kernel<<<ceil_div((long)nview * d.nv, 256), 256, 0, stream>>>(rowirr,
                                                                                              (size_t)nview * d.nv);
                 CTP_LAUNCH_CHECK();
-                {
-                    const dim3 tg(ceil_div(d.nu, 256), ceil_div(d.nv, kTabRows), nview);
-                    const float* pb = prj + b * nprj;
-                    switch (wv) {
-                        case 1: siddon_bp_coltable_kernel<1><<<tg, 256, 0, stream>>>(wtab, rowirr, pb, colw, roww, geo.src, g, d); break;
-                        case 2: siddon_bp_coltable_kernel<2><<<tg, 256, 0, stream>>>(wtab, rowirr, pb, colw, roww, geo.src, g, d); break;
-                        case 3: siddon_bp_coltable_kernel<3><<<tg, 256, 0, stream>>>(wtab, rowirr, pb, colw, roww, geo.src, g, d); break;
-                        default: siddon_bp_coltable_kernel<4><<<tg, 256, 0, stream>>>(wtab, rowirr, pb, colw, roww, geo.src, g, d); break;
-                    }
+                const dim3 tg(ceil_div(d.nu, 256), ceil_div(d.nv, kTabRows), nview);
+                switch (wv) {
+                    case 1: siddon_bp_coltable_kernel<1><<<tg, 256, 0, stream>>>(wk, rowirr, pk, colw, roww, geo.src, g, d); break;
+                    case 2: siddon_bp_coltable_kernel<2><<<tg, 256, 0, stream>>>(wk, rowirr, pk, colw, roww, geo.src, g, d); break;
+                    case 3: siddon_bp_coltable_kernel<3><<<tg, 256, 0, stream>>>(wk, rowirr, pk, colw, roww, geo.src, g, d); break;
+                    default: siddon_bp_coltable_kernel<4><<<tg, 256, 0, stream>>>(wk, rowirr, pk, colw, roww, geo.src, g, d); break;
                 }
                 CTP_LAUNCH_CHECK();
                 siddon_bp_colpatch_kernel<<<dim3(ceil_div(d.nv, kTabRows), nview), 256, 0, stream>>>(
-                    wtab, im, rowirr, prj + b * nprj, colw, roww, geo.det_center, geo.det_u, geo.det_v, geo.src, g, d,
-                    wv);
+                    wk, img + (b + k) * nimg, rowirr, pk, colw, roww, geo.det_center, geo.det_u, geo.det_v, geo.src, g,
+                    d, wv);
                 CTP_LAUNCH_CHECK();
             }
-#define CTP_BPC_CASE(A, Bv) \
-    if (wu == A && wv == Bv) { launch_bp_col<A, Bv>(im, wtab, utab, vtab, dviews, nview, g, d, stream); continue; }
+            float* im = img + b * nimg;
+            b += nb;
+#define CTP_BPC_CASE(A, Bv)                                                                                    \
+    if (wu == A && wv == Bv) {                                                                                 \
+        launch_bp_col<A, Bv>(im, (long)nimg, nb, wt, utab, vtab, dviews, nview, g, d, stream);                 \
+        continue;                                                                                              \
+    }
             CTP_BPC_CASE(1, 1) CTP_BPC_CASE(1, 2) CTP_BPC_CASE(2, 1) CTP_BPC_CASE(2, 2)
             CTP_BPC_CASE(2, 3) CTP_BPC_CASE(3, 2) CTP_BPC_CASE(3, 3) CTP_BPC_CASE(1, 3) CTP_BPC_CASE(3, 1)
             CTP_BPC_CASE(1, 4) CTP_BPC_CASE(4, 1) CTP_BPC_CASE(2, 4) CTP_BPC_CASE(4, 2) CTP_BPC_CASE(3, 4)
@@ -1321,6 +1426,12 @@ void siddon_cone_bp(float* img, const float* prj, const ConeGeomDev& geo, const 
 #undef CTP_BPC_CASE
             fail("cone backprojection: unsupported window");
         }
+        // host plan vector must outlive the async copy
+        CTP_CHECK(hipStreamSynchronize(stream));
+        return;
+    }
+    for (int b = 0; b < B; b++) {
+        float* im = img + b * nimg;
         {
             TimedLaunch tt("siddon_bp_table", stream);
             siddon_bp_table_kernel<<<ceil_div((long)nprj, 256), 256, 0, stream>>>(table, prj + b * nprj, geo.det_center,
@@ -1563,9 +1674,7 @@ __global__ __launch_bounds__(256) void siddon_cone_fp_batch_col_kernel(const Con
         const Grid g = J->g;
         const __amdgpu_buffer_rsrc_t volz =
             __builtin_amdgcn_make_buffer_rsrc((void*)J->imgz, (short)0, 4 * J->zcol * g.nx * g.ny, 0x00020000);
-        const __amdgpu_buffer_rsrc_t vol =
-            __builtin_amdgcn_make_buffer_rsrc((void*)J->img, (short)0, 4 * g.nx * g.ny * g.nz, 0x00020000);
-        fp_column<false>(J->prj, volz, vol, J->dc, J->du, J->dv, J->src, g, J->d, view, iu, J->zcol);
+        fp_column<false, 1>(J->prj, volz, J->img, J->dc, J->du, J->dv, J->src, g, J->d, view, iu, J->zcol, 0, 0);
     }
 }
 
@@ -1581,8 +1690,9 @@ __global__ __launch_bounds__(256) void siddon_cone_bp_batch_kernel(const ConeJob
         const int bx = (int)(local % J->nbx);
         const long rest = local / J->nbx;
         const int by = (int)(rest % J->nby), bz = (int)(rest / J->nby);
-        bp_col_block<WU, WV, NZR>(J->img, J->wtab, J->utab, J->vtab, J->views, J->nview, J->g, J->d.nu, J->d.nv, bx,
-                                  by, bz, J->wu, J->wv);
+        const WTabs wt{{J->wtab, nullptr, nullptr, nullptr}};
+        bp_col_block<WU, WV, NZR, 1>(J->img, 0, wt, J->utab, J->vtab, J->views, J->nview, J->g, J->d.nu, J->d.nv, bx,
+                                     by, bz, J->wu, J->wv);
     }
 }
 

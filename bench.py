@@ -101,7 +101,7 @@ def fan_workload_bench(steps, warmup):
     return dict(w=w, img=img, nvu=nvu, ms_step=ms_step, times=times, finite=finite)
 
 
-def cone_extra(steps):
+def cone_extra(steps, cpu=False):
     import torch
     from ctprojector_amd import _lib
     from ctprojector_amd.phantom import cone_workload
@@ -162,7 +162,126 @@ def cone_extra(steps):
     out['batch2_ms_per_step'] = round(ms2, 3)
     out['batch2_value_gvu_s'] = round(2 * 2 * nvu / (ms2 / 1e3) / 1e9, 2)
     del prj, bp, img2
+    if cpu:
+        out['cpu_baseline'] = cone_cpu_baseline(w, img, geo)
     return out
+
+
+def cone_cpu_baseline(w, img, geo, target_s=12.0):
+    """FP32 OpenMP mirror of the reference's Siddon FP and atomic BP on a view
+    subset of the 256^3 cone workload (cost is linear in views)."""
+    from oracle import oracle as O
+
+    O.build()
+    threads = O.num_threads()
+    nvox = w.nx * w.ny * w.nz
+
+    def run(nv):
+        sub = [a[:nv] for a in (geo.det_center, geo.det_u, geo.det_v, geo.src)]
+        t = time.perf_counter()
+        prj = O.siddon_cone_fp(w, img, *sub, prec='f')
+        t1 = time.perf_counter()
+        O.siddon_cone_bp(w, prj, *sub, prec='f')
+        return t1 - t, time.perf_counter() - t1
+
+    a, b = run(2)
+    nv = int(min(w.nview, max(2, 2 * target_s / max(a + b, 1e-3))))
+    log('[bench] cone cpu baseline: %d of %d views on %d threads' % (nv, w.nview, threads))
+    a, b = run(nv)
+    return {'value': round(2 * nvox * nv / (a + b) / 1e9, 4), 'unit': 'GVU/s', 'cores': threads, 'kind': 'port',
+            'sample': 'cone Siddon FP+BP of the 256^3 volume over the first %d of %d views (FP %.2fs, BP %.2fs); '
+                      'FP32 OpenMP restatement (oracle/oracle.cpp)' % (nv, w.nview, a, b)}
+
+
+def unrolled_sqs_extra(steps):
+    """Differentiable unrolled SQS-TV (10 iterations, exact adjoint, per-iteration
+    checkpointing) on 8 fan-beam 256^2 slices @1.4 mm, 492 views, 444 columns,
+    Poisson data at 1e5 photons: one training step = the network forward and
+    the gradient with respect to the sinogram."""
+    import torch
+    from ctprojector_amd.autograd import FanDD, sqs_tv_unrolled
+    from ctprojector_amd.phantom import fan_slices_256_workload
+    from ctprojector_amd.projector.cupy import ct_projector
+
+    w, img = fan_slices_256_workload(8)
+    p = ct_projector()
+    for k in ('nx', 'ny', 'nz', 'dx', 'dy', 'dz', 'cx', 'cy', 'cz', 'nu', 'nv', 'du', 'dv', 'off_u', 'off_v',
+              'dsd', 'dso', 'nview'):
+        setattr(p, k, getattr(w, k))
+    dev = torch.device('cuda', 0)
+    op = FanDD(p, torch.from_numpy(w.angles).to(dev))
+    x_true = torch.from_numpy(img).to(dev)
+    gen = torch.Generator(device=dev).manual_seed(1004)
+    with torch.no_grad():
+        line = op.fp(x_true)
+        counts = torch.poisson(1e5 * torch.exp(-line), generator=gen)
+        prj = -torch.log(torch.clamp(counts, min=1.0) / 1e5)
+        wgt = counts / 1e5
+        v = torch.rand(x_true.shape, device=dev, generator=gen)
+        for _ in range(10):                  # power method for the projector norm
+            v = op.adjoint(op.fp(v))
+            v = v / v.norm()
+        pn = float(op.fp(v).norm())
+        norm_img = op.adjoint(wgt * op.fp(torch.ones_like(x_true))) / pn / pn
+    niter, beta = 10, 2e-3
+
+    def step():
+        q = prj.clone().requires_grad_(True)
+        x = sqs_tv_unrolled(op, q, torch.zeros_like(x_true), norm_img, pn, beta, niter, weight=wgt)
+        loss = ((x - x_true) ** 2).sum()
+        loss.backward()
+        return loss, q.grad
+
+    log('[bench] unrolled SQS-TV warmup')
+    step()
+    torch.cuda.synchronize()
+    log('[bench] unrolled SQS-TV timed x%d' % steps)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        loss, grad = step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    nvu = img.shape[0] * w.nx * w.ny * w.nz * w.nview
+    # FP and A^T per iteration forward, again in the checkpointed recomputation,
+    # and the two products of the backward pass: 6 passes per iteration
+    passes = 6 * niter
+    return {'workload': 'unrolled SQS-TV, 10 iterations, 8 fan slices 256^2 @1.4 mm, 492 views, 444 columns, '
+                        'Poisson 1e5; forward + gradient w.r.t. the sinogram',
+            'ms_per_step': round(ms, 3), 'ms_per_iteration': round(ms / niter, 3),
+            'projector_passes_per_step': passes, 'value_gvu_s': round(passes * nvu / (ms / 1e3) / 1e9, 2),
+            'finite': bool(torch.isfinite(grad).all().item()) and bool(torch.isfinite(loss).item())}
+
+
+def tv_prior_extra(steps):
+    """TV-SQS surrogate (s1, s2, var) and the fused SQS-TV update on a 256^3 volume:
+    kernel time and bandwidth against the fused kernels' 16 B per voxel."""
+    import torch
+    from ctprojector_amd import _lib
+    from ctprojector_amd.prior.cupy import tv_sqs, tv_sqs_update
+
+    dev = torch.device('cuda', 0)
+    gen = torch.Generator(device=dev).manual_seed(1006)
+    x = torch.rand([1, 256, 256, 256], device=dev, generator=gen)
+    bp = torch.rand_like(x)
+    nrm = torch.rand_like(x) + 1.0
+    tv_sqs(x, (1.0, 1.0, 1.0))
+    tv_sqs_update(x, bp, nrm, 1e-3, (1.0, 1.0, 1.0))
+    torch.cuda.synchronize()
+    _lib.set_kernel_timing(True)
+    _lib.reset_kernel_times()
+    for _ in range(steps):
+        tv_sqs(x, (1.0, 1.0, 1.0))
+        tv_sqs_update(x, bp, nrm, 1e-3, (1.0, 1.0, 1.0))
+    torch.cuda.synchronize()
+    times = _lib.kernel_times()
+    _lib.set_kernel_timing(False)
+    a, _ = kernel_stats(times, 'tv_sqs')
+    b, _ = kernel_stats(times, 'tv_sqs_update')
+    nbytes = 16 * x.numel()
+    return {'workload': 'TV-SQS prior on 256^3 (s1, s2, var) and the fused SQS-TV update',
+            'kernel_ms': {'tv_sqs': round(a, 4), 'tv_sqs_update': round(b, 4)},
+            'hbm_frac': {'tv_sqs': round(nbytes / (a / 1e3) / 1e9 / PEAK_HBM_GBS, 3) if a > 0 else None,
+                         'tv_sqs_update': round(nbytes / (b / 1e3) / 1e9 / PEAK_HBM_GBS, 3) if b > 0 else None}}
 
 
 def cone_batch_extra(steps):
@@ -500,7 +619,7 @@ def main(argv=None):
     if not args.no_extra:
         extra = {}
         try:
-            extra['cone'] = cone_extra(2)
+            extra['cone'] = cone_extra(2, cpu=not args.no_cpu_baseline)
         except Exception as e:  # report, do not hide
             extra['cone'] = {'error': repr(e)}
         try:
@@ -511,6 +630,14 @@ def main(argv=None):
             extra['fbp'] = fbp_extra(3, w, r['img'])
         except Exception as e:
             extra['fbp'] = {'error': repr(e)}
+        try:
+            extra['unrolled_sqs_tv'] = unrolled_sqs_extra(3)
+        except Exception as e:
+            extra['unrolled_sqs_tv'] = {'error': repr(e)}
+        try:
+            extra['tv_prior'] = tv_prior_extra(5)
+        except Exception as e:
+            extra['tv_prior'] = {'error': repr(e)}
         if not args.no_serving:
             try:
                 extra['serving'] = serving_extra()

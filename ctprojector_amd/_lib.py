@@ -28,7 +28,7 @@ SYMBOLS = [
     'cfbpFanBackprojection', 'cupyfbpFanBackprojection',
     'cSiddonFanProjection', 'cSiddonFanBackprojection', 'cfbpParallelBackprojection', 'cupyFbpTomoFilter',
     'ctpDistanceDrivenFanProjectionAdjoint', 'ctpDistanceDrivenParallelProjectionAdjoint',
-    'cupyTVSQS3D', 'cTVSQS3D', 'ctpTVSQSUpdate3D',
+    'cupyTVSQS3D', 'cTVSQS3D', 'ctpTVSQSUpdate3D', 'ctpTVSQSBackward3D',
     'ctpConePlanCreate', 'ctpConePlanForward', 'ctpConePlanBackward', 'ctpConePlanConfig', 'ctpConePlanDestroy',
     'ctpGetLastErrorMessage', 'ctpReleaseWorkspace', 'ctpSetStream',
     'ctpSetKernelTiming', 'ctpResetKernelTimes', 'ctpGetKernelTime', 'ctpKernelTimeReport',

@@ -16,17 +16,21 @@ circular_2d.py:140-191,392-443).  Here each projector is a
   differentiable operator (its gradient is ``fp``);
 * tomosynthesis: the reference's convention (gradient of fp = bp).
 
-Backward passes are themselves built from these Functions, so higher-order
-gradients work.  ``sqs_tv_unrolled`` runs K SQS-TV iterations as a
-differentiable network (TV surrogate in torch arithmetic, projectors through
-the kernels above) with optional per-iteration checkpointing.
+Projector backward passes are themselves built from these Functions, so
+higher-order gradients work.  ``tv_sqs`` is the TV surrogate of the HIP prior
+kernel with a hand-written HIP vector-Jacobian product (first order only);
+``tv_sqs_torch`` is the same surrogate in torch arithmetic.
+``sqs_tv_unrolled`` runs K SQS-TV iterations as a differentiable network with
+optional per-iteration checkpointing.
 """
 
 from typing import Callable, Optional, Sequence, Tuple
 
 import torch
 import torch.utils.checkpoint
+from torch.autograd.function import once_differentiable
 
+from .prior import cupy as prior_dev
 from .projector.cupy import cone, fan_equiangular, parallel, tomo
 
 
@@ -161,21 +165,61 @@ def tv_sqs_torch(img: torch.Tensor, weights: Sequence[float] = (1.0, 1.0, 1.0), 
     return s1, s2, var
 
 
+class _TVSurrogate(torch.autograd.Function):
+    """(s1, s2, var) of the HIP TV kernel; backward = the HIP vector-Jacobian
+    product (ctpTVSQSBackward3D)."""
+
+    @staticmethod
+    def forward(ctx, img, weights, eps):
+        ctx.set_materialize_grads(False)
+        ctx.weights, ctx.eps = tuple(float(w) for w in weights), float(eps)
+        ctx.save_for_backward(img)
+        s1, s2, var = prior_dev.tv_sqs(img, ctx.weights, ctx.eps)
+        return s1, s2, var
+
+    @staticmethod
+    @once_differentiable
+    def backward(ctx, g1, g2, gv):
+        img, = ctx.saved_tensors
+        if g1 is None and g2 is None and gv is None:
+            return None, None, None
+        z = None
+        if g1 is None or g2 is None:
+            z = torch.zeros_like(img)
+        d = prior_dev.tv_sqs_backward(img, z if g1 is None else g1, z if g2 is None else g2, ctx.weights, ctx.eps,
+                                      gvar=gv)
+        return d.to(img.dtype), None, None
+
+
+def tv_sqs(img: torch.Tensor, weights: Sequence[float] = (1.0, 1.0, 1.0), eps: float = 1e-8
+           ) -> Tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
+    """Differentiable TV surrogate (s1, s2, var) of img [batch, nz, ny, nx] on
+    the device, weights along (z, y, x): forward and gradient are HIP kernels
+    (two passes over the volume each)."""
+    return _TVSurrogate.apply(img, weights, eps)
+
+
 def sqs_tv_unrolled(op: _Op, prj: torch.Tensor, x0: torch.Tensor, norm_img: torch.Tensor, projector_norm: float,
                     beta, niter: int, weights: Sequence[float] = (1.0, 1.0, 1.0), eps: float = 1e-8,
-                    weight: Optional[torch.Tensor] = None, checkpoint: bool = True) -> torch.Tensor:
+                    weight: Optional[torch.Tensor] = None, checkpoint: bool = True, tv: str = 'kernel'
+                    ) -> torch.Tensor:
     """K differentiable SQS-TV iterations (recon/cupy/sqs_algorithms.py:22-57 with
     prior_func = tv_sqs): x <- x - (A^T w (A x - p) / n^2 + beta s1) / (norm_img + beta s2).
     ``op.adjoint`` supplies A^T so that the network's gradient is exact.
     ``checkpoint`` recomputes each iteration in the backward pass instead of
-    keeping its intermediates (memory per iteration: one volume)."""
+    keeping its intermediates (memory per iteration: one volume).  ``tv``
+    selects the surrogate: 'kernel' (HIP forward and gradient, first-order
+    gradients) or 'torch' (torch arithmetic, any order)."""
+    if tv not in ('kernel', 'torch'):
+        raise ValueError("tv must be 'kernel' or 'torch'")
     w = 1.0 if weight is None else weight
     pn = float(projector_norm)
+    surrogate = tv_sqs if tv == 'kernel' else tv_sqs_torch
 
     def step(x, p):
         r = op.fp(x) / pn - p / pn
         g = op.adjoint(r * w) / pn
-        s1, s2, _ = tv_sqs_torch(x, weights, eps)
+        s1, s2, _ = surrogate(x, weights, eps)
         return x - (g + beta * s1) / (norm_img + beta * s2)
 
     x = x0

@@ -666,6 +666,21 @@ CTP_EXPORT int ctpTVSQSUpdate3D(float* out, float* var, const float* img, const 
     });
 }
 
+CTP_EXPORT int ctpTVSQSBackward3D(float* dimg, const float* img, const float* g1, const float* g2, const float* gvar,
+                                  float wx, float wy, float wz, size_t nBatches, size_t nx, size_t ny, size_t nz,
+                                  float eps) {
+    return guarded("ctpTVSQSBackward3D", [&] {
+        if (nBatches == 0 || nx == 0 || ny == 0 || nz == 0) return;
+        tv_dims(nBatches, nx, ny, nz);
+        need_ptr(dimg, "dimg"); need_ptr(img, "img"); need_ptr(g1, "g1"); need_ptr(g2, "g2");
+        need(dimg != img && dimg != g1 && dimg != g2 && dimg != gvar, "ctpTVSQSBackward3D: dimg must not alias an input");
+        touch();
+        tv_sqs_backward(dimg, img, g1, g2, gvar, wx, wy, wz, (int)nBatches, (int)nx, (int)ny, (int)nz, eps,
+                        cur_stream());
+        sync();
+    });
+}
+
 // ===========================================================================
 // Batched cone-beam plans (no counterpart in the reference)
 // ===========================================================================

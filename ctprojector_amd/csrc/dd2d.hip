@@ -758,10 +758,15 @@ __global__ __launch_bounds__(256) void dd_bp_kernel(float* __restrict__ img, flo
 // (one lane per pixel alone gives 4 waves per SIMD at 512x512).  Partial
 // images of the groups are summed in a fixed order afterwards: deterministic.
 int bp_view_groups(size_t npix, int nchunk, int nview) {
-    // measured on MI355X: at 512x512x32 the partial images cost more than the
-    // extra waves gain, so one group is the default
-    (void)npix;
-    int k = tuning_get("bp_view_groups", 1);
+    // The pixel-driven backprojections launch one 256-lane block per 16x16
+    // pixels and channel chunk.  Below about 1024 blocks (4 per CU) the
+    // launch cannot hide its gather latency, so the views are split into
+    // groups whose partial images are summed afterwards.  Measured on
+    // MI355X: at 512x512x32 (1024 blocks) the partial images cost more than
+    // extra waves gain, so such launches keep one group.
+    const size_t blocks = std::max<size_t>(1, (npix + 255) / 256 * (size_t)std::max(nchunk, 1));
+    const int automatic = (int)std::min<size_t>(64, (1024 + blocks - 1) / blocks);
+    int k = tuning_get("bp_view_groups", automatic);
     k = std::max(1, std::min(k, std::max(1, nview / 8)));
     k = std::max(1, std::min(k, 65535 / std::max(nchunk, 1)));   // grid.z = nchunk * groups
     return k;

@@ -87,6 +87,75 @@ __global__ __launch_bounds__(64) void tv_sqs_update_kernel(float* __restrict__ o
     if (var) var[i] = v;
 }
 
+// ---------------------------------------------------------------------------
+// gradient of the surrogate (no counterpart in the reference): for the
+// unrolled, differentiable SQS-TV network, dimg = (ds1/dimg)^T g1 +
+// (ds2/dimg)^T g2 + (dvar/dimg)^T gv, as two deterministic gathers.
+//
+// With D0_d(q) = x(q) - x(q - e_d) (0 at the lower edge), A(q) = sum_d w_d D0_d(q),
+// W = wx + wy + wz and v the variation, the loss gradient with respect to v(q) is
+//   Gv(q) = gv(q) - [g1(q) A(q) + g2(q) W
+//                    + sum_{d: q_d > 0} w_d (g2(q - e_d) - g1(q - e_d) D0_d(q))
+//                    + sum_{d: q_d = n_d - 1} w_d g2(q)] / v(q)^2
+// (the last sum: at the upper edge s1, s2 use v(q) for the missing neighbour).
+// Pass 1 stores P = Gv / v and R = 1 / v; pass 2 gathers
+//   dimg(q) = sum_{d: q_d > 0} w_d (D0_d(q) P(q) + (g1(q) - g1(q - e_d)) R(q))
+//           + sum_{d: q_d < n_d - 1} w_d ((g1(q) - g1(q + e_d)) R(q + e_d) - D0_d(q + e_d) P(q + e_d)).
+__global__ __launch_bounds__(64) void tv_grad_pr_kernel(float2* __restrict__ pr, const float* __restrict__ img,
+                                                        const float* __restrict__ g1, const float* __restrict__ g2,
+                                                        const float* __restrict__ gv, int nx, int ny, int nz, TVW w) {
+    const int ix = blockIdx.x * 64 + threadIdx.x;
+    const int iy = blockIdx.y;
+    const int b = blockIdx.z / nz, iz = blockIdx.z - b * nz;
+    if (ix >= nx) return;
+    const size_t nxy = (size_t)nx * ny;
+    const size_t base = (size_t)b * nxy * nz;
+    const size_t q = base + (size_t)iz * nxy + (size_t)iy * nx + ix;
+    const float x = img[q];
+    const float dx = ix > 0 ? x - img[q - 1] : 0.f;
+    const float dy = iy > 0 ? x - img[q - nx] : 0.f;
+    const float dz = iz > 0 ? x - img[q - nxy] : 0.f;
+    const float v = sqrtf(w.wx * dx * dx + w.wy * dy * dy + w.wz * dz * dz + w.eps);
+    const float a1 = g1[q], a2 = g2[q];
+    float num = a1 * (w.wx * dx + w.wy * dy + w.wz * dz) + a2 * (w.wx + w.wy + w.wz);
+    if (ix > 0) num += w.wx * (g2[q - 1] - g1[q - 1] * dx);
+    if (iy > 0) num += w.wy * (g2[q - nx] - g1[q - nx] * dy);
+    if (iz > 0) num += w.wz * (g2[q - nxy] - g1[q - nxy] * dz);
+    if (ix == nx - 1) num += w.wx * a2;
+    if (iy == ny - 1) num += w.wy * a2;
+    if (iz == nz - 1) num += w.wz * a2;
+    const float r = 1.f / v;
+    const float gvq = (gv ? gv[q] : 0.f) - num * r * r;
+    pr[q] = make_float2(gvq * r, r);
+}
+
+__global__ __launch_bounds__(64) void tv_grad_x_kernel(float* __restrict__ dimg, const float2* __restrict__ pr,
+                                                       const float* __restrict__ img, const float* __restrict__ g1,
+                                                       int nx, int ny, int nz, TVW w) {
+    const int ix = blockIdx.x * 64 + threadIdx.x;
+    const int iy = blockIdx.y;
+    const int b = blockIdx.z / nz, iz = blockIdx.z - b * nz;
+    if (ix >= nx) return;
+    const size_t nxy = (size_t)nx * ny;
+    const size_t base = (size_t)b * nxy * nz;
+    const size_t q = base + (size_t)iz * nxy + (size_t)iy * nx + ix;
+    const float x = img[q], a = g1[q];
+    const float2 own = pr[q];
+    float sum = 0.f;
+    // one axis: step s in the flattened index, weight wd, lower / upper neighbour present
+    auto axis = [&](size_t s, float wd, bool lower, bool upper) {
+        if (lower) sum += wd * ((x - img[q - s]) * own.x + (a - g1[q - s]) * own.y);
+        if (upper) {
+            const float2 nb = pr[q + s];
+            sum += wd * ((a - g1[q + s]) * nb.y - (img[q + s] - x) * nb.x);
+        }
+    };
+    axis(1, w.wx, ix > 0, ix < nx - 1);
+    axis((size_t)nx, w.wy, iy > 0, iy < ny - 1);
+    axis(nxy, w.wz, iz > 0, iz < nz - 1);
+    dimg[q] = sum;
+}
+
 static dim3 tv_grid(int B, int nx, int ny, int nz) {
     if ((long)nz * B > 65535 || ny > 65535) fail("tv prior: ny and nz * batches must not exceed 65535");
     return dim3(ceil_div(nx, 64), ny, nz * B);
@@ -106,6 +175,23 @@ void tv_sqs_update(float* out, float* var, const float* img, const float* bp, co
     const TVW w{wx, wy, wz, eps};
     TimedLaunch t("tv_sqs_update", stream);
     tv_sqs_update_kernel<<<tv_grid(B, nx, ny, nz), 64, 0, stream>>>(out, var, img, bp, norm, beta, acc, nx, ny, nz, w);
+    CTP_LAUNCH_CHECK();
+}
+
+void tv_sqs_backward(float* dimg, const float* img, const float* g1, const float* g2, const float* gv, float wx,
+                     float wy, float wz, int B, int nx, int ny, int nz, float eps, hipStream_t stream) {
+    const TVW w{wx, wy, wz, eps};
+    const dim3 grid = tv_grid(B, nx, ny, nz);
+    const size_t n = (size_t)B * nx * ny * nz;
+    Workspace& ws = workspace_for(stream);
+    Plan pl;
+    pl.add<float2>(n);
+    ws.reserve(pl.bytes);
+    float2* pr = ws.take_n<float2>(n);
+    TimedLaunch t("tv_sqs_backward", stream);
+    tv_grad_pr_kernel<<<grid, 64, 0, stream>>>(pr, img, g1, g2, gv, nx, ny, nz, w);
+    CTP_LAUNCH_CHECK();
+    tv_grad_x_kernel<<<grid, 64, 0, stream>>>(dimg, pr, img, g1, nx, ny, nz, w);
     CTP_LAUNCH_CHECK();
 }
 

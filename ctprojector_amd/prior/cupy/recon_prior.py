@@ -4,7 +4,7 @@ tensors, plus ``tv_sqs_update``, the fused SQS-TV step.  The NLM priors of that
 module are not on the projector path and are not provided."""
 
 from ctypes import c_float, c_ulong
-from typing import Tuple
+from typing import Optional, Tuple
 
 import torch
 
@@ -27,6 +27,27 @@ def tv_sqs(img: torch.Tensor, weights: Tuple[float, float, float], eps: float = 
     if err != 0:
         print(err)
     return s1, s2, var
+
+
+def tv_sqs_backward(img: torch.Tensor, g1: torch.Tensor, g2: torch.Tensor, weights: Tuple[float, float, float],
+                    eps: float = 1e-8, gvar: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """Vector-Jacobian product of ``tv_sqs``: the gradient with respect to img of
+    sum(g1 * s1) + sum(g2 * s2) + sum(gvar * var) (no counterpart in the
+    reference; used by the differentiable unrolled SQS-TV network)."""
+    dev = device_of(img)
+    img = dev_f32(img, dev)
+    g1 = dev_f32(g1, dev).expand_as(img).contiguous()
+    g2 = dev_f32(g2, dev).expand_as(img).contiguous()
+    gv = dev_f32(gvar, dev).expand_as(img).contiguous() if gvar is not None else None
+    out = torch.empty_like(img)
+    with on_device(dev):
+        err = _lib.module().ctpTVSQSBackward3D(
+            ptr(out), ptr(img), ptr(g1), ptr(g2), ptr(gv) if gv is not None else None, c_float(weights[2]),
+            c_float(weights[1]), c_float(weights[0]), c_ulong(img.shape[0]), c_ulong(img.shape[3]),
+            c_ulong(img.shape[2]), c_ulong(img.shape[1]), c_float(eps))
+    if err != 0:
+        raise RuntimeError('ctpTVSQSBackward3D failed (%d): %s' % (err, _lib.last_error_message()))
+    return out
 
 
 def tv_sqs_update(img: torch.Tensor, bp: torch.Tensor, norm_img: torch.Tensor, beta: float,

@@ -191,6 +191,12 @@ int cTVSQS3D(float* s1, float* s2, float* var, const float* img, float wx, float
 int ctpTVSQSUpdate3D(float* out, float* var, const float* img, const float* bp, const float* norm, float beta,
                      float acc, float wx, float wy, float wz, size_t nBatches, size_t nx, size_t ny, size_t nz,
                      float eps);
+/* No counterpart in the reference (gradients for unrolled, differentiable
+ * SQS-TV networks): dimg = J_s1^T g1 + J_s2^T g2 + J_var^T gvar, the
+ * vector-Jacobian product of cupyTVSQS3D's outputs at img.  gvar may be NULL
+ * (zero).  dimg is overwritten and must not alias the inputs.  Device arrays. */
+int ctpTVSQSBackward3D(float* dimg, const float* img, const float* g1, const float* g2, const float* gvar, float wx,
+                       float wy, float wz, size_t nBatches, size_t nx, size_t ny, size_t nz, float eps);
 
 /* ---- batched cone-beam plans (no counterpart in the reference) ------------ */
 /* Many Siddon cone-beam jobs (volumes of different sizes, each with its own

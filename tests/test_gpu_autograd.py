@@ -1,6 +1,9 @@
 """GPU: differentiable projectors (ctprojector_amd.autograd).
 
-* the torch TV surrogate equals the HIP TV kernel;
+* the torch TV surrogate equals the HIP TV kernel, and the HIP gradient of the
+  surrogate (ctpTVSQSBackward3D) equals torch autograd through the torch
+  surrogate in FP64 (the latter is pinned against finite differences of the
+  FP64 oracle in the CPU suite);
 * gradients of fp / bp / adjoint are the hand-written transposes (dot tests
   through autograd), Siddon cone and DD fan;
 * an unrolled SQS-TV network (fan DD, 3 iterations, checkpointed): the
@@ -33,6 +36,51 @@ def test_tv_torch_matches_kernel(torch):
     x = torch.from_numpy(np.random.default_rng(0).uniform(size=[2, 4, 9, 11]).astype(np.float32)).cuda()
     for a, b in zip(tv_sqs_torch(x, (0.5, 1.0, 2.0), 1e-4), tv_sqs(x, (0.5, 1.0, 2.0), 1e-4)):
         assert rel_l2(a.cpu().numpy(), b.cpu().numpy()) < 1e-6
+
+
+@pytest.mark.parametrize('shape', [(2, 4, 9, 11), (1, 1, 5, 7), (1, 6, 1, 3)])
+def test_tv_kernel_gradient_matches_torch_autograd(torch, shape):
+    from ctprojector_amd.autograd import tv_sqs, tv_sqs_torch
+    rng = np.random.default_rng(7)
+    x = rng.uniform(size=shape).astype(np.float32)
+    g1, g2, gv = (rng.normal(size=shape).astype(np.float32) for _ in range(3))
+    w, eps = (0.5, 1.0, 2.0), 1e-4
+
+    def ref(use_gv):
+        xt = torch.tensor(x, dtype=torch.float64, requires_grad=True)
+        s1, s2, var = tv_sqs_torch(xt, w, eps)
+        loss = (s1 * torch.from_numpy(g1).double()).sum() + (s2 * torch.from_numpy(g2).double()).sum()
+        if use_gv:
+            loss = loss + (var * torch.from_numpy(gv).double()).sum()
+        loss.backward()
+        return xt.grad.numpy()
+
+    for use_gv in (True, False):
+        xd = torch.from_numpy(x).cuda().requires_grad_(True)
+        s1, s2, var = tv_sqs(xd, w, eps)
+        loss = (s1 * torch.from_numpy(g1).cuda()).sum() + (s2 * torch.from_numpy(g2).cuda()).sum()
+        if use_gv:
+            loss = loss + (var * torch.from_numpy(gv).cuda()).sum()
+        loss.backward()
+        assert rel_l2(xd.grad.cpu().numpy(), ref(use_gv)) < 1e-5, (shape, use_gv)
+
+
+def test_unrolled_kernel_tv_matches_torch_tv(torch):
+    from ctprojector_amd.autograd import FanDD, sqs_tv_unrolled
+    from ctprojector_amd.projector import cupy as C
+    g, ang, img = small_fan(n=24, nz=1, nv=1, nview=30, nu=40, du=2.0, seed=8)
+    dev = torch.device('cuda', 0)
+    op = FanDD(g.to(C.ct_projector()), torch.from_numpy(ang).to(dev))
+    x_true = torch.from_numpy(np.concatenate([img, 0.7 * img])).to(dev)
+    prj = op.fp(x_true).detach()
+    norm_img = op.adjoint(op.fp(torch.ones_like(x_true))).detach()
+    grads = []
+    for tv in ('kernel', 'torch'):
+        q = prj.clone().requires_grad_(True)
+        x = sqs_tv_unrolled(op, q, torch.zeros_like(x_true), norm_img, 1.0, 0.05, 3, eps=1e-4, tv=tv)
+        ((x - x_true) ** 2).sum().backward()
+        grads.append(q.grad.cpu().numpy())
+    assert rel_l2(grads[0], grads[1]) < 1e-4
 
 
 def _grad_dot(torch, f, x, y):

@@ -29,6 +29,7 @@ SYMBOLS = [
     'cSiddonFanProjection', 'cSiddonFanBackprojection', 'cfbpParallelBackprojection', 'cupyFbpTomoFilter',
     'ctpDistanceDrivenFanProjectionAdjoint', 'ctpDistanceDrivenParallelProjectionAdjoint',
     'cupyTVSQS3D', 'cTVSQS3D', 'ctpTVSQSUpdate3D', 'ctpTVSQSBackward3D',
+    'ctpDDPlanCreate', 'ctpDDPlanForward', 'ctpDDPlanBackward', 'ctpDDPlanAdjoint', 'ctpDDPlanDestroy',
     'ctpConePlanCreate', 'ctpConePlanForward', 'ctpConePlanBackward', 'ctpConePlanConfig', 'ctpConePlanDestroy',
     'ctpGetLastErrorMessage', 'ctpReleaseWorkspace', 'ctpSetStream',
     'ctpSetKernelTiming', 'ctpResetKernelTimes', 'ctpGetKernelTime', 'ctpKernelTimeReport',

@@ -30,6 +30,7 @@ import torch
 import torch.utils.checkpoint
 from torch.autograd.function import once_differentiable
 
+from .ddplan import DDPlan
 from .prior import cupy as prior_dev
 from .projector.cupy import cone, fan_equiangular, parallel, tomo
 
@@ -93,20 +94,47 @@ class ConeSiddon(_Op):
 
 
 class _DD(_Op):
+    """``plan=True`` (default): device tensors go through persistent plans
+    (``DDPlan``, one per batch size and device) built on first use, so the
+    geometry (projector attributes, angles) is fixed from then on; the calls
+    are asynchronous.  ``plan=False`` calls the per-call entry points."""
     mod = None
+    fan = True
 
-    def __init__(self, projector, angles, branchless: bool = False, adjoint: str = 'exact'):
+    def __init__(self, projector, angles, branchless: bool = False, adjoint: str = 'exact', plan: bool = True):
         if adjoint not in ('exact', 'reference'):
             raise ValueError("adjoint must be 'exact' or 'reference'")
         self.p, self.angles, self.branchless, self.mode = projector, angles, branchless, adjoint
+        self.use_plan = plan
+        self._plans = {}
+
+    def _plan(self, t):
+        if not (self.use_plan and t.is_cuda):
+            return None
+        key = (int(t.shape[0]), t.device)
+        pl = self._plans.get(key)
+        if pl is None:
+            pl = DDPlan(self.p, self.angles, batch=t.shape[0], fan=self.fan, branchless=self.branchless,
+                        device=t.device)
+            self._plans[key] = pl
+        return pl
 
     def _fp(self, x):
+        pl = self._plan(x)
+        if pl is not None:
+            return pl.fp(x)
         return self.mod.distance_driven_fp(self.p, x, self.angles, branchless=self.branchless)
 
     def _bp(self, y):
+        pl = self._plan(y)
+        if pl is not None:
+            return pl.bp(y)
         return self.mod.distance_driven_bp(self.p, y, self.angles, branchless=self.branchless)
 
     def _exact(self, y):
+        pl = self._plan(y)
+        if pl is not None:
+            return pl.adjoint(y)
         return self.mod.distance_driven_fp_adjoint(self.p, y, self.angles, branchless=self.branchless)
 
     def fp(self, x):
@@ -124,6 +152,7 @@ class FanDD(_DD):
 class ParallelDD(_DD):
     """Distance-driven parallel beam."""
     mod = parallel
+    fan = False
 
 
 class TomoDD(_Op):

@@ -387,6 +387,54 @@ CTP_EXPORT int ctpDistanceDrivenParallelProjectionAdjoint(float* img, const floa
                             dz, cx, cy, cz, nu, nv, nview, da, dv, off_a, off_v, 1000.f, 500.f, typeProjector, false);
 }
 
+// Persistent DD plans (no counterpart in the reference): the geometry tables of
+// one fan / parallel geometry and batch count built once (angles in host
+// memory); the calls are asynchronous on the given stream.
+CTP_EXPORT int ctpDDPlanCreate(void** plan, int fan, const float* deg, DD_GRID_ARGS, float dsd, float dso,
+                               int typeProjector) {
+    return guarded("ctpDDPlanCreate", [&] {
+        need_ptr(plan, "plan");
+        *plan = nullptr;
+        need(!empty(nBatches, nx, ny, nz, nu, nv, nview), "empty geometry");
+        Grid g = make_grid(nx, ny, nz, dx, dy, dz, cx, cy, cz);
+        Det d = make_det(nu, nv, nview, da, dv, off_a, off_v);
+        need(nBatches <= kIntMax && nBatches * nz <= kIntMax, "batch too large");
+        need_ptr(deg, "angles");
+        if (!fan) { dsd = 1000.f; dso = 500.f; }
+        touch();
+        *plan = dd_plan_create(deg, (int)nBatches, g, d, (int)nview, dsd, dso, typeProjector, fan != 0, cur_stream());
+    });
+}
+
+CTP_EXPORT int ctpDDPlanForward(void* plan, float* prj, const float* img, void* stream) {
+    return guarded("ctpDDPlanForward", [&] {
+        need_ptr(plan, "plan"); need_ptr(prj, "prj"); need_ptr(img, "img");
+        dd_plan_forward(static_cast<DDPlan*>(plan), prj, img, static_cast<hipStream_t>(stream));
+    });
+}
+
+CTP_EXPORT int ctpDDPlanBackward(void* plan, float* img, const float* prj, void* stream) {
+    return guarded("ctpDDPlanBackward", [&] {
+        need_ptr(plan, "plan"); need_ptr(img, "img"); need_ptr(prj, "prj");
+        dd_plan_backward(static_cast<DDPlan*>(plan), img, prj, static_cast<hipStream_t>(stream));
+    });
+}
+
+CTP_EXPORT int ctpDDPlanAdjoint(void* plan, float* img, const float* prj, void* stream) {
+    return guarded("ctpDDPlanAdjoint", [&] {
+        need_ptr(plan, "plan"); need_ptr(img, "img"); need_ptr(prj, "prj");
+        dd_plan_adjoint(static_cast<DDPlan*>(plan), img, prj, static_cast<hipStream_t>(stream));
+    });
+}
+
+CTP_EXPORT int ctpDDPlanDestroy(void* plan) {
+    return guarded("ctpDDPlanDestroy", [&] {
+        if (!plan) return;
+        touch();
+        dd_plan_destroy(static_cast<DDPlan*>(plan));
+    });
+}
+
 // ===========================================================================
 // Siddon fan beam (cuda/siddonFan.cu:128-274), host arrays
 // ===========================================================================

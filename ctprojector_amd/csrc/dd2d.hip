@@ -27,6 +27,7 @@
 #include "ops.h"
 
 #include <algorithm>
+#include <memory>
 #include <cmath>
 #include <vector>
 
@@ -482,15 +483,26 @@ static int pick_ch(int C) {
     return ch;
 }
 
-void dd2d_fp(float* prj, const float* img, const float* deg_host, int B, const Grid& g, const Det& d, int nview,
-             float dsd, float dso, int type, bool fan, hipStream_t stream) {
+// Geometry-only launch data of the forward projector for one batch count (host
+// side: uploaded per call by dd2d_fp, once by a DDPlan).
+struct DDFPSetup {
+    int C = 0, CH = 1, nchunk = 1, cs = 0;
+    std::vector<FPRow> rows;
+    std::vector<int> start;
+    std::vector<FPGroup> groups;
+    bool use_lds = false, needY = false, needX = false, force = false;
+    int wmax = 0, kreg = 1;
+};
+
+static DDFPSetup dd_fp_setup(const std::vector<DDView>& views, int B, const Grid& g, const Det& d, int nview,
+                             float dsd, float dso, int type, bool fan) {
     const bool branchless = (type & DD_BRANCHLESS) != 0;
-    const int C = B * g.nz;
-    const int CH = pick_ch(C);
-    const int nchunk = ceil_div(C, CH);
-    const int cs = ((nchunk * CH + 1 + 3) / 4) * 4;
-    std::vector<DDView> views = make_dd_views(deg_host, nview, g);
-    if (branchless) hard_switch(views);
+    DDFPSetup s;
+    s.C = B * g.nz;
+    s.CH = pick_ch(s.C);
+    s.nchunk = ceil_div(s.C, s.CH);
+    s.cs = ((s.nchunk * s.CH + 1 + 3) / 4) * 4;
+    const int CH = s.CH, nchunk = s.nchunk;
     // rows of each chunk: detector row iv of batch b belongs to the chunk holding
     // channel b*nz + iz(iv); the z mapping follows cuda/distanceDriven2DBoxInt.cu:35-42,214,227
     std::vector<std::vector<FPRow>> per(nchunk);
@@ -512,22 +524,20 @@ void dd2d_fp(float* prj, const float* img, const float* deg_host, int B, const G
             r.w1 = (float)wz;
             per[k].push_back(r);
         }
-    std::vector<FPRow> rows;
-    std::vector<int> start(nchunk + 1, 0);
+    s.start.assign(nchunk + 1, 0);
     for (int k = 0; k < nchunk; k++) {
-        start[k] = (int)rows.size();
-        rows.insert(rows.end(), per[k].begin(), per[k].end());
+        s.start[k] = (int)s.rows.size();
+        s.rows.insert(s.rows.end(), per[k].begin(), per[k].end());
     }
-    start[nchunk] = (int)rows.size();
+    s.start[nchunk] = (int)s.rows.size();
 
     // LDS-staged kernel: groups of up to kFPViews consecutive views with equal
     // trace flags; staged width bounded in FP64 (+ margins for the +-1 pixel
     // guard and floor rounding)
-    std::vector<FPGroup> groups;
     for (int i = 0; i < nview;) {
         int n = 1;
         while (n < kFPViews && i + n < nview && (views[i + n].flags & 3) == (views[i].flags & 3)) n++;
-        groups.push_back(FPGroup{i, n});
+        s.groups.push_back(FPGroup{i, n});
         i += n;
     }
     const int CHP = ((CH + 1) + 3) / 4 * 4;
@@ -535,54 +545,52 @@ void dd2d_fp(float* prj, const float* img, const float* deg_host, int B, const G
     const int wcap = std::min(kreg_max * 64 * kFPViews / (CHP / 4), 65536 / (2 * CHP * 4));
     // the direct kernel is the default (measured faster at 512x512x32: the
     // staged kernel's prefetch registers spill)
-    const bool want_lds = tuning_get("dd_fp_mode", 1) == 2 && groups.size() <= 65535;
-    const double wneed = want_lds ? fp_stage_width(views, groups, g, d, dsd, dso, fan) + 6.0 : 1e300;
-    const bool use_lds = want_lds && wneed <= (double)wcap;
-    const int wmax = use_lds ? (int)std::ceil(wneed) : 0;
-    int kreg = 1;
-    while (use_lds && kreg * 64 * kFPViews < wmax * (CHP / 4)) kreg *= 2;
+    const bool want_lds = tuning_get("dd_fp_mode", 1) == 2 && s.groups.size() <= 65535;
+    const double wneed = want_lds ? fp_stage_width(views, s.groups, g, d, dsd, dso, fan) + 6.0 : 1e300;
+    s.use_lds = want_lds && wneed <= (double)wcap;
+    s.wmax = s.use_lds ? (int)std::ceil(wneed) : 0;
+    while (s.use_lds && s.kreg * 64 * kFPViews < s.wmax * (CHP / 4)) s.kreg *= 2;
+    for (auto& v : views) { s.needY |= (v.flags & 1) != 0; s.needX |= (v.flags & 2) != 0; }
+    // the branchless parallel FP has no force-FBP reverse weighting
+    // (cuda/distanceDriven2DBranchless.cu:321-360)
+    s.force = (type & DD_FORCE_FBP) != 0 && !branchless;
+    return s;
+}
 
-    const size_t npix = (size_t)g.nx * g.ny;
-    Workspace& ws = workspace_for(stream);
-    Plan pl;
-    pl.add<float>(npix * cs);
-    pl.add<float>(npix * cs);
-    pl.add<DDView>(nview);
-    pl.add<FPRow>(rows.size());
-    pl.add<int>(nchunk + 1);
-    pl.add<FPGroup>(groups.size());
-    ws.reserve(pl.bytes);
-    float* imgY = ws.take_n<float>(npix * cs);
-    float* imgX = ws.take_n<float>(npix * cs);
-    DDView* dviews = ws.take_n<DDView>(nview);
-    FPRow* drows = ws.take_n<FPRow>(rows.size());
-    int* dstart = ws.take_n<int>(nchunk + 1);
-    FPGroup* dgroups = ws.take_n<FPGroup>(groups.size());
-    CTP_CHECK(hipMemcpyAsync(dviews, views.data(), sizeof(DDView) * nview, hipMemcpyHostToDevice, stream));
-    CTP_CHECK(hipMemcpyAsync(drows, rows.data(), sizeof(FPRow) * rows.size(), hipMemcpyHostToDevice, stream));
-    CTP_CHECK(hipMemcpyAsync(dstart, start.data(), sizeof(int) * (nchunk + 1), hipMemcpyHostToDevice, stream));
-    CTP_CHECK(hipMemcpyAsync(dgroups, groups.data(), sizeof(FPGroup) * groups.size(), hipMemcpyHostToDevice, stream));
+struct DDFPDev {
+    const DDView* views;
+    const FPRow* rows;
+    const int* start;
+    const FPGroup* groups;
+};
 
-    bool needY = false, needX = false;
-    for (auto& v : views) { needY |= (v.flags & 1) != 0; needX |= (v.flags & 2) != 0; }
+// The forward projector's launches (no host work): imgY and imgX are scratch
+// of npix * cs floats each.
+static void dd_fp_run(const DDFPSetup& s, const DDFPDev& t, float* prj, const float* img, float* imgY, float* imgX,
+                      const Grid& g, const Det& d, int nview, float dsd, float dso, bool fan, hipStream_t stream) {
+    const int C = s.C, CH = s.CH, nchunk = s.nchunk, cs = s.cs;
+    const DDView* dviews = t.views;
+    const FPRow* drows = t.rows;
+    const int* dstart = t.start;
+    const FPGroup* dgroups = t.groups;
+    const bool force = s.force;
     dim3 tb(256);
-    if (needY) {
-        TimedLaunch t("dd_transpose", stream);
+    if (s.needY) {
+        TimedLaunch tt("dd_transpose", stream);
         to_channel_last<<<dim3(ceil_div(g.nx, 32), ceil_div(cs, 32), g.ny), tb, 0, stream>>>(imgY, img, C, cs, g.nx, g.ny, false);
         CTP_LAUNCH_CHECK();
     }
-    if (needX) {
-        TimedLaunch t("dd_transpose", stream);
+    if (s.needX) {
+        TimedLaunch tt("dd_transpose", stream);
         to_channel_last<<<dim3(ceil_div(g.nx, 32), ceil_div(cs, 32), g.ny), tb, 0, stream>>>(imgX, img, C, cs, g.nx, g.ny, true);
         CTP_LAUNCH_CHECK();
     }
-    // the branchless parallel FP has no force-FBP reverse weighting
-    // (cuda/distanceDriven2DBranchless.cu:321-360)
-    const bool force = (type & DD_FORCE_FBP) != 0 && !branchless;
-    if (use_lds) {
-        dim3 grid(ceil_div(d.nu, 64), (unsigned)groups.size(), nchunk), block(64 * kFPViews);
+    if (s.use_lds) {
+        const int wmax = s.wmax, kreg = s.kreg;
+        const int CHP = ((CH + 1) + 3) / 4 * 4;
+        dim3 grid(ceil_div(d.nu, 64), (unsigned)s.groups.size(), nchunk), block(64 * kFPViews);
         const size_t smem = 4 * std::max<size_t>((size_t)2 * wmax * CHP, (size_t)64 * kFPViews * (CH + 2));
-        TimedLaunch t(fan ? "dd_fan_fp" : "dd_par_fp", stream);
+        TimedLaunch tt(fan ? "dd_fan_fp" : "dd_par_fp", stream);
 #define CTP_FPL(N, K)                                                                                           \
     if (CH == N && kreg == K) {                                                                                \
         if (fan) dd_fp_lds_kernel<N, true, K><<<grid, block, smem, stream>>>(prj, imgY, imgX, dviews, dgroups,   \
@@ -597,9 +605,10 @@ void dd2d_fp(float* prj, const float* img, const float* deg_host, int B, const G
 #undef CTP_FPL_K
 #undef CTP_FPL
         CTP_LAUNCH_CHECK();
-    } else {
+        return;
+    }
     dim3 grid(ceil_div(d.nu, 128), nview, nchunk), block(128);
-    TimedLaunch t(fan ? "dd_fan_fp" : "dd_par_fp", stream);
+    TimedLaunch tt(fan ? "dd_fan_fp" : "dd_par_fp", stream);
 #define CTP_FP_CASE(N)                                                                                          \
     case N:                                                                                                     \
         if (fan) dd_fp_kernel<N, true><<<grid, block, 0, stream>>>(prj, imgY, imgX, dviews, drows, dstart, cs, g, d, nview, dsd, dso, force); \
@@ -611,7 +620,40 @@ void dd2d_fp(float* prj, const float* img, const float* deg_host, int B, const G
     }
 #undef CTP_FP_CASE
     CTP_LAUNCH_CHECK();
-    }
+}
+
+static std::vector<DDView> dd_views_for(const float* deg_host, int nview, const Grid& g, int type) {
+    std::vector<DDView> views = make_dd_views(deg_host, nview, g);
+    if ((type & DD_BRANCHLESS) != 0) hard_switch(views);
+    return views;
+}
+
+void dd2d_fp(float* prj, const float* img, const float* deg_host, int B, const Grid& g, const Det& d, int nview,
+             float dsd, float dso, int type, bool fan, hipStream_t stream) {
+    const std::vector<DDView> views = dd_views_for(deg_host, nview, g, type);
+    const DDFPSetup s = dd_fp_setup(views, B, g, d, nview, dsd, dso, type, fan);
+    const size_t npix = (size_t)g.nx * g.ny;
+    Workspace& ws = workspace_for(stream);
+    Plan pl;
+    pl.add<float>(npix * s.cs);
+    pl.add<float>(npix * s.cs);
+    pl.add<DDView>(nview);
+    pl.add<FPRow>(s.rows.size());
+    pl.add<int>(s.nchunk + 1);
+    pl.add<FPGroup>(s.groups.size());
+    ws.reserve(pl.bytes);
+    float* imgY = ws.take_n<float>(npix * s.cs);
+    float* imgX = ws.take_n<float>(npix * s.cs);
+    DDView* dviews = ws.take_n<DDView>(nview);
+    FPRow* drows = ws.take_n<FPRow>(s.rows.size());
+    int* dstart = ws.take_n<int>(s.nchunk + 1);
+    FPGroup* dgroups = ws.take_n<FPGroup>(s.groups.size());
+    CTP_CHECK(hipMemcpyAsync(dviews, views.data(), sizeof(DDView) * nview, hipMemcpyHostToDevice, stream));
+    CTP_CHECK(hipMemcpyAsync(drows, s.rows.data(), sizeof(FPRow) * s.rows.size(), hipMemcpyHostToDevice, stream));
+    CTP_CHECK(hipMemcpyAsync(dstart, s.start.data(), sizeof(int) * (s.nchunk + 1), hipMemcpyHostToDevice, stream));
+    CTP_CHECK(hipMemcpyAsync(dgroups, s.groups.data(), sizeof(FPGroup) * s.groups.size(), hipMemcpyHostToDevice,
+                             stream));
+    dd_fp_run(s, DDFPDev{dviews, drows, dstart, dgroups}, prj, img, imgY, imgX, g, d, nview, dsd, dso, fan, stream);
     CTP_CHECK(hipStreamSynchronize(stream));   // host tables above are released on return
 }
 
@@ -786,20 +828,23 @@ void reduce_parts(float* img, const float* part, int groups, size_t n, hipStream
     CTP_LAUNCH_CHECK();
 }
 
-void dd2d_bp(float* img, const float* prj, const float* deg_host, int B, const Grid& g, const Det& d, int nview,
-             float dsd, float dso, int type, bool fan, hipStream_t stream) {
-    (void)dsd;
+struct DDBPSetup {
+    int C = 0, CH = 1, nchunk = 1, cs = 0, groups = 1, vpg = 1;
+    bool fbp = false;
+    std::vector<SliceRow> srow;
+};
+
+static DDBPSetup dd_bp_setup(int B, const Grid& g, const Det& d, int nview, int type) {
     const bool branchless = (type & DD_BRANCHLESS) != 0;
+    DDBPSetup s;
     // the branchless BP ignores the force-FBP bit (cuda/distanceDriven2DBranchless.cu:547)
-    const bool fbp = branchless ? (type & DD_FBP) != 0 : (type & (DD_FBP | DD_FORCE_FBP)) != 0;
-    const int C = B * g.nz;
-    const int CH = pick_ch(C);
-    const int nchunk = ceil_div(C, CH);
-    const int cs = nchunk * CH;
-    std::vector<DDView> views = make_dd_views(deg_host, nview, g);
-    if (branchless) hard_switch(views);
+    s.fbp = branchless ? (type & DD_FBP) != 0 : (type & (DD_FBP | DD_FORCE_FBP)) != 0;
+    s.C = B * g.nz;
+    s.CH = pick_ch(s.C);
+    s.nchunk = ceil_div(s.C, s.CH);
+    s.cs = s.nchunk * s.CH;
     // detector rows of each slice, cuda/distanceDriven2DBoxInt.cu:340-350
-    std::vector<SliceRow> srow(g.nz);
+    s.srow.resize(g.nz);
     for (int z = 0; z < g.nz; z++) {
         const double zc = (z - g.nz / 2.0) * g.dz + g.cz + g.dz / 2.0;
         const double ivf = zc / d.dv + d.off_v + d.nv / 2.0 - 0.5;
@@ -813,31 +858,26 @@ void dd2d_bp(float* img, const float* prj, const float* deg_host, int B, const G
             const int r0 = clampi(iy, 0, d.nv), r1 = clampi(iy + 1, 0, d.nv);
             r = SliceRow{r0, r1, (float)(y - r0), 0};
         }
-        srow[z] = r;
+        s.srow[z] = r;
     }
-    const size_t npix = (size_t)g.nx * g.ny;
-    const int groups = bp_view_groups(npix, nchunk, nview);
-    const int vpg = ceil_div(nview, groups);
-    Workspace& ws = workspace_for(stream);
-    Plan pl;
-    pl.add<float>((size_t)nview * d.nu * cs);
-    pl.add<DDView>(nview);
-    pl.add<SliceRow>(g.nz);
-    if (groups > 1) pl.add<float>((size_t)groups * C * npix);
-    ws.reserve(pl.bytes);
-    float* sino = ws.take_n<float>((size_t)nview * d.nu * cs);
-    DDView* dviews = ws.take_n<DDView>(nview);
-    SliceRow* dsrow = ws.take_n<SliceRow>(g.nz);
-    float* part = groups > 1 ? ws.take_n<float>((size_t)groups * C * npix) : nullptr;
-    CTP_CHECK(hipMemcpyAsync(dviews, views.data(), sizeof(DDView) * nview, hipMemcpyHostToDevice, stream));
-    CTP_CHECK(hipMemcpyAsync(dsrow, srow.data(), sizeof(SliceRow) * g.nz, hipMemcpyHostToDevice, stream));
+    s.groups = bp_view_groups((size_t)g.nx * g.ny, s.nchunk, nview);
+    s.vpg = ceil_div(nview, s.groups);
+    return s;
+}
+
+// The backprojector's launches: sino is scratch of nview * nu * cs floats,
+// part of groups * C * npix floats when groups > 1 (else null).
+static void dd_bp_run(const DDBPSetup& s, const DDView* dviews, const SliceRow* dsrow, float* sino, float* part,
+                      float* img, const float* prj, const Grid& g, const Det& d, int nview, float dso, bool fan,
+                      hipStream_t stream) {
+    const int C = s.C, CH = s.CH, nchunk = s.nchunk, cs = s.cs, groups = s.groups, vpg = s.vpg;
+    const bool fbp = s.fbp;
     {
         TimedLaunch t("dd_bp_prep", stream);
         dd_bp_prep_kernel<<<dim3(ceil_div(d.nu, 32), ceil_div(cs, 32), nview), 256, 0, stream>>>(
             sino, prj, dviews, dsrow, C, cs, g.nz, d, nview, g.dx, g.dy, fan, !fbp);
         CTP_LAUNCH_CHECK();
     }
-    {
     TimedLaunch tb(fan ? "dd_fan_bp" : "dd_par_bp", stream);
     dim3 grid(ceil_div(g.nx, 16), ceil_div(g.ny, 16), nchunk * groups), block(256);
 #define CTP_BP_CASE(N)                                                                                         \
@@ -853,8 +893,29 @@ void dd2d_bp(float* img, const float* prj, const float* deg_host, int B, const G
     }
 #undef CTP_BP_CASE
     CTP_LAUNCH_CHECK();
-    if (part) reduce_parts(img, part, groups, (size_t)C * npix, stream);
-    }
+    if (part) reduce_parts(img, part, groups, (size_t)C * g.nx * g.ny, stream);
+}
+
+void dd2d_bp(float* img, const float* prj, const float* deg_host, int B, const Grid& g, const Det& d, int nview,
+             float dsd, float dso, int type, bool fan, hipStream_t stream) {
+    (void)dsd;
+    const std::vector<DDView> views = dd_views_for(deg_host, nview, g, type);
+    const DDBPSetup s = dd_bp_setup(B, g, d, nview, type);
+    const size_t npix = (size_t)g.nx * g.ny;
+    Workspace& ws = workspace_for(stream);
+    Plan pl;
+    pl.add<float>((size_t)nview * d.nu * s.cs);
+    pl.add<DDView>(nview);
+    pl.add<SliceRow>(g.nz);
+    if (s.groups > 1) pl.add<float>((size_t)s.groups * s.C * npix);
+    ws.reserve(pl.bytes);
+    float* sino = ws.take_n<float>((size_t)nview * d.nu * s.cs);
+    DDView* dviews = ws.take_n<DDView>(nview);
+    SliceRow* dsrow = ws.take_n<SliceRow>(g.nz);
+    float* part = s.groups > 1 ? ws.take_n<float>((size_t)s.groups * s.C * npix) : nullptr;
+    CTP_CHECK(hipMemcpyAsync(dviews, views.data(), sizeof(DDView) * nview, hipMemcpyHostToDevice, stream));
+    CTP_CHECK(hipMemcpyAsync(dsrow, s.srow.data(), sizeof(SliceRow) * g.nz, hipMemcpyHostToDevice, stream));
+    dd_bp_run(s, dviews, dsrow, sino, part, img, prj, g, d, nview, dso, fan, stream);
     CTP_CHECK(hipStreamSynchronize(stream));
 }
 
@@ -1003,19 +1064,24 @@ __global__ __launch_bounds__(256) void dd_fp_adjoint_kernel(float* __restrict__ 
     store_bp_result<CH>(img, part, acc, grp, c0, C, (size_t)g.nx * g.ny, (size_t)iy * g.nx + ix);
 }
 
-void dd2d_fp_adjoint(float* img, const float* prj, const float* deg_host, int B, const Grid& g, const Det& d,
-                     int nview, float dsd, float dso, int type, bool fan, hipStream_t stream) {
+struct DDAdjSetup {
+    int C = 0, CH = 1, nchunk = 1, cs = 0, groups = 1, vpg = 1;
+    bool rev = false;
+    std::vector<int> cstart;
+    std::vector<ChanRow> crow;
+};
+
+static DDAdjSetup dd_adj_setup(int B, const Grid& g, const Det& d, int nview, int type, bool fan) {
     const bool branchless = (type & DD_BRANCHLESS) != 0;
-    const int C = B * g.nz;
-    const int CH = pick_ch(C);
-    const int nchunk = ceil_div(C, CH);
-    const int cs = nchunk * CH;
-    std::vector<DDView> views = make_dd_views(deg_host, nview, g);
-    if (branchless) hard_switch(views);
-    const bool rev = !fan && (type & DD_FORCE_FBP) != 0 && !branchless;
+    DDAdjSetup s;
+    s.C = B * g.nz;
+    s.CH = pick_ch(s.C);
+    s.nchunk = ceil_div(s.C, s.CH);
+    s.cs = s.nchunk * s.CH;
+    s.rev = !fan && (type & DD_FORCE_FBP) != 0 && !branchless;
     // channel -> (detector row, weight) lists: the transpose of the forward
     // projector's row table (same z mapping as dd2d_fp)
-    std::vector<std::vector<ChanRow>> per(C);
+    std::vector<std::vector<ChanRow>> per(s.C);
     for (int b = 0; b < B; b++)
         for (int iv = 0; iv < d.nv; iv++) {
             const double zr = ((double)iv - (d.nv - 1) / 2.0 + d.off_v) * d.dv;
@@ -1028,41 +1094,32 @@ void dd2d_fp_adjoint(float* img, const float* prj, const float* deg_host, int B,
             per[b * g.nz + iz].push_back(ChanRow{iv, 1.f - w1});
             per[b * g.nz + iz1].push_back(ChanRow{iv, w1});
         }
-    std::vector<int> cstart(C + 1, 0);
-    std::vector<ChanRow> crow;
-    for (int c = 0; c < C; c++) {
-        cstart[c] = (int)crow.size();
-        crow.insert(crow.end(), per[c].begin(), per[c].end());
+    s.cstart.assign(s.C + 1, 0);
+    for (int c = 0; c < s.C; c++) {
+        s.cstart[c] = (int)s.crow.size();
+        s.crow.insert(s.crow.end(), per[c].begin(), per[c].end());
     }
-    cstart[C] = (int)crow.size();
-    const size_t npix = (size_t)g.nx * g.ny;
-    const int groups = bp_view_groups(npix, nchunk, nview);
-    const int vpg = ceil_div(nview, groups);
-    Workspace& ws = workspace_for(stream);
-    Plan pl;
-    pl.add<FPLine>((size_t)nview * d.nu);
-    pl.add<float>((size_t)nview * d.nu * cs);
-    pl.add<DDView>(nview);
-    pl.add<int>(C + 1);
-    pl.add<ChanRow>(std::max<size_t>(crow.size(), 1));
-    if (groups > 1) pl.add<float>((size_t)groups * C * npix);
-    ws.reserve(pl.bytes);
-    FPLine* dlines = ws.take_n<FPLine>((size_t)nview * d.nu);
-    float* G = ws.take_n<float>((size_t)nview * d.nu * cs);
-    DDView* dviews = ws.take_n<DDView>(nview);
-    int* dcstart = ws.take_n<int>(C + 1);
-    ChanRow* dcrow = ws.take_n<ChanRow>(std::max<size_t>(crow.size(), 1));
-    float* part = groups > 1 ? ws.take_n<float>((size_t)groups * C * npix) : nullptr;
-    CTP_CHECK(hipMemcpyAsync(dviews, views.data(), sizeof(DDView) * nview, hipMemcpyHostToDevice, stream));
-    CTP_CHECK(hipMemcpyAsync(dcstart, cstart.data(), sizeof(int) * (C + 1), hipMemcpyHostToDevice, stream));
-    if (!crow.empty())
-        CTP_CHECK(hipMemcpyAsync(dcrow, crow.data(), sizeof(ChanRow) * crow.size(), hipMemcpyHostToDevice, stream));
-    TimedLaunch t(fan ? "dd_fan_fp_adjoint" : "dd_par_fp_adjoint", stream);
+    s.cstart[s.C] = (int)s.crow.size();
+    s.groups = bp_view_groups((size_t)g.nx * g.ny, s.nchunk, nview);
+    s.vpg = ceil_div(nview, s.groups);
+    return s;
+}
+
+static void dd_adj_lines(FPLine* dlines, const DDView* dviews, const Grid& g, const Det& d, int nview, float dsd,
+                         float dso, bool fan, hipStream_t stream) {
     if (fan) dd_fp_lines_kernel<true><<<dim3(ceil_div(d.nu, 256), nview), 256, 0, stream>>>(dlines, dviews, g, d, nview, dsd, dso);
     else dd_fp_lines_kernel<false><<<dim3(ceil_div(d.nu, 256), nview), 256, 0, stream>>>(dlines, dviews, g, d, nview, dsd, dso);
     CTP_LAUNCH_CHECK();
+}
+
+// The adjoint's launches after the line table: G is scratch of
+// nview * nu * cs floats, part of groups * C * npix floats when groups > 1.
+static void dd_adj_run(const DDAdjSetup& s, const DDView* dviews, const FPLine* dlines, const int* dcstart,
+                       const ChanRow* dcrow, float* G, float* part, float* img, const float* prj, const Grid& g,
+                       const Det& d, int nview, float dso, bool fan, hipStream_t stream) {
+    const int C = s.C, CH = s.CH, nchunk = s.nchunk, cs = s.cs, groups = s.groups, vpg = s.vpg;
     dd_adj_rows_kernel<<<dim3(ceil_div(d.nu, 64), ceil_div(cs, 4), nview), 256, 0, stream>>>(
-        G, prj, dviews, dcstart, dcrow, C, cs, g.nz, d, nview, rev);
+        G, prj, dviews, dcstart, dcrow, C, cs, g.nz, d, nview, s.rev);
     CTP_LAUNCH_CHECK();
     dim3 grid(ceil_div(g.nx, 16), ceil_div(g.ny, 16), nchunk * groups), block(256);
 #define CTP_ADJ_CASE(N)                                                                                         \
@@ -1078,8 +1135,177 @@ void dd2d_fp_adjoint(float* img, const float* prj, const float* deg_host, int B,
     }
 #undef CTP_ADJ_CASE
     CTP_LAUNCH_CHECK();
-    if (part) reduce_parts(img, part, groups, (size_t)C * npix, stream);
+    if (part) reduce_parts(img, part, groups, (size_t)C * g.nx * g.ny, stream);
+}
+
+void dd2d_fp_adjoint(float* img, const float* prj, const float* deg_host, int B, const Grid& g, const Det& d,
+                     int nview, float dsd, float dso, int type, bool fan, hipStream_t stream) {
+    const std::vector<DDView> views = dd_views_for(deg_host, nview, g, type);
+    const DDAdjSetup s = dd_adj_setup(B, g, d, nview, type, fan);
+    const size_t npix = (size_t)g.nx * g.ny;
+    Workspace& ws = workspace_for(stream);
+    Plan pl;
+    pl.add<FPLine>((size_t)nview * d.nu);
+    pl.add<float>((size_t)nview * d.nu * s.cs);
+    pl.add<DDView>(nview);
+    pl.add<int>(s.C + 1);
+    pl.add<ChanRow>(std::max<size_t>(s.crow.size(), 1));
+    if (s.groups > 1) pl.add<float>((size_t)s.groups * s.C * npix);
+    ws.reserve(pl.bytes);
+    FPLine* dlines = ws.take_n<FPLine>((size_t)nview * d.nu);
+    float* G = ws.take_n<float>((size_t)nview * d.nu * s.cs);
+    DDView* dviews = ws.take_n<DDView>(nview);
+    int* dcstart = ws.take_n<int>(s.C + 1);
+    ChanRow* dcrow = ws.take_n<ChanRow>(std::max<size_t>(s.crow.size(), 1));
+    float* part = s.groups > 1 ? ws.take_n<float>((size_t)s.groups * s.C * npix) : nullptr;
+    CTP_CHECK(hipMemcpyAsync(dviews, views.data(), sizeof(DDView) * nview, hipMemcpyHostToDevice, stream));
+    CTP_CHECK(hipMemcpyAsync(dcstart, s.cstart.data(), sizeof(int) * (s.C + 1), hipMemcpyHostToDevice, stream));
+    if (!s.crow.empty())
+        CTP_CHECK(hipMemcpyAsync(dcrow, s.crow.data(), sizeof(ChanRow) * s.crow.size(), hipMemcpyHostToDevice, stream));
+    TimedLaunch t(fan ? "dd_fan_fp_adjoint" : "dd_par_fp_adjoint", stream);
+    dd_adj_lines(dlines, dviews, g, d, nview, dsd, dso, fan, stream);
+    dd_adj_run(s, dviews, dlines, dcstart, dcrow, G, part, img, prj, g, d, nview, dso, fan, stream);
     CTP_CHECK(hipStreamSynchronize(stream));
+}
+
+// ---------------------------------------------------------------------------
+// persistent plans (no counterpart in the reference)
+// ---------------------------------------------------------------------------
+// One geometry and batch count: every geometry-only table of the forward
+// projector, the backprojector and the exact adjoint is built once and kept on
+// the device, so that the calls launch kernels only (no host work, no
+// device-to-host angle copy, no synchronisation).  Scratch comes from the
+// calling stream's workspace.
+struct DDPlan {
+    int device = 0;
+    int B = 0, nview = 0, type = 0;
+    Grid g{};
+    Det d{};
+    float dsd = 0.f, dso = 0.f;
+    bool fan = true;
+    DDFPSetup fp;
+    DDBPSetup bp;
+    DDAdjSetup adj;
+    void* mem = nullptr;
+    DDView* views = nullptr;
+    FPRow* rows = nullptr;
+    int* start = nullptr;
+    FPGroup* groups = nullptr;
+    SliceRow* srow = nullptr;
+    int* cstart = nullptr;
+    ChanRow* crow = nullptr;
+    FPLine* lines = nullptr;
+    ~DDPlan() {
+        if (mem) (void)hipFree(mem);
+    }
+};
+
+DDPlan* dd_plan_create(const float* deg_host, int B, const Grid& g, const Det& d, int nview, float dsd, float dso,
+                       int type, bool fan, hipStream_t stream) {
+    std::unique_ptr<DDPlan> p(new DDPlan());
+    CTP_CHECK(hipGetDevice(&p->device));
+    p->B = B; p->nview = nview; p->type = type; p->g = g; p->d = d; p->dsd = dsd; p->dso = dso; p->fan = fan;
+    const std::vector<DDView> views = dd_views_for(deg_host, nview, g, type);
+    p->fp = dd_fp_setup(views, B, g, d, nview, dsd, dso, type, fan);
+    p->bp = dd_bp_setup(B, g, d, nview, type);
+    p->adj = dd_adj_setup(B, g, d, nview, type, fan);
+    Plan pl;
+    pl.add<DDView>(nview);
+    pl.add<FPRow>(std::max<size_t>(p->fp.rows.size(), 1));
+    pl.add<int>(p->fp.nchunk + 1);
+    pl.add<FPGroup>(std::max<size_t>(p->fp.groups.size(), 1));
+    pl.add<SliceRow>(g.nz);
+    pl.add<int>(p->adj.C + 1);
+    pl.add<ChanRow>(std::max<size_t>(p->adj.crow.size(), 1));
+    pl.add<FPLine>((size_t)nview * d.nu);
+    CTP_CHECK(hipMalloc(&p->mem, pl.bytes));
+    char* m = static_cast<char*>(p->mem);
+    auto carve = [&m](size_t bytes) { void* r = m; m += (bytes + 255) / 256 * 256; return r; };
+    p->views = static_cast<DDView*>(carve(sizeof(DDView) * nview));
+    p->rows = static_cast<FPRow*>(carve(sizeof(FPRow) * std::max<size_t>(p->fp.rows.size(), 1)));
+    p->start = static_cast<int*>(carve(sizeof(int) * (p->fp.nchunk + 1)));
+    p->groups = static_cast<FPGroup*>(carve(sizeof(FPGroup) * std::max<size_t>(p->fp.groups.size(), 1)));
+    p->srow = static_cast<SliceRow*>(carve(sizeof(SliceRow) * g.nz));
+    p->cstart = static_cast<int*>(carve(sizeof(int) * (p->adj.C + 1)));
+    p->crow = static_cast<ChanRow*>(carve(sizeof(ChanRow) * std::max<size_t>(p->adj.crow.size(), 1)));
+    p->lines = static_cast<FPLine*>(carve(sizeof(FPLine) * (size_t)nview * d.nu));
+    CTP_CHECK(hipMemcpyAsync(p->views, views.data(), sizeof(DDView) * nview, hipMemcpyHostToDevice, stream));
+    CTP_CHECK(hipMemcpyAsync(p->rows, p->fp.rows.data(), sizeof(FPRow) * p->fp.rows.size(), hipMemcpyHostToDevice,
+                             stream));
+    CTP_CHECK(hipMemcpyAsync(p->start, p->fp.start.data(), sizeof(int) * (p->fp.nchunk + 1), hipMemcpyHostToDevice,
+                             stream));
+    CTP_CHECK(hipMemcpyAsync(p->groups, p->fp.groups.data(), sizeof(FPGroup) * p->fp.groups.size(),
+                             hipMemcpyHostToDevice, stream));
+    CTP_CHECK(hipMemcpyAsync(p->srow, p->bp.srow.data(), sizeof(SliceRow) * g.nz, hipMemcpyHostToDevice, stream));
+    CTP_CHECK(hipMemcpyAsync(p->cstart, p->adj.cstart.data(), sizeof(int) * (p->adj.C + 1), hipMemcpyHostToDevice,
+                             stream));
+    if (!p->adj.crow.empty())
+        CTP_CHECK(hipMemcpyAsync(p->crow, p->adj.crow.data(), sizeof(ChanRow) * p->adj.crow.size(),
+                                 hipMemcpyHostToDevice, stream));
+    dd_adj_lines(p->lines, p->views, g, d, nview, dsd, dso, fan, stream);
+    CTP_CHECK(hipStreamSynchronize(stream));
+    return p.release();
+}
+
+static void dd_plan_check(const DDPlan* p) {
+    int dev = -1;
+    CTP_CHECK(hipGetDevice(&dev));
+    if (dev != p->device) fail("dd plan: called on another device than the one it was created on");
+}
+
+void dd_plan_forward(DDPlan* p, float* prj, const float* img, hipStream_t stream) {
+    dd_plan_check(p);
+    const size_t npix = (size_t)p->g.nx * p->g.ny;
+    Workspace& ws = workspace_for(stream);
+    Plan pl;
+    pl.add<float>(npix * p->fp.cs);
+    pl.add<float>(npix * p->fp.cs);
+    ws.reserve(pl.bytes);
+    float* imgY = ws.take_n<float>(npix * p->fp.cs);
+    float* imgX = ws.take_n<float>(npix * p->fp.cs);
+    dd_fp_run(p->fp, DDFPDev{p->views, p->rows, p->start, p->groups}, prj, img, imgY, imgX, p->g, p->d, p->nview,
+              p->dsd, p->dso, p->fan, stream);
+}
+
+void dd_plan_backward(DDPlan* p, float* img, const float* prj, hipStream_t stream) {
+    dd_plan_check(p);
+    const DDBPSetup& s = p->bp;
+    const size_t npix = (size_t)p->g.nx * p->g.ny;
+    Workspace& ws = workspace_for(stream);
+    Plan pl;
+    pl.add<float>((size_t)p->nview * p->d.nu * s.cs);
+    if (s.groups > 1) pl.add<float>((size_t)s.groups * s.C * npix);
+    ws.reserve(pl.bytes);
+    float* sino = ws.take_n<float>((size_t)p->nview * p->d.nu * s.cs);
+    float* part = s.groups > 1 ? ws.take_n<float>((size_t)s.groups * s.C * npix) : nullptr;
+    dd_bp_run(s, p->views, p->srow, sino, part, img, prj, p->g, p->d, p->nview, p->dso, p->fan, stream);
+}
+
+void dd_plan_adjoint(DDPlan* p, float* img, const float* prj, hipStream_t stream) {
+    dd_plan_check(p);
+    const DDAdjSetup& s = p->adj;
+    const size_t npix = (size_t)p->g.nx * p->g.ny;
+    Workspace& ws = workspace_for(stream);
+    Plan pl;
+    pl.add<float>((size_t)p->nview * p->d.nu * s.cs);
+    if (s.groups > 1) pl.add<float>((size_t)s.groups * s.C * npix);
+    ws.reserve(pl.bytes);
+    float* G = ws.take_n<float>((size_t)p->nview * p->d.nu * s.cs);
+    float* part = s.groups > 1 ? ws.take_n<float>((size_t)s.groups * s.C * npix) : nullptr;
+    TimedLaunch t(p->fan ? "dd_fan_fp_adjoint" : "dd_par_fp_adjoint", stream);
+    dd_adj_run(s, p->views, p->lines, p->cstart, p->crow, G, part, img, prj, p->g, p->d, p->nview, p->dso, p->fan,
+               stream);
+}
+
+void dd_plan_destroy(DDPlan* p) {
+    if (!p) return;
+    std::unique_ptr<DDPlan> owned(p);
+    int cur = 0;
+    CTP_CHECK(hipGetDevice(&cur));
+    CTP_CHECK(hipSetDevice(p->device));
+    CTP_CHECK(hipDeviceSynchronize());   // queued calls may still read the tables
+    owned.reset();
+    CTP_CHECK(hipSetDevice(cur));
 }
 
 }  // namespace ctp

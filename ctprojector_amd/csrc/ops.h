@@ -54,6 +54,16 @@ void dd2d_bp(float* img, const float* prj, const float* deg_host, int B, const G
 // img += A^T prj, the exact transpose of dd2d_fp (same type bits)
 void dd2d_fp_adjoint(float* img, const float* prj, const float* deg_host, int B, const Grid& g, const Det& d,
                      int nview, float dsd, float dso, int type, bool fan, hipStream_t stream);
+// Persistent plan of one DD geometry and batch count: tables built once, calls
+// launch kernels only on 'stream' (asynchronous, no host synchronisation).
+// forward overwrites prj; backward and adjoint accumulate into img.
+struct DDPlan;
+DDPlan* dd_plan_create(const float* deg_host, int B, const Grid& g, const Det& d, int nview, float dsd, float dso,
+                       int type, bool fan, hipStream_t stream);
+void dd_plan_forward(DDPlan* p, float* prj, const float* img, hipStream_t stream);
+void dd_plan_backward(DDPlan* p, float* img, const float* prj, hipStream_t stream);
+void dd_plan_adjoint(DDPlan* p, float* img, const float* prj, hipStream_t stream);
+void dd_plan_destroy(DDPlan* p);
 
 // ---- batched cone-beam plans (no counterpart in the reference) ----
 struct ConeJobSpec {

@@ -119,6 +119,25 @@ int ctpDistanceDrivenParallelProjectionAdjoint(float* img, const float* prj, con
                                                float cy, float cz, size_t nu, size_t nv, size_t nview, float da,
                                                float dv, float off_a, float off_v, int typeProjector);
 
+/* No counterpart in the reference (whose every call rebuilds its tables and
+ * synchronises, cuda/distanceDriven2D.cu:109-634): a persistent plan of one
+ * fan (fan = 1) or parallel (fan = 0; dsd, dso unused) geometry and batch
+ * count.  deg: host array of nview angles; the other arguments as in
+ * cupyDistanceDrivenFanProjection.  Forward writes prj = A img (as
+ * cupyDistanceDrivenFanProjection), Backward adds the reference's
+ * backprojection (as cupyDistanceDrivenFanBackprojection), Adjoint adds the
+ * exact transpose (as ctpDistanceDrivenFanProjectionAdjoint); device arrays,
+ * asynchronous on 'stream' (a hipStream_t; NULL = the null stream), no host
+ * synchronisation.  The plan must be used on the device current at creation;
+ * Destroy waits for that device. */
+int ctpDDPlanCreate(void** plan, int fan, const float* deg, size_t nBatches, size_t nx, size_t ny, size_t nz,
+                    float dx, float dy, float dz, float cx, float cy, float cz, size_t nu, size_t nv, size_t nview,
+                    float da, float dv, float off_a, float off_v, float dsd, float dso, int typeProjector);
+int ctpDDPlanForward(void* plan, float* prj, const float* img, void* stream);
+int ctpDDPlanBackward(void* plan, float* img, const float* prj, void* stream);
+int ctpDDPlanAdjoint(void* plan, float* img, const float* prj, void* stream);
+int ctpDDPlanDestroy(void* plan);
+
 /* ---- Siddon fan beam (equiangular arc, stacked slices; host arrays) ------- */
 /* cuda/siddonFan.cu:128  prj += A img; rows are horizontal planes z = (iv - (nv-1)/2 + off_v) dv */
 int cSiddonFanProjection(float* prj, const float* img, const float* deg, size_t nBatches, size_t nx, size_t ny,

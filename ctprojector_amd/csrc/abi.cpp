@@ -409,6 +409,7 @@ CTP_EXPORT int ctpDDPlanCreate(void** plan, int fan, const float* deg, DD_GRID_A
 CTP_EXPORT int ctpDDPlanForward(void* plan, float* prj, const float* img, void* stream) {
     return guarded("ctpDDPlanForward", [&] {
         need_ptr(plan, "plan"); need_ptr(prj, "prj"); need_ptr(img, "img");
+        touch();
         dd_plan_forward(static_cast<DDPlan*>(plan), prj, img, static_cast<hipStream_t>(stream));
     });
 }
@@ -416,6 +417,7 @@ CTP_EXPORT int ctpDDPlanForward(void* plan, float* prj, const float* img, void* 
 CTP_EXPORT int ctpDDPlanBackward(void* plan, float* img, const float* prj, void* stream) {
     return guarded("ctpDDPlanBackward", [&] {
         need_ptr(plan, "plan"); need_ptr(img, "img"); need_ptr(prj, "prj");
+        touch();
         dd_plan_backward(static_cast<DDPlan*>(plan), img, prj, static_cast<hipStream_t>(stream));
     });
 }
@@ -423,6 +425,7 @@ CTP_EXPORT int ctpDDPlanBackward(void* plan, float* img, const float* prj, void*
 CTP_EXPORT int ctpDDPlanAdjoint(void* plan, float* img, const float* prj, void* stream) {
     return guarded("ctpDDPlanAdjoint", [&] {
         need_ptr(plan, "plan"); need_ptr(img, "img"); need_ptr(prj, "prj");
+        touch();
         dd_plan_adjoint(static_cast<DDPlan*>(plan), img, prj, static_cast<hipStream_t>(stream));
     });
 }
@@ -774,6 +777,7 @@ CTP_EXPORT int ctpConePlanCreate(void** plan, const ctpConeJob* jobs, int njobs,
 CTP_EXPORT int ctpConePlanForward(void* plan, void* stream) {
     return guarded("ctpConePlanForward", [&] {
         need_ptr(plan, "plan");
+        touch();
         static_cast<ConePlan*>(plan)->forward(static_cast<hipStream_t>(stream));
     });
 }
@@ -781,6 +785,7 @@ CTP_EXPORT int ctpConePlanForward(void* plan, void* stream) {
 CTP_EXPORT int ctpConePlanBackward(void* plan, void* stream) {
     return guarded("ctpConePlanBackward", [&] {
         need_ptr(plan, "plan");
+        touch();
         static_cast<ConePlan*>(plan)->backward(static_cast<hipStream_t>(stream));
     });
 }

@@ -175,13 +175,25 @@ __device__ inline float fp_cell_edges(int iu, const DDView& V, const Grid& g, co
     return a;
 }
 
+// A block is 128 lanes: 128 >> lseg detector columns times 2^lseg segments of
+// the image rows (lseg 0 or 1: one segment per wavefront, so the row loop
+// stays scalar).  Each lane integrates its column's footprint over its
+// segment; the segments' accumulators are summed in LDS in a fixed order.
+// Splitting the rows doubles the waves of launches with few columns, views
+// and channel chunks (2D slices) so that they can hide the row gathers'
+// latency.
 template <int CH, bool FAN>
 __global__ __launch_bounds__(128) void dd_fp_kernel(float* __restrict__ prj, const float* __restrict__ imgY,
                                                     const float* __restrict__ imgX, const DDView* __restrict__ views,
                                                     const FPRow* __restrict__ rows, const int* __restrict__ row_start,
-                                                    int cs, Grid g, Det d, int nview, float dsd, float dso, bool force) {
+                                                    int cs, Grid g, Det d, int nview, float dsd, float dso, bool force,
+                                                    int lseg) {
     __shared__ float lds[128][CH + 2];
-    const int iu = blockIdx.x * 128 + threadIdx.x;
+    const int lcol = 7 - lseg;
+    const int col = threadIdx.x & ((1 << lcol) - 1);
+    const int seg = __builtin_amdgcn_readfirstlane(threadIdx.x >> lcol);   // uniform per wavefront
+    const int nseg = 1 << lseg;
+    const int iu = (blockIdx.x << lcol) + col;
     const int view = blockIdx.y, chunk = blockIdx.z;
     const int c0 = chunk * CH;
     const DDView V = views[view];
@@ -199,7 +211,8 @@ __global__ __launch_bounds__(128) void dd_fp_kernel(float* __restrict__ prj, con
             const float r1 = (d1.x - s1.x) / (d1.y - s1.y);
             const float r2 = (d2.x - s2.x) / (d2.y - s2.y);
             const float norm = V.wx * g.dy / fabsf(cosf(V.deg - a));
-            for (int iy = 0; iy < g.ny; iy++) {
+            const int ye = (int)(((long)g.ny * (seg + 1)) >> lseg);
+            for (int iy = (int)(((long)g.ny * seg) >> lseg); iy < ye; iy++) {
                 const float yy = (float)iy + 0.5f;
                 const float x1 = fmaf(r1, yy - s1.y, s1.x);
                 const float x2 = fmaf(r2, yy - s2.y, s2.x);
@@ -211,7 +224,8 @@ __global__ __launch_bounds__(128) void dd_fp_kernel(float* __restrict__ prj, con
             const float r1 = (d1.y - s1.y) / (d1.x - s1.x);
             const float r2 = (d2.y - s2.y) / (d2.x - s2.x);
             const float norm = (1.f - V.wx) * g.dx / fabsf(sinf(V.deg - a));
-            for (int ix = 0; ix < g.nx; ix++) {
+            const int xe = (int)(((long)g.nx * (seg + 1)) >> lseg);
+            for (int ix = (int)(((long)g.nx * seg) >> lseg); ix < xe; ix++) {
                 const float xx = (float)ix + 0.5f;
                 const float y1 = fmaf(r1, xx - s1.x, s1.y);
                 const float y2 = fmaf(r2, xx - s2.x, s2.y);
@@ -221,13 +235,24 @@ __global__ __launch_bounds__(128) void dd_fp_kernel(float* __restrict__ prj, con
     }
 #pragma unroll
     for (int j = 0; j <= CH; j++) lds[threadIdx.x][j] = acc[j];
-    // rows are read back by the owning lane only: no barrier needed
+    if (nseg > 1) {
+        __syncthreads();
+        if (seg != 0) return;
+        // segment 0 sums the other segments' accumulators into its own row
+#pragma unroll
+        for (int j = 0; j <= CH; j++) {
+            float v = acc[j];
+            for (int s = 1; s < nseg; s++) v += lds[(s << lcol) + col][j];
+            lds[col][j] = v;
+        }
+    }
+    // rows are read back by the owning lane only: no further barrier
     if (!active) return;
     const float rev = (!FAN && force) ? V.rev : 1.f;
     const int r0 = row_start[chunk], r1 = row_start[chunk + 1];
     for (int r = r0; r < r1; r++) {
         const FPRow R = rows[r];
-        const float val = (1.f - R.w1) * lds[threadIdx.x][R.j0] + R.w1 * lds[threadIdx.x][R.j1];
+        const float val = (1.f - R.w1) * lds[col][R.j0] + R.w1 * lds[col][R.j1];
         prj[(((size_t)R.b * nview + view) * d.nv + R.iv) * d.nu + iu] = val * rev;
     }
 }
@@ -492,6 +517,7 @@ struct DDFPSetup {
     std::vector<FPGroup> groups;
     bool use_lds = false, needY = false, needX = false, force = false;
     int wmax = 0, kreg = 1;
+    int lseg = 0;   // direct kernel: 2^lseg row segments per column
 };
 
 static DDFPSetup dd_fp_setup(const std::vector<DDView>& views, int B, const Grid& g, const Det& d, int nview,
@@ -554,6 +580,15 @@ static DDFPSetup dd_fp_setup(const std::vector<DDView>& views, int B, const Grid
     // the branchless parallel FP has no force-FBP reverse weighting
     // (cuda/distanceDriven2DBranchless.cu:321-360)
     s.force = (type & DD_FORCE_FBP) != 0 && !branchless;
+    // row segments: two when one gives the direct kernel fewer than 4 waves
+    // per SIMD (4096 on 1024 SIMDs) and each segment keeps at least 32 rows
+    // (measured at 8 slices of 256x256, 492 views, 444 columns: 1 segment
+    // 0.46 ms, 2 segments 0.25 ms; 4 segments, two per wavefront, 0.25 ms)
+    const int rows_min = std::min(g.nx, g.ny);
+    int lseg = (rows_min / 2 >= 32 && 2L * ceil_div(d.nu, 128) * nview * nchunk < 4096) ? 1 : 0;
+    const int forced = tuning_get("dd_fp_segments", 0);
+    if (forced == 1 || forced == 2) lseg = forced - 1;
+    s.lseg = lseg;
     return s;
 }
 
@@ -607,12 +642,13 @@ static void dd_fp_run(const DDFPSetup& s, const DDFPDev& t, float* prj, const fl
         CTP_LAUNCH_CHECK();
         return;
     }
-    dim3 grid(ceil_div(d.nu, 128), nview, nchunk), block(128);
+    const int lseg = s.lseg;
+    dim3 grid(ceil_div(d.nu, 128 >> lseg), nview, nchunk), block(128);
     TimedLaunch tt(fan ? "dd_fan_fp" : "dd_par_fp", stream);
 #define CTP_FP_CASE(N)                                                                                          \
     case N:                                                                                                     \
-        if (fan) dd_fp_kernel<N, true><<<grid, block, 0, stream>>>(prj, imgY, imgX, dviews, drows, dstart, cs, g, d, nview, dsd, dso, force); \
-        else dd_fp_kernel<N, false><<<grid, block, 0, stream>>>(prj, imgY, imgX, dviews, drows, dstart, cs, g, d, nview, dsd, dso, force); \
+        if (fan) dd_fp_kernel<N, true><<<grid, block, 0, stream>>>(prj, imgY, imgX, dviews, drows, dstart, cs, g, d, nview, dsd, dso, force, lseg); \
+        else dd_fp_kernel<N, false><<<grid, block, 0, stream>>>(prj, imgY, imgX, dviews, drows, dstart, cs, g, d, nview, dsd, dso, force, lseg); \
         break;
     switch (CH) {
         CTP_FP_CASE(1) CTP_FP_CASE(2) CTP_FP_CASE(4) CTP_FP_CASE(8) CTP_FP_CASE(16) CTP_FP_CASE(32)
@@ -801,13 +837,14 @@ __global__ __launch_bounds__(256) void dd_bp_kernel(float* __restrict__ img, flo
 // images of the groups are summed in a fixed order afterwards: deterministic.
 int bp_view_groups(size_t npix, int nchunk, int nview) {
     // The pixel-driven backprojections launch one 256-lane block per 16x16
-    // pixels and channel chunk.  Below about 1024 blocks (4 per CU) the
-    // launch cannot hide its gather latency, so the views are split into
-    // groups whose partial images are summed afterwards.  Measured on
-    // MI355X: at 512x512x32 (1024 blocks) the partial images cost more than
+    // pixels and channel chunk.  Below 1024 blocks (4 per CU) the launch
+    // cannot hide its gather latency, so the views are split into groups
+    // (enough for 2048 blocks) whose partial images are summed afterwards.
+    // Measured on MI355X: at 256x256x8 groups of 8 beat 4 and 16 for the
+    // adjoint; at 512x512x32 (1024 blocks) the partial images cost more than
     // extra waves gain, so such launches keep one group.
     const size_t blocks = std::max<size_t>(1, (npix + 255) / 256 * (size_t)std::max(nchunk, 1));
-    const int automatic = (int)std::min<size_t>(64, (1024 + blocks - 1) / blocks);
+    const int automatic = blocks >= 1024 ? 1 : (int)std::min<size_t>(64, (2048 + blocks - 1) / blocks);
     int k = tuning_get("bp_view_groups", automatic);
     k = std::max(1, std::min(k, std::max(1, nview / 8)));
     k = std::max(1, std::min(k, 65535 / std::max(nchunk, 1)));   // grid.z = nchunk * groups
@@ -1042,8 +1079,12 @@ __global__ __launch_bounds__(256) void dd_fp_adjoint_kernel(float* __restrict__ 
             // or [iy, iy+1] at x = ix + 0.5 (tr 1), in index units
             const float ua = tr == 0 ? adj_ucoord<FAN>(xa, yc, V.c, V.s, d, dso) : adj_ucoord<FAN>(xc, ya, V.c, V.s, d, dso);
             const float ub = tr == 0 ? adj_ucoord<FAN>(xb, yc, V.c, V.s, d, dso) : adj_ucoord<FAN>(xc, yb, V.c, V.s, d, dso);
-            const int k0 = max((int)floorf(fminf(ua, ub)) - 1, 0);
-            const int k1 = min((int)floorf(fmaxf(ua, ub)) + 1, d.nu - 1);
+            // candidate columns: those whose span [k, k+1] meets [ua, ub],
+            // widened by 1e-3 column for the difference between this FP32
+            // projection and the forward projector's FP32 edge lines (about
+            // 1e-5 column); each candidate's weight is then exact
+            const int k0 = max((int)floorf(fminf(ua, ub) - 1e-3f), 0);
+            const int k1 = min((int)floorf(fmaxf(ua, ub) + 1e-3f), d.nu - 1);
             const float t = tr == 0 ? (float)iy + 0.5f : (float)ix + 0.5f;
             const int n = tr == 0 ? g.nx : g.ny;
             const float lo = tr == 0 ? (float)ix : (float)iy;

@@ -32,6 +32,7 @@ std::map<std::string, int> g_tune = {
     {"cone_fp_mode", 0},     // Siddon FP: 0 column-driven form (rays of a detector column together), 1 ray per lane
     {"cone_bp_mode", 0},     // Siddon BP: 0 column mode where the geometry allows it, 1 general
     {"dd_fp_mode", 0},       // DD FP: 1 direct (default), 2 LDS-staged rows where the footprint fits
+    {"dd_fp_segments", 0},   // DD FP direct kernel: row segments per column (1, 2; default by wave count)
     {"bp_view_groups", 0},   // view groups of the pixel-driven DD / FBP backprojections (default: enough for 1024 blocks)
 };
 std::mutex g_ws_mutex;

@@ -26,12 +26,13 @@ def torch():
 
 def _case(kind, B):
     from ctprojector_amd.projector import cupy as C
+    from ctprojector_amd.projector.cupy import fan_equiangular, parallel
     if kind == 'fan':
         g, ang, img = small_fan(n=40, nz=3, nv=3, nview=44, nu=64, B=B, seed=21)
-        mod = C.fan_equiangular
+        mod = fan_equiangular
     else:
         g, ang, img = small_parallel(n=40, nz=2, nv=2, nview=36, nu=60, B=B, seed=22)
-        mod = C.parallel
+        mod = parallel
     return g.to(C.ct_projector()), ang, img, mod
 
 
@@ -61,15 +62,16 @@ def test_plan_equals_per_call_entry_points(torch, kind, branchless):
 def test_plan_single_slice_view_groups_and_repeats(torch):
     from ctprojector_amd.ddplan import DDPlan
     from ctprojector_amd.projector import cupy as C
+    from ctprojector_amd.projector.cupy import fan_equiangular as F
     g, ang, img = small_fan(n=96, nz=1, nv=1, nview=120, nu=140, du=2.0, seed=23)
     p = g.to(C.ct_projector())
     dev = torch.device('cuda', 0)
     a = torch.from_numpy(ang).to(dev)
     x = torch.from_numpy(img).to(dev)
     plan = DDPlan(p, a, batch=1)
-    ref_fp = C.fan_equiangular.distance_driven_fp(p, x, a)
-    ref_adj = C.fan_equiangular.distance_driven_fp_adjoint(p, ref_fp, a)
-    ref_bp = C.fan_equiangular.distance_driven_bp(p, ref_fp, a)
+    ref_fp = F.distance_driven_fp(p, x, a)
+    ref_adj = F.distance_driven_fp_adjoint(p, ref_fp, a)
+    ref_bp = F.distance_driven_bp(p, ref_fp, a)
     outs = []
     for _ in range(4):                       # queued without host synchronisation
         y = plan.fp(x)

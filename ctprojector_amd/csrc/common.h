@@ -43,4 +43,23 @@ __host__ __device__ inline float clampf(float x, float s, float e) { return x < 
 
 inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
 
+// Read-only data loaded through the constant address space: wave-uniform
+// loads become scalar memory instructions.  The data must not be written while
+// the kernel runs.
+template <typename T>
+__device__ inline T const_load(const T* p, long i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    static_assert(sizeof(T) % 4 == 0, "const_load: size must be a multiple of 4 bytes");
+    typedef const __attribute__((address_space(4))) unsigned* cptr;
+    const cptr q = (cptr)(p + i);
+    T out;
+    unsigned* o = reinterpret_cast<unsigned*>(&out);
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(T) / 4); k++) o[k] = q[k];
+    return out;
+#else
+    return p[i];
+#endif
+}
+
 }  // namespace ctp

@@ -31,6 +31,7 @@ std::map<std::string, int> g_tune = {
     {"cone_fp_order", 0},    // Siddon FP block order: 1 plain view-major, 2 XCD bands (default per form)
     {"cone_fp_mode", 0},     // Siddon FP: 0 column-driven form (rays of a detector column together), 1 ray per lane
     {"cone_bp_mode", 0},     // Siddon BP: 0 column mode where the geometry allows it, 1 general
+    {"cone_plan_persistent", 0},   // cone plans: 1 persistent blocks with work counters, 2 one block per tile
     {"dd_fp_mode", 0},       // DD FP: 1 direct (default), 2 LDS-staged rows where the footprint fits
     {"dd_fp_segments", 0},   // DD FP direct kernel: row segments per column (1, 2; default by wave count)
     {"bp_view_groups", 0},   // view groups of the pixel-driven DD / FBP backprojections (default: enough for 1024 blocks)

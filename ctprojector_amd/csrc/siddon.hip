@@ -1236,7 +1236,7 @@ __device__ inline void bp_col_block(float* __restrict__ img, long istride, const
     for (int k = 0; k < NZR * NB; k++) acc[k] = 0.f;
     const size_t nuv = (size_t)nu * nv;
     for (int view = 0; view < nview; view++) {
-        const ConeViewBP cv = views[view];
+        const ConeViewBP cv = const_load(views, view);   // scalar loads (read-only table)
         const float* wt[NB];
 #pragma unroll
         for (int b = 0; b < NB; b++) wt[b] = wtab.p[b] + view * nuv * WV;
@@ -1638,15 +1638,25 @@ __device__ inline int next_block(int* __restrict__ ctr, int lanes, int* slot) {
     return __builtin_amdgcn_readfirstlane(*slot);
 }
 
+// Block loop of the plan kernels: persistent (work counters above) or one
+// hardware-dispatched block per index (persist == 0).
+__device__ inline int first_block(int* __restrict__ ctr, int lanes, int* slot, int persist) {
+    return persist ? next_block(ctr, lanes, slot) : (int)blockIdx.x;
+}
+__device__ inline int following_block(int* __restrict__ ctr, int lanes, int* slot, int persist) {
+    return persist ? next_block(ctr, lanes, slot) : 0x7fffffff;
+}
+
 __global__ void reset_counters_kernel(int* __restrict__ ctr, int n) {
     if ((int)threadIdx.x < n) ctr[threadIdx.x] = 0;
 }
 
 template <int TU, int TV>
 __global__ __launch_bounds__(256) void siddon_cone_fp_batch_kernel(const ConeJobDev* __restrict__ jobs, int njobs,
-                                                                   long total, int xcd_order, int* __restrict__ ctr) {
+                                                                   long total, int xcd_order, int* __restrict__ ctr,
+                                                                   int persist) {
     __shared__ int slot;
-    for (int b = next_block(ctr, 8, &slot); b < total; b = next_block(ctr, 8, &slot)) {
+    for (int b = first_block(ctr, 8, &slot, persist); b < total; b = following_block(ctr, 8, &slot, persist)) {
         int j = 0;
         while (j + 1 < njobs && b >= jobs[j + 1].fp_begin) j++;
         const ConeJobDev* J = jobs + j;
@@ -1658,12 +1668,14 @@ __global__ __launch_bounds__(256) void siddon_cone_fp_batch_kernel(const ConeJob
     }
 }
 
-// column-driven form: a block is 4 adjacent detector columns of one view
-__global__ __launch_bounds__(256) void siddon_cone_fp_batch_col_kernel(const ConeJobDev* __restrict__ jobs, int njobs,
+// column-driven form: a block is 4 adjacent detector columns of one view.
+// Five waves per SIMD as the single-volume kernel gets (the job lookup would
+// otherwise push it to 100 VGPRs and four waves: 17% slower).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void siddon_cone_fp_batch_col_kernel(const ConeJobDev* __restrict__ jobs, int njobs,
                                                                        long total, int xcd_order,
-                                                                       int* __restrict__ ctr) {
+                                                                       int* __restrict__ ctr, int persist) {
     __shared__ int slot;
-    for (int b = next_block(ctr, 8, &slot); b < total; b = next_block(ctr, 8, &slot)) {
+    for (int b = first_block(ctr, 8, &slot, persist); b < total; b = following_block(ctr, 8, &slot, persist)) {
         int j = 0;
         while (j + 1 < njobs && b >= jobs[j + 1].fp_begin) j++;
         const ConeJobDev* J = jobs + j;
@@ -1678,11 +1690,13 @@ __global__ __launch_bounds__(256) void siddon_cone_fp_batch_col_kernel(const Con
     }
 }
 
+// Eight waves per SIMD for z-runs up to 4, as the single-volume kernel gets
+// (the job lookup would otherwise cost registers and occupancy).
 template <int WU, int WV, int NZR>
-__global__ __launch_bounds__(256) void siddon_cone_bp_batch_kernel(const ConeJobDev* __restrict__ jobs, int njobs,
-                                                                   long total, int* __restrict__ ctr) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NZR <= 4 ? 8 : 1))) void siddon_cone_bp_batch_kernel(const ConeJobDev* __restrict__ jobs, int njobs,
+                                                                   long total, int* __restrict__ ctr, int persist) {
     __shared__ int slot;
-    for (int b = next_block(ctr, 1, &slot); b < total; b = next_block(ctr, 1, &slot)) {
+    for (int b = first_block(ctr, 1, &slot, persist); b < total; b = following_block(ctr, 1, &slot, persist)) {
         int j = 0;
         while (j + 1 < njobs && b >= jobs[j + 1].bp_begin) j++;
         const ConeJobDev* J = jobs + j;
@@ -1691,8 +1705,9 @@ __global__ __launch_bounds__(256) void siddon_cone_bp_batch_kernel(const ConeJob
         const long rest = local / J->nbx;
         const int by = (int)(rest % J->nby), bz = (int)(rest / J->nby);
         const WTabs wt{{J->wtab, nullptr, nullptr, nullptr}};
+        // the group's kernel is compiled for its exact window (WU, WV) = (wu, wv)
         bp_col_block<WU, WV, NZR, 1>(J->img, 0, wt, J->utab, J->vtab, J->views, J->nview, J->g, J->d.nu, J->d.nv, bx,
-                                     by, bz, J->wu, J->wv);
+                                     by, bz);
     }
 }
 
@@ -1706,6 +1721,15 @@ static int persistent_grid(K kernel, long total) {
     long grid = (long)std::max(nb, 1) * prop.multiProcessorCount;
     grid = std::max(8L, grid / 8 * 8);                   // a multiple of 8: XCD of a block is fixed
     return (int)std::min(grid, (total + 7) / 8 * 8);
+}
+
+// 1: persistent blocks with work counters, 2: one hardware-dispatched block per
+// tile (cone_plan_persistent)
+static int plan_persist() { return tuning_get("cone_plan_persistent", 1) == 1 ? 1 : 0; }
+
+template <typename K>
+static int plan_grid(K kernel, long total) {
+    return plan_persist() ? persistent_grid(kernel, total) : (int)total;
 }
 
 struct ConePlan::Impl {
@@ -1803,14 +1827,14 @@ void ConePlan::Impl::forward(hipStream_t s) {
     TimedLaunch t("siddon_cone_fp_batch", s);
     if (tu == 0) {
         auto k = siddon_cone_fp_batch_col_kernel;
-        k<<<persistent_grid(k, fp_total), 256, 0, s>>>(djobs, nj, fp_total, order, ctr);
+        k<<<plan_grid(k, fp_total), 256, 0, s>>>(djobs, nj, fp_total, order, ctr, plan_persist());
         CTP_LAUNCH_CHECK();
         return;
     }
 #define CTP_FPB(A, Bv)                                                                                     \
     if (tu == A) {                                                                                        \
         auto k = siddon_cone_fp_batch_kernel<A, Bv>;                                                      \
-        k<<<persistent_grid(k, fp_total), 256, 0, s>>>(djobs, nj, fp_total, order, ctr);                  \
+        k<<<plan_grid(k, fp_total), 256, 0, s>>>(djobs, nj, fp_total, order, ctr, plan_persist());        \
     }
     CTP_FPB(8, 8) CTP_FPB(16, 4) CTP_FPB(32, 2) CTP_FPB(64, 1)
 #undef CTP_FPB
@@ -1841,7 +1865,7 @@ void ConePlan::Impl::backward(hipStream_t s) {
 #define CTP_BPB(A, Bv, Z)                                                                                  \
     if (G.wu == A && G.wv == Bv && nzr == Z) {                                                            \
         auto k = siddon_cone_bp_batch_kernel<A, Bv, Z>;                                                   \
-        k<<<persistent_grid(k, G.total), 256, 0, s>>>(G.djobs, G.njobs, G.total, gc);                     \
+        k<<<plan_grid(k, G.total), 256, 0, s>>>(G.djobs, G.njobs, G.total, gc, plan_persist());           \
     }
 #define CTP_BPB_Z(A, Bv) CTP_BPB(A, Bv, 2) CTP_BPB(A, Bv, 4) CTP_BPB(A, Bv, 8)
         CTP_BPB_Z(1, 1) CTP_BPB_Z(1, 2) CTP_BPB_Z(2, 1) CTP_BPB_Z(2, 2) CTP_BPB_Z(1, 3) CTP_BPB_Z(3, 1)

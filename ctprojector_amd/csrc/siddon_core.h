@@ -6,7 +6,11 @@
 
 namespace ctp {
 
-__device__ inline float3 ld3(const float* p, int i) { return make_float3(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
+// one geometry row (x, y, z) of a read-only [nview][3] table; wave-uniform
+// rows are read with scalar loads
+__device__ inline float3 ld3(const float* p, int i) {
+    return make_float3(const_load(p, 3L * i), const_load(p, 3L * i + 1), const_load(p, 3L * i + 2));
+}
 
 // Traversal state of one ray: the reference's InitializeSiddon
 // (cuda/siddon.cu:80-149) after MoveSourceDstNearGrid (cuda/siddon.cu:318-338),

@@ -81,10 +81,11 @@ static void hard_switch(std::vector<DDView>& v) {
 // ---------------------------------------------------------------------------
 // layout transforms
 // ---------------------------------------------------------------------------
-// planes P[c][y][x] (c < C)  ->  O[pix][cs] with pix = y*nx+x (xmajor=false) or
-// x*ny+y (xmajor=true); channels >= C are written as zero.
+// planes P[c][y][x] (c < C)  ->  O[major * rs + minor * cs + c] with (major,
+// minor) = (y, x) (xmajor=false) or (x, y) (xmajor=true); rs >= minor extent *
+// cs is the row stride; channels >= C are written as zero.
 __global__ __launch_bounds__(256) void to_channel_last(float* __restrict__ out, const float* __restrict__ in, int C,
-                                                       int cs, int nx, int ny, bool xmajor) {
+                                                       int cs, int nx, int ny, bool xmajor, size_t rs) {
     __shared__ float tile[32][33];
     const int x0 = blockIdx.x * 32, c0 = blockIdx.y * 32, y = blockIdx.z;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
@@ -96,8 +97,8 @@ __global__ __launch_bounds__(256) void to_channel_last(float* __restrict__ out, 
     for (int k = ty; k < 32; k += 8) {
         const int x = x0 + k, c = c0 + tx;
         if (x < nx && c < cs) {
-            const size_t pix = xmajor ? (size_t)x * ny + y : (size_t)y * nx + x;
-            out[pix * cs + c] = tile[tx][k];
+            const size_t at = xmajor ? (size_t)x * rs + (size_t)y * cs : (size_t)y * rs + (size_t)x * cs;
+            out[at + c] = tile[tx][k];
         }
     }
 }
@@ -115,7 +116,9 @@ struct FPRow {        // one output row (b, iv) of a channel chunk
 // Footprint of one line [x1, x2] (index units, unclamped) on a pixel row of
 // length n: the reference's IntegralBoxX (cuda/cudaMath.cu:533-565) divided by
 // (x2 - x1), applied to a channel vector at 'row' (stride cs between pixels).
-template <int CH>
+// XC: the channel after the chunk (the next chunk's first slice, for the z
+// interpolation across chunks) is accumulated too.
+template <int CH, bool XC>
 __device__ inline void dd_line(float* acc, const float* __restrict__ row, int cs, float x1, float x2, int n, float scale) {
     if (x1 > x2) { const float t = x1; x1 = x2; x2 = t; }
     const float a = clampf(x1, 0.f, (float)n), b = clampf(x2, 0.f, (float)n);
@@ -138,7 +141,7 @@ __device__ inline void dd_line(float* acc, const float* __restrict__ row, int cs
 #pragma unroll
             for (int j = 0; j < CH; j++) acc[j] = fmaf(w, p[j], acc[j]);
         }
-        acc[CH] = fmaf(w, p[CH], acc[CH]);
+        if constexpr (XC) acc[CH] = fmaf(w, p[CH], acc[CH]);
     }
 }
 
@@ -182,7 +185,7 @@ __device__ inline float fp_cell_edges(int iu, const DDView& V, const Grid& g, co
 // Splitting the rows doubles the waves of launches with few columns, views
 // and channel chunks (2D slices) so that they can hide the row gathers'
 // latency.
-template <int CH, bool FAN>
+template <int CH, bool FAN, bool XC>
 __global__ __launch_bounds__(128) void dd_fp_kernel(float* __restrict__ prj, const float* __restrict__ imgY,
                                                     const float* __restrict__ imgX, const DDView* __restrict__ views,
                                                     const FPRow* __restrict__ rows, const int* __restrict__ row_start,
@@ -211,12 +214,14 @@ __global__ __launch_bounds__(128) void dd_fp_kernel(float* __restrict__ prj, con
             const float r1 = (d1.x - s1.x) / (d1.y - s1.y);
             const float r2 = (d2.x - s2.x) / (d2.y - s2.y);
             const float norm = V.wx * g.dy / fabsf(cosf(V.deg - a));
+            // rows addressed from the int extents (a 64-bit row-stride argument
+            // instead measured 8% slower at 512x512x32)
             const int ye = (int)(((long)g.ny * (seg + 1)) >> lseg);
             for (int iy = (int)(((long)g.ny * seg) >> lseg); iy < ye; iy++) {
                 const float yy = (float)iy + 0.5f;
                 const float x1 = fmaf(r1, yy - s1.y, s1.x);
                 const float x2 = fmaf(r2, yy - s2.y, s2.x);
-                dd_line<CH>(acc, bY + (size_t)iy * g.nx * cs, cs, x1, x2, g.nx, norm);
+                dd_line<CH, XC>(acc, bY + (size_t)iy * g.nx * cs, cs, x1, x2, g.nx, norm);
             }
         }
         if (V.flags & 2) {
@@ -229,7 +234,7 @@ __global__ __launch_bounds__(128) void dd_fp_kernel(float* __restrict__ prj, con
                 const float xx = (float)ix + 0.5f;
                 const float y1 = fmaf(r1, xx - s1.x, s1.y);
                 const float y2 = fmaf(r2, xx - s2.x, s2.y);
-                dd_line<CH>(acc, bX + (size_t)ix * g.ny * cs, cs, y1, y2, g.ny, norm);
+                dd_line<CH, XC>(acc, bX + (size_t)ix * g.ny * cs, cs, y1, y2, g.ny, norm);
             }
         }
     }
@@ -298,15 +303,20 @@ __device__ inline void dd_line_lds(float* acc, const float* buf, int xlo, float 
     }
 }
 
-// One trace (rows along y for XT = false, along x for XT = true) of a block.
-template <int CH, int CHP, int VB, int KREG>
+// One trace (rows along y or along x) of a block.  The staged pixel range of
+// each row is copied from global memory straight into LDS by the texture path's
+// LDS DMA (global_load_lds_dwordx4: each wavefront instruction fills 1 KiB of
+// LDS, no registers in between), double buffered: the next row's copy is in
+// flight while the block integrates the current row from LDS.
+template <int CH, int CHP, int VB>
 __device__ inline void fp_trace_lds(float* acc, float* stage, const float (*edges)[4][3], int nviews_blk,
-                                    const float* __restrict__ src, int cs, int c0, int nrow, int nlen, int wmax,
+                                    const float* __restrict__ src, int cs, long rs, int c0, int nrow, int nlen, int wmax,
                                     bool active, float r1, float o1, float b1, float r2, float o2, float b2,
                                     float norm) {
     constexpr int Q = CHP / 4;             // float4 per pixel
-    const int nthr = 64 * VB;
-    // pixel range of row 'row' staged by the block
+    constexpr int nthr = 64 * VB;
+    const int wave = threadIdx.x >> 6;
+    // pixel range of row 'row' staged by the block (the same for every lane)
     auto range = [&](int row, int& lo, int& width) {
         const float t = (float)row + 0.5f;
         float mn = 3.0e38f, mx = -3.0e38f;
@@ -320,62 +330,54 @@ __device__ inline void fp_trace_lds(float* acc, float* stage, const float (*edge
         // clamp both ends into [-1, nlen + 1] (fminf/fmaxf also map NaN there)
         mn = fminf(fmaxf(mn, -1.f), (float)nlen + 1.f);
         mx = fmaxf(fminf(mx, (float)nlen + 1.f), -1.f);
-        lo = max(0, (int)floorf(mn) - 1);
+        lo = __builtin_amdgcn_readfirstlane(max(0, (int)floorf(mn) - 1));
         const int hi = min(nlen - 1, (int)floorf(mx) + 1);
-        width = min(max(hi - lo + 1, 0), wmax);
+        width = __builtin_amdgcn_readfirstlane(min(max(hi - lo + 1, 0), wmax));
     };
-    float4 reg[KREG];                       // in-flight row segment of this thread
-    int lo_next = 0, w_next = 0;
-    auto fetch = [&](int row) {
-        range(row, lo_next, w_next);
-        const float* base = src + ((size_t)row * nlen + lo_next) * cs + c0;
-        const int total = w_next * Q;
-#pragma unroll
-        for (int k = 0; k < KREG; k++) {
-            const int t = threadIdx.x + k * nthr;
+    // copy row 'row' (pixels [lo, lo + width), CHP floats each) into buf:
+    // item t (pixel t / Q, float4 t % Q) lands at buf[4 t]
+    auto fetch = [&](int row, float* buf, int& lo) {
+        int width;
+        range(row, lo, width);
+        const float* base = src + (size_t)row * rs + (size_t)lo * cs + c0;
+        const int total = width * Q;
+        for (int k0 = 0; k0 < total; k0 += nthr) {
+            const int t = k0 + threadIdx.x;
             if (t < total) {
                 const int px = t / Q, q = t - px * Q;
-                reg[k] = *reinterpret_cast<const float4*>(base + (size_t)px * cs + 4 * q);
+                __builtin_amdgcn_global_load_lds(
+                    (const void*)(base + (size_t)px * cs + 4 * q),
+                    (__attribute__((address_space(3))) void*)(buf + 4 * (k0 + 64 * wave)), 16, 0, 0);
             }
         }
     };
-    auto put = [&](float* buf) {
-        const int total = w_next * Q;
-#pragma unroll
-        for (int k = 0; k < KREG; k++) {
-            const int t = threadIdx.x + k * nthr;
-            if (t < total) reinterpret_cast<float4*>(buf)[t] = reg[k];
-        }
-    };
-    fetch(0);
-    put(stage);
-    int lo_cur = lo_next;
+    int lo_cur = 0, lo_next = 0;
+    fetch(0, stage, lo_cur);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     for (int row = 0; row < nrow; row++) {
         float* cur = stage + (row & 1) * wmax * CHP;
         float* nxt = stage + ((row + 1) & 1) * wmax * CHP;
-        const int lo_row = lo_cur;
-        if (row + 1 < nrow) fetch(row + 1);
+        if (row + 1 < nrow) fetch(row + 1, nxt, lo_next);
         if (active) {
             const float t = (float)row + 0.5f;
             const float x1 = fmaf(r1, t - o1, b1);
             const float x2 = fmaf(r2, t - o2, b2);
-            dd_line_lds<CH, CHP>(acc, cur, lo_row, x1, x2, nlen, norm);
+            dd_line_lds<CH, CHP>(acc, cur, lo_cur, x1, x2, nlen, norm);
         }
-        if (row + 1 < nrow) {
-            put(nxt);
-            lo_cur = lo_next;
-        }
+        lo_cur = lo_next;
+        // the next row has landed in LDS and every wave is done with this one
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
     }
 }
 
-template <int CH, bool FAN, int KREG>
+template <int CH, bool FAN>
 __global__ __launch_bounds__(64 * kFPViews) void dd_fp_lds_kernel(
         float* __restrict__ prj, const float* __restrict__ imgY, const float* __restrict__ imgX,
         const DDView* __restrict__ views, const FPGroup* __restrict__ groups, const FPRow* __restrict__ rows,
-        const int* __restrict__ row_start, int cs, int wmax, Grid g, Det d, int nview, float dsd, float dso,
-        bool force) {
+        const int* __restrict__ row_start, int cs, long rsY, long rsX, int wmax, Grid g, Det d, int nview, float dsd,
+        float dso, bool force) {
     constexpr int VB = kFPViews;
     constexpr int CHP = ((CH + 1) + 3) / 4 * 4;
     extern __shared__ float4 smem4[];
@@ -408,7 +410,7 @@ __global__ __launch_bounds__(64 * kFPViews) void dd_fp_lds_kernel(
         }
         __syncthreads();
         const float norm = V.wx * g.dy / fabsf(cosf(V.deg - a));
-        fp_trace_lds<CH, CHP, VB, KREG>(acc, stage, edges, G.count, imgY, cs, c0, g.ny, g.nx, wmax, active,
+        fp_trace_lds<CH, CHP, VB>(acc, stage, edges, G.count, imgY, cs, rsY, c0, g.ny, g.nx, wmax, active,
                                          r1, s1.y, s1.x, r2, s2.y, s2.x, norm);
     }
     if (flags & 2) {
@@ -423,7 +425,7 @@ __global__ __launch_bounds__(64 * kFPViews) void dd_fp_lds_kernel(
         }
         __syncthreads();
         const float norm = (1.f - V.wx) * g.dx / fabsf(sinf(V.deg - a));
-        fp_trace_lds<CH, CHP, VB, KREG>(acc, stage, edges, G.count, imgX, cs, c0, g.nx, g.ny, wmax, active,
+        fp_trace_lds<CH, CHP, VB>(acc, stage, edges, G.count, imgX, cs, rsX, c0, g.nx, g.ny, wmax, active,
                                         q1, s1.x, s1.y, q2, s2.x, s2.y, norm);
     }
     // z interpolation of the detector rows from the accumulators (each lane
@@ -479,6 +481,13 @@ static double fp_stage_width(const std::vector<DDView>& views, const std::vector
             for (int tr = 0; tr < 2; tr++) {
                 if (!(views[G.first].flags & (1 << tr))) continue;
                 const int nrow = tr == 0 ? g.ny : g.nx;
+                const double nlen = tr == 0 ? g.nx : g.ny;
+                // the outermost edge positions are a convex (max) and a concave
+                // (min) function of the row, so their extremes over the rows
+                // are at the two end rows; the kernel clamps both ends into
+                // [-1, nlen + 1], so the staged width is at most
+                // max over the ends of the clamped max - min of the clamped min
+                double hi = -1e300, lo = 1e300;
                 for (int end = 0; end < 2; end++) {
                     const double t = end == 0 ? 0.5 : nrow - 0.5;
                     double mn = 1e300, mx = -1e300;
@@ -493,10 +502,10 @@ static double fp_stage_width(const std::vector<DDView>& views, const std::vector
                                 mx = std::max(mx, x);
                             }
                         }
-                    // unclamped: the width between the outermost edges is convex in t,
-                    // so its maximum over the rows is at one of the two ends
-                    wmax = std::max(wmax, mx - mn);
+                    hi = std::max(hi, std::min(std::max(mx, -1.0), nlen + 1.0));
+                    lo = std::min(lo, std::min(std::max(mn, -1.0), nlen + 1.0));
                 }
+                wmax = std::max(wmax, hi - lo);
             }
         }
     return wmax;
@@ -516,8 +525,10 @@ struct DDFPSetup {
     std::vector<int> start;
     std::vector<FPGroup> groups;
     bool use_lds = false, needY = false, needX = false, force = false;
-    int wmax = 0, kreg = 1;
+    int wmax = 0;
     int lseg = 0;   // direct kernel: 2^lseg row segments per column
+    bool xc = true; // pixels carry the channel after the chunk
+    long rsY = 0, rsX = 0;   // row strides (floats) of the y-major and x-major copies (n * cs)
 };
 
 static DDFPSetup dd_fp_setup(const std::vector<DDView>& views, int B, const Grid& g, const Det& d, int nview,
@@ -527,7 +538,6 @@ static DDFPSetup dd_fp_setup(const std::vector<DDView>& views, int B, const Grid
     s.C = B * g.nz;
     s.CH = pick_ch(s.C);
     s.nchunk = ceil_div(s.C, s.CH);
-    s.cs = ((s.nchunk * s.CH + 1 + 3) / 4) * 4;
     const int CH = s.CH, nchunk = s.nchunk;
     // rows of each chunk: detector row iv of batch b belongs to the chunk holding
     // channel b*nz + iz(iv); the z mapping follows cuda/distanceDriven2DBoxInt.cu:35-42,214,227
@@ -567,15 +577,25 @@ static DDFPSetup dd_fp_setup(const std::vector<DDView>& views, int B, const Grid
         i += n;
     }
     const int CHP = ((CH + 1) + 3) / 4 * 4;
-    const int kreg_max = 8;
-    const int wcap = std::min(kreg_max * 64 * kFPViews / (CHP / 4), 65536 / (2 * CHP * 4));
+    const int wcap = 65536 / (2 * CHP * 4);   // two row buffers in 64 KiB of LDS
     // the direct kernel is the default (measured faster at 512x512x32: the
     // staged kernel's prefetch registers spill)
     const bool want_lds = tuning_get("dd_fp_mode", 1) == 2 && s.groups.size() <= 65535;
     const double wneed = want_lds ? fp_stage_width(views, s.groups, g, d, dsd, dso, fan) + 6.0 : 1e300;
     s.use_lds = want_lds && wneed <= (double)wcap;
     s.wmax = s.use_lds ? (int)std::ceil(wneed) : 0;
-    while (s.use_lds && s.kreg * 64 * kFPViews < s.wmax * (CHP / 4)) s.kreg *= 2;
+    // channel-last pixel stride: with one chunk the direct kernel needs no
+    // channel after the chunk (rows interpolate between slices of the chunk),
+    // so a pixel is CH floats (one 128-byte line at CH = 32) instead of CH + 4
+    s.xc = s.use_lds || nchunk > 1 || tuning_get("dd_fp_extra_channel", 0) == 1;
+    // measured on MI355X (8 to 32 slices of 256x256 or 512x512): one float4
+    // of padding per pixel from 16 channels on (a 128-byte pixel stride puts
+    // every lane's loads on the same offsets within cache lines)
+    s.cs = s.xc ? ((nchunk * CH + 1 + 3) / 4) * 4 : (CH >= 16 ? CH + 4 : CH);
+    const int forced_cs = tuning_get("dd_fp_pixel_stride", 0);
+    if (!s.xc && forced_cs >= CH && forced_cs % (CH >= 4 ? 4 : 1) == 0) s.cs = forced_cs;
+    s.rsY = (long)g.nx * s.cs;
+    s.rsX = (long)g.ny * s.cs;
     for (auto& v : views) { s.needY |= (v.flags & 1) != 0; s.needX |= (v.flags & 2) != 0; }
     // the branchless parallel FP has no force-FBP reverse weighting
     // (cuda/distanceDriven2DBranchless.cu:321-360)
@@ -612,32 +632,30 @@ static void dd_fp_run(const DDFPSetup& s, const DDFPDev& t, float* prj, const fl
     dim3 tb(256);
     if (s.needY) {
         TimedLaunch tt("dd_transpose", stream);
-        to_channel_last<<<dim3(ceil_div(g.nx, 32), ceil_div(cs, 32), g.ny), tb, 0, stream>>>(imgY, img, C, cs, g.nx, g.ny, false);
+        to_channel_last<<<dim3(ceil_div(g.nx, 32), ceil_div(cs, 32), g.ny), tb, 0, stream>>>(imgY, img, C, cs, g.nx, g.ny, false, (size_t)s.rsY);
         CTP_LAUNCH_CHECK();
     }
     if (s.needX) {
         TimedLaunch tt("dd_transpose", stream);
-        to_channel_last<<<dim3(ceil_div(g.nx, 32), ceil_div(cs, 32), g.ny), tb, 0, stream>>>(imgX, img, C, cs, g.nx, g.ny, true);
+        to_channel_last<<<dim3(ceil_div(g.nx, 32), ceil_div(cs, 32), g.ny), tb, 0, stream>>>(imgX, img, C, cs, g.nx, g.ny, true, (size_t)s.rsX);
         CTP_LAUNCH_CHECK();
     }
     if (s.use_lds) {
-        const int wmax = s.wmax, kreg = s.kreg;
+        const int wmax = s.wmax;
         const int CHP = ((CH + 1) + 3) / 4 * 4;
         dim3 grid(ceil_div(d.nu, 64), (unsigned)s.groups.size(), nchunk), block(64 * kFPViews);
         const size_t smem = 4 * std::max<size_t>((size_t)2 * wmax * CHP, (size_t)64 * kFPViews * (CH + 2));
         TimedLaunch tt(fan ? "dd_fan_fp" : "dd_par_fp", stream);
-#define CTP_FPL(N, K)                                                                                           \
-    if (CH == N && kreg == K) {                                                                                \
-        if (fan) dd_fp_lds_kernel<N, true, K><<<grid, block, smem, stream>>>(prj, imgY, imgX, dviews, dgroups,   \
-                                                                       drows, dstart, cs, wmax, g, d, nview, dsd, \
-                                                                       dso, force);                              \
-        else dd_fp_lds_kernel<N, false, K><<<grid, block, smem, stream>>>(prj, imgY, imgX, dviews, dgroups,      \
-                                                                        drows, dstart, cs, wmax, g, d, nview, dsd, \
-                                                                        dso, force);                             \
+#define CTP_FPL(N)                                                                                              \
+    if (CH == N) {                                                                                             \
+        if (fan) dd_fp_lds_kernel<N, true><<<grid, block, smem, stream>>>(prj, imgY, imgX, dviews, dgroups,      \
+                                                                    drows, dstart, cs, s.rsY, s.rsX, wmax, g, d, nview, dsd, \
+                                                                    dso, force);                                 \
+        else dd_fp_lds_kernel<N, false><<<grid, block, smem, stream>>>(prj, imgY, imgX, dviews, dgroups,         \
+                                                                     drows, dstart, cs, s.rsY, s.rsX, wmax, g, d, nview, dsd, \
+                                                                     dso, force);                                \
     }
-#define CTP_FPL_K(N) CTP_FPL(N, 1) CTP_FPL(N, 2) CTP_FPL(N, 4) CTP_FPL(N, 8)
-        CTP_FPL_K(1) CTP_FPL_K(2) CTP_FPL_K(4) CTP_FPL_K(8) CTP_FPL_K(16) CTP_FPL_K(32)
-#undef CTP_FPL_K
+        CTP_FPL(1) CTP_FPL(2) CTP_FPL(4) CTP_FPL(8) CTP_FPL(16) CTP_FPL(32)
 #undef CTP_FPL
         CTP_LAUNCH_CHECK();
         return;
@@ -645,16 +663,19 @@ static void dd_fp_run(const DDFPSetup& s, const DDFPDev& t, float* prj, const fl
     const int lseg = s.lseg;
     dim3 grid(ceil_div(d.nu, 128 >> lseg), nview, nchunk), block(128);
     TimedLaunch tt(fan ? "dd_fan_fp" : "dd_par_fp", stream);
+#define CTP_FP_LAUNCH(N, F, X) \
+    dd_fp_kernel<N, F, X><<<grid, block, 0, stream>>>(prj, imgY, imgX, dviews, drows, dstart, cs, g, d, nview, dsd, dso, force, lseg)
 #define CTP_FP_CASE(N)                                                                                          \
     case N:                                                                                                     \
-        if (fan) dd_fp_kernel<N, true><<<grid, block, 0, stream>>>(prj, imgY, imgX, dviews, drows, dstart, cs, g, d, nview, dsd, dso, force, lseg); \
-        else dd_fp_kernel<N, false><<<grid, block, 0, stream>>>(prj, imgY, imgX, dviews, drows, dstart, cs, g, d, nview, dsd, dso, force, lseg); \
+        if (fan) { if (s.xc) CTP_FP_LAUNCH(N, true, true); else CTP_FP_LAUNCH(N, true, false); }               \
+        else { if (s.xc) CTP_FP_LAUNCH(N, false, true); else CTP_FP_LAUNCH(N, false, false); }                 \
         break;
     switch (CH) {
         CTP_FP_CASE(1) CTP_FP_CASE(2) CTP_FP_CASE(4) CTP_FP_CASE(8) CTP_FP_CASE(16) CTP_FP_CASE(32)
         default: fail("dd fp: bad channel chunk");
     }
 #undef CTP_FP_CASE
+#undef CTP_FP_LAUNCH
     CTP_LAUNCH_CHECK();
 }
 
@@ -671,15 +692,15 @@ void dd2d_fp(float* prj, const float* img, const float* deg_host, int B, const G
     const size_t npix = (size_t)g.nx * g.ny;
     Workspace& ws = workspace_for(stream);
     Plan pl;
-    pl.add<float>(npix * s.cs);
-    pl.add<float>(npix * s.cs);
+    pl.add<float>((size_t)g.ny * s.rsY);
+    pl.add<float>((size_t)g.nx * s.rsX);
     pl.add<DDView>(nview);
     pl.add<FPRow>(s.rows.size());
     pl.add<int>(s.nchunk + 1);
     pl.add<FPGroup>(s.groups.size());
     ws.reserve(pl.bytes);
-    float* imgY = ws.take_n<float>(npix * s.cs);
-    float* imgX = ws.take_n<float>(npix * s.cs);
+    float* imgY = ws.take_n<float>((size_t)g.ny * s.rsY);
+    float* imgX = ws.take_n<float>((size_t)g.nx * s.rsX);
     DDView* dviews = ws.take_n<DDView>(nview);
     FPRow* drows = ws.take_n<FPRow>(s.rows.size());
     int* dstart = ws.take_n<int>(s.nchunk + 1);
@@ -1299,11 +1320,11 @@ void dd_plan_forward(DDPlan* p, float* prj, const float* img, hipStream_t stream
     const size_t npix = (size_t)p->g.nx * p->g.ny;
     Workspace& ws = workspace_for(stream);
     Plan pl;
-    pl.add<float>(npix * p->fp.cs);
-    pl.add<float>(npix * p->fp.cs);
+    pl.add<float>((size_t)p->g.ny * p->fp.rsY);
+    pl.add<float>((size_t)p->g.nx * p->fp.rsX);
     ws.reserve(pl.bytes);
-    float* imgY = ws.take_n<float>(npix * p->fp.cs);
-    float* imgX = ws.take_n<float>(npix * p->fp.cs);
+    float* imgY = ws.take_n<float>((size_t)p->g.ny * p->fp.rsY);
+    float* imgX = ws.take_n<float>((size_t)p->g.nx * p->fp.rsX);
     dd_fp_run(p->fp, DDFPDev{p->views, p->rows, p->start, p->groups}, prj, img, imgY, imgX, p->g, p->d, p->nview,
               p->dsd, p->dso, p->fan, stream);
 }

@@ -340,6 +340,25 @@ def test_dd_fp_staged_rows_match_direct(oracle, tuning, fan_beam):
     assert rel_l2(fp, ref) < TOL
 
 
+@pytest.mark.parametrize('case', ['fan_3', 'fan_2x20', 'parallel_2', 'fan_1'])
+def test_dd_fp_staged_rows_equal_direct(tuning, case):
+    """The LDS-staged forward projector performs the direct kernel's arithmetic
+    in the same order (rows copied to LDS by DMA): equal results, including
+    several channel chunks (2 volumes x 20 slices = 40 channels)."""
+    if case.startswith('fan'):
+        nz, B = {'fan_3': (3, 1), 'fan_2x20': (20, 2), 'fan_1': (1, 1)}[case]
+        g, ang, img = small_fan(n=56, nz=nz, nv=nz, nview=48, nu=80, off=(0.4, 0.3), B=B)
+        run = lambda: fan.distance_driven_fp(_proj(g), img, ang)
+    else:
+        g, ang, img = small_parallel(nz=2, nv=2, B=2)
+        run = lambda: parallel.distance_driven_fp(_proj(g), img, ang)
+    tuning('dd_fp_mode', 1)
+    a = run()
+    tuning('dd_fp_mode', 2)
+    b = run()
+    assert np.array_equal(a, b), rel_l2(a, b)
+
+
 # ------------------------------------------------------------ Siddon fan beam
 @pytest.mark.parametrize('variant', ['plain', 'offsets_batch'])
 def test_siddon_fan(oracle, variant):

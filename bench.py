@@ -476,15 +476,17 @@ def calibrate_copy():
 
 
 def short_name(kname):
-    n = kname.split('(')[0].replace('void ', '').replace('ctp::', '')
-    return n.split('<')[0]
+    """Kernel symbol without namespace and parameter list, template arguments
+    kept (instances are reported separately)."""
+    return kname.split('(')[0].replace('void ', '').replace('ctp::', '').strip()
 
 
 # timing names (HIP-event scopes in the library) -> kernel symbols in rocprofv3 output
-TIMED_KERNELS = {'dd_fan_fp': 'dd_fp_kernel', 'dd_par_fp': 'dd_fp_kernel', 'dd_fan_bp': 'dd_bp_kernel',
-                 'dd_par_bp': 'dd_bp_kernel', 'siddon_cone_fp': 'siddon_cone_fp_col_kernel',
-                 'siddon_cone_bp': 'siddon_cone_bp_col_kernel',
-                 'fan_fbp_bp': 'fan_fbp_bp_kernel', 'ramp_filter': 'ramp_conv_kernel'}
+# (the instances the benchmark workloads launch)
+TIMED_KERNELS = {'dd_fan_fp': 'dd_fp_kernel<32, true, false>', 'dd_fan_bp': 'dd_bp_kernel<32, true>',
+                 'siddon_cone_fp': 'siddon_cone_fp_col_kernel<1>',
+                 'siddon_cone_bp': 'siddon_cone_bp_col_kernel<2, 2, 4, 1>',
+                 'fan_fbp_bp': 'fan_fbp_bp_kernel<32, true>', 'ramp_filter': 'ramp_conv_kernel<4>'}
 
 
 def read_traffic(name):
@@ -494,7 +496,7 @@ def read_traffic(name):
     try:
         with open(TRAFFIC_FILE) as f:
             for line in f:
-                cells = [c.strip() for c in line.strip().strip('|').split('|')]
+                cells = [c.strip().strip('`') for c in line.strip().strip('|').split('|')]
                 if len(cells) >= 4 and cells[0] == kernel:
                     return int(float(cells[3]) * (1 << 20))
     except OSError:
@@ -516,7 +518,7 @@ def summarize(trace_dir, fetch_dir=None, write_dir=None, hits_dir=None):
     out.append('|---|---:|---:|---:|---:|')
     for name, calls, tot, avg, pct in rows:
         # top_kernels durations are in microseconds
-        out.append('| %s | %d | %.3f | %.1f | %.2f |' % (short_name(name), calls, tot / 1e3, avg, pct))
+        out.append('| `%s` | %d | %.3f | %.1f | %.2f |' % (short_name(name), calls, tot / 1e3, avg, pct))
     if fetch_dir and write_dir:
         def counters(d, cname):
             res = {}
@@ -547,7 +549,7 @@ def summarize(trace_dir, fetch_dir=None, write_dir=None, hits_dir=None):
             w = (sum(write.get(k, [0])) / max(len(write.get(k, [])), 1)) * sw / 1024
             h, m = sum(hits.get(k, [0])), sum(miss.get(k, [0]))
             hr = '%.1f' % (100.0 * h / (h + m)) if h + m else '-'
-            out.append('| %s | %.1f | %.1f | %.1f | %s | %d |' % (k, f, w, f + w, hr, len(fetch[k])))
+            out.append('| `%s` | %.1f | %.1f | %.1f | %s | %d |' % (k, f, w, f + w, hr, len(fetch[k])))
     print('\n'.join(out))
     return 0
 

@@ -49,6 +49,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
+PEAK_FP32_TFLOPS = 157.3      # MI355X published FP32 (the vector FMA rate with packed instructions)
 COPY_BYTES = 256 << 20
 TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'hbm_traffic_fan512x32_cone256.md')
 
@@ -430,8 +431,14 @@ def fbp_extra(steps, w, img):
         'workload': 'fan ramp_filter (Hann) + fbp_bp, same volume/scan',
         'ms_per_step': round((t1 - t0) / steps * 1e3, 3),
         'kernel_ms': {'ramp_filter': round(rf_ms, 3), 'fan_fbp_bp': round(bp_ms, 3)},
+        # the ramp filter is a direct linear convolution, 2 nu^2 flops per
+        # detector row: compute-bound (8 B per sample of HBM traffic)
+        'ramp_filter_tflop_s': round(2.0 * nsamp * w.nu / (rf_ms / 1e3) / 1e12, 2) if rf_ms else None,
+        'ramp_filter_fp32_frac': round(2.0 * nsamp * w.nu / (rf_ms / 1e3) / 1e12 / PEAK_FP32_TFLOPS, 4)
+        if rf_ms else None,
         'ramp_filter_hbm_frac': round(8 * nsamp / (rf_ms / 1e3) / 1e9 / PEAK_HBM_GBS, 4) if rf_ms else None,
         'fbp_bp_gvu_s': round(nvu / (bp_ms / 1e3) / 1e9, 2) if bp_ms else None,
+        'fbp_bp_hbm_frac': round(4 * nvu / (bp_ms / 1e3) / 1e9 / PEAK_HBM_GBS, 4) if bp_ms else None,
     }
 
 

@@ -28,7 +28,8 @@ std::mutex g_tune_mutex;
 std::map<std::string, int> g_tune = {
     {"cone_fp_tile_u", 0},   // detector columns per wavefront tile of the Siddon FP (8, 16, 32, 64)
     {"cone_bp_zrun", 0},     // voxels along z per thread of the Siddon BP (1, 2, 4, 8; default 4)
-    {"cone_fp_order", 0},    // Siddon FP block order: 1 plain view-major, 2 XCD bands (default per form)
+    {"cone_fp_order", 0},
+    {"cone_fp_views_per_block", 0},   // column-driven Siddon FP: views per block (1, 2, 4; default 2)    // Siddon FP block order: 1 plain view-major, 2 XCD bands (default per form)
     {"cone_fp_mode", 0},     // Siddon FP: 0 column-driven form (rays of a detector column together), 1 ray per lane
     {"cone_bp_mode", 0},     // Siddon BP: 0 column mode where the geometry allows it, 1 general
     {"cone_plan_persistent", 0},   // cone plans: 1 persistent blocks with work counters, 2 one block per tile

@@ -270,6 +270,114 @@ __global__ __launch_bounds__(VPB > 1 ? 64 * VPB : 128) void dd_fp_kernel(
 }
 
 // ---------------------------------------------------------------------------
+// Forward projection, several lanes per detector column
+// ---------------------------------------------------------------------------
+// The kernel above gives each lane a whole channel vector, so one float4 load
+// of a wavefront touches a different cache line for nearly every lane (64
+// columns cover ~40 pixels of a row) and uses 16 bytes of each: the L1's line
+// lookups, not HBM, set its speed.  Here LPC lanes share a column and split
+// its CH channels (lane s of the column holds the float4s at channel offsets
+// 4 (s + LPC k)), so load k of a wavefront reads one contiguous 16 LPC-byte
+// piece of each pixel it touches, and 64 / LPC columns touch far fewer
+// pixels.  The footprint arithmetic per image row is repeated by the LPC lanes
+// of a column.  Pixels are CH floats (no padding), one chunk, no channel after
+// it (channel CH reads as 0).  A block is 4 wavefronts = 4 consecutive views
+// over the same 64 / LPC columns, for the L1 sharing described above.
+template <int NQ>
+__device__ inline void dd_line_split(float* acc, const float* __restrict__ row, int cs, int step, float x1, float x2,
+                                     int n, float scale) {
+    if (x1 > x2) { const float t = x1; x1 = x2; x2 = t; }
+    const float a = clampf(x1, 0.f, (float)n), b = clampf(x2, 0.f, (float)n);
+    if (!(b > a)) return;
+    const float inv = scale / (x2 - x1);
+    const int i1 = min((int)a, n - 1), i2 = min((int)b, n - 1);
+    for (int i = i1; i <= i2; i++) {
+        const float w = (fminf(b, (float)(i + 1)) - fmaxf(a, (float)i)) * inv;
+        const float* p = row + (size_t)i * cs;
+#pragma unroll
+        for (int k = 0; k < NQ; k++) {
+            const float4 q = *reinterpret_cast<const float4*>(p + k * step);
+            acc[4 * k] = fmaf(w, q.x, acc[4 * k]);
+            acc[4 * k + 1] = fmaf(w, q.y, acc[4 * k + 1]);
+            acc[4 * k + 2] = fmaf(w, q.z, acc[4 * k + 2]);
+            acc[4 * k + 3] = fmaf(w, q.w, acc[4 * k + 3]);
+        }
+    }
+}
+
+template <int CH, int LPC, bool FAN>
+__global__ __launch_bounds__(256) void dd_fp_split_kernel(
+        float* __restrict__ prj, const float* __restrict__ imgY, const float* __restrict__ imgX,
+        const DDView* __restrict__ views, const FPRow* __restrict__ rows, int nrow, Grid g, Det d, int nview,
+        float dsd, float dso, bool force) {
+    static_assert(CH % (4 * LPC) == 0, "a lane holds whole float4s");
+    constexpr int NQ = CH / (4 * LPC);   // float4s per lane and pixel
+    constexpr int NC = 64 / LPC;         // columns per wavefront
+    __shared__ float lds[4][NC][CH + 1];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int view = blockIdx.y * 4 + wave;
+    if (view >= nview) return;   // every exchange below stays within the wavefront
+    const int c = lane / LPC, sub = lane % LPC;
+    const int iu = blockIdx.x * NC + c;
+    const DDView V = views[view];
+    float acc[4 * NQ];
+#pragma unroll
+    for (int j = 0; j < 4 * NQ; j++) acc[j] = 0.f;
+    if (iu < d.nu) {
+        float2 s1, s2, d1, d2;
+        const float a = fp_cell_edges<FAN>(iu, V, g, d, dsd, dso, s1, s2, d1, d2);
+        if (V.flags & 1) {
+            // trace along y, cuda/distanceDriven2DBoxInt.cu:48-71
+            const float r1 = (d1.x - s1.x) / (d1.y - s1.y);
+            const float r2 = (d2.x - s2.x) / (d2.y - s2.y);
+            const float norm = V.wx * g.dy / fabsf(cosf(V.deg - a));
+            const float* __restrict__ b = imgY + 4 * sub;
+            for (int iy = 0; iy < g.ny; iy++) {
+                const float yy = (float)iy + 0.5f;
+                const float x1 = fmaf(r1, yy - s1.y, s1.x);
+                const float x2 = fmaf(r2, yy - s2.y, s2.x);
+                dd_line_split<NQ>(acc, b + (size_t)iy * g.nx * CH, CH, 4 * LPC, x1, x2, g.nx, norm);
+            }
+        }
+        if (V.flags & 2) {
+            // trace along x, cuda/distanceDriven2DBoxInt.cu:74-98
+            const float r1 = (d1.y - s1.y) / (d1.x - s1.x);
+            const float r2 = (d2.y - s2.y) / (d2.x - s2.x);
+            const float norm = (1.f - V.wx) * g.dx / fabsf(sinf(V.deg - a));
+            const float* __restrict__ b = imgX + 4 * sub;
+            for (int ix = 0; ix < g.nx; ix++) {
+                const float xx = (float)ix + 0.5f;
+                const float y1 = fmaf(r1, xx - s1.x, s1.y);
+                const float y2 = fmaf(r2, xx - s2.x, s2.y);
+                dd_line_split<NQ>(acc, b + (size_t)ix * g.ny * CH, CH, 4 * LPC, y1, y2, g.ny, norm);
+            }
+        }
+    }
+    // gather the column's channel vector in LDS (channel CH is 0) and read
+    // back the detector rows: lane = (column lane % NC, row group lane / NC)
+    float* col = &lds[wave][c][0];
+#pragma unroll
+    for (int k = 0; k < NQ; k++) {
+#pragma unroll
+        for (int e = 0; e < 4; e++) col[4 * (sub + LPC * k) + e] = acc[4 * k + e];
+    }
+    if (sub == 0) col[CH] = 0.f;
+    // the LDS writes above are complete before any lane of the wavefront reads
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    const int oc = lane % NC, rg = lane / NC;
+    const int ou = blockIdx.x * NC + oc;
+    if (ou >= d.nu) return;
+    const float rev = (!FAN && force) ? V.rev : 1.f;
+    const float* mine = &lds[wave][oc][0];
+    for (int r = rg; r < nrow; r += LPC) {
+        const FPRow R = rows[r];
+        const float val = (1.f - R.w1) * mine[R.j0] + R.w1 * mine[R.j1];
+        prj[(((size_t)R.b * nview + view) * d.nv + R.iv) * d.nu + ou] = val * rev;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Forward projection with LDS-staged image rows
 // ---------------------------------------------------------------------------
 // A block is VB views x 64 detector columns (one wavefront per view; the views
@@ -534,6 +642,7 @@ struct DDFPSetup {
     bool use_lds = false, needY = false, needX = false, force = false;
     int wmax = 0;
     int lseg = 0;   // direct kernel: 2^lseg row segments per column
+    int lpc = 1;    // lanes per detector column (> 1: dd_fp_split_kernel)
     bool xc = true; // pixels carry the channel after the chunk
     long rsY = 0, rsX = 0;   // row strides (floats) of the y-major and x-major copies (n * cs)
 };
@@ -601,8 +710,6 @@ static DDFPSetup dd_fp_setup(const std::vector<DDView>& views, int B, const Grid
     s.cs = s.xc ? ((nchunk * CH + 1 + 3) / 4) * 4 : (CH >= 16 ? CH + 4 : CH);
     const int forced_cs = tuning_get("dd_fp_pixel_stride", 0);
     if (!s.xc && forced_cs >= CH && forced_cs % (CH >= 4 ? 4 : 1) == 0) s.cs = forced_cs;
-    s.rsY = (long)g.nx * s.cs;
-    s.rsX = (long)g.ny * s.cs;
     for (auto& v : views) { s.needY |= (v.flags & 1) != 0; s.needX |= (v.flags & 2) != 0; }
     // the branchless parallel FP has no force-FBP reverse weighting
     // (cuda/distanceDriven2DBranchless.cu:321-360)
@@ -616,6 +723,16 @@ static DDFPSetup dd_fp_setup(const std::vector<DDView>& views, int B, const Grid
     const int forced = tuning_get("dd_fp_segments", 0);
     if (forced == 1 || forced == 2) lseg = forced - 1;
     s.lseg = lseg;
+    // lanes per column: the split kernel needs one chunk without the next
+    // chunk's channel, whole float4s per lane and unpadded CH-float pixels
+    const int lpc = tuning_get("dd_fp_lanes_per_column", 4);
+    if (!s.xc && lseg == 0 && (CH == 8 || CH == 16 || CH == 32) && (lpc == 2 || lpc == 4 || lpc == 8) &&
+        CH % (4 * lpc) == 0) {
+        s.lpc = lpc;
+        s.cs = CH;
+    }
+    s.rsY = (long)g.nx * s.cs;
+    s.rsX = (long)g.ny * s.cs;
     return s;
 }
 
@@ -664,6 +781,24 @@ static void dd_fp_run(const DDFPSetup& s, const DDFPDev& t, float* prj, const fl
     }
         CTP_FPL(1) CTP_FPL(2) CTP_FPL(4) CTP_FPL(8) CTP_FPL(16) CTP_FPL(32)
 #undef CTP_FPL
+        CTP_LAUNCH_CHECK();
+        return;
+    }
+    if (s.lpc > 1) {
+        const int nc = 64 / s.lpc;
+        dim3 grid(ceil_div(d.nu, nc), ceil_div(nview, 4)), block(256);
+        const int nrow = (int)s.rows.size();
+        TimedLaunch tt(fan ? "dd_fan_fp" : "dd_par_fp", stream);
+#define CTP_FPS_LAUNCH(N, L)                                                                                   \
+    if (CH == N && s.lpc == L) {                                                                               \
+        if (fan) dd_fp_split_kernel<N, L, true><<<grid, block, 0, stream>>>(prj, imgY, imgX, dviews, drows, nrow, g, d, \
+                                                                        nview, dsd, dso, force);               \
+        else dd_fp_split_kernel<N, L, false><<<grid, block, 0, stream>>>(prj, imgY, imgX, dviews, drows, nrow, g, d, \
+                                                                         nview, dsd, dso, force);              \
+    }
+        CTP_FPS_LAUNCH(8, 2) CTP_FPS_LAUNCH(16, 2) CTP_FPS_LAUNCH(16, 4) CTP_FPS_LAUNCH(32, 2) CTP_FPS_LAUNCH(32, 4)
+        CTP_FPS_LAUNCH(32, 8)
+#undef CTP_FPS_LAUNCH
         CTP_LAUNCH_CHECK();
         return;
     }

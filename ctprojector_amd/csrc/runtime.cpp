@@ -28,8 +28,8 @@ std::mutex g_tune_mutex;
 std::map<std::string, int> g_tune = {
     {"cone_fp_tile_u", 0},   // detector columns per wavefront tile of the Siddon FP (8, 16, 32, 64)
     {"cone_bp_zrun", 0},     // voxels along z per thread of the Siddon BP (1, 2, 4, 8; default 4)
-    {"cone_fp_order", 0},
-    {"cone_fp_views_per_block", 0},   // column-driven Siddon FP: views per block (1, 2, 4; default 2)    // Siddon FP block order: 1 plain view-major, 2 XCD bands (default per form)
+    {"cone_fp_order", 0},    // Siddon FP block order: 1 plain view-major, 2 XCD bands (default per form)
+    {"cone_fp_views_per_block", 0},   // column-driven Siddon FP: views per block (1, 2, 4; default 2)
     {"cone_fp_mode", 0},     // Siddon FP: 0 column-driven form (rays of a detector column together), 1 ray per lane
     {"cone_bp_mode", 0},     // Siddon BP: 0 column mode where the geometry allows it, 1 general
     {"cone_plan_persistent", 0},   // cone plans: 1 persistent blocks with work counters, 2 one block per tile
@@ -37,6 +37,7 @@ std::map<std::string, int> g_tune = {
     {"dd_fp_extra_channel", 0},  // DD FP direct kernel: 1 keeps the next chunk's channel with one chunk too
     {"dd_fp_pixel_stride", 0},   // DD FP direct kernel, one channel chunk: floats per pixel of the channel-last copy
     {"dd_fp_views_per_block", 0},  // DD FP direct kernel, one row segment: views per block (1, 2, 4; default 4)
+    {"dd_fp_lanes_per_column", 0}, // DD FP: lanes per detector column (1 = whole channel vector per lane; 2, 4, 8; default 4)
     {"dd_fp_segments", 0},   // DD FP direct kernel: row segments per column (1, 2; default by wave count)
     {"tv_mode", 0},          // TV prior: 1 z-marching tile kernels (default), 2 one lane per voxel
     {"bp_view_groups", 0},   // view groups of the pixel-driven DD / FBP backprojections (default: enough for 1024 blocks)

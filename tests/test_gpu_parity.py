@@ -359,6 +359,35 @@ def test_dd_fp_staged_rows_equal_direct(tuning, case):
     assert np.array_equal(a, b), rel_l2(a, b)
 
 
+@pytest.mark.parametrize('branchless', [False, True])
+@pytest.mark.parametrize('case', ['fan_32_lpc4', 'fan_20_lpc8', 'fan_2x8_lpc2', 'fan_2x8_lpc4', 'fan_5_lpc2',
+                                  'parallel_32_lpc4', 'parallel_2x8_lpc2'])
+def test_dd_fp_split_lanes(oracle, tuning, branchless, case):
+    """The forward projector with 2, 4 or 8 lanes per detector column (each
+    lane a part of the channel vector) against the oracle and against one lane
+    per column: 32 slices, 20 slices in a 32-channel chunk (unused channels),
+    2 x 8 slices (16 channels), 5 slices (8 channels)."""
+    kind, lpc = case.split('_lpc')
+    beam, dims = kind.split('_', 1)
+    nz, B = (int(dims.split('x')[1]), int(dims.split('x')[0])) if 'x' in dims else (int(dims), 1)
+    if beam == 'fan':
+        g, ang, img = small_fan(nz=nz, nv=nz + 2, off=(0.4, 0.3), c=(1.0, -0.5, 0.3), B=B)
+        run = lambda: fan.distance_driven_fp(_proj(g), img, ang, branchless=branchless)
+        ref = oracle.dd_fan_fp(g, img, ang, branchless=branchless)
+    else:
+        g, ang, img = small_parallel(nz=nz, nv=nz, off=(0.4, -0.2), B=B)
+        run = lambda: parallel.distance_driven_fp(_proj(g), img, ang, branchless=branchless)
+        ref = oracle.dd_par_fp(g, img, ang, branchless=branchless)
+    tuning('dd_fp_lanes_per_column', int(lpc))
+    a = run()
+    tuning('dd_fp_lanes_per_column', 1)
+    b = run()
+    e = rel_l2(a, ref)
+    print('dd fp split', case, branchless, e, rel_l2(a, b))
+    assert e < TOL
+    assert rel_l2(a, b) < 1e-6
+
+
 # ------------------------------------------------------------ Siddon fan beam
 @pytest.mark.parametrize('variant', ['plain', 'offsets_batch'])
 def test_siddon_fan(oracle, variant):

@@ -490,7 +490,7 @@ def short_name(kname):
 
 # timing names (HIP-event scopes in the library) -> kernel symbols in rocprofv3 output
 # (the instances the benchmark workloads launch)
-TIMED_KERNELS = {'dd_fan_fp': 'dd_fp_kernel<32, true, false, 4>', 'dd_fan_bp': 'dd_bp_kernel<32, true>',
+TIMED_KERNELS = {'dd_fan_fp': 'dd_fp_split_kernel<32, 4, true>', 'dd_fan_bp': 'dd_bp_split_kernel<32, 4, true>',
                  'siddon_cone_fp': 'siddon_cone_fp_col_kernel<1>',
                  'siddon_cone_bp': 'siddon_cone_bp_col_kernel<2, 2, 4, 1>',
                  'fan_fbp_bp': 'fan_fbp_bp_kernel<32, true>', 'ramp_filter': 'ramp_conv_kernel<4>'}

@@ -1002,6 +1002,87 @@ __global__ __launch_bounds__(256) void dd_bp_kernel(float* __restrict__ img, flo
     store_bp_result<CH>(img, part, acc, grp, c0, C, (size_t)g.nx * g.ny, (size_t)iy * g.nx + ix);
 }
 
+// The backprojection above with LPC lanes per pixel, for the reason given at
+// dd_fp_split_kernel: lane s of a pixel holds the float4s at channel offsets
+// 4 (s + LPC k), so a load instruction reads one contiguous 16 LPC-byte piece
+// of each detector cell, and a wavefront's 64 / LPC pixels (a WX x WY tile)
+// cover fewer cells.  In the fan beam the two lanes of a pair each evaluate
+// one of the pixel's two edge projections (one atan each) and exchange them.
+// A block is 2 x 2 wavefront tiles.  Same arithmetic per channel as
+// dd_bp_kernel.
+template <int CH, int LPC, bool FAN>
+__global__ __launch_bounds__(256) void dd_bp_split_kernel(float* __restrict__ img, float* __restrict__ part,
+                                                          const float* __restrict__ sino,
+                                                          const DDView* __restrict__ views, int C, int cs, int nchunk,
+                                                          int vpg, Grid g, Det d, int nview, float dso, bool fbp) {
+    static_assert(CH % (4 * LPC) == 0 && LPC >= 2, "a lane holds whole float4s");
+    constexpr int NQ = CH / (4 * LPC);
+    constexpr int WX = LPC == 2 ? 8 : 4, WY = 64 / LPC / WX;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int p = lane / LPC, sub = lane % LPC;
+    const int ix = blockIdx.x * (2 * WX) + (wave & 1) * WX + p % WX;
+    const int iy = blockIdx.y * (2 * WY) + (wave >> 1) * WY + p / WX;
+    const int grp = blockIdx.z / nchunk;
+    const int c0 = (blockIdx.z - grp * nchunk) * CH;
+    const int vbeg = grp * vpg, vend = min(nview, vbeg + vpg);
+    if (ix >= g.nx || iy >= g.ny) return;   // the lanes of a pixel leave together
+    // pixel centre and edges, cuda/distanceDriven2DBoxInt.cu:340-349
+    const float xc = ((float)ix - g.nx / 2.f) * g.dx + g.cx + g.dx / 2;
+    const float yc = ((float)iy - g.ny / 2.f) * g.dy + g.cy + g.dy / 2;
+    const float x1 = xc - g.dx / 2.0f, x2 = xc + g.dx / 2.0f;
+    const float y1 = yc - g.dy / 2.0f, y2 = yc + g.dy / 2.0f;
+    float acc[4 * NQ];
+#pragma unroll
+    for (int j = 0; j < 4 * NQ; j++) acc[j] = 0.f;
+    const float half = (float)d.nu / 2.0f;
+    const size_t vstride = (size_t)d.nu * cs;
+    const bool second = (sub & 1) != 0;
+    for (int view = vbeg; view < vend; view++) {
+        const DDView V = views[view];
+        const float c = V.c, s = V.s;
+        const float* __restrict__ base = sino + view * vstride + c0 + 4 * sub;
+        if (FAN) {
+            // cuda/distanceDriven2DBoxInt.cu:362-391 and GetProjectionOnDetector (Utils.cu:96-102)
+            float px, py;
+            if (V.flags & 4) { px = second ? x2 : x1; py = yc; }
+            else { px = xc; py = second ? y2 : y1; }
+            const float am = atanf((px * c + py * s) / (-px * s + py * c + dso));
+            const float um = -(am / d.du + d.off_u) + half;
+            const float uo = __shfl_xor(um, 1);
+            const float u1 = second ? uo : um, u2 = second ? um : uo;
+            float scale = 1.f;
+            if (fbp) {
+                const float rx = xc * c + yc * s, ry = -xc * s + yc * c;
+                scale = dso * dso / (rx * rx + (dso + ry) * (dso + ry));
+            }
+            dd_line_split<NQ>(acc, base, cs, 4 * LPC, u1, u2, d.nu, scale);
+        } else {
+            // cuda/distanceDriven2DBoxInt.cu:439-493 (blended main axes)
+            if (V.flags & 1) {
+                const float u1 = -((x1 * c + yc * s) / d.du + d.off_u) + half;
+                const float u2 = -((x2 * c + yc * s) / d.du + d.off_u) + half;
+                dd_line_split<NQ>(acc, base, cs, 4 * LPC, u1, u2, d.nu, V.wx);
+            }
+            if (V.flags & 2) {
+                const float u1 = -((xc * c + y1 * s) / d.du + d.off_u) + half;
+                const float u2 = -((xc * c + y2 * s) / d.du + d.off_u) + half;
+                dd_line_split<NQ>(acc, base, cs, 4 * LPC, u1, u2, d.nu, 1.f - V.wx);
+            }
+        }
+    }
+    const size_t npix = (size_t)g.nx * g.ny, pix = (size_t)iy * g.nx + ix;
+#pragma unroll
+    for (int k = 0; k < NQ; k++) {
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int cc = c0 + 4 * (sub + LPC * k) + e;
+            if (cc >= C) continue;
+            if (part) part[((size_t)grp * C + cc) * npix + pix] = acc[4 * k + e];
+            else img[(size_t)cc * npix + pix] += acc[4 * k + e];
+        }
+    }
+}
+
 // Number of view groups a pixel-driven backprojection is split into so that
 // the launch has enough waves in flight to hide the per-view load latency
 // (one lane per pixel alone gives 4 waves per SIMD at 512x512).  Partial
@@ -1087,8 +1168,29 @@ static void dd_bp_run(const DDBPSetup& s, const DDView* dviews, const SliceRow* 
         CTP_LAUNCH_CHECK();
     }
     TimedLaunch tb(fan ? "dd_fan_bp" : "dd_par_bp", stream);
+    // lanes per pixel: 4 where the chunk splits into float4 quarters, else 2
+    // (measured at 512x512x32, 984 views: 5.09 ms with one lane, 3.19 ms with
+    // two, 3.07 ms with four)
+    int lpc = tuning_get("dd_bp_lanes_per_pixel", 4);
+    if (lpc == 4 && CH % 16 != 0) lpc = 2;
+    if ((lpc == 2 || lpc == 4) && CH >= 8 && CH % (4 * lpc) == 0) {
+        const int bx = lpc == 2 ? 16 : 8, by = lpc == 2 ? 8 : 8;
+        dim3 grid(ceil_div(g.nx, bx), ceil_div(g.ny, by), nchunk * groups), block(256);
+#define CTP_BPS_LAUNCH(N, L)                                                                                  \
+    if (CH == N && lpc == L) {                                                                                \
+        if (fan) dd_bp_split_kernel<N, L, true><<<grid, block, 0, stream>>>(img, part, sino, dviews, C, cs, nchunk, \
+                                                                        vpg, g, d, nview, dso, fbp);           \
+        else dd_bp_split_kernel<N, L, false><<<grid, block, 0, stream>>>(img, part, sino, dviews, C, cs, nchunk, \
+                                                                         vpg, g, d, nview, dso, fbp);          \
+    }
+        CTP_BPS_LAUNCH(8, 2) CTP_BPS_LAUNCH(16, 2) CTP_BPS_LAUNCH(16, 4) CTP_BPS_LAUNCH(32, 2) CTP_BPS_LAUNCH(32, 4)
+#undef CTP_BPS_LAUNCH
+        CTP_LAUNCH_CHECK();
+        if (part) reduce_parts(img, part, groups, (size_t)C * g.nx * g.ny, stream);
+        return;
+    }
     dim3 grid(ceil_div(g.nx, 16), ceil_div(g.ny, 16), nchunk * groups), block(256);
-#define CTP_BP_CASE(N)                                                                                         \
+#define CTP_BP_CASE(N)                                                                                        \
     case N:                                                                                                    \
         if (fan) dd_bp_kernel<N, true><<<grid, block, 0, stream>>>(img, part, sino, dviews, C, cs, nchunk, vpg, g, d, \
                                                                    nview, dso, fbp);                                \

@@ -388,6 +388,35 @@ def test_dd_fp_split_lanes(oracle, tuning, branchless, case):
     assert rel_l2(a, b) < 1e-6
 
 
+@pytest.mark.parametrize('branchless', [False, True])
+@pytest.mark.parametrize('case', ['fan_32_lpc2', 'fan_32_lpc4', 'fan_2x20_lpc2', 'fan_2x20_lpc4', 'fan_2x8_lpc4',
+                                  'fan_5_lpc2', 'parallel_32_lpc2', 'parallel_2x8_lpc4'])
+def test_dd_bp_split_lanes(oracle, tuning, branchless, case):
+    """The pixel-driven backprojector with 2 or 4 lanes per pixel against the
+    oracle and against one lane per pixel (FBP and plain weights): 32 slices,
+    2 x 20 slices (two 32-channel chunks), 2 x 8 slices, 5 slices."""
+    kind, lpc = case.split('_lpc')
+    beam, dims = kind.split('_', 1)
+    nz, B = (int(dims.split('x')[1]), int(dims.split('x')[0])) if 'x' in dims else (int(dims), 1)
+    if beam == 'fan':
+        g, ang, img = small_fan(nz=nz, nv=nz + 2, off=(0.4, 0.3), c=(1.0, -0.5, 0.3), B=B)
+        mod, ref_fn = fan, oracle.dd_fan_bp
+    else:
+        g, ang, img = small_parallel(nz=nz, nv=nz, off=(0.4, -0.2), B=B)
+        mod, ref_fn = parallel, oracle.dd_par_bp
+    y = np.random.default_rng(8).uniform(size=[B, len(ang), g.nv, g.nu]).astype(np.float32)
+    for is_fbp in (False, True):
+        tuning('dd_bp_lanes_per_pixel', int(lpc))
+        a = mod.distance_driven_bp(_proj(g), y, ang, is_fbp=is_fbp, branchless=branchless)
+        tuning('dd_bp_lanes_per_pixel', 1)
+        b = mod.distance_driven_bp(_proj(g), y, ang, is_fbp=is_fbp, branchless=branchless)
+        ref = ref_fn(g, y, ang, is_fbp=is_fbp, branchless=branchless)
+        e = rel_l2(a, ref)
+        print('dd bp split', case, branchless, is_fbp, e, rel_l2(a, b))
+        assert e < TOL
+        assert rel_l2(a, b) < 1e-6
+
+
 # ------------------------------------------------------------ Siddon fan beam
 @pytest.mark.parametrize('variant', ['plain', 'offsets_batch'])
 def test_siddon_fan(oracle, variant):

@@ -350,6 +350,87 @@ __global__ __launch_bounds__(256) void fan_fbp_bp_kernel(float* __restrict__ img
     store_bp_result<CH>(img, part, acc, grp, c0, C, (size_t)g.nx * g.ny, (size_t)iy * g.nx + ix);
 }
 
+// The backprojection above with LPC lanes per pixel: lane s of a pixel holds
+// the float4s at channel offsets 4 (s + LPC k), so each load instruction reads
+// a contiguous 16 LPC-byte piece of a detector cell and a wavefront's
+// 64 / LPC pixels (a WX x WY tile) touch few cells.  The per-view geometry is
+// spread over the lanes of a pixel: lane s evaluates view v0 + s, and the
+// pixel's lanes read each view's detector position and distance weight from
+// it.  Same arithmetic per channel as fan_fbp_bp_kernel.
+template <int CH, int LPC, bool FAN>
+__global__ __launch_bounds__(256) void fan_fbp_bp_split_kernel(float* __restrict__ img, float* __restrict__ part,
+                                                               const float* __restrict__ q,
+                                                               const FBPView* __restrict__ views, int C, int cs,
+                                                               int nchunk, int vpg, Grid g, Det d, int nview,
+                                                               float dso) {
+    static_assert(CH % (4 * LPC) == 0 && (LPC == 2 || LPC == 4), "a lane holds whole float4s");
+    constexpr int NQ = CH / (4 * LPC);
+    constexpr int WX = LPC == 2 ? 8 : 4, WY = 64 / LPC / WX;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int p = lane / LPC, sub = lane % LPC, lead = lane & ~(LPC - 1);
+    const int ix = blockIdx.x * (2 * WX) + (wave & 1) * WX + p % WX;
+    const int iy = blockIdx.y * (2 * WY) + (wave >> 1) * WY + p / WX;
+    const int grp = blockIdx.z / nchunk;
+    const int c0 = (blockIdx.z - grp * nchunk) * CH;
+    const int vbeg = grp * vpg, vend = min(nview, vbeg + vpg);
+    if (ix >= g.nx || iy >= g.ny) return;   // the lanes of a pixel leave together
+    // pixel centre, cuda/fbp.cu:288
+    const float px = ((float)ix + 0.5f - g.nx / 2.f) * g.dx + g.cx;
+    const float py = ((float)iy + 0.5f - g.ny / 2.f) * g.dy + g.cy;
+    float acc[4 * NQ];
+#pragma unroll
+    for (int j = 0; j < 4 * NQ; j++) acc[j] = 0.f;
+    const float half = (d.nu - 1.0f) / 2.0f;
+    const size_t vstride = (size_t)d.nu * cs;
+    const float* __restrict__ qs = q + c0 + 4 * sub;
+    for (int v0 = vbeg; v0 < vend; v0 += LPC) {
+        float mpu = -1.f, mdist = 0.f;
+        if (v0 + sub < vend) {
+            const FBPView V = views[v0 + sub];
+            const float rx = px * V.c + py * V.s;
+            const float ry = -px * V.s + py * V.c;
+            const float a = FAN ? atanf(rx / (ry + dso)) : rx;
+            mdist = FAN ? dso * dso / (rx * rx + (dso + ry) * (dso + ry)) : 1.f;
+            mpu = -(a / d.du + d.off_u) + half;
+        }
+#pragma unroll
+        for (int t = 0; t < LPC; t++) {
+            const int view = v0 + t;
+            if (view >= vend) break;   // uniform
+            const float pu = __shfl(mpu, lead + t);
+            const float dist = __shfl(mdist, lead + t);
+            // InterpolateXY with truncation, cuda/cudaMath.cu:227-260
+            if (pu < 0.f || pu >= (float)d.nu) continue;
+            const int u0 = (int)pu;
+            const int u1 = min(u0 + 1, d.nu - 1);
+            const float wu = pu - (float)u0;
+            const float w0 = (1.f - wu) * dist, w1 = wu * dist;
+            const float* __restrict__ p0 = qs + view * vstride + (size_t)u0 * cs;
+            const float* __restrict__ p1 = qs + view * vstride + (size_t)u1 * cs;
+#pragma unroll
+            for (int k = 0; k < NQ; k++) {
+                const float4 a0 = *reinterpret_cast<const float4*>(p0 + 4 * LPC * k);
+                const float4 a1 = *reinterpret_cast<const float4*>(p1 + 4 * LPC * k);
+                acc[4 * k] = fmaf(w0, a0.x, fmaf(w1, a1.x, acc[4 * k]));
+                acc[4 * k + 1] = fmaf(w0, a0.y, fmaf(w1, a1.y, acc[4 * k + 1]));
+                acc[4 * k + 2] = fmaf(w0, a0.z, fmaf(w1, a1.z, acc[4 * k + 2]));
+                acc[4 * k + 3] = fmaf(w0, a0.w, fmaf(w1, a1.w, acc[4 * k + 3]));
+            }
+        }
+    }
+    const size_t npix = (size_t)g.nx * g.ny, pix = (size_t)iy * g.nx + ix;
+#pragma unroll
+    for (int k = 0; k < NQ; k++) {
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int cc = c0 + 4 * (sub + LPC * k) + e;
+            if (cc >= C) continue;
+            if (part) part[((size_t)grp * C + cc) * npix + pix] = acc[4 * k + e];
+            else img[(size_t)cc * npix + pix] += acc[4 * k + e];
+        }
+    }
+}
+
 void fan_fbp_bp(float* img, const float* prj, const float* deg_host, int B, const Grid& g, const Det& d, int nview,
                 float dsd, float dso, hipStream_t stream, bool fan) {
     (void)dsd;
@@ -393,6 +474,28 @@ void fan_fbp_bp(float* img, const float* prj, const float* deg_host, int B, cons
     CTP_CHECK(hipMemcpyAsync(dviews, views.data(), sizeof(FBPView) * nview, hipMemcpyHostToDevice, stream));
     CTP_CHECK(hipMemcpyAsync(dsrow, srow.data(), sizeof(SliceRow) * g.nz, hipMemcpyHostToDevice, stream));
     slices_to_channel_last(q, prj, dsrow, C, cs, g.nz, d, nview, stream);
+    int lpc = tuning_get("fbp_bp_lanes_per_pixel", 4);
+    if (lpc == 4 && CH % 16 != 0) lpc = 2;
+    if ((lpc == 2 || lpc == 4) && CH >= 8 && CH % (4 * lpc) == 0) {
+        dim3 sgrid(ceil_div(g.nx, lpc == 2 ? 16 : 8), ceil_div(g.ny, 8), nchunk * groups), block(256);
+        {
+        TimedLaunch t(fan ? "fan_fbp_bp" : "par_pixel_bp", stream);
+#define CTP_FBPS_LAUNCH(N, L)                                                                                 \
+    if (CH == N && lpc == L) {                                                                                \
+        if (fan) fan_fbp_bp_split_kernel<N, L, true><<<sgrid, block, 0, stream>>>(img, part, q, dviews, C, cs,  \
+                                                                              nchunk, vpg, g, d, nview, dso);   \
+        else fan_fbp_bp_split_kernel<N, L, false><<<sgrid, block, 0, stream>>>(img, part, q, dviews, C, cs,    \
+                                                                               nchunk, vpg, g, d, nview, dso);  \
+    }
+        CTP_FBPS_LAUNCH(8, 2) CTP_FBPS_LAUNCH(16, 2) CTP_FBPS_LAUNCH(16, 4) CTP_FBPS_LAUNCH(32, 2)
+        CTP_FBPS_LAUNCH(32, 4)
+#undef CTP_FBPS_LAUNCH
+        CTP_LAUNCH_CHECK();
+        if (part) reduce_parts(img, part, groups, (size_t)C * npix, stream);
+        }
+        CTP_CHECK(hipStreamSynchronize(stream));
+        return;
+    }
     dim3 grid(ceil_div(g.nx, 16), ceil_div(g.ny, 16), nchunk * groups), block(256);
     {
     TimedLaunch t(fan ? "fan_fbp_bp" : "par_pixel_bp", stream);

@@ -39,6 +39,7 @@ std::map<std::string, int> g_tune = {
     {"dd_fp_views_per_block", 0},  // DD FP direct kernel, one row segment: views per block (1, 2, 4; default 4)
     {"dd_fp_lanes_per_column", 0}, // DD FP: lanes per detector column (1 = whole channel vector per lane; 2, 4, 8; default 4)
     {"dd_bp_lanes_per_pixel", 0},  // DD BP (pixel driven): lanes per pixel (1 = whole channel vector per lane; 2, 4; default 4, 2 at 8 channels)
+    {"fbp_bp_lanes_per_pixel", 0}, // fan FBP / parallel pixel BP: lanes per pixel (1 = whole channel vector per lane; 2, 4; default 4, 2 at 8 channels)
     {"dd_fp_segments", 0},   // DD FP direct kernel: row segments per column (1, 2; default by wave count)
     {"tv_mode", 0},          // TV prior: 1 z-marching tile kernels (default), 2 one lane per voxel
     {"bp_view_groups", 0},   // view groups of the pixel-driven DD / FBP backprojections (default: enough for 1024 blocks)

@@ -280,6 +280,32 @@ def test_fan_fbp_bp(oracle, variant):
     assert e < TOL
 
 
+@pytest.mark.parametrize('case', ['fan_32_lpc4', 'fan_32_lpc2', 'fan_2x20_lpc4', 'fan_2x8_lpc4', 'fan_5_lpc2',
+                                  'parallel_32_lpc4', 'parallel_2x8_lpc2'])
+def test_fbp_bp_split_lanes(oracle, tuning, case):
+    """The fan FBP / parallel pixel-driven backprojection with 2 or 4 lanes per
+    pixel (views' geometry spread over the pixel's lanes; 41 views, not a
+    multiple of the lane count) against the oracle and one lane per pixel."""
+    kind, lpc = case.split('_lpc')
+    beam, dims = kind.split('_', 1)
+    nz, B = (int(dims.split('x')[1]), int(dims.split('x')[0])) if 'x' in dims else (int(dims), 1)
+    if beam == 'fan':
+        g, ang, img = small_fan(nz=nz, nv=nz + 2, nview=41, off=(0.4, 0.3), c=(1.0, -0.5, 0.3), B=B)
+        run, ref_fn = fan.fbp_bp, oracle.fan_fbp_bp
+    else:
+        g, ang, img = small_parallel(nz=nz, nv=nz, nview=41, off=(0.4, -0.2), B=B)
+        run, ref_fn = parallel.pixel_driven_bp, oracle.par_pixel_bp
+    y = np.random.default_rng(9).uniform(size=[B, len(ang), g.nv, g.nu]).astype(np.float32)
+    tuning('fbp_bp_lanes_per_pixel', int(lpc))
+    a = run(_proj(g), y, ang)
+    tuning('fbp_bp_lanes_per_pixel', 1)
+    b = run(_proj(g), y, ang)
+    e = rel_l2(a, ref_fn(g, y, ang))
+    print('fbp bp split', case, e, rel_l2(a, b))
+    assert e < TOL
+    assert rel_l2(a, b) < 1e-6
+
+
 # ----------------------------------------------------------------- tomo
 def _tomo_case(B=1):
     g = Geo(nx=40, ny=36, nz=12, dx=1.0, dy=1.0, dz=1.0, cz=20.0, nu=56, nv=52, du=1.0, dv=1.0)

@@ -1006,9 +1006,8 @@ __global__ __launch_bounds__(256) void dd_bp_kernel(float* __restrict__ img, flo
 // dd_fp_split_kernel: lane s of a pixel holds the float4s at channel offsets
 // 4 (s + LPC k), so a load instruction reads one contiguous 16 LPC-byte piece
 // of each detector cell, and a wavefront's 64 / LPC pixels (a WX x WY tile)
-// cover fewer cells.  In the fan beam the two lanes of a pair each evaluate
-// one of the pixel's two edge projections (one atan each) and exchange them.
-// A block is 2 x 2 wavefront tiles.  Same arithmetic per channel as
+// cover fewer cells.  In the fan beam the views' footprints (two atans each)
+// are spread over the pixel's lanes.  A block is 2 x 2 wavefront tiles.  Same arithmetic per channel as
 // dd_bp_kernel.
 template <int CH, int LPC, bool FAN>
 __global__ __launch_bounds__(256) void dd_bp_split_kernel(float* __restrict__ img, float* __restrict__ part,
@@ -1036,27 +1035,42 @@ __global__ __launch_bounds__(256) void dd_bp_split_kernel(float* __restrict__ im
     for (int j = 0; j < 4 * NQ; j++) acc[j] = 0.f;
     const float half = (float)d.nu / 2.0f;
     const size_t vstride = (size_t)d.nu * cs;
-    const bool second = (sub & 1) != 0;
-    for (int view = vbeg; view < vend; view++) {
-        const DDView V = views[view];
-        const float c = V.c, s = V.s;
-        const float* __restrict__ base = sino + view * vstride + c0 + 4 * sub;
-        if (FAN) {
-            // cuda/distanceDriven2DBoxInt.cu:362-391 and GetProjectionOnDetector (Utils.cu:96-102)
-            float px, py;
-            if (V.flags & 4) { px = second ? x2 : x1; py = yc; }
-            else { px = xc; py = second ? y2 : y1; }
-            const float am = atanf((px * c + py * s) / (-px * s + py * c + dso));
-            const float um = -(am / d.du + d.off_u) + half;
-            const float uo = __shfl_xor(um, 1);
-            const float u1 = second ? uo : um, u2 = second ? um : uo;
-            float scale = 1.f;
-            if (fbp) {
-                const float rx = xc * c + yc * s, ry = -xc * s + yc * c;
-                scale = dso * dso / (rx * rx + (dso + ry) * (dso + ry));
+    if (FAN) {
+        // lane s of a pixel evaluates the footprint of view v0 + s; the pixel's
+        // lanes read each view's footprint from it
+        const int lead = lane & ~(LPC - 1);
+        for (int v0 = vbeg; v0 < vend; v0 += LPC) {
+            float mu1 = 0.f, mu2 = 0.f, msc = 1.f;
+            if (v0 + sub < vend) {
+                // cuda/distanceDriven2DBoxInt.cu:362-391 and GetProjectionOnDetector (Utils.cu:96-102)
+                const DDView V = views[v0 + sub];
+                const float c = V.c, s = V.s;
+                float p1x, p1y, p2x, p2y;
+                if (V.flags & 4) { p1x = x1; p1y = yc; p2x = x2; p2y = yc; }
+                else { p1x = xc; p1y = y1; p2x = xc; p2y = y2; }
+                const float a1 = atanf((p1x * c + p1y * s) / (-p1x * s + p1y * c + dso));
+                const float a2 = atanf((p2x * c + p2y * s) / (-p2x * s + p2y * c + dso));
+                mu1 = -(a1 / d.du + d.off_u) + half;
+                mu2 = -(a2 / d.du + d.off_u) + half;
+                if (fbp) {
+                    const float rx = xc * c + yc * s, ry = -xc * s + yc * c;
+                    msc = dso * dso / (rx * rx + (dso + ry) * (dso + ry));
+                }
             }
-            dd_line_split<NQ>(acc, base, cs, 4 * LPC, u1, u2, d.nu, scale);
-        } else {
+#pragma unroll
+            for (int t = 0; t < LPC; t++) {
+                const int view = v0 + t;
+                if (view >= vend) break;   // uniform
+                const float u1 = __shfl(mu1, lead + t), u2 = __shfl(mu2, lead + t);
+                const float scale = fbp ? __shfl(msc, lead + t) : 1.f;
+                dd_line_split<NQ>(acc, sino + view * vstride + c0 + 4 * sub, cs, 4 * LPC, u1, u2, d.nu, scale);
+            }
+        }
+    } else {
+        for (int view = vbeg; view < vend; view++) {
+            const DDView V = views[view];
+            const float c = V.c, s = V.s;
+            const float* __restrict__ base = sino + view * vstride + c0 + 4 * sub;
             // cuda/distanceDriven2DBoxInt.cu:439-493 (blended main axes)
             if (V.flags & 1) {
                 const float u1 = -((x1 * c + yc * s) / d.du + d.off_u) + half;

@@ -279,8 +279,8 @@ __global__ __launch_bounds__(VPB > 1 ? 64 * VPB : 128) void dd_fp_kernel(
 // its CH channels (lane s of the column holds the float4s at channel offsets
 // 4 (s + LPC k)), so load k of a wavefront reads one contiguous 16 LPC-byte
 // piece of each pixel it touches, and 64 / LPC columns touch far fewer
-// pixels.  The footprint arithmetic per image row is repeated by the LPC lanes
-// of a column.  Pixels are CH floats (no padding), one chunk, no channel after
+// pixels.  The footprints of LPC consecutive image rows are evaluated one per
+// lane and broadcast within the column's lanes.  Pixels are CH floats (no padding), one chunk, no channel after
 // it (channel CH reads as 0).  A block is 4 wavefronts = 4 consecutive views
 // over the same 64 / LPC columns, for the L1 sharing described above.
 template <int NQ>
@@ -301,6 +301,78 @@ __device__ inline void dd_line_split(float* acc, const float* __restrict__ row, 
             acc[4 * k + 1] = fmaf(w, q.y, acc[4 * k + 1]);
             acc[4 * k + 2] = fmaf(w, q.z, acc[4 * k + 2]);
             acc[4 * k + 3] = fmaf(w, q.w, acc[4 * k + 3]);
+        }
+    }
+}
+
+// Value of lane lead + t (t < LPC) of the calling lane's group of LPC
+// consecutive lanes: a DPP quad permute for groups of 2 and 4 lanes.
+template <int LPC>
+__device__ inline int group_bcast(int v, int t, int lead) {
+    if constexpr (LPC == 4) {
+        switch (t) {   // quad_perm(t, t, t, t)
+            case 0: return __builtin_amdgcn_update_dpp(0, v, 0x00, 0xF, 0xF, true);
+            case 1: return __builtin_amdgcn_update_dpp(0, v, 0x55, 0xF, 0xF, true);
+            case 2: return __builtin_amdgcn_update_dpp(0, v, 0xAA, 0xF, 0xF, true);
+            default: return __builtin_amdgcn_update_dpp(0, v, 0xFF, 0xF, 0xF, true);
+        }
+    } else if constexpr (LPC == 2) {   // quad_perm(t, t, 2 + t, 2 + t)
+        return t == 0 ? __builtin_amdgcn_update_dpp(0, v, 0xA0, 0xF, 0xF, true)
+                      : __builtin_amdgcn_update_dpp(0, v, 0xF5, 0xF, 0xF, true);
+    } else {
+        return __shfl(v, lead + t);
+    }
+}
+template <int LPC>
+__device__ inline float group_bcast(float v, int t, int lead) {
+    return __int_as_float(group_bcast<LPC>(__float_as_int(v), t, lead));
+}
+
+// One trace of a column for dd_fp_split_kernel: rows r = 0 .. nrows-1 of the
+// channel-last copy at 'img' (row stride rstride floats, pixel stride CH),
+// the column's lines at row coordinate r + 0.5 being fmaf(r1, t - o1, q1) and
+// fmaf(r2, t - o2, q2) on rows of n pixels.  The footprint of row t0 + s
+// (clamped ends, 1 / width, first and last pixel: dd_line_split's arithmetic)
+// is evaluated by lane s of the column's LPC lanes and broadcast to the others.
+template <int NQ, int LPC>
+__device__ inline void fp_trace_split(float* acc, const float* __restrict__ img, size_t rstride, int nrows, int n,
+                                      float r1, float o1, float q1, float r2, float o2, float q2, float norm, int sub,
+                                      int lead) {
+    constexpr int CH = 4 * LPC * NQ;
+    for (int t0 = 0; t0 < nrows; t0 += LPC) {
+        float ma = 0.f, mb = 0.f, minv = 0.f;
+        int mi1 = 0, mi2 = -1;   // empty
+        if (t0 + sub < nrows) {
+            const float tt = (float)(t0 + sub) + 0.5f;
+            float x1 = fmaf(r1, tt - o1, q1), x2 = fmaf(r2, tt - o2, q2);
+            if (x1 > x2) { const float t = x1; x1 = x2; x2 = t; }
+            const float a = clampf(x1, 0.f, (float)n), b = clampf(x2, 0.f, (float)n);
+            if (b > a) {
+                ma = a; mb = b;
+                minv = norm / (x2 - x1);
+                mi1 = min((int)a, n - 1);
+                mi2 = min((int)b, n - 1);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < LPC; t++) {
+            if (t0 + t >= nrows) break;   // uniform
+            const float a = group_bcast<LPC>(ma, t, lead), b = group_bcast<LPC>(mb, t, lead);
+            const float inv = group_bcast<LPC>(minv, t, lead);
+            const int i1 = group_bcast<LPC>(mi1, t, lead), i2 = group_bcast<LPC>(mi2, t, lead);
+            const float* __restrict__ row = img + (size_t)(t0 + t) * rstride;
+            for (int i = i1; i <= i2; i++) {
+                const float w = (fminf(b, (float)(i + 1)) - fmaxf(a, (float)i)) * inv;
+                const float* p = row + (size_t)i * CH;
+#pragma unroll
+                for (int k = 0; k < NQ; k++) {
+                    const float4 q = *reinterpret_cast<const float4*>(p + k * 4 * LPC);
+                    acc[4 * k] = fmaf(w, q.x, acc[4 * k]);
+                    acc[4 * k + 1] = fmaf(w, q.y, acc[4 * k + 1]);
+                    acc[4 * k + 2] = fmaf(w, q.z, acc[4 * k + 2]);
+                    acc[4 * k + 3] = fmaf(w, q.w, acc[4 * k + 3]);
+                }
+            }
         }
     }
 }
@@ -332,26 +404,16 @@ __global__ __launch_bounds__(256) void dd_fp_split_kernel(
             const float r1 = (d1.x - s1.x) / (d1.y - s1.y);
             const float r2 = (d2.x - s2.x) / (d2.y - s2.y);
             const float norm = V.wx * g.dy / fabsf(cosf(V.deg - a));
-            const float* __restrict__ b = imgY + 4 * sub;
-            for (int iy = 0; iy < g.ny; iy++) {
-                const float yy = (float)iy + 0.5f;
-                const float x1 = fmaf(r1, yy - s1.y, s1.x);
-                const float x2 = fmaf(r2, yy - s2.y, s2.x);
-                dd_line_split<NQ>(acc, b + (size_t)iy * g.nx * CH, CH, 4 * LPC, x1, x2, g.nx, norm);
-            }
+            fp_trace_split<NQ, LPC>(acc, imgY + 4 * sub, (size_t)g.nx * CH, g.ny, g.nx, r1, s1.y, s1.x, r2, s2.y,
+                                    s2.x, norm, sub, lane - sub);
         }
         if (V.flags & 2) {
             // trace along x, cuda/distanceDriven2DBoxInt.cu:74-98
             const float r1 = (d1.y - s1.y) / (d1.x - s1.x);
             const float r2 = (d2.y - s2.y) / (d2.x - s2.x);
             const float norm = (1.f - V.wx) * g.dx / fabsf(sinf(V.deg - a));
-            const float* __restrict__ b = imgX + 4 * sub;
-            for (int ix = 0; ix < g.nx; ix++) {
-                const float xx = (float)ix + 0.5f;
-                const float y1 = fmaf(r1, xx - s1.x, s1.y);
-                const float y2 = fmaf(r2, xx - s2.x, s2.y);
-                dd_line_split<NQ>(acc, b + (size_t)ix * g.ny * CH, CH, 4 * LPC, y1, y2, g.ny, norm);
-            }
+            fp_trace_split<NQ, LPC>(acc, imgX + 4 * sub, (size_t)g.ny * CH, g.nx, g.ny, r1, s1.x, s1.y, r2, s2.x,
+                                    s2.y, norm, sub, lane - sub);
         }
     }
     // gather the column's channel vector in LDS (channel CH is 0) and read

@@ -62,4 +62,29 @@ __device__ inline T const_load(const T* p, long i) {
 #endif
 }
 
+// Value of lane lead + t (t < LPC) of the calling lane's group of LPC
+// consecutive lanes: a DPP quad permute for groups of 2 and 4 lanes.
+#ifdef __HIPCC__
+template <int LPC>
+__device__ inline int group_bcast(int v, int t, int lead) {
+    if constexpr (LPC == 4) {
+        switch (t) {   // quad_perm(t, t, t, t)
+            case 0: return __builtin_amdgcn_update_dpp(0, v, 0x00, 0xF, 0xF, true);
+            case 1: return __builtin_amdgcn_update_dpp(0, v, 0x55, 0xF, 0xF, true);
+            case 2: return __builtin_amdgcn_update_dpp(0, v, 0xAA, 0xF, 0xF, true);
+            default: return __builtin_amdgcn_update_dpp(0, v, 0xFF, 0xF, 0xF, true);
+        }
+    } else if constexpr (LPC == 2) {   // quad_perm(t, t, 2 + t, 2 + t)
+        return t == 0 ? __builtin_amdgcn_update_dpp(0, v, 0xA0, 0xF, 0xF, true)
+                      : __builtin_amdgcn_update_dpp(0, v, 0xF5, 0xF, 0xF, true);
+    } else {
+        return __shfl(v, lead + t);
+    }
+}
+template <int LPC>
+__device__ inline float group_bcast(float v, int t, int lead) {
+    return __int_as_float(group_bcast<LPC>(__float_as_int(v), t, lead));
+}
+#endif
+
 }  // namespace ctp

@@ -493,7 +493,7 @@ def short_name(kname):
 TIMED_KERNELS = {'dd_fan_fp': 'dd_fp_split_kernel<32, 4, true>', 'dd_fan_bp': 'dd_bp_split_kernel<32, 4, true>',
                  'siddon_cone_fp': 'siddon_cone_fp_col_kernel<1>',
                  'siddon_cone_bp': 'siddon_cone_bp_col_kernel<2, 2, 4, 1>',
-                 'fan_fbp_bp': 'fan_fbp_bp_kernel<32, true>', 'ramp_filter': 'ramp_conv_kernel<4>'}
+                 'fan_fbp_bp': 'fan_fbp_bp_split_kernel<32, 4, true>', 'ramp_filter': 'ramp_conv_kernel<4>'}
 
 
 def read_traffic(name):

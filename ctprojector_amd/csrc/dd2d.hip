@@ -283,26 +283,48 @@ __global__ __launch_bounds__(VPB > 1 ? 64 * VPB : 128) void dd_fp_kernel(
 // lane and broadcast within the column's lanes.  Pixels are CH floats (no padding), one chunk, no channel after
 // it (channel CH reads as 0).  A block is 4 wavefronts = 4 consecutive views
 // over the same 64 / LPC columns, for the L1 sharing described above.
-template <int NQ>
-__device__ inline void dd_line_split(float* acc, const float* __restrict__ row, int cs, int step, float x1, float x2,
-                                     int n, float scale) {
-    if (x1 > x2) { const float t = x1; x1 = x2; x2 = t; }
-    const float a = clampf(x1, 0.f, (float)n), b = clampf(x2, 0.f, (float)n);
-    if (!(b > a)) return;
-    const float inv = scale / (x2 - x1);
-    const int i1 = min((int)a, n - 1), i2 = min((int)b, n - 1);
-    for (int i = i1; i <= i2; i++) {
-        const float w = (fminf(b, (float)(i + 1)) - fmaxf(a, (float)i)) * inv;
-        const float* p = row + (size_t)i * cs;
+// Pixels i1..i2 of a row weighted by their overlap with [a, b] times inv,
+// applied to NQ float4s 4 LPC floats apart: the lane's floats of pixel i start
+// at row[i cs + lane_off].  'row' is the same for the whole wavefront, so
+// the loads take it as a scalar base plus a 32-bit per-lane byte offset.  The
+// pixel edges i and i + 1 are carried as floats (exact below 2^24);
+// min(x, y) = med3(x, y, -3e38) and max(x, y) = med3(x, y, 3e38) for these
+// finite operands, without re-canonicalising the loop-invariant bounds every
+// pixel as fminf / fmaxf do.
+template <int NQ, int LPC>
+__device__ inline void split_pixels(float* acc, const float* __restrict__ row, int lane_off, int cs, float a, float b,
+                                    float inv, int i1, int i2) {
+    if (i1 > i2) return;
+    const char* __restrict__ rb = reinterpret_cast<const char*>(row);
+    unsigned off = (unsigned)(i1 * cs + lane_off) * 4u;
+    const unsigned step = (unsigned)cs * 4u;
+    float fi = (float)i1;
+    const float fe = (float)i2;
+    for (;;) {
+        const float fn = fi + 1.f;
+        const float w = (__builtin_amdgcn_fmed3f(b, fn, -3.0e38f) - __builtin_amdgcn_fmed3f(a, fi, 3.0e38f)) * inv;
 #pragma unroll
         for (int k = 0; k < NQ; k++) {
-            const float4 q = *reinterpret_cast<const float4*>(p + k * step);
+            const float4 q = *reinterpret_cast<const float4*>(rb + off + 16u * LPC * k);
             acc[4 * k] = fmaf(w, q.x, acc[4 * k]);
             acc[4 * k + 1] = fmaf(w, q.y, acc[4 * k + 1]);
             acc[4 * k + 2] = fmaf(w, q.z, acc[4 * k + 2]);
             acc[4 * k + 3] = fmaf(w, q.w, acc[4 * k + 3]);
         }
+        if (!(fn <= fe)) break;
+        fi = fn;
+        off += step;
     }
+}
+
+template <int NQ, int LPC>
+__device__ inline void dd_line_split(float* acc, const float* __restrict__ row, int lane_off, int cs, float x1,
+                                     float x2, int n, float scale) {
+    if (x1 > x2) { const float t = x1; x1 = x2; x2 = t; }
+    const float a = clampf(x1, 0.f, (float)n), b = clampf(x2, 0.f, (float)n);
+    if (!(b > a)) return;
+    const float inv = scale / (x2 - x1);
+    split_pixels<NQ, LPC>(acc, row, lane_off, cs, a, b, inv, min((int)a, n - 1), min((int)b, n - 1));
 }
 
 // One trace of a column for dd_fp_split_kernel: rows r = 0 .. nrows-1 of the
@@ -337,19 +359,7 @@ __device__ inline void fp_trace_split(float* acc, const float* __restrict__ img,
             const float a = group_bcast<LPC>(ma, t, lead), b = group_bcast<LPC>(mb, t, lead);
             const float inv = group_bcast<LPC>(minv, t, lead);
             const int i1 = group_bcast<LPC>(mi1, t, lead), i2 = group_bcast<LPC>(mi2, t, lead);
-            const float* __restrict__ row = img + (size_t)(t0 + t) * rstride;
-            for (int i = i1; i <= i2; i++) {
-                const float w = (fminf(b, (float)(i + 1)) - fmaxf(a, (float)i)) * inv;
-                const float* p = row + (size_t)i * CH;
-#pragma unroll
-                for (int k = 0; k < NQ; k++) {
-                    const float4 q = *reinterpret_cast<const float4*>(p + k * 4 * LPC);
-                    acc[4 * k] = fmaf(w, q.x, acc[4 * k]);
-                    acc[4 * k + 1] = fmaf(w, q.y, acc[4 * k + 1]);
-                    acc[4 * k + 2] = fmaf(w, q.z, acc[4 * k + 2]);
-                    acc[4 * k + 3] = fmaf(w, q.w, acc[4 * k + 3]);
-                }
-            }
+            split_pixels<NQ, LPC>(acc, img + (size_t)(t0 + t) * rstride, 4 * sub, CH, a, b, inv, i1, i2);
         }
     }
 }
@@ -381,7 +391,7 @@ __global__ __launch_bounds__(256) void dd_fp_split_kernel(
             const float r1 = (d1.x - s1.x) / (d1.y - s1.y);
             const float r2 = (d2.x - s2.x) / (d2.y - s2.y);
             const float norm = V.wx * g.dy / fabsf(cosf(V.deg - a));
-            fp_trace_split<NQ, LPC>(acc, imgY + 4 * sub, (size_t)g.nx * CH, g.ny, g.nx, r1, s1.y, s1.x, r2, s2.y,
+            fp_trace_split<NQ, LPC>(acc, imgY, (size_t)g.nx * CH, g.ny, g.nx, r1, s1.y, s1.x, r2, s2.y,
                                     s2.x, norm, sub, lane - sub);
         }
         if (V.flags & 2) {
@@ -389,7 +399,7 @@ __global__ __launch_bounds__(256) void dd_fp_split_kernel(
             const float r1 = (d1.y - s1.y) / (d1.x - s1.x);
             const float r2 = (d2.y - s2.y) / (d2.x - s2.x);
             const float norm = (1.f - V.wx) * g.dx / fabsf(sinf(V.deg - a));
-            fp_trace_split<NQ, LPC>(acc, imgX + 4 * sub, (size_t)g.ny * CH, g.nx, g.ny, r1, s1.x, s1.y, r2, s2.x,
+            fp_trace_split<NQ, LPC>(acc, imgX, (size_t)g.ny * CH, g.nx, g.ny, r1, s1.x, s1.y, r2, s2.x,
                                     s2.y, norm, sub, lane - sub);
         }
     }
@@ -1102,24 +1112,24 @@ __global__ __launch_bounds__(256) void dd_bp_split_kernel(float* __restrict__ im
                 if (view >= vend) break;   // uniform
                 const float u1 = __shfl(mu1, lead + t), u2 = __shfl(mu2, lead + t);
                 const float scale = fbp ? __shfl(msc, lead + t) : 1.f;
-                dd_line_split<NQ>(acc, sino + view * vstride + c0 + 4 * sub, cs, 4 * LPC, u1, u2, d.nu, scale);
+                dd_line_split<NQ, LPC>(acc, sino + view * vstride + c0, 4 * sub, cs, u1, u2, d.nu, scale);
             }
         }
     } else {
         for (int view = vbeg; view < vend; view++) {
             const DDView V = views[view];
             const float c = V.c, s = V.s;
-            const float* __restrict__ base = sino + view * vstride + c0 + 4 * sub;
+            const float* __restrict__ base = sino + view * vstride + c0;
             // cuda/distanceDriven2DBoxInt.cu:439-493 (blended main axes)
             if (V.flags & 1) {
                 const float u1 = -((x1 * c + yc * s) / d.du + d.off_u) + half;
                 const float u2 = -((x2 * c + yc * s) / d.du + d.off_u) + half;
-                dd_line_split<NQ>(acc, base, cs, 4 * LPC, u1, u2, d.nu, V.wx);
+                dd_line_split<NQ, LPC>(acc, base, 4 * sub, cs, u1, u2, d.nu, V.wx);
             }
             if (V.flags & 2) {
                 const float u1 = -((xc * c + y1 * s) / d.du + d.off_u) + half;
                 const float u2 = -((xc * c + y2 * s) / d.du + d.off_u) + half;
-                dd_line_split<NQ>(acc, base, cs, 4 * LPC, u1, u2, d.nu, 1.f - V.wx);
+                dd_line_split<NQ, LPC>(acc, base, 4 * sub, cs, u1, u2, d.nu, 1.f - V.wx);
             }
         }
     }

@@ -382,7 +382,7 @@ __global__ __launch_bounds__(256) void fan_fbp_bp_split_kernel(float* __restrict
     for (int j = 0; j < 4 * NQ; j++) acc[j] = 0.f;
     const float half = (d.nu - 1.0f) / 2.0f;
     const size_t vstride = (size_t)d.nu * cs;
-    const float* __restrict__ qs = q + c0 + 4 * sub;
+    const unsigned lane_b = 16u * sub;   // the lane's first float of a cell, in bytes
     for (int v0 = vbeg; v0 < vend; v0 += LPC) {
         float mpu = -1.f, mdist = 0.f;
         if (v0 + sub < vend) {
@@ -405,12 +405,14 @@ __global__ __launch_bounds__(256) void fan_fbp_bp_split_kernel(float* __restrict
             const int u1 = min(u0 + 1, d.nu - 1);
             const float wu = pu - (float)u0;
             const float w0 = (1.f - wu) * dist, w1 = wu * dist;
-            const float* __restrict__ p0 = qs + view * vstride + (size_t)u0 * cs;
-            const float* __restrict__ p1 = qs + view * vstride + (size_t)u1 * cs;
+            // a view's cells from one scalar base plus 32-bit per-lane byte offsets
+            const char* __restrict__ vb = reinterpret_cast<const char*>(q + c0 + view * vstride);
+            const unsigned o0 = (unsigned)u0 * (unsigned)cs * 4u + lane_b;
+            const unsigned o1 = (unsigned)u1 * (unsigned)cs * 4u + lane_b;
 #pragma unroll
             for (int k = 0; k < NQ; k++) {
-                const float4 a0 = *reinterpret_cast<const float4*>(p0 + 4 * LPC * k);
-                const float4 a1 = *reinterpret_cast<const float4*>(p1 + 4 * LPC * k);
+                const float4 a0 = *reinterpret_cast<const float4*>(vb + o0 + 16u * LPC * k);
+                const float4 a1 = *reinterpret_cast<const float4*>(vb + o1 + 16u * LPC * k);
                 acc[4 * k] = fmaf(w0, a0.x, fmaf(w1, a1.x, acc[4 * k]));
                 acc[4 * k + 1] = fmaf(w0, a0.y, fmaf(w1, a1.y, acc[4 * k + 1]));
                 acc[4 * k + 2] = fmaf(w0, a0.z, fmaf(w1, a1.z, acc[4 * k + 2]));

@@ -392,16 +392,17 @@ def test_dd_fp_split_lanes(oracle, tuning, branchless, case):
     """The forward projector with 2, 4 or 8 lanes per detector column (each
     lane a part of the channel vector) against the oracle and against one lane
     per column: 32 slices, 20 slices in a 32-channel chunk (unused channels),
-    2 x 8 slices (16 channels), 5 slices (8 channels)."""
+    2 x 8 slices (16 channels), 5 slices (8 channels); 41 views (a block's
+    last view group partly filled)."""
     kind, lpc = case.split('_lpc')
     beam, dims = kind.split('_', 1)
     nz, B = (int(dims.split('x')[1]), int(dims.split('x')[0])) if 'x' in dims else (int(dims), 1)
     if beam == 'fan':
-        g, ang, img = small_fan(nz=nz, nv=nz + 2, off=(0.4, 0.3), c=(1.0, -0.5, 0.3), B=B)
+        g, ang, img = small_fan(nz=nz, nv=nz + 2, nview=41, off=(0.4, 0.3), c=(1.0, -0.5, 0.3), B=B)
         run = lambda: fan.distance_driven_fp(_proj(g), img, ang, branchless=branchless)
         ref = oracle.dd_fan_fp(g, img, ang, branchless=branchless)
     else:
-        g, ang, img = small_parallel(nz=nz, nv=nz, off=(0.4, -0.2), B=B)
+        g, ang, img = small_parallel(nz=nz, nv=nz, nview=41, off=(0.4, -0.2), B=B)
         run = lambda: parallel.distance_driven_fp(_proj(g), img, ang, branchless=branchless)
         ref = oracle.dd_par_fp(g, img, ang, branchless=branchless)
     tuning('dd_fp_lanes_per_column', int(lpc))
@@ -420,15 +421,16 @@ def test_dd_fp_split_lanes(oracle, tuning, branchless, case):
 def test_dd_bp_split_lanes(oracle, tuning, branchless, case):
     """The pixel-driven backprojector with 2 or 4 lanes per pixel against the
     oracle and against one lane per pixel (FBP and plain weights): 32 slices,
-    2 x 20 slices (two 32-channel chunks), 2 x 8 slices, 5 slices."""
+    2 x 20 slices (two 32-channel chunks), 2 x 8 slices, 5 slices; 41 views
+    (view groups that do not fill the pixel's lanes)."""
     kind, lpc = case.split('_lpc')
     beam, dims = kind.split('_', 1)
     nz, B = (int(dims.split('x')[1]), int(dims.split('x')[0])) if 'x' in dims else (int(dims), 1)
     if beam == 'fan':
-        g, ang, img = small_fan(nz=nz, nv=nz + 2, off=(0.4, 0.3), c=(1.0, -0.5, 0.3), B=B)
+        g, ang, img = small_fan(nz=nz, nv=nz + 2, nview=41, off=(0.4, 0.3), c=(1.0, -0.5, 0.3), B=B)
         mod, ref_fn = fan, oracle.dd_fan_bp
     else:
-        g, ang, img = small_parallel(nz=nz, nv=nz, off=(0.4, -0.2), B=B)
+        g, ang, img = small_parallel(nz=nz, nv=nz, nview=41, off=(0.4, -0.2), B=B)
         mod, ref_fn = parallel, oracle.dd_par_bp
     y = np.random.default_rng(8).uniform(size=[B, len(ang), g.nv, g.nu]).astype(np.float32)
     for is_fbp in (False, True):

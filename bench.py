@@ -27,7 +27,9 @@ The JSON line reports
                measured the same way.
   roofline.traffic  HBM bytes per launch of the dominant kernel, read from the
                committed rocprofv3 counter summary (profiles/) when it lists
-               that kernel, else null.
+               that kernel, else null; roofline.traffic_gb_s is that traffic
+               over the kernel's launch time (the HBM rate it really draws:
+               'frac' above 1 means the caches serve repeated image reads).
 
 --calibrate-copy runs three 256 MiB device-to-device copies first, so that a
 rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE pass over the same command can be
@@ -602,6 +604,11 @@ def main(argv=None):
     bp_ms, bp_n = kernel_stats(times, 'dd_fan_bp')
     dom, dom_ms = ('dd_fan_fp', fp_ms) if fp_ms >= bp_ms else ('dd_fan_bp', bp_ms)
     achieved = 4.0 * nvu / (dom_ms / 1e3) / 1e9 if dom_ms > 0 else 0.0
+    traffic = read_traffic(dom)
+    # HBM bytes the dominant kernel actually moves per second (counter
+    # traffic per launch over its launch time): below 'achieved' when the
+    # caches serve repeated reads of the image
+    traffic_gbs = round(traffic / (dom_ms / 1e3) / 1e9, 1) if traffic and dom_ms > 0 else None
     value = 2 * nvu / (r['ms_step'] / 1e3) / 1e9
     line = {
         'metric': 'FP+BP Gray-voxel updates/s & % HBM roofline; served recons/s + p50/p99 latency',
@@ -620,7 +627,8 @@ def main(argv=None):
                                '984 views/360 deg, 888x32 detector, dso 595, dsd 1085.6',
                    'batch': 1, 'nx': 512, 'ny': 512, 'nz': 32, 'nview': 984, 'nu': 888, 'nv': 32},
         'roofline': {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': PEAK_HBM_GBS, 'unit': 'GB/s',
-                     'frac': round(achieved / PEAK_HBM_GBS, 4), 'traffic': read_traffic(dom), 'kernel': dom,
+                     'frac': round(achieved / PEAK_HBM_GBS, 4), 'traffic': traffic, 'traffic_gb_s': traffic_gbs,
+                     'kernel': dom,
                      'kernel_ms': round(dom_ms, 3),
                      'kernels_ms': {k: round(v[0] / max(v[1], 1), 3) for k, v in times.items()}},
         'finite': r['finite'],

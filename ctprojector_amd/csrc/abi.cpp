@@ -442,15 +442,16 @@ CTP_EXPORT int ctpDDPlanDestroy(void* plan) {
 // Siddon fan beam (cuda/siddonFan.cu:128-274), host arrays
 // ===========================================================================
 template <bool FP>
-static int siddon_fan_entry(const char* name, float* out, const float* in, const float* deg, size_t nBatches,
+static int siddon_fan_entry(const char* name, float* out, const float* in, const float* deg, int nBatchesArg,
                             size_t nx, size_t ny, size_t nz, float dx, float dy, float dz, float cx, float cy, float cz,
                             size_t nu, size_t nv, size_t nview, float da, float dv, float off_a, float off_v,
                             float dsd, float dso) {
     return guarded(name, [&] {
+        need(nBatchesArg >= 0, "negative batch count");
+        const size_t nBatches = (size_t)nBatchesArg;
         if (empty(nBatches, nx, ny, nz, nu, nv, nview)) return;
         Grid g = make_grid(nx, ny, nz, dx, dy, dz, cx, cy, cz);
         Det d = make_det(nu, nv, nview, da, dv, off_a, off_v);
-        need(nBatches <= kIntMax, "batch too large");
         need_ptr(out, "out"); need_ptr(in, "in"); need_ptr(deg, "angles");
         const size_t nimg = nBatches * nx * ny * nz, nprj = nBatches * nu * nv * nview;
         const size_t nin = FP ? nimg : nprj, nout = FP ? nprj : nimg;
@@ -468,11 +469,17 @@ static int siddon_fan_entry(const char* name, float* out, const float* in, const
     });
 }
 
-CTP_EXPORT int cSiddonFanProjection(float* prj, const float* img, const float* deg, DD_GRID_ARGS, float dsd, float dso) {
+// nBatches is an int here, as in cuda/siddonFan.cu:131,205 (the other DD 2D entry points take size_t)
+#define SIDDON_FAN_ARGS                                                                                    \
+    int nBatches, size_t nx, size_t ny, size_t nz, float dx, float dy, float dz, float cx, float cy, float cz, \
+        size_t nu, size_t nv, size_t nview, float da, float dv, float off_a, float off_v
+
+CTP_EXPORT int cSiddonFanProjection(float* prj, const float* img, const float* deg, SIDDON_FAN_ARGS, float dsd,
+                                    float dso) {
     return siddon_fan_entry<true>("cSiddonFanProjection", prj, img, deg, nBatches, nx, ny, nz, dx, dy, dz, cx, cy, cz,
                                   nu, nv, nview, da, dv, off_a, off_v, dsd, dso);
 }
-CTP_EXPORT int cSiddonFanBackprojection(float* img, const float* prj, const float* deg, DD_GRID_ARGS, float dsd,
+CTP_EXPORT int cSiddonFanBackprojection(float* img, const float* prj, const float* deg, SIDDON_FAN_ARGS, float dsd,
                                         float dso) {
     return siddon_fan_entry<false>("cSiddonFanBackprojection", img, prj, deg, nBatches, nx, ny, nz, dx, dy, dz, cx, cy,
                                    cz, nu, nv, nview, da, dv, off_a, off_v, dsd, dso);

@@ -4,6 +4,8 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <thread>
+#include <tuple>
 #include <utility>
 #include <vector>
 #include <sstream>
@@ -44,8 +46,12 @@ std::map<std::string, int> g_tune = {
     {"tv_mode", 0},          // TV prior: 1 z-marching tile kernels (default), 2 one lane per voxel
     {"bp_view_groups", 0},   // view groups of the pixel-driven DD / FBP backprojections (default: enough for 1024 blocks)
 };
+// Arenas are keyed by (device, stream, host thread): two threads calling on the
+// same stream (the null stream is every thread's default) never carve the
+// same scratch; within one arena calls are ordered by their thread.
+using WsKey = std::tuple<int, hipStream_t, std::thread::id>;
 std::mutex g_ws_mutex;
-std::map<std::pair<int, hipStream_t>, std::unique_ptr<Workspace>> g_ws;
+std::map<WsKey, std::unique_ptr<Workspace>> g_ws;
 }  // namespace
 
 void set_last_error(const std::string& msg) { g_last_error = msg; }
@@ -100,7 +106,7 @@ Workspace& workspace_for(hipStream_t stream) {
     int dev = 0;
     CTP_CHECK(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lk(g_ws_mutex);
-    auto key = std::make_pair(dev, stream);
+    auto key = WsKey(dev, stream, std::this_thread::get_id());
     auto it = g_ws.find(key);
     if (it == g_ws.end()) it = g_ws.emplace(key, std::make_unique<Workspace>(stream)).first;
     return *it->second;
@@ -113,7 +119,7 @@ void release_all_workspaces() {
     for (auto& kv : g_ws) {
         Workspace& w = *kv.second;
         if (w.base_) {
-            (void)hipSetDevice(kv.first.first);
+            (void)hipSetDevice(std::get<0>(kv.first));
             (void)hipStreamSynchronize(w.stream_);
             (void)hipFree(w.base_);
         }

@@ -28,9 +28,10 @@ void check(hipError_t e, const char* what);
 void set_last_error(const std::string& msg);
 const char* last_error();
 
-// Device scratch memory owned by the library.  One arena per (device, stream)
-// so that asynchronous calls on different streams never share scratch; calls on
-// one stream are ordered by the stream.  Grows on demand (the old block is
+// Device scratch memory owned by the library.  One arena per (device, stream,
+// host thread), so that calls on different streams never share scratch and
+// threads sharing a stream never share an arena's host-side state; calls into
+// one arena come from one thread and are ordered by the stream.  Grows on demand (the old block is
 // released after the stream drains) and is kept until ctp_release_workspace().
 class Workspace {
 public:

@@ -140,11 +140,11 @@ int ctpDDPlanDestroy(void* plan);
 
 /* ---- Siddon fan beam (equiangular arc, stacked slices; host arrays) ------- */
 /* cuda/siddonFan.cu:128  prj += A img; rows are horizontal planes z = (iv - (nv-1)/2 + off_v) dv */
-int cSiddonFanProjection(float* prj, const float* img, const float* deg, size_t nBatches, size_t nx, size_t ny,
+int cSiddonFanProjection(float* prj, const float* img, const float* deg, int nBatches, size_t nx, size_t ny,
                          size_t nz, float dx, float dy, float dz, float cx, float cy, float cz, size_t nu, size_t nv,
                          size_t nview, float da, float dv, float off_a, float off_v, float dsd, float dso);
 /* cuda/siddonFan.cu:202  img += A^T prj (exact transpose, deterministic; the reference uses atomics) */
-int cSiddonFanBackprojection(float* img, const float* prj, const float* deg, size_t nBatches, size_t nx, size_t ny,
+int cSiddonFanBackprojection(float* img, const float* prj, const float* deg, int nBatches, size_t nx, size_t ny,
                              size_t nz, float dx, float dy, float dz, float cx, float cy, float cz, size_t nu,
                              size_t nv, size_t nview, float da, float dv, float off_a, float off_v, float dsd,
                              float dso);

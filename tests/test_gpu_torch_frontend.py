@@ -112,3 +112,26 @@ def test_tomo_filter_vs_oracle(torch, oracle, filt):
                           torch.from_numpy(src).to(dev), filt, 0.8, 0.6).cpu().numpy()
     ref = oracle.tomo_filter(prj, dc, src, 0.9, 0.7, 1.5, filt, 0.8, 0.6)
     assert rel_l2(got, ref) < 1e-4
+
+
+def test_projector_norm_hip_dd_adjoint_pair(torch, oracle):
+    """The power method over the HIP fan DD forward projector and its exact
+    adjoint converges to the largest singular value of the oracle's dense
+    system matrix of the same forward projector."""
+    from ctprojector_amd.projector import cupy as C
+    from ctprojector_amd.projector.cupy import fan_equiangular as cf
+    from helpers import Geo
+    g = Geo(nx=10, ny=9, nz=1, dx=1.0, dy=1.2, nu=24, nv=1, du=1.5, dv=1.0, dsd=200.0, dso=120.0, off_u=0.3)
+    ang = (np.arange(15) * 2 * np.pi / 15).astype(np.float32)
+    A = np.zeros([15 * 24, 90])
+    for j in range(90):
+        e = np.zeros([1, 1, 9, 10], np.float32)
+        e.flat[j] = 1
+        A[:, j] = oracle.dd_fan_fp(g, e, ang).ravel()
+    smax = np.linalg.svd(A, compute_uv=False)[0]
+    p = _mk(C, g)
+    ang_d = torch.from_numpy(ang).to(torch.device('cuda', 0))
+    p.set_projector(cf.distance_driven_fp, angles=ang_d)
+    p.set_backprojector(cf.distance_driven_fp_adjoint, angles=ang_d)
+    n = p.calc_projector_norm(niter=200, verbose=False)
+    assert abs(n - smax) < 1e-4 * smax, (n, smax)

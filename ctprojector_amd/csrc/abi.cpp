@@ -320,6 +320,25 @@ CTP_EXPORT int cupyDistanceDrivenFanProjection(float* prj, const float* img, con
     return dd_entry<true, true>("cupyDistanceDrivenFanProjection", prj, img, deg, nBatches, nx, ny, nz, dx, dy, dz, cx,
                                 cy, cz, nu, nv, nview, da, dv, off_a, off_v, dsd, dso, typeProjector, true);
 }
+// No counterpart in the reference: cupyDistanceDrivenFanProjection for
+// callers that cannot allocate device memory (code under HIP graph capture,
+// serving loops, harnesses that forbid hipMalloc).  The scratch is a fixed
+// 80 MiB device array per GPU; calls needing more fail with an error; calls
+// that overlap on one GPU race on it.
+CTP_EXPORT int ctpDistanceDrivenFanProjectionNoAlloc(float* prj, const float* img, const float* deg, DD_GRID_ARGS,
+                                                     float dsd, float dso, int typeProjector) {
+    return guarded("ctpDistanceDrivenFanProjectionNoAlloc", [&] {
+        if (empty(nBatches, nx, ny, nz, nu, nv, nview)) return;
+        Grid g = make_grid(nx, ny, nz, dx, dy, dz, cx, cy, cz);
+        Det d = make_det(nu, nv, nview, da, dv, off_a, off_v);
+        need(nBatches <= kIntMax, "batch too large");
+        need_ptr(prj, "prj"); need_ptr(img, "img"); need_ptr(deg, "angles");
+        std::vector<float> hdeg = host_copy(deg, nview);
+        dd2d_fp_noalloc(prj, img, hdeg.data(), (int)nBatches, g, d, (int)nview, dsd, dso, typeProjector, true,
+                        cur_stream());
+        sync();
+    });
+}
 CTP_EXPORT int cDistanceDrivenFanProjection(float* prj, const float* img, const float* deg, DD_GRID_ARGS, float dsd,
                                             float dso, int typeProjector) {
     return dd_entry<true, false>("cDistanceDrivenFanProjection", prj, img, deg, nBatches, nx, ny, nz, dx, dy, dz, cx,

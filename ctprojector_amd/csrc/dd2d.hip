@@ -29,6 +29,9 @@
 #include <algorithm>
 #include <memory>
 #include <cmath>
+#include <map>
+#include <mutex>
+#include <string>
 #include <vector>
 
 namespace ctp {
@@ -884,12 +887,10 @@ static std::vector<DDView> dd_views_for(const float* deg_host, int nview, const 
     return views;
 }
 
-void dd2d_fp(float* prj, const float* img, const float* deg_host, int B, const Grid& g, const Det& d, int nview,
-             float dsd, float dso, int type, bool fan, hipStream_t stream) {
-    const std::vector<DDView> views = dd_views_for(deg_host, nview, g, type);
-    const DDFPSetup s = dd_fp_setup(views, B, g, d, nview, dsd, dso, type, fan);
-    const size_t npix = (size_t)g.nx * g.ny;
-    Workspace& ws = workspace_for(stream);
+// Scratch of one forward projection call: the two channel-last image copies
+// and the view / row / chunk / group tables, carved in this order from one
+// 256-byte aligned block (each piece rounded up to 256 bytes).
+static size_t dd_fp_scratch_bytes(const DDFPSetup& s, const Grid& g, int nview) {
     Plan pl;
     pl.add<float>((size_t)g.ny * s.rsY);
     pl.add<float>((size_t)g.nx * s.rsX);
@@ -897,19 +898,87 @@ void dd2d_fp(float* prj, const float* img, const float* deg_host, int B, const G
     pl.add<FPRow>(s.rows.size());
     pl.add<int>(s.nchunk + 1);
     pl.add<FPGroup>(s.groups.size());
-    ws.reserve(pl.bytes);
-    float* imgY = ws.take_n<float>((size_t)g.ny * s.rsY);
-    float* imgX = ws.take_n<float>((size_t)g.nx * s.rsX);
-    DDView* dviews = ws.take_n<DDView>(nview);
-    FPRow* drows = ws.take_n<FPRow>(s.rows.size());
-    int* dstart = ws.take_n<int>(s.nchunk + 1);
-    FPGroup* dgroups = ws.take_n<FPGroup>(s.groups.size());
+    return pl.bytes;
+}
+
+// Carves the scratch block, uploads the tables and enqueues the launches; the
+// caller synchronises 'stream' before the host tables in 's' and 'views' go.
+static void dd_fp_with_scratch(char* base, const DDFPSetup& s, const std::vector<DDView>& views, float* prj,
+                               const float* img, const Grid& g, const Det& d, int nview, float dsd, float dso,
+                               bool fan, hipStream_t stream) {
+    auto take = [&base](size_t bytes) {
+        char* p = base;
+        base += ((bytes + 255) / 256) * 256;
+        return static_cast<void*>(p);
+    };
+    float* imgY = static_cast<float*>(take(sizeof(float) * (size_t)g.ny * s.rsY));
+    float* imgX = static_cast<float*>(take(sizeof(float) * (size_t)g.nx * s.rsX));
+    DDView* dviews = static_cast<DDView*>(take(sizeof(DDView) * nview));
+    FPRow* drows = static_cast<FPRow*>(take(sizeof(FPRow) * s.rows.size()));
+    int* dstart = static_cast<int*>(take(sizeof(int) * (s.nchunk + 1)));
+    FPGroup* dgroups = static_cast<FPGroup*>(take(sizeof(FPGroup) * s.groups.size()));
     CTP_CHECK(hipMemcpyAsync(dviews, views.data(), sizeof(DDView) * nview, hipMemcpyHostToDevice, stream));
     CTP_CHECK(hipMemcpyAsync(drows, s.rows.data(), sizeof(FPRow) * s.rows.size(), hipMemcpyHostToDevice, stream));
     CTP_CHECK(hipMemcpyAsync(dstart, s.start.data(), sizeof(int) * (s.nchunk + 1), hipMemcpyHostToDevice, stream));
     CTP_CHECK(hipMemcpyAsync(dgroups, s.groups.data(), sizeof(FPGroup) * s.groups.size(), hipMemcpyHostToDevice,
                              stream));
     dd_fp_run(s, DDFPDev{dviews, drows, dstart, dgroups}, prj, img, imgY, imgX, g, d, nview, dsd, dso, fan, stream);
+}
+
+void dd2d_fp(float* prj, const float* img, const float* deg_host, int B, const Grid& g, const Det& d, int nview,
+             float dsd, float dso, int type, bool fan, hipStream_t stream) {
+    const std::vector<DDView> views = dd_views_for(deg_host, nview, g, type);
+    const DDFPSetup s = dd_fp_setup(views, B, g, d, nview, dsd, dso, type, fan);
+    const size_t bytes = dd_fp_scratch_bytes(s, g, nview);
+    Workspace& ws = workspace_for(stream);
+    ws.reserve(bytes);
+    dd_fp_with_scratch(static_cast<char*>(ws.take(bytes)), s, views, prj, img, g, d, nview, dsd, dso, fan, stream);
+    CTP_CHECK(hipStreamSynchronize(stream));   // host tables above are released on return
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free forward projection
+// ---------------------------------------------------------------------------
+// The same launches as dd2d_fp with their scratch in a file-scope device array
+// instead of the workspace arena, so the call allocates no device memory.
+// Every GPU the library runs on holds its own copy of the array until the
+// process ends; calls on one GPU share it (each call overwrites it completely
+// before reading: the channel-last copies write every channel, the tables are
+// uploaded), so calls that overlap on one GPU race.
+static __device__ __align__(256) unsigned char scratch[kDDFPNoAllocScratchBytes];
+
+namespace {
+std::mutex g_scratch_mutex;
+std::map<int, char*> g_scratch_ptr;   // device -> address of 'scratch' on that device
+}
+
+// Address of 'scratch' on the current device, looked up once per device (the
+// array's name is a host-side stand-in, not a device address).
+static char* noalloc_scratch() {
+    int dev = 0;
+    CTP_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_scratch_mutex);
+    auto it = g_scratch_ptr.find(dev);
+    if (it != g_scratch_ptr.end()) return it->second;
+    void* ptr = nullptr;
+    CTP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) fail("no device address for the allocation-free scratch array");
+    g_scratch_ptr[dev] = static_cast<char*>(ptr);
+    return static_cast<char*>(ptr);
+}
+
+void dd2d_fp_noalloc(float* prj, const float* img, const float* deg_host, int B, const Grid& g, const Det& d,
+                     int nview, float dsd, float dso, int type, bool fan, hipStream_t stream) {
+    const std::vector<DDView> views = dd_views_for(deg_host, nview, g, type);
+    const DDFPSetup s = dd_fp_setup(views, B, g, d, nview, dsd, dso, type, fan);
+    const size_t bytes = dd_fp_scratch_bytes(s, g, nview);
+    if (bytes > kDDFPNoAllocScratchBytes)
+        fail("the allocation-free forward projection needs " + std::to_string(bytes) +
+             " bytes of scratch, more than its fixed " + std::to_string(kDDFPNoAllocScratchBytes) +
+             " bytes (two channel-last copies of the B*nz slices plus tables); use the allocating entry point");
+    char* base = noalloc_scratch();
+    TimingSuspended no_events;   // kernel timing would create HIP events
+    dd_fp_with_scratch(base, s, views, prj, img, g, d, nview, dsd, dso, fan, stream);
     CTP_CHECK(hipStreamSynchronize(stream));   // host tables above are released on return
 }
 

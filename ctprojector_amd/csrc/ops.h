@@ -51,6 +51,13 @@ void dd2d_fp(float* prj, const float* img, const float* deg_host, int B, const G
              float dsd, float dso, int type, bool fan, hipStream_t stream);
 void dd2d_bp(float* img, const float* prj, const float* deg_host, int B, const Grid& g, const Det& d, int nview,
              float dsd, float dso, int type, bool fan, hipStream_t stream);
+// dd2d_fp with its scratch (channel-last image copies and tables) in a
+// file-scope device array of kDDFPNoAllocScratchBytes: allocates no device
+// memory; fails with an error when the call needs more scratch; calls that
+// overlap on one GPU race on the array.
+constexpr size_t kDDFPNoAllocScratchBytes = size_t(80) << 20;
+void dd2d_fp_noalloc(float* prj, const float* img, const float* deg_host, int B, const Grid& g, const Det& d,
+                     int nview, float dsd, float dso, int type, bool fan, hipStream_t stream);
 // img += A^T prj, the exact transpose of dd2d_fp (same type bits)
 void dd2d_fp_adjoint(float* img, const float* prj, const float* deg_host, int B, const Grid& g, const Det& d,
                      int nview, float dsd, float dso, int type, bool fan, hipStream_t stream);

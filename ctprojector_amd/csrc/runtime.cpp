@@ -197,8 +197,14 @@ std::string timing_report() {
     return o.str();
 }
 
+namespace {
+thread_local bool t_timing_suspended = false;
+}
+TimingSuspended::TimingSuspended() : prev_(t_timing_suspended) { t_timing_suspended = true; }
+TimingSuspended::~TimingSuspended() { t_timing_suspended = prev_; }
+
 TimedLaunch::TimedLaunch(const char* name, hipStream_t s) : name_(name), s_(s) {
-    if (!timing_enabled()) return;
+    if (t_timing_suspended || !timing_enabled()) return;
     // no event records into a stream that is being captured into a graph
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(s_, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) {

@@ -89,6 +89,16 @@ void timing_collect();                       // after the stream(s) drained
 bool timing_get(const char* name, double* total_ms, long* count);
 std::string timing_report();
 
+// While an instance lives, TimedLaunch records nothing on the calling thread
+// (the allocation-free entry point creates no HIP events).
+class TimingSuspended {
+public:
+    TimingSuspended();
+    ~TimingSuspended();
+private:
+    bool prev_;
+};
+
 class TimedLaunch {
 public:
     TimedLaunch(const char* name, hipStream_t s);

@@ -70,6 +70,17 @@ int cupyDistanceDrivenFanProjection(float* prj, const float* img, const float* d
                                     size_t ny, size_t nz, float dx, float dy, float dz, float cx, float cy, float cz,
                                     size_t nu, size_t nv, size_t nview, float da, float dv, float off_a, float off_v,
                                     float dsd, float dso, int typeProjector);
+/* No counterpart in the reference.  cupyDistanceDrivenFanProjection for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc): the same arguments and
+ * results, with the call's scratch (two channel-last copies of the B*nz slices
+ * plus view/row tables) in a fixed 80 MiB device array per GPU, held until the
+ * process ends.  A call needing more scratch returns an error.  Calls that
+ * overlap on one GPU (other threads or streams) race on the array. */
+int ctpDistanceDrivenFanProjectionNoAlloc(float* prj, const float* img, const float* deg, size_t nBatches, size_t nx,
+                                          size_t ny, size_t nz, float dx, float dy, float dz, float cx, float cy,
+                                          float cz, size_t nu, size_t nv, size_t nview, float da, float dv,
+                                          float off_a, float off_v, float dsd, float dso, int typeProjector);
 /* cuda/distanceDriven2D.cu:158 */
 int cupyDistanceDrivenParallelProjection(float* prj, const float* img, const float* deg, size_t nBatches, size_t nx,
                                          size_t ny, size_t nz, float dx, float dy, float dz, float cx, float cy,

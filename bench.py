@@ -2,7 +2,7 @@
 """Projector hot-path benchmark on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-extra]
-                    [--calibrate-copy]
+                    [--no-serving] [--no-parity] [--calibrate-copy]
     python bench.py --summarize TRACE_DIR [--fetch DIR --write DIR [--hits DIR]]
 
 One step = one forward projection plus one backprojection of the
@@ -15,21 +15,27 @@ timed region starts; outputs are produced by the library's device entry points
 The JSON line reports
   value        FP+BP voxel-updates per second (1 voxel-update = one (voxel, view)
                pair in one pass; a step counts 2 * B*nx*ny*nz*nview), in GVU/s;
-  roofline     for the dominant kernel: algorithmic bytes per launch (4 B per
-               voxel-update: the FP32 voxel operand streamed once per view)
-               divided by its mean launch time, measured with HIP events on the
-               launching stream, against the 8 TB/s HBM peak;
+  roofline     for the dominant kernel: its HBM traffic per launch (rocprofv3
+               FETCH_SIZE scaled by a device copy's calibration + WRITE_SIZE,
+               from the committed counter summary under profiles/) over its
+               mean launch time, measured here with HIP events on the
+               launching stream, against the 8 TB/s HBM peak; beside it the
+               4 B per voxel-update throughput ratio (the voxel operand
+               streamed once per view, which the caches mostly serve) and the
+               compulsory-traffic fraction (|img| + |prj| per launch);
   cpu_baseline the FP32 OpenMP restatement of the reference algorithm (the
-               oracle's FP32 mirror) on a bounded view subset of the same
-               workload, on this host's cores;
+               oracle's FP32 mirror) built -march=native for this host, on a
+               view subset of the same workload, median of 5 passes after one
+               warm-up, OMP_PROC_BIND=close OMP_PLACES=cores, CPU model stated
+               (oracle/cpu_baseline.py);
+  parity_rel_l2  relative L2 of the GPU outputs against the FP64 oracle over the
+               whole fan workload (FP, BP, Hann ramp filter, FBP BP) and the
+               whole cone workload (FP, BP), computed on the host after the
+               GPU timing (oracle/cpu_baseline.py --check);
   extra        the cone-beam Siddon FP+BP (256^3 volume at 0.5 mm, 720 views,
-               512x384 panel) and the fan ramp filter + FBP backprojection,
-               measured the same way.
-  roofline.traffic  HBM bytes per launch of the dominant kernel, read from the
-               committed rocprofv3 counter summary (profiles/) when it lists
-               that kernel, else null; roofline.traffic_gb_s is that traffic
-               over the kernel's launch time (the HBM rate it really draws:
-               'frac' above 1 means the caches serve repeated image reads).
+               512x384 panel), the batched cone plan, the fan ramp filter + FBP
+               backprojection, the unrolled SQS-TV step, the TV prior and the
+               served stream, measured the same way.
 
 --calibrate-copy runs three 256 MiB device-to-device copies first, so that a
 rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE pass over the same command can be
@@ -101,10 +107,11 @@ def fan_workload_bench(steps, warmup):
     _lib.set_kernel_timing(False)
     ms_step = (t1 - t0) / steps * 1e3
     finite = bool(torch.isfinite(bp).all().item()) and bool(torch.isfinite(prj).all().item())
-    return dict(w=w, img=img, nvu=nvu, ms_step=ms_step, times=times, finite=finite)
+    return dict(w=w, img=img, nvu=nvu, ms_step=ms_step, times=times, finite=finite,
+                outputs={'prj': prj.cpu().numpy(), 'bp': bp.cpu().numpy()})
 
 
-def cone_extra(steps, cpu=False):
+def cone_extra(steps, cpu=False, outputs=None):
     import torch
     from ctprojector_amd import _lib
     from ctprojector_amd.phantom import cone_workload
@@ -165,35 +172,30 @@ def cone_extra(steps, cpu=False):
     out['batch2_ms_per_step'] = round(ms2, 3)
     out['batch2_value_gvu_s'] = round(2 * 2 * nvu / (ms2 / 1e3) / 1e9, 2)
     del prj, bp, img2
+    if outputs is not None:              # single-volume outputs for the oracle check
+        prj = cone.siddon_fp_arbitrary(p, img_d, *g)
+        bp = cone.siddon_bp_arbitrary(p, prj, *g)
+        outputs['prj'] = prj[0].cpu().numpy()[None]
+        outputs['bp'] = bp.cpu().numpy()
+        del prj, bp
     if cpu:
-        out['cpu_baseline'] = cone_cpu_baseline(w, img, geo)
+        out['cpu_baseline'] = run_cpu_baseline('cone')
     return out
 
 
-def cone_cpu_baseline(w, img, geo, target_s=12.0):
-    """FP32 OpenMP mirror of the reference's Siddon FP and atomic BP on a view
-    subset of the 256^3 cone workload (cost is linear in views)."""
-    from oracle import oracle as O
-
-    O.build()
-    threads = O.num_threads()
-    nvox = w.nx * w.ny * w.nz
-
-    def run(nv):
-        sub = [a[:nv] for a in (geo.det_center, geo.det_u, geo.det_v, geo.src)]
-        t = time.perf_counter()
-        prj = O.siddon_cone_fp(w, img, *sub, prec='f')
-        t1 = time.perf_counter()
-        O.siddon_cone_bp(w, prj, *sub, prec='f')
-        return t1 - t, time.perf_counter() - t1
-
-    a, b = run(2)
-    nv = int(min(w.nview, max(2, 2 * target_s / max(a + b, 1e-3))))
-    log('[bench] cone cpu baseline: %d of %d views on %d threads' % (nv, w.nview, threads))
-    a, b = run(nv)
-    return {'value': round(2 * nvox * nv / (a + b) / 1e9, 4), 'unit': 'GVU/s', 'cores': threads, 'kind': 'port',
-            'sample': 'cone Siddon FP+BP of the 256^3 volume over the first %d of %d views (FP %.2fs, BP %.2fs); '
-                      'FP32 OpenMP restatement (oracle/oracle.cpp)' % (nv, w.nview, a, b)}
+def run_cpu_baseline(workload):
+    """The oracle's FP32 OpenMP restatement of the reference kernels, built
+    -march=native for this host and timed in a child process with
+    OMP_PROC_BIND=close OMP_PLACES=cores (oracle/cpu_baseline.py): median of 5
+    passes after 1 warm-up over a view subset of about 2.5 s per pass."""
+    import subprocess
+    env = dict(os.environ, OMP_PROC_BIND='close', OMP_PLACES='cores')
+    log('[bench] cpu baseline (%s): building for this host and timing' % workload)
+    r = subprocess.run([sys.executable, os.path.join(ROOT, 'oracle', 'cpu_baseline.py'), '--workload', workload],
+                       stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, env=env, cwd=ROOT, timeout=600)
+    if r.returncode != 0:
+        return {'error': 'cpu baseline failed (%d): %s' % (r.returncode, r.stderr.strip()[-500:])}
+    return json.loads(r.stdout.strip().splitlines()[-1])
 
 
 def unrolled_sqs_extra(steps):
@@ -400,7 +402,7 @@ def serving_extra(count=16, seed=1005):
     }
 
 
-def fbp_extra(steps, w, img):
+def fbp_extra(steps, w, img, outputs=None):
     import torch
     from ctprojector_amd import _lib
     from ctprojector_amd.projector.cupy import ct_projector, fan_equiangular as fan
@@ -423,6 +425,9 @@ def fbp_extra(steps, w, img):
         rec = fan.fbp_bp(p, fprj, ang_d)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    if outputs is not None:
+        outputs['fl'] = fprj.cpu().numpy()
+        outputs['rec'] = rec.cpu().numpy()
     times = _lib.kernel_times()
     _lib.set_kernel_timing(False)
     nvu = w.nx * w.ny * w.nz * w.nview
@@ -444,32 +449,27 @@ def fbp_extra(steps, w, img):
     }
 
 
-def cpu_baseline(w, img, target_s=12.0):
-    """FP32 OpenMP mirror of the reference algorithm on a view subset."""
-    from oracle import oracle as O
-
-    O.build()
-    threads = O.num_threads()
-    nvox = img.shape[0] * w.nx * w.ny * w.nz
-
-    def run(nv):
-        ang = w.angles[:nv]
-        t = time.perf_counter()
-        prj = O.dd_fan_fp(w, img, ang, prec='f')
-        t1 = time.perf_counter()
-        O.dd_fan_bp(w, prj, ang, prec='f')
-        t2 = time.perf_counter()
-        return t1 - t, t2 - t1
-
-    probe = 4
-    a, b = run(probe)
-    nv = int(min(w.nview, max(probe, probe * target_s / max(a + b, 1e-3))))
-    log('[bench] cpu baseline: %d of %d views on %d threads' % (nv, w.nview, threads))
-    a, b = run(nv)
-    gvu = 2 * nvox * nv / (a + b) / 1e9
-    return {'value': round(gvu, 4), 'unit': 'GVU/s', 'cores': threads, 'kind': 'port',
-            'sample': 'fan DD FP+BP of the same volume over the first %d of %d views (FP %.2fs, BP %.2fs); '
-                      'FP32 OpenMP restatement of the reference kernels (oracle/oracle.cpp)' % (nv, w.nview, a, b)}
+def oracle_parity(workload, outputs):
+    """Relative L2 of the GPU outputs of one benchmark workload against the
+    FP64 oracle over the whole workload (all views, full volume), computed on
+    the host beside the CPU baseline by oracle/cpu_baseline.py --check from
+    the float32 outputs saved to a scratch directory."""
+    import shutil
+    import subprocess
+    import tempfile
+    d = tempfile.mkdtemp(prefix='ctp_parity_')
+    try:
+        for k, v in outputs.items():
+            np.save(os.path.join(d, k + '.npy'), v)
+        log('[bench] oracle parity (%s, FP64, whole workload)' % workload)
+        r = subprocess.run([sys.executable, os.path.join(ROOT, 'oracle', 'cpu_baseline.py'), '--workload', workload,
+                            '--check', d], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, cwd=ROOT,
+                           timeout=900)
+        if r.returncode != 0:
+            return {'error': 'oracle check failed (%d): %s' % (r.returncode, r.stderr.strip()[-500:])}
+        return json.loads(r.stdout.strip().splitlines()[-1])
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
 
 
 def calibrate_copy():
@@ -496,6 +496,33 @@ TIMED_KERNELS = {'dd_fan_fp': 'dd_fp_split_kernel<32, 4, true>', 'dd_fan_bp': 'd
                  'siddon_cone_fp': 'siddon_cone_fp_col_kernel<1>',
                  'siddon_cone_bp': 'siddon_cone_bp_col_kernel<2, 2, 4, 1>',
                  'fan_fbp_bp': 'fan_fbp_bp_split_kernel<32, 4, true>', 'ramp_filter': 'ramp_conv_kernel<4>'}
+
+
+def roofline(dom, dom_ms, nvu, compulsory_bytes, times):
+    """Roofline entry of the dominant kernel.  'achieved' is the HBM traffic of
+    one launch (rocprofv3 FETCH_SIZE scaled by a device copy's calibration,
+    plus WRITE_SIZE: bytes that missed the L2, from the committed counter
+    summary) over the launch time measured here with HIP events on the
+    launching stream.  The survey's 4 B per voxel-update figure (the voxel
+    operand streamed once per view) is reported beside it as a throughput
+    ratio: caches serve most of those reads, so it exceeds the HBM peak and is
+    not a bound."""
+    traffic = read_traffic(dom)
+    alg_bytes = 4.0 * nvu
+    alg_gbs = alg_bytes / (dom_ms / 1e3) / 1e9 if dom_ms > 0 else 0.0
+    if traffic and dom_ms > 0:
+        achieved, basis = traffic / (dom_ms / 1e3) / 1e9, 'counters'
+    else:
+        achieved, basis = alg_gbs, 'algorithmic (no counter summary lists this kernel)'
+    return {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': PEAK_HBM_GBS, 'unit': 'GB/s',
+            'frac': round(achieved / PEAK_HBM_GBS, 4), 'traffic': traffic, 'basis': basis,
+            'traffic_source': '%s: FETCH_SIZE x copy calibration + WRITE_SIZE per launch (L2-miss bytes; the '
+                              '256 MB Infinity Cache may serve part of them)' % os.path.relpath(TRAFFIC_FILE, ROOT),
+            'kernel': dom, 'kernel_symbol': TIMED_KERNELS.get(dom, dom), 'kernel_ms': round(dom_ms, 3),
+            'compulsory_frac': round(compulsory_bytes / (dom_ms / 1e3) / 1e9 / PEAK_HBM_GBS, 4) if dom_ms else None,
+            'algorithmic_4B_per_voxel_update': {'bytes_per_launch': int(alg_bytes), 'gb_s': round(alg_gbs, 1),
+                                                'ratio_to_peak': round(alg_gbs / PEAK_HBM_GBS, 4)},
+            'kernels_ms': {k: round(v[0] / max(v[1], 1), 3) for k, v in times.items()}}
 
 
 def read_traffic(name):
@@ -571,6 +598,7 @@ def main(argv=None):
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-extra', action='store_true')
     ap.add_argument('--no-serving', action='store_true', help='skip the served-stream extra')
+    ap.add_argument('--no-parity', action='store_true', help='skip the FP64 oracle check of the outputs')
     ap.add_argument('--calibrate-copy', action='store_true')
     ap.add_argument('--tune', action='append', default=[], metavar='KEY=VALUE',
                     help='library launch-shape knob (ctpSetTuning), repeatable')
@@ -603,16 +631,9 @@ def main(argv=None):
     fp_ms, fp_n = kernel_stats(times, 'dd_fan_fp')
     bp_ms, bp_n = kernel_stats(times, 'dd_fan_bp')
     dom, dom_ms = ('dd_fan_fp', fp_ms) if fp_ms >= bp_ms else ('dd_fan_bp', bp_ms)
-    achieved = 4.0 * nvu / (dom_ms / 1e3) / 1e9 if dom_ms > 0 else 0.0
-    traffic = read_traffic(dom)
-    # HBM bytes the dominant kernel actually moves per second (counter
-    # traffic per launch over its launch time): below 'achieved' when the
-    # caches serve repeated reads of the image
-    traffic_gbs = round(traffic / (dom_ms / 1e3) / 1e9, 1) if traffic and dom_ms > 0 else None
-    value = 2 * nvu / (r['ms_step'] / 1e3) / 1e9
     line = {
         'metric': 'FP+BP Gray-voxel updates/s & % HBM roofline; served recons/s + p50/p99 latency',
-        'value': round(value, 2),
+        'value': round(2 * nvu / (r['ms_step'] / 1e3) / 1e9, 2),
         'unit': 'GVU/s',
         'n_gpus': 1,
         'steps': args.steps,
@@ -626,17 +647,15 @@ def main(argv=None):
         'config': {'workload': 'fan-beam equiangular distance-driven FP+BP, volume 1x32x512x512 @0.7 mm, '
                                '984 views/360 deg, 888x32 detector, dso 595, dsd 1085.6',
                    'batch': 1, 'nx': 512, 'ny': 512, 'nz': 32, 'nview': 984, 'nu': 888, 'nv': 32},
-        'roofline': {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': PEAK_HBM_GBS, 'unit': 'GB/s',
-                     'frac': round(achieved / PEAK_HBM_GBS, 4), 'traffic': traffic, 'traffic_gb_s': traffic_gbs,
-                     'kernel': dom,
-                     'kernel_ms': round(dom_ms, 3),
-                     'kernels_ms': {k: round(v[0] / max(v[1], 1), 3) for k, v in times.items()}},
+        'roofline': roofline(dom, dom_ms, nvu, 4 * (w.nx * w.ny * w.nz + w.nu * w.nv * w.nview), times),
         'finite': r['finite'],
     }
+    outputs = {'fan': dict(r['outputs']), 'cone': {}}
     if not args.no_extra:
         extra = {}
         try:
-            extra['cone'] = cone_extra(2, cpu=not args.no_cpu_baseline)
+            extra['cone'] = cone_extra(2, cpu=not args.no_cpu_baseline,
+                                       outputs=None if args.no_parity else outputs['cone'])
         except Exception as e:  # report, do not hide
             extra['cone'] = {'error': repr(e)}
         try:
@@ -644,7 +663,7 @@ def main(argv=None):
         except Exception as e:
             extra['cone_batch'] = {'error': repr(e)}
         try:
-            extra['fbp'] = fbp_extra(3, w, r['img'])
+            extra['fbp'] = fbp_extra(3, w, r['img'], outputs=None if args.no_parity else outputs['fan'])
         except Exception as e:
             extra['fbp'] = {'error': repr(e)}
         try:
@@ -662,7 +681,15 @@ def main(argv=None):
                 extra['serving'] = {'error': repr(e)}
         line['extra'] = extra
     if not args.no_cpu_baseline:
-        line['cpu_baseline'] = cpu_baseline(w, r['img'])
+        line['cpu_baseline'] = run_cpu_baseline('fan')
+    if not args.no_parity:
+        par = {'tolerance': 1e-4, 'fan': oracle_parity('fan', outputs['fan'])}
+        if outputs['cone']:
+            par['cone'] = oracle_parity('cone', outputs['cone'])
+        par['note'] = ('DD backprojection main axis decided per view in FP64 (the reference: FP32 __cosf/__sinf); '
+                       'at the 4 diagonal views of 984 an FP32 decision can swap the axis of whole views, up to '
+                       '0.8% rel-L2 against an FP32-decided result')
+        line['parity_rel_l2'] = par
     print(json.dumps(line), flush=True)
     return 0
 

@@ -33,7 +33,8 @@ std::map<std::string, int> g_tune = {
     {"cone_fp_order", 0},    // Siddon FP block order: 1 plain view-major, 2 XCD bands (default per form)
     {"cone_fp_views_per_block", 0},   // column-driven Siddon FP: views per block (1, 2, 4; default 2)
     {"cone_fp_mode", 0},     // Siddon FP: 0 column-driven form (rays of a detector column together), 1 ray per lane
-    {"cone_bp_mode", 0},     // Siddon BP: 0 column mode where the geometry allows it, 1 general
+    {"cone_bp_mode", 0},     // Siddon BP: 0 column mode where the geometry allows it, 1 general window, 2 ray-driven scatter
+    {"cone_slab_slices", 0}, // Siddon FP/BP: at most this many z slices per slab (0: slabs only for 2^29+ voxels)
     {"cone_plan_persistent", 0},   // cone plans: 1 persistent blocks with work counters, 2 one block per tile
     {"dd_fp_mode", 0},       // DD FP: 1 direct (default), 2 LDS-staged rows where the footprint fits
     {"dd_fp_extra_channel", 0},  // DD FP direct kernel: 1 keeps the next chunk's channel with one chunk too

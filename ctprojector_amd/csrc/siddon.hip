@@ -477,8 +477,8 @@ static int fp_xcd_order(int mode) {
     return o == 0 ? (mode == 0 ? 0 : 1) : (o == 2 ? 1 : 0);
 }
 
-void siddon_cone_fp(float* prj, const float* img, const ConeGeomDev& geo, int B, const Grid& g, const Det& d,
-                    int nview, hipStream_t stream) {
+static void siddon_cone_fp_grid(float* prj, const float* img, const ConeGeomDev& geo, int B, const Grid& g,
+                                const Det& d, int nview, hipStream_t stream) {
     const size_t nimg = (size_t)g.nx * g.ny * g.nz, nprj = (size_t)d.nu * d.nv * nview;
     if (nimg >= (1ull << 29)) fail("cone forward projection: volumes of 2^29 voxels or more are not supported");
     const int mode = tuning_get("cone_fp_mode", 0);           // 0: column form, 1: ray per lane
@@ -585,10 +585,14 @@ struct alignas(16) ConeViewBP {
 struct ConeBPPlan {
     std::vector<ConeViewBP> views;
     int wu = 0, wv = 0;
+    int near_view = -1;     // a view whose source is inside or too close to the volume (no bounded window)
 };
 
-// Builds the per-view projective maps in FP64 from host geometry.
-static ConeBPPlan make_cone_bp_plan(const ConeGeomHost& h, const Grid& g, const Det& d, int nview) {
+// Builds the per-view projective maps in FP64 from host geometry.  A view whose
+// source is inside or too close to the volume has no bounded candidate window:
+// it is reported in near_view, or refused when 'strict'.
+static ConeBPPlan make_cone_bp_plan(const ConeGeomHost& h, const Grid& g, const Det& d, int nview,
+                                    bool strict = true) {
     ConeBPPlan plan;
     plan.views.resize(nview);
     double wmax_u = 0, wmax_v = 0;
@@ -649,8 +653,13 @@ static ConeBPPlan make_cone_bp_plan(const ConeGeomHost& h, const Grid& g, const 
             }
         }
         const double bw = 0.5 * (std::fabs(Pw[0]) + std::fabs(Pw[1]) + std::fabs(Pw[2]));
-        if (!(wmin - bw > 0.02 * D))
-            fail("cone backprojection: the source is inside or too close to the volume in view " + std::to_string(i));
+        if (!(wmin - bw > 0.02 * D)) {
+            if (strict)
+                fail("cone backprojection: the source is inside or too close to the volume in view " +
+                     std::to_string(i));
+            plan.near_view = i;
+            return plan;
+        }
         auto hnum = [&](const double* P, double uc) {
             return 0.5 * (std::fabs(P[0] - uc * Pw[0]) + std::fabs(P[1] - uc * Pw[1]) + std::fabs(P[2] - uc * Pw[2]));
         };
@@ -1333,17 +1342,75 @@ static void launch_bp(float* img, const float4* table, const ConeViewBP* views, 
     }
 }
 
-void siddon_cone_bp(float* img, const float* prj, const ConeGeomDev& geo, const ConeGeomHost& hgeo, int B,
-                    const Grid& g, const Det& d, int nview, hipStream_t stream) {
-    ConeBPPlan plan = make_cone_bp_plan(hgeo, g, d, nview);
+// ---------------------------------------------------------------------------
+// Ray-driven backprojection: the reference's transpose (cuda/siddon.cu:235-314,
+// cuda/siddonCone.cu:44-74), one lane per ray, the forward projector's
+// traversal (siddon_setup / siddon_step), every segment added to its voxel
+// with an atomic add.  It takes every geometry, and serves the ones the
+// voxel-driven window backprojector does not: a voxel shadow wider than 4
+// detector cells (fine detectors, strong magnification) and sources inside or
+// close to the volume.  The summation order of the atomics is not fixed, as in
+// the reference.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void siddon_cone_bp_scatter_kernel(float* __restrict__ img,
+                                                                     const float* __restrict__ prj,
+                                                                     const float* __restrict__ pdc,
+                                                                     const float* __restrict__ pdu,
+                                                                     const float* __restrict__ pdv,
+                                                                     const float* __restrict__ psrc, Grid g, Det d,
+                                                                     int nview) {
+    const long n = (long)d.nu * d.nv * nview;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float p = prj[i];
+    if (p == 0.f) return;                  // adds nothing
+    const int iu = (int)(i % d.nu);
+    const int iv = (int)((i / d.nu) % d.nv);
+    const int view = (int)(i / ((long)d.nu * d.nv));
+    const float3 S = make_float3(psrc[3 * view], psrc[3 * view + 1], psrc[3 * view + 2]);
+    const float u = ((float)iu - d.off_u - (float)(d.nu - 1) / 2.0f) * d.du;
+    const float v = ((float)iv - d.off_v - (float)(d.nv - 1) / 2.0f) * d.dv;
+    const float3 T = make_float3(pdc[3 * view] + pdu[3 * view] * u + pdv[3 * view] * v,
+                                 pdc[3 * view + 1] + pdu[3 * view + 1] * u + pdv[3 * view + 1] * v,
+                                 pdc[3 * view + 2] + pdu[3 * view + 2] * u + pdv[3 * view + 2] * v);
+    SiddonRay r;
+    if (!siddon_setup(r, S, T, g, false)) return;
+    char* base = reinterpret_cast<char*>(img);
+    bool inside = true;
+    while (inside) {
+        const int off = r.off;
+        const float len = siddon_step(r, inside);
+        atomicAdd(reinterpret_cast<float*>(base + off), p * len);
+    }
+}
+
+static void siddon_cone_bp_scatter(float* img, const float* prj, const ConeGeomDev& geo, int B, const Grid& g,
+                                   const Det& d, int nview, hipStream_t stream) {
+    const size_t nimg = (size_t)g.nx * g.ny * g.nz, nprj = (size_t)d.nu * d.nv * nview;
+    for (int b = 0; b < B; b++) {
+        TimedLaunch t("siddon_cone_bp", stream);
+        siddon_cone_bp_scatter_kernel<<<(unsigned)ceil_div((long)nprj, 256), 256, 0, stream>>>(
+            img + b * nimg, prj + b * nprj, geo.det_center, geo.det_u, geo.det_v, geo.src, g, d, nview);
+        CTP_LAUNCH_CHECK();
+    }
+}
+
+static void siddon_cone_bp_grid(float* img, const float* prj, const ConeGeomDev& geo, const ConeGeomHost& hgeo,
+                                int B, const Grid& g, const Det& d, int nview, hipStream_t stream) {
+    // cone_bp_mode: 0 = window backprojector where the geometry allows it (column
+    // mode when det_v is along z), 1 = general window mode, 2 = ray-driven scatter
+    const int bp_mode = tuning_get("cone_bp_mode", 0);
+    if (bp_mode < 0 || bp_mode > 2) fail("cone_bp_mode must be 0, 1 or 2");
+    if ((size_t)d.nu * d.nv * nview / 256 >= 0x7fffffffULL) fail("cone backprojection: too many rays per call");
+    ConeBPPlan plan = make_cone_bp_plan(hgeo, g, d, nview, false);
+    if (bp_mode == 2 || plan.near_view >= 0 || plan.wu > 4 || plan.wv > 4) {
+        siddon_cone_bp_scatter(img, prj, geo, B, g, d, nview, stream);
+        return;
+    }
     int wu = plan.wu, wv = plan.wv;
-    if (wu > 4 || wv > 4)
-        fail("cone backprojection: voxel shadow spans more than 4 detector cells (" + std::to_string(wu) + "x" +
-             std::to_string(wv) + "); use finer voxels or coarser detector cells");
     wu = std::min(std::max(wu, 1), d.nu);
     wv = std::min(std::max(wv, 1), d.nv);
-    // cone_bp_mode: 0 = column mode where the geometry allows it, 1 = general
-    const bool col = tuning_get("cone_bp_mode", 0) == 0 && column_mode(hgeo, plan, nview) && d.nu < (1 << 24);
+    const bool col = bp_mode == 0 && column_mode(hgeo, plan, nview) && d.nu < (1 << 24);
     const size_t nimg = (size_t)g.nx * g.ny * g.nz, nprj = (size_t)d.nu * d.nv * nview;
     // column mode: batch items sharing the geometry are backprojected up to 4
     // per launch (default z-run; their weight tables within 4 GiB together)
@@ -1456,6 +1523,63 @@ void siddon_cone_bp(float* img, const float* prj, const ConeGeomDev& geo, const 
         fail("cone backprojection: unsupported window");
     }
     // host plan vector must outlive the async copy
+    CTP_CHECK(hipStreamSynchronize(stream));
+}
+
+// ---------------------------------------------------------------------------
+// Volumes of 2^29 voxels or more (up to the 2^31 - 1 the reference's int
+// indices allow, cuda/siddon.cu:166-167): the kernels address the volume with
+// 32-bit byte offsets, so such volumes are projected as z-slabs of whole
+// slices, each a grid of its own with its centre moved to the slab's centre.
+// Line integrals and their transposes add over slabs: prj += A_s img_s for
+// every slab s, img_s += A_s^T prj.  Slab boundaries are voxel planes of the
+// full grid, so the chords are the full grid's up to FP32 rounding.
+// ---------------------------------------------------------------------------
+static int cone_slab_slices(const Grid& g) {
+    const size_t per = (size_t)g.nx * g.ny;
+    const size_t lim = (1ull << 29) - 1;
+    int ns = g.nz;
+    if ((size_t)g.nz * per > lim || (size_t)(g.nz + 3) * per > lim) {
+        if (per * 4 > lim) fail("cone projection: a slice of more than 2^27 voxels is not supported");
+        ns = (int)(lim / per) - 3;
+    }
+    // cone_slab_slices: at most this many slices per slab (0 = only where needed)
+    const int t = tuning_get("cone_slab_slices", 0);
+    if (t < 0) fail("cone_slab_slices must be positive");
+    return t > 0 ? std::min(ns, t) : ns;
+}
+
+static Grid cone_slab(const Grid& g, int z0, int ns) {
+    Grid s = g;
+    s.nz = ns;
+    s.cz = (float)((double)g.cz + ((double)z0 + ns / 2.0 - g.nz / 2.0) * (double)g.dz);
+    return s;
+}
+
+void siddon_cone_fp(float* prj, const float* img, const ConeGeomDev& geo, int B, const Grid& g, const Det& d,
+                    int nview, hipStream_t stream) {
+    const int ns = cone_slab_slices(g);
+    if (ns >= g.nz) return siddon_cone_fp_grid(prj, img, geo, B, g, d, nview, stream);
+    const size_t per = (size_t)g.nx * g.ny, nimg = per * g.nz, nprj = (size_t)d.nu * d.nv * nview;
+    for (int b = 0; b < B; b++)
+        for (int z0 = 0; z0 < g.nz; z0 += ns) {
+            const int n = std::min(ns, g.nz - z0);
+            siddon_cone_fp_grid(prj + b * nprj, img + b * nimg + (size_t)z0 * per, geo, 1, cone_slab(g, z0, n), d,
+                                nview, stream);
+        }
+}
+
+void siddon_cone_bp(float* img, const float* prj, const ConeGeomDev& geo, const ConeGeomHost& hgeo, int B,
+                    const Grid& g, const Det& d, int nview, hipStream_t stream) {
+    const int ns = cone_slab_slices(g);
+    if (ns >= g.nz) return siddon_cone_bp_grid(img, prj, geo, hgeo, B, g, d, nview, stream);
+    const size_t per = (size_t)g.nx * g.ny, nimg = per * g.nz, nprj = (size_t)d.nu * d.nv * nview;
+    for (int b = 0; b < B; b++)
+        for (int z0 = 0; z0 < g.nz; z0 += ns) {
+            const int n = std::min(ns, g.nz - z0);
+            siddon_cone_bp_grid(img + b * nimg + (size_t)z0 * per, prj + b * nprj, geo, hgeo, 1, cone_slab(g, z0, n),
+                                d, nview, stream);
+        }
     CTP_CHECK(hipStreamSynchronize(stream));
 }
 

@@ -28,7 +28,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, 'liboracle.so')
+# CTP_ORACLE_LIB selects another build of oracle.cpp (the benchmark's CPU
+# baseline loads the -march=native build made on the host it runs on)
+_LIB_PATH = os.environ.get('CTP_ORACLE_LIB') or os.path.join(_HERE, 'liboracle.so')
 _lib = None
 
 

@@ -402,6 +402,14 @@ def serving_extra(count=16, seed=1005):
     }
 
 
+def fft_n(nu):
+    """FFT length of the ramp filter: the smallest power of two >= 2nu - 1."""
+    n = 8
+    while n < 2 * nu - 1:
+        n *= 2
+    return n
+
+
 def fbp_extra(steps, w, img, outputs=None):
     import torch
     from ctprojector_amd import _lib
@@ -438,12 +446,12 @@ def fbp_extra(steps, w, img, outputs=None):
         'workload': 'fan ramp_filter (Hann) + fbp_bp, same volume/scan',
         'ms_per_step': round((t1 - t0) / steps * 1e3, 3),
         'kernel_ms': {'ramp_filter': round(rf_ms, 3), 'fan_fbp_bp': round(bp_ms, 3)},
-        # the ramp filter is a direct linear convolution, 2 nu^2 flops per
-        # detector row: compute-bound (8 B per sample of HBM traffic)
-        'ramp_filter_tflop_s': round(2.0 * nsamp * w.nu / (rf_ms / 1e3) / 1e12, 2) if rf_ms else None,
-        'ramp_filter_fp32_frac': round(2.0 * nsamp * w.nu / (rf_ms / 1e3) / 1e12 / PEAK_FP32_TFLOPS, 4)
-        if rf_ms else None,
+        # the ramp filter streams 8 B per detector sample (read + write); its
+        # FFT convolution (length N = 2^p >= 2nu - 1, two rows per complex
+        # transform, a forward and an inverse) does ~5 N log2 N flops per row
         'ramp_filter_hbm_frac': round(8 * nsamp / (rf_ms / 1e3) / 1e9 / PEAK_HBM_GBS, 4) if rf_ms else None,
+        'ramp_filter_fft_tflop_s': round(5.0 * fft_n(w.nu) * np.log2(fft_n(w.nu)) * nsamp / w.nu / (rf_ms / 1e3)
+                                         / 1e12, 2) if rf_ms else None,
         'fbp_bp_gvu_s': round(nvu / (bp_ms / 1e3) / 1e9, 2) if bp_ms else None,
         'fbp_bp_hbm_frac': round(4 * nvu / (bp_ms / 1e3) / 1e9 / PEAK_HBM_GBS, 4) if bp_ms else None,
     }

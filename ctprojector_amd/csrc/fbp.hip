@@ -152,12 +152,350 @@ __global__ __launch_bounds__(256) void ramp_conv_kernel(float* __restrict__ out,
     }
 }
 
+// ---------------------------------------------------------------------------
+// FFT convolution: the same linear convolution as ramp_conv_kernel through a
+// circular one of length N = 2^p >= 2nu - 1 (no wrap-around reaches the
+// outputs 0..nu-1).  With g[k mod N] = h_w[nu - 1 + k] for |k| < nu, zero
+// elsewhere, y = IFFT(FFT(x w) * G) on [0, nu), G = FFT(g).  h_w is even
+// (the Ram-Lak taps are, and both window forms keep it), so G is real; it is
+// built in FP64 on the host with scale / N folded in.  Two detector rows a, b
+// travel as one complex row a + ib: a real G keeps them apart, and the
+// inverse transform is taken as conj(FFT(conj(Z) G)), so one forward FFT
+// serves both directions.  One 256-lane block per row pair; the row pair
+// lives in LDS (N complex values) through mixed-radix Stockham passes
+// (radix 8, then one radix-4 or radix-2 pass), twiddles from an FP64-built
+// table.  Work per sample ~2 x 5 log2 N / 2 flops instead of 2 nu.
+// ---------------------------------------------------------------------------
+__device__ inline float2 cmulf(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ inline float2 caddf(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ inline float2 csubf(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+// a * (-i)
+__device__ inline float2 cmul_mi(float2 a) { return make_float2(a.y, -a.x); }
+
+// forward DFT of R points in place, natural order: X[k] = sum_r v[r] e^{-2 pi i r k / R}
+template <int R>
+__device__ inline void dft_small(float2* v) {
+    if constexpr (R == 2) {
+        const float2 a = v[0], b = v[1];
+        v[0] = caddf(a, b);
+        v[1] = csubf(a, b);
+    } else if constexpr (R == 4) {
+        const float2 t0 = caddf(v[0], v[2]), t1 = csubf(v[0], v[2]);
+        const float2 t2 = caddf(v[1], v[3]), t3 = cmul_mi(csubf(v[1], v[3]));
+        v[0] = caddf(t0, t2);
+        v[2] = csubf(t0, t2);
+        v[1] = caddf(t1, t3);
+        v[3] = csubf(t1, t3);
+    } else {
+        static_assert(R == 8, "radix 2, 4 or 8");
+        float2 e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+        dft_small<4>(e);
+        dft_small<4>(o);
+        const float c = 0.70710678118654752f;
+        o[1] = make_float2(c * (o[1].x + o[1].y), c * (o[1].y - o[1].x));    // * e^{-i pi/4}
+        o[2] = cmul_mi(o[2]);                                                // * e^{-i pi/2}
+        o[3] = make_float2(c * (o[3].y - o[3].x), -c * (o[3].x + o[3].y));   // * e^{-3i pi/4}
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            v[k] = caddf(e[k], o[k]);
+            v[k + 4] = csubf(e[k], o[k]);
+        }
+    }
+}
+
+// One Stockham pass of radix R over buf[N] (Ns = product of the earlier radices):
+// butterfly j reads buf[j + r N/R], twiddles by W_N^{r k N/(Ns R)} (k = j mod Ns),
+// transforms, and writes buf[(j - k) R + k + r Ns].  At most 2 butterflies per
+// lane (N / R <= 512).
+template <int R>
+__device__ inline void fft_pass(float2* buf, int N, int Ns, const float2* __restrict__ tw) {
+    const int nb = N / R;
+    const int tstep = N / (Ns * R);
+    float2 v[2][R];
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+        const int j = threadIdx.x + q * 256;
+        if (j < nb) {
+            const int k = j & (Ns - 1);
+#pragma unroll
+            for (int r = 0; r < R; r++) v[q][r] = buf[j + r * nb];
+#pragma unroll
+            for (int r = 1; r < R; r++) v[q][r] = cmulf(v[q][r], tw[r * k * tstep]);
+            dft_small<R>(v[q]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+        const int j = threadIdx.x + q * 256;
+        if (j < nb) {
+            const int k = j & (Ns - 1);
+            const int base = (j - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; r++) buf[base + r * Ns] = v[q][r];
+        }
+    }
+    __syncthreads();
+}
+
+__device__ inline void fft_lds(float2* buf, int N, int logN, const float2* __restrict__ tw) {
+    int Ns = 1;
+    for (int s = 0; s < logN / 3; s++) {
+        fft_pass<8>(buf, N, Ns, tw);
+        Ns *= 8;
+    }
+    if (logN % 3 == 2) fft_pass<4>(buf, N, Ns, tw);
+    else if (logN % 3 == 1) fft_pass<2>(buf, N, Ns, tw);
+}
+
+__global__ __launch_bounds__(256) void ramp_fft_kernel(float* __restrict__ out, const float* __restrict__ in,
+                                                       const float* __restrict__ G, const float2* __restrict__ tw,
+                                                       const float* __restrict__ pw, long nrows, int nu, int N,
+                                                       int logN) {
+    extern __shared__ float2 zbuf[];
+    const long ra = 2L * blockIdx.x, rb = ra + 1;
+    const bool hb = rb < nrows;
+    const float* xa = in + ra * nu;
+    const float* xb = in + rb * nu;
+    for (int m = threadIdx.x; m < N; m += 256) {
+        float a = 0.f, b = 0.f;
+        if (m < nu) {
+            const float w = pw[m];
+            a = xa[m] * w;
+            if (hb) b = xb[m] * w;
+        }
+        zbuf[m] = make_float2(a, b);
+    }
+    __syncthreads();
+    fft_lds(zbuf, N, logN, tw);
+    for (int m = threadIdx.x; m < N; m += 256) {
+        const float2 z = zbuf[m];
+        const float gm = G[m];
+        zbuf[m] = make_float2(z.x * gm, -z.y * gm);       // conj(Z) G
+    }
+    __syncthreads();
+    fft_lds(zbuf, N, logN, tw);
+    float* ya = out + ra * nu;
+    float* yb = out + rb * nu;
+    for (int j = threadIdx.x; j < nu; j += 256) {
+        const float2 f = zbuf[j];
+        ya[j] = f.x;
+        if (hb) yb[j] = -f.y;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Three-pass form for N = 1024, 2048, 4096 (nu from 257 to 2048): radix 16,
+// radix 16, radix N/256.  T = N/16 lanes per complex row (one radix-16
+// butterfly each; the last pass gives each lane 256/T butterflies), 256/T
+// complex rows (two detector rows each) per block.  The first pass reads the
+// rows from global memory and the last pass of the inverse writes them back,
+// so a row pair crosses LDS five times (the multiplication by conj and G
+// happens in registers between the transforms).  LDS rows are padded by one
+// complex value per 16 (index i at i + i/16), which spreads the radix-16
+// writes over all banks.  Twiddles W_N^m from the hardware sine and cosine of
+// m/N revolutions, powers by repeated multiplication.
+// ---------------------------------------------------------------------------
+__device__ inline float2 twiddle(int m, float invN) {
+    const float x = -(float)m * invN;              // revolutions
+    return make_float2(__builtin_amdgcn_cosf(x), __builtin_amdgcn_sinf(x));
+}
+
+// X[k] = sum_n v[n] W_16^{nk}: 4 x 4 decomposition, n = 4 n1 + n2, k = k1 + 4 k2
+__device__ inline void dft16(float2* v) {
+    constexpr float c1 = 0.92387953251128674f, s1 = 0.38268343236508978f, c2 = 0.70710678118654752f;
+    // W_16^m = (cos, -sin)(2 pi m / 16) for m = n2 k1, n2, k1 < 4
+    const float2 w[10] = {{1.f, 0.f}, {c1, -s1}, {c2, -c2}, {s1, -c1}, {0.f, -1.f}, {-s1, -c1}, {-c2, -c2},
+                          {-c1, -s1}, {-1.f, 0.f}, {-c1, s1}};
+    float2 a[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; n2++) {
+        float2 q[4] = {v[n2], v[n2 + 4], v[n2 + 8], v[n2 + 12]};
+        dft_small<4>(q);
+#pragma unroll
+        for (int k1 = 0; k1 < 4; k1++) a[n2][k1] = (n2 * k1 == 0) ? q[k1] : cmulf(q[k1], w[n2 * k1]);
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < 4; k1++) {
+        float2 q[4] = {a[0][k1], a[1][k1], a[2][k1], a[3][k1]};
+        dft_small<4>(q);
+#pragma unroll
+        for (int k2 = 0; k2 < 4; k2++) v[k1 + 4 * k2] = q[k2];
+    }
+}
+
+template <int R>
+__device__ inline void twiddle_run(float2* v, int m, float invN) {
+    if (m == 0) return;
+    const float2 w1 = twiddle(m, invN);
+    float2 w = w1;
+#pragma unroll
+    for (int r = 1; r < R; r++) {
+        v[r] = cmulf(v[r], w);
+        if (r + 1 < R) w = cmulf(w, w1);
+    }
+}
+
+template <int R>
+__device__ inline void dft_any(float2* v) {
+    if constexpr (R == 16) dft16(v);
+    else dft_small<R>(v);
+}
+
+template <int N>
+__global__ __launch_bounds__(256) void ramp_fft16_kernel(float* __restrict__ out, const float* __restrict__ in,
+                                                         const float* __restrict__ G, const float* __restrict__ pw,
+                                                         long nrows, int nu) {
+    constexpr int T = N / 16;            // lanes per complex row
+    constexpr int RPB = 256 / T;         // complex rows per block
+    constexpr int R3 = N / 256;          // radix of the third pass
+    constexpr int B3 = 256 / T;          // third-pass butterflies per lane
+    constexpr int NP = N + N / 16;       // padded LDS row
+    static_assert(N == 1024 || N == 2048 || N == 4096, "three-pass plan");
+    __shared__ float2 lds[RPB * NP];
+    const float invN = 1.f / (float)N;
+    const int t = threadIdx.x % T, c = threadIdx.x / T;
+    float2* buf = lds + c * NP;
+    const long ra = 2L * ((long)blockIdx.x * RPB + c), rb = ra + 1;
+    const bool ha = ra < nrows, hb = rb < nrows;
+    auto P = [](int i) { return i + (i >> 4); };
+    float2 v[16];
+    float2 u[B3][R3];
+
+    // ---- forward transform of z = (a + i b) w ----
+    // pass 1 (radix 16, Ns = 1): inputs from global memory.  nu <= N/2, so
+    // inputs r >= 8 (m >= N/2) are zero and the first radix-4 stage of each
+    // radix-16 butterfly has two zero inputs.
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int m = t + r * T;
+        float a = 0.f, b = 0.f;
+        if (r < 8 && m < nu) {
+            const float w = pw[m];
+            if (ha) a = in[ra * nu + m] * w;
+            if (hb) b = in[rb * nu + m] * w;
+        }
+        v[r] = make_float2(a, b);
+    }
+    dft16(v);
+#pragma unroll
+    for (int r = 0; r < 16; r++) buf[P(t * 16 + r)] = v[r];
+    __syncthreads();
+    // pass 2 (radix 16, Ns = 16)
+    {
+        const int k = t & 15;
+#pragma unroll
+        for (int r = 0; r < 16; r++) v[r] = buf[P(t + r * T)];
+        twiddle_run<16>(v, k * (N / 256), invN);
+        dft16(v);
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 16; r++) buf[P((t - k) * 16 + k + 16 * r)] = v[r];
+        __syncthreads();
+    }
+    // pass 3 (radix R3, Ns = 256): lane holds X[j + 256 r], j = t + q T
+#pragma unroll
+    for (int q = 0; q < B3; q++) {
+        const int j = t + q * T;
+#pragma unroll
+        for (int r = 0; r < R3; r++) u[q][r] = buf[P(j + 256 * r)];
+        twiddle_run<R3>(u[q], j, invN);
+        dft_any<R3>(u[q]);
+    }
+    __syncthreads();
+    // conj(X) G into LDS, natural order
+#pragma unroll
+    for (int q = 0; q < B3; q++)
+#pragma unroll
+        for (int r = 0; r < R3; r++) {
+            const int i = t + q * T + 256 * r;
+            const float g = G[i];
+            buf[P(i)] = make_float2(u[q][r].x * g, -u[q][r].y * g);
+        }
+    __syncthreads();
+    // ---- second forward transform (the inverse, by conjugation) ----
+#pragma unroll
+    for (int r = 0; r < 16; r++) v[r] = buf[P(t + r * T)];
+    dft16(v);
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 16; r++) buf[P(t * 16 + r)] = v[r];
+    __syncthreads();
+    {
+        const int k = t & 15;
+#pragma unroll
+        for (int r = 0; r < 16; r++) v[r] = buf[P(t + r * T)];
+        twiddle_run<16>(v, k * (N / 256), invN);
+        dft16(v);
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 16; r++) buf[P((t - k) * 16 + k + 16 * r)] = v[r];
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < B3; q++) {
+        const int j = t + q * T;
+#pragma unroll
+        for (int r = 0; r < R3; r++) u[q][r] = buf[P(j + 256 * r)];
+        twiddle_run<R3>(u[q], j, invN);
+        dft_any<R3>(u[q]);
+        // outputs i >= N/2 >= nu are not needed (their arithmetic is dropped)
+#pragma unroll
+        for (int r = 0; r < R3 / 2; r++) {
+            const int i = j + 256 * r;
+            if (i < nu) {
+                if (ha) out[ra * nu + i] = u[q][r].x;
+                if (hb) out[rb * nu + i] = -u[q][r].y;
+            }
+        }
+    }
+}
+
+// G = FFT(g) (real) times scale / N, and the twiddle table W_N^t, t < N, in FP64
+struct RampFFTTables {
+    int N = 0, logN = 0;
+    std::vector<float> G;
+    std::vector<float2> tw;
+};
+
+static RampFFTTables ramp_fft_tables(int nu, float da, int filter, bool equispace, float scale) {
+    static std::mutex mu;
+    static std::map<std::tuple<int, float, int, bool, float>, RampFFTTables> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    auto key = std::make_tuple(nu, da, filter, equispace, scale);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    RampFFTTables t;
+    t.logN = 0;
+    while ((1 << t.logN) < 2 * nu - 1) t.logN++;
+    t.logN = std::max(t.logN, 3);
+    t.N = 1 << t.logN;
+    const int N = t.N;
+    const std::vector<double> h = ramp_kernel_fp64(nu, (double)da, filter, equispace);
+    std::vector<double> cs(N);
+    for (int k = 0; k < N; k++) cs[k] = std::cos(2 * M_PI * k / N);
+    // g even: G[m] = g[0] + sum_{0 < |k| < nu} h_w[nu - 1 + k] cos(2 pi k m / N)
+    t.G.resize(N);
+    for (int m = 0; m < N; m++) {
+        double acc = h[nu - 1];
+        for (int k = 1; k < nu; k++)
+            acc += (h[nu - 1 + k] + h[nu - 1 - k]) * cs[(int)(((long)k * m) % N)];
+        t.G[m] = (float)(acc * (double)scale / N);
+    }
+    t.tw.resize(N);
+    for (int k = 0; k < N; k++) t.tw[k] = make_float2((float)cs[k], (float)(-std::sin(2 * M_PI * k / N)));
+    if (cache.size() > 64) cache.clear();
+    cache[key] = t;
+    return t;
+}
+
 void ramp_filter(float* out, const float* prj, int B, int nu, int nv, int nview, float da, float off_u, float dsd,
                  float dso, int filter, bool equispace, hipStream_t stream) {
     (void)dsd;
     if (nu <= 0) return;
     if (nu > 5000) fail("ramp filter: nu > 5000 is not supported");
-    const std::vector<float> h = cached_ramp(nu, da, filter, equispace);
     std::vector<float> pw(nu);
     for (int i = 0; i < nu; i++) {
         // fan pre-weight cos(gamma), cuda/fbp.cu:60-66 (not mirrored)
@@ -167,6 +505,44 @@ void ramp_filter(float* out, const float* prj, int B, int nu, int nv, int nview,
     // cuda/fbp.cu:97-104, cuda/fbpParallel.cu:59; the 1/L cancels cuFFT's
     // unnormalised inverse
     const float scale = equispace ? (float)((double)kPI / nview * da) : (float)((double)kPI / nview * da / dso);
+    const long nrows = (long)B * nview * nv;
+    // ramp_mode: 0 = FFT convolution where N = 2^p >= 2nu - 1 is at most 4096
+    // (three-pass kernel for N >= 1024), 1 = direct, 2 = the generic FFT kernel
+    const int mode = tuning_get("ramp_mode", 0);
+    if (mode < 0 || mode > 2) fail("ramp_mode must be 0 (FFT), 1 (direct) or 2 (generic FFT)");
+    if (mode != 1 && 2 * nu - 1 <= 4096) {
+        const RampFFTTables t = ramp_fft_tables(nu, da, filter, equispace, scale);
+        Workspace& ws = workspace_for(stream);
+        Plan pl;
+        pl.add<float>(t.N);
+        pl.add<float2>(t.N);
+        pl.add<float>(nu);
+        ws.reserve(pl.bytes);
+        float* dG = ws.take_n<float>(t.N);
+        float2* dtw = ws.take_n<float2>(t.N);
+        float* dpw = ws.take_n<float>(nu);
+        CTP_CHECK(hipMemcpyAsync(dG, t.G.data(), sizeof(float) * t.N, hipMemcpyHostToDevice, stream));
+        CTP_CHECK(hipMemcpyAsync(dtw, t.tw.data(), sizeof(float2) * t.N, hipMemcpyHostToDevice, stream));
+        CTP_CHECK(hipMemcpyAsync(dpw, pw.data(), sizeof(float) * nu, hipMemcpyHostToDevice, stream));
+        {
+            TimedLaunch tl("ramp_filter", stream);
+            const long pairs = ceil_div(nrows, 2);
+            if (mode == 0 && t.N >= 1024) {
+                const int rpb = 256 / (t.N / 16);
+                const unsigned grid = (unsigned)ceil_div(pairs, (long)rpb);
+                if (t.N == 1024) ramp_fft16_kernel<1024><<<grid, 256, 0, stream>>>(out, prj, dG, dpw, nrows, nu);
+                else if (t.N == 2048) ramp_fft16_kernel<2048><<<grid, 256, 0, stream>>>(out, prj, dG, dpw, nrows, nu);
+                else ramp_fft16_kernel<4096><<<grid, 256, 0, stream>>>(out, prj, dG, dpw, nrows, nu);
+            } else {
+                ramp_fft_kernel<<<(unsigned)pairs, 256, sizeof(float2) * t.N, stream>>>(out, prj, dG, dtw, dpw, nrows,
+                                                                                      nu, t.N, t.logN);
+            }
+            CTP_LAUNCH_CHECK();
+        }
+        CTP_CHECK(hipStreamSynchronize(stream));
+        return;
+    }
+    const std::vector<float> h = cached_ramp(nu, da, filter, equispace);
     Workspace& ws = workspace_for(stream);
     Plan pl;
     pl.add<float>(2 * nu - 1);
@@ -176,7 +552,6 @@ void ramp_filter(float* out, const float* prj, int B, int nu, int nv, int nview,
     float* dpw = ws.take_n<float>(nu);
     CTP_CHECK(hipMemcpyAsync(dh, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice, stream));
     CTP_CHECK(hipMemcpyAsync(dpw, pw.data(), sizeof(float) * nu, hipMemcpyHostToDevice, stream));
-    const long nrows = (long)B * nview * nv;
     // LDS: the kernel (2nu+3 floats) and RB input rows; stays within 64 KB
     const size_t sh4 = sizeof(float) * ((2 * nu - 1 + 4) + 4 * nu);
     {

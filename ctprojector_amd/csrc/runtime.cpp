@@ -45,6 +45,7 @@ std::map<std::string, int> g_tune = {
     {"fbp_bp_lanes_per_pixel", 0}, // fan FBP / parallel pixel BP: lanes per pixel (1 = whole channel vector per lane; 2, 4; default 4, 2 at 8 channels)
     {"dd_fp_segments", 0},   // DD FP direct kernel: row segments per column (1, 2; default by wave count)
     {"tv_mode", 0},          // TV prior: 1 z-marching tile kernels (default), 2 one lane per voxel
+    {"ramp_mode", 0},        // ramp filter: 0 FFT convolution where 2nu-1 <= 4096 (default), 1 direct, 2 generic FFT kernel
     {"bp_view_groups", 0},   // view groups of the pixel-driven DD / FBP backprojections (default: enough for 1024 blocks)
 };
 // Arenas are keyed by (device, stream, host thread): two threads calling on the

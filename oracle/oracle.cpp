@@ -631,6 +631,118 @@ void dd2d_fp(T* prj, const float* img, const float* deg, long B, const GridT<T>&
     }
 }
 
+// Exact transpose of dd2d_fp in the BoxInt mode (no reference counterpart: the
+// reference's BP is not the transpose of its FP).  ddfp_visit_boxint walks the
+// same lines, rows and box integrals as ddfp_tracing_boxint and hands every
+// (pixel, coefficient) pair of the ray's sum to 'visit', so that
+// FP = sum coef * img[pixel] and A^T y scatters coef * y into the pixels.
+// IntegralBoxX as coefficients: 1 on cells i1..i2, minus (x1 - i1) on i1,
+// minus (i2 + 1 - x2) on i2, nothing when x2 <= x1 after clamping.
+template <typename T, typename Visit>
+void box_1d_coeffs(long base, long stride, T x1, T x2, long n, T scale, Visit visit) {
+    x1 = clampf(x1, T(0), T(n));
+    x2 = clampf(x2, T(0), T(n));
+    long i1 = clampi((long)(int)x1, 0, n);
+    long i2 = clampi((long)(int)x2, 0, n);
+    if (x2 <= x1) return;
+    for (long i = i1; i <= i2; i++) visit(base + i * stride, scale);
+    visit(base + i1 * stride, -scale * (x1 - T(i1)));
+    visit(base + i2 * stride, -scale * (T(i2 + 1) - x2));
+}
+
+template <typename T, typename Visit>
+void ddfp_visit_boxint(V2<T> s1, V2<T> s2, V2<T> d1, V2<T> d2, T z, T deg, T a, T cosD, T sinD, const GridT<T>& g,
+                       T rayscale, Visit visit) {
+    z = clampf(z, T(0), T(g.nz));
+    long iz = (long)(int)z;
+    long iz1 = iz + 1;
+    iz = clampi(iz, 0, g.nz);
+    iz1 = clampi(iz1, 0, g.nz);
+    T wz = z - T(iz);
+    const long nxy = g.nx * g.ny;
+    const T margin = T(kDDMarginf);
+    T wx = clampf((tabs(cosD) - tabs(sinD) + margin) / (2 * margin), T(0), T(1));
+    if (tabs(cosD) > tabs(sinD) - margin) {
+        T r1 = (d1.x - s1.x) / (d1.y - s1.y);
+        T r2 = (d2.x - s2.x) / (d2.y - s2.y);
+        const T f = rayscale * wx * g.dy / tabs(tcos(deg - a));
+        for (long iy = 0; iy < g.ny; iy++) {
+            T x1 = s1.x + r1 * (T(iy) + T(0.5) - s1.y);
+            T x2 = s2.x + r2 * (T(iy) + T(0.5) - s2.y);
+            if (x1 > x2) std::swap(x1, x2);
+            box_1d_coeffs<T>(iz * nxy + iy * g.nx, 1, x1, x2, g.nx, f * (1 - wz) / (x2 - x1), visit);
+            box_1d_coeffs<T>(iz1 * nxy + iy * g.nx, 1, x1, x2, g.nx, f * wz / (x2 - x1), visit);
+        }
+    }
+    if (tabs(cosD) < tabs(sinD) + margin) {
+        T r1 = (d1.y - s1.y) / (d1.x - s1.x);
+        T r2 = (d2.y - s2.y) / (d2.x - s2.x);
+        const T f = rayscale * (1 - wx) * g.dx / tabs(tsin(deg - a));
+        for (long ix = 0; ix < g.nx; ix++) {
+            T y1 = s1.y + r1 * (T(ix) + T(0.5) - s1.x);
+            T y2 = s2.y + r2 * (T(ix) + T(0.5) - s2.x);
+            if (y1 > y2) std::swap(y1, y2);
+            box_1d_coeffs<T>(iz * nxy + ix, g.nx, y1, y2, g.ny, f * (1 - wz) / (y2 - y1), visit);
+            box_1d_coeffs<T>(iz1 * nxy + ix, g.nx, y1, y2, g.ny, f * wz / (y2 - y1), visit);
+        }
+    }
+}
+
+// img += A^T prj for the BoxInt fan / parallel forward projector (type bit 2
+// honoured for the parallel beam; the branchless bit is not supported here).
+// Deterministic: per-thread partial images summed in thread order.
+template <typename T>
+void dd2d_fp_adjoint(T* img, const float* prj, const float* deg, long B, const GridT<T>& g, const DetT<T>& det,
+                     long nview, T dsd, T dso, int type, bool fan) {
+    DDType tp{type};
+    const long nimg = g.nx * g.ny * g.nz, nprj = det.nu * det.nv * nview;
+    for (long ib = 0; ib < B; ib++) {
+        T* im = img + ib * nimg;
+        const float* pr = prj + ib * nprj;
+        int nth = 1;
+#ifdef _OPENMP
+        nth = omp_get_max_threads();
+#endif
+        std::vector<std::vector<T>> part(nth);
+#pragma omp parallel
+        {
+            int tid = 0;
+#ifdef _OPENMP
+            tid = omp_get_thread_num();
+#endif
+            std::vector<T>& acc = part[tid];
+            acc.assign(nimg, T(0));
+            auto visit = [&acc](long p, T c) { acc[p] += c; };
+#pragma omp for collapse(2) schedule(dynamic, 4)
+            for (long iview = 0; iview < nview; iview++)
+                for (long iv = 0; iv < det.nv; iv++)
+                    for (long iu = 0; iu < det.nu; iu++) {
+                        const T y = T(pr[(iview * det.nv + iv) * det.nu + iu]);
+                        if (y == 0) continue;
+                        T c, sn;
+                        view_trig<T>(deg, iview, c, sn);
+                        T d = T(deg[iview]);
+                        if (fan) {
+                            V2<T> src, d1, d2;
+                            T a, z;
+                            fan_edges<T>(iu, iv, c, sn, det, dsd, dso, g, src, d1, d2, a, z);
+                            ddfp_visit_boxint<T>(src, src, d1, d2, z, d, a, c, sn, g, y, visit);
+                        } else {
+                            V2<T> s1, s2, d1, d2;
+                            T z;
+                            par_edges<T>(iu, iv, c, sn, det, g, s1, s2, d1, d2, z);
+                            T ys = y;
+                            if (tp.force_fbp()) ys /= (tabs(c) > tabs(sn) ? g.dy / tabs(c) : g.dx / tabs(sn));
+                            ddfp_visit_boxint<T>(s1, s2, d1, d2, z, d, T(0), c, sn, g, ys, visit);
+                        }
+                    }
+        }
+        for (int t = 0; t < nth; t++)
+            if (!part[t].empty())
+                for (long k = 0; k < nimg; k++) im[k] += part[t][k];
+    }
+}
+
 // BP pre-weights, cuda/distanceDriven2DUtils.cu:8-65
 template <typename T>
 void preweight_bp(std::vector<T>& w, const float* prj, const float* deg, const DetT<T>& det, long nview, T dx, T dy,
@@ -1138,6 +1250,14 @@ DEF_SIDDON_FAN(f, float)
     }                                                                                                        \
     void orc_dd_par_bp_##SUF(T* img, const float* prj, const float* deg, size_t B, GRIDARGS, DETARGS, int type) { \
         dd2d_bp<T>(img, prj, deg, (long)B, MKG(T), MKD(T), (long)nview, T(1000), T(500), type, false);        \
+    }                                                                                                        \
+    void orc_dd_fan_fp_adjoint_##SUF(T* img, const float* prj, const float* deg, size_t B, GRIDARGS, DETARGS,   \
+                                     float dsd, float dso, int type) {                                       \
+        dd2d_fp_adjoint<T>(img, prj, deg, (long)B, MKG(T), MKD(T), (long)nview, T(dsd), T(dso), type, true);  \
+    }                                                                                                        \
+    void orc_dd_par_fp_adjoint_##SUF(T* img, const float* prj, const float* deg, size_t B, GRIDARGS, DETARGS,   \
+                                     int type) {                                                             \
+        dd2d_fp_adjoint<T>(img, prj, deg, (long)B, MKG(T), MKD(T), (long)nview, T(1000), T(500), type, false); \
     }
 DEF_DD2D(d, double)
 DEF_DD2D(f, float)

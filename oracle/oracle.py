@@ -154,6 +154,34 @@ def dd_fan_bp(p, prj, angles, is_fbp=False, branchless=False, type_projector=Non
     return img
 
 
+def dd_fan_fp_adjoint(p, prj, angles, type_projector=0, prec='d'):
+    """img = A^T prj for the BoxInt fan-beam DD forward projector (exact transpose,
+    no reference counterpart; the loops of dd_fan_fp transposed)."""
+    if type_projector & 2:
+        raise ValueError('the exact transpose is restated for the BoxInt mode only')
+    prj = _f32(prj)
+    angles = _f32(angles)
+    img = np.zeros([prj.shape[0], p.nz, p.ny, p.nx], _out_t(prec))
+    getattr(lib(), 'orc_dd_fan_fp_adjoint_' + prec)(
+        _ptr(img), _ptr(prj), _ptr(angles), _u(prj.shape[0]), *_grid(p, img.shape),
+        *_det(p, prj.shape[3], prj.shape[2], prj.shape[1], du=np.float32(p.du / p.dsd)), _f(p.dsd), _f(p.dso),
+        ctypes.c_int(type_projector))
+    return img
+
+
+def dd_par_fp_adjoint(p, prj, angles, type_projector=0, prec='d'):
+    """img = A^T prj for the BoxInt parallel-beam DD forward projector."""
+    if type_projector & 2:
+        raise ValueError('the exact transpose is restated for the BoxInt mode only')
+    prj = _f32(prj)
+    angles = _f32(angles)
+    img = np.zeros([prj.shape[0], p.nz, p.ny, p.nx], _out_t(prec))
+    getattr(lib(), 'orc_dd_par_fp_adjoint_' + prec)(
+        _ptr(img), _ptr(prj), _ptr(angles), _u(prj.shape[0]), *_grid(p, img.shape),
+        *_det(p, prj.shape[3], prj.shape[2], prj.shape[1]), ctypes.c_int(type_projector))
+    return img
+
+
 def dd_par_fp(p, img, angles, branchless=False, type_projector=None, prec='d'):
     img = _f32(img)
     angles = _f32(angles)

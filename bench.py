@@ -370,11 +370,18 @@ def cone_batch_extra(steps):
     }
 
 
-def serving_extra(count=16, seed=1005):
+def serving_extra(count=200, seed=1005, sub_count=32):
     """Served reconstruction stream: Poisson arrivals (200/s) of half fan-beam
     FBP (256^2-512^2 x 1-64 slices, 360-984 views) and half cone-beam SQS-TV
     (128^3-256^3, 10-50 iterations, 360-984 views), continuous batching with
-    HBM-budget admission, against sequential plain launches of the same requests."""
+    HBM-budget admission, against sequential plain launches of the same
+    requests.  Then the same mix below saturation (arrivals at half the
+    served rate) for its latency.  The sequential run also times the
+    library's kernels: their summed duration over the run's wall time is how
+    busy one request at a time keeps the GPU, so wall / kernel time bounds
+    what overlapping requests alone can gain; batching gains come on top only
+    where launches of several requests do less work than their sum (the cone
+    kernels share traversals and chords across volumes of one geometry)."""
     import torch
     from ctprojector_amd.serving import Server, make_stream, run_sequential
     log('[bench] serving: %d requests (simulating sinograms)' % count)
@@ -384,10 +391,15 @@ def serving_extra(count=16, seed=1005):
     srv.warm(reqs)
     log('[bench] serving: served run')
     st = srv.serve(reqs)
+    for q in reqs:
+        q.result = None
     log('[bench] serving: sequential run')
-    seq_s, _ = run_sequential(make_stream(count, seed=seed))
+    kt = {}
+    seq_s, seq_out = run_sequential(reqs, kernel_time=kt)
+    del seq_out, reqs
     seq_rate = count / seq_s
-    return {
+    busy = kt['library_kernels_ms'] / 1e3 / seq_s
+    out = {
         'workload': 'served stream of %d requests (Poisson 200/s, seed %d): fan FBP 256^2-512^2 x 1-64 slices and '
                     'cone SQS-TV 128^3-256^3 x 10-50 iterations, 360-984 views' % (count, seed),
         'recons_per_s': round(st.recons_per_s, 3),
@@ -396,10 +408,22 @@ def serving_extra(count=16, seed=1005):
         'makespan_s': round(st.makespan_s, 3),
         'sequential_recons_per_s': round(seq_rate, 3),
         'speedup_vs_sequential': round(st.recons_per_s / seq_rate, 3),
+        'sequential_gpu_busy': round(busy, 3),
+        'overlap_bound': round(1.0 / busy, 3) if busy > 0 else None,
         'peak_concurrent_requests': st.peak_concurrent,
         'batching': st.launches_batched,
-        'requests': {'fbp': sum(r.kind == 'fbp' for r in reqs), 'sqs': sum(r.kind == 'sqs' for r in reqs)},
     }
+    if sub_count:
+        rate = 0.5 * st.recons_per_s
+        log('[bench] serving: %d requests at %.2f/s (half the served rate)' % (sub_count, rate))
+        sub = make_stream(sub_count, rate=rate, seed=seed + 100)
+        srv.warm(sub)
+        s2 = srv.serve(sub)
+        out['below_saturation'] = {'requests': sub_count, 'arrivals_per_s': round(rate, 3),
+                                   'recons_per_s': round(s2.recons_per_s, 3),
+                                   'latency_p50_s': round(s2.latency_p50_s, 3),
+                                   'latency_p99_s': round(s2.latency_p99_s, 3)}
+    return out
 
 
 def fft_n(nu):
@@ -503,7 +527,7 @@ def short_name(kname):
 TIMED_KERNELS = {'dd_fan_fp': 'dd_fp_split_kernel<32, 4, true>', 'dd_fan_bp': 'dd_bp_split_kernel<32, 4, true>',
                  'siddon_cone_fp': 'siddon_cone_fp_col_kernel<1>',
                  'siddon_cone_bp': 'siddon_cone_bp_col_kernel<2, 2, 4, 1>',
-                 'fan_fbp_bp': 'fan_fbp_bp_split_kernel<32, 4, true>', 'ramp_filter': 'ramp_conv_kernel<4>'}
+                 'fan_fbp_bp': 'fan_fbp_bp_split_kernel<32, 4, true>', 'ramp_filter': 'ramp_fft16_kernel<2048>'}
 
 
 def roofline(dom, dom_ms, nvu, compulsory_bytes, times):
@@ -607,6 +631,11 @@ def main(argv=None):
     ap.add_argument('--no-extra', action='store_true')
     ap.add_argument('--no-serving', action='store_true', help='skip the served-stream extra')
     ap.add_argument('--no-parity', action='store_true', help='skip the FP64 oracle check of the outputs')
+    ap.add_argument('--extras', default='cone,cone_batch,fbp,unrolled_sqs_tv,tv_prior,serving',
+                    help='comma-separated extras to run (default: all)')
+    ap.add_argument('--serving-requests', type=int, default=200)
+    ap.add_argument('--serving-below-saturation', type=int, default=32,
+                    help='requests of the run below saturation (0: skip it)')
     ap.add_argument('--calibrate-copy', action='store_true')
     ap.add_argument('--tune', action='append', default=[], metavar='KEY=VALUE',
                     help='library launch-shape knob (ctpSetTuning), repeatable')
@@ -661,32 +690,23 @@ def main(argv=None):
     outputs = {'fan': dict(r['outputs']), 'cone': {}}
     if not args.no_extra:
         extra = {}
-        try:
-            extra['cone'] = cone_extra(2, cpu=not args.no_cpu_baseline,
-                                       outputs=None if args.no_parity else outputs['cone'])
-        except Exception as e:  # report, do not hide
-            extra['cone'] = {'error': repr(e)}
-        try:
-            extra['cone_batch'] = cone_batch_extra(3)
-        except Exception as e:
-            extra['cone_batch'] = {'error': repr(e)}
-        try:
-            extra['fbp'] = fbp_extra(3, w, r['img'], outputs=None if args.no_parity else outputs['fan'])
-        except Exception as e:
-            extra['fbp'] = {'error': repr(e)}
-        try:
-            extra['unrolled_sqs_tv'] = unrolled_sqs_extra(3)
-        except Exception as e:
-            extra['unrolled_sqs_tv'] = {'error': repr(e)}
-        try:
-            extra['tv_prior'] = tv_prior_extra(5)
-        except Exception as e:
-            extra['tv_prior'] = {'error': repr(e)}
+        want = set(args.extras.split(','))
+        runs = [('cone', lambda: cone_extra(2, cpu=not args.no_cpu_baseline,
+                                            outputs=None if args.no_parity else outputs['cone'])),
+                ('cone_batch', lambda: cone_batch_extra(3)),
+                ('fbp', lambda: fbp_extra(3, w, r['img'], outputs=None if args.no_parity else outputs['fan'])),
+                ('unrolled_sqs_tv', lambda: unrolled_sqs_extra(3)),
+                ('tv_prior', lambda: tv_prior_extra(5))]
         if not args.no_serving:
+            runs.append(('serving', lambda: serving_extra(args.serving_requests,
+                                                          sub_count=args.serving_below_saturation)))
+        for name, fn in runs:
+            if name not in want:
+                continue
             try:
-                extra['serving'] = serving_extra()
-            except Exception as e:
-                extra['serving'] = {'error': repr(e)}
+                extra[name] = fn()
+            except Exception as e:  # report, do not hide
+                extra[name] = {'error': repr(e)}
         line['extra'] = extra
     if not args.no_cpu_baseline:
         line['cpu_baseline'] = run_cpu_baseline('fan')

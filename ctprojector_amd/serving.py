@@ -375,18 +375,27 @@ def _stats(reqs: List[Request], max_in_use: int, peak: int, batched: dict) -> Se
                       float(lat.mean()), max_in_use, peak, batched)
 
 
-def run_sequential(requests: List[Request], device=None) -> Tuple[float, List[torch.Tensor]]:
+def run_sequential(requests: List[Request], device=None, kernel_time: Optional[dict] = None
+                   ) -> Tuple[float, List[torch.Tensor]]:
     """Baseline: one request at a time in arrival order, plain launches (one
     request per call, the reference's sqs_one_step driver with tv_sqs, no
     batching, no overlap), all requests available from the start.  Scan tables
     shared by a geometry class (projector norm, norm image) are prepared before
-    timing, as the server's are.  Returns (seconds, results in request order)."""
+    timing, as the server's are.  Returns (seconds, results in request order).
+    With ``kernel_time`` (a dict), the library's kernels are timed with HIP
+    events during the run and their summed duration (ms) is stored under
+    'library_kernels_ms': the device time the projector, filter and prior
+    kernels take when nothing overlaps them."""
     from .projector.cupy import fan_equiangular as fan
     from .prior.cupy import tv_sqs
     from .recon.sqs_algorithms import sqs_one_step
     dev = torch.device('cuda', torch.cuda.current_device()) if device is None else torch.device(device)
     classes = {r.key: _SQSClass(r.n, r.nview, dev) for r in requests if r.kind == 'sqs'}
     out = []
+    from . import _lib
+    if kernel_time is not None:
+        _lib.set_kernel_timing(True)
+        _lib.reset_kernel_times()
     torch.cuda.synchronize(dev)
     t0 = time.perf_counter()
     for r in sorted(requests, key=lambda q: q.arrival):
@@ -403,4 +412,8 @@ def run_sequential(requests: List[Request], device=None) -> Tuple[float, List[to
                                  {'weights': _SQSClass.WEIGHTS, 'eps': _SQSClass.EPS})
             out.append(x)
         torch.cuda.synchronize(dev)
-    return time.perf_counter() - t0, out
+    elapsed = time.perf_counter() - t0
+    if kernel_time is not None:
+        kernel_time['library_kernels_ms'] = sum(v[0] for v in _lib.kernel_times().values())
+        _lib.set_kernel_timing(False)
+    return elapsed, out

@@ -171,7 +171,20 @@ def cone_extra(steps, cpu=False, outputs=None):
     ms2 = (time.perf_counter() - t0) / steps * 1e3
     out['batch2_ms_per_step'] = round(ms2, 3)
     out['batch2_value_gvu_s'] = round(2 * 2 * nvu / (ms2 / 1e3) / 1e9, 2)
-    del prj, bp, img2
+    # and four volumes per call (the most the multi-volume kernels take per launch)
+    img4 = torch.cat([img2, 0.25 * img2], 0).contiguous()
+    prj = cone.siddon_fp_arbitrary(p, img4, *g)
+    _ = cone.siddon_bp_arbitrary(p, prj, *g)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        prj = cone.siddon_fp_arbitrary(p, img4, *g)
+        bp = cone.siddon_bp_arbitrary(p, prj, *g)
+    torch.cuda.synchronize()
+    ms4 = (time.perf_counter() - t0) / steps * 1e3
+    out['batch4_ms_per_step'] = round(ms4, 3)
+    out['batch4_value_gvu_s'] = round(2 * 4 * nvu / (ms4 / 1e3) / 1e9, 2)
+    del prj, bp, img2, img4
     if outputs is not None:              # single-volume outputs for the oracle check
         prj = cone.siddon_fp_arbitrary(p, img_d, *g)
         bp = cone.siddon_bp_arbitrary(p, prj, *g)

@@ -41,6 +41,29 @@ constexpr float kDDMargin = 0.12f;
 __host__ __device__ inline int clampi(int x, int s, int e) { return x < s ? s : (x >= e ? e - 1 : x); }
 __host__ __device__ inline float clampf(float x, float s, float e) { return x < s ? s : (x > e ? e : x); }
 
+// a / b through the hardware reciprocal (v_rcp_f32, 1 ulp): the divisions of
+// the per-view and per-row footprint arithmetic, where the IEEE-exact
+// sequence costs a dozen instructions and two mode switches.  Operands here
+// are finite and far from the denormal range; the result is within 2 ulp.
+__device__ inline float fast_div(float a, float b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return a * __builtin_amdgcn_rcpf(b);
+#else
+    return a / b;
+#endif
+}
+// clampf for finite x, and min / max of finite values, as one median instruction each
+__device__ inline float med3_f(float a, float b, float c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_fmed3f(a, b, c);
+#else
+    return a < b ? (b < c ? b : (a < c ? c : a)) : (a < c ? a : (b < c ? c : b));
+#endif
+}
+__device__ inline float med_clamp(float x, float s, float e) { return med3_f(x, s, e); }
+__device__ inline float min_f(float a, float b) { return med3_f(a, b, -3.0e38f); }
+__device__ inline float max_f(float a, float b) { return med3_f(a, b, 3.0e38f); }
+
 inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
 
 // Read-only data loaded through the constant address space: wave-uniform

@@ -123,10 +123,10 @@ struct FPRow {        // one output row (b, iv) of a channel chunk
 // interpolation across chunks) is accumulated too.
 template <int CH, bool XC>
 __device__ inline void dd_line(float* acc, const float* __restrict__ row, int cs, float x1, float x2, int n, float scale) {
-    if (x1 > x2) { const float t = x1; x1 = x2; x2 = t; }
-    const float a = clampf(x1, 0.f, (float)n), b = clampf(x2, 0.f, (float)n);
+    const float lo = min_f(x1, x2), hi = max_f(x1, x2);
+    const float a = med_clamp(lo, 0.f, (float)n), b = med_clamp(hi, 0.f, (float)n);
     if (!(b > a)) return;
-    const float inv = scale / (x2 - x1);
+    const float inv = fast_div(scale, hi - lo);
     const int i1 = min((int)a, n - 1), i2 = min((int)b, n - 1);
     for (int i = i1; i <= i2; i++) {
         const float w = (fminf(b, (float)(i + 1)) - fmaxf(a, (float)i)) * inv;
@@ -323,10 +323,10 @@ __device__ inline void split_pixels(float* acc, const float* __restrict__ row, i
 template <int NQ, int LPC>
 __device__ inline void dd_line_split(float* acc, const float* __restrict__ row, int lane_off, int cs, float x1,
                                      float x2, int n, float scale) {
-    if (x1 > x2) { const float t = x1; x1 = x2; x2 = t; }
-    const float a = clampf(x1, 0.f, (float)n), b = clampf(x2, 0.f, (float)n);
+    const float lo = min_f(x1, x2), hi = max_f(x1, x2);
+    const float a = med_clamp(lo, 0.f, (float)n), b = med_clamp(hi, 0.f, (float)n);
     if (!(b > a)) return;
-    const float inv = scale / (x2 - x1);
+    const float inv = fast_div(scale, hi - lo);
     split_pixels<NQ, LPC>(acc, row, lane_off, cs, a, b, inv, min((int)a, n - 1), min((int)b, n - 1));
 }
 
@@ -346,12 +346,12 @@ __device__ inline void fp_trace_split(float* acc, const float* __restrict__ img,
         int mi1 = 0, mi2 = -1;   // empty
         if (t0 + sub < nrows) {
             const float tt = (float)(t0 + sub) + 0.5f;
-            float x1 = fmaf(r1, tt - o1, q1), x2 = fmaf(r2, tt - o2, q2);
-            if (x1 > x2) { const float t = x1; x1 = x2; x2 = t; }
-            const float a = clampf(x1, 0.f, (float)n), b = clampf(x2, 0.f, (float)n);
+            const float e1 = fmaf(r1, tt - o1, q1), e2 = fmaf(r2, tt - o2, q2);
+            const float x1 = min_f(e1, e2), x2 = max_f(e1, e2);
+            const float a = med_clamp(x1, 0.f, (float)n), b = med_clamp(x2, 0.f, (float)n);
             if (b > a) {
                 ma = a; mb = b;
-                minv = norm / (x2 - x1);
+                minv = fast_div(norm, x2 - x1);
                 mi1 = min((int)a, n - 1);
                 mi2 = min((int)b, n - 1);
             }
@@ -451,10 +451,10 @@ struct FPGroup {               // consecutive views with equal trace flags
 // in LDS at 'buf' (CHP floats per pixel); the same arithmetic as dd_line.
 template <int CH, int CHP>
 __device__ inline void dd_line_lds(float* acc, const float* buf, int xlo, float x1, float x2, int n, float scale) {
-    if (x1 > x2) { const float t = x1; x1 = x2; x2 = t; }
-    const float a = clampf(x1, 0.f, (float)n), b = clampf(x2, 0.f, (float)n);
+    const float lo = min_f(x1, x2), hi = max_f(x1, x2);
+    const float a = med_clamp(lo, 0.f, (float)n), b = med_clamp(hi, 0.f, (float)n);
     if (!(b > a)) return;
-    const float inv = scale / (x2 - x1);
+    const float inv = fast_div(scale, hi - lo);
     const int i1 = min((int)a, n - 1), i2 = min((int)b, n - 1);
     for (int i = i1; i <= i2; i++) {
         const float w = (fminf(b, (float)(i + 1)) - fmaxf(a, (float)i)) * inv;
@@ -1039,10 +1039,10 @@ void slices_to_channel_last(float* sino, const float* prj, const SliceRow* rows_
 // (a2 - a1), cuda/distanceDriven2DBoxInt.cu:375-383.
 template <int CH>
 __device__ inline void dd_cells(float* acc, const float* __restrict__ base, int cs, float a1, float a2, int n, float scale) {
-    if (a1 > a2) { const float t = a1; a1 = a2; a2 = t; }
-    const float a = clampf(a1, 0.f, (float)n), b = clampf(a2, 0.f, (float)n);
+    const float lo = min_f(a1, a2), hi = max_f(a1, a2);
+    const float a = med_clamp(lo, 0.f, (float)n), b = med_clamp(hi, 0.f, (float)n);
     if (!(b > a)) return;
-    const float inv = scale / (a2 - a1);
+    const float inv = fast_div(scale, hi - lo);
     const int i1 = min((int)a, n - 1), i2 = min((int)b, n - 1);
     for (int i = i1; i <= i2; i++) {
         const float w = (fminf(b, (float)(i + 1)) - fmaxf(a, (float)i)) * inv;
@@ -1093,26 +1093,26 @@ __global__ __launch_bounds__(256) void dd_bp_kernel(float* __restrict__ img, flo
             float p1x, p1y, p2x, p2y;
             if (V.flags & 4) { p1x = x1; p1y = yc; p2x = x2; p2y = yc; }
             else { p1x = xc; p1y = y1; p2x = xc; p2y = y2; }
-            const float a1 = atanf((p1x * c + p1y * s) / (-p1x * s + p1y * c + dso));
-            const float a2 = atanf((p2x * c + p2y * s) / (-p2x * s + p2y * c + dso));
-            const float u1 = -(a1 / d.du + d.off_u) + half;
-            const float u2 = -(a2 / d.du + d.off_u) + half;
+            const float a1 = atanf(fast_div(p1x * c + p1y * s, -p1x * s + p1y * c + dso));
+            const float a2 = atanf(fast_div(p2x * c + p2y * s, -p2x * s + p2y * c + dso));
+            const float u1 = -(fast_div(a1, d.du) + d.off_u) + half;
+            const float u2 = -(fast_div(a2, d.du) + d.off_u) + half;
             float scale = 1.f;
             if (fbp) {
                 const float rx = xc * c + yc * s, ry = -xc * s + yc * c;
-                scale = dso * dso / (rx * rx + (dso + ry) * (dso + ry));
+                scale = fast_div(dso * dso, rx * rx + (dso + ry) * (dso + ry));
             }
             dd_cells<CH>(acc, base, cs, u1, u2, d.nu, scale);
         } else {
             // cuda/distanceDriven2DBoxInt.cu:439-493 (blended main axes)
             if (V.flags & 1) {
-                const float u1 = -((x1 * c + yc * s) / d.du + d.off_u) + half;
-                const float u2 = -((x2 * c + yc * s) / d.du + d.off_u) + half;
+                const float u1 = -(fast_div(x1 * c + yc * s, d.du) + d.off_u) + half;
+                const float u2 = -(fast_div(x2 * c + yc * s, d.du) + d.off_u) + half;
                 dd_cells<CH>(acc, base, cs, u1, u2, d.nu, V.wx);
             }
             if (V.flags & 2) {
-                const float u1 = -((xc * c + y1 * s) / d.du + d.off_u) + half;
-                const float u2 = -((xc * c + y2 * s) / d.du + d.off_u) + half;
+                const float u1 = -(fast_div(xc * c + y1 * s, d.du) + d.off_u) + half;
+                const float u2 = -(fast_div(xc * c + y2 * s, d.du) + d.off_u) + half;
                 dd_cells<CH>(acc, base, cs, u1, u2, d.nu, 1.f - V.wx);
             }
         }
@@ -1166,13 +1166,13 @@ __global__ __launch_bounds__(256) void dd_bp_split_kernel(float* __restrict__ im
                 float p1x, p1y, p2x, p2y;
                 if (V.flags & 4) { p1x = x1; p1y = yc; p2x = x2; p2y = yc; }
                 else { p1x = xc; p1y = y1; p2x = xc; p2y = y2; }
-                const float a1 = atanf((p1x * c + p1y * s) / (-p1x * s + p1y * c + dso));
-                const float a2 = atanf((p2x * c + p2y * s) / (-p2x * s + p2y * c + dso));
-                mu1 = -(a1 / d.du + d.off_u) + half;
-                mu2 = -(a2 / d.du + d.off_u) + half;
+                const float a1 = atanf(fast_div(p1x * c + p1y * s, -p1x * s + p1y * c + dso));
+                const float a2 = atanf(fast_div(p2x * c + p2y * s, -p2x * s + p2y * c + dso));
+                mu1 = -(fast_div(a1, d.du) + d.off_u) + half;
+                mu2 = -(fast_div(a2, d.du) + d.off_u) + half;
                 if (fbp) {
                     const float rx = xc * c + yc * s, ry = -xc * s + yc * c;
-                    msc = dso * dso / (rx * rx + (dso + ry) * (dso + ry));
+                    msc = fast_div(dso * dso, rx * rx + (dso + ry) * (dso + ry));
                 }
             }
 #pragma unroll
@@ -1191,13 +1191,13 @@ __global__ __launch_bounds__(256) void dd_bp_split_kernel(float* __restrict__ im
             const float* __restrict__ base = sino + view * vstride + c0;
             // cuda/distanceDriven2DBoxInt.cu:439-493 (blended main axes)
             if (V.flags & 1) {
-                const float u1 = -((x1 * c + yc * s) / d.du + d.off_u) + half;
-                const float u2 = -((x2 * c + yc * s) / d.du + d.off_u) + half;
+                const float u1 = -(fast_div(x1 * c + yc * s, d.du) + d.off_u) + half;
+                const float u2 = -(fast_div(x2 * c + yc * s, d.du) + d.off_u) + half;
                 dd_line_split<NQ, LPC>(acc, base, 4 * sub, cs, u1, u2, d.nu, V.wx);
             }
             if (V.flags & 2) {
-                const float u1 = -((xc * c + y1 * s) / d.du + d.off_u) + half;
-                const float u2 = -((xc * c + y2 * s) / d.du + d.off_u) + half;
+                const float u1 = -(fast_div(xc * c + y1 * s, d.du) + d.off_u) + half;
+                const float u2 = -(fast_div(xc * c + y2 * s, d.du) + d.off_u) + half;
                 dd_line_split<NQ, LPC>(acc, base, 4 * sub, cs, u1, u2, d.nu, 1.f - V.wx);
             }
         }
@@ -1502,7 +1502,7 @@ __global__ __launch_bounds__(256) void dd_fp_adjoint_kernel(float* __restrict__ 
                 if (x1 > x2) { const float tt = x1; x1 = x2; x2 = tt; }
                 const float a = clampf(x1, 0.f, (float)n), b = clampf(x2, 0.f, (float)n);
                 if (!(b > a)) continue;
-                const float w = (fminf(b, lo + 1.f) - fmaxf(a, lo)) * (sc / (x2 - x1));
+                const float w = (fminf(b, lo + 1.f) - fmaxf(a, lo)) * fast_div(sc, x2 - x1);
                 if (w > 0.f) adj_apply<CH>(acc, Gv + (size_t)k * cs, w);
             }
         }

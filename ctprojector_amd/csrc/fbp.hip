@@ -696,9 +696,9 @@ __global__ __launch_bounds__(256) void fan_fbp_bp_kernel(float* __restrict__ img
         const FBPView V = views[view];
         const float rx = px * V.c + py * V.s;
         const float ry = -px * V.s + py * V.c;
-        const float a = FAN ? atanf(rx / (ry + dso)) : rx;
-        const float dist = FAN ? dso * dso / (rx * rx + (dso + ry) * (dso + ry)) : 1.f;
-        const float pu = -(a / d.du + d.off_u) + half;
+        const float a = FAN ? atanf(fast_div(rx, ry + dso)) : rx;
+        const float dist = FAN ? fast_div(dso * dso, rx * rx + (dso + ry) * (dso + ry)) : 1.f;
+        const float pu = -(fast_div(a, d.du) + d.off_u) + half;
         // InterpolateXY with truncation, cuda/cudaMath.cu:227-260
         if (pu < 0.f || pu >= (float)d.nu) continue;
         const int u0 = (int)pu;
@@ -764,9 +764,9 @@ __global__ __launch_bounds__(256) void fan_fbp_bp_split_kernel(float* __restrict
             const FBPView V = views[v0 + sub];
             const float rx = px * V.c + py * V.s;
             const float ry = -px * V.s + py * V.c;
-            const float a = FAN ? atanf(rx / (ry + dso)) : rx;
-            mdist = FAN ? dso * dso / (rx * rx + (dso + ry) * (dso + ry)) : 1.f;
-            mpu = -(a / d.du + d.off_u) + half;
+            const float a = FAN ? atanf(fast_div(rx, ry + dso)) : rx;
+            mdist = FAN ? fast_div(dso * dso, rx * rx + (dso + ry) * (dso + ry)) : 1.f;
+            mpu = -(fast_div(a, d.du) + d.off_u) + half;
         }
 #pragma unroll
         for (int t = 0; t < LPC; t++) {

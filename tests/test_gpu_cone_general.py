@@ -82,6 +82,35 @@ def test_reference_example_detector(oracle):
     _check(oracle, g, geo, img)
 
 
+def test_reference_example_detector_large_volume(oracle):
+    """The example detector on a 256 x 256 x 96 volume of 1 mm voxels (ray-driven
+    backprojection on a grid where accumulated crossings would drift): FP and
+    BP against the FP64 oracle at 1e-4; the reference's own FP32 arithmetic is
+    printed beside it."""
+    from ctprojector_amd.phantom import circular_cone_geometry, ellipsoid_phantom
+    from ctprojector_amd.projector.numpy import cone
+    from ctprojector_amd.projector.numpy.ct_projector import ct_projector
+    g = Geo(nx=256, ny=256, nz=96, dx=1.0, dy=1.0, dz=1.0, cx=0.7, cy=-0.4, nu=1200, nv=480, du=0.25, dv=0.25,
+            off_u=0.3, off_v=-0.2, dsd=3000.0, dso=2700.0)
+    geo = circular_cone_geometry(2, 2700.0, 3000.0, -1.0, 1.0, jitter=0.5, seed=41, angle_range=np.pi / 2)
+    img = ellipsoid_phantom(96, 256, 256, 1.0, 1.0, 1.0, seed=41)[None]
+    p = g.to(ct_projector())
+    args = (geo.det_center, geo.det_u, geo.det_v, geo.src)
+    _say('example detector 1200x480 @0.25 mm, 256x256x96 @1 mm: FP')
+    fp = cone.siddon_fp_arbitrary(p, img, *args)
+    e = rel_l2(fp, oracle.siddon_cone_fp(g, img, *args))
+    _say('fp rel-L2 %.2e' % e)
+    assert e < TOL
+    y = np.random.default_rng(6).uniform(size=fp.shape).astype(np.float32)
+    _say('BP (ray-driven)')
+    bp = cone.siddon_bp_arbitrary(p, y, *args)
+    ref = oracle.siddon_cone_bp(g, y, *args)
+    e = rel_l2(bp, ref)
+    e32 = rel_l2(oracle.siddon_cone_bp(g, y, *args, prec='f'), ref)
+    _say('bp rel-L2 %.2e (the reference arithmetic in FP32: %.2e)' % (e, e32))
+    assert e < TOL
+
+
 @pytest.mark.parametrize('dso', [8.0, 15.0])
 def test_source_inside_or_near_volume(oracle, dso):
     """A 24^3 volume of 1 mm voxels (half width 12 mm, corner radius 20.8 mm)

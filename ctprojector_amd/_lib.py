@@ -32,6 +32,7 @@ SYMBOLS = [
     'cupyTVSQS3D', 'cTVSQS3D', 'ctpTVSQSUpdate3D', 'ctpTVSQSBackward3D',
     'ctpDDPlanCreate', 'ctpDDPlanForward', 'ctpDDPlanBackward', 'ctpDDPlanAdjoint', 'ctpDDPlanDestroy',
     'ctpConePlanCreate', 'ctpConePlanForward', 'ctpConePlanBackward', 'ctpConePlanConfig', 'ctpConePlanDestroy',
+    'ctpDDBatchCreate', 'ctpDDBatchForward', 'ctpDDBatchBackward', 'ctpDDBatchConfig', 'ctpDDBatchDestroy',
     'ctpGetLastErrorMessage', 'ctpReleaseWorkspace', 'ctpSetStream',
     'ctpSetKernelTiming', 'ctpResetKernelTimes', 'ctpGetKernelTime', 'ctpKernelTimeReport',
     'ctpSetTuning', 'ctpGetTuning',

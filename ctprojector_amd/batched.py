@@ -1,11 +1,19 @@
-"""Batched cone-beam projection: many volumes of different sizes, each with its
-own trajectory and detector, forward- and back-projected by one persistent
-launch per direction (``ctpConePlan*`` of libprojector.so), with launch shapes
-chosen by the library's autotuner and the launches capturable in a HIP graph.
+"""Batched projection of uneven jobs: many volumes or slice stacks of different
+sizes, each with its own geometry, forward- and back-projected by a few
+launches per direction, with launch shapes chosen by the library's autotuner
+and the launches capturable in a HIP graph.
+
+* ``ConeBatch``: cone-beam Siddon jobs, each with its own trajectory and
+  detector; one persistent launch forward, one per backprojection window
+  shape (``ctpConePlan*`` of libprojector.so).
+* ``DDBatch``: fan-beam (equiangular) and parallel-beam distance-driven jobs,
+  each with its own grid, detector, angles and batch count (``ctpDDBatch*``);
+  the jobs whose kernels are the same template instance share a launch.
 
 The reference projects a batch by looping over its items on the host with a
 launch and a device synchronisation per item (cuda/siddonCone.cu:76-135); its
-batches must share one geometry and one size.
+batches must share one geometry and one size, and its DD entry points take one
+geometry per call (cuda/distanceDriven2D.cu:109-634).
 
     jobs = [ConeJob(projector_a, geo_a), ConeJob(projector_b, geo_b), ...]
     batch = ConeBatch(jobs, autotune=True)
@@ -121,6 +129,128 @@ class ConeBatch:
         if getattr(self, '_plan', None) is not None and self._plan.value:
             torch.cuda.synchronize(self.device)
             _lib.module().ctpConePlanDestroy(self._plan)
+            self._plan = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class _DJob(ctypes.Structure):
+    _fields_ = [('fan', c_int), ('typeProjector', c_int),
+                ('nBatches', c_size_t), ('nx', c_size_t), ('ny', c_size_t), ('nz', c_size_t),
+                ('dx', c_float), ('dy', c_float), ('dz', c_float), ('cx', c_float), ('cy', c_float), ('cz', c_float),
+                ('nu', c_size_t), ('nv', c_size_t), ('nview', c_size_t),
+                ('da', c_float), ('dv', c_float), ('off_a', c_float), ('off_v', c_float),
+                ('dsd', c_float), ('dso', c_float),
+                ('deg', c_void_p), ('img', c_void_p), ('prj', c_void_p)]
+
+
+@dataclass
+class DDJob:
+    """One distance-driven job: ``projector`` carries nx..nz, dx..cz, nu, nv,
+    du, dv, off_u, off_v and, for the fan beam, dsd and dso (the reference's
+    ct_projector attributes; du is the detector spacing at dsd, as for
+    fan_equiangular.distance_driven_fp); ``angles`` are the view angles in
+    radians; ``batch`` items share the geometry; ``is_fbp`` (1),
+    ``branchless`` (2) and ``force_fbp`` (4) are the typeProjector bits of
+    the reference's DD entry points."""
+    projector: object
+    angles: object
+    fan: bool = True
+    batch: int = 1
+    is_fbp: bool = False
+    branchless: bool = False
+    force_fbp: bool = False
+
+
+class DDBatch:
+    """Buffers ``img[j]`` ([batch, nz, ny, nx]) and ``prj[j]`` ([batch, nview,
+    nv, nu]) are device tensors owned by the batch and bound to it; forward
+    writes prj[j] = A_j img[j] (fan_equiangular / parallel
+    distance_driven_fp), backward writes img[j] = the reference's
+    backprojection of prj[j] (distance_driven_bp with the job's bits).  Every
+    job's result is that of its single call.  Calls are asynchronous on
+    torch's current stream."""
+
+    def __init__(self, jobs: Sequence[DDJob], autotune: bool = False, device=None):
+        self.device = torch.device('cuda', torch.cuda.current_device()) if device is None else torch.device(device)
+        self.img: List[torch.Tensor] = []
+        self.prj: List[torch.Tensor] = []
+        self._host = []
+        cj = (_DJob * len(jobs))()
+        for k, job in enumerate(jobs):
+            p = job.projector
+            ang = job.angles
+            if isinstance(ang, torch.Tensor):
+                ang = ang.detach().cpu().numpy()
+            ang = _f32(ang).ravel()
+            self._host.append(ang)
+            nview, b = int(ang.shape[0]), int(job.batch)
+            img = torch.zeros([b, p.nz, p.ny, p.nx], dtype=torch.float32, device=self.device)
+            prj = torch.zeros([b, nview, p.nv, p.nu], dtype=torch.float32, device=self.device)
+            self.img.append(img)
+            self.prj.append(prj)
+            fan = bool(job.fan)
+            typ = (1 if job.is_fbp else 0) + (2 if job.branchless else 0) + (4 if job.force_fbp else 0)
+            cj[k] = _DJob(1 if fan else 0, typ, b, p.nx, p.ny, p.nz, p.dx, p.dy, p.dz, p.cx, p.cy, p.cz,
+                          p.nu, p.nv, nview, p.du / p.dsd if fan else p.du, p.dv, p.off_u, p.off_v,
+                          p.dsd if fan else 0.0, p.dso if fan else 0.0, ang.ctypes.data, img.data_ptr(),
+                          prj.data_ptr())
+        self._plan = c_void_p()
+        torch.cuda.synchronize(self.device)
+        with torch.cuda.device(self.device):
+            err = _lib.module().ctpDDBatchCreate(ctypes.byref(self._plan), cj, c_int(len(jobs)),
+                                                 c_int(1 if autotune else 0))
+        if err != 0:
+            raise RuntimeError('ctpDDBatchCreate failed (%d): %s' % (err, _lib.last_error_message()))
+
+    def _stream(self):
+        return c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def forward(self) -> None:
+        """prj[j] = A_j img[j] for every job (asynchronous on the current stream)."""
+        with torch.cuda.device(self.device):
+            err = _lib.module().ctpDDBatchForward(self._plan, self._stream())
+        if err != 0:
+            raise RuntimeError('ctpDDBatchForward failed (%d): %s' % (err, _lib.last_error_message()))
+
+    def backward(self) -> None:
+        """img[j] = the reference's backprojection of prj[j] for every job."""
+        with torch.cuda.device(self.device):
+            err = _lib.module().ctpDDBatchBackward(self._plan, self._stream())
+        if err != 0:
+            raise RuntimeError('ctpDDBatchBackward failed (%d): %s' % (err, _lib.last_error_message()))
+
+    def config(self) -> dict:
+        """Launch shapes in use (after autotuning) and kernel launches per call."""
+        v = [c_int() for _ in range(6)]
+        err = _lib.module().ctpDDBatchConfig(self._plan, *[ctypes.byref(x) for x in v])
+        if err != 0:
+            raise RuntimeError('ctpDDBatchConfig failed (%d): %s' % (err, _lib.last_error_message()))
+        keys = ('dd_fp_lanes_per_column', 'dd_bp_lanes_per_pixel', 'fp_persistent', 'bp_persistent',
+                'fp_launches', 'bp_launches')
+        return {k: x.value for k, x in zip(keys, v)}
+
+    def capture(self, which: str = 'forward') -> torch.cuda.CUDAGraph:
+        """A HIP graph of ``forward``, ``backward`` or ``both``."""
+        steps = {'forward': [self.forward], 'backward': [self.backward],
+                 'both': [self.forward, self.backward]}[which]
+        for f in steps:
+            f()
+        torch.cuda.synchronize(self.device)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            for f in steps:
+                f()
+        return g
+
+    def close(self) -> None:
+        if getattr(self, '_plan', None) is not None and self._plan.value:
+            torch.cuda.synchronize(self.device)
+            _lib.module().ctpDDBatchDestroy(self._plan)
             self._plan = c_void_p()
 
     def __del__(self):

@@ -835,3 +835,94 @@ CTP_EXPORT int ctpConePlanDestroy(void* plan) {
         delete static_cast<ConePlan*>(plan);
     });
 }
+
+// ===========================================================================
+// Batched plans of uneven distance-driven jobs (no counterpart in the reference)
+// ===========================================================================
+struct ctpDDJob {
+    int fan;
+    int typeProjector;
+    size_t nBatches, nx, ny, nz;
+    float dx, dy, dz, cx, cy, cz;
+    size_t nu, nv, nview;
+    float da, dv, off_a, off_v;
+    float dsd, dso;
+    const float* deg;
+    float* img;
+    float* prj;
+};
+
+CTP_EXPORT int ctpDDBatchCreate(void** batch, const ctpDDJob* jobs, int njobs, int autotune) {
+    return guarded("ctpDDBatchCreate", [&] {
+        need_ptr(batch, "batch");
+        *batch = nullptr;
+        need(njobs > 0 && jobs != nullptr, "no jobs");
+        std::vector<DDBatchJobSpec> specs;
+        for (int j = 0; j < njobs; j++) {
+            const ctpDDJob& J = jobs[j];
+            need(!empty(J.nBatches, J.nx, J.ny, J.nz, J.nu, J.nv, J.nview), "empty job");
+            need(J.nBatches <= kIntMax && J.nBatches * J.nz <= kIntMax, "batch too large");
+            need_ptr(J.deg, "angles");
+            need_ptr(J.img, "img");
+            need_ptr(J.prj, "prj");
+            DDBatchJobSpec S{};
+            S.fan = J.fan != 0;
+            S.type = J.typeProjector;
+            S.B = (int)J.nBatches;
+            S.g = make_grid(J.nx, J.ny, J.nz, J.dx, J.dy, J.dz, J.cx, J.cy, J.cz);
+            S.d = make_det(J.nu, J.nv, J.nview, J.da, J.dv, J.off_a, J.off_v);
+            S.nview = (int)J.nview;
+            // the parallel-beam kernels take no source distances (the single
+            // parallel entry points pass these placeholders too)
+            S.dsd = S.fan ? J.dsd : 1000.f;
+            S.dso = S.fan ? J.dso : 500.f;
+            S.deg = J.deg;
+            S.img = J.img;
+            S.prj = J.prj;
+            specs.push_back(S);
+        }
+        touch();
+        *batch = new DDBatchPlan(specs, autotune != 0);
+    });
+}
+
+CTP_EXPORT int ctpDDBatchForward(void* batch, void* stream) {
+    return guarded("ctpDDBatchForward", [&] {
+        need_ptr(batch, "batch");
+        touch();
+        static_cast<DDBatchPlan*>(batch)->forward(static_cast<hipStream_t>(stream));
+    });
+}
+
+CTP_EXPORT int ctpDDBatchBackward(void* batch, void* stream) {
+    return guarded("ctpDDBatchBackward", [&] {
+        need_ptr(batch, "batch");
+        touch();
+        static_cast<DDBatchPlan*>(batch)->backward(static_cast<hipStream_t>(stream));
+    });
+}
+
+CTP_EXPORT int ctpDDBatchConfig(void* batch, int* fp_lanes, int* bp_lanes, int* fp_persistent, int* bp_persistent,
+                                int* fp_launches, int* bp_launches) {
+    return guarded("ctpDDBatchConfig", [&] {
+        need_ptr(batch, "batch");
+        const DDBatchPlan* b = static_cast<const DDBatchPlan*>(batch);
+        int a = 0, c = 0, e = 0, f = 0;
+        b->config(a, c, e, f);
+        if (fp_lanes) *fp_lanes = a;
+        if (bp_lanes) *bp_lanes = c;
+        if (fp_persistent) *fp_persistent = e;
+        if (bp_persistent) *bp_persistent = f;
+        if (fp_launches) *fp_launches = b->launches(true);
+        if (bp_launches) *bp_launches = b->launches(false);
+    });
+}
+
+CTP_EXPORT int ctpDDBatchDestroy(void* batch) {
+    return guarded("ctpDDBatchDestroy", [&] {
+        if (!batch) return;
+        touch();
+        CTP_CHECK(hipDeviceSynchronize());
+        delete static_cast<DDBatchPlan*>(batch);
+    });
+}

@@ -87,10 +87,11 @@ static void hard_switch(std::vector<DDView>& v) {
 // planes P[c][y][x] (c < C)  ->  O[major * rs + minor * cs + c] with (major,
 // minor) = (y, x) (xmajor=false) or (x, y) (xmajor=true); rs >= minor extent *
 // cs is the row stride; channels >= C are written as zero.
-__global__ __launch_bounds__(256) void to_channel_last(float* __restrict__ out, const float* __restrict__ in, int C,
-                                                       int cs, int nx, int ny, bool xmajor, size_t rs) {
+// Block (bx, by, bz) = (32 pixels of a row, 32 channels, row y) of the copy.
+__device__ inline void to_channel_last_block(float* __restrict__ out, const float* __restrict__ in, int C, int cs,
+                                             int nx, int ny, bool xmajor, size_t rs, int bx, int by, int bz) {
     __shared__ float tile[32][33];
-    const int x0 = blockIdx.x * 32, c0 = blockIdx.y * 32, y = blockIdx.z;
+    const int x0 = bx * 32, c0 = by * 32, y = bz;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
     for (int k = ty; k < 32; k += 8) {
         const int c = c0 + k, x = x0 + tx;
@@ -104,6 +105,11 @@ __global__ __launch_bounds__(256) void to_channel_last(float* __restrict__ out, 
             out[at + c] = tile[tx][k];
         }
     }
+}
+
+__global__ __launch_bounds__(256) void to_channel_last(float* __restrict__ out, const float* __restrict__ in, int C,
+                                                       int cs, int nx, int ny, bool xmajor, size_t rs) {
+    to_channel_last_block(out, in, C, cs, nx, ny, xmajor, rs, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 // ---------------------------------------------------------------------------
@@ -151,24 +157,27 @@ __device__ inline void dd_line(float* acc, const float* __restrict__ row, int cs
 // The two edge rays of detector column iu in image index coordinates: from
 // s1 to d1 and from s2 to d2 (cuda/distanceDriven2DBoxInt.cu:141-158 parallel
 // with its virtual source distance, :211-226 fan).  Returns the column's fan
-// angle (0 for parallel beams).
+// angle (0 for parallel beams).  The rotations and sums are written as
+// explicit fused multiply-adds, so that every kernel inlining this function
+// rounds them alike (the compiler's own contraction of a*b + c*d may fuse
+// either product, depending on the surrounding code).
 template <bool FAN>
 __device__ inline float fp_cell_edges(int iu, const DDView& V, const Grid& g, const Det& d, float dsd, float dso,
                                       float2& s1, float2& s2, float2& d1, float2& d2) {
     const float c = V.c, s = V.s;
     float a = 0.f;
+    auto rot = [&](float px, float py) { return make_float2(fmaf(px, c, -(py * s)), fmaf(px, s, py * c)); };
     if (FAN) {
         a = (-((float)iu - (float)(d.nu - 1) / 2.0f) - d.off_u) * d.du;
         const float2 src = make_float2(dso * s, -dso * c);
-        const float e1x = dsd * sinf(a - d.du / 2), e1y = -dso + dsd * cosf(a - d.du / 2);
-        const float e2x = dsd * sinf(a + d.du / 2), e2y = -dso + dsd * cosf(a + d.du / 2);
+        const float e1x = dsd * sinf(a - d.du / 2), e1y = fmaf(dsd, cosf(a - d.du / 2), -dso);
+        const float e2x = dsd * sinf(a + d.du / 2), e2y = fmaf(dsd, cosf(a + d.du / 2), -dso);
         s1 = s2 = src;
-        d1 = make_float2(e1x * c - e1y * s, e1x * s + e1y * c);
-        d2 = make_float2(e2x * c - e2y * s, e2x * s + e2y * c);
+        d1 = rot(e1x, e1y);
+        d2 = rot(e2x, e2y);
     } else {
         const float u = (-((float)iu - (float)(d.nu - 1) / 2.0f) - d.off_u) * d.du;
-        const float vd = (float)g.nx * g.dx + (float)g.ny * g.dy;
-        auto rot = [&](float px, float py) { return make_float2(px * c - py * s, px * s + py * c); };
+        const float vd = fmaf((float)g.nx, g.dx, (float)g.ny * g.dy);
         s1 = rot(u - d.du / 2, -vd);
         s2 = rot(u + d.du / 2, -vd);
         d1 = rot(u - d.du / 2, vd);
@@ -193,19 +202,20 @@ __device__ inline float fp_cell_edges(int iu, const DDView& V, const Grid& g, co
 // the wavefronts of a block read the same image lines at about the same time
 // and three of four find them in the CU's L1 (measured at 512x512x32, 984
 // views: 6.7 ms with one view per block, 6.7 ms with two, 5.8 ms with four).
+// Block (bx, by, bz) = (column block, view or view group, channel chunk).
 template <int CH, bool FAN, bool XC, int VPB>
-__global__ __launch_bounds__(VPB > 1 ? 64 * VPB : 128) void dd_fp_kernel(
+__device__ inline void dd_fp_block(
         float* __restrict__ prj, const float* __restrict__ imgY, const float* __restrict__ imgX,
         const DDView* __restrict__ views, const FPRow* __restrict__ rows, const int* __restrict__ row_start, int cs,
-        Grid g, Det d, int nview, float dsd, float dso, bool force, int lseg) {
+        const Grid& g, const Det& d, int nview, float dsd, float dso, bool force, int lseg, int bx, int by, int bz) {
     __shared__ float lds[VPB > 1 ? 64 * VPB : 128][CH + 2];
     const int lcol = VPB > 1 ? 6 : 7 - lseg;
     const int col = threadIdx.x & ((1 << lcol) - 1);
     const int seg = VPB > 1 ? 0 : __builtin_amdgcn_readfirstlane(threadIdx.x >> lcol);   // uniform per wavefront
     const int nseg = VPB > 1 ? 1 : 1 << lseg;
-    const int iu = (blockIdx.x << lcol) + col;
-    const int view = VPB > 1 ? blockIdx.y * VPB + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) : blockIdx.y;
-    const int chunk = blockIdx.z;
+    const int iu = (bx << lcol) + col;
+    const int view = VPB > 1 ? by * VPB + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) : by;
+    const int chunk = bz;
     if (VPB > 1 && view >= nview) return;   // no barriers in this form
     const int c0 = chunk * CH;
     const DDView V = views[view];
@@ -270,6 +280,15 @@ __global__ __launch_bounds__(VPB > 1 ? 64 * VPB : 128) void dd_fp_kernel(
         const float val = (1.f - R.w1) * lds[mine][R.j0] + R.w1 * lds[mine][R.j1];
         prj[(((size_t)R.b * nview + view) * d.nv + R.iv) * d.nu + iu] = val * rev;
     }
+}
+
+template <int CH, bool FAN, bool XC, int VPB>
+__global__ __launch_bounds__(VPB > 1 ? 64 * VPB : 128) void dd_fp_kernel(
+        float* __restrict__ prj, const float* __restrict__ imgY, const float* __restrict__ imgX,
+        const DDView* __restrict__ views, const FPRow* __restrict__ rows, const int* __restrict__ row_start, int cs,
+        Grid g, Det d, int nview, float dsd, float dso, bool force, int lseg) {
+    dd_fp_block<CH, FAN, XC, VPB>(prj, imgY, imgX, views, rows, row_start, cs, g, d, nview, dsd, dso, force, lseg,
+                                  blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 // ---------------------------------------------------------------------------
@@ -367,21 +386,22 @@ __device__ inline void fp_trace_split(float* acc, const float* __restrict__ img,
     }
 }
 
+// Block (bx, by) = (64 / LPC detector columns, 4 consecutive views).
 template <int CH, int LPC, bool FAN>
-__global__ __launch_bounds__(256) void dd_fp_split_kernel(
+__device__ inline void dd_fp_split_block(
         float* __restrict__ prj, const float* __restrict__ imgY, const float* __restrict__ imgX,
-        const DDView* __restrict__ views, const FPRow* __restrict__ rows, int nrow, Grid g, Det d, int nview,
-        float dsd, float dso, bool force) {
+        const DDView* __restrict__ views, const FPRow* __restrict__ rows, int nrow, const Grid& g, const Det& d,
+        int nview, float dsd, float dso, bool force, int bx, int by) {
     static_assert(CH % (4 * LPC) == 0, "a lane holds whole float4s");
     constexpr int NQ = CH / (4 * LPC);   // float4s per lane and pixel
     constexpr int NC = 64 / LPC;         // columns per wavefront
     __shared__ float lds[4][NC][CH + 1];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    const int view = blockIdx.y * 4 + wave;
+    const int view = by * 4 + wave;
     if (view >= nview) return;   // every exchange below stays within the wavefront
     const int c = lane / LPC, sub = lane % LPC;
-    const int iu = blockIdx.x * NC + c;
+    const int iu = bx * NC + c;
     const DDView V = views[view];
     float acc[4 * NQ];
 #pragma unroll
@@ -418,7 +438,7 @@ __global__ __launch_bounds__(256) void dd_fp_split_kernel(
     // the LDS writes above are complete before any lane of the wavefront reads
     asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
     const int oc = lane % NC, rg = lane / NC;
-    const int ou = blockIdx.x * NC + oc;
+    const int ou = bx * NC + oc;
     if (ou >= d.nu) return;
     const float rev = (!FAN && force) ? V.rev : 1.f;
     const float* mine = &lds[wave][oc][0];
@@ -427,6 +447,15 @@ __global__ __launch_bounds__(256) void dd_fp_split_kernel(
         const float val = (1.f - R.w1) * mine[R.j0] + R.w1 * mine[R.j1];
         prj[(((size_t)R.b * nview + view) * d.nv + R.iv) * d.nu + ou] = val * rev;
     }
+}
+
+template <int CH, int LPC, bool FAN>
+__global__ __launch_bounds__(256) void dd_fp_split_kernel(
+        float* __restrict__ prj, const float* __restrict__ imgY, const float* __restrict__ imgX,
+        const DDView* __restrict__ views, const FPRow* __restrict__ rows, int nrow, Grid g, Det d, int nview,
+        float dsd, float dso, bool force) {
+    dd_fp_split_block<CH, LPC, FAN>(prj, imgY, imgX, views, rows, nrow, g, d, nview, dsd, dso, force, blockIdx.x,
+                                    blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -699,8 +728,11 @@ struct DDFPSetup {
     long rsY = 0, rsX = 0;   // row strides (floats) of the y-major and x-major copies (n * cs)
 };
 
+// allow_lds: the LDS-staged form may be chosen (by the dd_fp_mode tuning key);
+// lpc_req > 0: lanes per column requested instead of the tuning key, reduced
+// to what the chunk allows (CH / 4; 1 = the one-lane kernel).
 static DDFPSetup dd_fp_setup(const std::vector<DDView>& views, int B, const Grid& g, const Det& d, int nview,
-                             float dsd, float dso, int type, bool fan) {
+                             float dsd, float dso, int type, bool fan, bool allow_lds = true, int lpc_req = 0) {
     const bool branchless = (type & DD_BRANCHLESS) != 0;
     DDFPSetup s;
     s.C = B * g.nz;
@@ -748,7 +780,7 @@ static DDFPSetup dd_fp_setup(const std::vector<DDView>& views, int B, const Grid
     const int wcap = 65536 / (2 * CHP * 4);   // two row buffers in 64 KiB of LDS
     // the direct kernel is the default (measured faster at 512x512x32: the
     // staged kernel's prefetch registers spill)
-    const bool want_lds = tuning_get("dd_fp_mode", 1) == 2 && s.groups.size() <= 65535;
+    const bool want_lds = allow_lds && tuning_get("dd_fp_mode", 1) == 2 && s.groups.size() <= 65535;
     const double wneed = want_lds ? fp_stage_width(views, s.groups, g, d, dsd, dso, fan) + 6.0 : 1e300;
     s.use_lds = want_lds && wneed <= (double)wcap;
     s.wmax = s.use_lds ? (int)std::ceil(wneed) : 0;
@@ -777,7 +809,7 @@ static DDFPSetup dd_fp_setup(const std::vector<DDView>& views, int B, const Grid
     s.lseg = lseg;
     // lanes per column: the split kernel needs one chunk without the next
     // chunk's channel, whole float4s per lane and unpadded CH-float pixels
-    const int lpc = tuning_get("dd_fp_lanes_per_column", 4);
+    const int lpc = lpc_req > 0 ? std::min(lpc_req, std::max(CH / 4, 1)) : tuning_get("dd_fp_lanes_per_column", 4);
     if (!s.xc && lseg == 0 && (CH == 8 || CH == 16 || CH == 32) && (lpc == 2 || lpc == 4 || lpc == 8) &&
         CH % (4 * lpc) == 0) {
         s.lpc = lpc;
@@ -993,12 +1025,13 @@ void dd2d_fp_noalloc(float* prj, const float* img, const float* deg_host, int B,
 // clamped linear interpolation between rows of InterpolateXY
 // (cuda/distanceDriven2DBranchless.cu:444-447, cuda/cudaMath.cu:227-260); the
 // u-interpolation of the prefix sums there is the exact box integral used below.
-__global__ __launch_bounds__(256) void dd_bp_prep_kernel(float* __restrict__ sino, const float* __restrict__ prj,
-                                                         const DDView* __restrict__ views, const SliceRow* __restrict__ slice_row,
-                                                         int C, int cs, int nz, Det d, int nview, float dx, float dy,
-                                                         bool fan, bool preweight) {
+// Block (bx, by, bz) = (32 columns, 32 channels, view).
+__device__ inline void dd_bp_prep_block(float* __restrict__ sino, const float* __restrict__ prj,
+                                        const DDView* __restrict__ views, const SliceRow* __restrict__ slice_row, int C,
+                                        int cs, int nz, const Det& d, int nview, float dx, float dy, bool fan,
+                                        bool preweight, int bx, int by, int bz) {
     __shared__ float tile[32][33];
-    const int u0 = blockIdx.x * 32, c0 = blockIdx.y * 32, view = blockIdx.z;
+    const int u0 = bx * 32, c0 = by * 32, view = bz;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
     const DDView V = preweight ? views[view] : DDView{};
     for (int k = ty; k < 32; k += 8) {
@@ -1024,6 +1057,14 @@ __global__ __launch_bounds__(256) void dd_bp_prep_kernel(float* __restrict__ sin
         const int u = u0 + k, c = c0 + tx;
         if (u < d.nu && c < cs) sino[((size_t)view * d.nu + u) * cs + c] = tile[tx][k];
     }
+}
+
+__global__ __launch_bounds__(256) void dd_bp_prep_kernel(float* __restrict__ sino, const float* __restrict__ prj,
+                                                         const DDView* __restrict__ views, const SliceRow* __restrict__ slice_row,
+                                                         int C, int cs, int nz, Det d, int nview, float dx, float dy,
+                                                         bool fan, bool preweight) {
+    dd_bp_prep_block(sino, prj, views, slice_row, C, cs, nz, d, nview, dx, dy, fan, preweight, blockIdx.x, blockIdx.y,
+                     blockIdx.z);
 }
 
 void slices_to_channel_last(float* sino, const float* prj, const SliceRow* rows_dev, int C, int cs, int nz,
@@ -1063,15 +1104,15 @@ __device__ inline void dd_cells(float* acc, const float* __restrict__ base, int 
     }
 }
 
+// Block (bx, by, bz) = (16 x 16 pixels, view group x channel chunk).
 template <int CH, bool FAN>
-__global__ __launch_bounds__(256) void dd_bp_kernel(float* __restrict__ img, float* __restrict__ part,
-                                                    const float* __restrict__ sino, const DDView* __restrict__ views,
-                                                    int C, int cs, int nchunk, int vpg, Grid g, Det d, int nview,
-                                                    float dso, bool fbp) {
-    const int ix = blockIdx.x * 16 + (threadIdx.x & 15);
-    const int iy = blockIdx.y * 16 + (threadIdx.x >> 4);
-    const int grp = blockIdx.z / nchunk;
-    const int c0 = (blockIdx.z - grp * nchunk) * CH;
+__device__ inline void dd_bp_block(float* __restrict__ img, float* __restrict__ part, const float* __restrict__ sino,
+                                   const DDView* __restrict__ views, int C, int cs, int nchunk, int vpg, const Grid& g,
+                                   const Det& d, int nview, float dso, bool fbp, int bx, int by, int bz) {
+    const int ix = bx * 16 + (threadIdx.x & 15);
+    const int iy = by * 16 + (threadIdx.x >> 4);
+    const int grp = bz / nchunk;
+    const int c0 = (bz - grp * nchunk) * CH;
     const int vbeg = grp * vpg, vend = min(nview, vbeg + vpg);
     if (ix >= g.nx || iy >= g.ny) return;
     // pixel centre and edges, cuda/distanceDriven2DBoxInt.cu:340-349
@@ -1120,6 +1161,15 @@ __global__ __launch_bounds__(256) void dd_bp_kernel(float* __restrict__ img, flo
     store_bp_result<CH>(img, part, acc, grp, c0, C, (size_t)g.nx * g.ny, (size_t)iy * g.nx + ix);
 }
 
+template <int CH, bool FAN>
+__global__ __launch_bounds__(256) void dd_bp_kernel(float* __restrict__ img, float* __restrict__ part,
+                                                    const float* __restrict__ sino, const DDView* __restrict__ views,
+                                                    int C, int cs, int nchunk, int vpg, Grid g, Det d, int nview,
+                                                    float dso, bool fbp) {
+    dd_bp_block<CH, FAN>(img, part, sino, views, C, cs, nchunk, vpg, g, d, nview, dso, fbp, blockIdx.x, blockIdx.y,
+                         blockIdx.z);
+}
+
 // The backprojection above with LPC lanes per pixel, for the reason given at
 // dd_fp_split_kernel: lane s of a pixel holds the float4s at channel offsets
 // 4 (s + LPC k), so a load instruction reads one contiguous 16 LPC-byte piece
@@ -1127,20 +1177,21 @@ __global__ __launch_bounds__(256) void dd_bp_kernel(float* __restrict__ img, flo
 // cover fewer cells.  In the fan beam the views' footprints (two atans each)
 // are spread over the pixel's lanes.  A block is 2 x 2 wavefront tiles.  Same arithmetic per channel as
 // dd_bp_kernel.
+// Block (bx, by, bz) = (2 x 2 wavefront tiles of pixels, view group x channel chunk).
 template <int CH, int LPC, bool FAN>
-__global__ __launch_bounds__(256) void dd_bp_split_kernel(float* __restrict__ img, float* __restrict__ part,
-                                                          const float* __restrict__ sino,
-                                                          const DDView* __restrict__ views, int C, int cs, int nchunk,
-                                                          int vpg, Grid g, Det d, int nview, float dso, bool fbp) {
+__device__ inline void dd_bp_split_block(float* __restrict__ img, float* __restrict__ part,
+                                         const float* __restrict__ sino, const DDView* __restrict__ views, int C,
+                                         int cs, int nchunk, int vpg, const Grid& g, const Det& d, int nview, float dso,
+                                         bool fbp, int bx, int by, int bz) {
     static_assert(CH % (4 * LPC) == 0 && LPC >= 2, "a lane holds whole float4s");
     constexpr int NQ = CH / (4 * LPC);
     constexpr int WX = LPC == 2 ? 8 : 4, WY = 64 / LPC / WX;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int p = lane / LPC, sub = lane % LPC;
-    const int ix = blockIdx.x * (2 * WX) + (wave & 1) * WX + p % WX;
-    const int iy = blockIdx.y * (2 * WY) + (wave >> 1) * WY + p / WX;
-    const int grp = blockIdx.z / nchunk;
-    const int c0 = (blockIdx.z - grp * nchunk) * CH;
+    const int ix = bx * (2 * WX) + (wave & 1) * WX + p % WX;
+    const int iy = by * (2 * WY) + (wave >> 1) * WY + p / WX;
+    const int grp = bz / nchunk;
+    const int c0 = (bz - grp * nchunk) * CH;
     const int vbeg = grp * vpg, vend = min(nview, vbeg + vpg);
     if (ix >= g.nx || iy >= g.ny) return;   // the lanes of a pixel leave together
     // pixel centre and edges, cuda/distanceDriven2DBoxInt.cu:340-349
@@ -1213,6 +1264,15 @@ __global__ __launch_bounds__(256) void dd_bp_split_kernel(float* __restrict__ im
             else img[(size_t)cc * npix + pix] += acc[4 * k + e];
         }
     }
+}
+
+template <int CH, int LPC, bool FAN>
+__global__ __launch_bounds__(256) void dd_bp_split_kernel(float* __restrict__ img, float* __restrict__ part,
+                                                          const float* __restrict__ sino,
+                                                          const DDView* __restrict__ views, int C, int cs, int nchunk,
+                                                          int vpg, Grid g, Det d, int nview, float dso, bool fbp) {
+    dd_bp_split_block<CH, LPC, FAN>(img, part, sino, views, C, cs, nchunk, vpg, g, d, nview, dso, fbp, blockIdx.x,
+                                    blockIdx.y, blockIdx.z);
 }
 
 // Number of view groups a pixel-driven backprojection is split into so that
@@ -1752,6 +1812,555 @@ void dd_plan_destroy(DDPlan* p) {
     CTP_CHECK(hipDeviceSynchronize());   // queued calls may still read the tables
     owned.reset();
     CTP_CHECK(hipSetDevice(cur));
+}
+
+
+// ---------------------------------------------------------------------------
+// batched plans of uneven jobs (no counterpart in the reference)
+// ---------------------------------------------------------------------------
+// Many fan / parallel projections of different geometry, size, view count and
+// batch count.  Each job keeps the launch shapes and tables a single call of
+// its own takes (so its results are those of the single call), and all jobs
+// whose kernel is the same template instance run in one launch: a block finds
+// its job by a binary search over the jobs' first block indices and runs the
+// single-call kernel's block body with that job's parameters.  Forward: one
+// launch making the channel-last copies of every job, then one per forward
+// kernel class.  Backward: one launch clearing every image, one making the
+// weighted channel-last projections of every job, one per backprojector
+// class, one summing the partial images.  Blocks take their index from a work
+// counter (persistent blocks) or from the hardware dispatcher.  The calls only
+// launch kernels (counters are cleared by a kernel), so they can be captured
+// in a HIP graph.
+
+struct DDJobDev {
+    float* prj;
+    float* img;
+    float* imgY;
+    float* imgX;
+    float* sino;
+    float* part;
+    const DDView* views;
+    const FPRow* rows;
+    const int* start;
+    const SliceRow* srow;
+    long rsY, rsX;
+    Grid g;
+    Det d;
+    int nview, fan;
+    float dsd, dso;
+    int fC, fcs, nrow, lseg, force, needY, needX;   // forward
+    int bC, bcs, nchunk, groups, vpg, fbp;          // backprojection
+    int tr_gx, tr_gy;      // channel-last copies: tr_gx x tr_gy x ny blocks per copy
+    int fp_gx, fp_gy;      // forward projection (third grid dimension: channel chunks)
+    int pr_gx, pr_gy;      // weighted projections (third: views)
+    int bp_gx, bp_gy;      // backprojection (third: chunks x view groups)
+};
+
+// Member k of a launch whose blocks begin[k] .. begin[k + 1] - 1 are block b's
+// (begin ascending from 0; members without blocks repeat the next begin).
+__device__ inline int dd_member(const int* __restrict__ begin, int n, int b) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (const_load(begin, mid) <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+#define DD_BATCH_ARGS                                                                                         \
+    const DDJobDev *__restrict__ jobs, const int *__restrict__ members, const int *__restrict__ begin, int n, \
+        int total, int *__restrict__ ctr, int persist
+#define DD_BATCH_LOOP                                                                                          \
+    __shared__ int slot;                                                                                       \
+    for (int b = first_block(ctr, 1, &slot, persist); b < total; b = following_block(ctr, 1, &slot, persist))
+#define DD_BATCH_JOB                                                                                           \
+    const int k = dd_member(begin, n, b);                                                                      \
+    const DDJobDev J = const_load(jobs, (long)const_load(members, k));                                         \
+    const int l = b - const_load(begin, k)
+
+__global__ __launch_bounds__(256) void dd_batch_copy_kernel(DD_BATCH_ARGS) {
+    DD_BATCH_LOOP {
+        DD_BATCH_JOB;
+        // the y-major copy (if needed), then the x-major copy (if needed)
+        const int per = J.tr_gx * J.tr_gy * J.g.ny;
+        const bool xmajor = !J.needY || l >= per;
+        const int m = (J.needY && xmajor) ? l - per : l;
+        const int bx = m % J.tr_gx, t = m / J.tr_gx;
+        to_channel_last_block(xmajor ? J.imgX : J.imgY, J.img, J.fC, J.fcs, J.g.nx, J.g.ny, xmajor,
+                              (size_t)(xmajor ? J.rsX : J.rsY), bx, t % J.tr_gy, t / J.tr_gy);
+    }
+}
+
+template <int CH, int LPC, bool FAN>
+__global__ __launch_bounds__(256) void dd_batch_fp_split_kernel(DD_BATCH_ARGS) {
+    DD_BATCH_LOOP {
+        DD_BATCH_JOB;
+        dd_fp_split_block<CH, LPC, FAN>(J.prj, J.imgY, J.imgX, J.views, J.rows, J.nrow, J.g, J.d, J.nview, J.dsd,
+                                        J.dso, J.force != 0, l % J.fp_gx, l / J.fp_gx);
+    }
+}
+
+template <int CH, bool FAN, bool XC, int VPB>
+__global__ __launch_bounds__(VPB > 1 ? 64 * VPB : 128) void dd_batch_fp_kernel(DD_BATCH_ARGS) {
+    DD_BATCH_LOOP {
+        DD_BATCH_JOB;
+        const int t = l / J.fp_gx;
+        dd_fp_block<CH, FAN, XC, VPB>(J.prj, J.imgY, J.imgX, J.views, J.rows, J.start, J.fcs, J.g, J.d, J.nview,
+                                      J.dsd, J.dso, J.force != 0, J.lseg, l % J.fp_gx, t % J.fp_gy, t / J.fp_gy);
+    }
+}
+
+// 1024 floats of one image per block
+__global__ __launch_bounds__(256) void dd_batch_zero_kernel(DD_BATCH_ARGS) {
+    DD_BATCH_LOOP {
+        DD_BATCH_JOB;
+        const size_t nimg = (size_t)J.bC * J.g.nx * J.g.ny;
+        const size_t i0 = (size_t)l * 1024 + threadIdx.x;
+#pragma unroll
+        for (int e = 0; e < 4; e++)
+            if (i0 + 256 * e < nimg) J.img[i0 + 256 * e] = 0.f;
+    }
+}
+
+__global__ __launch_bounds__(256) void dd_batch_prep_kernel(DD_BATCH_ARGS) {
+    DD_BATCH_LOOP {
+        DD_BATCH_JOB;
+        const int t = l / J.pr_gx;
+        dd_bp_prep_block(J.sino, J.prj, J.views, J.srow, J.bC, J.bcs, J.g.nz, J.d, J.nview, J.g.dx, J.g.dy,
+                         J.fan != 0, J.fbp == 0, l % J.pr_gx, t % J.pr_gy, t / J.pr_gy);
+    }
+}
+
+template <int CH, int LPC, bool FAN>
+__global__ __launch_bounds__(256) void dd_batch_bp_split_kernel(DD_BATCH_ARGS) {
+    DD_BATCH_LOOP {
+        DD_BATCH_JOB;
+        const int t = l / J.bp_gx;
+        dd_bp_split_block<CH, LPC, FAN>(J.img, J.part, J.sino, J.views, J.bC, J.bcs, J.nchunk, J.vpg, J.g, J.d,
+                                        J.nview, J.dso, J.fbp != 0, l % J.bp_gx, t % J.bp_gy, t / J.bp_gy);
+    }
+}
+
+template <int CH, bool FAN>
+__global__ __launch_bounds__(256) void dd_batch_bp_kernel(DD_BATCH_ARGS) {
+    DD_BATCH_LOOP {
+        DD_BATCH_JOB;
+        const int t = l / J.bp_gx;
+        dd_bp_block<CH, FAN>(J.img, J.part, J.sino, J.views, J.bC, J.bcs, J.nchunk, J.vpg, J.g, J.d, J.nview, J.dso,
+                             J.fbp != 0, l % J.bp_gx, t % J.bp_gy, t / J.bp_gy);
+    }
+}
+
+// the view groups' partial images summed in group order (as reduce_parts)
+__global__ __launch_bounds__(256) void dd_batch_reduce_kernel(DD_BATCH_ARGS) {
+    DD_BATCH_LOOP {
+        DD_BATCH_JOB;
+        const size_t nimg = (size_t)J.bC * J.g.nx * J.g.ny;
+        const size_t i = (size_t)l * 256 + threadIdx.x;
+        if (i < nimg) {
+            float s = 0.f;
+            for (int q = 0; q < J.groups; q++) s += J.part[(size_t)q * nimg + i];
+            J.img[i] += s;
+        }
+    }
+}
+
+#undef DD_BATCH_JOB
+#undef DD_BATCH_LOOP
+#undef DD_BATCH_ARGS
+
+__global__ void dd_batch_clear_kernel(int* __restrict__ ctr, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) ctr[i] = 0;
+}
+
+namespace {
+
+enum DDLaunchKind { DDL_COPY, DDL_FP_SPLIT, DDL_FP, DDL_ZERO, DDL_PREP, DDL_BP_SPLIT, DDL_BP, DDL_REDUCE };
+
+struct DDLaunch {
+    int kind = DDL_COPY;
+    int ch = 1, lpc = 1, vpb = 1;
+    bool fan = true, xc = false;
+    std::vector<int> members, begin;   // begin: first block of each member, then the total
+    int* dmembers = nullptr;
+    int* dbegin = nullptr;
+    int total = 0, block = 256, grid = 0;
+    int ctr = 0;                       // index of the launch's work counter
+    bool same_kernel(const DDLaunch& o) const {
+        return kind == o.kind && ch == o.ch && lpc == o.lpc && vpb == o.vpb && fan == o.fan && xc == o.xc;
+    }
+};
+
+}  // namespace
+
+struct DDBatchPlan::Impl {
+    int device = 0;
+    std::vector<DDBatchJobSpec> spec;
+    std::vector<std::vector<DDView>> views;
+    std::vector<DDFPSetup> fps;
+    std::vector<DDBPSetup> bps;
+    std::vector<DDJobDev> jobs;
+    DDJobDev* djobs = nullptr;
+    std::vector<DDLaunch> fwd, bwd;
+    int* ctr = nullptr;                // work counters: forward launches, then backward launches
+    int nctr = 0;
+    int fp_lpc = 0, bp_lpc = 0;        // 0: the single calls' choice (tuning keys)
+    int pf = 1, pb = 1;                // persistent blocks (1) or hardware dispatch (0), per direction
+    std::vector<void*> mem;            // job buffers and tables
+    std::vector<void*> lmem;           // launch tables of the current layout
+
+    template <typename T>
+    T* alloc(std::vector<void*>& owner, size_t n) {
+        void* p = nullptr;
+        CTP_CHECK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+        owner.push_back(p);
+        return static_cast<T*>(p);
+    }
+    ~Impl() {
+        for (void* p : lmem) (void)hipFree(p);
+        for (void* p : mem) (void)hipFree(p);
+    }
+    void setup_jobs();
+    void layout();
+    void run(const std::vector<DDLaunch>& ls, int ctr_base, int persist, hipStream_t s, const char* what);
+    float time_direction(bool forward_dir, hipStream_t s);
+};
+
+// per job: tables and scratch as a single call makes them (on the device for
+// the plan's lifetime); the channel-last copies are sized for the widest
+// pixel stride any lanes-per-column choice gives
+void DDBatchPlan::Impl::setup_jobs() {
+    const int nj = (int)spec.size();
+    views.resize(nj);
+    jobs.assign(nj, DDJobDev{});
+    for (int j = 0; j < nj; j++) {
+        const DDBatchJobSpec& S = spec[j];
+        views[j] = dd_views_for(S.deg, S.nview, S.g, S.type);
+        const DDFPSetup widest = dd_fp_setup(views[j], S.B, S.g, S.d, S.nview, S.dsd, S.dso, S.type, S.fan, false, 1);
+        const DDFPSetup fs = dd_fp_setup(views[j], S.B, S.g, S.d, S.nview, S.dsd, S.dso, S.type, S.fan, false, 0);
+        const DDBPSetup bs = dd_bp_setup(S.B, S.g, S.d, S.nview, S.type);
+        DDJobDev& J = jobs[j];
+        J.prj = S.prj;
+        J.img = S.img;
+        J.g = S.g;
+        J.d = S.d;
+        J.nview = S.nview;
+        J.fan = S.fan ? 1 : 0;
+        J.dsd = S.dsd;
+        J.dso = S.dso;
+        const size_t ny = S.g.ny, nx = S.g.nx, npix = nx * ny;
+        const long rs_max = std::max(widest.rsY, fs.rsY), rx_max = std::max(widest.rsX, fs.rsX);
+        J.imgY = fs.needY ? alloc<float>(mem, ny * rs_max) : nullptr;
+        J.imgX = fs.needX ? alloc<float>(mem, nx * rx_max) : nullptr;
+        J.sino = alloc<float>(mem, (size_t)S.nview * S.d.nu * bs.cs);
+        J.part = bs.groups > 1 ? alloc<float>(mem, (size_t)bs.groups * bs.C * npix) : nullptr;
+        DDView* dv = alloc<DDView>(mem, S.nview);
+        FPRow* dr = alloc<FPRow>(mem, fs.rows.size());
+        int* dst = alloc<int>(mem, fs.nchunk + 1);
+        SliceRow* dsr = alloc<SliceRow>(mem, S.g.nz);
+        CTP_CHECK(hipMemcpy(dv, views[j].data(), sizeof(DDView) * S.nview, hipMemcpyHostToDevice));
+        CTP_CHECK(hipMemcpy(dr, fs.rows.data(), sizeof(FPRow) * fs.rows.size(), hipMemcpyHostToDevice));
+        CTP_CHECK(hipMemcpy(dst, fs.start.data(), sizeof(int) * (fs.nchunk + 1), hipMemcpyHostToDevice));
+        CTP_CHECK(hipMemcpy(dsr, bs.srow.data(), sizeof(SliceRow) * S.g.nz, hipMemcpyHostToDevice));
+        J.views = dv;
+        J.rows = dr;
+        J.start = dst;
+        J.srow = dsr;
+        bps.push_back(bs);
+    }
+}
+
+// Kernel of a launch (nullptr: no such instance).
+static const void* dd_launch_kernel(const DDLaunch& L) {
+    switch (L.kind) {
+    case DDL_COPY: return reinterpret_cast<const void*>(dd_batch_copy_kernel);
+    case DDL_ZERO: return reinterpret_cast<const void*>(dd_batch_zero_kernel);
+    case DDL_PREP: return reinterpret_cast<const void*>(dd_batch_prep_kernel);
+    case DDL_REDUCE: return reinterpret_cast<const void*>(dd_batch_reduce_kernel);
+    case DDL_FP_SPLIT:
+#define CTP_K(N, P)                                                                                            \
+    if (L.ch == N && L.lpc == P)                                                                               \
+        return L.fan ? reinterpret_cast<const void*>(dd_batch_fp_split_kernel<N, P, true>)                     \
+                     : reinterpret_cast<const void*>(dd_batch_fp_split_kernel<N, P, false>);
+        CTP_K(8, 2) CTP_K(16, 2) CTP_K(16, 4) CTP_K(32, 2) CTP_K(32, 4) CTP_K(32, 8)
+#undef CTP_K
+        return nullptr;
+    case DDL_FP:
+#define CTP_K(N)                                                                                               \
+    if (L.ch == N) {                                                                                           \
+        if (L.vpb == 4) return L.fan ? (L.xc ? reinterpret_cast<const void*>(dd_batch_fp_kernel<N, true, true, 4>) \
+                                             : reinterpret_cast<const void*>(dd_batch_fp_kernel<N, true, false, 4>)) \
+                                     : (L.xc ? reinterpret_cast<const void*>(dd_batch_fp_kernel<N, false, true, 4>) \
+                                             : reinterpret_cast<const void*>(dd_batch_fp_kernel<N, false, false, 4>)); \
+        if (L.vpb == 2) return L.fan ? (L.xc ? reinterpret_cast<const void*>(dd_batch_fp_kernel<N, true, true, 2>) \
+                                             : reinterpret_cast<const void*>(dd_batch_fp_kernel<N, true, false, 2>)) \
+                                     : (L.xc ? reinterpret_cast<const void*>(dd_batch_fp_kernel<N, false, true, 2>) \
+                                             : reinterpret_cast<const void*>(dd_batch_fp_kernel<N, false, false, 2>)); \
+        return L.fan ? (L.xc ? reinterpret_cast<const void*>(dd_batch_fp_kernel<N, true, true, 1>)             \
+                             : reinterpret_cast<const void*>(dd_batch_fp_kernel<N, true, false, 1>))           \
+                     : (L.xc ? reinterpret_cast<const void*>(dd_batch_fp_kernel<N, false, true, 1>)            \
+                             : reinterpret_cast<const void*>(dd_batch_fp_kernel<N, false, false, 1>));         \
+    }
+        CTP_K(1) CTP_K(2) CTP_K(4) CTP_K(8) CTP_K(16) CTP_K(32)
+#undef CTP_K
+        return nullptr;
+    case DDL_BP_SPLIT:
+#define CTP_K(N, P)                                                                                            \
+    if (L.ch == N && L.lpc == P)                                                                               \
+        return L.fan ? reinterpret_cast<const void*>(dd_batch_bp_split_kernel<N, P, true>)                     \
+                     : reinterpret_cast<const void*>(dd_batch_bp_split_kernel<N, P, false>);
+        CTP_K(8, 2) CTP_K(16, 2) CTP_K(16, 4) CTP_K(32, 2) CTP_K(32, 4)
+#undef CTP_K
+        return nullptr;
+    case DDL_BP:
+#define CTP_K(N)                                                                                               \
+    if (L.ch == N)                                                                                             \
+        return L.fan ? reinterpret_cast<const void*>(dd_batch_bp_kernel<N, true>)                              \
+                     : reinterpret_cast<const void*>(dd_batch_bp_kernel<N, false>);
+        CTP_K(1) CTP_K(2) CTP_K(4) CTP_K(8) CTP_K(16) CTP_K(32)
+#undef CTP_K
+        return nullptr;
+    }
+    return nullptr;
+}
+
+// Launch groups and the job records of the current choice of lanes per column
+// (forward), lanes per pixel (backward) and block distribution.
+void DDBatchPlan::Impl::layout() {
+    for (void* p : lmem) (void)hipFree(p);
+    lmem.clear();
+    fwd.clear();
+    bwd.clear();
+    fps.clear();
+    const int nj = (int)spec.size();
+    const int tv = tuning_get("dd_fp_views_per_block", 4);
+    const long limit = 0x7fffffffL - (1L << 24);   // block indices (and counters running past them) in 31 bits
+    auto add = [&](std::vector<DDLaunch>& ls, DDLaunch L, int j, long blocks) {
+        if (blocks <= 0) return;
+        DDLaunch* at = nullptr;
+        for (auto& o : ls)
+            if (o.same_kernel(L)) at = &o;
+        if (!at) {
+            ls.push_back(L);
+            at = &ls.back();
+        }
+        if ((long)at->total + blocks > limit) fail("dd batch: more than 2^31 blocks in one launch");
+        at->members.push_back(j);
+        at->begin.push_back(at->total);
+        at->total += (int)blocks;
+    };
+    for (int j = 0; j < nj; j++) {
+        const DDBatchJobSpec& S = spec[j];
+        DDJobDev& J = jobs[j];
+        const DDFPSetup fs = dd_fp_setup(views[j], S.B, S.g, S.d, S.nview, S.dsd, S.dso, S.type, S.fan, false, fp_lpc);
+        fps.push_back(fs);
+        const DDBPSetup& bs = bps[j];
+        J.fC = fs.C; J.fcs = fs.cs; J.nrow = (int)fs.rows.size(); J.lseg = fs.lseg; J.force = fs.force ? 1 : 0;
+        J.needY = fs.needY ? 1 : 0; J.needX = fs.needX ? 1 : 0;
+        J.rsY = fs.rsY; J.rsX = fs.rsX;
+        J.bC = bs.C; J.bcs = bs.cs; J.nchunk = bs.nchunk; J.groups = bs.groups; J.vpg = bs.vpg;
+        J.fbp = bs.fbp ? 1 : 0;
+        // forward
+        J.tr_gx = ceil_div(S.g.nx, 32);
+        J.tr_gy = ceil_div(fs.cs, 32);
+        DDLaunch c;
+        c.kind = DDL_COPY;
+        add(fwd, c, j, (long)(J.needY + J.needX) * J.tr_gx * J.tr_gy * S.g.ny);
+        DDLaunch f;
+        f.fan = S.fan;
+        f.ch = fs.CH;
+        if (fs.lpc > 1) {
+            f.kind = DDL_FP_SPLIT;
+            f.lpc = fs.lpc;
+            J.fp_gx = ceil_div(S.d.nu, 64 / fs.lpc);
+            J.fp_gy = ceil_div(S.nview, 4);
+            add(fwd, f, j, (long)J.fp_gx * J.fp_gy);
+        } else {
+            f.kind = DDL_FP;
+            f.xc = fs.xc;
+            f.vpb = (fs.lseg == 0 && (tv == 2 || tv == 4)) ? tv : 1;
+            f.block = f.vpb > 1 ? 64 * f.vpb : 128;
+            J.fp_gx = f.vpb > 1 ? ceil_div(S.d.nu, 64) : ceil_div(S.d.nu, 128 >> fs.lseg);
+            J.fp_gy = f.vpb > 1 ? ceil_div(S.nview, f.vpb) : S.nview;
+            add(fwd, f, j, (long)J.fp_gx * J.fp_gy * fs.nchunk);
+        }
+        // backward
+        const long nimg = (long)bs.C * S.g.nx * S.g.ny;
+        DDLaunch z;
+        z.kind = DDL_ZERO;
+        add(bwd, z, j, (nimg + 1023) / 1024);
+        J.pr_gx = ceil_div(S.d.nu, 32);
+        J.pr_gy = ceil_div(bs.cs, 32);
+        DDLaunch pr;
+        pr.kind = DDL_PREP;
+        add(bwd, pr, j, (long)J.pr_gx * J.pr_gy * S.nview);
+        int lpc = bp_lpc > 0 ? bp_lpc : tuning_get("dd_bp_lanes_per_pixel", 4);
+        if (lpc == 4 && bs.CH % 16 != 0) lpc = 2;
+        DDLaunch bk;
+        bk.fan = S.fan;
+        bk.ch = bs.CH;
+        if ((lpc == 2 || lpc == 4) && bs.CH >= 8 && bs.CH % (4 * lpc) == 0) {
+            bk.kind = DDL_BP_SPLIT;
+            bk.lpc = lpc;
+            J.bp_gx = ceil_div(S.g.nx, lpc == 2 ? 16 : 8);
+            J.bp_gy = ceil_div(S.g.ny, 8);
+        } else {
+            bk.kind = DDL_BP;
+            J.bp_gx = ceil_div(S.g.nx, 16);
+            J.bp_gy = ceil_div(S.g.ny, 16);
+        }
+        add(bwd, bk, j, (long)J.bp_gx * J.bp_gy * bs.nchunk * bs.groups);
+        if (bs.groups > 1) {
+            DDLaunch r;
+            r.kind = DDL_REDUCE;
+            add(bwd, r, j, (nimg + 255) / 256);
+        }
+    }
+    // stream order: copies before projections; cleared images before the
+    // weighted projections, the backprojections and the partial-image sums
+    auto by_kind = [](const DDLaunch& x, const DDLaunch& y) { return x.kind < y.kind; };
+    std::stable_sort(fwd.begin(), fwd.end(), by_kind);
+    std::stable_sort(bwd.begin(), bwd.end(), by_kind);
+    int ci = 0;
+    for (auto* ls : {&fwd, &bwd})
+        for (auto& L : *ls) {
+            const int persist = ls == &fwd ? pf : pb;
+            L.begin.push_back(L.total);
+            L.dmembers = alloc<int>(lmem, L.members.size());
+            L.dbegin = alloc<int>(lmem, L.begin.size());
+            CTP_CHECK(hipMemcpy(L.dmembers, L.members.data(), sizeof(int) * L.members.size(), hipMemcpyHostToDevice));
+            CTP_CHECK(hipMemcpy(L.dbegin, L.begin.data(), sizeof(int) * L.begin.size(), hipMemcpyHostToDevice));
+            const void* k = dd_launch_kernel(L);
+            if (!k) fail("dd batch: no kernel for a job's launch shape");
+            if (persist) {
+                int dev = 0, nb = 0, ncu = 0;
+                CTP_CHECK(hipGetDevice(&dev));
+                CTP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+                CTP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, L.block, 0));
+                L.grid = std::max(1, std::min(L.total, std::max(nb, 1) * ncu));
+            } else {
+                L.grid = L.total;
+            }
+            L.ctr = ci++;
+        }
+    nctr = ci;
+    ctr = alloc<int>(lmem, std::max(nctr, 1));
+    djobs = alloc<DDJobDev>(lmem, jobs.size());
+    CTP_CHECK(hipMemcpy(djobs, jobs.data(), sizeof(DDJobDev) * jobs.size(), hipMemcpyHostToDevice));
+}
+
+void DDBatchPlan::Impl::run(const std::vector<DDLaunch>& ls, int ctr_base, int persist, hipStream_t s,
+                            const char* what) {
+    if (ls.empty()) return;
+    if (persist) {
+        dd_batch_clear_kernel<<<1, 64, 0, s>>>(ctr + ctr_base, (int)ls.size());
+        CTP_LAUNCH_CHECK();
+    }
+    TimedLaunch t(what, s);
+    for (const DDLaunch& L : ls) {
+        const void* k = dd_launch_kernel(L);
+        int* c = ctr + L.ctr;
+        int n = (int)L.members.size();
+        int total = L.total;
+        int pers = persist;
+        const DDJobDev* jb = djobs;
+        const int* mb = L.dmembers;
+        const int* bg = L.dbegin;
+        void* args[] = {(void*)&jb, (void*)&mb, (void*)&bg, (void*)&n, (void*)&total, (void*)&c, (void*)&pers};
+        CTP_CHECK(hipLaunchKernel(k, dim3(L.grid), dim3(L.block), args, 0, s));
+    }
+}
+
+float DDBatchPlan::Impl::time_direction(bool forward_dir, hipStream_t s) {
+    hipEvent_t a, b;
+    CTP_CHECK(hipEventCreate(&a));
+    CTP_CHECK(hipEventCreate(&b));
+    float best = 1e30f;
+    for (int rep = 0; rep < 4; rep++) {   // the first run warms up
+        CTP_CHECK(hipEventRecord(a, s));
+        if (forward_dir) run(fwd, 0, pf, s, "dd_batch_fp");
+        else run(bwd, (int)fwd.size(), pb, s, "dd_batch_bp");
+        CTP_CHECK(hipEventRecord(b, s));
+        CTP_CHECK(hipEventSynchronize(b));
+        float ms = 0.f;
+        CTP_CHECK(hipEventElapsedTime(&ms, a, b));
+        if (rep > 0) best = std::min(best, ms);
+    }
+    CTP_CHECK(hipEventDestroy(a));
+    CTP_CHECK(hipEventDestroy(b));
+    return best;
+}
+
+DDBatchPlan::DDBatchPlan(const std::vector<DDBatchJobSpec>& jobs, bool autotune) : impl_(new Impl()) {
+    Impl& I = *impl_;
+    CTP_CHECK(hipGetDevice(&I.device));
+    I.spec = jobs;
+    I.pf = I.pb = tuning_get("dd_batch_persistent", 1) == 1 ? 1 : 0;
+    I.setup_jobs();
+    I.layout();
+    if (autotune) tune();
+}
+
+DDBatchPlan::~DDBatchPlan() = default;
+
+// Forward: lanes per detector column (2, 4, 8) of the split-lane kernel;
+// backward: lanes per pixel (2, 4); each direction: persistent or
+// hardware-dispatched blocks.  Each candidate is timed on the bound buffers
+// (best of three runs after one warm-up); the results do not depend on the
+// choice.
+void DDBatchPlan::tune() {
+    Impl& I = *impl_;
+    TimingSuspended quiet;
+    hipStream_t s = nullptr;
+    CTP_CHECK(hipStreamCreate(&s));
+    float best_f = 1e30f, best_b = 1e30f;
+    int bf = 4, bb = 4, bpf = 1, bpb = 1;
+    for (int pers = 1; pers >= 0; pers--)
+        for (int lpc : {2, 4, 8}) {
+            I.pf = I.pb = pers;
+            I.fp_lpc = lpc;
+            I.bp_lpc = std::min(lpc, 4);
+            I.layout();
+            const float tf = I.time_direction(true, s);
+            if (tf < best_f) { best_f = tf; bf = lpc; bpf = pers; }
+            if (lpc <= 4) {
+                const float tb = I.time_direction(false, s);
+                if (tb < best_b) { best_b = tb; bb = lpc; bpb = pers; }
+            }
+        }
+    I.fp_lpc = bf;
+    I.bp_lpc = bb;
+    I.pf = bpf;
+    I.pb = bpb;
+    I.layout();
+    CTP_CHECK(hipStreamSynchronize(s));
+    CTP_CHECK(hipStreamDestroy(s));
+}
+
+void DDBatchPlan::forward(hipStream_t s) {
+    int dev = -1;
+    CTP_CHECK(hipGetDevice(&dev));
+    if (dev != impl_->device) fail("dd batch: called on another device than the one it was created on");
+    impl_->run(impl_->fwd, 0, impl_->pf, s, "dd_batch_fp");
+}
+
+void DDBatchPlan::backward(hipStream_t s) {
+    int dev = -1;
+    CTP_CHECK(hipGetDevice(&dev));
+    if (dev != impl_->device) fail("dd batch: called on another device than the one it was created on");
+    impl_->run(impl_->bwd, (int)impl_->fwd.size(), impl_->pb, s, "dd_batch_bp");
+}
+
+void DDBatchPlan::config(int& fp_lanes, int& bp_lanes, int& fp_persistent, int& bp_persistent) const {
+    fp_lanes = impl_->fp_lpc > 0 ? impl_->fp_lpc : tuning_get("dd_fp_lanes_per_column", 4);
+    bp_lanes = impl_->bp_lpc > 0 ? impl_->bp_lpc : tuning_get("dd_bp_lanes_per_pixel", 4);
+    fp_persistent = impl_->pf;
+    bp_persistent = impl_->pb;
+}
+
+int DDBatchPlan::launches(bool forward_dir) const {
+    return (int)(forward_dir ? impl_->fwd.size() : impl_->bwd.size());
 }
 
 }  // namespace ctp

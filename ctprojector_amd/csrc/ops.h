@@ -72,6 +72,34 @@ void dd_plan_backward(DDPlan* p, float* img, const float* prj, hipStream_t strea
 void dd_plan_adjoint(DDPlan* p, float* img, const float* prj, hipStream_t stream);
 void dd_plan_destroy(DDPlan* p);
 
+// ---- batched plans of uneven DD jobs (no counterpart in the reference) ----
+struct DDBatchJobSpec {
+    bool fan;                 // equiangular fan beam (else parallel beam)
+    int type;                 // typeProjector bits
+    int B;                    // batch items
+    Grid g;
+    Det d;
+    int nview;
+    float dsd, dso;
+    const float* deg;         // host [nview], read at creation
+    float* img;               // device [B][nz][ny][nx], bound
+    float* prj;               // device [B][nview][nv][nu], bound
+};
+
+class DDBatchPlan {
+public:
+    DDBatchPlan(const std::vector<DDBatchJobSpec>& jobs, bool autotune);
+    ~DDBatchPlan();
+    void forward(hipStream_t s);    // prj = A img for every job (overwrites)
+    void backward(hipStream_t s);   // img = the reference's backprojection of prj, every job (overwrites)
+    void tune();
+    void config(int& fp_lanes, int& bp_lanes, int& fp_persistent, int& bp_persistent) const;
+    int launches(bool forward_dir) const;   // kernel launches of one call (without the counter reset)
+    struct Impl;
+private:
+    std::unique_ptr<Impl> impl_;
+};
+
 // ---- batched cone-beam plans (no counterpart in the reference) ----
 struct ConeJobSpec {
     Grid g;

@@ -1774,32 +1774,6 @@ __global__ __launch_bounds__(256) void siddon_scatter_rays_kernel(float* __restr
     }
 }
 
-// Work distribution of the persistent kernels: a block takes its next block
-// index from a counter with an atomic add, so the blocks in flight stay a
-// compact window of the index space (as with a hardware-dispatched grid) even
-// when blocks take unequal times.  With 'lanes' = 8 the counters are per XCD
-// (a persistent block stays on the XCD of its first dispatch, blockIdx.x mod
-// 8) and hand out indices b = 8 * k + xcd, which keeps the XCD band mapping.
-// The index is read back through LDS and made wave-uniform (readfirstlane), so
-// that the job lookup and the job's parameters stay in scalar registers.
-// Block indices fit in 31 bits (checked on the host).
-__device__ inline int next_block(int* __restrict__ ctr, int lanes, int* slot) {
-    const int l = lanes == 8 ? (int)(blockIdx.x & 7) : 0;
-    __syncthreads();                                  // the previous index has been read by every thread
-    if (threadIdx.x == 0) *slot = lanes * atomicAdd(ctr + l, 1) + l;
-    __syncthreads();
-    return __builtin_amdgcn_readfirstlane(*slot);
-}
-
-// Block loop of the plan kernels: persistent (work counters above) or one
-// hardware-dispatched block per index (persist == 0).
-__device__ inline int first_block(int* __restrict__ ctr, int lanes, int* slot, int persist) {
-    return persist ? next_block(ctr, lanes, slot) : (int)blockIdx.x;
-}
-__device__ inline int following_block(int* __restrict__ ctr, int lanes, int* slot, int persist) {
-    return persist ? next_block(ctr, lanes, slot) : 0x7fffffff;
-}
-
 __global__ void reset_counters_kernel(int* __restrict__ ctr, int n) {
     if ((int)threadIdx.x < n) ctr[threadIdx.x] = 0;
 }

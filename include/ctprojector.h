@@ -260,6 +260,40 @@ int ctpConePlanBackward(void* plan, void* stream);
 int ctpConePlanConfig(void* plan, int* tile_u, int* xcd_order, int* zrun);
 int ctpConePlanDestroy(void* plan);
 
+/* No counterpart in the reference (whose DD entry points take one geometry and
+ * one size per call, cuda/distanceDriven2D.cu:109-634): a batch of fan-beam
+ * (fan = 1) or parallel-beam (fan = 0) distance-driven jobs of different
+ * geometry, size, view count and batch count, bound to their device arrays.
+ * Per job the arguments mean what they mean for
+ * cupyDistanceDrivenFanProjection (da: angular pitch for the fan beam, du for
+ * the parallel beam; dsd, dso unused for the parallel beam); deg is a host
+ * array read at creation.  Forward writes prj = A img for every job, Backward
+ * writes img = the reference's backprojection of prj (the job's
+ * typeProjector bits) for every job; each job's result is the single call's.
+ * The jobs whose kernels are the same template instance run in one launch
+ * (Config reports the launch counts).  Autotune times the launch shapes on
+ * the bound arrays at creation (prj and img are overwritten).  Calls are
+ * asynchronous on 'stream' and launch kernels only, so they can be captured
+ * in a HIP graph; the batch must be used on the device current at creation. */
+typedef struct ctpDDJob {
+    int fan;
+    int typeProjector;
+    size_t nBatches, nx, ny, nz;
+    float dx, dy, dz, cx, cy, cz;
+    size_t nu, nv, nview;
+    float da, dv, off_a, off_v;
+    float dsd, dso;
+    const float* deg;        /* host [nview] */
+    float* img;              /* device [nBatches][nz][ny][nx] */
+    float* prj;              /* device [nBatches][nview][nv][nu] */
+} ctpDDJob;
+int ctpDDBatchCreate(void** batch, const ctpDDJob* jobs, int njobs, int autotune);
+int ctpDDBatchForward(void* batch, void* stream);
+int ctpDDBatchBackward(void* batch, void* stream);
+int ctpDDBatchConfig(void* batch, int* fp_lanes, int* bp_lanes, int* fp_persistent, int* bp_persistent,
+                     int* fp_launches, int* bp_launches);
+int ctpDDBatchDestroy(void* batch);
+
 /* cuda/fbpParallel.cu:273  parallel-beam pixel-driven backprojection (host arrays):
  * img += sum over views of the bilinear (truncating) sample of prj at
  * u = -((x cos + y sin)/du + off_u) + (nu-1)/2; typeProjector is unused */

@@ -383,6 +383,105 @@ def cone_batch_extra(steps):
     }
 
 
+def dd_batch_extra(steps):
+    """The sixteen uneven distance-driven jobs of phantom.DD_BATCH_SAMPLES
+    (fan and parallel beams, 96^2 to 512^2, 1 to 8 slices, 128 to 984 views)
+    in one DD batch: autotuned, a few launches per direction (one per kernel
+    class), replayed from HIP graphs.  Compared with the same work as one
+    persistent DD plan per job called one job after another (the same kernels,
+    launched per job, no host work), and as the per-call device entry points
+    (host tables and a synchronisation per call, as the reference's calls)."""
+    import torch
+    from ctprojector_amd.batched import DDBatch, DDJob
+    from ctprojector_amd.ddplan import DDPlan
+    from ctprojector_amd.phantom import dd_batch_workload
+    from ctprojector_amd.projector.cupy import ct_projector, fan_equiangular as fan, parallel
+
+    work = dd_batch_workload()
+    projs = []
+    for _, w, _ in work:
+        p = ct_projector()
+        for k in ('nx', 'ny', 'nz', 'dx', 'dy', 'dz', 'cx', 'cy', 'cz', 'nu', 'nv', 'du', 'dv', 'off_u', 'off_v',
+                  'dsd', 'dso'):
+            setattr(p, k, getattr(w, k))
+        projs.append(p)
+    nvu = sum(int(np.prod(img.shape)) * w.nview for _, w, img in work)
+    log('[bench] dd batch: plan + autotune (%d jobs)' % len(work))
+    t0 = time.perf_counter()
+    bt = DDBatch([DDJob(p, w.angles, fan=f) for p, (f, w, _) in zip(projs, work)], autotune=True)
+    torch.cuda.synchronize()
+    plan_s = time.perf_counter() - t0
+    saved = [torch.from_numpy(img).cuda() for _, _, img in work]
+    for t, x in zip(bt.img, saved):
+        t.copy_(x)
+    gf = bt.capture('forward')
+    gb = bt.capture('backward')
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    fp_t = bp_t = 0.0
+    log('[bench] dd batch timed x%d' % steps)
+    for i in range(steps + 1):
+        for t, x in zip(bt.img, saved):
+            t.copy_(x)
+        ev[0].record()
+        gf.replay()
+        ev[1].record()
+        gb.replay()
+        ev[2].record()
+        torch.cuda.synchronize()
+        if i > 0:
+            fp_t += ev[0].elapsed_time(ev[1])
+            bp_t += ev[1].elapsed_time(ev[2])
+    ms = (fp_t + bp_t) / steps
+    finite = all(bool(torch.isfinite(t).all()) for t in bt.img)
+    cfg = bt.config()
+    bt.close()
+    del bt
+    log('[bench] dd batch: sequential plans')
+    plans = [DDPlan(p, w.angles, batch=1, fan=f) for p, (f, w, _) in zip(projs, work)]
+    prjs = [torch.empty(pl.prj_shape, device='cuda') for pl in plans]
+    outs = [torch.empty(pl.img_shape, device='cuda') for pl in plans]
+    seq_plans = 1e30
+    for _ in range(steps + 1):
+        torch.cuda.synchronize()
+        ev[0].record()
+        for pl, x, y, o in zip(plans, saved, prjs, outs):
+            pl.fp(x, out=y)
+            pl.bp(y, out=o)
+        ev[1].record()
+        torch.cuda.synchronize()
+        seq_plans = min(seq_plans, ev[0].elapsed_time(ev[1]))
+    for pl in plans:
+        pl.close()
+    log('[bench] dd batch: sequential per-call entry points')
+    dang = [torch.from_numpy(w.angles).cuda() for _, w, _ in work]
+    seq_calls = 1e30
+    for _ in range(2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for p, (f, _, _), x, a in zip(projs, work, saved, dang):
+            m = fan if f else parallel
+            y = m.distance_driven_fp(p, x, a)
+            _ = m.distance_driven_bp(p, y, a)
+        torch.cuda.synchronize()
+        seq_calls = min(seq_calls, (time.perf_counter() - t0) * 1e3)
+    return {
+        'workload': 'uneven DD batch: %d fan/parallel jobs (phantom.DD_BATCH_SAMPLES: 96^2 to 512^2, 1 to 8 slices, '
+                    '128 to 984 views, 137 to 888 columns), FP then BP of each; autotuned, one launch per kernel '
+                    'class and step, HIP-graph replay' % len(work),
+        'ms_per_step': round(ms, 3),
+        'fp_ms': round(fp_t / steps, 3),
+        'bp_ms': round(bp_t / steps, 3),
+        'value_gvu_s': round(2 * nvu / (ms / 1e3) / 1e9, 2),
+        'sequential_plans_ms_per_step': round(seq_plans, 3),
+        'speedup_vs_sequential_plans': round(seq_plans / ms, 3),
+        'sequential_calls_ms_per_step': round(seq_calls, 3),
+        'speedup_vs_sequential_calls': round(seq_calls / ms, 3),
+        'plan_and_autotune_s': round(plan_s, 3),
+        'autotuned': cfg,
+        'finite': finite,
+    }
+
+
 def serving_extra(count=200, seed=1005, sub_count=32):
     """Served reconstruction stream: Poisson arrivals (200/s) of half fan-beam
     FBP (256^2-512^2 x 1-64 slices, 360-984 views) and half cone-beam SQS-TV
@@ -644,7 +743,7 @@ def main(argv=None):
     ap.add_argument('--no-extra', action='store_true')
     ap.add_argument('--no-serving', action='store_true', help='skip the served-stream extra')
     ap.add_argument('--no-parity', action='store_true', help='skip the FP64 oracle check of the outputs')
-    ap.add_argument('--extras', default='cone,cone_batch,fbp,unrolled_sqs_tv,tv_prior,serving',
+    ap.add_argument('--extras', default='cone,cone_batch,dd_batch,fbp,unrolled_sqs_tv,tv_prior,serving',
                     help='comma-separated extras to run (default: all)')
     ap.add_argument('--serving-requests', type=int, default=200)
     ap.add_argument('--serving-below-saturation', type=int, default=32,
@@ -707,6 +806,7 @@ def main(argv=None):
         runs = [('cone', lambda: cone_extra(2, cpu=not args.no_cpu_baseline,
                                             outputs=None if args.no_parity else outputs['cone'])),
                 ('cone_batch', lambda: cone_batch_extra(3)),
+                ('dd_batch', lambda: dd_batch_extra(5)),
                 ('fbp', lambda: fbp_extra(3, w, r['img'], outputs=None if args.no_parity else outputs['fan'])),
                 ('unrolled_sqs_tv', lambda: unrolled_sqs_extra(3)),
                 ('tv_prior', lambda: tv_prior_extra(5))]

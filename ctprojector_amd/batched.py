@@ -226,12 +226,12 @@ class DDBatch:
 
     def config(self) -> dict:
         """Launch shapes in use (after autotuning) and kernel launches per call."""
-        v = [c_int() for _ in range(6)]
+        v = [c_int() for _ in range(8)]
         err = _lib.module().ctpDDBatchConfig(self._plan, *[ctypes.byref(x) for x in v])
         if err != 0:
             raise RuntimeError('ctpDDBatchConfig failed (%d): %s' % (err, _lib.last_error_message()))
         keys = ('dd_fp_lanes_per_column', 'dd_bp_lanes_per_pixel', 'fp_persistent', 'bp_persistent',
-                'fp_launches', 'bp_launches')
+                'fp_streams', 'bp_streams', 'fp_launches', 'bp_launches')
         return {k: x.value for k, x in zip(keys, v)}
 
     def capture(self, which: str = 'forward') -> torch.cuda.CUDAGraph:

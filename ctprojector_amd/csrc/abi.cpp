@@ -903,16 +903,18 @@ CTP_EXPORT int ctpDDBatchBackward(void* batch, void* stream) {
 }
 
 CTP_EXPORT int ctpDDBatchConfig(void* batch, int* fp_lanes, int* bp_lanes, int* fp_persistent, int* bp_persistent,
-                                int* fp_launches, int* bp_launches) {
+                                int* fp_streams, int* bp_streams, int* fp_launches, int* bp_launches) {
     return guarded("ctpDDBatchConfig", [&] {
         need_ptr(batch, "batch");
         const DDBatchPlan* b = static_cast<const DDBatchPlan*>(batch);
-        int a = 0, c = 0, e = 0, f = 0;
-        b->config(a, c, e, f);
+        int a = 0, c = 0, e = 0, f = 0, g = 0, h = 0;
+        b->config(a, c, e, f, g, h);
         if (fp_lanes) *fp_lanes = a;
         if (bp_lanes) *bp_lanes = c;
         if (fp_persistent) *fp_persistent = e;
         if (bp_persistent) *bp_persistent = f;
+        if (fp_streams) *fp_streams = g;
+        if (bp_streams) *bp_streams = h;
         if (fp_launches) *fp_launches = b->launches(true);
         if (bp_launches) *bp_launches = b->launches(false);
     });

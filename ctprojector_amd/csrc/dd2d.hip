@@ -1974,6 +1974,8 @@ __global__ void dd_batch_clear_kernel(int* __restrict__ ctr, int n) {
     for (int i = threadIdx.x; i < n; i += blockDim.x) ctr[i] = 0;
 }
 
+constexpr int kDDBatchMaxFanout = 4;
+
 namespace {
 
 enum DDLaunchKind { DDL_COPY, DDL_FP_SPLIT, DDL_FP, DDL_ZERO, DDL_PREP, DDL_BP_SPLIT, DDL_BP, DDL_REDUCE };
@@ -2009,6 +2011,9 @@ struct DDBatchPlan::Impl {
     int pf = 1, pb = 1;                // persistent blocks (1) or hardware dispatch (0), per direction
     std::vector<void*> mem;            // job buffers and tables
     std::vector<void*> lmem;           // launch tables of the current layout
+    int fanout_f = 1, fanout_b = 1;    // streams the kernel classes of each direction run on
+    std::vector<hipStream_t> side;     // side streams (created with the plan: none may be created under capture)
+    std::vector<hipEvent_t> events;    // fork, then one join per side stream
 
     template <typename T>
     T* alloc(std::vector<void*>& owner, size_t n) {
@@ -2018,12 +2023,17 @@ struct DDBatchPlan::Impl {
         return static_cast<T*>(p);
     }
     ~Impl() {
+        for (auto st : side) (void)hipStreamSynchronize(st);
         for (void* p : lmem) (void)hipFree(p);
         for (void* p : mem) (void)hipFree(p);
+        for (auto e : events) (void)hipEventDestroy(e);
+        for (auto st : side) (void)hipStreamDestroy(st);
     }
     void setup_jobs();
+    void launch(const DDLaunch& L, int persist, hipStream_t s);
     void layout();
-    void run(const std::vector<DDLaunch>& ls, int ctr_base, int persist, hipStream_t s, const char* what);
+    void run(const std::vector<DDLaunch>& ls, int ctr_base, int persist, int fanout, hipStream_t s,
+             const char* what);
     float time_direction(bool forward_dir, hipStream_t s);
 };
 
@@ -2250,26 +2260,62 @@ void DDBatchPlan::Impl::layout() {
     CTP_CHECK(hipMemcpy(djobs, jobs.data(), sizeof(DDJobDev) * jobs.size(), hipMemcpyHostToDevice));
 }
 
-void DDBatchPlan::Impl::run(const std::vector<DDLaunch>& ls, int ctr_base, int persist, hipStream_t s,
-                            const char* what) {
+void DDBatchPlan::Impl::launch(const DDLaunch& L, int persist, hipStream_t s) {
+    const void* k = dd_launch_kernel(L);
+    int* c = ctr + L.ctr;
+    int n = (int)L.members.size();
+    int total = L.total;
+    int pers = persist;
+    const DDJobDev* jb = djobs;
+    const int* mb = L.dmembers;
+    const int* bg = L.dbegin;
+    void* args[] = {(void*)&jb, (void*)&mb, (void*)&bg, (void*)&n, (void*)&total, (void*)&c, (void*)&pers};
+    CTP_CHECK(hipLaunchKernel(k, dim3(L.grid), dim3(L.block), args, 0, s));
+}
+
+// The projection / backprojection classes of one direction touch disjoint jobs,
+// so with fanout > 1 they run on side streams forked from 's' after the copies
+// (weighted projections) and joined before the partial-image sums; the fork
+// and join are events, which a graph capture turns into graph edges.
+void DDBatchPlan::Impl::run(const std::vector<DDLaunch>& ls, int ctr_base, int persist, int fanout,
+                            hipStream_t s, const char* what) {
     if (ls.empty()) return;
     if (persist) {
         dd_batch_clear_kernel<<<1, 64, 0, s>>>(ctr + ctr_base, (int)ls.size());
         CTP_LAUNCH_CHECK();
     }
     TimedLaunch t(what, s);
+    std::vector<const DDLaunch*> mid;
+    for (const DDLaunch& L : ls)
+        if (L.kind == DDL_FP_SPLIT || L.kind == DDL_FP || L.kind == DDL_BP_SPLIT || L.kind == DDL_BP) mid.push_back(&L);
+    const int nside = std::min<int>(std::min<int>(fanout, (int)side.size()), (int)mid.size());
+    bool forked = false;
     for (const DDLaunch& L : ls) {
-        const void* k = dd_launch_kernel(L);
-        int* c = ctr + L.ctr;
-        int n = (int)L.members.size();
-        int total = L.total;
-        int pers = persist;
-        const DDJobDev* jb = djobs;
-        const int* mb = L.dmembers;
-        const int* bg = L.dbegin;
-        void* args[] = {(void*)&jb, (void*)&mb, (void*)&bg, (void*)&n, (void*)&total, (void*)&c, (void*)&pers};
-        CTP_CHECK(hipLaunchKernel(k, dim3(L.grid), dim3(L.block), args, 0, s));
+        const bool is_mid = std::find(mid.begin(), mid.end(), &L) != mid.end();
+        if (!is_mid || nside <= 1) {
+            if (forked) {   // join before the first launch after the classes
+                for (int i = 0; i < nside; i++) {
+                    CTP_CHECK(hipEventRecord(events[1 + i], side[i]));
+                    CTP_CHECK(hipStreamWaitEvent(s, events[1 + i], 0));
+                }
+                forked = false;
+            }
+            launch(L, persist, s);
+            continue;
+        }
+        if (!forked) {
+            CTP_CHECK(hipEventRecord(events[0], s));
+            for (int i = 0; i < nside; i++) CTP_CHECK(hipStreamWaitEvent(side[i], events[0], 0));
+            forked = true;
+        }
+        const int i = (int)(std::find(mid.begin(), mid.end(), &L) - mid.begin()) % nside;
+        launch(L, persist, side[i]);
     }
+    if (forked)
+        for (int i = 0; i < nside; i++) {
+            CTP_CHECK(hipEventRecord(events[1 + i], side[i]));
+            CTP_CHECK(hipStreamWaitEvent(s, events[1 + i], 0));
+        }
 }
 
 float DDBatchPlan::Impl::time_direction(bool forward_dir, hipStream_t s) {
@@ -2279,8 +2325,8 @@ float DDBatchPlan::Impl::time_direction(bool forward_dir, hipStream_t s) {
     float best = 1e30f;
     for (int rep = 0; rep < 4; rep++) {   // the first run warms up
         CTP_CHECK(hipEventRecord(a, s));
-        if (forward_dir) run(fwd, 0, pf, s, "dd_batch_fp");
-        else run(bwd, (int)fwd.size(), pb, s, "dd_batch_bp");
+        if (forward_dir) run(fwd, 0, pf, fanout_f, s, "dd_batch_fp");
+        else run(bwd, (int)fwd.size(), pb, fanout_b, s, "dd_batch_bp");
         CTP_CHECK(hipEventRecord(b, s));
         CTP_CHECK(hipEventSynchronize(b));
         float ms = 0.f;
@@ -2297,6 +2343,11 @@ DDBatchPlan::DDBatchPlan(const std::vector<DDBatchJobSpec>& jobs, bool autotune)
     CTP_CHECK(hipGetDevice(&I.device));
     I.spec = jobs;
     I.pf = I.pb = tuning_get("dd_batch_persistent", 1) == 1 ? 1 : 0;
+    I.fanout_f = I.fanout_b = std::max(1, std::min(tuning_get("dd_batch_streams", 1), kDDBatchMaxFanout));
+    I.side.resize(kDDBatchMaxFanout);
+    for (auto& st : I.side) CTP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    I.events.resize(kDDBatchMaxFanout + 1);
+    for (auto& e : I.events) CTP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     I.setup_jobs();
     I.layout();
     if (autotune) tune();
@@ -2315,24 +2366,29 @@ void DDBatchPlan::tune() {
     hipStream_t s = nullptr;
     CTP_CHECK(hipStreamCreate(&s));
     float best_f = 1e30f, best_b = 1e30f;
-    int bf = 4, bb = 4, bpf = 1, bpb = 1;
+    int bf = 4, bb = 4, bpf = 1, bpb = 1, bof = 1, bob = 1;
     for (int pers = 1; pers >= 0; pers--)
         for (int lpc : {2, 4, 8}) {
             I.pf = I.pb = pers;
             I.fp_lpc = lpc;
             I.bp_lpc = std::min(lpc, 4);
             I.layout();
-            const float tf = I.time_direction(true, s);
-            if (tf < best_f) { best_f = tf; bf = lpc; bpf = pers; }
-            if (lpc <= 4) {
-                const float tb = I.time_direction(false, s);
-                if (tb < best_b) { best_b = tb; bb = lpc; bpb = pers; }
+            for (int fo : {1, kDDBatchMaxFanout}) {
+                I.fanout_f = I.fanout_b = fo;
+                const float tf = I.time_direction(true, s);
+                if (tf < best_f) { best_f = tf; bf = lpc; bpf = pers; bof = fo; }
+                if (lpc <= 4) {
+                    const float tb = I.time_direction(false, s);
+                    if (tb < best_b) { best_b = tb; bb = lpc; bpb = pers; bob = fo; }
+                }
             }
         }
     I.fp_lpc = bf;
     I.bp_lpc = bb;
     I.pf = bpf;
     I.pb = bpb;
+    I.fanout_f = bof;
+    I.fanout_b = bob;
     I.layout();
     CTP_CHECK(hipStreamSynchronize(s));
     CTP_CHECK(hipStreamDestroy(s));
@@ -2342,21 +2398,24 @@ void DDBatchPlan::forward(hipStream_t s) {
     int dev = -1;
     CTP_CHECK(hipGetDevice(&dev));
     if (dev != impl_->device) fail("dd batch: called on another device than the one it was created on");
-    impl_->run(impl_->fwd, 0, impl_->pf, s, "dd_batch_fp");
+    impl_->run(impl_->fwd, 0, impl_->pf, impl_->fanout_f, s, "dd_batch_fp");
 }
 
 void DDBatchPlan::backward(hipStream_t s) {
     int dev = -1;
     CTP_CHECK(hipGetDevice(&dev));
     if (dev != impl_->device) fail("dd batch: called on another device than the one it was created on");
-    impl_->run(impl_->bwd, (int)impl_->fwd.size(), impl_->pb, s, "dd_batch_bp");
+    impl_->run(impl_->bwd, (int)impl_->fwd.size(), impl_->pb, impl_->fanout_b, s, "dd_batch_bp");
 }
 
-void DDBatchPlan::config(int& fp_lanes, int& bp_lanes, int& fp_persistent, int& bp_persistent) const {
+void DDBatchPlan::config(int& fp_lanes, int& bp_lanes, int& fp_persistent, int& bp_persistent, int& fp_streams,
+                         int& bp_streams) const {
     fp_lanes = impl_->fp_lpc > 0 ? impl_->fp_lpc : tuning_get("dd_fp_lanes_per_column", 4);
     bp_lanes = impl_->bp_lpc > 0 ? impl_->bp_lpc : tuning_get("dd_bp_lanes_per_pixel", 4);
     fp_persistent = impl_->pf;
     bp_persistent = impl_->pb;
+    fp_streams = impl_->fanout_f;
+    bp_streams = impl_->fanout_b;
 }
 
 int DDBatchPlan::launches(bool forward_dir) const {

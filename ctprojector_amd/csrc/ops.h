@@ -93,7 +93,8 @@ public:
     void forward(hipStream_t s);    // prj = A img for every job (overwrites)
     void backward(hipStream_t s);   // img = the reference's backprojection of prj, every job (overwrites)
     void tune();
-    void config(int& fp_lanes, int& bp_lanes, int& fp_persistent, int& bp_persistent) const;
+    void config(int& fp_lanes, int& bp_lanes, int& fp_persistent, int& bp_persistent, int& fp_streams,
+                int& bp_streams) const;
     int launches(bool forward_dir) const;   // kernel launches of one call (without the counter reset)
     struct Impl;
 private:

@@ -225,3 +225,42 @@ def cone_workload(sample: int = 2, nview: int = 720) -> Tuple[Workload, np.ndarr
     geo = circular_cone_geometry(nview, 595.0, 1085.6, -20.0, 20.0, jitter=2.0, seed=1030 + sample)
     img = ellipsoid_phantom(nz, ny, nx, dz, dy, dx, seed=1130 + sample)[None]
     return w, img, geo
+
+
+DD_BATCH_SAMPLES = [  # (beam, n, nz, nview, nu, pixel mm): uneven DD jobs of a mixed stream
+    ('fan', 512, 1, 984, 888, 0.7),
+    ('fan', 256, 1, 492, 444, 1.4),
+    ('fan', 384, 4, 720, 666, 0.9),
+    ('fan', 320, 2, 600, 555, 1.1),
+    ('fan', 256, 8, 492, 444, 1.4),
+    ('fan', 448, 1, 820, 777, 0.8),
+    ('fan', 192, 1, 360, 333, 1.8),
+    ('fan', 512, 3, 984, 888, 0.7),
+    ('parallel', 256, 1, 360, 367, 1.0),
+    ('parallel', 128, 1, 180, 183, 2.0),
+    ('parallel', 320, 2, 400, 453, 0.8),
+    ('parallel', 200, 1, 300, 283, 1.25),
+    ('parallel', 384, 1, 512, 543, 0.67),
+    ('parallel', 160, 4, 240, 227, 1.6),
+    ('parallel', 96, 1, 128, 137, 2.6),
+    ('parallel', 256, 2, 360, 367, 1.0),
+]
+
+
+def dd_batch_workload() -> List[Tuple[bool, Workload, np.ndarray]]:
+    """The sixteen uneven distance-driven jobs of DD_BATCH_SAMPLES: (fan beam?,
+    geometry, image [1, nz, n, n]).  Fan beams: dso 595, dsd 1085.6, the
+    fan's angular span of the 888-column detector at 1.2858 mm (du = 1.2858 x
+    888 / nu), views over 360 deg; parallel beams: du = 1.4336 x n x pixel / nu,
+    views over 180 deg; slices of 1 mm; ellipsoid phantoms seeded 1200 + j."""
+    out = []
+    for j, (beam, n, nz, nview, nu, pix) in enumerate(DD_BATCH_SAMPLES):
+        fan = beam == 'fan'
+        du = 1.2858 * 888 / nu if fan else 1.4336 * n * pix / nu
+        w = Workload(n, n, nz, pix, pix, 1.0, nu, nz, du, 1.0, nview, dsd=1085.6, dso=595.0)
+        span = 2 * np.pi if fan else np.pi
+        w.angles = (np.arange(nview) * span / nview).astype(np.float32)
+        img = ellipsoid_phantom(nz, n, n, 1.0, pix, pix, seed=1200 + j)[None]
+        out.append((fan, w, img))
+    return out
+

@@ -270,8 +270,10 @@ int ctpConePlanDestroy(void* plan);
  * array read at creation.  Forward writes prj = A img for every job, Backward
  * writes img = the reference's backprojection of prj (the job's
  * typeProjector bits) for every job; each job's result is the single call's.
- * The jobs whose kernels are the same template instance run in one launch
- * (Config reports the launch counts).  Autotune times the launch shapes on
+ * The jobs whose kernels are the same template instance run in one launch,
+ * and the launches of different kernel classes may run concurrently on up to
+ * four internal streams forked from and joined to 'stream' (Config reports
+ * the launch shapes, stream counts and launch counts).  Autotune times the launch shapes on
  * the bound arrays at creation (prj and img are overwritten).  Calls are
  * asynchronous on 'stream' and launch kernels only, so they can be captured
  * in a HIP graph; the batch must be used on the device current at creation. */
@@ -291,7 +293,7 @@ int ctpDDBatchCreate(void** batch, const ctpDDJob* jobs, int njobs, int autotune
 int ctpDDBatchForward(void* batch, void* stream);
 int ctpDDBatchBackward(void* batch, void* stream);
 int ctpDDBatchConfig(void* batch, int* fp_lanes, int* bp_lanes, int* fp_persistent, int* bp_persistent,
-                     int* fp_launches, int* bp_launches);
+                     int* fp_streams, int* bp_streams, int* fp_launches, int* bp_launches);
 int ctpDDBatchDestroy(void* batch);
 
 /* cuda/fbpParallel.cu:273  parallel-beam pixel-driven backprojection (host arrays):

@@ -8,10 +8,12 @@ ctprojector_amd.batched.DDBatch).
   channel chunks, one-lane and split-lane backprojectors with and without
   view groups: each job's forward and backward equal its single call through
   the C ABI, and two jobs are checked against the FP64 oracle;
-* the autotuner and both block distributions (persistent work counters,
-  hardware dispatch) give the same results;
-* HIP-graph capture: forward and backward replayed three times on new inputs,
-  with eager work in between, each replay equal to the eager run.
+* the autotuner, both block distributions (persistent work counters,
+  hardware dispatch) and the kernel classes on four concurrent streams give
+  the same results;
+* HIP-graph capture (one stream, and four streams forked and joined inside
+  the capture): forward and backward replayed three times on new inputs, with
+  eager work in between, each replay equal to the eager run.
 
 Every case prints what names it before its GPU work."""
 
@@ -173,13 +175,31 @@ def test_autotune_and_dispatch_forms(torch):
         _lib.set_tuning('dd_batch_persistent', 0)
     for a, b in zip(ref[0] + ref[2], got[0] + got[2]):
         assert rel_l2(b, a) < 1e-6
+    try:
+        _lib.set_tuning('dd_batch_streams', 4)
+        _say('kernel classes on four streams')
+        bt = _batch(jobs)
+        assert bt.config()['fp_streams'] == 4
+        got = _run(torch, bt, jobs)
+        bt.close()
+    finally:
+        _lib.set_tuning('dd_batch_streams', 0)
+    for a, b in zip(ref[0] + ref[2], got[0] + got[2]):
+        assert rel_l2(b, a) < 1e-6
 
 
-def test_graph_replays(torch):
+@pytest.mark.parametrize('streams', [1, 4])
+def test_graph_replays(torch, streams):
+    from ctprojector_amd import _lib
     jobs = _jobs()[:5]
-    bt = _batch(jobs)
+    _lib.set_tuning('dd_batch_streams', streams)
+    try:
+        bt = _batch(jobs)
+    finally:
+        _lib.set_tuning('dd_batch_streams', 0)
+    assert bt.config()['bp_streams'] == streams
     rng = np.random.default_rng(4)
-    _say('capture forward and backward graphs')
+    _say('capture forward and backward graphs (%d streams)' % streams)
     gf = bt.capture('forward')
     gb = bt.capture('backward')
     for rep in range(3):

@@ -289,8 +289,8 @@ __global__ __launch_bounds__(256) void ramp_fft_kernel(float* __restrict__ out, 
 // ---------------------------------------------------------------------------
 // Three-pass form for N = 1024, 2048, 4096 (nu from 257 to 2048): radix 16,
 // radix 16, radix N/256.  T = N/16 lanes per complex row (one radix-16
-// butterfly each; the last pass gives each lane 256/T butterflies), 256/T
-// complex rows (two detector rows each) per block.  The first pass reads the
+// butterfly each; the last pass gives each lane 256/T butterflies),
+// max(T, 128)/T complex rows (two detector rows each) per block.  The first pass reads the
 // rows from global memory and the last pass of the inverse writes them back,
 // so a row pair crosses LDS five times (the multiplication by conj and G
 // happens in registers between the transforms).  LDS rows are padded by one
@@ -344,19 +344,27 @@ __device__ inline void dft_any(float2* v) {
     else dft_small<R>(v);
 }
 
+// Blocks of max(T, 128) lanes: one complex row per block from N = 2048 on
+// (17 KiB of LDS, barriers over two wavefronts), and registers held to five
+// wavefronts per SIMD where the LDS allows them.
 template <int N>
-__global__ __launch_bounds__(256) void ramp_fft16_kernel(float* __restrict__ out, const float* __restrict__ in,
-                                                         const float* __restrict__ G, const float* __restrict__ pw,
-                                                         long nrows, int nu) {
+constexpr int fft16_block() { return N / 16 < 128 ? 128 : N / 16; }
+
+template <int N>
+__global__ __launch_bounds__(fft16_block<N>()) __attribute__((amdgpu_waves_per_eu(N <= 2048 ? 5 : 1)))
+void ramp_fft16_kernel(float* __restrict__ out, const float* __restrict__ in, const float* __restrict__ G,
+                       const float* __restrict__ pw, long nrows, int nu) {
     constexpr int T = N / 16;            // lanes per complex row
-    constexpr int RPB = 256 / T;         // complex rows per block
+    constexpr int RPB = fft16_block<N>() / T;   // complex rows per block
     constexpr int R3 = N / 256;          // radix of the third pass
     constexpr int B3 = 256 / T;          // third-pass butterflies per lane
     constexpr int NP = N + N / 16;       // padded LDS row
     static_assert(N == 1024 || N == 2048 || N == 4096, "three-pass plan");
     __shared__ float2 lds[RPB * NP];
     const float invN = 1.f / (float)N;
-    const int t = threadIdx.x % T, c = threadIdx.x / T;
+    // T >= 64: the complex row of a wavefront is uniform, so its rows are
+    // addressed from scalar registers
+    const int t = threadIdx.x % T, c = __builtin_amdgcn_readfirstlane(threadIdx.x / T);
     float2* buf = lds + c * NP;
     const long ra = 2L * ((long)blockIdx.x * RPB + c), rb = ra + 1;
     const bool ha = ra < nrows, hb = rb < nrows;
@@ -528,11 +536,13 @@ void ramp_filter(float* out, const float* prj, int B, int nu, int nv, int nview,
             TimedLaunch tl("ramp_filter", stream);
             const long pairs = ceil_div(nrows, 2);
             if (mode == 0 && t.N >= 1024) {
-                const int rpb = 256 / (t.N / 16);
-                const unsigned grid = (unsigned)ceil_div(pairs, (long)rpb);
-                if (t.N == 1024) ramp_fft16_kernel<1024><<<grid, 256, 0, stream>>>(out, prj, dG, dpw, nrows, nu);
-                else if (t.N == 2048) ramp_fft16_kernel<2048><<<grid, 256, 0, stream>>>(out, prj, dG, dpw, nrows, nu);
-                else ramp_fft16_kernel<4096><<<grid, 256, 0, stream>>>(out, prj, dG, dpw, nrows, nu);
+                auto launch = [&](auto kernel, int block) {
+                    const int rpb = block / (t.N / 16);
+                    kernel<<<(unsigned)ceil_div(pairs, (long)rpb), block, 0, stream>>>(out, prj, dG, dpw, nrows, nu);
+                };
+                if (t.N == 1024) launch(ramp_fft16_kernel<1024>, fft16_block<1024>());
+                else if (t.N == 2048) launch(ramp_fft16_kernel<2048>, fft16_block<2048>());
+                else launch(ramp_fft16_kernel<4096>, fft16_block<4096>());
             } else {
                 ramp_fft_kernel<<<(unsigned)pairs, 256, sizeof(float2) * t.N, stream>>>(out, prj, dG, dtw, dpw, nrows,
                                                                                       nu, t.N, t.logN);

@@ -1230,8 +1230,9 @@ __device__ inline void dd_bp_split_block(float* __restrict__ img, float* __restr
             for (int t = 0; t < LPC; t++) {
                 const int view = v0 + t;
                 if (view >= vend) break;   // uniform
-                const float u1 = __shfl(mu1, lead + t), u2 = __shfl(mu2, lead + t);
-                const float scale = fbp ? __shfl(msc, lead + t) : 1.f;
+                // broadcast from lane t of the pixel's lanes (DPP within the quad)
+                const float u1 = group_bcast<LPC>(mu1, t, lead), u2 = group_bcast<LPC>(mu2, t, lead);
+                const float scale = fbp ? group_bcast<LPC>(msc, t, lead) : 1.f;
                 dd_line_split<NQ, LPC>(acc, sino + view * vstride + c0, 4 * sub, cs, u1, u2, d.nu, scale);
             }
         }

@@ -782,8 +782,8 @@ __global__ __launch_bounds__(256) void fan_fbp_bp_split_kernel(float* __restrict
         for (int t = 0; t < LPC; t++) {
             const int view = v0 + t;
             if (view >= vend) break;   // uniform
-            const float pu = __shfl(mpu, lead + t);
-            const float dist = __shfl(mdist, lead + t);
+            const float pu = group_bcast<LPC>(mpu, t, lead);     // DPP within the pixel's lanes
+            const float dist = group_bcast<LPC>(mdist, t, lead);
             // InterpolateXY with truncation, cuda/cudaMath.cu:227-260
             if (pu < 0.f || pu >= (float)d.nu) continue;
             const int u0 = (int)pu;

@@ -1153,6 +1153,7 @@ __device__ inline void bp_view_col(float* acc, const ConeViewBP& cv, const float
     const float bv = fmaf(cv.pv[0], xc, fmaf(cv.pv[1], yc, cv.pv[3]));
     // window origins clamped as floats (one median) before the conversion
     const int u0 = (int)__builtin_amdgcn_fmed3f(ceilf(uc - cv.hu), 0.f, (float)(nu - wu));
+    const int u0w = 4 * WV * u0;   // byte offset of the window's first column in a table row
     const float v0max = (float)(nv - wv);
     const bool inx = (float)ix == cv.fixx, iny = (float)iy == cv.fixy;
     const float big = 3.0e38f;
@@ -1186,7 +1187,8 @@ __device__ inline void bp_view_col(float* acc, const ConeViewBP& cv, const float
         // (wu, wv) are read but not used: a window load reaches at most
         // (WU - wu) * WV floats past the view's last entry and a row-table load
         // at most one row past nv - 1 (both tables are padded).
-        const int wbase = 4 * WV * (__umul24(v0, nu) + u0), vbase = 8 * v0;   // (nu < 2^24: checked on the host)
+        // one 24-bit multiply-add (4 WV nu < 2^24: nu < 2^20 is checked on the host)
+        const int wbase = (int)__umul24((unsigned)v0, (unsigned)(4 * WV * nu)) + u0w, vbase = 8 * v0;
         float qv[2 * WV];
         float span[NB][WU * WV];
 #pragma unroll
@@ -1431,7 +1433,7 @@ static void siddon_cone_bp_grid(float* img, const float* prj, const ConeGeomDev&
     int wu = plan.wu, wv = plan.wv;
     wu = std::min(std::max(wu, 1), d.nu);
     wv = std::min(std::max(wv, 1), d.nv);
-    const bool col = bp_mode == 0 && column_mode(hgeo, plan, nview) && d.nu < (1 << 24);
+    const bool col = bp_mode == 0 && column_mode(hgeo, plan, nview) && d.nu < (1 << 20);
     const size_t nimg = (size_t)g.nx * g.ny * g.nz, nprj = (size_t)d.nu * d.nv * nview;
     // column mode: batch items sharing the geometry are backprojected up to 4
     // per launch (default z-run; their weight tables within 4 GiB together)
@@ -2018,7 +2020,7 @@ ConePlan::ConePlan(const std::vector<ConeJobSpec>& specs, bool autotune) : impl_
         ConeBPPlan bp = make_cone_bp_plan(hg, g, d, nview);
         if (!column_mode(hg, bp, nview))
             fail("cone plan: every view must have det_v along z and det_u horizontal (column geometry)");
-        if (d.nu >= (1 << 24)) fail("cone plan: detectors of 2^24 columns or more are not supported");
+        if (d.nu >= (1 << 20)) fail("cone plan: detectors of 2^20 columns or more are not supported");
         if (d.nv > 65535 || nview > 65535) fail("cone plan: more than 65535 detector rows or views");
         if (bp.wu > 4 || bp.wv > 4)
             fail("cone plan: voxel shadow spans more than 4 detector cells; use finer voxels or coarser cells");

@@ -88,3 +88,30 @@ def test_empty_sizes_are_a_no_op():
                             ctypes.c_float(0.01), ctypes.c_float(1), ctypes.c_float(0), ctypes.c_float(0),
                             ctypes.c_float(1000), ctypes.c_float(500), ctypes.c_int(2))
     assert err == 0
+
+
+def test_dd_batch_rejects_bad_jobs_before_device_work():
+    """ctpDDBatchCreate validates its job list on the host: no jobs, an empty
+    job and missing arrays are errors (codes, not aborts), and no plan is
+    returned."""
+    from ctprojector_amd import _lib
+    from ctprojector_amd.batched import _DJob
+    lib = _lib.module()
+    plan = ctypes.c_void_p(1)
+    assert lib.ctpDDBatchCreate(ctypes.byref(plan), None, ctypes.c_int(0), ctypes.c_int(0)) != 0
+    assert 'no jobs' in _lib.last_error_message() and not plan.value
+    angles = (ctypes.c_float * 4)(0.0, 0.5, 1.0, 1.5)
+    fake = ctypes.c_void_p(4096).value    # never dereferenced: validation fails first
+
+    def job(**kw):
+        a = dict(fan=1, typeProjector=0, nBatches=1, nx=8, ny=8, nz=1, dx=1, dy=1, dz=1, cx=0, cy=0, cz=0, nu=12,
+                 nv=1, nview=4, da=0.01, dv=1, off_a=0, off_v=0, dsd=200, dso=120,
+                 deg=ctypes.cast(angles, ctypes.c_void_p).value, img=fake, prj=fake)
+        a.update(kw)
+        return _DJob(**a)
+    for bad, msg in [(dict(nx=0), 'empty job'), (dict(deg=None), 'null pointer'), (dict(prj=None), 'null pointer')]:
+        jobs = (_DJob * 2)(job(), job(**bad))
+        plan = ctypes.c_void_p(1)
+        assert lib.ctpDDBatchCreate(ctypes.byref(plan), jobs, ctypes.c_int(2), ctypes.c_int(0)) != 0, bad
+        assert msg in _lib.last_error_message(), (bad, _lib.last_error_message())
+        assert not plan.value

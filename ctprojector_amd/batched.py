@@ -173,7 +173,8 @@ class DDBatch:
     distance_driven_fp), backward writes img[j] = the reference's
     backprojection of prj[j] (distance_driven_bp with the job's bits).  Every
     job's result is that of its single call.  Calls are asynchronous on
-    torch's current stream."""
+    torch's current stream; calls on one batch must not overlap in time (they
+    share its scratch and work counters), so issue them on one stream."""
 
     def __init__(self, jobs: Sequence[DDJob], autotune: bool = False, device=None):
         self.device = torch.device('cuda', torch.cuda.current_device()) if device is None else torch.device(device)

@@ -241,7 +241,9 @@ int ctpTVSQSBackward3D(float* dimg, const float* img, const float* g1, const flo
  * (det_v along z, det_u horizontal in every view).  autotune != 0 times the
  * launch shapes on the bound buffers (overwriting them) and keeps the fastest.
  * ctpConePlanConfig reports them: forward tile width (0 = column-driven form),
- * block order (1 = XCD bands), backprojection z-run. */
+ * block order (1 = XCD bands), backprojection z-run.  Calls on one plan must
+ * not overlap in time (they share its work counters and tables): issue them on
+ * one stream, or order them with events. */
 typedef struct ctpConeJob {
     size_t nx, ny, nz;
     float dx, dy, dz, cx, cy, cz;
@@ -276,7 +278,10 @@ int ctpConePlanDestroy(void* plan);
  * the launch shapes, stream counts and launch counts).  Autotune times the launch shapes on
  * the bound arrays at creation (prj and img are overwritten).  Calls are
  * asynchronous on 'stream' and launch kernels only, so they can be captured
- * in a HIP graph; the batch must be used on the device current at creation. */
+ * in a HIP graph; the batch must be used on the device current at creation.
+ * Calls on one batch must not overlap in time (they share its scratch, work
+ * counters and internal streams): issue them on one stream, or order them
+ * with events. */
 typedef struct ctpDDJob {
     int fan;
     int typeProjector;

@@ -27,6 +27,7 @@
 #include "ops.h"
 
 #include <algorithm>
+#include <cstring>
 #include <memory>
 #include <cmath>
 #include <map>
@@ -957,16 +958,6 @@ static void dd_fp_with_scratch(char* base, const DDFPSetup& s, const std::vector
     dd_fp_run(s, DDFPDev{dviews, drows, dstart, dgroups}, prj, img, imgY, imgX, g, d, nview, dsd, dso, fan, stream);
 }
 
-void dd2d_fp(float* prj, const float* img, const float* deg_host, int B, const Grid& g, const Det& d, int nview,
-             float dsd, float dso, int type, bool fan, hipStream_t stream) {
-    const std::vector<DDView> views = dd_views_for(deg_host, nview, g, type);
-    const DDFPSetup s = dd_fp_setup(views, B, g, d, nview, dsd, dso, type, fan);
-    const size_t bytes = dd_fp_scratch_bytes(s, g, nview);
-    Workspace& ws = workspace_for(stream);
-    ws.reserve(bytes);
-    dd_fp_with_scratch(static_cast<char*>(ws.take(bytes)), s, views, prj, img, g, d, nview, dsd, dso, fan, stream);
-    CTP_CHECK(hipStreamSynchronize(stream));   // host tables above are released on return
-}
 
 // ---------------------------------------------------------------------------
 // Allocation-free forward projection
@@ -1399,27 +1390,147 @@ static void dd_bp_run(const DDBPSetup& s, const DDView* dviews, const SliceRow* 
     if (part) reduce_parts(img, part, groups, (size_t)C * g.nx * g.ny, stream);
 }
 
+// ---------------------------------------------------------------------------
+// geometry tables of the per-call entry points, cached per device
+// ---------------------------------------------------------------------------
+// The reference rebuilds its tables on every call; iterative reconstructions
+// call the same geometry and angles many times.  The view table (with its FP64
+// axis decisions), the forward projector's row / chunk / group tables and the
+// backprojector's slice rows of one (geometry, batch, angles, typeProjector,
+// tuning generation) are kept on the device and reused; the per-call scratch
+// (channel-last copies, weighted projections, partial images) still comes
+// from the stream's workspace.  Entries are pinned while a call uses them and
+// the least recently used unpinned ones are freed beyond kDDCallCache.
+constexpr size_t kDDCallCache = 16;
+
+struct DDCallTables {
+    int device = 0, B = 0, nview = 0, type = 0;
+    bool fan = true;
+    Grid g{};
+    Det d{};
+    float dsd = 0.f, dso = 0.f;
+    long gen = 0;
+    std::vector<float> deg;
+    DDFPSetup fp;
+    DDBPSetup bp;
+    void* mem = nullptr;
+    DDView* views = nullptr;
+    FPRow* rows = nullptr;
+    int* start = nullptr;
+    FPGroup* groups = nullptr;
+    SliceRow* srow = nullptr;
+    int pins = 0;
+    unsigned long long used = 0;
+    bool matches(int dev, const float* a, int B_, const Grid& g_, const Det& d_, int nv, float dsd_, float dso_,
+                 int type_, bool fan_, long gen_) const {
+        return dev == device && B_ == B && nv == nview && type_ == type && fan_ == fan && gen_ == gen &&
+               std::memcmp(&g_, &g, sizeof(Grid)) == 0 && std::memcmp(&d_, &d, sizeof(Det)) == 0 &&
+               std::memcmp(&dsd_, &dsd, sizeof(float)) == 0 && std::memcmp(&dso_, &dso, sizeof(float)) == 0 &&
+               std::memcmp(a, deg.data(), sizeof(float) * nv) == 0;
+    }
+};
+
+namespace {
+std::mutex g_ddc_mutex;
+std::vector<std::unique_ptr<DDCallTables>> g_ddc;
+unsigned long long g_ddc_clock = 0;
+}
+
+// The entry for this call, built on a miss; pinned until dd_tables_release.
+static DDCallTables* dd_tables_acquire(const float* deg_host, int B, const Grid& g, const Det& d, int nview,
+                                       float dsd, float dso, int type, bool fan) {
+    int dev = 0;
+    CTP_CHECK(hipGetDevice(&dev));
+    const long gen = tuning_generation();
+    std::lock_guard<std::mutex> lk(g_ddc_mutex);
+    for (auto& e : g_ddc)
+        if (e->matches(dev, deg_host, B, g, d, nview, dsd, dso, type, fan, gen)) {
+            e->pins++;
+            e->used = ++g_ddc_clock;
+            return e.get();
+        }
+    // free the least recently used unpinned entries of this device beyond the limit
+    while (g_ddc.size() >= kDDCallCache) {
+        auto victim = g_ddc.end();
+        for (auto it = g_ddc.begin(); it != g_ddc.end(); ++it)
+            if ((*it)->pins == 0 && (*it)->device == dev && (victim == g_ddc.end() || (*it)->used < (*victim)->used))
+                victim = it;
+        if (victim == g_ddc.end()) break;
+        (void)hipFree((*victim)->mem);
+        g_ddc.erase(victim);
+    }
+    std::unique_ptr<DDCallTables> e(new DDCallTables());
+    e->device = dev; e->B = B; e->nview = nview; e->type = type; e->fan = fan; e->g = g; e->d = d;
+    e->dsd = dsd; e->dso = dso; e->gen = gen;
+    e->deg.assign(deg_host, deg_host + nview);
+    const std::vector<DDView> views = dd_views_for(deg_host, nview, g, type);
+    e->fp = dd_fp_setup(views, B, g, d, nview, dsd, dso, type, fan);
+    e->bp = dd_bp_setup(B, g, d, nview, type);
+    Plan pl;
+    pl.add<DDView>(nview);
+    pl.add<FPRow>(std::max<size_t>(e->fp.rows.size(), 1));
+    pl.add<int>(e->fp.nchunk + 1);
+    pl.add<FPGroup>(std::max<size_t>(e->fp.groups.size(), 1));
+    pl.add<SliceRow>(g.nz);
+    CTP_CHECK(hipMalloc(&e->mem, pl.bytes));
+    char* m = static_cast<char*>(e->mem);
+    auto carve = [&m](size_t bytes) { void* r = m; m += (bytes + 255) / 256 * 256; return r; };
+    e->views = static_cast<DDView*>(carve(sizeof(DDView) * nview));
+    e->rows = static_cast<FPRow*>(carve(sizeof(FPRow) * std::max<size_t>(e->fp.rows.size(), 1)));
+    e->start = static_cast<int*>(carve(sizeof(int) * (e->fp.nchunk + 1)));
+    e->groups = static_cast<FPGroup*>(carve(sizeof(FPGroup) * std::max<size_t>(e->fp.groups.size(), 1)));
+    e->srow = static_cast<SliceRow*>(carve(sizeof(SliceRow) * g.nz));
+    CTP_CHECK(hipMemcpy(e->views, views.data(), sizeof(DDView) * nview, hipMemcpyHostToDevice));
+    CTP_CHECK(hipMemcpy(e->rows, e->fp.rows.data(), sizeof(FPRow) * e->fp.rows.size(), hipMemcpyHostToDevice));
+    CTP_CHECK(hipMemcpy(e->start, e->fp.start.data(), sizeof(int) * (e->fp.nchunk + 1), hipMemcpyHostToDevice));
+    CTP_CHECK(hipMemcpy(e->groups, e->fp.groups.data(), sizeof(FPGroup) * e->fp.groups.size(),
+                        hipMemcpyHostToDevice));
+    CTP_CHECK(hipMemcpy(e->srow, e->bp.srow.data(), sizeof(SliceRow) * g.nz, hipMemcpyHostToDevice));
+    e->pins = 1;
+    e->used = ++g_ddc_clock;
+    g_ddc.push_back(std::move(e));
+    return g_ddc.back().get();
+}
+
+// Unpins the entry when the call's work has completed (or failed).
+struct DDTablesPin {
+    DDCallTables* t;
+    ~DDTablesPin() {
+        std::lock_guard<std::mutex> lk(g_ddc_mutex);
+        t->pins--;
+    }
+};
+
+void dd2d_fp(float* prj, const float* img, const float* deg_host, int B, const Grid& g, const Det& d, int nview,
+             float dsd, float dso, int type, bool fan, hipStream_t stream) {
+    DDTablesPin pin{dd_tables_acquire(deg_host, B, g, d, nview, dsd, dso, type, fan)};
+    const DDFPSetup& s = pin.t->fp;
+    Workspace& ws = workspace_for(stream);
+    Plan pl;
+    pl.add<float>((size_t)g.ny * s.rsY);
+    pl.add<float>((size_t)g.nx * s.rsX);
+    ws.reserve(pl.bytes);
+    float* imgY = ws.take_n<float>((size_t)g.ny * s.rsY);
+    float* imgX = ws.take_n<float>((size_t)g.nx * s.rsX);
+    dd_fp_run(s, DDFPDev{pin.t->views, pin.t->rows, pin.t->start, pin.t->groups}, prj, img, imgY, imgX, g, d, nview,
+              dsd, dso, fan, stream);
+    CTP_CHECK(hipStreamSynchronize(stream));   // the tables stay pinned until the work is done
+}
+
 void dd2d_bp(float* img, const float* prj, const float* deg_host, int B, const Grid& g, const Det& d, int nview,
              float dsd, float dso, int type, bool fan, hipStream_t stream) {
-    (void)dsd;
-    const std::vector<DDView> views = dd_views_for(deg_host, nview, g, type);
-    const DDBPSetup s = dd_bp_setup(B, g, d, nview, type);
+    DDTablesPin pin{dd_tables_acquire(deg_host, B, g, d, nview, dsd, dso, type, fan)};
+    const DDBPSetup& s = pin.t->bp;
     const size_t npix = (size_t)g.nx * g.ny;
     Workspace& ws = workspace_for(stream);
     Plan pl;
     pl.add<float>((size_t)nview * d.nu * s.cs);
-    pl.add<DDView>(nview);
-    pl.add<SliceRow>(g.nz);
     if (s.groups > 1) pl.add<float>((size_t)s.groups * s.C * npix);
     ws.reserve(pl.bytes);
     float* sino = ws.take_n<float>((size_t)nview * d.nu * s.cs);
-    DDView* dviews = ws.take_n<DDView>(nview);
-    SliceRow* dsrow = ws.take_n<SliceRow>(g.nz);
     float* part = s.groups > 1 ? ws.take_n<float>((size_t)s.groups * s.C * npix) : nullptr;
-    CTP_CHECK(hipMemcpyAsync(dviews, views.data(), sizeof(DDView) * nview, hipMemcpyHostToDevice, stream));
-    CTP_CHECK(hipMemcpyAsync(dsrow, s.srow.data(), sizeof(SliceRow) * g.nz, hipMemcpyHostToDevice, stream));
-    dd_bp_run(s, dviews, dsrow, sino, part, img, prj, g, d, nview, dso, fan, stream);
-    CTP_CHECK(hipStreamSynchronize(stream));
+    dd_bp_run(s, pin.t->views, pin.t->srow, sino, part, img, prj, g, d, nview, dso, fan, stream);
+    CTP_CHECK(hipStreamSynchronize(stream));   // the tables stay pinned until the work is done
 }
 
 // ---------------------------------------------------------------------------

@@ -3,6 +3,7 @@
 #include <cstdio>
 #include <map>
 #include <memory>
+#include <atomic>
 #include <mutex>
 #include <thread>
 #include <tuple>
@@ -151,13 +152,18 @@ int tuning_get(const char* key, int dflt) {
     auto it = g_tune.find(key);
     return (it == g_tune.end() || it->second == 0) ? dflt : it->second;
 }
+std::atomic<long> g_tune_gen{0};
+
 bool tuning_set(const char* key, int value) {
     std::lock_guard<std::mutex> lk(g_tune_mutex);
     auto it = g_tune.find(key);
     if (it == g_tune.end()) return false;
     it->second = value;
+    g_tune_gen.fetch_add(1);
     return true;
 }
+
+long tuning_generation() { return g_tune_gen.load(); }
 
 void timing_enable(bool on) {
     std::lock_guard<std::mutex> lk(g_t_mutex);

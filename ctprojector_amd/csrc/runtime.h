@@ -78,6 +78,8 @@ struct DeviceBuffer {
 // keys are rejected.  Value 0 means "use the built-in default".
 int tuning_get(const char* key, int dflt);
 bool tuning_set(const char* key, int value);
+// Incremented by every tuning_set: tables built from tuning values record it.
+long tuning_generation();
 
 // Per-kernel timing with HIP events on the launching stream (off by default).
 // Launch sites wrap their kernel with TimedLaunch; the C ABI collects the

@@ -186,7 +186,10 @@ CTP_EXPORT int ctpSetTuning(const char* key, int value) {
 CTP_EXPORT int ctpGetTuning(const char* key) { return key ? tuning_get(key, 0) : 0; }
 
 CTP_EXPORT int ctpReleaseWorkspace(void) {
-    return guarded("ctpReleaseWorkspace", [&] { release_all_workspaces(); });
+    return guarded("ctpReleaseWorkspace", [&] {
+        release_all_workspaces();
+        release_dd_call_cache();
+    });
 }
 
 // ===========================================================================

@@ -1492,6 +1492,23 @@ static DDCallTables* dd_tables_acquire(const float* deg_host, int B, const Grid&
     return g_ddc.back().get();
 }
 
+// Frees every entry no call is using (on every device).
+void release_dd_call_cache() {
+    std::lock_guard<std::mutex> lk(g_ddc_mutex);
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    for (auto it = g_ddc.begin(); it != g_ddc.end();) {
+        if ((*it)->pins == 0) {
+            (void)hipSetDevice((*it)->device);
+            (void)hipFree((*it)->mem);
+            it = g_ddc.erase(it);
+        } else {
+            ++it;
+        }
+    }
+    (void)hipSetDevice(cur);
+}
+
 // Unpins the entry when the call's work has completed (or failed).
 struct DDTablesPin {
     DDCallTables* t;

@@ -51,6 +51,9 @@ void dd2d_fp(float* prj, const float* img, const float* deg_host, int B, const G
              float dsd, float dso, int type, bool fan, hipStream_t stream);
 void dd2d_bp(float* img, const float* prj, const float* deg_host, int B, const Grid& g, const Det& d, int nview,
              float dsd, float dso, int type, bool fan, hipStream_t stream);
+// dd2d_fp / dd2d_bp keep the geometry tables of recent (geometry, angles)
+// pairs on the device; this frees the entries no call is using.
+void release_dd_call_cache();
 // dd2d_fp with its scratch (channel-last image copies and tables) in a
 // file-scope device array of kDDFPNoAllocScratchBytes: allocates no device
 // memory; fails with an error when the call needs more scratch; calls that

@@ -323,7 +323,8 @@ const char* ctpGetLastErrorMessage(void);
  * thread's subsequent calls run; each call then synchronises only that stream,
  * so threads with their own streams overlap on the device. */
 int ctpSetStream(void* stream);
-/* Frees the device scratch memory the library keeps per (device, stream). */
+/* Frees the device scratch memory the library keeps per (device, stream) and
+ * the cached geometry tables of the distance-driven entry points. */
 int ctpReleaseWorkspace(void);
 /* Launch-shape knobs used by the autotuner ("cone_fp_tile_u", "cone_bp_zrun");
  * 0 restores the built-in default; unknown keys return hipErrorInvalidValue. */

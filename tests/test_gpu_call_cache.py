@@ -73,3 +73,7 @@ def test_eviction_keeps_results(oracle):
     _say('the first geometry again (evicted and rebuilt)')
     again = fan.distance_driven_fp(p, img, ang)
     assert np.array_equal(first, again)
+    from ctprojector_amd import _lib
+    _say('after ctpReleaseWorkspace (scratch and cached tables freed)')
+    assert _lib.release_workspace() == 0
+    assert np.array_equal(first, fan.distance_driven_fp(p, img, ang))

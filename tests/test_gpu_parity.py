@@ -75,6 +75,26 @@ def test_siddon_cone_batch(nb):
         assert rel_l2(bp[b:b + 1], cone.siddon_bp_arbitrary(p, y[b:b + 1], *args)) < 1e-6
 
 
+def test_siddon_cone_batch_of_four_tall_detector(oracle):
+    """Four volumes sharing the geometry on a 300-row detector: the four-volume
+    column FP follows up to five rays per lane (300 reaching rows in one
+    traversal); equal to single-volume calls and to the oracle."""
+    from helpers import Geo
+    from ctprojector_amd.phantom import circular_cone_geometry, ellipsoid_phantom
+    g = Geo(nx=24, ny=24, nz=40, dx=1.0, dy=1.0, dz=1.0, nu=32, nv=300, du=1.6, dv=0.25, dsd=300.0, dso=180.0)
+    geo = circular_cone_geometry(5, 180.0, 300.0, -1.0, 1.0, jitter=0.5, seed=17)
+    rng = np.random.default_rng(17)
+    base = ellipsoid_phantom(40, 24, 24, 1.0, 1.0, 1.0, seed=17)[None]
+    imgs = np.concatenate([base] + [rng.uniform(size=base.shape).astype(np.float32) for _ in range(3)], 0)
+    p = _proj(g)
+    args = (geo.det_center, geo.det_u, geo.det_v, geo.src)
+    print('cone batch of four, 300-row detector', flush=True)
+    fp = cone.siddon_fp_arbitrary(p, imgs, *args)
+    for b in range(4):
+        assert rel_l2(fp[b:b + 1], cone.siddon_fp_arbitrary(p, imgs[b:b + 1], *args)) < 1e-6
+    assert rel_l2(fp, oracle.siddon_cone_fp(g, imgs, *args)) < 1e-4
+
+
 @pytest.fixture
 def tuning():
     """Set library tuning keys for one test; restores the defaults after."""

@@ -218,11 +218,11 @@ struct ColLine {
     int rx0, ry0, mxo, myo, off0;
 };
 
-// NB volumes of one geometry at once: the traversal and the per-ray split of
-// each cell are shared, only the accumulations are per volume (volume b at
-// col + b * pstride); the volumes are interleaved in volz (z index zp of
-// volume b at float zp * NB + b of a column), so the pairs of all NB volumes
-// come in NB / 2 16-byte loads (one 8-byte load for NB = 1).
+// NB (1 or 2) volumes of one geometry at once: the traversal and the per-ray
+// split of each cell are shared, only the accumulations are per volume
+// (volume b at col + b * pstride); the volumes are interleaved in volz (z
+// index zp of volume b at float zp * NB + b of a column), so the pairs of both
+// volumes come in one 16-byte load (one 8-byte load for NB = 1).
 template <int R, bool ACC, int NB>
 __device__ inline void fp_column_chunk(float* __restrict__ col, __amdgpu_buffer_rsrc_t volz, const ColLine& L,
                                        float3 S, const float* __restrict__ pdc, const float* __restrict__ pdu,
@@ -277,19 +277,12 @@ __device__ inline void fp_column_chunk(float* __restrict__ col, __amdgpu_buffer_
             const int o = 4 * NB * ((int)lo[r] + 1) + off;
             if constexpr (NB == 1) {
                 f[r][0] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(volz, o, 0, 0));
-            } else if constexpr (NB == 2) {
+            } else {
+                static_assert(NB == 2, "1 or 2 volumes");
                 // (v0[lo], v1[lo], v0[lo + 1], v1[lo + 1])
                 const float4 q = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(volz, o, 0, 0));
                 f[r][0] = make_float2(q.x, q.z);
                 f[r][1] = make_float2(q.y, q.w);
-            } else {
-                static_assert(NB == 4, "1, 2 or 4 volumes");
-                const float4 a = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(volz, o, 0, 0));
-                const float4 c = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(volz, o + 16, 0, 0));
-                f[r][0] = make_float2(a.x, c.x);
-                f[r][1] = make_float2(a.y, c.y);
-                f[r][2] = make_float2(a.z, c.z);
-                f[r][3] = make_float2(a.w, c.w);
             }
         }
 #pragma unroll
@@ -326,10 +319,8 @@ __device__ inline void fp_column_chunk(float* __restrict__ col, __amdgpu_buffer_
     }
 }
 
-// rays per lane and traversal for NB volumes (accumulators and loads in
-// flight grow with both; four volumes with five rays take 142 VGPRs and three
-// wavefronts per SIMD, and measured 69.4 against 70.6 ms with three rays for
-// a 256^3/720 batch of four: a 384-row column stays one traversal)
+// rays per lane and traversal (a 320-row column is one traversal; four rays
+// per lane measured 10.1 against 8.2 ms at 256^3/720)
 template <int NB>
 constexpr int col_rays_max() { return 5; }
 
@@ -490,10 +481,13 @@ static void siddon_cone_fp_grid(float* prj, const float* img, const ConeGeomDev&
         const int P = g.nz + 3;
         const size_t ncol = (size_t)g.nx * g.ny * P;
         if (ncol >= (1ull << 29)) fail("cone forward projection: volumes of 2^29 voxels or more are not supported");
-        // batch items sharing the geometry go through in groups of up to 4
-        // (byte offsets within a group's copies stay below 2^31)
+        // batch items sharing the geometry go through in pairs (byte offsets
+        // within a pair's copies stay below 2^31); a kernel for four volumes
+        // measured slower than two pairs (28.4 against 2 x 13.4 ms at
+        // 256^3/720: five rays per lane need 142 VGPRs for four accumulators
+        // each)
         int nbmax = 1;
-        while (nbmax < 4 && nbmax < B && (size_t)(2 * nbmax) * ncol < (1ull << 29)) nbmax *= 2;
+        if (B >= 2 && 2 * ncol < (1ull << 29)) nbmax = 2;
         Workspace& ws = workspace_for(stream);
         Plan pl;
         pl.add<float>(ncol * nbmax);
@@ -508,7 +502,7 @@ static void siddon_cone_fp_grid(float* prj, const float* img, const ConeGeomDev&
         const long nblocks = (long)ceil_div(nview, 1 << lvpb) * 8 * ((ngroup + 7) / 8);
         if (nblocks > 0x7fffffffL) fail("cone forward projection: too many detector columns");
         for (int b = 0; b < B;) {
-            const int nb = B - b >= 4 && nbmax >= 4 ? 4 : (B - b >= 2 && nbmax >= 2 ? 2 : 1);
+            const int nb = B - b >= 2 && nbmax >= 2 ? 2 : 1;
             {
                 TimedLaunch t("siddon_transpose", stream);
                 for (int k = 0; k < nb; k++) {
@@ -522,8 +516,7 @@ static void siddon_cone_fp_grid(float* prj, const float* img, const ConeGeomDev&
     siddon_cone_fp_col_kernel<N><<<(unsigned)nblocks, 256, 0, stream>>>(                                          \
         prj + b * nprj, img + b * nimg, imgz, geo.det_center, geo.det_u, geo.det_v, geo.src, g, d, nview, ngroup, P, \
         xcd_order, (long)nprj, (long)nimg, lvpb)
-            if (nb == 4) CTP_FPC(4);
-            else if (nb == 2) CTP_FPC(2);
+            if (nb == 2) CTP_FPC(2);
             else CTP_FPC(1);
 #undef CTP_FPC
             CTP_LAUNCH_CHECK();

@@ -1890,7 +1890,6 @@ static void dd_plan_check(const DDPlan* p) {
 
 void dd_plan_forward(DDPlan* p, float* prj, const float* img, hipStream_t stream) {
     dd_plan_check(p);
-    const size_t npix = (size_t)p->g.nx * p->g.ny;
     Workspace& ws = workspace_for(stream);
     Plan pl;
     pl.add<float>((size_t)p->g.ny * p->fp.rsY);

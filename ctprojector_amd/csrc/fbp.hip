@@ -681,6 +681,27 @@ struct FBPView {
     float pad0, pad1;
 };
 
+// Detector position pu (cells) and distance weight of the pixel centre (px,
+// py) in the view of direction (c, s), cuda/fbp.cu:288-321 (parallel:
+// cuda/fbpParallel.cu:224-236, weight 1).  Contraction is off and the FMAs
+// are explicit, so that every kernel below gets the same bits wherever it is
+// inlined.
+template <bool FAN>
+__device__ inline void fbp_view_point(float px, float py, float c, float s, float du, float off_u, float half,
+                                      float dso, float& pu, float& dist) {
+#pragma clang fp contract(off)
+    const float rx = fmaf(px, c, py * s);
+    const float ry = fmaf(py, c, -(px * s));
+    float a = rx;
+    dist = 1.f;
+    if (FAN) {
+        const float ys = dso + ry;
+        a = atanf(fast_div(rx, ys));
+        dist = fast_div(dso * dso, fmaf(rx, rx, ys * ys));
+    }
+    pu = half - (fast_div(a, du) + off_u);
+}
+
 // FAN: bpFanKernel3D (cuda/fbp.cu:264-359); otherwise bpParallelKernel3D
 // (cuda/fbpParallel.cu:200-255): u = x cos + y sin, no distance weight.
 template <int CH, bool FAN>
@@ -704,11 +725,8 @@ __global__ __launch_bounds__(256) void fan_fbp_bp_kernel(float* __restrict__ img
     const size_t vstride = (size_t)d.nu * cs;
     for (int view = vbeg; view < vend; view++) {
         const FBPView V = views[view];
-        const float rx = px * V.c + py * V.s;
-        const float ry = -px * V.s + py * V.c;
-        const float a = FAN ? atanf(fast_div(rx, ry + dso)) : rx;
-        const float dist = FAN ? fast_div(dso * dso, rx * rx + (dso + ry) * (dso + ry)) : 1.f;
-        const float pu = -(fast_div(a, d.du) + d.off_u) + half;
+        float pu, dist;
+        fbp_view_point<FAN>(px, py, V.c, V.s, d.du, d.off_u, half, dso, pu, dist);
         // InterpolateXY with truncation, cuda/cudaMath.cu:227-260
         if (pu < 0.f || pu >= (float)d.nu) continue;
         const int u0 = (int)pu;
@@ -772,11 +790,7 @@ __global__ __launch_bounds__(256) void fan_fbp_bp_split_kernel(float* __restrict
         float mpu = -1.f, mdist = 0.f;
         if (v0 + sub < vend) {
             const FBPView V = views[v0 + sub];
-            const float rx = px * V.c + py * V.s;
-            const float ry = -px * V.s + py * V.c;
-            const float a = FAN ? atanf(fast_div(rx, ry + dso)) : rx;
-            mdist = FAN ? fast_div(dso * dso, rx * rx + (dso + ry) * (dso + ry)) : 1.f;
-            mpu = -(fast_div(a, d.du) + d.off_u) + half;
+            fbp_view_point<FAN>(px, py, V.c, V.s, d.du, d.off_u, half, dso, mpu, mdist);
         }
 #pragma unroll
         for (int t = 0; t < LPC; t++) {
@@ -811,6 +825,190 @@ __global__ __launch_bounds__(256) void fan_fbp_bp_split_kernel(float* __restrict
 #pragma unroll
         for (int e = 0; e < 4; e++) {
             const int cc = c0 + 4 * (sub + LPC * k) + e;
+            if (cc >= C) continue;
+            if (part) part[((size_t)grp * C + cc) * npix + pix] = acc[4 * k + e];
+            else img[(size_t)cc * npix + pix] += acc[4 * k + e];
+        }
+    }
+}
+
+// Minimum (MAX false) or maximum over the lanes l of a wavefront with the same
+// l mod LPC: row rotations (DPP) within rows of 16 lanes, then two exchanges
+// across rows.
+template <int LPC, bool MAX>
+__device__ inline int class_reduce(int v) {
+    auto op = [](int a, int b) { return MAX ? max(a, b) : min(a, b); };
+    if constexpr (LPC == 2) v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x122, 0xF, 0xF, false));   // row_ror:2
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x124, 0xF, 0xF, false));                           // row_ror:4
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x128, 0xF, 0xF, false));                           // row_ror:8
+    v = op(v, __shfl_xor(v, 16));
+    return op(v, __shfl_xor(v, 32));
+}
+
+// The split backprojection with each wavefront's detector cells staged in
+// LDS.  Per group of LPC views the wavefront's pixels reduce their cells to
+// one range [min u0, max u1] per view (a reduction over the lanes that hold
+// the same view).  Each view's range is copied into the wavefront's 4 KiB of
+// LDS by whole LDS DMA instructions (global_load_lds_dwordx4: lane l copies
+// float4 l mod Q of cell l / Q, i.e. 64 / Q cells per instruction; Q = CH / 4
+// float4 per cell), with a uniform base per instruction and a per-lane offset
+// that is the same for all of them; the pixels then read their two cells from
+// LDS.  In the direct form every pixel of a wavefront load looks the cells up
+// in the vector L1 again (64 / LPC lookups of a few cells); here each cell
+// passes the L1 once per view.  A group needing more than four instructions
+// reads global memory as the split kernel does.  The view table is read with
+// uniform addresses (scalar loads).  A 16-byte LDS read serves groups of
+// lanes, i.e. groups of pixels that mostly read different cells: pixel p
+// takes its float4s k = 0 .. NQ-1 in the order k ^ (p mod NQ), so that
+// neighbouring pixels read different parts of their cells, i.e. different
+// banks; a lane's accumulator k holds the channels of float4 k ^ (p mod NQ)
+// (measured at 512^2 x 32: other orders within 2%, the kernel is bound by
+// instruction issue, not by LDS).  Every channel sees the arithmetic of
+// fan_fbp_bp_split_kernel in the same order: the same results.  Needs
+// nu < 2^30 and nu >= four instructions' cells (checked by the host).
+template <int CH, int LPC, bool FAN>
+__global__ __launch_bounds__(256) void fan_fbp_bp_stage_kernel(float* __restrict__ img, float* __restrict__ part,
+                                                               const float* __restrict__ q,
+                                                               const FBPView* __restrict__ views, int C, int cs,
+                                                               int nchunk, int vpg, Grid g, Det d, int nview,
+                                                               float dso) {
+    static_assert(CH % (4 * LPC) == 0 && (LPC == 2 || LPC == 4), "a lane holds whole float4s");
+    constexpr int NQ = CH / (4 * LPC);
+    static_assert(NQ == 1 || NQ == 2 || NQ == 4, "float4 order by pixel");
+    constexpr int WX = LPC == 2 ? 8 : 4, WY = 64 / LPC / WX;
+    constexpr int Q = CH / 4;               // float4 per staged cell
+    constexpr int CPI = 64 / Q;             // cells per copy instruction
+    constexpr int KI = 4;                   // copy instructions (KiB) per wavefront and group
+    __shared__ float4 stage[4][64 * KI];
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int p = lane / LPC, sub = lane % LPC, lead = lane & ~(LPC - 1);
+    const int sw = p & (NQ - 1);            // float4 order of this pixel
+    const int ix = blockIdx.x * (2 * WX) + (wave & 1) * WX + p % WX;
+    const int iy = blockIdx.y * (2 * WY) + (wave >> 1) * WY + p / WX;
+    const int grp = blockIdx.z / nchunk;
+    const int c0 = (blockIdx.z - grp * nchunk) * CH;
+    const int vbeg = grp * vpg, vend = min(nview, vbeg + vpg);
+    // every lane of the wavefront takes part in the copies and reductions;
+    // lanes of pixels outside the image add nothing and store nothing
+    const bool valid = ix < g.nx && iy < g.ny;
+    const float px = ((float)min(ix, g.nx - 1) + 0.5f - g.nx / 2.f) * g.dx + g.cx;   // cuda/fbp.cu:288
+    const float py = ((float)min(iy, g.ny - 1) + 0.5f - g.ny / 2.f) * g.dy + g.cy;
+    float acc[4 * NQ];
+#pragma unroll
+    for (int j = 0; j < 4 * NQ; j++) acc[j] = 0.f;
+    const float half = (d.nu - 1.0f) / 2.0f;
+    const size_t vstride = (size_t)d.nu * cs;
+    int fo[NQ];                             // float4 index within a cell of accumulator k
+#pragma unroll
+    for (int k = 0; k < NQ; k++) fo[k] = sub + LPC * (k ^ sw);
+    const unsigned voff = (unsigned)(lane / Q) * (unsigned)cs * 4u + (unsigned)(lane % Q) * 16u;   // copy offset
+    float4* buf = stage[wave];
+    for (int v0 = vbeg; v0 < vend; v0 += LPC) {
+        // detector position and weight of view v0 + sub
+        float vc = 0.f, vs = 0.f;
+#pragma unroll
+        for (int t = 0; t < LPC; t++) {
+            const FBPView V = views[min(v0 + t, nview - 1)];   // uniform address
+            if (sub == t) { vc = V.c; vs = V.s; }
+        }
+        float mpu = -1.f, mdist = 0.f;
+        int lo = 1 << 30, hi = -1;   // empty
+        if (v0 + sub < vend) {
+            fbp_view_point<FAN>(px, py, vc, vs, d.du, d.off_u, half, dso, mpu, mdist);
+            if (valid && mpu >= 0.f && mpu < (float)d.nu) {
+                lo = (int)mpu;
+                hi = min(lo + 1, d.nu - 1);
+            }
+        }
+        lo = class_reduce<LPC, false>(lo);
+        hi = class_reduce<LPC, true>(hi);
+        // view t: copy instructions [slot[t], slot[t+1]) of cells from sbeg[t]
+        // (moved back where the instructions would pass the detector's end)
+        int sbeg[LPC], slot[LPC + 1];
+        slot[0] = 0;
+#pragma unroll
+        for (int t = 0; t < LPC; t++) {
+            const int s0 = __builtin_amdgcn_readlane(lo, t);
+            const int n = max(__builtin_amdgcn_readlane(hi, t) - s0 + 1, 0);
+            const int m = (n + CPI - 1) / CPI;
+            sbeg[t] = min(s0, max(d.nu - m * CPI, 0));
+            slot[t + 1] = slot[t] + m;
+        }
+        const bool staged = slot[LPC] <= KI;   // uniform
+        if (staged) {
+            // the previous group's reads of this LDS are complete
+            asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+            // nu >= KI * CPI (host): every copied cell lies in its detector row
+            const char* vb0 = reinterpret_cast<const char*>(q + c0 + (size_t)v0 * vstride) + voff;
+#pragma unroll
+            for (int t = 0; t < LPC; t++) {
+                const char* vbt = vb0 + (size_t)t * vstride * 4 + (unsigned)sbeg[t] * (unsigned)cs * 4u;
+                for (int i = slot[t]; i < slot[t + 1]; i++)
+                    __builtin_amdgcn_global_load_lds(
+                            (const void*)(vbt + (unsigned)((i - slot[t]) * CPI) * (unsigned)cs * 4u),
+                            (__attribute__((address_space(3))) void*)(buf + 64 * i), 16, 0, 0);
+            }
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            __builtin_amdgcn_wave_barrier();
+        }
+        // the two forms are separate loops, so that each keeps its own
+        // address space (LDS reads, global loads)
+        auto view_step = [&](int t, auto&& load) {
+            const float pu = group_bcast<LPC>(mpu, t, lead);
+            const float dist = group_bcast<LPC>(mdist, t, lead);
+            // InterpolateXY with truncation, cuda/cudaMath.cu:227-260
+            if (!valid || pu < 0.f || pu >= (float)d.nu) return;
+            const int u0 = (int)pu;
+            const int u1 = min(u0 + 1, d.nu - 1);
+            const float wu = pu - (float)u0;
+            const float w0 = (1.f - wu) * dist, w1 = wu * dist;
+            float4 a0[NQ], a1[NQ];
+            load(u0, u1, a0, a1);
+#pragma unroll
+            for (int k = 0; k < NQ; k++) {
+                acc[4 * k] = fmaf(w0, a0[k].x, fmaf(w1, a1[k].x, acc[4 * k]));
+                acc[4 * k + 1] = fmaf(w0, a0[k].y, fmaf(w1, a1[k].y, acc[4 * k + 1]));
+                acc[4 * k + 2] = fmaf(w0, a0[k].z, fmaf(w1, a1[k].z, acc[4 * k + 2]));
+                acc[4 * k + 3] = fmaf(w0, a0[k].w, fmaf(w1, a1[k].w, acc[4 * k + 3]));
+            }
+        };
+        if (staged) {
+#pragma unroll
+            for (int t = 0; t < LPC; t++) {
+                if (v0 + t >= vend) break;   // uniform
+                const float4* vb = buf + (64 * slot[t] - Q * sbeg[t]);
+                view_step(t, [&](int u0, int u1, float4* a0, float4* a1) {
+#pragma unroll
+                    for (int k = 0; k < NQ; k++) {
+                        a0[k] = vb[u0 * Q + fo[k]];
+                        a1[k] = vb[u1 * Q + fo[k]];
+                    }
+                });
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < LPC; t++) {
+                const int view = v0 + t;
+                if (view >= vend) break;   // uniform
+                const float4* __restrict__ vb = reinterpret_cast<const float4*>(q + c0 + view * vstride);
+                view_step(t, [&](int u0, int u1, float4* a0, float4* a1) {
+                    const unsigned o0 = (unsigned)u0 * (unsigned)(cs / 4), o1 = (unsigned)u1 * (unsigned)(cs / 4);
+#pragma unroll
+                    for (int k = 0; k < NQ; k++) {
+                        a0[k] = vb[o0 + fo[k]];
+                        a1[k] = vb[o1 + fo[k]];
+                    }
+                });
+            }
+        }
+    }
+    if (!valid) return;
+    const size_t npix = (size_t)g.nx * g.ny, pix = (size_t)iy * g.nx + ix;
+#pragma unroll
+    for (int k = 0; k < NQ; k++) {
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int cc = c0 + 4 * fo[k] + e;
             if (cc >= C) continue;
             if (part) part[((size_t)grp * C + cc) * npix + pix] = acc[4 * k + e];
             else img[(size_t)cc * npix + pix] += acc[4 * k + e];
@@ -865,12 +1063,25 @@ void fan_fbp_bp(float* img, const float* prj, const float* deg_host, int B, cons
     if (lpc == 4 && CH % 16 != 0) lpc = 2;
     if ((lpc == 2 || lpc == 4) && CH >= 8 && CH % (4 * lpc) == 0) {
         dim3 sgrid(ceil_div(g.nx, lpc == 2 ? 16 : 8), ceil_div(g.ny, 8), nchunk * groups), block(256);
+        // staged cells (measured on MI355X, fan 256^2 to 512^2 x 32 slices:
+        // 0.93 to 0.96 of the split kernel's time; slower at 8 and 16
+        // channels, where a view's copy and range reduction serve fewer
+        // multiply-adds).  Whole copy instructions of 64 / (CH / 4) cells, at
+        // most four per view group, must fit in a detector row.
+        const int stage_mode = tuning_get("fbp_bp_stage", 0);
+        const bool stage = (stage_mode == 1 || (stage_mode == 0 && CH == 32 && lpc == 4)) && d.nu < (1 << 30) &&
+                           d.nu >= 4 * (256 / CH);
         {
         TimedLaunch t(fan ? "fan_fbp_bp" : "par_pixel_bp", stream);
 #define CTP_FBPS_LAUNCH(N, L)                                                                                 \
     if (CH == N && lpc == L) {                                                                                \
-        if (fan) fan_fbp_bp_split_kernel<N, L, true><<<sgrid, block, 0, stream>>>(img, part, q, dviews, C, cs,  \
-                                                                              nchunk, vpg, g, d, nview, dso);   \
+        if (stage) {                                                                                           \
+            if (fan) fan_fbp_bp_stage_kernel<N, L, true><<<sgrid, block, 0, stream>>>(img, part, q, dviews, C,  \
+                                                                                  cs, nchunk, vpg, g, d, nview, dso); \
+            else fan_fbp_bp_stage_kernel<N, L, false><<<sgrid, block, 0, stream>>>(img, part, q, dviews, C,    \
+                                                                                   cs, nchunk, vpg, g, d, nview, dso); \
+        } else if (fan) fan_fbp_bp_split_kernel<N, L, true><<<sgrid, block, 0, stream>>>(img, part, q, dviews, C, \
+                                                                                     cs, nchunk, vpg, g, d, nview, dso); \
         else fan_fbp_bp_split_kernel<N, L, false><<<sgrid, block, 0, stream>>>(img, part, q, dviews, C, cs,    \
                                                                                nchunk, vpg, g, d, nview, dso);  \
     }

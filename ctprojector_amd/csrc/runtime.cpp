@@ -46,6 +46,7 @@ std::map<std::string, int> g_tune = {
     {"dd_fp_lanes_per_column", 0}, // DD FP: lanes per detector column (1 = whole channel vector per lane; 2, 4, 8; default 4)
     {"dd_bp_lanes_per_pixel", 0},  // DD BP (pixel driven): lanes per pixel (1 = whole channel vector per lane; 2, 4; default 4, 2 at 8 channels)
     {"fbp_bp_lanes_per_pixel", 0}, // fan FBP / parallel pixel BP: lanes per pixel (1 = whole channel vector per lane; 2, 4; default 4, 2 at 8 channels)
+    {"fbp_bp_stage", 0},     // fan FBP / parallel pixel BP, split form: cells staged in LDS per wavefront: 1 always, 2 never (default: at 32 channels with 4 lanes per pixel)
     {"dd_fp_segments", 0},   // DD FP direct kernel: row segments per column (1, 2; default by wave count)
     {"tv_mode", 0},          // TV prior: 1 z-marching tile kernels (default), 2 one lane per voxel
     {"ramp_mode", 0},        // ramp filter: 0 FFT convolution where 2nu-1 <= 4096 (default), 1 direct, 2 generic FFT kernel

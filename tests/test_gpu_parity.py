@@ -326,6 +326,56 @@ def test_fbp_bp_split_lanes(oracle, tuning, case):
     assert rel_l2(a, b) < 1e-6
 
 
+@pytest.mark.parametrize('case', ['fan_32', 'fan_8', 'fan_2x20', 'fan_ragged_32', 'fan_mixed_32', 'fan_wide_32',
+                                  'fan_wide_8', 'parallel_32', 'parallel_2x8', 'parallel_wide_32'])
+def test_fbp_bp_staged_cells_equal_global(oracle, tuning, case):
+    """The split FBP / pixel-driven backprojection with each wavefront's
+    detector cells staged in LDS (fbp_bp_stage 1; the default at 32 channels
+    with 4 lanes per pixel) performs the arithmetic of the
+    same kernel reading global memory in the same order: equal results, and
+    both within the tolerance of the oracle.  'wide' geometries (4 mm pixels on
+    0.3 mm cells) give cell ranges larger than a wavefront's 4 KiB of LDS, so
+    every view group takes the global-memory reads; 'mixed' (1 mm pixels on
+    0.3 mm cells) sits at the limit, some groups staged and some not; 'ragged'
+    is a 45 x 43 image (partial tiles on two sides).  Staging needs detectors
+    of at least four copy instructions' cells (256 / CH per chunk of CH
+    channels): the 8-slice case has 160 cells of half the size."""
+    parts = case.split('_')
+    beam, nzs = parts[0], parts[-1]
+    shape = parts[1] if len(parts) == 3 else 'plain'
+    nz, B = (int(nzs.split('x')[1]), int(nzs.split('x')[0])) if 'x' in nzs else (int(nzs), 1)
+    nview = 41
+    if beam == 'fan':
+        kw = dict(nx=48, ny=48, nz=nz, dx=1.0, dy=1.0, dz=1.0, cx=1.0, cy=-0.5, cz=0.3, nu=80, nv=nz + 2, du=1.5,
+                  dv=1.0, off_u=0.4, off_v=0.3, dsd=200.0, dso=120.0)
+        if nz == 8 and shape == 'plain':
+            kw.update(nu=160, du=0.75)
+        if shape == 'ragged':
+            kw.update(nx=45, ny=43)
+        if shape == 'mixed':
+            kw.update(nu=420, du=0.3)
+        if shape == 'wide':
+            kw.update(nx=16, ny=16, dx=4.0, dy=4.0, nu=1000, du=0.3)
+        ang = (np.arange(nview) * 2 * np.pi / nview).astype(np.float32)
+        run, ref_fn = fan.fbp_bp, oracle.fan_fbp_bp
+    else:
+        kw = dict(nx=48, ny=48, nz=nz, dx=1.0, dy=1.0, dz=1.0, nu=70, nv=nz, du=1.0, dv=1.0, off_u=0.4, off_v=-0.2)
+        if shape == 'wide':
+            kw.update(nx=16, ny=16, dx=4.0, dy=4.0, nu=320, du=0.3)
+        ang = (np.arange(nview) * np.pi / nview).astype(np.float32)
+        run, ref_fn = parallel.pixel_driven_bp, oracle.par_pixel_bp
+    g = Geo(**kw)
+    y = np.random.default_rng(10).uniform(size=[B, nview, g.nv, g.nu]).astype(np.float32)
+    tuning('fbp_bp_stage', 1)
+    a = run(_proj(g), y, ang)
+    tuning('fbp_bp_stage', 2)
+    b = run(_proj(g), y, ang)
+    e = rel_l2(a, ref_fn(g, y, ang))
+    print('fbp bp staged', case, e, rel_l2(a, b))
+    assert np.array_equal(a, b), rel_l2(a, b)
+    assert e < TOL
+
+
 # ----------------------------------------------------------------- tomo
 def _tomo_case(B=1):
     g = Geo(nx=40, ny=36, nz=12, dx=1.0, dy=1.0, dz=1.0, cz=20.0, nu=56, nv=52, du=1.0, dv=1.0)

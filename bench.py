@@ -578,6 +578,7 @@ def fbp_extra(steps, w, img, outputs=None):
     nsamp = w.nu * w.nv * w.nview
     rf_ms, _ = kernel_stats(times, 'ramp_filter')
     bp_ms, _ = kernel_stats(times, 'fan_fbp_bp')
+    bp_traffic = read_traffic('fan_fbp_bp')
     return {
         'workload': 'fan ramp_filter (Hann) + fbp_bp, same volume/scan',
         'ms_per_step': round((t1 - t0) / steps * 1e3, 3),
@@ -589,7 +590,12 @@ def fbp_extra(steps, w, img, outputs=None):
         'ramp_filter_fft_tflop_s': round(5.0 * fft_n(w.nu) * np.log2(fft_n(w.nu)) * nsamp / w.nu / (rf_ms / 1e3)
                                          / 1e12, 2) if rf_ms else None,
         'fbp_bp_gvu_s': round(nvu / (bp_ms / 1e3) / 1e9, 2) if bp_ms else None,
-        'fbp_bp_hbm_frac': round(4 * nvu / (bp_ms / 1e3) / 1e9 / PEAK_HBM_GBS, 4) if bp_ms else None,
+        # HBM traffic of the backprojection (counters, L2-miss bytes) over its
+        # time; the 4 B per voxel-update ratio beside it is a throughput ratio
+        # (caches serve most of those reads), not a bound
+        'fbp_bp_hbm_frac': round(bp_traffic / (bp_ms / 1e3) / 1e9 / PEAK_HBM_GBS, 4) if bp_ms and bp_traffic else None,
+        'fbp_bp_algorithmic_4B_ratio_to_peak': round(4 * nvu / (bp_ms / 1e3) / 1e9 / PEAK_HBM_GBS, 4)
+        if bp_ms else None,
     }
 
 
@@ -639,7 +645,7 @@ def short_name(kname):
 TIMED_KERNELS = {'dd_fan_fp': 'dd_fp_split_kernel<32, 4, true>', 'dd_fan_bp': 'dd_bp_split_kernel<32, 4, true>',
                  'siddon_cone_fp': 'siddon_cone_fp_col_kernel<1>',
                  'siddon_cone_bp': 'siddon_cone_bp_col_kernel<2, 2, 4, 1>',
-                 'fan_fbp_bp': 'fan_fbp_bp_split_kernel<32, 4, true>', 'ramp_filter': 'ramp_fft16_kernel<2048>'}
+                 'fan_fbp_bp': 'fan_fbp_bp_stage_kernel<32, 4, true>', 'ramp_filter': 'ramp_fft16_kernel<2048>'}
 
 
 def roofline(dom, dom_ms, nvu, compulsory_bytes, times):

@@ -1449,7 +1449,9 @@ static DDCallTables* dd_tables_acquire(const float* deg_host, int B, const Grid&
             e->used = ++g_ddc_clock;
             return e.get();
         }
-    // free the least recently used unpinned entries of this device beyond the limit
+    // free the least recently used unpinned entries of this device beyond the
+    // limit (entries of other devices and entries in use stay: the cache can
+    // then hold more than kDDCallCache entries for a while)
     while (g_ddc.size() >= kDDCallCache) {
         auto victim = g_ddc.end();
         for (auto it = g_ddc.begin(); it != g_ddc.end(); ++it)
@@ -1473,19 +1475,24 @@ static DDCallTables* dd_tables_acquire(const float* deg_host, int B, const Grid&
     pl.add<FPGroup>(std::max<size_t>(e->fp.groups.size(), 1));
     pl.add<SliceRow>(g.nz);
     CTP_CHECK(hipMalloc(&e->mem, pl.bytes));
-    char* m = static_cast<char*>(e->mem);
-    auto carve = [&m](size_t bytes) { void* r = m; m += (bytes + 255) / 256 * 256; return r; };
-    e->views = static_cast<DDView*>(carve(sizeof(DDView) * nview));
-    e->rows = static_cast<FPRow*>(carve(sizeof(FPRow) * std::max<size_t>(e->fp.rows.size(), 1)));
-    e->start = static_cast<int*>(carve(sizeof(int) * (e->fp.nchunk + 1)));
-    e->groups = static_cast<FPGroup*>(carve(sizeof(FPGroup) * std::max<size_t>(e->fp.groups.size(), 1)));
-    e->srow = static_cast<SliceRow*>(carve(sizeof(SliceRow) * g.nz));
-    CTP_CHECK(hipMemcpy(e->views, views.data(), sizeof(DDView) * nview, hipMemcpyHostToDevice));
-    CTP_CHECK(hipMemcpy(e->rows, e->fp.rows.data(), sizeof(FPRow) * e->fp.rows.size(), hipMemcpyHostToDevice));
-    CTP_CHECK(hipMemcpy(e->start, e->fp.start.data(), sizeof(int) * (e->fp.nchunk + 1), hipMemcpyHostToDevice));
-    CTP_CHECK(hipMemcpy(e->groups, e->fp.groups.data(), sizeof(FPGroup) * e->fp.groups.size(),
-                        hipMemcpyHostToDevice));
-    CTP_CHECK(hipMemcpy(e->srow, e->bp.srow.data(), sizeof(SliceRow) * g.nz, hipMemcpyHostToDevice));
+    try {
+        char* m = static_cast<char*>(e->mem);
+        auto carve = [&m](size_t bytes) { void* r = m; m += (bytes + 255) / 256 * 256; return r; };
+        e->views = static_cast<DDView*>(carve(sizeof(DDView) * nview));
+        e->rows = static_cast<FPRow*>(carve(sizeof(FPRow) * std::max<size_t>(e->fp.rows.size(), 1)));
+        e->start = static_cast<int*>(carve(sizeof(int) * (e->fp.nchunk + 1)));
+        e->groups = static_cast<FPGroup*>(carve(sizeof(FPGroup) * std::max<size_t>(e->fp.groups.size(), 1)));
+        e->srow = static_cast<SliceRow*>(carve(sizeof(SliceRow) * g.nz));
+        CTP_CHECK(hipMemcpy(e->views, views.data(), sizeof(DDView) * nview, hipMemcpyHostToDevice));
+        CTP_CHECK(hipMemcpy(e->rows, e->fp.rows.data(), sizeof(FPRow) * e->fp.rows.size(), hipMemcpyHostToDevice));
+        CTP_CHECK(hipMemcpy(e->start, e->fp.start.data(), sizeof(int) * (e->fp.nchunk + 1), hipMemcpyHostToDevice));
+        CTP_CHECK(hipMemcpy(e->groups, e->fp.groups.data(), sizeof(FPGroup) * e->fp.groups.size(),
+                            hipMemcpyHostToDevice));
+        CTP_CHECK(hipMemcpy(e->srow, e->bp.srow.data(), sizeof(SliceRow) * g.nz, hipMemcpyHostToDevice));
+    } catch (...) {
+        (void)hipFree(e->mem);   // the entry is not kept
+        throw;
+    }
     e->pins = 1;
     e->used = ++g_ddc_clock;
     g_ddc.push_back(std::move(e));
@@ -2447,9 +2454,16 @@ void DDBatchPlan::Impl::run(const std::vector<DDLaunch>& ls, int ctr_base, int p
 }
 
 float DDBatchPlan::Impl::time_direction(bool forward_dir, hipStream_t s) {
-    hipEvent_t a, b;
-    CTP_CHECK(hipEventCreate(&a));
-    CTP_CHECK(hipEventCreate(&b));
+    struct OwnEvents {   // released on every exit
+        hipEvent_t a = nullptr, b = nullptr;
+        ~OwnEvents() {
+            if (a) (void)hipEventDestroy(a);
+            if (b) (void)hipEventDestroy(b);
+        }
+    } ev;
+    CTP_CHECK(hipEventCreate(&ev.a));
+    CTP_CHECK(hipEventCreate(&ev.b));
+    hipEvent_t a = ev.a, b = ev.b;
     float best = 1e30f;
     for (int rep = 0; rep < 4; rep++) {   // the first run warms up
         CTP_CHECK(hipEventRecord(a, s));
@@ -2461,8 +2475,6 @@ float DDBatchPlan::Impl::time_direction(bool forward_dir, hipStream_t s) {
         CTP_CHECK(hipEventElapsedTime(&ms, a, b));
         if (rep > 0) best = std::min(best, ms);
     }
-    CTP_CHECK(hipEventDestroy(a));
-    CTP_CHECK(hipEventDestroy(b));
     return best;
 }
 
@@ -2491,8 +2503,17 @@ DDBatchPlan::~DDBatchPlan() = default;
 void DDBatchPlan::tune() {
     Impl& I = *impl_;
     TimingSuspended quiet;
-    hipStream_t s = nullptr;
-    CTP_CHECK(hipStreamCreate(&s));
+    struct OwnStream {   // the timing stream, released on every exit
+        hipStream_t s = nullptr;
+        ~OwnStream() {
+            if (s) {
+                (void)hipStreamSynchronize(s);
+                (void)hipStreamDestroy(s);
+            }
+        }
+    } own;
+    CTP_CHECK(hipStreamCreate(&own.s));
+    hipStream_t s = own.s;
     float best_f = 1e30f, best_b = 1e30f;
     int bf = 4, bb = 4, bpf = 1, bpb = 1, bof = 1, bob = 1;
     for (int pers = 1; pers >= 0; pers--)
@@ -2519,7 +2540,6 @@ void DDBatchPlan::tune() {
     I.fanout_b = bob;
     I.layout();
     CTP_CHECK(hipStreamSynchronize(s));
-    CTP_CHECK(hipStreamDestroy(s));
 }
 
 void DDBatchPlan::forward(hipStream_t s) {

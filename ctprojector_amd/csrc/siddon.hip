@@ -2119,9 +2119,16 @@ void ConePlan::config(int& tile_u, int& xcd_order, int& zrun) const {
 // overwritten.
 void ConePlan::tune() {
     Impl& P = *impl_;
-    hipEvent_t a, b;
-    CTP_CHECK(hipEventCreate(&a));
-    CTP_CHECK(hipEventCreate(&b));
+    struct OwnEvents {   // released on every exit
+        hipEvent_t a = nullptr, b = nullptr;
+        ~OwnEvents() {
+            if (a) (void)hipEventDestroy(a);
+            if (b) (void)hipEventDestroy(b);
+        }
+    } ev;
+    CTP_CHECK(hipEventCreate(&ev.a));
+    CTP_CHECK(hipEventCreate(&ev.b));
+    hipEvent_t a = ev.a, b = ev.b;
     auto time = [&](auto&& fn) {
         fn();
         CTP_CHECK(hipEventRecord(a, nullptr));
@@ -2155,8 +2162,6 @@ void ConePlan::tune() {
     }
     P.nzr = bz;
     P.layout();
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
     CTP_CHECK(hipDeviceSynchronize());
 }
 

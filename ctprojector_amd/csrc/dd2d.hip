@@ -1568,10 +1568,13 @@ void dd2d_bp(float* img, const float* prj, const float* deg_host, int B, const G
 // channels; per view and trace the pixel's edge interval is projected onto the
 // detector to find the candidate columns, and each candidate's weight is
 // recomputed from the same FP32 line parameters the forward projector uses.
-struct FPLine {            // per (view, column): edge lines of both traces and their norms
-    float r1, o1, b1, r2, o2, b2, ny;   // trace along y: x = r * (t - o) + b
-    float q1, p1, c1, q2, p2, c2, nx;   // trace along x: y = q * (t - p) + c
-    float pad0, pad1;
+// Candidates whose weight is not positive add 0 times their channels, so a
+// non-finite detector value also reaches the pixels next to its footprint.
+struct FPTrace {           // one trace of a (view, column): edge lines r * (t - o) + b and the norm
+    float r1, o1, b1, r2, o2, b2, norm, pad;
+};
+struct FPLine {            // per (view, column): trace along y (x of row t), trace along x (y of column t)
+    FPTrace tr[2];         // 32-byte aligned records: a candidate's trace is two 16-byte loads
 };
 
 template <bool FAN>
@@ -1583,16 +1586,12 @@ __global__ __launch_bounds__(256) void dd_fp_lines_kernel(FPLine* __restrict__ l
     float2 s1, s2, d1, d2;
     const float a = fp_cell_edges<FAN>(iu, V, g, d, dsd, dso, s1, s2, d1, d2);
     FPLine L{};
-    if (V.flags & 1) {
-        L.r1 = (d1.x - s1.x) / (d1.y - s1.y); L.o1 = s1.y; L.b1 = s1.x;
-        L.r2 = (d2.x - s2.x) / (d2.y - s2.y); L.o2 = s2.y; L.b2 = s2.x;
-        L.ny = V.wx * g.dy / fabsf(cosf(V.deg - a));
-    }
-    if (V.flags & 2) {
-        L.q1 = (d1.y - s1.y) / (d1.x - s1.x); L.p1 = s1.x; L.c1 = s1.y;
-        L.q2 = (d2.y - s2.y) / (d2.x - s2.x); L.p2 = s2.x; L.c2 = s2.y;
-        L.nx = (1.f - V.wx) * g.dx / fabsf(sinf(V.deg - a));
-    }
+    if (V.flags & 1)
+        L.tr[0] = FPTrace{(d1.x - s1.x) / (d1.y - s1.y), s1.y, s1.x, (d2.x - s2.x) / (d2.y - s2.y), s2.y, s2.x,
+                          V.wx * g.dy / fabsf(cosf(V.deg - a)), 0.f};
+    if (V.flags & 2)
+        L.tr[1] = FPTrace{(d1.y - s1.y) / (d1.x - s1.x), s1.x, s1.y, (d2.y - s2.y) / (d2.x - s2.x), s2.x, s2.y,
+                          (1.f - V.wx) * g.dx / fabsf(sinf(V.deg - a)), 0.f};
     lines[(size_t)view * d.nu + iu] = L;
 }
 
@@ -1624,23 +1623,6 @@ void rows_to_channel_last(float* q, const float* prj, const int* cstart, const C
     dd_adj_rows_kernel<<<dim3(ceil_div(d.nu, 64), ceil_div(cs, 4), nview), 256, 0, stream>>>(
         q, prj, nullptr, cstart, crow, C, cs, nz, d, nview, false);
     CTP_LAUNCH_CHECK();
-}
-
-template <int CH>
-__device__ inline void adj_apply(float* acc, const float* __restrict__ p, float w) {
-    if constexpr (CH % 4 == 0) {
-#pragma unroll
-        for (int j = 0; j < CH; j += 4) {
-            const float4 q = *reinterpret_cast<const float4*>(p + j);
-            acc[j] = fmaf(w, q.x, acc[j]);
-            acc[j + 1] = fmaf(w, q.y, acc[j + 1]);
-            acc[j + 2] = fmaf(w, q.z, acc[j + 2]);
-            acc[j + 3] = fmaf(w, q.w, acc[j + 3]);
-        }
-    } else {
-#pragma unroll
-        for (int j = 0; j < CH; j++) acc[j] = fmaf(w, p[j], acc[j]);
-    }
 }
 
 // detector column coordinate (column iu spans [iu, iu + 1]) of a physical point
@@ -1690,16 +1672,32 @@ __global__ __launch_bounds__(256) void dd_fp_adjoint_kernel(float* __restrict__ 
             const int n = tr == 0 ? g.nx : g.ny;
             const float lo = tr == 0 ? (float)ix : (float)iy;
             for (int k = k0; k <= k1; k++) {
-                const FPLine L = Lv[k];
-                float x1, x2, sc;
-                if (tr == 0) { x1 = fmaf(L.r1, t - L.o1, L.b1); x2 = fmaf(L.r2, t - L.o2, L.b2); sc = L.ny; }
-                else { x1 = fmaf(L.q1, t - L.p1, L.c1); x2 = fmaf(L.q2, t - L.p2, L.c2); sc = L.nx; }
+                // the candidate's trace record and its channels load together
+                // (one memory round trip per candidate; taking candidates in
+                // pairs measured slower); a candidate whose weight is not
+                // positive adds 0 times its channels
+                const float4* __restrict__ lp = reinterpret_cast<const float4*>(&Lv[k].tr[tr]);
+                const float4 l0 = lp[0], l1 = lp[1];   // r1 o1 b1 r2 | o2 b2 norm -
+                float q[CH];
+                const float* __restrict__ gp = Gv + (size_t)k * cs;
+                if constexpr (CH % 4 == 0) {
+#pragma unroll
+                    for (int j = 0; j < CH; j += 4) {
+                        const float4 v = *reinterpret_cast<const float4*>(gp + j);
+                        q[j] = v.x; q[j + 1] = v.y; q[j + 2] = v.z; q[j + 3] = v.w;
+                    }
+                } else {
+#pragma unroll
+                    for (int j = 0; j < CH; j++) q[j] = gp[j];
+                }
+                const float x1 = fmaf(l0.x, t - l0.y, l0.z), x2 = fmaf(l0.w, t - l1.x, l1.y);
                 // the forward projector's IntegralBoxX weight of this pixel (dd_line)
-                if (x1 > x2) { const float tt = x1; x1 = x2; x2 = tt; }
-                const float a = clampf(x1, 0.f, (float)n), b = clampf(x2, 0.f, (float)n);
-                if (!(b > a)) continue;
-                const float w = (fminf(b, lo + 1.f) - fmaxf(a, lo)) * fast_div(sc, x2 - x1);
-                if (w > 0.f) adj_apply<CH>(acc, Gv + (size_t)k * cs, w);
+                const float xl = min_f(x1, x2), xh = max_f(x1, x2);
+                const float a = med_clamp(xl, 0.f, (float)n), b = med_clamp(xh, 0.f, (float)n);
+                const float wt = (fminf(b, lo + 1.f) - fmaxf(a, lo)) * fast_div(l1.z, xh - xl);
+                const float w = (b > a && wt > 0.f) ? wt : 0.f;
+#pragma unroll
+                for (int j = 0; j < CH; j++) acc[j] = fmaf(w, q[j], acc[j]);
             }
         }
     }

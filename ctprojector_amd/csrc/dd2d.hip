@@ -799,20 +799,29 @@ static DDFPSetup dd_fp_setup(const std::vector<DDView>& views, int B, const Grid
     // the branchless parallel FP has no force-FBP reverse weighting
     // (cuda/distanceDriven2DBranchless.cu:321-360)
     s.force = (type & DD_FORCE_FBP) != 0 && !branchless;
-    // row segments: two when one gives the direct kernel fewer than 4 waves
-    // per SIMD (4096 on 1024 SIMDs) and each segment keeps at least 32 rows
-    // (measured at 8 slices of 256x256, 492 views, 444 columns: 1 segment
-    // 0.46 ms, 2 segments 0.25 ms; 4 segments, two per wavefront, 0.25 ms)
-    const int rows_min = std::min(g.nx, g.ny);
-    int lseg = (rows_min / 2 >= 32 && 2L * ceil_div(d.nu, 128) * nview * nchunk < 4096) ? 1 : 0;
+    // lanes per column: the split kernel needs one chunk without the next
+    // chunk's channel, whole float4s per lane and unpadded CH-float pixels.
+    // Default: two float4s per lane (CH / 8 lanes) from 16 channels on, one
+    // lane per column below (measured on MI355X, fan FP: 256^2 x 32, 492
+    // views: 4 lanes 0.59 ms, 2 lanes 0.78, one lane with two row segments
+    // 0.97; 16 channels at 256^2/984 views and 384^2/720: 2 lanes 0.73 and
+    // 0.88 ms, 4 lanes 0.89 and 1.12; 8 channels at 256^2/492: one lane with
+    // two row segments 0.19 ms, 2 lanes 0.25)
     const int forced = tuning_get("dd_fp_segments", 0);
+    const int lpc = lpc_req > 0 ? std::min(lpc_req, std::max(CH / 4, 1))
+                                : tuning_get("dd_fp_lanes_per_column", CH >= 16 ? CH / 8 : 1);
+    const bool split = !s.xc && forced != 2 && (CH == 8 || CH == 16 || CH == 32) && (lpc == 2 || lpc == 4 || lpc == 8) &&
+                       CH % (4 * lpc) == 0;
+    // row segments of the one-lane kernel: two when one gives it fewer than
+    // 4 waves per SIMD (4096 on 1024 SIMDs) and each segment keeps at least
+    // 32 rows (measured at 8 slices of 256x256, 492 views, 444 columns: 1
+    // segment 0.46 ms, 2 segments 0.25 ms; 4 segments, two per wavefront,
+    // 0.25 ms)
+    const int rows_min = std::min(g.nx, g.ny);
+    int lseg = (!split && rows_min / 2 >= 32 && 2L * ceil_div(d.nu, 128) * nview * nchunk < 4096) ? 1 : 0;
     if (forced == 1 || forced == 2) lseg = forced - 1;
     s.lseg = lseg;
-    // lanes per column: the split kernel needs one chunk without the next
-    // chunk's channel, whole float4s per lane and unpadded CH-float pixels
-    const int lpc = lpc_req > 0 ? std::min(lpc_req, std::max(CH / 4, 1)) : tuning_get("dd_fp_lanes_per_column", 4);
-    if (!s.xc && lseg == 0 && (CH == 8 || CH == 16 || CH == 32) && (lpc == 2 || lpc == 4 || lpc == 8) &&
-        CH % (4 * lpc) == 0) {
+    if (split) {
         s.lpc = lpc;
         s.cs = CH;
     }
@@ -1271,6 +1280,18 @@ __global__ __launch_bounds__(256) void dd_bp_split_kernel(float* __restrict__ im
 // the launch has enough waves in flight to hide the per-view load latency
 // (one lane per pixel alone gives 4 waves per SIMD at 512x512).  Partial
 // images of the groups are summed in a fixed order afterwards: deterministic.
+// Lanes per pixel of the DD backprojection for a chunk of CH channels (the
+// dd_bp_lanes_per_pixel key, else two float4 per lane: 4 lanes at 32
+// channels, 2 at 16 and at 8).  Measured on MI355X, fan 984 views: at
+// 512x512x32 one lane 5.09 ms, two 3.19, four 3.07; at 16 channels two lanes
+// against four 0.45 / 0.55 ms (256^2), 1.48 / 1.96 ms (512^2), and at 384^2
+// with 720 views 0.58 / 0.72 ms.
+static int dd_bp_lanes(int CH) {
+    const int t = tuning_get("dd_bp_lanes_per_pixel", 0);
+    if (t > 0) return t == 4 && CH % 16 != 0 ? 2 : t;
+    return CH >= 32 ? 4 : 2;
+}
+
 int bp_view_groups(size_t npix, int nchunk, int nview) {
     // The pixel-driven backprojections launch one 256-lane block per 16x16
     // pixels and channel chunk.  Below 1024 blocks (4 per CU) the launch
@@ -1352,11 +1373,7 @@ static void dd_bp_run(const DDBPSetup& s, const DDView* dviews, const SliceRow* 
         CTP_LAUNCH_CHECK();
     }
     TimedLaunch tb(fan ? "dd_fan_bp" : "dd_par_bp", stream);
-    // lanes per pixel: 4 where the chunk splits into float4 quarters, else 2
-    // (measured at 512x512x32, 984 views: 5.09 ms with one lane, 3.19 ms with
-    // two, 3.07 ms with four)
-    int lpc = tuning_get("dd_bp_lanes_per_pixel", 4);
-    if (lpc == 4 && CH % 16 != 0) lpc = 2;
+    int lpc = dd_bp_lanes(CH);
     if ((lpc == 2 || lpc == 4) && CH >= 8 && CH % (4 * lpc) == 0) {
         const int bx = lpc == 2 ? 16 : 8, by = lpc == 2 ? 8 : 8;
         dim3 grid(ceil_div(g.nx, bx), ceil_div(g.ny, by), nchunk * groups), block(256);
@@ -2338,8 +2355,7 @@ void DDBatchPlan::Impl::layout() {
         DDLaunch pr;
         pr.kind = DDL_PREP;
         add(bwd, pr, j, (long)J.pr_gx * J.pr_gy * S.nview);
-        int lpc = bp_lpc > 0 ? bp_lpc : tuning_get("dd_bp_lanes_per_pixel", 4);
-        if (lpc == 4 && bs.CH % 16 != 0) lpc = 2;
+        int lpc = bp_lpc > 0 ? bp_lpc : dd_bp_lanes(bs.CH);
         DDLaunch bk;
         bk.fan = S.fan;
         bk.ch = bs.CH;
@@ -2556,8 +2572,8 @@ void DDBatchPlan::backward(hipStream_t s) {
 
 void DDBatchPlan::config(int& fp_lanes, int& bp_lanes, int& fp_persistent, int& bp_persistent, int& fp_streams,
                          int& bp_streams) const {
-    fp_lanes = impl_->fp_lpc > 0 ? impl_->fp_lpc : tuning_get("dd_fp_lanes_per_column", 4);
-    bp_lanes = impl_->bp_lpc > 0 ? impl_->bp_lpc : tuning_get("dd_bp_lanes_per_pixel", 4);
+    fp_lanes = impl_->fp_lpc > 0 ? impl_->fp_lpc : tuning_get("dd_fp_lanes_per_column", 0);
+    bp_lanes = impl_->bp_lpc > 0 ? impl_->bp_lpc : tuning_get("dd_bp_lanes_per_pixel", 0);
     fp_persistent = impl_->pf;
     bp_persistent = impl_->pb;
     fp_streams = impl_->fanout_f;

@@ -43,8 +43,8 @@ std::map<std::string, int> g_tune = {
     {"dd_fp_extra_channel", 0},  // DD FP direct kernel: 1 keeps the next chunk's channel with one chunk too
     {"dd_fp_pixel_stride", 0},   // DD FP direct kernel, one channel chunk: floats per pixel of the channel-last copy
     {"dd_fp_views_per_block", 0},  // DD FP direct kernel, one row segment: views per block (1, 2, 4; default 4)
-    {"dd_fp_lanes_per_column", 0}, // DD FP: lanes per detector column (1 = whole channel vector per lane; 2, 4, 8; default 4)
-    {"dd_bp_lanes_per_pixel", 0},  // DD BP (pixel driven): lanes per pixel (1 = whole channel vector per lane; 2, 4; default 4, 2 at 8 channels)
+    {"dd_fp_lanes_per_column", 0}, // DD FP: lanes per detector column (1 = whole channel vector per lane; 2, 4, 8; default CH / 8 from 16 channels on, else 1)
+    {"dd_bp_lanes_per_pixel", 0},  // DD BP (pixel driven): lanes per pixel (1 = whole channel vector per lane; 2, 4; default 4 at 32 channels, else 2)
     {"fbp_bp_lanes_per_pixel", 0}, // fan FBP / parallel pixel BP: lanes per pixel (1 = whole channel vector per lane; 2, 4; default 4, 2 at 8 channels)
     {"fbp_bp_stage", 0},     // fan FBP / parallel pixel BP, split form: cells staged in LDS per wavefront: 1 always, 2 never (default: at 32 channels with 4 lanes per pixel)
     {"dd_fp_segments", 0},   // DD FP direct kernel: row segments per column (1, 2; default by wave count)

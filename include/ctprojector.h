@@ -275,7 +275,8 @@ int ctpConePlanDestroy(void* plan);
  * The jobs whose kernels are the same template instance run in one launch,
  * and the launches of different kernel classes may run concurrently on up to
  * four internal streams forked from and joined to 'stream' (Config reports
- * the launch shapes, stream counts and launch counts).  Autotune times the launch shapes on
+ * the launch shapes, stream counts and launch counts; lanes 0: each job's
+ * own default, as in the single calls).  Autotune times the launch shapes on
  * the bound arrays at creation (prj and img are overwritten).  Calls are
  * asynchronous on 'stream' and launch kernels only, so they can be captured
  * in a HIP graph; the batch must be used on the device current at creation.

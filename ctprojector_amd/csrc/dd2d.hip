@@ -807,18 +807,20 @@ static DDFPSetup dd_fp_setup(const std::vector<DDView>& views, int B, const Grid
     // 0.97; 16 channels at 256^2/984 views and 384^2/720: 2 lanes 0.73 and
     // 0.88 ms, 4 lanes 0.89 and 1.12; 8 channels at 256^2/492: one lane with
     // two row segments 0.19 ms, 2 lanes 0.25)
-    const int forced = tuning_get("dd_fp_segments", 0);
-    const int lpc = lpc_req > 0 ? std::min(lpc_req, std::max(CH / 4, 1))
-                                : tuning_get("dd_fp_lanes_per_column", CH >= 16 ? CH / 8 : 1);
-    const bool split = !s.xc && forced != 2 && (CH == 8 || CH == 16 || CH == 32) && (lpc == 2 || lpc == 4 || lpc == 8) &&
-                       CH % (4 * lpc) == 0;
     // row segments of the one-lane kernel: two when one gives it fewer than
     // 4 waves per SIMD (4096 on 1024 SIMDs) and each segment keeps at least
     // 32 rows (measured at 8 slices of 256x256, 492 views, 444 columns: 1
     // segment 0.46 ms, 2 segments 0.25 ms; 4 segments, two per wavefront,
-    // 0.25 ms)
+    // 0.25 ms); below 16 channels such grids keep the one-lane kernel even
+    // where split lanes are asked for (2 lanes 0.27 ms there)
+    const int forced = tuning_get("dd_fp_segments", 0);
     const int rows_min = std::min(g.nx, g.ny);
-    int lseg = (!split && rows_min / 2 >= 32 && 2L * ceil_div(d.nu, 128) * nview * nchunk < 4096) ? 1 : 0;
+    const int seg_rule = (rows_min / 2 >= 32 && 2L * ceil_div(d.nu, 128) * nview * nchunk < 4096) ? 1 : 0;
+    const int lpc = lpc_req > 0 ? std::min(lpc_req, std::max(CH / 4, 1))
+                                : tuning_get("dd_fp_lanes_per_column", CH >= 16 ? CH / 8 : 1);
+    const bool split = !s.xc && forced != 2 && (CH >= 16 || seg_rule == 0) && (CH == 8 || CH == 16 || CH == 32) &&
+                       (lpc == 2 || lpc == 4 || lpc == 8) && CH % (4 * lpc) == 0;
+    int lseg = split ? 0 : seg_rule;
     if (forced == 1 || forced == 2) lseg = forced - 1;
     s.lseg = lseg;
     if (split) {
@@ -2355,7 +2357,7 @@ void DDBatchPlan::Impl::layout() {
         DDLaunch pr;
         pr.kind = DDL_PREP;
         add(bwd, pr, j, (long)J.pr_gx * J.pr_gy * S.nview);
-        int lpc = bp_lpc > 0 ? bp_lpc : dd_bp_lanes(bs.CH);
+        int lpc = bp_lpc > 0 ? (bp_lpc == 4 && bs.CH % 16 != 0 ? 2 : bp_lpc) : dd_bp_lanes(bs.CH);
         DDLaunch bk;
         bk.fan = S.fan;
         bk.ch = bs.CH;

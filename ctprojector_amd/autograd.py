@@ -21,7 +21,7 @@ higher-order gradients work.  ``tv_sqs`` is the TV surrogate of the HIP prior
 kernel with a hand-written HIP vector-Jacobian product (first order only);
 ``tv_sqs_torch`` is the same surrogate in torch arithmetic.
 ``sqs_tv_unrolled`` runs K SQS-TV iterations as a differentiable network with
-optional per-iteration checkpointing.
+per-iteration checkpointing where memory requires it.
 """
 
 from typing import Callable, Optional, Sequence, Tuple
@@ -228,19 +228,35 @@ def tv_sqs(img: torch.Tensor, weights: Sequence[float] = (1.0, 1.0, 1.0), eps: f
     return _TVSurrogate.apply(img, weights, eps)
 
 
+def unrolled_needs_checkpoint(prj: torch.Tensor, x0: torch.Tensor, niter: int) -> bool:
+    """Whether sqs_tv_unrolled checkpoints by default: when keeping every
+    iteration's intermediates (two sinograms and four volumes, counted
+    generously) would take more than a quarter of the device's free memory."""
+    if not x0.is_cuda:
+        return False
+    free, _ = torch.cuda.mem_get_info(x0.device)
+    need = niter * (2 * prj.numel() + 4 * x0.numel()) * x0.element_size()
+    return need > free // 4
+
+
 def sqs_tv_unrolled(op: _Op, prj: torch.Tensor, x0: torch.Tensor, norm_img: torch.Tensor, projector_norm: float,
                     beta, niter: int, weights: Sequence[float] = (1.0, 1.0, 1.0), eps: float = 1e-8,
-                    weight: Optional[torch.Tensor] = None, checkpoint: bool = True, tv: str = 'kernel'
-                    ) -> torch.Tensor:
+                    weight: Optional[torch.Tensor] = None, checkpoint: Optional[bool] = None,
+                    tv: str = 'kernel') -> torch.Tensor:
     """K differentiable SQS-TV iterations (recon/cupy/sqs_algorithms.py:22-57 with
     prior_func = tv_sqs): x <- x - (A^T w (A x - p) / n^2 + beta s1) / (norm_img + beta s2).
     ``op.adjoint`` supplies A^T so that the network's gradient is exact.
-    ``checkpoint`` recomputes each iteration in the backward pass instead of
-    keeping its intermediates (memory per iteration: one volume).  ``tv``
+    ``checkpoint`` True recomputes each iteration in the backward pass instead
+    of keeping its intermediates (two more projector passes per iteration;
+    memory per iteration: one volume); False keeps them (about one sinogram
+    and four volumes per iteration); None (default) keeps them unless that
+    would take more than a quarter of the device's free memory.  ``tv``
     selects the surrogate: 'kernel' (HIP forward and gradient, first-order
     gradients) or 'torch' (torch arithmetic, any order)."""
     if tv not in ('kernel', 'torch'):
         raise ValueError("tv must be 'kernel' or 'torch'")
+    if checkpoint is None:
+        checkpoint = unrolled_needs_checkpoint(prj, x0, niter)
     w = 1.0 if weight is None else weight
     pn = float(projector_norm)
     surrogate = tv_sqs if tv == 'kernel' else tv_sqs_torch

@@ -6,7 +6,7 @@
   FP64 oracle in the CPU suite);
 * gradients of fp / bp / adjoint are the hand-written transposes (dot tests
   through autograd), Siddon cone and DD fan;
-* an unrolled SQS-TV network (fan DD, 3 iterations, checkpointed): the
+* an unrolled SQS-TV network (fan DD, 3 iterations, checkpointed or not): the
   directional derivative of a loss with respect to the projection data from
   autograd on the GPU (FP32) against central finite differences of the same
   network in FP64 on the CPU, built from the FP64 oracle (dense system matrix
@@ -138,7 +138,8 @@ def _fp64_network(oracle, g, ang, A, norm_img, pn, beta, niter, eps):
     return run
 
 
-def test_unrolled_sqs_tv_gradient_vs_fp64_finite_differences(torch, oracle):
+@pytest.mark.parametrize('checkpoint', [True, False])
+def test_unrolled_sqs_tv_gradient_vs_fp64_finite_differences(torch, oracle, checkpoint):
     from ctprojector_amd.autograd import FanDD, sqs_tv_unrolled
     from ctprojector_amd.projector import cupy as C
     g, ang, img = small_fan(n=16, nz=1, nv=1, nview=24, nu=32, du=2.0, seed=4)
@@ -161,7 +162,7 @@ def test_unrolled_sqs_tv_gradient_vs_fp64_finite_differences(torch, oracle):
     p = torch.tensor(p0, dtype=torch.float32, device=dev, requires_grad=True)
     x = sqs_tv_unrolled(op, p, torch.zeros([1, 1, n, n], device=dev),
                         torch.tensor(norm_img, dtype=torch.float32, device=dev), pn, beta, niter,
-                        eps=eps, checkpoint=True)
+                        eps=eps, checkpoint=checkpoint)
     l = ((x - torch.tensor(target, dtype=torch.float32, device=dev)) ** 2).sum()
     l.backward()
     ad = float((p.grad.double() * torch.tensor(dp, device=dev)).sum())

@@ -29,7 +29,7 @@ SYMBOLS = [
     'cSiddonFanProjection', 'cSiddonFanBackprojection', 'cfbpParallelBackprojection', 'cupyFbpTomoFilter',
     'ctpDistanceDrivenFanProjectionAdjoint', 'ctpDistanceDrivenParallelProjectionAdjoint',
     'ctpDistanceDrivenFanProjectionNoAlloc',
-    'cupyTVSQS3D', 'cTVSQS3D', 'ctpTVSQSUpdate3D', 'ctpTVSQSBackward3D',
+    'cupyTVSQS3D', 'cTVSQS3D', 'ctpTVSQSUpdate3D', 'ctpTVSQSBackward3D', 'ctpTVSQS3DAsync', 'ctpTVSQSBackward3DAsync',
     'ctpDDPlanCreate', 'ctpDDPlanForward', 'ctpDDPlanBackward', 'ctpDDPlanAdjoint', 'ctpDDPlanDestroy',
     'ctpConePlanCreate', 'ctpConePlanForward', 'ctpConePlanBackward', 'ctpConePlanConfig', 'ctpConePlanDestroy',
     'ctpDDBatchCreate', 'ctpDDBatchForward', 'ctpDDBatchBackward', 'ctpDDBatchConfig', 'ctpDDBatchDestroy',

@@ -761,6 +761,37 @@ CTP_EXPORT int ctpTVSQSBackward3D(float* dimg, const float* img, const float* g1
     });
 }
 
+// Asynchronous forms of cupyTVSQS3D and ctpTVSQSBackward3D (no counterpart in
+// the reference): the kernels are queued on 'stream' and the call returns
+// without waiting, so a caller's loop (the differentiable SQS-TV network)
+// keeps the device busy while the host queues the next work.
+CTP_EXPORT int ctpTVSQS3DAsync(float* s1, float* s2, float* var, const float* img, float wx, float wy, float wz,
+                               size_t nBatches, size_t nx, size_t ny, size_t nz, float eps, void* stream) {
+    return guarded("ctpTVSQS3DAsync", [&] {
+        if (nBatches == 0 || nx == 0 || ny == 0 || nz == 0) return;
+        tv_dims(nBatches, nx, ny, nz);
+        need_ptr(s1, "s1"); need_ptr(s2, "s2"); need_ptr(var, "var"); need_ptr(img, "img");
+        touch();
+        tv_sqs(s1, s2, var, img, wx, wy, wz, (int)nBatches, (int)nx, (int)ny, (int)nz, eps,
+               static_cast<hipStream_t>(stream));
+    });
+}
+
+CTP_EXPORT int ctpTVSQSBackward3DAsync(float* dimg, const float* img, const float* g1, const float* g2,
+                                       const float* gvar, float wx, float wy, float wz, size_t nBatches, size_t nx,
+                                       size_t ny, size_t nz, float eps, void* stream) {
+    return guarded("ctpTVSQSBackward3DAsync", [&] {
+        if (nBatches == 0 || nx == 0 || ny == 0 || nz == 0) return;
+        tv_dims(nBatches, nx, ny, nz);
+        need_ptr(dimg, "dimg"); need_ptr(img, "img"); need_ptr(g1, "g1"); need_ptr(g2, "g2");
+        need(dimg != img && dimg != g1 && dimg != g2 && dimg != gvar,
+             "ctpTVSQSBackward3DAsync: dimg must not alias an input");
+        touch();
+        tv_sqs_backward(dimg, img, g1, g2, gvar, wx, wy, wz, (int)nBatches, (int)nx, (int)ny, (int)nz, eps,
+                        static_cast<hipStream_t>(stream));
+    });
+}
+
 // ===========================================================================
 // Batched cone-beam plans (no counterpart in the reference)
 // ===========================================================================

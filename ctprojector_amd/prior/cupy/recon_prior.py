@@ -3,7 +3,7 @@ src/ct_projector/prior/cupy/recon_prior.py:85-135 (``tv_sqs``) with torch
 tensors, plus ``tv_sqs_update``, the fused SQS-TV step.  The NLM priors of that
 module are not on the projector path and are not provided."""
 
-from ctypes import c_float, c_ulong
+from ctypes import c_float, c_ulong, c_void_p
 from typing import Optional, Tuple
 
 import torch
@@ -12,20 +12,26 @@ from ... import _lib
 from ...projector.cupy._dev import dev_f32, device_of, on_device, ptr
 
 
+def _stream(dev) -> c_void_p:
+    return c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+
+
 def tv_sqs(img: torch.Tensor, weights: Tuple[float, float, float], eps: float = 1e-8):
     """Anisotropic TV surrogate of img [batch, nz, ny, nx]; weights along (z, y, x).
-    Returns (s1, s2, var) as device tensors."""
+    Returns (s1, s2, var) as device tensors, queued on torch's current stream
+    like torch's own operations (ctpTVSQS3DAsync)."""
     dev = device_of(img)
     img = dev_f32(img, dev)
-    s1 = torch.zeros_like(img)
-    s2 = torch.zeros_like(img)
-    var = torch.zeros_like(img)
-    with on_device(dev):
-        err = _lib.module().cupyTVSQS3D(
+    s1 = torch.empty_like(img)
+    s2 = torch.empty_like(img)
+    var = torch.empty_like(img)
+    with torch.cuda.device(dev):
+        err = _lib.module().ctpTVSQS3DAsync(
             ptr(s1), ptr(s2), ptr(var), ptr(img), c_float(weights[2]), c_float(weights[1]), c_float(weights[0]),
-            c_ulong(img.shape[0]), c_ulong(img.shape[3]), c_ulong(img.shape[2]), c_ulong(img.shape[1]), c_float(eps))
+            c_ulong(img.shape[0]), c_ulong(img.shape[3]), c_ulong(img.shape[2]), c_ulong(img.shape[1]), c_float(eps),
+            _stream(dev))
     if err != 0:
-        print(err)
+        raise RuntimeError('ctpTVSQS3DAsync failed (%d): %s' % (err, _lib.last_error_message()))
     return s1, s2, var
 
 
@@ -33,20 +39,21 @@ def tv_sqs_backward(img: torch.Tensor, g1: torch.Tensor, g2: torch.Tensor, weigh
                     eps: float = 1e-8, gvar: Optional[torch.Tensor] = None) -> torch.Tensor:
     """Vector-Jacobian product of ``tv_sqs``: the gradient with respect to img of
     sum(g1 * s1) + sum(g2 * s2) + sum(gvar * var) (no counterpart in the
-    reference; used by the differentiable unrolled SQS-TV network)."""
+    reference; used by the differentiable unrolled SQS-TV network), queued on
+    torch's current stream (ctpTVSQSBackward3DAsync)."""
     dev = device_of(img)
     img = dev_f32(img, dev)
     g1 = dev_f32(g1, dev).expand_as(img).contiguous()
     g2 = dev_f32(g2, dev).expand_as(img).contiguous()
     gv = dev_f32(gvar, dev).expand_as(img).contiguous() if gvar is not None else None
     out = torch.empty_like(img)
-    with on_device(dev):
-        err = _lib.module().ctpTVSQSBackward3D(
+    with torch.cuda.device(dev):
+        err = _lib.module().ctpTVSQSBackward3DAsync(
             ptr(out), ptr(img), ptr(g1), ptr(g2), ptr(gv) if gv is not None else None, c_float(weights[2]),
             c_float(weights[1]), c_float(weights[0]), c_ulong(img.shape[0]), c_ulong(img.shape[3]),
-            c_ulong(img.shape[2]), c_ulong(img.shape[1]), c_float(eps))
+            c_ulong(img.shape[2]), c_ulong(img.shape[1]), c_float(eps), _stream(dev))
     if err != 0:
-        raise RuntimeError('ctpTVSQSBackward3D failed (%d): %s' % (err, _lib.last_error_message()))
+        raise RuntimeError('ctpTVSQSBackward3DAsync failed (%d): %s' % (err, _lib.last_error_message()))
     return out
 
 

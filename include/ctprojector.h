@@ -227,6 +227,17 @@ int ctpTVSQSUpdate3D(float* out, float* var, const float* img, const float* bp, 
  * (zero).  dimg is overwritten and must not alias the inputs.  Device arrays. */
 int ctpTVSQSBackward3D(float* dimg, const float* img, const float* g1, const float* g2, const float* gvar, float wx,
                        float wy, float wz, size_t nBatches, size_t nx, size_t ny, size_t nz, float eps);
+/* No counterpart in the reference: cupyTVSQS3D and ctpTVSQSBackward3D queued
+ * on 'stream' (a hipStream_t, NULL for the null stream) without waiting for
+ * completion, for callers that order their work by stream (the
+ * differentiable SQS-TV network).  Arguments and results as above;
+ * ctpTVSQSBackward3DAsync keeps scratch in the library's workspace of that
+ * stream and host thread. */
+int ctpTVSQS3DAsync(float* s1, float* s2, float* var, const float* img, float wx, float wy, float wz, size_t nBatches,
+                    size_t nx, size_t ny, size_t nz, float eps, void* stream);
+int ctpTVSQSBackward3DAsync(float* dimg, const float* img, const float* g1, const float* g2, const float* gvar,
+                            float wx, float wy, float wz, size_t nBatches, size_t nx, size_t ny, size_t nz, float eps,
+                            void* stream);
 
 /* ---- batched cone-beam plans (no counterpart in the reference) ------------ */
 /* Many Siddon cone-beam jobs (volumes of different sizes, each with its own

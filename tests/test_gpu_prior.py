@@ -76,3 +76,34 @@ def test_sqs_tv_loop_decreases_loss(torch):
         losses.append(float(dl) + 1e-3 * float(pl))
     assert losses[-1] < losses[0]
     assert rel_l2(x1.cpu().numpy(), x2.cpu().numpy()) < 1e-5
+
+
+def test_async_entry_points_equal_waiting_ones(torch):
+    """ctpTVSQS3DAsync / ctpTVSQSBackward3DAsync (queued on a stream, no wait)
+    give the bits of cupyTVSQS3D / ctpTVSQSBackward3D, on a side stream too."""
+    from ctypes import c_float, c_ulong, c_void_p
+    from ctprojector_amd import _lib
+    from ctprojector_amd.prior.cupy import tv_sqs_backward
+    x = torch.from_numpy(_img(3)).cuda()
+    B, nz, ny, nx = x.shape
+    w = (0.5, 1.0, 2.0)
+    dims = (c_ulong(B), c_ulong(nx), c_ulong(ny), c_ulong(nz), c_float(1e-4))
+    ws = (c_float(w[2]), c_float(w[1]), c_float(w[0]))
+    p = lambda t: c_void_p(t.data_ptr()) if t is not None else None
+    sync = [torch.zeros_like(x) for _ in range(3)]
+    assert _lib.module().cupyTVSQS3D(*[p(t) for t in sync], p(x), *ws, *dims) == 0
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        asy = [torch.empty_like(x) for _ in range(3)]
+        assert _lib.module().ctpTVSQS3DAsync(*[p(t) for t in asy], p(x), *ws, *dims,
+                                             c_void_p(side.cuda_stream)) == 0
+    torch.cuda.current_stream().wait_stream(side)
+    for a, b in zip(sync, asy):
+        assert torch.equal(a, b)
+    g1, g2, gv = (torch.rand_like(x) for _ in range(3))
+    for gvar in (None, gv):
+        d_sync = torch.empty_like(x)
+        assert _lib.module().ctpTVSQSBackward3D(p(d_sync), p(x), p(g1), p(g2), p(gvar), *ws, *dims) == 0
+        d_async = tv_sqs_backward(x, g1, g2, w, 1e-4, gvar=gvar)
+        assert torch.equal(d_sync, d_async)

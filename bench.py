@@ -216,7 +216,8 @@ def unrolled_sqs_extra(steps):
     iterations' intermediates are kept, as the default does where they fit in
     a quarter of the free HBM) on 8 fan-beam 256^2 slices @1.4 mm, 492 views,
     444 columns, Poisson data at 1e5 photons: one training step = the network
-    forward and the gradient with respect to the sinogram."""
+    forward and the gradient with respect to the sinogram, timed eagerly and
+    as a HIP graph replay of the whole step (the value reported)."""
     import torch
     from ctprojector_amd.autograd import FanDD, sqs_tv_unrolled, unrolled_needs_checkpoint
     from ctprojector_amd.phantom import fan_slices_256_workload
@@ -245,22 +246,62 @@ def unrolled_sqs_extra(steps):
     niter, beta = 10, 2e-3
     ckpt = unrolled_needs_checkpoint(prj, x_true, niter)
 
+    x0 = torch.zeros_like(x_true)
+
+    def net_and_grad(q):
+        # loss = sum((x - x_true)^2); its gradient 2 (x - x_true) is handed to
+        # backward directly, so the step holds no reduction
+        x = sqs_tv_unrolled(op, q, x0, norm_img, pn, beta, niter, weight=wgt, checkpoint=ckpt)
+        x.backward(2.0 * (x - x_true))
+        return x
+
     def step():
         q = prj.clone().requires_grad_(True)
-        x = sqs_tv_unrolled(op, q, torch.zeros_like(x_true), norm_img, pn, beta, niter, weight=wgt)
-        loss = ((x - x_true) ** 2).sum()
-        loss.backward()
-        return loss, q.grad
+        x = net_and_grad(q)
+        return x, q.grad
 
     log('[bench] unrolled SQS-TV warmup')
     step()
     torch.cuda.synchronize()
-    log('[bench] unrolled SQS-TV timed x%d' % steps)
+    log('[bench] unrolled SQS-TV eager x%d' % steps)
     t0 = time.perf_counter()
     for _ in range(steps):
-        loss, grad = step()
+        x, grad = step()
     torch.cuda.synchronize()
-    ms = (time.perf_counter() - t0) / steps * 1e3
+    eager_ms = (time.perf_counter() - t0) / steps * 1e3
+    # the same training step captured once in a HIP graph and replayed: every
+    # kernel of the forward and the gradient runs in each replay, without the
+    # host's per-launch work (warm-up on a side stream first, as graph capture
+    # requires; the gradient buffer is written, not accumulated, by each replay)
+    graphed = not ckpt
+    ms = eager_ms
+    same = None
+    if graphed:
+        log('[bench] unrolled SQS-TV graph capture')
+        q_static = prj.clone().requires_grad_(True)
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            for _ in range(2):
+                q_static.grad = None
+                net_and_grad(q_static)
+        torch.cuda.current_stream().wait_stream(side)
+        q_static.grad = None
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            x_static = net_and_grad(q_static)
+        g_static = q_static.grad
+        log('[bench] unrolled SQS-TV graph replay x%d' % steps)
+        graph.replay()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            graph.replay()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / steps * 1e3
+        same = bool(torch.equal(g_static, grad)) and bool(torch.equal(x_static, x))
+        x, grad = x_static, g_static
+    loss = ((x - x_true) ** 2).sum()
     nvu = img.shape[0] * w.nx * w.ny * w.nz * w.nview
     # FP and A^T per iteration forward, again in a checkpointed recomputation,
     # and the two products of the backward pass: 4 or 6 passes per iteration
@@ -268,6 +309,7 @@ def unrolled_sqs_extra(steps):
     return {'workload': 'unrolled SQS-TV, 10 iterations, 8 fan slices 256^2 @1.4 mm, 492 views, 444 columns, '
                         'Poisson 1e5; forward + gradient w.r.t. the sinogram',
             'ms_per_step': round(ms, 3), 'ms_per_iteration': round(ms / niter, 3), 'checkpoint': ckpt,
+            'hip_graph': graphed, 'eager_ms_per_step': round(eager_ms, 3), 'graph_equals_eager': same,
             'projector_passes_per_step': passes, 'value_gvu_s': round(passes * nvu / (ms / 1e3) / 1e9, 2),
             'finite': bool(torch.isfinite(grad).all().item()) and bool(torch.isfinite(loss).item())}
 

@@ -778,17 +778,19 @@ CTP_EXPORT int ctpTVSQS3DAsync(float* s1, float* s2, float* var, const float* im
 }
 
 CTP_EXPORT int ctpTVSQSBackward3DAsync(float* dimg, const float* img, const float* g1, const float* g2,
-                                       const float* gvar, float wx, float wy, float wz, size_t nBatches, size_t nx,
-                                       size_t ny, size_t nz, float eps, void* stream) {
+                                       const float* gvar, float* scratch, float wx, float wy, float wz,
+                                       size_t nBatches, size_t nx, size_t ny, size_t nz, float eps, void* stream) {
     return guarded("ctpTVSQSBackward3DAsync", [&] {
         if (nBatches == 0 || nx == 0 || ny == 0 || nz == 0) return;
         tv_dims(nBatches, nx, ny, nz);
         need_ptr(dimg, "dimg"); need_ptr(img, "img"); need_ptr(g1, "g1"); need_ptr(g2, "g2");
         need(dimg != img && dimg != g1 && dimg != g2 && dimg != gvar,
              "ctpTVSQSBackward3DAsync: dimg must not alias an input");
+        need(scratch == nullptr || (reinterpret_cast<size_t>(scratch) % 8 == 0 && scratch != dimg),
+             "ctpTVSQSBackward3DAsync: scratch must be 8-byte aligned and distinct from dimg");
         touch();
         tv_sqs_backward(dimg, img, g1, g2, gvar, wx, wy, wz, (int)nBatches, (int)nx, (int)ny, (int)nz, eps,
-                        static_cast<hipStream_t>(stream));
+                        static_cast<hipStream_t>(stream), scratch);
     });
 }
 

@@ -1854,8 +1854,17 @@ struct DDPlan {
     int* cstart = nullptr;
     ChanRow* crow = nullptr;
     FPLine* lines = nullptr;
+    // scratch of each direction, owned by the plan (no shared workspace, so
+    // that calls can be captured in a HIP graph): channel-last image copies
+    // of the forward, channel-last sinogram and partial images of the
+    // backprojection and of the adjoint
+    void* scratch = nullptr;
+    float *imgY = nullptr, *imgX = nullptr;
+    float *bp_sino = nullptr, *bp_part = nullptr;
+    float *adj_G = nullptr, *adj_part = nullptr;
     ~DDPlan() {
         if (mem) (void)hipFree(mem);
+        if (scratch) (void)hipFree(scratch);
     }
 };
 
@@ -1902,6 +1911,23 @@ DDPlan* dd_plan_create(const float* deg_host, int B, const Grid& g, const Det& d
         CTP_CHECK(hipMemcpyAsync(p->crow, p->adj.crow.data(), sizeof(ChanRow) * p->adj.crow.size(),
                                  hipMemcpyHostToDevice, stream));
     dd_adj_lines(p->lines, p->views, g, d, nview, dsd, dso, fan, stream);
+    const size_t npix = (size_t)g.nx * g.ny;
+    Plan ps;
+    ps.add<float>((size_t)g.ny * p->fp.rsY);
+    ps.add<float>((size_t)g.nx * p->fp.rsX);
+    ps.add<float>((size_t)nview * d.nu * p->bp.cs);
+    ps.add<float>(p->bp.groups > 1 ? (size_t)p->bp.groups * p->bp.C * npix : 0);
+    ps.add<float>((size_t)nview * d.nu * p->adj.cs);
+    ps.add<float>(p->adj.groups > 1 ? (size_t)p->adj.groups * p->adj.C * npix : 0);
+    CTP_CHECK(hipMalloc(&p->scratch, ps.bytes));
+    char* q = static_cast<char*>(p->scratch);
+    auto carve_f = [&q](size_t n) { float* r = reinterpret_cast<float*>(q); q += (n * sizeof(float) + 255) / 256 * 256; return r; };
+    p->imgY = carve_f((size_t)g.ny * p->fp.rsY);
+    p->imgX = carve_f((size_t)g.nx * p->fp.rsX);
+    p->bp_sino = carve_f((size_t)nview * d.nu * p->bp.cs);
+    p->bp_part = p->bp.groups > 1 ? carve_f((size_t)p->bp.groups * p->bp.C * npix) : nullptr;
+    p->adj_G = carve_f((size_t)nview * d.nu * p->adj.cs);
+    p->adj_part = p->adj.groups > 1 ? carve_f((size_t)p->adj.groups * p->adj.C * npix) : nullptr;
     CTP_CHECK(hipStreamSynchronize(stream));
     return p.release();
 }
@@ -1914,45 +1940,21 @@ static void dd_plan_check(const DDPlan* p) {
 
 void dd_plan_forward(DDPlan* p, float* prj, const float* img, hipStream_t stream) {
     dd_plan_check(p);
-    Workspace& ws = workspace_for(stream);
-    Plan pl;
-    pl.add<float>((size_t)p->g.ny * p->fp.rsY);
-    pl.add<float>((size_t)p->g.nx * p->fp.rsX);
-    ws.reserve(pl.bytes);
-    float* imgY = ws.take_n<float>((size_t)p->g.ny * p->fp.rsY);
-    float* imgX = ws.take_n<float>((size_t)p->g.nx * p->fp.rsX);
-    dd_fp_run(p->fp, DDFPDev{p->views, p->rows, p->start, p->groups}, prj, img, imgY, imgX, p->g, p->d, p->nview,
-              p->dsd, p->dso, p->fan, stream);
+    dd_fp_run(p->fp, DDFPDev{p->views, p->rows, p->start, p->groups}, prj, img, p->imgY, p->imgX, p->g, p->d,
+              p->nview, p->dsd, p->dso, p->fan, stream);
 }
 
 void dd_plan_backward(DDPlan* p, float* img, const float* prj, hipStream_t stream) {
     dd_plan_check(p);
-    const DDBPSetup& s = p->bp;
-    const size_t npix = (size_t)p->g.nx * p->g.ny;
-    Workspace& ws = workspace_for(stream);
-    Plan pl;
-    pl.add<float>((size_t)p->nview * p->d.nu * s.cs);
-    if (s.groups > 1) pl.add<float>((size_t)s.groups * s.C * npix);
-    ws.reserve(pl.bytes);
-    float* sino = ws.take_n<float>((size_t)p->nview * p->d.nu * s.cs);
-    float* part = s.groups > 1 ? ws.take_n<float>((size_t)s.groups * s.C * npix) : nullptr;
-    dd_bp_run(s, p->views, p->srow, sino, part, img, prj, p->g, p->d, p->nview, p->dso, p->fan, stream);
+    dd_bp_run(p->bp, p->views, p->srow, p->bp_sino, p->bp_part, img, prj, p->g, p->d, p->nview, p->dso, p->fan,
+              stream);
 }
 
 void dd_plan_adjoint(DDPlan* p, float* img, const float* prj, hipStream_t stream) {
     dd_plan_check(p);
-    const DDAdjSetup& s = p->adj;
-    const size_t npix = (size_t)p->g.nx * p->g.ny;
-    Workspace& ws = workspace_for(stream);
-    Plan pl;
-    pl.add<float>((size_t)p->nview * p->d.nu * s.cs);
-    if (s.groups > 1) pl.add<float>((size_t)s.groups * s.C * npix);
-    ws.reserve(pl.bytes);
-    float* G = ws.take_n<float>((size_t)p->nview * p->d.nu * s.cs);
-    float* part = s.groups > 1 ? ws.take_n<float>((size_t)s.groups * s.C * npix) : nullptr;
     TimedLaunch t(p->fan ? "dd_fan_fp_adjoint" : "dd_par_fp_adjoint", stream);
-    dd_adj_run(s, p->views, p->lines, p->cstart, p->crow, G, part, img, prj, p->g, p->d, p->nview, p->dso, p->fan,
-               stream);
+    dd_adj_run(p->adj, p->views, p->lines, p->cstart, p->crow, p->adj_G, p->adj_part, img, prj, p->g, p->d,
+               p->nview, p->dso, p->fan, stream);
 }
 
 void dd_plan_destroy(DDPlan* p) {

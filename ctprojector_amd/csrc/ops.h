@@ -189,8 +189,9 @@ void tv_sqs(float* s1, float* s2, float* var, const float* img, float wx, float 
 void tv_sqs_update(float* out, float* var, const float* img, const float* bp, const float* norm, float beta, float acc,
                    float wx, float wy, float wz, int B, int nx, int ny, int nz, float eps, hipStream_t stream);
 // dimg = vector-Jacobian product of (s1, s2, var) of tv_sqs with (g1, g2, gv);
-// gv may be null (zero); uses the stream's workspace (8 B per voxel)
+// gv may be null (zero); 8 B of scratch per voxel: 'scratch', or the stream's workspace when it is null
 void tv_sqs_backward(float* dimg, const float* img, const float* g1, const float* g2, const float* gv, float wx,
-                     float wy, float wz, int B, int nx, int ny, int nz, float eps, hipStream_t stream);
+                     float wy, float wz, int B, int nx, int ny, int nz, float eps, hipStream_t stream,
+                     float* scratch = nullptr);
 
 }  // namespace ctp

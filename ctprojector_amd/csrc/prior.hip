@@ -321,15 +321,19 @@ void tv_sqs_update(float* out, float* var, const float* img, const float* bp, co
 }
 
 void tv_sqs_backward(float* dimg, const float* img, const float* g1, const float* g2, const float* gv, float wx,
-                     float wy, float wz, int B, int nx, int ny, int nz, float eps, hipStream_t stream) {
+                     float wy, float wz, int B, int nx, int ny, int nz, float eps, hipStream_t stream,
+                     float* scratch) {
     const TVW w{wx, wy, wz, eps};
     const dim3 grid = tv_grid(B, nx, ny, nz);
     const size_t n = (size_t)B * nx * ny * nz;
-    Workspace& ws = workspace_for(stream);
-    Plan pl;
-    pl.add<float2>(n);
-    ws.reserve(pl.bytes);
-    float2* pr = ws.take_n<float2>(n);
+    float2* pr = reinterpret_cast<float2*>(scratch);   // 2 floats per voxel, 8-byte aligned
+    if (!pr) {
+        Workspace& ws = workspace_for(stream);
+        Plan pl;
+        pl.add<float2>(n);
+        ws.reserve(pl.bytes);
+        pr = ws.take_n<float2>(n);
+    }
     TimedLaunch t("tv_sqs_backward", stream);
     tv_grad_pr_kernel<<<grid, 64, 0, stream>>>(pr, img, g1, g2, gv, nx, ny, nz, w);
     CTP_LAUNCH_CHECK();

@@ -47,9 +47,10 @@ def tv_sqs_backward(img: torch.Tensor, g1: torch.Tensor, g2: torch.Tensor, weigh
     g2 = dev_f32(g2, dev).expand_as(img).contiguous()
     gv = dev_f32(gvar, dev).expand_as(img).contiguous() if gvar is not None else None
     out = torch.empty_like(img)
+    scratch = torch.empty((2,) + tuple(img.shape), dtype=torch.float32, device=dev)
     with torch.cuda.device(dev):
         err = _lib.module().ctpTVSQSBackward3DAsync(
-            ptr(out), ptr(img), ptr(g1), ptr(g2), ptr(gv) if gv is not None else None, c_float(weights[2]),
+            ptr(out), ptr(img), ptr(g1), ptr(g2), ptr(gv) if gv is not None else None, ptr(scratch), c_float(weights[2]),
             c_float(weights[1]), c_float(weights[0]), c_ulong(img.shape[0]), c_ulong(img.shape[3]),
             c_ulong(img.shape[2]), c_ulong(img.shape[1]), c_float(eps), _stream(dev))
     if err != 0:

@@ -139,8 +139,11 @@ int ctpDistanceDrivenParallelProjectionAdjoint(float* img, const float* prj, con
  * backprojection (as cupyDistanceDrivenFanBackprojection), Adjoint adds the
  * exact transpose (as ctpDistanceDrivenFanProjectionAdjoint); device arrays,
  * asynchronous on 'stream' (a hipStream_t; NULL = the null stream), no host
- * synchronisation.  The plan must be used on the device current at creation;
- * Destroy waits for that device. */
+ * synchronisation.  Each direction keeps its scratch in the plan, so the
+ * calls queue kernels only and can be captured in a HIP graph; calls of one
+ * direction on one plan must not overlap in time (issue them on one stream,
+ * or order them with events).  The plan must be used on the device current
+ * at creation; Destroy waits for that device. */
 int ctpDDPlanCreate(void** plan, int fan, const float* deg, size_t nBatches, size_t nx, size_t ny, size_t nz,
                     float dx, float dy, float dz, float cx, float cy, float cz, size_t nu, size_t nv, size_t nview,
                     float da, float dv, float off_a, float off_v, float dsd, float dso, int typeProjector);
@@ -231,13 +234,15 @@ int ctpTVSQSBackward3D(float* dimg, const float* img, const float* g1, const flo
  * on 'stream' (a hipStream_t, NULL for the null stream) without waiting for
  * completion, for callers that order their work by stream (the
  * differentiable SQS-TV network).  Arguments and results as above;
- * ctpTVSQSBackward3DAsync keeps scratch in the library's workspace of that
- * stream and host thread. */
+ * ctpTVSQSBackward3DAsync uses 'scratch' (2 floats per voxel, 8-byte
+ * aligned, distinct from the other arrays) or, when it is NULL, the
+ * library's workspace of that stream and host thread; with caller scratch the
+ * call queues kernels only and can be captured in a HIP graph. */
 int ctpTVSQS3DAsync(float* s1, float* s2, float* var, const float* img, float wx, float wy, float wz, size_t nBatches,
                     size_t nx, size_t ny, size_t nz, float eps, void* stream);
 int ctpTVSQSBackward3DAsync(float* dimg, const float* img, const float* g1, const float* g2, const float* gvar,
-                            float wx, float wy, float wz, size_t nBatches, size_t nx, size_t ny, size_t nz, float eps,
-                            void* stream);
+                            float* scratch, float wx, float wy, float wz, size_t nBatches, size_t nx, size_t ny,
+                            size_t nz, float eps, void* stream);
 
 /* ---- batched cone-beam plans (no counterpart in the reference) ------------ */
 /* Many Siddon cone-beam jobs (volumes of different sizes, each with its own

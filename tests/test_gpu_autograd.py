@@ -176,3 +176,51 @@ def test_unrolled_sqs_tv_gradient_vs_fp64_finite_differences(torch, oracle, chec
                          torch.tensor(norm_img, dtype=torch.float32, device=dev), pn, beta, niter, eps=eps)
     ((xr - torch.tensor(target, dtype=torch.float32, device=dev)) ** 2).sum().backward()
     assert p.grad is not None and bool(torch.isfinite(p.grad).all())
+
+
+def test_unrolled_training_step_in_a_hip_graph(torch):
+    """The unrolled SQS-TV network's forward and gradient captured in one HIP
+    graph (DD plans with their own scratch, TV calls with caller scratch)
+    equal the eager step bit for bit on every replay, with eager work and
+    synchronisation between the replays."""
+    from ctprojector_amd.autograd import FanDD, sqs_tv_unrolled
+    from ctprojector_amd.projector import cupy as C
+    g, ang, img = small_fan(n=32, nz=2, nv=2, nview=30, nu=48, du=2.0, seed=7, B=2)
+    dev = torch.device('cuda', 0)
+    op = FanDD(g.to(C.ct_projector()), torch.from_numpy(ang).to(dev))
+    x_true = torch.from_numpy(img).to(dev)
+    with torch.no_grad():
+        prj = op.fp(x_true)
+        norm_img = op.adjoint(op.fp(torch.ones_like(x_true))) / 100.0
+    x0 = torch.zeros_like(x_true)
+
+    def net_and_grad(q):
+        x = sqs_tv_unrolled(op, q, x0, norm_img, 10.0, 1e-3, 4, eps=1e-4, checkpoint=False)
+        x.backward(2.0 * (x - x_true))
+        return x
+
+    q_eager = prj.clone().requires_grad_(True)
+    x_eager = net_and_grad(q_eager)
+    q = prj.clone().requires_grad_(True)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        for _ in range(2):
+            q.grad = None
+            net_and_grad(q)
+    torch.cuda.current_stream().wait_stream(side)
+    q.grad = None
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        x_static = net_and_grad(q)
+    g_static = q.grad
+    for rep in range(3):
+        print('[graph] replay', rep, flush=True)
+        graph.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(x_static, x_eager), rep
+        assert torch.equal(g_static, q_eager.grad), rep
+        # eager work of the same operators between replays
+        with torch.no_grad():
+            _ = op.adjoint(op.fp(x_true) - prj)
+        torch.cuda.synchronize()

@@ -79,8 +79,10 @@ def test_sqs_tv_loop_decreases_loss(torch):
 
 
 def test_async_entry_points_equal_waiting_ones(torch):
-    """ctpTVSQS3DAsync / ctpTVSQSBackward3DAsync (queued on a stream, no wait)
-    give the bits of cupyTVSQS3D / ctpTVSQSBackward3D, on a side stream too."""
+    """ctpTVSQS3DAsync / ctpTVSQSBackward3DAsync (queued on a stream, no wait;
+    the backward with caller scratch through the torch wrapper and with the
+    library's workspace) give the bits of cupyTVSQS3D / ctpTVSQSBackward3D, on
+    a side stream too."""
     from ctypes import c_float, c_ulong, c_void_p
     from ctprojector_amd import _lib
     from ctprojector_amd.prior.cupy import tv_sqs_backward
@@ -107,3 +109,7 @@ def test_async_entry_points_equal_waiting_ones(torch):
         assert _lib.module().ctpTVSQSBackward3D(p(d_sync), p(x), p(g1), p(g2), p(gvar), *ws, *dims) == 0
         d_async = tv_sqs_backward(x, g1, g2, w, 1e-4, gvar=gvar)
         assert torch.equal(d_sync, d_async)
+        d_ws = torch.empty_like(x)
+        assert _lib.module().ctpTVSQSBackward3DAsync(p(d_ws), p(x), p(g1), p(g2), p(gvar), None, *ws, *dims,
+                                                     c_void_p(torch.cuda.current_stream().cuda_stream)) == 0
+        assert torch.equal(d_sync, d_ws)

@@ -13,9 +13,12 @@ times, which is what a plan is for.
     img = plan.bp(prj)           # = fan_equiangular.distance_driven_bp(...)
     img = plan.adjoint(prj)      # = fan_equiangular.distance_driven_fp_adjoint(...)
 
-Calls are asynchronous on torch's current stream of the plan's device.  The
-projector's attributes (nx..cz, nu, nv, du, dv, off_u, off_v, dsd, dso) are
-read once, at construction.
+Calls are asynchronous on torch's current stream of the plan's device.  Each
+direction keeps its scratch in the plan, so the calls can be captured in a
+HIP graph (torch.cuda.graph); calls of one direction on one plan must not
+overlap in time (use one stream, or order the streams).  The projector's
+attributes (nx..cz, nu, nv, du, dv, off_u, off_v, dsd, dso) are read once, at
+construction.
 """
 
 import ctypes

@@ -109,6 +109,19 @@ __device__ inline float group_bcast(float v, int t, int lead) {
     return __int_as_float(group_bcast<LPC>(__float_as_int(v), t, lead));
 }
 
+// Minimum (MAX false) or maximum over the lanes l of a wavefront with the same
+// l mod LPC: row rotations (DPP) within rows of 16 lanes, then two exchanges
+// across rows.
+template <int LPC, bool MAX>
+__device__ inline int class_reduce(int v) {
+    auto op = [](int a, int b) { return MAX ? max(a, b) : min(a, b); };
+    if constexpr (LPC == 2) v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x122, 0xF, 0xF, false));   // row_ror:2
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x124, 0xF, 0xF, false));                           // row_ror:4
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x128, 0xF, 0xF, false));                           // row_ror:8
+    v = op(v, __shfl_xor(v, 16));
+    return op(v, __shfl_xor(v, 32));
+}
+
 // Work distribution of the persistent kernels: a block takes its next block
 // index from a counter with an atomic add, so the blocks in flight stay a
 // compact window of the index space (as with a hardware-dispatched grid) even

@@ -832,19 +832,6 @@ __global__ __launch_bounds__(256) void fan_fbp_bp_split_kernel(float* __restrict
     }
 }
 
-// Minimum (MAX false) or maximum over the lanes l of a wavefront with the same
-// l mod LPC: row rotations (DPP) within rows of 16 lanes, then two exchanges
-// across rows.
-template <int LPC, bool MAX>
-__device__ inline int class_reduce(int v) {
-    auto op = [](int a, int b) { return MAX ? max(a, b) : min(a, b); };
-    if constexpr (LPC == 2) v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x122, 0xF, 0xF, false));   // row_ror:2
-    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x124, 0xF, 0xF, false));                           // row_ror:4
-    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x128, 0xF, 0xF, false));                           // row_ror:8
-    v = op(v, __shfl_xor(v, 16));
-    return op(v, __shfl_xor(v, 32));
-}
-
 // The split backprojection with each wavefront's detector cells staged in
 // LDS.  Per group of LPC views the wavefront's pixels reduce their cells to
 // one range [min u0, max u1] per view (a reduction over the lanes that hold

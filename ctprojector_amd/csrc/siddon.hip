@@ -1130,7 +1130,7 @@ __device__ inline void load_floats(float* out, __amdgpu_buffer_rsrc_t r, int vof
 // uniform across the block) is the window used.
 // NB volumes of one geometry: wt[b] is volume b's weight table of the view;
 // the chords are computed once and accumulated into acc[k * NB + b].
-template <int WU, int WV, int NZR, bool SLOW, int NB>
+template <int WU, int WV, int NZR, bool SLOW, int NB, bool AHEAD = true>
 __device__ inline void bp_view_col(float* acc, const ConeViewBP& cv, const float* const* wt,
                                    const float2* __restrict__ ut, const float2* __restrict__ vt, int nu, int nv, int ix,
                                    int iy, int iz0, float xc, float yc, int wu, int wv) {
@@ -1146,6 +1146,14 @@ __device__ inline void bp_view_col(float* acc, const ConeViewBP& cv, const float
     const float rd = __builtin_amdgcn_rcpf(fmaf(cv.pw[0], xc, fmaf(cv.pw[1], yc, cv.pw[3])));
     const float uc = fmaf(cv.pu[0], xc, fmaf(cv.pu[1], yc, cv.pu[3])) * rd;
     const float bv = fmaf(cv.pv[0], xc, fmaf(cv.pv[1], yc, cv.pv[3]));
+    // Candidate rows of voxel k start at v0 = floor(vc_k - hv + 1), clamped.
+    // Rays of a nonzero chord lie strictly within hv - 1e-3 of vc (the margin
+    // of the host's bound) and wv > 2 hv, so every origin between
+    // ceil(vc - hv) and floor(vc - hv + 1) holds all of them: vc - hv + 1 is
+    // evaluated as an affine function of k (a few ulp from the direct form),
+    // and an origin moved by that rounding only adds or drops rows of zero
+    // chord, which leave the sums unchanged.
+    const float y1 = fmaf(cv.pv[2], (float)iz0 + 0.5f, bv) * rd + (1.f - cv.hv), dy1 = cv.pv[2] * rd;
     // window origins clamped as floats (one median) before the conversion
     const int u0 = (int)__builtin_amdgcn_fmed3f(ceilf(uc - cv.hu), 0.f, (float)(nu - wu));
     const int u0w = 4 * WV * u0;   // byte offset of the window's first column in a table row
@@ -1169,13 +1177,10 @@ __device__ inline void bp_view_col(float* acc, const ConeViewBP& cv, const float
         lo[j] = fmaxf(fmaxf(x0, y0), 0.f);
         hi[j] = fmaxf(fminf(x1, y1), lo[j]);       // empty interval: hi = lo
     }
-#pragma unroll
-    for (int k = 0; k < NZR; k++) {
-        const float zc = (float)(iz0 + k) + 0.5f;
-        const float vc = fmaf(cv.pv[2], zc, bv) * rd;
-        const int v0 = (int)__builtin_amdgcn_fmed3f(ceilf(vc - cv.hv), 0.f, v0max);
-        const float ez = zc - cv.sz;
-        const bool inz = (float)(iz0 + k) == cv.fixz;
+    // voxel k's candidate rows (row table) and window weights
+    auto fetch = [&](int k, float* qv, float (*span)[WU * WV]) {
+        // origin >= 0 after the clamp, so the conversion's truncation is the floor
+        const int v0 = (int)__builtin_amdgcn_fmed3f(k == 0 ? y1 : fmaf((float)k, dy1, y1), 0.f, v0max);
         // The weights of the whole WU x WV window are contiguous in the window
         // layout (entry (iv, iu, jv) = weight of ray (iv + jv, iu)): one to
         // four loads; the row table takes one load per two rows.  Entries past
@@ -1184,8 +1189,6 @@ __device__ inline void bp_view_col(float* acc, const ConeViewBP& cv, const float
         // at most one row past nv - 1 (both tables are padded).
         // one 24-bit multiply-add (4 WV nu < 2^24: nu < 2^20 is checked on the host)
         const int wbase = (int)__umul24((unsigned)v0, (unsigned)(4 * WV * nu)) + u0w, vbase = 8 * v0;
-        float qv[2 * WV];
-        float span[NB][WU * WV];
 #pragma unroll
         for (int jv = 0; jv < WV; jv += 2) {
             const int r = min(jv, wv - 1);
@@ -1202,6 +1205,12 @@ __device__ inline void bp_view_col(float* acc, const ConeViewBP& cv, const float
                 else if (N - c == 2) load_floats<2>(span[b] + c, rw[b], wbase, 4 * c);
                 else load_floats<1>(span[b] + c, rw[b], wbase, 4 * c);
             }
+    };
+    // chords of voxel k's candidate rays, weighted, into its accumulators
+    auto apply = [&](int k, const float* qv, const float (*span)[WU * WV]) {
+        const float zc = (float)(iz0 + k) + 0.5f;
+        const float ez = zc - cv.sz;
+        const bool inz = (float)(iz0 + k) == cv.fixz;
         float2 qz[WV];
 #pragma unroll
         for (int jv = 0; jv < WV; jv++) qz[jv] = make_float2(qv[2 * jv], qv[2 * jv + 1]);
@@ -1225,6 +1234,37 @@ __device__ inline void bp_view_col(float* acc, const ConeViewBP& cv, const float
         }
 #pragma unroll
         for (int b = 0; b < NB; b++) acc[k * NB + b] += a[b];
+    };
+    // The loads do not depend on the chords, so (with AHEAD) they are issued
+    // ahead of them and their latencies overlap the arithmetic: the whole
+    // z-run's loads before the first chord when they take at most 32 floats
+    // per lane, else voxel k + 1's loads before voxel k's chords when two
+    // voxels' take at most 32, else voxel by voxel.
+    constexpr int per = 2 * WV + NB * WU * WV;   // floats loaded per voxel
+    if constexpr (AHEAD && NZR * per <= 32) {
+        float qv[NZR][2 * WV];
+        float span[NZR][NB][WU * WV];
+#pragma unroll
+        for (int k = 0; k < NZR; k++) fetch(k, qv[k], span[k]);
+#pragma unroll
+        for (int k = 0; k < NZR; k++) apply(k, qv[k], span[k]);
+    } else if constexpr (AHEAD && 2 * per <= 32) {
+        float qv[2][2 * WV];
+        float span[2][NB][WU * WV];
+        fetch(0, qv[0], span[0]);
+#pragma unroll
+        for (int k = 0; k < NZR; k++) {
+            if (k + 1 < NZR) fetch(k + 1, qv[(k + 1) & 1], span[(k + 1) & 1]);
+            apply(k, qv[k & 1], span[k & 1]);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < NZR; k++) {
+            float qv[2 * WV];
+            float span[NB][WU * WV];
+            fetch(k, qv, span);
+            apply(k, qv, span);
+        }
     }
 }
 
@@ -1235,7 +1275,8 @@ struct WTabs {
 
 // One block of the column-mode backprojector: 16x16 voxel columns x NZR slices,
 // img += A^T y over all views, for NB volumes (volume b at img + b * istride).
-template <int WU, int WV, int NZR, int NB>
+// AHEAD: loads issued ahead of the chords (bp_view_col).
+template <int WU, int WV, int NZR, int NB, bool AHEAD = true>
 __device__ inline void bp_col_block(float* __restrict__ img, long istride, const WTabs& wtab,
                                     const float2* __restrict__ utab, const float2* __restrict__ vtab,
                                     const ConeViewBP* __restrict__ views, int nview, const Grid& g, int nu, int nv,
@@ -1256,8 +1297,9 @@ __device__ inline void bp_col_block(float* __restrict__ img, long istride, const
         for (int b = 0; b < NB; b++) wt[b] = wtab.p[b] + view * nuv * WV;
         const float2* ut = utab + (size_t)view * nu;
         const float2* vt = vtab + (size_t)view * nv;
-        if (cv.flags & 1) bp_view_col<WU, WV, NZR, true, NB>(acc, cv, wt, ut, vt, nu, nv, ix, iy, iz0, xc, yc, wu, wv);
-        else bp_view_col<WU, WV, NZR, false, NB>(acc, cv, wt, ut, vt, nu, nv, ix, iy, iz0, xc, yc, wu, wv);
+        if (cv.flags & 1)
+            bp_view_col<WU, WV, NZR, true, NB, AHEAD>(acc, cv, wt, ut, vt, nu, nv, ix, iy, iz0, xc, yc, wu, wv);
+        else bp_view_col<WU, WV, NZR, false, NB, AHEAD>(acc, cv, wt, ut, vt, nu, nv, ix, iy, iz0, xc, yc, wu, wv);
     }
 #pragma unroll
     for (int k = 0; k < NZR; k++) {
@@ -1829,9 +1871,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NZR <= 4 ? 
         const long rest = local / J->nbx;
         const int by = (int)(rest % J->nby), bz = (int)(rest / J->nby);
         const WTabs wt{{J->wtab, nullptr, nullptr, nullptr}};
-        // the group's kernel is compiled for its exact window (WU, WV) = (wu, wv)
-        bp_col_block<WU, WV, NZR, 1>(J->img, 0, wt, J->utab, J->vtab, J->views, J->nview, J->g, J->d.nu, J->d.nv, bx,
-                                     by, bz);
+        // the group's kernel is compiled for its exact window (WU, WV) = (wu, wv);
+        // its loads stay voxel by voxel (issued ahead, the plan's backward
+        // measured 1% slower: 48.96 against 48.45 ms over four volumes)
+        bp_col_block<WU, WV, NZR, 1, false>(J->img, 0, wt, J->utab, J->vtab, J->views, J->nview, J->g, J->d.nu,
+                                            J->d.nv, bx, by, bz);
     }
 }
 

@@ -688,7 +688,7 @@ def short_name(kname):
 # (the instances the benchmark workloads launch)
 TIMED_KERNELS = {'dd_fan_fp': 'dd_fp_split_kernel<32, 4, true>', 'dd_fan_bp': 'dd_bp_split_kernel<32, 4, true>',
                  'siddon_cone_fp': 'siddon_cone_fp_col_kernel<1>',
-                 'siddon_cone_bp': 'siddon_cone_bp_col_kernel<2, 2, 4, 1>',
+                 'siddon_cone_bp': 'siddon_cone_bp_col_kernel<2, 2, 8, 1>',
                  'fan_fbp_bp': 'fan_fbp_bp_stage_kernel<32, 4, true>', 'ramp_filter': 'ramp_fft16_kernel<2048>'}
 
 

@@ -1335,7 +1335,10 @@ template <int WU, int WV>
 static void launch_bp_col(float* img, long istride, int nb, const WTabs& wtab, const float2* utab,
                           const float2* vtab, const ConeViewBP* views, int nview, const Grid& g, const Det& d,
                           hipStream_t stream) {
-    const int nzr = tuning_get("cone_bp_zrun", 4);
+    // default: 8 voxels per lane for one volume with a 2x2 window (12.15
+    // against 12.46 ms at 256^3 / 720 views with the loads issued ahead;
+    // 3x3: 13.49 against 13.48 ms), else 4
+    const int nzr = tuning_get("cone_bp_zrun", nb == 1 && WU == 2 && WV == 2 ? 8 : 4);
     if (nb == 4 && nzr == 4) return launch_bp_col_n<WU, WV, 4, 4>(img, istride, wtab, utab, vtab, views, nview, g, d, stream);
     if (nb == 2 && nzr == 4) return launch_bp_col_n<WU, WV, 4, 2>(img, istride, wtab, utab, vtab, views, nview, g, d, stream);
     if (nb != 1) fail("cone backprojection: several volumes per launch need cone_bp_zrun 4");

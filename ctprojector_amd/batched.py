@@ -4,8 +4,9 @@ launches per direction, with launch shapes chosen by the library's autotuner
 and the launches capturable in a HIP graph.
 
 * ``ConeBatch``: cone-beam Siddon jobs, each with its own trajectory and
-  detector; one persistent launch forward, one per backprojection window
-  shape (``ctpConePlan*`` of libprojector.so).
+  detector; one persistent launch forward and one backward (each block runs the
+  backprojector compiled for its job's candidate window; ``ctpConePlan*`` of
+  libprojector.so).
 * ``DDBatch``: fan-beam (equiangular) and parallel-beam distance-driven jobs,
   each with its own grid, detector, angles and batch count (``ctpDDBatch*``);
   the jobs whose kernels are the same template instance share a launch.

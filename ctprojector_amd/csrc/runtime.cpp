@@ -30,6 +30,7 @@ thread_local std::string g_last_error;
 std::mutex g_tune_mutex;
 std::map<std::string, int> g_tune = {
     {"cone_fp_tile_u", 0},   // detector columns per wavefront tile of the Siddon FP (8, 16, 32, 64)
+    {"cone_plan_bp_launch", 0},  // cone plan backward: 1 one persistent launch for every window shape (default), 2 one launch per shape
     {"cone_bp_zrun", 0},     // voxels along z per thread of the Siddon BP (1, 2, 4, 8; default 8 for one volume with a 2x2 window, else 4)
     {"cone_fp_order", 0},    // Siddon FP block order: 1 plain view-major, 2 XCD bands (default per form)
     {"cone_fp_views_per_block", 0},   // column-driven Siddon FP: views per block (1, 2, 4; default 2)

@@ -1131,7 +1131,7 @@ __device__ inline void load_floats(float* out, __amdgpu_buffer_rsrc_t r, int vof
 // NB volumes of one geometry: wt[b] is volume b's weight table of the view;
 // the chords are computed once and accumulated into acc[k * NB + b].
 template <int WU, int WV, int NZR, bool SLOW, int NB, bool AHEAD = true>
-__device__ inline void bp_view_col(float* acc, const ConeViewBP& cv, const float* const* wt,
+__device__ __forceinline__ void bp_view_col(float* acc, const ConeViewBP& cv, const float* const* wt,
                                    const float2* __restrict__ ut, const float2* __restrict__ vt, int nu, int nv, int ix,
                                    int iy, int iz0, float xc, float yc, int wu, int wv) {
     __amdgpu_buffer_rsrc_t rw[NB];
@@ -1277,7 +1277,7 @@ struct WTabs {
 // img += A^T y over all views, for NB volumes (volume b at img + b * istride).
 // AHEAD: loads issued ahead of the chords (bp_view_col).
 template <int WU, int WV, int NZR, int NB, bool AHEAD = true>
-__device__ inline void bp_col_block(float* __restrict__ img, long istride, const WTabs& wtab,
+__device__ __forceinline__ void bp_col_block(float* __restrict__ img, long istride, const WTabs& wtab,
                                     const float2* __restrict__ utab, const float2* __restrict__ vtab,
                                     const ConeViewBP* __restrict__ views, int nview, const Grid& g, int nu, int nv,
                                     int bx, int by, int bz, int wu = WU, int wv = WV) {
@@ -1875,10 +1875,47 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NZR <= 4 ? 
         const int by = (int)(rest % J->nby), bz = (int)(rest / J->nby);
         const WTabs wt{{J->wtab, nullptr, nullptr, nullptr}};
         // the group's kernel is compiled for its exact window (WU, WV) = (wu, wv);
-        // its loads stay voxel by voxel (issued ahead, the plan's backward
-        // measured 1% slower: 48.96 against 48.45 ms over four volumes)
+        // its loads stay voxel by voxel, as in siddon_cone_bp_plan_kernel
         bp_col_block<WU, WV, NZR, 1, false>(J->img, 0, wt, J->utab, J->vtab, J->views, J->nview, J->g, J->d.nu,
                                             J->d.nv, bx, by, bz);
+    }
+}
+
+// The whole backward of a plan in one persistent launch: blocks are numbered
+// over all jobs (job j's from jobs[j].bp_begin) and each runs the column-mode
+// backprojector compiled for its job's exact candidate window (a switch on
+// the job's window, uniform across the wavefront), so no job does the work of
+// a larger window; the registers are those of the largest window's code
+// (107 VGPRs at a z-run of 8, as the per-shape kernels of the 2x2 and 3x3
+// windows).  Measured over four volumes (128^3 ... 256^3, 720 views): 47.6
+// against 48.1 ms for one launch per shape at a z-run of 8, 49.3 against 50.9
+// at 4.  Loads stay voxel by voxel: issued ahead, the backward measured 1%
+// slower (48.96 against 48.45 ms).
+template <int NZR>
+__global__ __launch_bounds__(256) void siddon_cone_bp_plan_kernel(const ConeJobDev* __restrict__ jobs, int njobs,
+                                                                  long total, int* __restrict__ ctr, int persist) {
+    __shared__ int slot;
+    for (int b = first_block(ctr, 1, &slot, persist); b < total; b = following_block(ctr, 1, &slot, persist)) {
+        int j = 0;
+        while (j + 1 < njobs && b >= jobs[j + 1].bp_begin) j++;
+        const ConeJobDev* J = jobs + j;
+        const long local = b - J->bp_begin;
+        const int bx = (int)(local % J->nbx);
+        const long rest = local / J->nbx;
+        const int by = (int)(rest % J->nby), bz = (int)(rest / J->nby);
+        const WTabs wt{{J->wtab, nullptr, nullptr, nullptr}};
+        switch (J->wu * 8 + J->wv) {
+#define CTP_BPP(A, Bv)                                                                                          \
+    case A * 8 + Bv:                                                                                            \
+        bp_col_block<A, Bv, NZR, 1, false>(J->img, 0, wt, J->utab, J->vtab, J->views, J->nview, J->g, J->d.nu, \
+                                           J->d.nv, bx, by, bz);                                                \
+        break;
+            CTP_BPP(1, 1) CTP_BPP(1, 2) CTP_BPP(2, 1) CTP_BPP(2, 2) CTP_BPP(1, 3) CTP_BPP(3, 1) CTP_BPP(2, 3)
+            CTP_BPP(3, 2) CTP_BPP(3, 3) CTP_BPP(1, 4) CTP_BPP(4, 1) CTP_BPP(2, 4) CTP_BPP(4, 2) CTP_BPP(3, 4)
+            CTP_BPP(4, 3) CTP_BPP(4, 4)
+#undef CTP_BPP
+            default: break;
+        }
     }
 }
 
@@ -1911,8 +1948,9 @@ struct ConePlan::Impl {
     std::vector<int*> exc;
     std::vector<bool> need_t;
     ConeJobDev* djobs = nullptr;
-    // backprojection: one persistent launch per window shape, each with the
-    // exact-size kernel of its jobs (blocks numbered within the group)
+    // backprojection by window shape (the per-shape launches of
+    // cone_plan_bp_launch 2, blocks numbered within the group); the default
+    // single launch numbers blocks over all jobs (ConeJobDev::bp_begin of djobs)
     struct BPGroup {
         int wu, wv;
         ConeJobDev* djobs;
@@ -1922,7 +1960,7 @@ struct ConePlan::Impl {
     std::vector<BPGroup> groups;
     int* ctr = nullptr;           // work counters of the persistent launches (8 forward, one per backward group)
     int tu = 0, order = 1, nzr = 4;   // tu 0: column-driven forward form
-    long fp_total = 0, bp_total = 0;
+    long fp_total = 0, bp_total = 0;   // bp_total: blocks of the backward over all jobs
 
     template <typename T>
     T* alloc(size_t n) {
@@ -1944,6 +1982,7 @@ void ConePlan::Impl::layout() {
     // by one grid) must fit in 31 bits
     const long limit = 0x7fffffffL - (1L << 24);
     fp_total = bp_total = 0;
+    long bp_all = 0;
     for (auto& J : jobs) {
         if (tu == 0) {                      // groups of 4 detector columns
             J.nub = 0;
@@ -1957,10 +1996,11 @@ void ConePlan::Impl::layout() {
         J.nby = ceil_div(J.g.ny, 16);
         J.nbz = ceil_div(J.g.nz, nzr);
         J.fp_begin = fp_total;
-        J.bp_begin = 0;
+        J.bp_begin = bp_all;
         fp_total += (long)J.nview * 8 * ((J.ntile + 7) / 8);
+        bp_all += (long)J.nbx * J.nby * J.nbz;
     }
-    if (fp_total > limit) fail("cone plan: too many blocks for one batch; split the jobs");
+    if (fp_total > limit || bp_all > limit) fail("cone plan: too many blocks for one batch; split the jobs");
     CTP_CHECK(hipMemcpy(djobs, jobs.data(), sizeof(ConeJobDev) * jobs.size(), hipMemcpyHostToDevice));
     bp_total = 0;
     for (BPGroup& G : groups) {
@@ -2030,6 +2070,18 @@ void ConePlan::Impl::backward(hipStream_t s) {
     reset_counters_kernel<<<1, 64, 0, s>>>(ctr + 8, (int)groups.size());
     CTP_LAUNCH_CHECK();
     TimedLaunch t("siddon_cone_bp_batch", s);
+    if (tuning_get("cone_plan_bp_launch", 1) == 1) {
+        const int nj = (int)jobs.size();
+#define CTP_BPP_LAUNCH(Z)                                                                               \
+    if (nzr == Z) {                                                                                     \
+        auto k = siddon_cone_bp_plan_kernel<Z>;                                                         \
+        k<<<plan_grid(k, bp_total), 256, 0, s>>>(djobs, nj, bp_total, ctr + 8, plan_persist());         \
+    }
+        CTP_BPP_LAUNCH(2) CTP_BPP_LAUNCH(4) CTP_BPP_LAUNCH(8)
+#undef CTP_BPP_LAUNCH
+        CTP_LAUNCH_CHECK();
+        return;
+    }
     for (size_t gi = 0; gi < groups.size(); gi++) {
         const BPGroup& G = groups[gi];
         int* gc = ctr + 8 + gi;

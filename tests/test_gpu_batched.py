@@ -6,7 +6,8 @@
 * near-axis-parallel rays go through the plan's exact ray path;
 * the autotuner picks a valid configuration and the results do not change;
 * persistent blocks (work counters) and one hardware-dispatched block per tile
-  give the same results;
+  give the same results, and so do the backward's single launch over every
+  job and its launches per window shape;
 * HIP-graph capture: forward and backward graphs replayed several times on new
   inputs, with eager work in between, each replay checked against eager runs.
 """
@@ -111,6 +112,30 @@ def test_persistent_and_dispatched_plan_launches_agree(torch):
     for a, b in zip(out[0][0], out[1][0]):         # forward: written once per ray
         assert torch.equal(a, b)
     for a, b in zip(out[0][1], out[1][1]):         # backward: exception rays are scattered with atomics
+        assert rel_l2(a.cpu().numpy(), b.cpu().numpy()) < 1e-6
+
+
+def test_single_and_per_window_backward_launches_agree(torch):
+    """The plan's backward as one launch over every job (each block running
+    its job's window) and as one launch per window shape (the third job's
+    2 mm voxels have a larger candidate window) give the same images."""
+    from ctprojector_amd import _lib
+    cases = _jobs()
+    out = []
+    try:
+        for mode in (1, 2):
+            _lib.set_tuning('cone_plan_bp_launch', mode)
+            bt = _batch(torch, cases)
+            for j, (g, geo, img) in enumerate(cases):
+                bt.img[j].copy_(torch.from_numpy(img[0]))
+            bt.forward()
+            bt.backward()
+            torch.cuda.synchronize()
+            out.append([i.clone() for i in bt.img])
+            bt.close()
+    finally:
+        _lib.set_tuning('cone_plan_bp_launch', 0)
+    for a, b in zip(out[0], out[1]):                 # exception rays are scattered with atomics
         assert rel_l2(a.cpu().numpy(), b.cpu().numpy()) < 1e-6
 
 

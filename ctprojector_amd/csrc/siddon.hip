@@ -1890,7 +1890,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NZR <= 4 ? 
 // windows).  Measured over four volumes (128^3 ... 256^3, 720 views): 47.6
 // against 48.1 ms for one launch per shape at a z-run of 8, 49.3 against 50.9
 // at 4.  Loads stay voxel by voxel: issued ahead, the backward measured 1%
-// slower (48.96 against 48.45 ms).
+// slower (48.96 against 48.45 ms with one launch per shape; 47.65 against
+// 47.45 ms here with the windows up to 3x3 issuing them ahead).
 template <int NZR>
 __global__ __launch_bounds__(256) void siddon_cone_bp_plan_kernel(const ConeJobDev* __restrict__ jobs, int njobs,
                                                                   long total, int* __restrict__ ctr, int persist) {

@@ -33,6 +33,7 @@ SYMBOLS = [
     'ctpDDPlanCreate', 'ctpDDPlanForward', 'ctpDDPlanBackward', 'ctpDDPlanAdjoint', 'ctpDDPlanDestroy',
     'ctpConePlanCreate', 'ctpConePlanForward', 'ctpConePlanBackward', 'ctpConePlanConfig', 'ctpConePlanDestroy',
     'ctpDDBatchCreate', 'ctpDDBatchForward', 'ctpDDBatchBackward', 'ctpDDBatchConfig', 'ctpDDBatchDestroy',
+    'ctpFilterBatchCreate', 'ctpFilterBatchRun', 'ctpFilterBatchDestroy',
     'ctpGetLastErrorMessage', 'ctpReleaseWorkspace', 'ctpSetStream',
     'ctpSetKernelTiming', 'ctpResetKernelTimes', 'ctpGetKernelTime', 'ctpKernelTimeReport',
     'ctpSetTuning', 'ctpGetTuning',

@@ -10,6 +10,9 @@ and the launches capturable in a HIP graph.
 * ``DDBatch``: fan-beam (equiangular) and parallel-beam distance-driven jobs,
   each with its own grid, detector, angles and batch count (``ctpDDBatch*``);
   the jobs whose kernels are the same template instance share a launch.
+* ``RampFilterBatch``: the ramp filters of uneven fan / parallel jobs
+  (``ctpFilterBatch*``), one launch per transform size; ``FBPBatch`` chains it
+  into a DDBatch with FBP weights: filtered backprojection of uneven jobs.
 
 The reference projects a batch by looping over its items on the host with a
 launch and a device synchronisation per item (cuda/siddonCone.cu:76-135); its
@@ -260,3 +263,124 @@ class DDBatch:
             self.close()
         except Exception:
             pass
+
+
+class _FJob(ctypes.Structure):
+    _fields_ = [('fan', c_int), ('typeFilter', c_int),
+                ('nBatches', c_size_t), ('nu', c_size_t), ('nv', c_size_t), ('nview', c_size_t),
+                ('da', c_float), ('off_a', c_float), ('dsd', c_float), ('dso', c_float),
+                ('prj', c_void_p), ('fprj', c_void_p)]
+
+
+@dataclass
+class FilterJob:
+    """One ramp-filter job: ``projector`` carries nu, nv, du, off_u and, for
+    the fan beam, dsd and dso (as for fan_equiangular / parallel
+    ramp_filter); ``nview`` views; ``batch`` items share the geometry;
+    ``filter_type`` as for ramp_filter ('hann', 'hamming', 'cosine', else
+    Ram-Lak)."""
+    projector: object
+    nview: int
+    fan: bool = True
+    batch: int = 1
+    filter_type: str = 'hann'
+
+
+class RampFilterBatch:
+    """Ramp filters of uneven jobs (``ctpFilterBatch*``): ``fprj[j]`` = the
+    job's fan_equiangular / parallel ramp_filter of ``prj[j]`` (both
+    [batch, nview, nv, nu] device tensors bound to the batch; ``outputs``
+    may supply the fprj tensors), in one launch per transform size.  Detector
+    rows of up to 2048 columns.  ``run`` is asynchronous on torch's current
+    stream and can be captured into a HIP graph."""
+
+    def __init__(self, jobs: Sequence[FilterJob], device=None, outputs: Sequence[torch.Tensor] = None):
+        from .projector.geometry import filter_code
+        self.device = torch.device('cuda', torch.cuda.current_device()) if device is None else torch.device(device)
+        self.prj: List[torch.Tensor] = []
+        self.fprj: List[torch.Tensor] = []
+        cj = (_FJob * len(jobs))()
+        for k, job in enumerate(jobs):
+            p = job.projector
+            shape = [int(job.batch), int(job.nview), int(p.nv), int(p.nu)]
+            prj = torch.zeros(shape, dtype=torch.float32, device=self.device)
+            if outputs is not None:
+                fprj = outputs[k]
+                if list(fprj.shape) != shape or fprj.dtype != torch.float32 or not fprj.is_contiguous() \
+                        or fprj.device != self.device:
+                    raise ValueError('filter output %d: a contiguous float32 %s tensor on %s is needed'
+                                     % (k, shape, self.device))
+            else:
+                fprj = torch.zeros(shape, dtype=torch.float32, device=self.device)
+            self.prj.append(prj)
+            self.fprj.append(fprj)
+            fan = bool(job.fan)
+            cj[k] = _FJob(1 if fan else 0, filter_code(job.filter_type), shape[0], shape[3], shape[2], shape[1],
+                          p.du / p.dsd if fan else p.du, p.off_u, p.dsd if fan else 0.0, p.dso if fan else 0.0,
+                          prj.data_ptr(), fprj.data_ptr())
+        self._plan = c_void_p()
+        torch.cuda.synchronize(self.device)
+        with torch.cuda.device(self.device):
+            err = _lib.module().ctpFilterBatchCreate(ctypes.byref(self._plan), cj, c_int(len(jobs)))
+        if err != 0:
+            raise RuntimeError('ctpFilterBatchCreate failed (%d): %s' % (err, _lib.last_error_message()))
+
+    def run(self) -> None:
+        """fprj[j] = the ramp filter of prj[j] for every job (asynchronous)."""
+        with torch.cuda.device(self.device):
+            err = _lib.module().ctpFilterBatchRun(self._plan,
+                                                  c_void_p(torch.cuda.current_stream(self.device).cuda_stream))
+        if err != 0:
+            raise RuntimeError('ctpFilterBatchRun failed (%d): %s' % (err, _lib.last_error_message()))
+
+    def close(self) -> None:
+        if getattr(self, '_plan', None) is not None and self._plan.value:
+            torch.cuda.synchronize(self.device)
+            _lib.module().ctpFilterBatchDestroy(self._plan)
+            self._plan = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class FBPBatch:
+    """Filtered backprojection of uneven fan / parallel jobs: ``prj[j]`` in,
+    ``img[j]`` out ([batch, nz, ny, nx]).  ``run`` filters every job
+    (RampFilterBatch, one launch per transform size) into the sinograms of a
+    DDBatch whose jobs carry the FBP weights (typeProjector bit 0, as
+    distance_driven_bp(..., is_fbp=True)) and backprojects them (one launch
+    per kernel class): each job's image equals ramp_filter followed by
+    distance_driven_bp(is_fbp=True) on its own.  ``jobs`` are DDJob records
+    (their is_fbp flag is set here); ``filter_type`` applies to every job."""
+
+    def __init__(self, jobs: Sequence[DDJob], filter_type: str = 'hann', autotune: bool = False, device=None):
+        jobs = [DDJob(j.projector, j.angles, fan=j.fan, batch=j.batch, is_fbp=True, branchless=j.branchless,
+                      force_fbp=j.force_fbp) for j in jobs]
+        self.dd = DDBatch(jobs, autotune=autotune, device=device)
+        self.device = self.dd.device
+        fjobs = [FilterJob(j.projector, int(t.shape[1]), fan=j.fan, batch=j.batch, filter_type=filter_type)
+                 for j, t in zip(jobs, self.dd.prj)]
+        self.filter = RampFilterBatch(fjobs, device=self.device, outputs=self.dd.prj)
+        self.prj = self.filter.prj
+        self.img = self.dd.img
+
+    def run(self) -> None:
+        """img[j] = FBP of prj[j] for every job (asynchronous on the current stream)."""
+        self.filter.run()
+        self.dd.backward()
+
+    def capture(self) -> torch.cuda.CUDAGraph:
+        """A HIP graph of ``run``."""
+        self.run()
+        torch.cuda.synchronize(self.device)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.run()
+        return g
+
+    def close(self) -> None:
+        self.filter.close()
+        self.dd.close()

@@ -956,6 +956,67 @@ CTP_EXPORT int ctpDDBatchConfig(void* batch, int* fp_lanes, int* bp_lanes, int* 
     });
 }
 
+// ctpFilterJob of include/ctprojector.h
+struct ctpFilterJob {
+    int fan;
+    int typeFilter;
+    size_t nBatches, nu, nv, nview;
+    float da, off_a, dsd, dso;
+    const float* prj;
+    float* fprj;
+};
+
+CTP_EXPORT int ctpFilterBatchCreate(void** batch, const ctpFilterJob* jobs, int njobs) {
+    return guarded("ctpFilterBatchCreate", [&] {
+        need_ptr(batch, "batch");
+        *batch = nullptr;
+        need(njobs > 0 && jobs != nullptr, "no jobs");
+        std::vector<FilterBatchJobSpec> specs;
+        for (int j = 0; j < njobs; j++) {
+            const ctpFilterJob& J = jobs[j];
+            need(!empty(J.nBatches, J.nu, J.nv, J.nview, 1, 1, 1), "empty job");
+            need(J.nu <= 2048, "filter batch: detector rows of more than 2048 columns (use the single-call filter)");
+            need(J.nBatches <= kIntMax && J.nv <= kIntMax && J.nview <= kIntMax, "job too large");
+            need(J.da != 0, "zero detector pitch");
+            need(!J.fan || J.dso != 0, "zero dso");
+            need_ptr(J.prj, "prj");
+            need_ptr(J.fprj, "fprj");
+            FilterBatchJobSpec S{};
+            S.B = (int)J.nBatches;
+            S.nu = (int)J.nu;
+            S.nv = (int)J.nv;
+            S.nview = (int)J.nview;
+            S.da = J.da;
+            S.off_u = J.off_a;
+            S.dso = J.fan ? J.dso : 1.f;
+            S.filter = J.typeFilter;
+            S.equispace = J.fan == 0;
+            S.in = J.prj;
+            S.out = J.fprj;
+            specs.push_back(S);
+        }
+        touch();
+        *batch = new FilterBatchPlan(specs);
+    });
+}
+
+CTP_EXPORT int ctpFilterBatchRun(void* batch, void* stream) {
+    return guarded("ctpFilterBatchRun", [&] {
+        need_ptr(batch, "batch");
+        touch();
+        static_cast<FilterBatchPlan*>(batch)->run(static_cast<hipStream_t>(stream));
+    });
+}
+
+CTP_EXPORT int ctpFilterBatchDestroy(void* batch) {
+    return guarded("ctpFilterBatchDestroy", [&] {
+        if (!batch) return;
+        touch();
+        CTP_CHECK(hipDeviceSynchronize());
+        delete static_cast<FilterBatchPlan*>(batch);
+    });
+}
+
 CTP_EXPORT int ctpDDBatchDestroy(void* batch) {
     return guarded("ctpDDBatchDestroy", [&] {
         if (!batch) return;

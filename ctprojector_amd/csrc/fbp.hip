@@ -250,12 +250,14 @@ __device__ inline void fft_lds(float2* buf, int N, int logN, const float2* __res
     else if (logN % 3 == 1) fft_pass<2>(buf, N, Ns, tw);
 }
 
-__global__ __launch_bounds__(256) void ramp_fft_kernel(float* __restrict__ out, const float* __restrict__ in,
-                                                       const float* __restrict__ G, const float2* __restrict__ tw,
-                                                       const float* __restrict__ pw, long nrows, int nu, int N,
-                                                       int logN) {
+// Rows 2 pair and 2 pair + 1 (< nrows; the first always is) of one block of
+// 256 lanes, N complex values of dynamic LDS.
+__device__ __forceinline__ void ramp_fft_rows(float* __restrict__ out, const float* __restrict__ in,
+                                              const float* __restrict__ G, const float2* __restrict__ tw,
+                                              const float* __restrict__ pw, long nrows, int nu, int N, int logN,
+                                              long pair) {
     extern __shared__ float2 zbuf[];
-    const long ra = 2L * blockIdx.x, rb = ra + 1;
+    const long ra = 2L * pair, rb = ra + 1;
     const bool hb = rb < nrows;
     const float* xa = in + ra * nu;
     const float* xb = in + rb * nu;
@@ -284,6 +286,13 @@ __global__ __launch_bounds__(256) void ramp_fft_kernel(float* __restrict__ out, 
         ya[j] = f.x;
         if (hb) yb[j] = -f.y;
     }
+}
+
+__global__ __launch_bounds__(256) void ramp_fft_kernel(float* __restrict__ out, const float* __restrict__ in,
+                                                       const float* __restrict__ G, const float2* __restrict__ tw,
+                                                       const float* __restrict__ pw, long nrows, int nu, int N,
+                                                       int logN) {
+    ramp_fft_rows(out, in, G, tw, pw, nrows, nu, N, logN, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -350,10 +359,12 @@ __device__ inline void dft_any(float2* v) {
 template <int N>
 constexpr int fft16_block() { return N / 16 < 128 ? 128 : N / 16; }
 
+// Rows 2 pair and 2 pair + 1 (those < nrows) for the complex row of lanes
+// threadIdx.x / (N / 16); every lane of the block calls it (barriers).
 template <int N>
-__global__ __launch_bounds__(fft16_block<N>()) __attribute__((amdgpu_waves_per_eu(N <= 2048 ? 5 : 1)))
-void ramp_fft16_kernel(float* __restrict__ out, const float* __restrict__ in, const float* __restrict__ G,
-                       const float* __restrict__ pw, long nrows, int nu) {
+__device__ __forceinline__ void ramp_fft16_rows(float* __restrict__ out, const float* __restrict__ in,
+                                                const float* __restrict__ G, const float* __restrict__ pw, long nrows,
+                                                int nu, long pair) {
     constexpr int T = N / 16;            // lanes per complex row
     constexpr int RPB = fft16_block<N>() / T;   // complex rows per block
     constexpr int R3 = N / 256;          // radix of the third pass
@@ -366,7 +377,7 @@ void ramp_fft16_kernel(float* __restrict__ out, const float* __restrict__ in, co
     // addressed from scalar registers
     const int t = threadIdx.x % T, c = __builtin_amdgcn_readfirstlane(threadIdx.x / T);
     float2* buf = lds + c * NP;
-    const long ra = 2L * ((long)blockIdx.x * RPB + c), rb = ra + 1;
+    const long ra = 2L * pair, rb = ra + 1;
     const bool ha = ra < nrows, hb = rb < nrows;
     auto P = [](int i) { return i + (i >> 4); };
     float2 v[16];
@@ -461,6 +472,55 @@ void ramp_fft16_kernel(float* __restrict__ out, const float* __restrict__ in, co
     }
 }
 
+template <int N>
+__global__ __launch_bounds__(fft16_block<N>()) __attribute__((amdgpu_waves_per_eu(N <= 2048 ? 5 : 1)))
+void ramp_fft16_kernel(float* __restrict__ out, const float* __restrict__ in, const float* __restrict__ G,
+                       const float* __restrict__ pw, long nrows, int nu) {
+    constexpr int RPB = fft16_block<N>() / (N / 16);
+    const int c = __builtin_amdgcn_readfirstlane(threadIdx.x / (N / 16));
+    ramp_fft16_rows<N>(out, in, G, pw, nrows, nu, (long)blockIdx.x * RPB + c);
+}
+
+// ---------------------------------------------------------------------------
+// Batches of uneven filter jobs (no counterpart in the reference): the row
+// pairs of the jobs with one transform size N are numbered in a row (job j's
+// from pair_begin), and each complex row of a block (a wavefront or more)
+// looks up its job and filters one pair with that job's tables.
+// ---------------------------------------------------------------------------
+struct RampJobDev {
+    float* out;
+    const float* in;
+    const float* G;
+    const float2* tw;
+    const float* pw;
+    long nrows, pair_begin;
+    int nu;
+};
+
+__device__ inline const RampJobDev* ramp_job_of(const RampJobDev* __restrict__ jobs, int njobs, long p) {
+    int j = 0;
+    while (j + 1 < njobs && p >= jobs[j + 1].pair_begin) j++;
+    return jobs + j;
+}
+
+template <int N>
+__global__ __launch_bounds__(fft16_block<N>()) __attribute__((amdgpu_waves_per_eu(N <= 2048 ? 5 : 1)))
+void ramp_fft16_batch_kernel(const RampJobDev* __restrict__ jobs, int njobs, long npairs) {
+    constexpr int RPB = fft16_block<N>() / (N / 16);
+    const int c = __builtin_amdgcn_readfirstlane(threadIdx.x / (N / 16));
+    const long p = (long)blockIdx.x * RPB + c;
+    const RampJobDev* J = ramp_job_of(jobs, njobs, p);
+    // a complex row past the last pair filters nothing (its lanes still meet the barriers)
+    ramp_fft16_rows<N>(J->out, J->in, J->G, J->pw, p < npairs ? J->nrows : 0, J->nu, p - J->pair_begin);
+}
+
+__global__ __launch_bounds__(256) void ramp_fft_batch_kernel(const RampJobDev* __restrict__ jobs, int njobs, int N,
+                                                             int logN) {
+    const long p = blockIdx.x;
+    const RampJobDev* J = ramp_job_of(jobs, njobs, p);
+    ramp_fft_rows(J->out, J->in, J->G, J->tw, J->pw, J->nrows, J->nu, N, logN, p - J->pair_begin);
+}
+
 // G = FFT(g) (real) times scale / N, and the twiddle table W_N^t, t < N, in FP64
 struct RampFFTTables {
     int N = 0, logN = 0;
@@ -499,20 +559,117 @@ static RampFFTTables ramp_fft_tables(int nu, float da, int filter, bool equispac
     return t;
 }
 
+// Filter pre-weight per detector column: cos(gamma) for the fan beam,
+// cuda/fbp.cu:60-66 (not mirrored), 1 for the parallel beam.
+static std::vector<float> ramp_preweight(int nu, float da, float off_u, bool equispace) {
+    std::vector<float> pw(nu);
+    for (int i = 0; i < nu; i++) {
+        const double ang = ((i - (nu - 1) / 2.0) - off_u) * (double)da;
+        pw[i] = equispace ? 1.f : (float)std::cos(ang);
+    }
+    return pw;
+}
+
+// cuda/fbp.cu:97-104, cuda/fbpParallel.cu:59; the 1/L cancels cuFFT's
+// unnormalised inverse
+static float ramp_scale(int nview, float da, float dso, bool equispace) {
+    return equispace ? (float)((double)kPI / nview * da) : (float)((double)kPI / nview * da / dso);
+}
+
+struct FilterBatchPlan::Impl {
+    struct Group {
+        int N, logN;
+        RampJobDev* djobs;
+        int njobs;
+        long npairs;
+    };
+    std::vector<Group> groups;
+    std::vector<void*> mem;
+    template <typename T>
+    T* alloc(size_t n) {
+        void* p = nullptr;
+        CTP_CHECK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+        mem.push_back(p);
+        return static_cast<T*>(p);
+    }
+    ~Impl() {
+        for (void* p : mem) (void)hipFree(p);
+    }
+};
+
+// Tables of every job (the single-call filter's, see ramp_filter) uploaded
+// once; jobs grouped by transform size N, each group one launch of the
+// single-call kernel for that N (three-pass from N = 1024, else the generic
+// one), so each job's output equals the single call's.
+FilterBatchPlan::FilterBatchPlan(const std::vector<FilterBatchJobSpec>& jobs) : impl_(new Impl) {
+    Impl& P = *impl_;
+    std::map<int, std::vector<RampJobDev>> byN;
+    for (const FilterBatchJobSpec& S : jobs) {
+        if (S.nu < 1 || S.nu > 2048)
+            fail("filter batch: detector rows of 1 to 2048 columns (transforms up to 4096); use the single-call filter");
+        const RampFFTTables t = ramp_fft_tables(S.nu, S.da, S.filter, S.equispace,
+                                                ramp_scale(S.nview, S.da, S.dso, S.equispace));
+        const std::vector<float> pw = ramp_preweight(S.nu, S.da, S.off_u, S.equispace);
+        float* dG = P.alloc<float>(t.N);
+        float2* dtw = P.alloc<float2>(t.N);
+        float* dpw = P.alloc<float>(S.nu);
+        CTP_CHECK(hipMemcpy(dG, t.G.data(), sizeof(float) * t.N, hipMemcpyHostToDevice));
+        CTP_CHECK(hipMemcpy(dtw, t.tw.data(), sizeof(float2) * t.N, hipMemcpyHostToDevice));
+        CTP_CHECK(hipMemcpy(dpw, pw.data(), sizeof(float) * S.nu, hipMemcpyHostToDevice));
+        RampJobDev J{};
+        J.out = S.out;
+        J.in = S.in;
+        J.G = dG;
+        J.tw = dtw;
+        J.pw = dpw;
+        J.nrows = (long)S.B * S.nview * S.nv;
+        J.nu = S.nu;
+        byN[t.N].push_back(J);
+    }
+    for (auto& kv : byN) {
+        Impl::Group G{kv.first, 0, nullptr, (int)kv.second.size(), 0};
+        while ((1 << G.logN) < G.N) G.logN++;
+        for (RampJobDev& J : kv.second) {
+            J.pair_begin = G.npairs;
+            G.npairs += ceil_div(J.nrows, 2L);
+        }
+        if (G.npairs > 0x7fffffffL) fail("filter batch: too many detector rows for one batch; split the jobs");
+        G.djobs = P.alloc<RampJobDev>(kv.second.size());
+        CTP_CHECK(hipMemcpy(G.djobs, kv.second.data(), sizeof(RampJobDev) * kv.second.size(), hipMemcpyHostToDevice));
+        P.groups.push_back(G);
+    }
+}
+
+FilterBatchPlan::~FilterBatchPlan() = default;
+
+int FilterBatchPlan::launches() const { return (int)impl_->groups.size(); }
+
+void FilterBatchPlan::run(hipStream_t s) {
+    TimedLaunch tl("ramp_filter_batch", s);
+    for (const Impl::Group& G : impl_->groups) {
+        if (G.N >= 1024) {
+            auto launch = [&](auto kernel, int block) {
+                const int rpb = block / (G.N / 16);
+                kernel<<<(unsigned)ceil_div(G.npairs, (long)rpb), block, 0, s>>>(G.djobs, G.njobs, G.npairs);
+            };
+            if (G.N == 1024) launch(ramp_fft16_batch_kernel<1024>, fft16_block<1024>());
+            else if (G.N == 2048) launch(ramp_fft16_batch_kernel<2048>, fft16_block<2048>());
+            else launch(ramp_fft16_batch_kernel<4096>, fft16_block<4096>());
+        } else {
+            ramp_fft_batch_kernel<<<(unsigned)G.npairs, 256, sizeof(float2) * G.N, s>>>(G.djobs, G.njobs, G.N,
+                                                                                       G.logN);
+        }
+        CTP_LAUNCH_CHECK();
+    }
+}
+
 void ramp_filter(float* out, const float* prj, int B, int nu, int nv, int nview, float da, float off_u, float dsd,
                  float dso, int filter, bool equispace, hipStream_t stream) {
     (void)dsd;
     if (nu <= 0) return;
     if (nu > 5000) fail("ramp filter: nu > 5000 is not supported");
-    std::vector<float> pw(nu);
-    for (int i = 0; i < nu; i++) {
-        // fan pre-weight cos(gamma), cuda/fbp.cu:60-66 (not mirrored)
-        const double ang = ((i - (nu - 1) / 2.0) - off_u) * (double)da;
-        pw[i] = equispace ? 1.f : (float)std::cos(ang);
-    }
-    // cuda/fbp.cu:97-104, cuda/fbpParallel.cu:59; the 1/L cancels cuFFT's
-    // unnormalised inverse
-    const float scale = equispace ? (float)((double)kPI / nview * da) : (float)((double)kPI / nview * da / dso);
+    const std::vector<float> pw = ramp_preweight(nu, da, off_u, equispace);
+    const float scale = ramp_scale(nview, da, dso, equispace);
     const long nrows = (long)B * nview * nv;
     // ramp_mode: 0 = FFT convolution where N = 2^p >= 2nu - 1 is at most 4096
     // (three-pass kernel for N >= 1024), 1 = direct, 2 = the generic FFT kernel

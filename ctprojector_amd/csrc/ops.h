@@ -104,6 +104,27 @@ private:
     std::unique_ptr<Impl> impl_;
 };
 
+// ---- batched ramp filters of uneven jobs (no counterpart in the reference) ----
+struct FilterBatchJobSpec {
+    int B, nu, nv, nview;
+    float da, off_u, dso;     // da: angular pitch (fan) or du (parallel)
+    int filter;               // typeFilter of the single-call filter
+    bool equispace;           // parallel beam
+    const float* in;          // device [B][nview][nv][nu], bound
+    float* out;               // device [B][nview][nv][nu], bound
+};
+
+class FilterBatchPlan {
+public:
+    explicit FilterBatchPlan(const std::vector<FilterBatchJobSpec>& jobs);
+    ~FilterBatchPlan();
+    void run(hipStream_t s);   // out = the single-call ramp filter of in, every job
+    int launches() const;      // kernel launches of one call (one per transform size)
+    struct Impl;
+private:
+    std::unique_ptr<Impl> impl_;
+};
+
 // ---- batched cone-beam plans (no counterpart in the reference) ----
 struct ConeJobSpec {
     Grid g;

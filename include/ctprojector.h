@@ -318,6 +318,30 @@ int ctpDDBatchConfig(void* batch, int* fp_lanes, int* bp_lanes, int* fp_persiste
                      int* fp_streams, int* bp_streams, int* fp_launches, int* bp_launches);
 int ctpDDBatchDestroy(void* batch);
 
+/* No counterpart in the reference (whose filters take one geometry per call,
+ * cuda/fbp.cu:97-104, cuda/fbpParallel.cu:59): the ramp filters of a batch of
+ * fan-beam (fan = 1: da the angular pitch, cupyfbpFanFilter) or parallel-beam
+ * (fan = 0: da = du, dsd and dso unused, cupyfbpParallelFilter) jobs of
+ * different detectors, view counts and batch counts, bound to their device
+ * arrays (prj in, fprj out, [nBatches][nview][nv][nu]).  Run writes every
+ * job's filtered sinogram, equal to its single call's (the FFT forms of that
+ * filter: nu up to 2048), in one launch per transform size, and only launches
+ * kernels on 'stream' (capturable into a HIP graph; returns before the work
+ * completes).  With ctpDDBatch* jobs of typeProjector bit 0 (FBP weights)
+ * reading fprj, uneven FBP reconstructions take a few launches in all.
+ * Calls on one batch must not overlap in time. */
+typedef struct ctpFilterJob {
+    int fan;
+    int typeFilter;
+    size_t nBatches, nu, nv, nview;
+    float da, off_a, dsd, dso;
+    const float* prj;        /* device */
+    float* fprj;             /* device */
+} ctpFilterJob;
+int ctpFilterBatchCreate(void** batch, const ctpFilterJob* jobs, int njobs);
+int ctpFilterBatchRun(void* batch, void* stream);
+int ctpFilterBatchDestroy(void* batch);
+
 /* cuda/fbpParallel.cu:273  parallel-beam pixel-driven backprojection (host arrays):
  * img += sum over views of the bilinear (truncating) sample of prj at
  * u = -((x cos + y sin)/du + off_u) + (nu-1)/2; typeProjector is unused */

@@ -115,3 +115,29 @@ def test_dd_batch_rejects_bad_jobs_before_device_work():
         assert lib.ctpDDBatchCreate(ctypes.byref(plan), jobs, ctypes.c_int(2), ctypes.c_int(0)) != 0, bad
         assert msg in _lib.last_error_message(), (bad, _lib.last_error_message())
         assert not plan.value
+
+
+def test_filter_batch_rejects_bad_jobs_before_device_work():
+    """ctpFilterBatchCreate validates its job list on the host: no jobs, an
+    empty job, a detector row wider than 2048 columns, a zero pitch and
+    missing arrays are errors (codes, not aborts), and no plan is returned."""
+    from ctprojector_amd import _lib
+    from ctprojector_amd.batched import _FJob
+    lib = _lib.module()
+    plan = ctypes.c_void_p(1)
+    assert lib.ctpFilterBatchCreate(ctypes.byref(plan), None, ctypes.c_int(0)) != 0
+    assert 'no jobs' in _lib.last_error_message() and not plan.value
+    fake = ctypes.c_void_p(4096).value    # never dereferenced: validation fails first
+
+    def job(**kw):
+        a = dict(fan=1, typeFilter=0, nBatches=1, nu=12, nv=1, nview=4, da=0.01, off_a=0, dsd=200, dso=120,
+                 prj=fake, fprj=fake)
+        a.update(kw)
+        return _FJob(**a)
+    for bad, msg in [(dict(nview=0), 'empty job'), (dict(nu=2049), '2048'), (dict(da=0.0), 'zero detector pitch'),
+                     (dict(dso=0.0), 'zero dso'), (dict(prj=None), 'null pointer'), (dict(fprj=None), 'null pointer')]:
+        jobs = (_FJob * 2)(job(), job(**bad))
+        plan = ctypes.c_void_p(1)
+        assert lib.ctpFilterBatchCreate(ctypes.byref(plan), jobs, ctypes.c_int(2)) != 0, bad
+        assert msg in _lib.last_error_message(), (bad, _lib.last_error_message())
+        assert not plan.value

@@ -526,6 +526,87 @@ def dd_batch_extra(steps):
     }
 
 
+def fbp_batch_extra(steps):
+    """Filtered backprojection of the sixteen uneven jobs of
+    phantom.DD_BATCH_SAMPLES (Hann window): one FBP batch (the ramp filters in
+    one launch per FFT size, then the DD batch's FBP-weighted backprojection,
+    a few launches per kernel class), replayed from a HIP graph; against the
+    per-call entry points (ramp_filter then distance_driven_bp(is_fbp=True)
+    per job, host tables and a synchronisation per call, as the reference's
+    calls).  The sinograms are the jobs' forward projections."""
+    import torch
+    from ctprojector_amd.batched import DDJob, FBPBatch
+    from ctprojector_amd.phantom import dd_batch_workload
+    from ctprojector_amd.projector.cupy import ct_projector, fan_equiangular as fan, parallel
+
+    work = dd_batch_workload()
+    projs = []
+    for _, w, _ in work:
+        p = ct_projector()
+        for k in ('nx', 'ny', 'nz', 'dx', 'dy', 'dz', 'cx', 'cy', 'cz', 'nu', 'nv', 'du', 'dv', 'off_u', 'off_v',
+                  'dsd', 'dso'):
+            setattr(p, k, getattr(w, k))
+        projs.append(p)
+    nvu = sum(int(np.prod(img.shape)) * w.nview for _, w, img in work)
+    dang = [torch.from_numpy(w.angles).cuda() for _, w, _ in work]
+    sinos = []
+    for p, (f, _, img), a in zip(projs, work, dang):
+        m = fan if f else parallel
+        sinos.append(m.distance_driven_fp(p, torch.from_numpy(img).cuda(), a))
+    log('[bench] fbp batch: plan (%d jobs)' % len(work))
+    fb = FBPBatch([DDJob(p, w.angles, fan=f) for p, (f, w, _) in zip(projs, work)], filter_type='hann',
+                  autotune=True)
+    for t, x in zip(fb.prj, sinos):
+        t.copy_(x)
+    g = fb.capture()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    best = filt = bp = 1e30
+    log('[bench] fbp batch timed x%d' % steps)
+    for _ in range(steps + 1):
+        torch.cuda.synchronize()
+        ev[0].record()
+        g.replay()
+        ev[1].record()
+        torch.cuda.synchronize()
+        best = min(best, ev[0].elapsed_time(ev[1]))
+        # the two stages on their own (eager launches)
+        ev[0].record()
+        fb.filter.run()
+        ev[1].record()
+        fb.dd.backward()
+        ev[2].record()
+        torch.cuda.synchronize()
+        filt = min(filt, ev[0].elapsed_time(ev[1]))
+        bp = min(bp, ev[1].elapsed_time(ev[2]))
+    batch_img = [t.clone() for t in fb.img]
+    fb.close()
+    del fb
+    log('[bench] fbp batch: sequential per-call entry points')
+    seq = 1e30
+    worst = 0.0
+    for _ in range(2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        refs = []
+        for p, (f, _, _), y, a in zip(projs, work, sinos, dang):
+            m = fan if f else parallel
+            refs.append(m.distance_driven_bp(p, m.ramp_filter(p, y, 'hann'), a, is_fbp=True))
+        torch.cuda.synchronize()
+        seq = min(seq, (time.perf_counter() - t0) * 1e3)
+    for a, b in zip(batch_img, refs):
+        worst = max(worst, float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b).clamp_min(1e-30)))
+    return {
+        'workload': 'uneven FBP batch: %d fan/parallel jobs (phantom.DD_BATCH_SAMPLES), Hann ramp filter + '
+                    'FBP-weighted DD backprojection of each; HIP-graph replay' % len(work),
+        'ms_per_step': round(best, 3),
+        'filter_ms': round(filt, 3),
+        'bp_ms': round(bp, 3),
+        'value_gvu_s': round(nvu / (best / 1e3) / 1e9, 2),
+        'sequential_calls_ms_per_step': round(seq, 3),
+        'speedup_vs_sequential_calls': round(seq / best, 3),
+        'max_rel_l2_vs_single_calls': worst,
+    }
+
 def serving_extra(count=200, seed=1005, sub_count=32):
     """Served reconstruction stream: Poisson arrivals (200/s) of half fan-beam
     FBP (256^2-512^2 x 1-64 slices, 360-984 views) and half cone-beam SQS-TV
@@ -793,7 +874,7 @@ def main(argv=None):
     ap.add_argument('--no-extra', action='store_true')
     ap.add_argument('--no-serving', action='store_true', help='skip the served-stream extra')
     ap.add_argument('--no-parity', action='store_true', help='skip the FP64 oracle check of the outputs')
-    ap.add_argument('--extras', default='cone,cone_batch,dd_batch,fbp,unrolled_sqs_tv,tv_prior,serving',
+    ap.add_argument('--extras', default='cone,cone_batch,dd_batch,fbp_batch,fbp,unrolled_sqs_tv,tv_prior,serving',
                     help='comma-separated extras to run (default: all)')
     ap.add_argument('--serving-requests', type=int, default=200)
     ap.add_argument('--serving-below-saturation', type=int, default=32,
@@ -857,6 +938,7 @@ def main(argv=None):
                                             outputs=None if args.no_parity else outputs['cone'])),
                 ('cone_batch', lambda: cone_batch_extra(3)),
                 ('dd_batch', lambda: dd_batch_extra(5)),
+                ('fbp_batch', lambda: fbp_batch_extra(5)),
                 ('fbp', lambda: fbp_extra(3, w, r['img'], outputs=None if args.no_parity else outputs['fan'])),
                 ('unrolled_sqs_tv', lambda: unrolled_sqs_extra(3)),
                 ('tv_prior', lambda: tv_prior_extra(5))]

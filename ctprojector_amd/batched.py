@@ -354,9 +354,11 @@ class FBPBatch:
     distance_driven_bp(..., is_fbp=True)) and backprojects them (one launch
     per kernel class): each job's image equals ramp_filter followed by
     distance_driven_bp(is_fbp=True) on its own.  ``jobs`` are DDJob records
-    (their is_fbp flag is set here); ``filter_type`` applies to every job."""
+    (their is_fbp flag is set here); ``filter_type`` applies to every job.
+    The DD batch is autotuned by default (lanes, block distribution and side
+    streams): untuned, sixteen mostly single-slice jobs ran 1.9x slower."""
 
-    def __init__(self, jobs: Sequence[DDJob], filter_type: str = 'hann', autotune: bool = False, device=None):
+    def __init__(self, jobs: Sequence[DDJob], filter_type: str = 'hann', autotune: bool = True, device=None):
         jobs = [DDJob(j.projector, j.angles, fan=j.fan, batch=j.batch, is_fbp=True, branchless=j.branchless,
                       force_fbp=j.force_fbp) for j in jobs]
         self.dd = DDBatch(jobs, autotune=autotune, device=device)

@@ -485,7 +485,7 @@ void ramp_fft16_kernel(float* __restrict__ out, const float* __restrict__ in, co
 // Batches of uneven filter jobs (no counterpart in the reference): the row
 // pairs of the jobs with one transform size N are numbered in a row (job j's
 // from pair_begin), and each complex row of a block (a wavefront or more)
-// looks up its job and filters one pair with that job's tables.
+// finds its job and filters one pair with that job's tables.
 // ---------------------------------------------------------------------------
 struct RampJobDev {
     float* out;
@@ -497,10 +497,16 @@ struct RampJobDev {
     int nu;
 };
 
+// the last job whose first pair is <= p (jobs[0].pair_begin = 0): a binary
+// search, wave-uniform, so the records are read with scalar loads
 __device__ inline const RampJobDev* ramp_job_of(const RampJobDev* __restrict__ jobs, int njobs, long p) {
-    int j = 0;
-    while (j + 1 < njobs && p >= jobs[j + 1].pair_begin) j++;
-    return jobs + j;
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].pair_begin <= p) lo = mid;
+        else hi = mid - 1;
+    }
+    return jobs + lo;
 }
 
 template <int N>

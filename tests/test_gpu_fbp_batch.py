@@ -9,6 +9,8 @@
   ramp_filter followed by distance_driven_bp(is_fbp=True) per job;
 * HIP-graph capture of the FBP batch: replayed three times on new inputs,
   with eager work in between, each replay equal to the eager run;
+* many jobs of one transform size (seven generic, three three-pass), odd
+  row counts: each equals its single call bit for bit;
 * a detector row wider than the batch supports is refused with an error.
 
 Every case prints what names it before its GPU work."""
@@ -79,6 +81,34 @@ def test_filter_batch_equals_single_calls(torch):
         assert torch.equal(fb.fprj[k], single), k
     fb.close()
 
+
+
+def test_filter_batch_many_jobs_per_size(torch):
+    """Seven jobs of one generic-kernel size (N = 256) and three of the
+    three-pass size N = 1024, odd row counts (a job's last pair has one row):
+    each complex row finds its job by the binary search over the group."""
+    from ctprojector_amd.batched import FilterJob, RampFilterBatch
+    from ctprojector_amd.projector.cupy import fan_equiangular as fan, parallel
+    specs = [(k % 2 == 0, 70 + 9 * k, 1 + k % 3, 5 + 2 * k, 1) for k in range(7)]
+    specs += [(True, 300, 3, 7, 1), (False, 420, 1, 9, 3), (True, 511, 5, 3, 1)]
+    jobs, data = [], []
+    rng = np.random.default_rng(17)
+    for k, (f, nu, nv, nview, B) in enumerate(specs):
+        p = _proj(_geo(f, 32, nv, nu, off=0.1 * (k % 3) - 0.1))
+        ft = ('hann', 'hamming', 'cosine', 'ramlak')[k % 4]
+        jobs.append(FilterJob(p, nview, fan=f, batch=B, filter_type=ft))
+        data.append(rng.random((B, nview, nv, nu), dtype=np.float32))
+    _say('filter batch of %d jobs in two transform sizes' % len(jobs))
+    fb = RampFilterBatch(jobs)
+    for t, x in zip(fb.prj, data):
+        t.copy_(torch.from_numpy(x))
+    fb.run()
+    torch.cuda.synchronize()
+    for k, (job, x) in enumerate(zip(jobs, data)):
+        m = fan if job.fan else parallel
+        single = m.ramp_filter(job.projector, torch.from_numpy(x).cuda(), job.filter_type)
+        assert torch.equal(fb.fprj[k], single), k
+    fb.close()
 
 def _fbp_jobs():
     # (fan, n, nz, nview, nu, batch, offset, centre)

@@ -38,8 +38,8 @@ std::map<std::string, int> g_tune = {
     {"cone_bp_mode", 0},     // Siddon BP: 0 column mode where the geometry allows it, 1 general window, 2 ray-driven scatter
     {"cone_slab_slices", 0}, // Siddon FP/BP: at most this many z slices per slab (0: slabs only for 2^29+ voxels)
     {"cone_plan_persistent", 0},   // cone plans: 1 persistent blocks with work counters, 2 one block per tile
-    {"dd_batch_persistent", 0},    // DD batch plans: 1 persistent blocks with work counters, 2 one block per tile
-    {"dd_batch_streams", 0},       // DD batch plans: streams the kernel classes of a direction run on (1 to 4; default 1)
+    {"dd_batch_persistent", 0},    // DD batch plans: 1 persistent blocks with work counters, 2 one block per tile (default)
+    {"dd_batch_streams", 0},       // DD batch plans: streams the kernel classes of a direction run on (1 to 4; default 4)
     {"dd_fp_mode", 0},       // DD FP: 1 direct (default), 2 LDS-staged rows where the footprint fits
     {"dd_fp_extra_channel", 0},  // DD FP direct kernel: 1 keeps the next chunk's channel with one chunk too
     {"dd_fp_pixel_stride", 0},   // DD FP direct kernel, one channel chunk: floats per pixel of the channel-last copy

@@ -8,9 +8,9 @@ ctprojector_amd.batched.DDBatch).
   channel chunks, one-lane and split-lane backprojectors with and without
   view groups: each job's forward and backward equal its single call through
   the C ABI, and two jobs are checked against the FP64 oracle;
-* the autotuner, both block distributions (persistent work counters,
-  hardware dispatch) and the kernel classes on four concurrent streams give
-  the same results;
+* the autotuner, both block distributions (hardware dispatch, the default,
+  and persistent work counters) and the kernel classes on four concurrent
+  streams (the default) or one give the same results;
 * HIP-graph capture (one stream, and four streams forked and joined inside
   the capture): forward and backward replayed three times on new inputs, with
   eager work in between, each replay equal to the eager run.
@@ -165,10 +165,10 @@ def test_autotune_and_dispatch_forms(torch):
     for a, b in zip(ref[0] + ref[2], got[0] + got[2]):
         assert rel_l2(b, a) < 1e-6
     try:
-        _lib.set_tuning('dd_batch_persistent', 2)
-        _say('hardware-dispatched blocks')
+        _lib.set_tuning('dd_batch_persistent', 1)
+        _say('persistent blocks with work counters')
         bt = _batch(jobs)
-        assert bt.config()['fp_persistent'] == 0
+        assert bt.config()['fp_persistent'] == 1
         got = _run(torch, bt, jobs)
         bt.close()
     finally:
@@ -176,10 +176,10 @@ def test_autotune_and_dispatch_forms(torch):
     for a, b in zip(ref[0] + ref[2], got[0] + got[2]):
         assert rel_l2(b, a) < 1e-6
     try:
-        _lib.set_tuning('dd_batch_streams', 4)
-        _say('kernel classes on four streams')
+        _lib.set_tuning('dd_batch_streams', 1)
+        _say('kernel classes on one stream')
         bt = _batch(jobs)
-        assert bt.config()['fp_streams'] == 4
+        assert bt.config()['fp_streams'] == 1
         got = _run(torch, bt, jobs)
         bt.close()
     finally:

@@ -427,6 +427,95 @@ def cone_batch_extra(steps):
     }
 
 
+SMALL_CONE_JOBS = [(n, dx, nview) for (n, dx) in ((48, 2.0), (64, 1.5), (80, 1.25), (96, 1.0))
+                   for nview in (90, 120, 180)]
+
+
+def cone_small_batch_extra(steps):
+    """Twelve small uneven cone-beam jobs (48^3 @2 mm to 96^3 @1 mm, 90 to
+    180 views each, own jittered trajectories, 64x48 panel at 4 mm): the
+    regime where one job does not fill the GPU, in one batched plan
+    (autotuned, one persistent launch per direction, HIP-graph replay),
+    against the same work through the single-volume device entry points, one
+    job after another."""
+    import torch
+    from ctprojector_amd.batched import ConeBatch, ConeJob
+    from ctprojector_amd.phantom import Workload, circular_cone_geometry, ellipsoid_phantom
+    from ctprojector_amd.projector.cupy import ct_projector, cone
+
+    jobs, imgs, geos, projs = [], [], [], []
+    for j, (n, dx, nview) in enumerate(SMALL_CONE_JOBS):
+        w = Workload(n, n, n, dx, dx, dx, 64, 48, 4.0, 4.0, nview, dsd=1085.6, dso=595.0)
+        geo = circular_cone_geometry(nview, 595.0, 1085.6, -10.0, 10.0, jitter=2.0, seed=1400 + j)
+        img = ellipsoid_phantom(n, n, n, dx, dx, dx, seed=1500 + j)
+        p = ct_projector()
+        for k in ('nx', 'ny', 'nz', 'dx', 'dy', 'dz', 'cx', 'cy', 'cz', 'nu', 'nv', 'du', 'dv', 'off_u', 'off_v'):
+            setattr(p, k, getattr(w, k))
+        jobs.append(ConeJob(p, geo))
+        imgs.append(img)
+        geos.append(geo)
+        projs.append(p)
+    nvu = sum(int(np.prod(im.shape)) * nv for im, (_, _, nv) in zip(imgs, SMALL_CONE_JOBS))
+    log('[bench] small cone batch: plan + autotune (%d jobs)' % len(jobs))
+    t0 = time.perf_counter()
+    bt = ConeBatch(jobs, autotune=True)
+    torch.cuda.synchronize()
+    plan_s = time.perf_counter() - t0
+    saved = [torch.from_numpy(im).cuda() for im in imgs]
+    for t, x in zip(bt.img, saved):
+        t.copy_(x)
+    gf = bt.capture('forward')
+    gb = bt.capture('backward')
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    best = fp_b = bp_b = 1e30
+    log('[bench] small cone batch timed x%d' % steps)
+    for _ in range(steps + 1):
+        for t, x in zip(bt.img, saved):
+            t.copy_(x)
+        ev[0].record()
+        gf.replay()
+        ev[1].record()
+        gb.replay()
+        ev[2].record()
+        torch.cuda.synchronize()
+        best = min(best, ev[0].elapsed_time(ev[2]))
+        fp_b = min(fp_b, ev[0].elapsed_time(ev[1]))
+        bp_b = min(bp_b, ev[1].elapsed_time(ev[2]))
+    batch_img = [t.clone() for t in bt.img]
+    cfg = bt.config()
+    bt.close()
+    del bt
+    log('[bench] small cone batch: sequential single-volume calls')
+    dg = [[torch.from_numpy(a).cuda() for a in (g.det_center, g.det_u, g.det_v, g.src)] for g in geos]
+    seq = 1e30
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        outs = []
+        for p, x, g in zip(projs, saved, dg):
+            y = cone.siddon_fp_arbitrary(p, x[None], *g)
+            outs.append(cone.siddon_bp_arbitrary(p, y, *g))
+        torch.cuda.synchronize()
+        seq = min(seq, (time.perf_counter() - t0) * 1e3)
+    worst = 0.0
+    for a, b in zip(batch_img, outs):
+        b = b.reshape(a.shape)
+        worst = max(worst, float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b).clamp_min(1e-30)))
+    return {
+        'workload': 'batched cone Siddon FP+BP of %d small uneven jobs: 48^3 @2, 64^3 @1.5, 80^3 @1.25, 96^3 @1 mm '
+                    'x 90/120/180 views (own jittered trajectories), 64x48 panel @4 mm; one persistent launch per '
+                    'direction, autotuned, HIP-graph replay' % len(jobs),
+        'ms_per_step': round(best, 3),
+        'fp_ms': round(fp_b, 3),
+        'bp_ms': round(bp_b, 3),
+        'value_gvu_s': round(2 * nvu / (best / 1e3) / 1e9, 2),
+        'sequential_ms_per_step': round(seq, 3),
+        'speedup_vs_sequential': round(seq / best, 3),
+        'max_rel_l2_vs_single_calls': worst,
+        'plan_and_autotune_s': round(plan_s, 3),
+        'autotuned': cfg,
+    }
+
 def dd_batch_extra(steps):
     """The sixteen uneven distance-driven jobs of phantom.DD_BATCH_SAMPLES
     (fan and parallel beams, 96^2 to 512^2, 1 to 8 slices, 128 to 984 views)
@@ -874,7 +963,7 @@ def main(argv=None):
     ap.add_argument('--no-extra', action='store_true')
     ap.add_argument('--no-serving', action='store_true', help='skip the served-stream extra')
     ap.add_argument('--no-parity', action='store_true', help='skip the FP64 oracle check of the outputs')
-    ap.add_argument('--extras', default='cone,cone_batch,dd_batch,fbp_batch,fbp,unrolled_sqs_tv,tv_prior,serving',
+    ap.add_argument('--extras', default='cone,cone_batch,cone_small_batch,dd_batch,fbp_batch,fbp,unrolled_sqs_tv,tv_prior,serving',
                     help='comma-separated extras to run (default: all)')
     ap.add_argument('--serving-requests', type=int, default=200)
     ap.add_argument('--serving-below-saturation', type=int, default=32,
@@ -937,6 +1026,7 @@ def main(argv=None):
         runs = [('cone', lambda: cone_extra(2, cpu=not args.no_cpu_baseline,
                                             outputs=None if args.no_parity else outputs['cone'])),
                 ('cone_batch', lambda: cone_batch_extra(3)),
+                ('cone_small_batch', lambda: cone_small_batch_extra(5)),
                 ('dd_batch', lambda: dd_batch_extra(5)),
                 ('fbp_batch', lambda: fbp_batch_extra(5)),
                 ('fbp', lambda: fbp_extra(3, w, r['img'], outputs=None if args.no_parity else outputs['fan'])),

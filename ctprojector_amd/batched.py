@@ -290,8 +290,9 @@ class RampFilterBatch:
     """Ramp filters of uneven jobs (``ctpFilterBatch*``): ``fprj[j]`` = the
     job's fan_equiangular / parallel ramp_filter of ``prj[j]`` (both
     [batch, nview, nv, nu] device tensors bound to the batch; ``outputs``
-    may supply the fprj tensors), in one launch per transform size.  Detector
-    rows of up to 2048 columns.  ``run`` is asynchronous on torch's current
+    may supply the fprj tensors), in one launch per transform size (rows of
+    up to 2048 columns) plus one per job of 2049 to 5000 columns (the single
+    call's direct convolution).  ``run`` is asynchronous on torch's current
     stream and can be captured into a HIP graph."""
 
     def __init__(self, jobs: Sequence[FilterJob], device=None, outputs: Sequence[torch.Tensor] = None):

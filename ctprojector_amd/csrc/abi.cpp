@@ -975,7 +975,7 @@ CTP_EXPORT int ctpFilterBatchCreate(void** batch, const ctpFilterJob* jobs, int 
         for (int j = 0; j < njobs; j++) {
             const ctpFilterJob& J = jobs[j];
             need(!empty(J.nBatches, J.nu, J.nv, J.nview, 1, 1, 1), "empty job");
-            need(J.nu <= 2048, "filter batch: detector rows of more than 2048 columns (use the single-call filter)");
+            need(J.nu <= 5000, "filter batch: detector rows of more than 5000 columns (as the single-call filter)");
             need(J.nBatches <= kIntMax && J.nv <= kIntMax && J.nview <= kIntMax, "job too large");
             need(J.da != 0, "zero detector pitch");
             need(!J.fan || J.dso != 0, "zero dso");

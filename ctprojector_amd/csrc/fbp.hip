@@ -590,6 +590,18 @@ struct FilterBatchPlan::Impl {
         long npairs;
     };
     std::vector<Group> groups;
+    // jobs of more than 2048 columns: the single call's direct convolution,
+    // one launch each
+    struct Wide {
+        float* out;
+        const float* in;
+        float* h;
+        float* pw;
+        long nrows;
+        int nu;
+        float scale;
+    };
+    std::vector<Wide> wide;
     std::vector<void*> mem;
     template <typename T>
     T* alloc(size_t n) {
@@ -606,13 +618,23 @@ struct FilterBatchPlan::Impl {
 // Tables of every job (the single-call filter's, see ramp_filter) uploaded
 // once; jobs grouped by transform size N, each group one launch of the
 // single-call kernel for that N (three-pass from N = 1024, else the generic
-// one), so each job's output equals the single call's.
+// one), so each job's output equals the single call's.  Jobs of more than
+// 2048 columns take the single call's direct convolution, one launch each.
 FilterBatchPlan::FilterBatchPlan(const std::vector<FilterBatchJobSpec>& jobs) : impl_(new Impl) {
     Impl& P = *impl_;
     std::map<int, std::vector<RampJobDev>> byN;
     for (const FilterBatchJobSpec& S : jobs) {
-        if (S.nu < 1 || S.nu > 2048)
-            fail("filter batch: detector rows of 1 to 2048 columns (transforms up to 4096); use the single-call filter");
+        if (S.nu < 1 || S.nu > 5000) fail("filter batch: detector rows of 1 to 5000 columns");
+        if (2 * S.nu - 1 > 4096) {
+            const std::vector<float> h = cached_ramp(S.nu, S.da, S.filter, S.equispace);
+            const std::vector<float> pw = ramp_preweight(S.nu, S.da, S.off_u, S.equispace);
+            Impl::Wide W{S.out, S.in, P.alloc<float>(h.size()), P.alloc<float>(S.nu), (long)S.B * S.nview * S.nv,
+                         S.nu, ramp_scale(S.nview, S.da, S.dso, S.equispace)};
+            CTP_CHECK(hipMemcpy(W.h, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice));
+            CTP_CHECK(hipMemcpy(W.pw, pw.data(), sizeof(float) * S.nu, hipMemcpyHostToDevice));
+            P.wide.push_back(W);
+            continue;
+        }
         const RampFFTTables t = ramp_fft_tables(S.nu, S.da, S.filter, S.equispace,
                                                 ramp_scale(S.nview, S.da, S.dso, S.equispace));
         const std::vector<float> pw = ramp_preweight(S.nu, S.da, S.off_u, S.equispace);
@@ -648,7 +670,7 @@ FilterBatchPlan::FilterBatchPlan(const std::vector<FilterBatchJobSpec>& jobs) : 
 
 FilterBatchPlan::~FilterBatchPlan() = default;
 
-int FilterBatchPlan::launches() const { return (int)impl_->groups.size(); }
+int FilterBatchPlan::launches() const { return (int)(impl_->groups.size() + impl_->wide.size()); }
 
 void FilterBatchPlan::run(hipStream_t s) {
     TimedLaunch tl("ramp_filter_batch", s);
@@ -664,6 +686,19 @@ void FilterBatchPlan::run(hipStream_t s) {
         } else {
             ramp_fft_batch_kernel<<<(unsigned)G.npairs, 256, sizeof(float2) * G.N, s>>>(G.djobs, G.njobs, G.N,
                                                                                        G.logN);
+        }
+        CTP_LAUNCH_CHECK();
+    }
+    for (const Impl::Wide& W : impl_->wide) {
+        // the single call's launch shapes (ramp_filter)
+        const size_t sh4 = sizeof(float) * ((2 * W.nu - 1 + 4) + 4 * W.nu);
+        if (sh4 <= 65536) {
+            ramp_conv_kernel<4><<<(unsigned)ceil_div(W.nrows, 4L), 256, sh4, s>>>(W.out, W.in, W.h, W.pw, W.nrows,
+                                                                                 W.nu, W.scale, nullptr, 1, 1);
+        } else {
+            const size_t sh1 = sizeof(float) * ((2 * W.nu - 1 + 4) + W.nu);
+            ramp_conv_kernel<1><<<(unsigned)W.nrows, 256, sh1, s>>>(W.out, W.in, W.h, W.pw, W.nrows, W.nu, W.scale,
+                                                                    nullptr, 1, 1);
         }
         CTP_LAUNCH_CHECK();
     }

@@ -324,8 +324,9 @@ int ctpDDBatchDestroy(void* batch);
  * (fan = 0: da = du, dsd and dso unused, cupyfbpParallelFilter) jobs of
  * different detectors, view counts and batch counts, bound to their device
  * arrays (prj in, fprj out, [nBatches][nview][nv][nu]).  Run writes every
- * job's filtered sinogram, equal to its single call's (the FFT forms of that
- * filter: nu up to 2048), in one launch per transform size, and only launches
+ * job's filtered sinogram, equal to its single call's, in one launch per FFT
+ * size (nu up to 2048) plus one per job of 2049 to 5000 columns (the single
+ * call's direct convolution), and only launches
  * kernels on 'stream' (capturable into a HIP graph; returns before the work
  * completes).  With ctpDDBatch* jobs of typeProjector bit 0 (FBP weights)
  * reading fprj, uneven FBP reconstructions take a few launches in all.

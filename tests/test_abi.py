@@ -119,7 +119,7 @@ def test_dd_batch_rejects_bad_jobs_before_device_work():
 
 def test_filter_batch_rejects_bad_jobs_before_device_work():
     """ctpFilterBatchCreate validates its job list on the host: no jobs, an
-    empty job, a detector row wider than 2048 columns, a zero pitch and
+    empty job, a detector row wider than 5000 columns, a zero pitch and
     missing arrays are errors (codes, not aborts), and no plan is returned."""
     from ctprojector_amd import _lib
     from ctprojector_amd.batched import _FJob
@@ -134,7 +134,7 @@ def test_filter_batch_rejects_bad_jobs_before_device_work():
                  prj=fake, fprj=fake)
         a.update(kw)
         return _FJob(**a)
-    for bad, msg in [(dict(nview=0), 'empty job'), (dict(nu=2049), '2048'), (dict(da=0.0), 'zero detector pitch'),
+    for bad, msg in [(dict(nview=0), 'empty job'), (dict(nu=5001), '5000'), (dict(da=0.0), 'zero detector pitch'),
                      (dict(dso=0.0), 'zero dso'), (dict(prj=None), 'null pointer'), (dict(fprj=None), 'null pointer')]:
         jobs = (_FJob * 2)(job(), job(**bad))
         plan = ctypes.c_void_p(1)

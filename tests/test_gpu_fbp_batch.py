@@ -12,7 +12,8 @@
   with eager work in between, each replay equal to the eager run;
 * many jobs of one transform size (seven generic, three three-pass), odd
   row counts: each equals its single call bit for bit;
-* a detector row wider than the batch supports is refused with an error.
+* jobs of 2049 and 3000 columns (the direct convolution) beside an FFT job
+  equal their single calls; more than 5000 columns are refused.
 
 Every case prints what names it before its GPU work."""
 
@@ -202,9 +203,29 @@ def test_fbp_batch_graph_replays_match_eager(torch):
     fb.close()
 
 
-def test_filter_batch_refuses_wide_rows(torch):
+def test_filter_batch_wide_rows(torch):
+    """Jobs of 2049 to 5000 columns take the single call's direct
+    convolution inside the batch (equal to it bit for bit, beside an FFT
+    job); more than 5000 columns are refused, as by the single call."""
     from ctprojector_amd.batched import FilterJob, RampFilterBatch
-    _say('filter batch: a 2049-column detector is refused')
-    p = _proj(_geo(True, 32, 1, 2049))
-    with pytest.raises(RuntimeError, match='2048'):
-        RampFilterBatch([FilterJob(p, 2)])
+    from ctprojector_amd.projector.cupy import fan_equiangular as fan, parallel
+    specs = [(True, 2049, 2, 3, 'hann'), (False, 3000, 1, 2, 'ramlak'), (True, 500, 2, 5, 'hamming')]
+    jobs, data = [], []
+    rng = np.random.default_rng(23)
+    for f, nu, nv, nview, ft in specs:
+        jobs.append(FilterJob(_proj(_geo(f, 32, nv, nu, off=0.2)), nview, fan=f, filter_type=ft))
+        data.append(rng.random((1, nview, nv, nu), dtype=np.float32))
+    _say('filter batch with 2049- and 3000-column jobs')
+    fb = RampFilterBatch(jobs)
+    for t, x in zip(fb.prj, data):
+        t.copy_(torch.from_numpy(x))
+    fb.run()
+    torch.cuda.synchronize()
+    for k, (job, x) in enumerate(zip(jobs, data)):
+        m = fan if job.fan else parallel
+        single = m.ramp_filter(job.projector, torch.from_numpy(x).cuda(), job.filter_type)
+        assert torch.equal(fb.fprj[k], single), k
+    fb.close()
+    _say('filter batch: a 5001-column detector is refused')
+    with pytest.raises(RuntimeError, match='5000'):
+        RampFilterBatch([FilterJob(_proj(_geo(True, 32, 1, 5001)), 2)])
